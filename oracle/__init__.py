@@ -1,0 +1,203 @@
+"""ctypes front end of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP implementation.  Only tests/,
+__graft_entry__.smoke() and bench.py's CPU-baseline leg import this module.
+Record layouts are documented in oracle/oracle.h.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+BODY = 32
+GEOM = 16
+CONTACT = 12
+JOINT = 16
+
+# params vector indices (oracle.h ORC_P_*)
+P_H, P_GX, P_GY, P_GZ, P_ERP, P_CFM, P_SOR_W, P_ITERS, P_MAX_VEL, P_MIN_DEPTH = range(10)
+P_MODE, P_MU, P_MU2, P_BOUNCE, P_BOUNCE_VEL, P_SOFT_ERP, P_SOFT_CFM, P_MAXC, P_SPACE = range(10, 19)
+P_MOTION1, P_MOTION2, P_MOTIONN, P_SLIP1, P_SLIP2, P_SKIP_CONNECTED, P_LAMBDA_FLOOR = range(19, 26)
+
+
+def build():
+    subprocess.check_call(["sh", os.path.join(_HERE, "build.sh")])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = ctypes.CDLL(_LIB)
+        _lib.orc_lcp_residual.restype = ctypes.c_double
+        _lib.orc_drand.restype = ctypes.c_uint
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _real(dtype):
+    return "f32" if np.dtype(dtype) == np.float32 else "f64"
+
+
+def params(**kw):
+    """Default step parameters (dWorldCreate defaults of objects.cpp:37-99 for
+    dSINGLE, deterministic QuickStep, contact surface mu=0.5 with Approx1)."""
+    p = np.zeros(32, np.float64)
+    p[P_H] = 0.01
+    p[P_GZ] = -9.81
+    p[P_ERP] = 0.2
+    p[P_CFM] = 1e-5
+    p[P_SOR_W] = 1.3
+    p[P_ITERS] = 20
+    p[P_MAX_VEL] = np.inf
+    p[P_MIN_DEPTH] = 0.0
+    p[P_MODE] = 0x7000
+    p[P_MU] = 0.5
+    p[P_MAXC] = 4
+    p[P_SPACE] = 0
+    p[P_LAMBDA_FLOOR] = 1e-3
+    names = {
+        "h": P_H, "erp": P_ERP, "cfm": P_CFM, "w": P_SOR_W, "iters": P_ITERS, "max_vel": P_MAX_VEL,
+        "min_depth": P_MIN_DEPTH, "mode": P_MODE, "mu": P_MU, "mu2": P_MU2, "bounce": P_BOUNCE,
+        "bounce_vel": P_BOUNCE_VEL, "soft_erp": P_SOFT_ERP, "soft_cfm": P_SOFT_CFM, "maxc": P_MAXC,
+        "space": P_SPACE, "skip_connected": P_SKIP_CONNECTED, "lambda_floor": P_LAMBDA_FLOOR,
+    }
+    for k, v in kw.items():
+        if k == "gravity":
+            p[P_GX:P_GZ + 1] = v
+        else:
+            p[names[k]] = v
+    return p
+
+
+def drand_seq(seed, n):
+    s = ctypes.c_uint(seed)
+    return [lib().orc_drand(ctypes.byref(s)) for _ in range(n)]
+
+
+def drand_int_seq(seed, n, k):
+    s = ctypes.c_uint(seed)
+    return [lib().orc_drand_int(ctypes.byref(s), k) for _ in range(n)]
+
+
+def collide(g1, g2, b1=None, b2=None, maxc=4, dtype=np.float64):
+    g1 = np.ascontiguousarray(g1, dtype)
+    g2 = np.ascontiguousarray(g2, dtype)
+    b1 = np.ascontiguousarray(b1 if b1 is not None else np.zeros(BODY), dtype)
+    b2 = np.ascontiguousarray(b2 if b2 is not None else np.zeros(BODY), dtype)
+    out = np.zeros((16, CONTACT), dtype)
+    fn = getattr(lib(), "orc_collide_" + _real(dtype))
+    n = fn(_p(g1), _p(g2), _p(b1), _p(b2), ctypes.c_int(maxc), _p(out))
+    return out[:n]
+
+
+def aabbs(geoms, bodies, dtype=np.float32):
+    geoms = np.ascontiguousarray(geoms, dtype)
+    bodies = np.ascontiguousarray(bodies, dtype)
+    out = np.zeros((len(geoms), 6), dtype)
+    getattr(lib(), "orc_aabbs_" + _real(dtype))(ctypes.c_int(len(geoms)), _p(geoms), _p(bodies), _p(out))
+    return out
+
+
+def pairs(aabb, body, space="sap", cat=None, col=None, levels=(-3, 10)):
+    aabb = np.ascontiguousarray(aabb)
+    dtype = aabb.dtype
+    body = np.ascontiguousarray(body, np.int32)
+    ng = len(body)
+    cap = max(16, ng * ng)
+    out = np.zeros(2 * cap, np.int32)
+    catp = _p(np.ascontiguousarray(cat, np.uint32)) if cat is not None else None
+    colp = _p(np.ascontiguousarray(col, np.uint32)) if col is not None else None
+    if space == "sap":
+        n = getattr(lib(), "orc_pairs_sap_" + _real(dtype))(ctypes.c_int(ng), _p(aabb), _p(body), catp, colp, _p(out), ctypes.c_int(cap))
+    else:
+        n = getattr(lib(), "orc_pairs_hash_" + _real(dtype))(ctypes.c_int(ng), _p(aabb), _p(body), catp, colp,
+                                                           ctypes.c_int(levels[0]), ctypes.c_int(levels[1]), _p(out), ctypes.c_int(cap))
+    return out[: 2 * n].reshape(n, 2)
+
+
+def contact_rows(contact, surface, body1, body2=None, reversed_=False, fps=100.0, erp=0.0, min_depth=0.0, max_vel=np.inf):
+    contact = np.ascontiguousarray(contact, np.float64)
+    surface = np.ascontiguousarray(surface, np.float64)
+    body1 = np.ascontiguousarray(body1, np.float64)
+    b2 = np.ascontiguousarray(body2, np.float64) if body2 is not None else None
+    J = np.zeros((6, 16), np.float64)
+    fi = np.zeros(6, np.int32)
+    m = lib().orc_contact_rows_f64(_p(contact), _p(surface), _p(body1), _p(b2), ctypes.c_int(int(reversed_)),
+                                   ctypes.c_double(fps), ctypes.c_double(erp), ctypes.c_double(min_depth),
+                                   ctypes.c_double(max_vel), _p(J), _p(fi))
+    return J[:m], fi[:m]
+
+
+def quickstep(bodies, joints, prm, dtype=np.float64):
+    bodies = np.ascontiguousarray(bodies, dtype).copy()
+    joints = np.ascontiguousarray(joints, dtype).reshape(-1, JOINT)
+    lam = np.zeros(max(1, 6 * len(joints)), dtype)
+    m = getattr(lib(), "orc_quickstep_" + _real(dtype))(ctypes.c_int(len(bodies)), _p(bodies), ctypes.c_int(len(joints)),
+                                                       _p(joints), _p(np.ascontiguousarray(prm, np.float64)), _p(lam),
+                                                       ctypes.c_int(len(lam)))
+    return bodies, lam[:m]
+
+
+def world_step(bodies, geoms, prm, balls=None, dtype=np.float32):
+    """One canonical world step; returns (new_bodies, contacts[nc,12], lambda[m])."""
+    bodies = np.ascontiguousarray(bodies, dtype).copy()
+    geoms = np.ascontiguousarray(geoms, dtype)
+    balls = np.zeros((0, JOINT), dtype) if balls is None else np.ascontiguousarray(balls, dtype)
+    ng = len(geoms)
+    ccap = 4 * ng * ng + 64
+    contacts = np.zeros((ccap, CONTACT), dtype)
+    lam = np.zeros(3 * ccap + 3 * len(balls) + 8, dtype)
+    m = ctypes.c_int(0)
+    nc = getattr(lib(), "orc_world_step_" + _real(dtype))(
+        ctypes.c_int(len(bodies)), _p(bodies), ctypes.c_int(ng), _p(geoms), ctypes.c_int(len(balls)), _p(balls),
+        _p(np.ascontiguousarray(prm, np.float64)), _p(contacts), ctypes.c_int(ccap), _p(lam), ctypes.c_int(len(lam)),
+        ctypes.byref(m))
+    return bodies, contacts[:nc], lam[: m.value]
+
+
+def contacts_to_joints(contacts, geoms):
+    """Contact records -> JOINT records (type 0, body(g1), body(g2), pos, normal, depth)."""
+    nc = len(contacts)
+    j = np.zeros((nc, JOINT), np.float64)
+    if nc:
+        g1 = contacts[:, 7].astype(int)
+        g2 = contacts[:, 8].astype(int)
+        j[:, 1] = geoms[g1, 1]
+        j[:, 2] = geoms[g2, 1]
+        j[:, 3:6] = contacts[:, 0:3]
+        j[:, 6:9] = contacts[:, 3:6]
+        j[:, 9] = contacts[:, 6]
+    return j
+
+
+def lcp_residual(bodies, joints, prm, lam, balls=None):
+    bodies = np.ascontiguousarray(bodies, np.float64)
+    joints = np.ascontiguousarray(joints, np.float64).reshape(-1, JOINT)
+    balls = np.zeros((0, JOINT)) if balls is None else np.ascontiguousarray(balls, np.float64)
+    lam = np.ascontiguousarray(lam, np.float64)
+    return lib().orc_lcp_residual(ctypes.c_int(len(bodies)), _p(bodies), ctypes.c_int(len(balls)), _p(balls),
+                                  ctypes.c_int(len(joints)), _p(joints), _p(np.ascontiguousarray(prm, np.float64)), _p(lam))
+
+
+def solve_to_residual(bodies, joints, prm, target, maxit, balls=None):
+    bodies = np.ascontiguousarray(bodies, np.float64)
+    joints = np.ascontiguousarray(joints, np.float64).reshape(-1, JOINT)
+    balls = np.zeros((0, JOINT)) if balls is None else np.ascontiguousarray(balls, np.float64)
+    lam = np.zeros(3 * len(joints) + 3 * len(balls) + 1)
+    res = ctypes.c_double(0)
+    it = lib().orc_solve_to_residual(ctypes.c_int(len(bodies)), _p(bodies), ctypes.c_int(len(balls)), _p(balls),
+                                     ctypes.c_int(len(joints)), _p(joints), _p(np.ascontiguousarray(prm, np.float64)),
+                                     ctypes.c_double(target), ctypes.c_int(maxit), _p(lam), ctypes.byref(res))
+    return it, res.value, lam

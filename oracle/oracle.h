@@ -1,0 +1,128 @@
+/*
+ * oracle.h -- CPU oracle for the ODE step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain, sequential C++ restatement of the
+ * reference algorithm (Open Dynamics Engine 0.16, thomasmarsh/ODE) used as the
+ * checker for the HIP implementation in ode_amd/.  Only tests/, the smoke()
+ * entry point and bench.py's CPU-baseline leg may load it.  The product path
+ * never calls it.
+ *
+ * Every exported function exists twice: *_f32 computes in float (the reference's
+ * dSINGLE build) and *_f64 in double (dDOUBLE).  Record layouts are shared with
+ * the GPU library (include/odx.h) so the same numpy buffers feed both.
+ *
+ * Record layouts (element type = the variant's real type):
+ *   BODY  [32]: 0-2 pos, 4-7 q(w,x,y,z), 8-10 lvel, 12-14 avel, 16-18 facc,
+ *               20-22 tacc, 24 mass, 25-30 I (xx,yy,zz,xy,xz,yz),
+ *               31 flags (bit0 = no gravity, bit1 = gyroscopic)
+ *   GEOM  [16]: 0 class (0 sphere, 1 box, 2 capsule, 4 plane), 1 body (-1 = static),
+ *               2-5 params (sphere r | box lx,ly,lz | capsule r,len | plane a,b,c,d),
+ *               6-8 static pos, 9-12 static q
+ *   CONTACT [12]: 0-2 pos, 3-5 normal, 6 depth, 7 g1, 8 g2, 9 side1, 10 side2
+ *   PARAMS (double[32]) : see ORC_P_* below.
+ */
+#ifndef ODX_ORACLE_H
+#define ODX_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ORC_BODY_STRIDE = 32,
+    ORC_GEOM_STRIDE = 16,
+    ORC_CONTACT_STRIDE = 12,
+    ORC_JOINT_STRIDE = 16,
+};
+
+enum {
+    ORC_P_H = 0, ORC_P_GX, ORC_P_GY, ORC_P_GZ,
+    ORC_P_ERP, ORC_P_CFM, ORC_P_SOR_W, ORC_P_ITERS,
+    ORC_P_MAX_VEL, ORC_P_MIN_DEPTH,
+    ORC_P_MODE, ORC_P_MU, ORC_P_MU2, ORC_P_BOUNCE, ORC_P_BOUNCE_VEL,
+    ORC_P_SOFT_ERP, ORC_P_SOFT_CFM, ORC_P_MAXC, ORC_P_SPACE,
+    ORC_P_MOTION1, ORC_P_MOTION2, ORC_P_MOTIONN, ORC_P_SLIP1, ORC_P_SLIP2,
+    ORC_P_SKIP_CONNECTED, ORC_P_LAMBDA_FLOOR,
+    ORC_P__MAX = 32
+};
+
+/* dRand LCG (misc.cpp:35-48) and dRandInt (misc.cpp:78-130). */
+unsigned int orc_drand(unsigned int *seed);
+int orc_drand_int(unsigned int *seed, int n);
+
+/* dCollide semantics for one geom pair; returns contact count. */
+int orc_collide_f32(const float *g1, const float *g2, const float *b1, const float *b2,
+                    int flags, float *contacts);
+int orc_collide_f64(const double *g1, const double *g2, const double *b1, const double *b2,
+                    int flags, double *contacts);
+
+/* Geom AABBs [ng][6] for the given bodies. */
+void orc_aabbs_f32(int ng, const float *geoms, const float *bodies, float *aabb);
+void orc_aabbs_f64(int ng, const double *geoms, const double *bodies, double *aabb);
+
+/* Broadphase pair sets.  sap: dxSAPSpace::collide emission order (axes XYZ).
+ * hash: dxHashSpace semantics as the canonical sorted set.  Each returns the
+ * number of pairs; pairs[2k], pairs[2k+1] are geom indices (first, second as
+ * passed to the near callback).  cat/col may be NULL (all bits set). */
+int orc_pairs_sap_f32(int ng, const float *aabb, const int *body, const unsigned *cat,
+                      const unsigned *col, int *pairs, int cap);
+int orc_pairs_hash_f32(int ng, const float *aabb, const int *body, const unsigned *cat,
+                       const unsigned *col, int minlevel, int maxlevel, int *pairs, int cap);
+int orc_pairs_sap_f64(int ng, const double *aabb, const int *body, const unsigned *cat,
+                      const unsigned *col, int *pairs, int cap);
+int orc_pairs_hash_f64(int ng, const double *aabb, const int *body, const unsigned *cat,
+                       const unsigned *col, int minlevel, int maxlevel, int *pairs, int cap);
+
+/* Contact joint rows (dxJointContact::getInfo1/getInfo2) for one contact.
+ * surface: double[16] = mode, mu, mu2, bounce, bounce_vel, soft_erp, soft_cfm,
+ *          motion1, motion2, motionN, slip1, slip2, fdir1[3].
+ * Writes m rows of 16 reals (J layout l1 a1 rhs cfm l2 a2 lo hi) and findex[m];
+ * returns m. */
+int orc_contact_rows_f64(const double *contact, const double *surface, const double *body1,
+                         const double *body2, int reversed, double fps, double erp,
+                         double min_depth, double max_vel, double *J, int *findex);
+
+/* One QuickStep over an explicit joint list (joints in island order).
+ * JOINT record [16]: 0 type (0 contact, 1 ball), 1 b1, 2 b2 (-1 none),
+ *   contact: 3-5 pos, 6-8 normal, 9 depth, 10-12 fdir1
+ *   ball:    3-5 anchor1 (body1 frame), 6-8 anchor2 (body2 frame or world)
+ * Updates bodies in place, writes lambda in natural row order (rows per joint
+ * back to back) and returns m.  lambda may be NULL. */
+int orc_quickstep_f32(int nb, float *bodies, int nj, const float *joints,
+                      const double *params, float *lambda, int lambda_cap);
+int orc_quickstep_f64(int nb, double *bodies, int nj, const double *joints,
+                      const double *params, double *lambda, int lambda_cap);
+
+/* Full canonical world step: AABBs -> pairs (geom index order) -> dCollide ->
+ * contact joints -> QuickStep -> integrate.  Ball joints given in balls[nball][16]
+ * come first in the joint list.  Outputs contacts [cap][12] and lambda.  Returns
+ * the contact count; *m_out receives the row count. */
+int orc_world_step_f32(int nb, float *bodies, int ng, const float *geoms, int nball,
+                       const float *balls, const double *params, float *contacts,
+                       int contact_cap, float *lambda, int lambda_cap, int *m_out);
+int orc_world_step_f64(int nb, double *bodies, int ng, const double *geoms, int nball,
+                       const double *balls, const double *params, double *contacts,
+                       int contact_cap, double *lambda, int lambda_cap, int *m_out);
+
+/* LCP residual (double precision) of lambda for the world state *before* the
+ * step (bodies as they were at step start) and the given contacts.  The rows
+ * are rebuilt in double; res = max|clamp(l - w/Aii) - l| / max(max|l|, floor). */
+double orc_lcp_residual(int nb, const double *bodies, int nball, const double *balls,
+                        int nc, const double *contacts, const double *params,
+                        const double *lambda);
+
+/* Cold solve to a target residual in double: returns iterations used (<= maxit)
+ * and writes lambda (natural row order). */
+int orc_solve_to_residual(int nb, const double *bodies, int nball, const double *balls,
+                          int nc, const double *contacts, const double *params,
+                          double target, int maxit, double *lambda, double *res_out);
+
+/* dxSafeNormalize3 with the dNormalize3 fallback; dxPlaneSpace. */
+void orc_normalize3_f32(float *a);
+void orc_normalize3_f64(double *a);
+void orc_plane_space_f64(const double *n, double *p, double *q);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
