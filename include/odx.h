@@ -1,0 +1,114 @@
+/*
+ * odx.h -- C ABI of libodx: the ODE step hot path on AMD Instinct MI355X.
+ *
+ * Two groups of entry points:
+ *
+ *  1. The ODE-compatible subset an existing ODE program calls on its step path
+ *     (see ode.h in this directory): dWorldQuickStep, dSpaceCollide with its
+ *     near-callback, dCollide, dJointCreateContact and the world/body/geom/
+ *     joint-group calls they need.  Same names, argument meaning and return
+ *     codes as the reference's include/ode/ *.h; dReal is float.
+ *
+ *  2. A batched-worlds extension (odxBatch*): thousands of independent worlds
+ *     stepped by one launch sequence.  No ODE counterpart; it replaces the
+ *     caller's frame loop (dSpaceCollide + near-callback + dWorldQuickStep +
+ *     dJointGroupEmpty, e.g. ode/demo/demo_space_stress.cpp:363-369) for many
+ *     worlds at once, with the near-callback policy given as a surface table.
+ *
+ * Plain pointers and sizes only; no exceptions cross this boundary.  Functions
+ * returning int return 1 (or a count) on success and 0 on failure, with a
+ * message through the handler installed by odxSetMessageHandler.
+ *
+ * Record layouts (float arrays):
+ *   BODY  [32]: 0-2 pos, 4-7 q (w,x,y,z), 8-10 lvel, 12-14 avel, 16-18 facc,
+ *               20-22 tacc, 24 mass, 25-30 I (xx,yy,zz,xy,xz,yz),
+ *               31 flags (bit0 no gravity, bit1 gyroscopic)
+ *   GEOM  [16]: 0 class (0 sphere, 1 box, 2 capsule, 4 plane), 1 body (world-
+ *               local index, -1 static), 2-5 params (sphere r | box lx,ly,lz |
+ *               capsule r,len | plane a,b,c,d), 6-8 static pos, 9-12 static q
+ *   JOINT [16]: 0 type (0 contact, 1 ball), 1 body1, 2 body2 (-1 none);
+ *               contact: 3-5 pos 6-8 normal 9 depth 10-12 fdir1 13 surface 14 g1 15 g2
+ *               ball:    3-5 anchor1 (body1 frame) 6-8 anchor2 (body2 frame or world) 9 erp 10 cfm
+ *   PARAMS double[32]: 0 h, 1-3 gravity, 4 erp, 5 cfm, 6 sor w, 7 iterations,
+ *               8 contact max correcting vel, 9 contact surface layer,
+ *               10 surface mode, 11 mu, 12 mu2, 13 bounce, 14 bounce_vel,
+ *               15 soft_erp, 16 soft_cfm, 17 max contacts per pair, 18 space
+ *               (0 sweep-and-prune, 1 hash), 19 motion1, 20 motion2, 21 motionN,
+ *               22 slip1, 23 slip2, 24 skip pairs joined by a ball joint,
+ *               25 residual floor
+ */
+#ifndef ODX_H
+#define ODX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ODX_BODY_STRIDE 32
+#define ODX_GEOM_STRIDE 16
+#define ODX_JOINT_STRIDE 16
+#define ODX_FEAT_DIM 16
+
+typedef void odxMessageFunction(int errnum, const char *msg);
+
+/* Library version string and a message sink (NULL restores stderr). */
+const char *odxVersion(void);
+void odxSetMessageHandler(odxMessageFunction *fn);
+/* Number of visible GPUs (0 when none). */
+int odxDeviceCount(void);
+
+/* ------------------------------------------------------------ batched API */
+typedef struct odxBatch odxBatch;
+
+/* Creates nworlds worlds on the current device.  body_start/geom_start/
+ * ball_start are prefix offsets [nworlds+1] into the record arrays; bodies are
+ * world-local in GEOM and JOINT records.  Worlds may hold at most 64 bodies and
+ * 96 geoms.  contacts_per_body sizes each world's contact capacity
+ * (capacity = contacts_per_body * bodies + 16).  Returns NULL on failure. */
+odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies, const int *geom_start,
+                         const float *geoms, const int *ball_start, const float *balls, const double *params,
+                         int contacts_per_body);
+void odxBatchDestroy(odxBatch *b);
+
+/* Solver controls.  order: 0 = reference row order (deterministic; results
+ * match a sequential QuickStep with row reordering off), 1 = coloured contact
+ * blocks.  tol <= 0 runs exactly max_iters sweeps; otherwise sweeps stop per
+ * world once the LCP residual is <= tol (checked every check_every sweeps),
+ * capped at max_iters.  warm: 0 cold start, 1 lambda0 from odxBatchSetLambda0. */
+int odxBatchSetSolver(odxBatch *b, int order, int max_iters, float tol, int check_every, int warm);
+
+/* Runs nsteps steps on the batch's stream (captured once into a hipGraph and
+ * replayed).  Returns 1 on success. */
+int odxBatchStep(odxBatch *b, int nsteps);
+/* Blocks until queued work is done. */
+int odxBatchSync(odxBatch *b);
+
+/* Times nsteps graph-replayed steps with hipEvents on the batch stream
+ * (total milliseconds). */
+int odxBatchTime(odxBatch *b, int nsteps, float *total_ms);
+/* Runs nsteps directly launched steps with hipEvents around each kernel and
+ * returns the average durations [prepare, solve] in milliseconds. */
+int odxBatchKernelTimes(odxBatch *b, int nsteps, float *kernel_ms);
+
+/* Readback.  bodies: [NB][32].  stats per world: contacts, rows, sweeps,
+ * residual (NULL pointers are skipped).  Returns the total count written. */
+int odxBatchGetBodies(odxBatch *b, float *bodies);
+int odxBatchSetBodies(odxBatch *b, const float *bodies);
+int odxBatchGetStats(odxBatch *b, int *ncontacts, int *nrows, int *niters, float *residual);
+/* Joint records of world w after the last step (balls first, then contacts). */
+int odxBatchGetJoints(odxBatch *b, int w, float *joints, int cap);
+/* Lambda of world w in natural row order. */
+int odxBatchGetLambda(odxBatch *b, int w, float *lambda, int cap);
+/* Assembled rows of world w ([m][32]: J1 rhs cfm J2 lo hi iMJ1 iMJ2 b1 b2 findex Ad). */
+int odxBatchGetRows(odxBatch *b, int w, float *rows, int cap);
+/* Overflow counters: [pairs, contacts, rows, phases] dropped since creation. */
+int odxBatchGetOverflow(odxBatch *b, int *counts);
+/* Row-capacity offsets [nworlds+1] (the lambda0 layout). */
+int odxBatchRowStarts(odxBatch *b, int *row_start);
+/* Warm-start values per row in the row-capacity layout (host array). */
+int odxBatchSetLambda0(odxBatch *b, const float *lambda0, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

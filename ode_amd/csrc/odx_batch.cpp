@@ -1,0 +1,535 @@
+// odx_batch.cpp -- host runtime of the batched step path (odxBatch* in odx.h).
+//
+// Owns the device SoA state of many independent worlds, builds per-world
+// capacities, launches the prepare/solve kernels on one stream and replays the
+// step as a hipGraph.  Reports errors through the message handler; nothing
+// here throws across the C ABI.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/odx.h"
+#include "odx_device.h"
+#include "odx_host.h"
+
+namespace odx {
+void launch_prepare_small(const DevWorlds &D, const StepParams &P, hipStream_t s);
+void launch_solve_small(const DevWorlds &D, const StepParams &P, hipStream_t s);
+}  // namespace odx
+
+using namespace odx;
+
+namespace {
+odxMessageFunction *g_handler = nullptr;
+}
+
+void odx_message(int err, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (g_handler) g_handler(err, buf);
+    else fprintf(stderr, "odx: %s\n", buf);
+}
+
+bool odx_check(hipError_t e, const char *what) {
+    if (e == hipSuccess) return true;
+    odx_message(int(e), "%s failed: %s", what, hipGetErrorString(e));
+    return false;
+}
+
+// Body-frame inverse inertia by Cholesky factorisation, as dInvertPDMatrix
+// (matrix.cpp:563) does for dBodySetMass (ode.cpp:488-504).  Falls back to the
+// identity for a non positive-definite tensor, as dBodySetMass does.
+void odx_invert_pd3(const float *I, float *out) {
+    float a[3][3] = {{I[0], I[1], I[2]}, {I[4], I[5], I[6]}, {I[8], I[9], I[10]}};
+    float L[3][3] = {{0.f}}, Li[3][3] = {{0.f}};
+    bool ok = true;
+    for (int i = 0; i < 3 && ok; i++) {
+        for (int j = 0; j <= i; j++) {
+            float s = a[i][j];
+            for (int k = 0; k < j; k++) s -= L[i][k] * L[j][k];
+            if (i == j) {
+                if (s <= 0.f) { ok = false; break; }
+                L[i][i] = std::sqrt(s);
+            } else {
+                L[i][j] = s / L[j][j];
+            }
+        }
+    }
+    std::memset(out, 0, 12 * sizeof(float));
+    if (!ok) { out[0] = out[5] = out[10] = 1.f; return; }
+    for (int i = 0; i < 3; i++) {
+        Li[i][i] = 1.f / L[i][i];
+        for (int j = 0; j < i; j++) {
+            float s = 0.f;
+            for (int k = j; k < i; k++) s += L[i][k] * Li[k][j];
+            Li[i][j] = -s / L[i][i];
+        }
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            float s = 0.f;
+            for (int k = 0; k < 3; k++) s += Li[k][i] * Li[k][j];
+            out[i * 4 + j] = s;
+        }
+}
+
+template <class T> static bool dalloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return odx_check(hipMalloc((void **)p, n * sizeof(T)), "hipMalloc");
+}
+template <class T> static bool upload(T *d, const T *h, size_t n) {
+    if (n == 0) return true;
+    return odx_check(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+template <class T> static bool download(T *h, const T *d, size_t n) {
+    if (n == 0) return true;
+    return odx_check(hipMemcpy(h, d, n * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+
+struct odxBatch {
+    int nw = 0, nb = 0, ng = 0, nball = 0, jcap = 0, rcap = 0;
+    std::vector<int> body_start, geom_start, ball_start, joint_start, row_start;
+    std::vector<void *> allocs;
+    DevWorlds D{};
+    StepParams P{};
+    Surface surf{};
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    float *lambda0 = nullptr;
+
+    template <class T> bool alloc(T **p, size_t n) {
+        if (!dalloc(p, n)) return false;
+        allocs.push_back((void *)*p);
+        return true;
+    }
+    ~odxBatch() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+        for (void *p : allocs) (void)hipFree(p);
+    }
+    void drop_graph() {
+        if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }
+    }
+    bool enqueue_step() {
+        launch_prepare_small(D, P, stream);
+        if (!odx_check(hipGetLastError(), "k_prepare_small launch")) return false;
+        launch_solve_small(D, P, stream);
+        return odx_check(hipGetLastError(), "k_solve_small launch");
+    }
+};
+
+static void params_from(const double *p, StepParams &P, Surface &S) {
+    P.h = (float)p[0];
+    P.gx = (float)p[1]; P.gy = (float)p[2]; P.gz = (float)p[3];
+    P.erp = (float)p[4]; P.cfm = (float)p[5]; P.sor_w = (float)p[6];
+    P.max_iters = (int)p[7];
+    P.tol = 0.f;
+    P.check_every = 1;
+    P.max_vel = (float)p[8]; P.min_depth = (float)p[9];
+    P.maxc = (int)p[17] > 0 ? (int)p[17] : 1;
+    P.space = (int)p[18];
+    P.skip_connected = (int)p[24];
+    P.order = 0;
+    P.warm = 0;
+    P.lambda_floor = p[25] > 0 ? (float)p[25] : 1e-3f;
+    P.final_residual = 1;
+    P.collide = 1;
+    P.features = 0;
+    S.mode = (int)p[10];
+    S.mu = (float)p[11]; S.mu2 = (float)p[12]; S.bounce = (float)p[13]; S.bounce_vel = (float)p[14];
+    S.soft_erp = (float)p[15]; S.soft_cfm = (float)p[16];
+    S.motion1 = (float)p[19]; S.motion2 = (float)p[20]; S.motionN = (float)p[21];
+    S.slip1 = (float)p[22]; S.slip2 = (float)p[23];
+}
+
+extern "C" {
+
+const char *odxVersion(void) { return "odx 0.1 (gfx950)"; }
+
+void odxSetMessageHandler(odxMessageFunction *fn) { g_handler = fn; }
+
+int odxDeviceCount(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies, const int *geom_start,
+                         const float *geoms, const int *ball_start, const float *balls, const double *params,
+                         int contacts_per_body) {
+    if (nworlds <= 0 || !body_start || !bodies || !geom_start || !geoms || !params) {
+        odx_message(1, "odxBatchCreate: bad arguments");
+        return nullptr;
+    }
+    auto *B = new odxBatch();
+    B->nw = nworlds;
+    B->body_start.assign(body_start, body_start + nworlds + 1);
+    B->geom_start.assign(geom_start, geom_start + nworlds + 1);
+    if (ball_start) B->ball_start.assign(ball_start, ball_start + nworlds + 1);
+    else B->ball_start.assign(nworlds + 1, 0);
+    B->nb = B->body_start[nworlds];
+    B->ng = B->geom_start[nworlds];
+    B->nball = B->ball_start[nworlds];
+    if (contacts_per_body <= 0) contacts_per_body = 8;
+    B->joint_start.resize(nworlds + 1);
+    B->row_start.resize(nworlds + 1);
+    B->joint_start[0] = 0;
+    B->row_start[0] = 0;
+    for (int w = 0; w < nworlds; w++) {
+        int nbw = B->body_start[w + 1] - B->body_start[w];
+        int ngw = B->geom_start[w + 1] - B->geom_start[w];
+        int nlw = B->ball_start[w + 1] - B->ball_start[w];
+        if (nbw < 0 || nbw > SW_MAXB || ngw < 0 || ngw > SW_MAXG) {
+            odx_message(2, "odxBatchCreate: world %d has %d bodies / %d geoms (limits %d / %d)", w, nbw, ngw, SW_MAXB,
+                        SW_MAXG);
+            delete B;
+            return nullptr;
+        }
+        // every body index a kernel will dereference must lie inside its world
+        for (int g = B->geom_start[w]; g < B->geom_start[w + 1]; g++) {
+            const int gb = (int)geoms[(size_t)g * GEOM_STRIDE + 1];
+            const int cls = (int)geoms[(size_t)g * GEOM_STRIDE + 0];
+            if (gb < -1 || gb >= nbw || (cls != 0 && cls != 1 && cls != 2 && cls != 4)) {
+                odx_message(2, "odxBatchCreate: geom %d of world %d has class %d / body %d", g, w, cls, gb);
+                delete B;
+                return nullptr;
+            }
+        }
+        for (int l = B->ball_start[w]; l < B->ball_start[w + 1]; l++) {
+            const float *r = balls + (size_t)l * JOINT_STRIDE;
+            const int x1 = (int)r[1], x2 = (int)r[2];
+            if ((int)r[0] != 1 || x1 < 0 || x1 >= nbw || x2 < -1 || x2 >= nbw || x1 == x2) {
+                odx_message(2, "odxBatchCreate: ball joint %d of world %d has bodies %d,%d", l, w, x1, x2);
+                delete B;
+                return nullptr;
+            }
+        }
+        int cc = contacts_per_body * nbw + 16;
+        int jc = nlw + cc;
+        int rc = 3 * jc;
+        if (rc > SW_MAXROWS) { rc = SW_MAXROWS; jc = rc / 3; }
+        B->joint_start[w + 1] = B->joint_start[w] + jc;
+        B->row_start[w + 1] = B->row_start[w] + rc;
+    }
+    B->jcap = B->joint_start[nworlds];
+    B->rcap = B->row_start[nworlds];
+    params_from(params, B->P, B->surf);
+
+    // host-side SoA conversion
+    const int nb = B->nb, ng = B->ng;
+    std::vector<float4> pos(nb), quat(nb), lvel(nb), avel(nb), facc(nb), tacc(nb), mass(nb);
+    std::vector<float> Ib((size_t)nb * 12), invIb((size_t)nb * 12);
+    for (int i = 0; i < nb; i++) {
+        const float *r = bodies + (size_t)i * BODY_STRIDE;
+        pos[i] = make_float4(r[0], r[1], r[2], 0.f);
+        float q[4] = {r[4], r[5], r[6], r[7]};
+        float l = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+        if (l > 0.f) { l = 1.f / std::sqrt(l); for (float &x : q) x *= l; }
+        else { q[0] = 1.f; q[1] = q[2] = q[3] = 0.f; }
+        quat[i] = make_float4(q[0], q[1], q[2], q[3]);
+        lvel[i] = make_float4(r[8], r[9], r[10], 0.f);
+        avel[i] = make_float4(r[12], r[13], r[14], 0.f);
+        facc[i] = make_float4(r[16], r[17], r[18], 0.f);
+        tacc[i] = make_float4(r[20], r[21], r[22], 0.f);
+        mass[i] = make_float4(r[24], 1.f / r[24], r[31], 0.f);
+        float *I = &Ib[(size_t)i * 12];
+        I[0] = r[25]; I[5] = r[26]; I[10] = r[27];
+        I[1] = I[4] = r[28]; I[2] = I[8] = r[29]; I[6] = I[9] = r[30];
+        I[3] = I[7] = I[11] = 0.f;
+        odx_invert_pd3(I, &invIb[(size_t)i * 12]);
+    }
+    std::vector<float4> g0(ng), g1(ng), gp(ng), gq(ng);
+    for (int i = 0; i < ng; i++) {
+        const float *r = geoms + (size_t)i * GEOM_STRIDE;
+        g0[i] = make_float4(r[0], r[1], r[2], r[3]);
+        g1[i] = make_float4(r[4], r[5], 0.f, 0.f);
+        gp[i] = make_float4(r[6], r[7], r[8], 0.f);
+        float q[4] = {r[9], r[10], r[11], r[12]};
+        float l = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+        if (l > 0.f) { l = 1.f / std::sqrt(l); for (float &x : q) x *= l; }
+        else { q[0] = 1.f; q[1] = q[2] = q[3] = 0.f; }
+        gq[i] = make_float4(q[0], q[1], q[2], q[3]);
+    }
+
+    DevWorlds &D = B->D;
+    D.nworlds = nworlds;
+    int *ds = nullptr;
+    bool ok = true;
+    ok = ok && B->alloc(&ds, nworlds + 1) && upload(ds, B->body_start.data(), nworlds + 1);
+    D.body_start = ds;
+    ok = ok && B->alloc(&ds, nworlds + 1) && upload(ds, B->geom_start.data(), nworlds + 1);
+    D.geom_start = ds;
+    ok = ok && B->alloc(&ds, nworlds + 1) && upload(ds, B->ball_start.data(), nworlds + 1);
+    D.ball_start = ds;
+    ok = ok && B->alloc(&ds, nworlds + 1) && upload(ds, B->joint_start.data(), nworlds + 1);
+    D.joint_start = ds;
+    ok = ok && B->alloc(&ds, nworlds + 1) && upload(ds, B->row_start.data(), nworlds + 1);
+    D.row_start = ds;
+    ok = ok && B->alloc(&D.pos, nb) && upload(D.pos, pos.data(), nb);
+    ok = ok && B->alloc(&D.quat, nb) && upload(D.quat, quat.data(), nb);
+    ok = ok && B->alloc(&D.lvel, nb) && upload(D.lvel, lvel.data(), nb);
+    ok = ok && B->alloc(&D.avel, nb) && upload(D.avel, avel.data(), nb);
+    ok = ok && B->alloc(&D.facc, nb) && upload(D.facc, facc.data(), nb);
+    ok = ok && B->alloc(&D.tacc, nb) && upload(D.tacc, tacc.data(), nb);
+    float4 *m4 = nullptr;
+    ok = ok && B->alloc(&m4, nb) && upload(m4, mass.data(), nb);
+    D.mass = m4;
+    float *f = nullptr;
+    ok = ok && B->alloc(&f, (size_t)nb * 12) && upload(f, invIb.data(), (size_t)nb * 12);
+    D.invIb = f;
+    ok = ok && B->alloc(&f, (size_t)nb * 12) && upload(f, Ib.data(), (size_t)nb * 12);
+    D.Ib = f;
+    float4 *g4 = nullptr;
+    ok = ok && B->alloc(&g4, ng) && upload(g4, g0.data(), ng);
+    D.geom0 = g4;
+    ok = ok && B->alloc(&g4, ng) && upload(g4, g1.data(), ng);
+    D.geom1 = g4;
+    ok = ok && B->alloc(&g4, ng) && upload(g4, gp.data(), ng);
+    D.gpos = g4;
+    ok = ok && B->alloc(&g4, ng) && upload(g4, gq.data(), ng);
+    D.gquat = g4;
+    ok = ok && B->alloc(&f, (size_t)B->nball * JOINT_STRIDE) &&
+         upload(f, balls ? balls : (const float *)nullptr, balls ? (size_t)B->nball * JOINT_STRIDE : 0);
+    D.balls = f;
+    Surface *sd = nullptr;
+    ok = ok && B->alloc(&sd, 1) && upload(sd, &B->surf, 1);
+    D.surf = sd;
+    ok = ok && B->alloc(&D.joints, (size_t)B->jcap * JOINT_STRIDE);
+    ok = ok && B->alloc(&D.njoints, nworlds) && B->alloc(&D.ncontacts, nworlds) && B->alloc(&D.nrows, nworlds) &&
+         B->alloc(&D.nphases, nworlds) && B->alloc(&D.niters, nworlds) && B->alloc(&D.residual, nworlds);
+    ok = ok && B->alloc(&D.rows, (size_t)B->rcap * ROW_STRIDE) && B->alloc(&D.lambda, B->rcap);
+    ok = ok && B->alloc(&D.sched, B->rcap) && B->alloc(&D.phase_start, (size_t)B->rcap + nworlds);
+    ok = ok && B->alloc(&D.bscr, (size_t)nb * BSCR_STRIDE);
+    ok = ok && B->alloc(&D.feat, (size_t)B->jcap * FEAT_DIM);
+    ok = ok && B->alloc(&D.prev, (size_t)B->jcap * JOINT_STRIDE) && B->alloc(&D.nprev, nworlds);
+    ok = ok && B->alloc(&D.overflow, 4);
+    ok = ok && B->alloc(&B->lambda0, B->rcap);
+    if (ok) {
+        ok = odx_check(hipMemset(D.overflow, 0, 4 * sizeof(int)), "hipMemset") &&
+             odx_check(hipMemset(D.nrows, 0, nworlds * sizeof(int)), "hipMemset") &&
+             odx_check(hipMemset(D.ncontacts, 0, nworlds * sizeof(int)), "hipMemset") &&
+             odx_check(hipMemset(D.niters, 0, nworlds * sizeof(int)), "hipMemset") &&
+             odx_check(hipMemset(D.nprev, 0, nworlds * sizeof(int)), "hipMemset") &&
+             odx_check(hipMemset(B->lambda0, 0, (size_t)B->rcap * sizeof(float)), "hipMemset") &&
+             odx_check(hipMemset(D.lambda, 0, (size_t)B->rcap * sizeof(float)), "hipMemset");
+    }
+    D.lambda0 = B->lambda0;
+    ok = ok && odx_check(hipStreamCreate(&B->stream), "hipStreamCreate");
+    for (auto &e : B->ev) ok = ok && odx_check(hipEventCreate(&e), "hipEventCreate");
+    if (!ok) {
+        delete B;
+        return nullptr;
+    }
+    return B;
+}
+
+void odxBatchDestroy(odxBatch *b) { delete b; }
+
+int odxBatchSetSolver(odxBatch *b, int order, int max_iters, float tol, int check_every, int warm) {
+    if (!b || max_iters < 0 || check_every <= 0 || order < 0 || order > 1) {
+        odx_message(1, "odxBatchSetSolver: bad arguments");
+        return 0;
+    }
+    b->P.order = order;
+    b->P.max_iters = max_iters;
+    b->P.tol = tol;
+    b->P.check_every = check_every;
+    b->P.warm = warm;
+    b->drop_graph();
+    return 1;
+}
+
+int odxBatchStep(odxBatch *b, int nsteps) {
+    if (!b || nsteps < 0) return 0;
+    if (nsteps == 0) return 1;
+    if (!b->graph) {
+        hipGraph_t g = nullptr;
+        if (!odx_check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
+            return 0;
+        bool ok = b->enqueue_step();
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (!ok || !odx_check(e, "hipStreamEndCapture")) {
+            if (g) (void)hipGraphDestroy(g);
+            return 0;
+        }
+        e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (!odx_check(e, "hipGraphInstantiate")) { b->graph = nullptr; return 0; }
+    }
+    for (int s = 0; s < nsteps; s++)
+        if (!odx_check(hipGraphLaunch(b->graph, b->stream), "hipGraphLaunch")) return 0;
+    return 1;
+}
+
+int odxBatchSync(odxBatch *b) {
+    if (!b) return 0;
+    return odx_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize") ? 1 : 0;
+}
+
+int odxBatchTime(odxBatch *b, int nsteps, float *total_ms) {
+    if (!b || nsteps <= 0) return 0;
+    if (!odxBatchStep(b, 0) || !odxBatchSync(b)) return 0;
+    if (!b->graph && !odxBatchStep(b, 1)) return 0;  // instantiate outside the timed region
+    if (!odxBatchSync(b)) return 0;
+    if (!odx_check(hipEventRecord(b->ev[0], b->stream), "hipEventRecord")) return 0;
+    if (!odxBatchStep(b, nsteps)) return 0;
+    if (!odx_check(hipEventRecord(b->ev[1], b->stream), "hipEventRecord")) return 0;
+    if (!odx_check(hipEventSynchronize(b->ev[1]), "hipEventSynchronize")) return 0;
+    float tot = 0.f;
+    if (!odx_check(hipEventElapsedTime(&tot, b->ev[0], b->ev[1]), "hipEventElapsedTime")) return 0;
+    if (total_ms) *total_ms = tot;
+    return 1;
+}
+
+int odxBatchKernelTimes(odxBatch *b, int nsteps, float *kernel_ms) {
+    if (!b || nsteps <= 0 || !kernel_ms) return 0;
+    if (!odxBatchSync(b)) return 0;
+    std::vector<hipEvent_t> ev(2 * (size_t)nsteps + 1, nullptr);
+    bool ok = true;
+    for (auto &e : ev) ok = ok && odx_check(hipEventCreate(&e), "hipEventCreate");
+    for (int s = 0; ok && s < nsteps; s++) {
+        ok = ok && odx_check(hipEventRecord(ev[2 * s], b->stream), "hipEventRecord");
+        launch_prepare_small(b->D, b->P, b->stream);
+        ok = ok && odx_check(hipGetLastError(), "k_prepare_small launch");
+        ok = ok && odx_check(hipEventRecord(ev[2 * s + 1], b->stream), "hipEventRecord");
+        launch_solve_small(b->D, b->P, b->stream);
+        ok = ok && odx_check(hipGetLastError(), "k_solve_small launch");
+    }
+    ok = ok && odx_check(hipEventRecord(ev[2 * (size_t)nsteps], b->stream), "hipEventRecord");
+    ok = ok && odx_check(hipEventSynchronize(ev[2 * (size_t)nsteps]), "hipEventSynchronize");
+    double ta = 0.0, tc = 0.0;
+    for (int s = 0; ok && s < nsteps; s++) {
+        float a = 0.f, c = 0.f;
+        ok = ok && odx_check(hipEventElapsedTime(&a, ev[2 * s], ev[2 * s + 1]), "hipEventElapsedTime");
+        ok = ok && odx_check(hipEventElapsedTime(&c, ev[2 * s + 1], ev[2 * s + 2]), "hipEventElapsedTime");
+        ta += a;
+        tc += c;
+    }
+    for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+    if (!ok) return 0;
+    kernel_ms[0] = (float)(ta / nsteps);
+    kernel_ms[1] = (float)(tc / nsteps);
+    return 1;
+}
+
+int odxBatchGetBodies(odxBatch *b, float *out) {
+    if (!b || !out) return 0;
+    if (!odxBatchSync(b)) return 0;
+    const int nb = b->nb;
+    std::vector<float4> pos(nb), quat(nb), lvel(nb), avel(nb), facc(nb), tacc(nb), mass(nb);
+    std::vector<float> Ib((size_t)nb * 12);
+    bool ok = download(pos.data(), b->D.pos, nb) && download(quat.data(), b->D.quat, nb) &&
+              download(lvel.data(), b->D.lvel, nb) && download(avel.data(), b->D.avel, nb) &&
+              download(facc.data(), b->D.facc, nb) && download(tacc.data(), b->D.tacc, nb) &&
+              download(mass.data(), (float4 *)b->D.mass, nb) && download(Ib.data(), (float *)b->D.Ib, (size_t)nb * 12);
+    if (!ok) return 0;
+    for (int i = 0; i < nb; i++) {
+        float *r = out + (size_t)i * BODY_STRIDE;
+        std::memset(r, 0, BODY_STRIDE * sizeof(float));
+        r[0] = pos[i].x; r[1] = pos[i].y; r[2] = pos[i].z;
+        r[4] = quat[i].x; r[5] = quat[i].y; r[6] = quat[i].z; r[7] = quat[i].w;
+        r[8] = lvel[i].x; r[9] = lvel[i].y; r[10] = lvel[i].z;
+        r[12] = avel[i].x; r[13] = avel[i].y; r[14] = avel[i].z;
+        r[16] = facc[i].x; r[17] = facc[i].y; r[18] = facc[i].z;
+        r[20] = tacc[i].x; r[21] = tacc[i].y; r[22] = tacc[i].z;
+        r[24] = mass[i].x;
+        const float *I = &Ib[(size_t)i * 12];
+        r[25] = I[0]; r[26] = I[5]; r[27] = I[10]; r[28] = I[1]; r[29] = I[2]; r[30] = I[6];
+        r[31] = mass[i].z;
+    }
+    return nb;
+}
+
+int odxBatchSetBodies(odxBatch *b, const float *bodies) {
+    if (!b || !bodies) return 0;
+    if (!odxBatchSync(b)) return 0;
+    const int nb = b->nb;
+    std::vector<float4> pos(nb), quat(nb), lvel(nb), avel(nb), facc(nb), tacc(nb);
+    for (int i = 0; i < nb; i++) {
+        const float *r = bodies + (size_t)i * BODY_STRIDE;
+        pos[i] = make_float4(r[0], r[1], r[2], 0.f);
+        quat[i] = make_float4(r[4], r[5], r[6], r[7]);
+        lvel[i] = make_float4(r[8], r[9], r[10], 0.f);
+        avel[i] = make_float4(r[12], r[13], r[14], 0.f);
+        facc[i] = make_float4(r[16], r[17], r[18], 0.f);
+        tacc[i] = make_float4(r[20], r[21], r[22], 0.f);
+    }
+    bool ok = upload(b->D.pos, pos.data(), nb) && upload(b->D.quat, quat.data(), nb) &&
+              upload(b->D.lvel, lvel.data(), nb) && upload(b->D.avel, avel.data(), nb) &&
+              upload(b->D.facc, facc.data(), nb) && upload(b->D.tacc, tacc.data(), nb);
+    return ok ? nb : 0;
+}
+
+int odxBatchGetStats(odxBatch *b, int *ncontacts, int *nrows, int *niters, float *residual) {
+    if (!b) return 0;
+    if (!odxBatchSync(b)) return 0;
+    bool ok = true;
+    if (ncontacts) ok = ok && download(ncontacts, b->D.ncontacts, b->nw);
+    if (nrows) ok = ok && download(nrows, b->D.nrows, b->nw);
+    if (niters) ok = ok && download(niters, b->D.niters, b->nw);
+    if (residual) ok = ok && download(residual, b->D.residual, b->nw);
+    return ok ? b->nw : 0;
+}
+
+int odxBatchGetJoints(odxBatch *b, int w, float *joints, int cap) {
+    if (!b || w < 0 || w >= b->nw) return 0;
+    if (!odxBatchSync(b)) return 0;
+    int nj = 0;
+    if (!download(&nj, b->D.njoints + w, 1)) return 0;
+    int n = nj < cap ? nj : cap;
+    if (!download(joints, b->D.joints + (size_t)b->joint_start[w] * JOINT_STRIDE, (size_t)n * JOINT_STRIDE)) return 0;
+    return n;
+}
+
+int odxBatchGetLambda(odxBatch *b, int w, float *lambda, int cap) {
+    if (!b || w < 0 || w >= b->nw) return 0;
+    if (!odxBatchSync(b)) return 0;
+    int m = 0;
+    if (!download(&m, b->D.nrows + w, 1)) return 0;
+    int n = m < cap ? m : cap;
+    if (!download(lambda, b->D.lambda + b->row_start[w], n)) return 0;
+    return n;
+}
+
+int odxBatchGetRows(odxBatch *b, int w, float *rows, int cap) {
+    if (!b || w < 0 || w >= b->nw) return 0;
+    if (!odxBatchSync(b)) return 0;
+    int m = 0;
+    if (!download(&m, b->D.nrows + w, 1)) return 0;
+    int n = m < cap ? m : cap;
+    if (!download(rows, b->D.rows + (size_t)b->row_start[w] * ROW_STRIDE, (size_t)n * ROW_STRIDE)) return 0;
+    return n;
+}
+
+int odxBatchGetOverflow(odxBatch *b, int *counts) {
+    if (!b || !counts) return 0;
+    if (!odxBatchSync(b)) return 0;
+    return download(counts, b->D.overflow, 4) ? 4 : 0;
+}
+
+int odxBatchRowStarts(odxBatch *b, int *row_start) {
+    if (!b || !row_start) return 0;
+    std::memcpy(row_start, b->row_start.data(), (b->nw + 1) * sizeof(int));
+    return b->nw + 1;
+}
+
+int odxBatchSetLambda0(odxBatch *b, const float *lambda0, int n) {
+    if (!b || !lambda0 || n != b->rcap) {
+        odx_message(1, "odxBatchSetLambda0: expected %d values", b ? b->rcap : 0);
+        return 0;
+    }
+    if (!odxBatchSync(b)) return 0;
+    return upload(b->lambda0, lambda0, (size_t)n) ? 1 : 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,78 @@
+// odx_device.h -- shared device/host layouts of the batched step path.
+#pragma once
+#include <stdint.h>
+
+namespace odx {
+
+// Record strides (floats) shared with include/odx.h and the test harness.
+constexpr int BODY_STRIDE = 32;
+constexpr int GEOM_STRIDE = 16;
+constexpr int JOINT_STRIDE = 16;
+constexpr int ROW_STRIDE = 32;    // one assembled constraint row, 128 B
+constexpr int BSCR_STRIDE = 32;   // per-body scratch: R(12) invI_w(12) rhs_tmp(6) invMass
+constexpr int FEAT_DIM = 16;      // warm-start feature vector per contact
+
+// Per-world limits of the one-wave-per-world kernels.
+constexpr int SW_MAXB = 64;       // bodies
+constexpr int SW_MAXG = 96;       // geoms
+constexpr int SW_MAXPAIRS = 1024; // candidate pairs after broadphase
+constexpr int SW_MAXROWS = 2048;  // constraint rows
+constexpr int SW_MAXPH = 2048;    // schedule phases
+
+// Row record layout (floats): J1[0..5] rhs[6] cfm[7] J2[8..13] lo[14] hi[15]
+// iMJ1[16..21] iMJ2[22..27] b1[28] b2[29] findex[30] (int bits) Ad[31].
+enum { R_J1 = 0, R_RHS = 6, R_CFM = 7, R_J2 = 8, R_LO = 14, R_HI = 15, R_IMJ1 = 16, R_IMJ2 = 22,
+       R_B1 = 28, R_B2 = 29, R_FINDEX = 30, R_AD = 31 };
+
+// JOINT record (floats): 0 type (0 contact, 1 ball), 1 b1, 2 b2 (world-local,
+// -1 none), contact: 3-5 pos 6-8 normal 9 depth 10-12 fdir1 13 surface 14 g1 15 g2;
+// ball: 3-5 anchor1 6-8 anchor2 9 erp 10 cfm.
+
+struct Surface {
+    int mode;
+    float mu, mu2, bounce, bounce_vel, soft_erp, soft_cfm, motion1, motion2, motionN, slip1, slip2;
+};
+
+struct StepParams {
+    float h, gx, gy, gz, erp, cfm, sor_w;
+    int max_iters;       // sweep cap (the fixed count when tol <= 0)
+    float tol;           // LCP residual target; <= 0 runs exactly max_iters sweeps
+    int check_every;     // residual check period in sweeps
+    float max_vel, min_depth;
+    int maxc;            // contacts per geom pair
+    int space;           // 0 SAP semantics, 1 hash semantics
+    int skip_connected;  // drop contacts between bodies joined by a ball joint
+    int order;           // 0 reference row order (dependency levels), 1 coloured contact blocks
+    int warm;            // 0 cold start, 1 lambda0 per row, 2 lambda0 per contact (3 values)
+    float lambda_floor;
+    int final_residual;  // 1: compute the residual after the last sweep
+    int collide;         // 1: generate contacts; 0: joints were uploaded
+    int features;        // 1: write warm-start features per contact
+};
+
+struct DevWorlds {
+    int nworlds;
+    const int *body_start, *geom_start, *ball_start, *joint_start, *row_start;  // [W+1]
+    float4 *pos, *quat, *lvel, *avel, *facc, *tacc;   // [NB]
+    const float4 *mass;                                // [NB] mass, invMass, flags, 0
+    const float *invIb;                                // [NB*12] body-frame inverse inertia
+    const float *Ib;                                   // [NB*12] body-frame inertia
+    const float4 *geom0, *geom1, *gpos, *gquat;        // [NG] cls/body/p0/p1, p2/p3, static pose
+    const float *balls;                                // [NBALL*16]
+    const Surface *surf;
+    float *joints;          // [joint cap * 16]
+    int *njoints, *ncontacts, *nrows, *nphases, *niters;  // [W]
+    float *residual;        // [W]
+    float *rows;            // [row cap * 32]
+    float *lambda;          // [row cap]
+    const float *lambda0;   // warm start values or null
+    int *sched;             // [row cap] work items
+    int *phase_start;       // [row cap + W] per-world phase offsets (row_start[w] + w based)
+    float *bscr;            // [NB * 32]
+    float *feat;            // [joint cap * 16] features (contacts only)
+    float *prev;            // [joint cap * 16] previous-step contacts (for matching)
+    int *nprev;             // [W]
+    int *overflow;          // [4] pairs / contacts / rows / phases overflow counters
+};
+
+}  // namespace odx

@@ -1,0 +1,855 @@
+// odx_step.hip -- batched world-step kernels for gfx950 (one wavefront per world).
+//
+// Kernel A (k_prepare_small): body stage (dxQuickStepIsland Stage0_Bodies and
+//   Stage2b, quickstep.cpp:1176-1306, 1644-1690), geom AABBs, broadphase with
+//   dxSAPSpace / dxHashSpace pair semantics, narrowphase (dCollide), contact and
+//   ball rows (getInfo2 + Stage2a/2c + compute_invM_JT + Ad,
+//   quickstep.cpp:1486-1747, 859-897, 2251-2316) and the sweep schedule.
+// Kernel C (k_solve_small): SOR-LCP sweeps (IterationStep quickstep.cpp:2917-3033)
+//   over the schedule with body accumulators fc in LDS and rows streamed from
+//   HBM, an LCP residual check, then Stage4b/6a/6b and dxStepBody
+//   (quickstep.cpp:3082-3102, 3299-3337; util.cpp:583-699).
+//
+// Schedules.  Gauss-Seidel is sequential; order 0 reproduces the reference's
+// order exactly: rows are processed in "dependency levels" where a row's level
+// is one more than the last level that touched either of its bodies, taken in
+// the reference's ReorderPrep order (findex == -1 rows first, :2329-2355).
+// Rows on one level touch disjoint bodies, so executing a level in parallel
+// gives the same iterate as the sequential loop.  Order 1 colours whole joints
+// (a contact's normal row then its friction rows in one lane) greedily so that
+// no two joints of a colour share a body: a different, equally valid
+// Gauss-Seidel order with far fewer phases.
+#include <hip/hip_runtime.h>
+
+#include "odx_collide.h"
+#include "odx_device.h"
+#include "odx_math.h"
+
+namespace odx {
+
+__device__ __forceinline__ int lane_id() { return (int)threadIdx.x & 63; }
+__device__ __forceinline__ int as_int(float f) { return __float_as_int(f); }
+__device__ __forceinline__ float as_float(int i) { return __int_as_float(i); }
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int l = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(v, o, 64);
+        if (l >= o) v += y;
+    }
+    return v;
+}
+__device__ __forceinline__ int wave_bcast_last(int v) { return __shfl(v, 63, 64); }
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// ------------------------------------------------------------ body stage --
+__device__ void body_prep(const DevWorlds &D, const StepParams &P, int b) {
+    float4 p4 = D.pos[b], q4 = D.quat[b], lv = D.lvel[b], av = D.avel[b], fa = D.facc[b], ta = D.tacc[b];
+    float4 ms = D.mass[b];
+    const float mass = ms.x, im = ms.y;
+    const int flags = (int)ms.z;
+    float q[4] = {q4.x, q4.y, q4.z, q4.w};
+    float R[12];
+    RfromQ(R, q);
+    float f[3] = {fa.x, fa.y, fa.z}, tq[3] = {ta.x, ta.y, ta.z};
+    if (!(flags & 1)) {
+        if (P.gx != 0.f) f[0] += mass * P.gx;
+        if (P.gy != 0.f) f[1] += mass * P.gy;
+        if (P.gz != 0.f) f[2] += mass * P.gz;
+    }
+    float ib[12], tmp[12], iw[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) ib[k] = D.invIb[(size_t)b * 12 + k];
+    mul2_333(tmp, ib, R);
+    mul0_333(iw, R, tmp);
+    if ((flags & 2) && im > 0.f) {
+        // implicit gyroscopic torque (Lacoursiere 2006), quickstep.cpp:1239-1303
+        float Ibd[12], Iw[12], L[3], w[3] = {av.x, av.y, av.z};
+#pragma unroll
+        for (int k = 0; k < 12; k++) Ibd[k] = D.Ib[(size_t)b * 12 + k];
+        mul2_333(tmp, Ibd, R);
+        mul0_333(Iw, R, tmp);
+        mul0_331(L, Iw, w);
+        float It[12] = {0.f, L[2], -L[1], 0.f, -L[2], 0.f, L[0], 0.f, L[1], -L[0], 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < 12; k++) It[k] = It[k] * P.h + Iw[k];
+        const float hr = 1.f / P.h;
+        L[0] *= hr; L[1] *= hr; L[2] *= hr;
+        float itInv[12];
+        if (invert3(itInv, It) != 0.f) {
+            mul0_333(It, Iw, itInv);
+            It[0] -= 1.f; It[5] -= 1.f; It[10] -= 1.f;
+            float t0[3];
+            mul0_331(t0, It, L);
+            tq[0] += t0[0]; tq[1] += t0[1]; tq[2] += t0[2];
+        }
+    }
+    D.facc[b] = make_float4(f[0], f[1], f[2], 0.f);
+    D.tacc[b] = make_float4(tq[0], tq[1], tq[2], 0.f);
+    // Stage2b: rhs_tmp = -(v/h + M^-1 f)
+    const float hr = 1.f / P.h;
+    float ra[3];
+    mul0_331(ra, iw, tq);
+    float *o = D.bscr + (size_t)b * BSCR_STRIDE;
+#pragma unroll
+    for (int k = 0; k < 12; k++) { o[k] = R[k]; o[12 + k] = iw[k]; }
+    o[24] = -(f[0] * im + lv.x * hr);
+    o[25] = -(f[1] * im + lv.y * hr);
+    o[26] = -(f[2] * im + lv.z * hr);
+    o[27] = -(av.x * hr) - ra[0];
+    o[28] = -(av.y * hr) - ra[1];
+    o[29] = -(av.z * hr) - ra[2];
+    o[30] = im;
+    o[31] = 0.f;
+    (void)p4;
+}
+
+// ------------------------------------------------------------- geoms ------
+__device__ void load_geom(const DevWorlds &D, int g, int b0, DGeom &G) {
+    float4 a = D.geom0[g], c = D.geom1[g];
+    G.cls = (int)a.x;
+    G.body = (int)a.y;
+    G.p[0] = a.z; G.p[1] = a.w; G.p[2] = c.x; G.p[3] = c.y;
+    if (G.body >= 0) {
+        float4 p = D.pos[b0 + G.body];
+        G.pos[0] = p.x; G.pos[1] = p.y; G.pos[2] = p.z;
+        const float *s = D.bscr + (size_t)(b0 + G.body) * BSCR_STRIDE;
+#pragma unroll
+        for (int k = 0; k < 12; k++) G.R[k] = s[k];
+    } else {
+        float4 p = D.gpos[g], q4 = D.gquat[g];
+        G.pos[0] = p.x; G.pos[1] = p.y; G.pos[2] = p.z;
+        float q[4] = {q4.x, q4.y, q4.z, q4.w};
+        RfromQ(G.R, q);
+    }
+}
+
+// computeAABB: sphere.cpp:59-67, box.cpp:60-76, capsule.cpp:60-74, plane.cpp:81-105
+__device__ void geom_aabb(const DGeom &G, float *ab) {
+    const float *p = G.pos, *R = G.R;
+    switch (G.cls) {
+    case CLS_SPHERE: {
+        float r = G.p[0];
+        ab[0] = p[0] - r; ab[1] = p[0] + r; ab[2] = p[1] - r; ab[3] = p[1] + r; ab[4] = p[2] - r; ab[5] = p[2] + r;
+        break;
+    }
+    case CLS_BOX: {
+        const float *s = G.p;
+        float xr = 0.5f * (fabsf(R[0] * s[0]) + fabsf(R[1] * s[1]) + fabsf(R[2] * s[2]));
+        float yr = 0.5f * (fabsf(R[4] * s[0]) + fabsf(R[5] * s[1]) + fabsf(R[6] * s[2]));
+        float zr = 0.5f * (fabsf(R[8] * s[0]) + fabsf(R[9] * s[1]) + fabsf(R[10] * s[2]));
+        ab[0] = p[0] - xr; ab[1] = p[0] + xr; ab[2] = p[1] - yr; ab[3] = p[1] + yr; ab[4] = p[2] - zr; ab[5] = p[2] + zr;
+        break;
+    }
+    case CLS_CAPSULE: {
+        float r = G.p[0], lz = G.p[1];
+        float xr = fabsf(R[2] * lz) * 0.5f + r, yr = fabsf(R[6] * lz) * 0.5f + r, zr = fabsf(R[10] * lz) * 0.5f + r;
+        ab[0] = p[0] - xr; ab[1] = p[0] + xr; ab[2] = p[1] - yr; ab[3] = p[1] + yr; ab[4] = p[2] - zr; ab[5] = p[2] + zr;
+        break;
+    }
+    case CLS_PLANE: {
+        const float *n = G.p;
+        ab[0] = -kInf; ab[1] = kInf; ab[2] = -kInf; ab[3] = kInf; ab[4] = -kInf; ab[5] = kInf;
+        if (n[1] == 0.f && n[2] == 0.f) {
+            ab[0] = (n[0] > 0.f) ? -kInf : -n[3];
+            ab[1] = (n[0] > 0.f) ? n[3] : kInf;
+        } else if (n[0] == 0.f && n[2] == 0.f) {
+            ab[2] = (n[1] > 0.f) ? -kInf : -n[3];
+            ab[3] = (n[1] > 0.f) ? n[3] : kInf;
+        } else if (n[0] == 0.f && n[1] == 0.f) {
+            ab[4] = (n[2] > 0.f) ? -kInf : -n[3];
+            ab[5] = (n[2] > 0.f) ? n[3] : kInf;
+        }
+        break;
+    }
+    default:
+        ab[0] = -kInf; ab[1] = kInf; ab[2] = -kInf; ab[3] = kInf; ab[4] = -kInf; ab[5] = kInf;
+    }
+}
+
+// ------------------------------------------------------------- rows -------
+__device__ __forceinline__ int contact_rows_m(const Surface &s) {
+    // getInfo1 contact.cpp:48-123 (negative mu counts as zero)
+    if (s.mode & 0x001) return 1 + (s.mu > 0.f ? 1 : 0) + (s.mu2 > 0.f ? 1 : 0);
+    return 1 + (s.mu > 0.f ? 2 : 0);
+}
+
+struct RowBuf {
+    float J1[6], rhs, cfm, J2[6], lo, hi;
+    int findex;
+};
+
+// getInfo2 of a contact joint, contact.cpp:125-345 (rolling friction not on
+// this path).  b1: body row index >= 0, b2 may be -1.
+__device__ int contact_info2(const DevWorlds &D, const StepParams &P, const float *rec, int b0, RowBuf *rows) {
+    const Surface s = D.surf[(int)rec[13]];
+    const int m = contact_rows_m(s);
+    const int b1 = (int)rec[1], b2 = (int)rec[2];
+    const int mode = s.mode;
+    const float fps = 1.f / P.h;
+    float erp = (mode & 0x008) ? s.soft_erp : P.erp;
+    float k = fps * erp;
+    float depth = rec[9] - P.min_depth;
+    if (depth < 0.f) depth = 0.f;
+    float motionN = (mode & 0x080) ? s.motionN : 0.f;
+    const float pushout = k * depth + motionN;
+    const bool bounce = (mode & 0x004) && s.bounce_vel >= 0.f;
+    float c = pushout > P.max_vel ? P.max_vel : pushout;
+    float n[3] = {rec[6], rec[7], rec[8]};
+    float pos[3] = {rec[3], rec[4], rec[5]};
+    float c1[3], c2[3] = {0.f, 0.f, 0.f};
+    float4 p1 = D.pos[b0 + b1];
+    c1[0] = pos[0] - p1.x; c1[1] = pos[1] - p1.y; c1[2] = pos[2] - p1.z;
+    float outgoing = 0.f;
+    if (b2 >= 0) {
+        float4 p2 = D.pos[b0 + b2];
+        c2[0] = pos[0] - p2.x; c2[1] = pos[1] - p2.y; c2[2] = pos[2] - p2.z;
+        rows[0].J2[0] = -n[0]; rows[0].J2[1] = -n[1]; rows[0].J2[2] = -n[2];
+        cross3(rows[0].J2 + 3, n, c2);
+        if (bounce) {
+            float4 av = D.avel[b0 + b2], lv = D.lvel[b0 + b2];
+            float a[3] = {av.x, av.y, av.z}, l[3] = {lv.x, lv.y, lv.z};
+            outgoing = dot3(rows[0].J2 + 3, a) - dot3(n, l);
+        }
+    }
+    rows[0].J1[0] = n[0]; rows[0].J1[1] = n[1]; rows[0].J1[2] = n[2];
+    cross3(rows[0].J1 + 3, c1, n);
+    if (bounce) {
+        float4 av = D.avel[b0 + b1], lv = D.lvel[b0 + b1];
+        float a[3] = {av.x, av.y, av.z}, l[3] = {lv.x, lv.y, lv.z};
+        outgoing += dot3(rows[0].J1 + 3, a) + dot3(n, l);
+        float neg = motionN - outgoing;
+        if (neg > s.bounce_vel) {
+            float newc = s.bounce * neg + motionN;
+            if (newc > c) c = newc;
+        }
+    }
+    rows[0].rhs = c;
+    if (mode & 0x010) rows[0].cfm = s.soft_cfm;
+    rows[0].lo = 0.f;
+    rows[0].hi = kInf;
+    if (m > 1) {
+        float t1[3], t2[3];
+        if (mode & 0x002) {
+            t1[0] = rec[10]; t1[1] = rec[11]; t1[2] = rec[12];
+            cross3(t2, n, t1);
+        } else {
+            planeSpace(n, t1, t2);
+        }
+        int row = 1;
+        if (s.mu > 0.f) {
+            rows[row].J1[0] = t1[0]; rows[row].J1[1] = t1[1]; rows[row].J1[2] = t1[2];
+            cross3(rows[row].J1 + 3, c1, t1);
+            if (b2 >= 0) {
+                rows[row].J2[0] = -t1[0]; rows[row].J2[1] = -t1[1]; rows[row].J2[2] = -t1[2];
+                cross3(rows[row].J2 + 3, t1, c2);
+            }
+            if (mode & 0x020) rows[row].rhs = s.motion1;
+            if (mode & 0x100) rows[row].cfm = s.slip1;
+            rows[row].lo = -s.mu; rows[row].hi = s.mu;
+            if (mode & 0x1000) rows[row].findex = 0;
+            row++;
+        }
+        const float mu2 = (mode & 0x001) ? s.mu2 : s.mu;
+        if (mu2 > 0.f && row < m) {
+            rows[row].J1[0] = t2[0]; rows[row].J1[1] = t2[1]; rows[row].J1[2] = t2[2];
+            cross3(rows[row].J1 + 3, c1, t2);
+            if (b2 >= 0) {
+                rows[row].J2[0] = -t2[0]; rows[row].J2[1] = -t2[1]; rows[row].J2[2] = -t2[2];
+                cross3(rows[row].J2 + 3, t2, c2);
+            }
+            if (mode & 0x040) rows[row].rhs = s.motion2;
+            if (mode & 0x200) rows[row].cfm = s.slip2;
+            rows[row].lo = -mu2; rows[row].hi = mu2;
+            if (mode & 0x2000) rows[row].findex = 0;
+            row++;
+        }
+    }
+    return m;
+}
+
+// getInfo2 of a ball joint, ball.cpp:58-67 -> setBall joint.cpp:111-162
+__device__ int ball_info2(const DevWorlds &D, const StepParams &P, const float *rec, int b0, RowBuf *rows) {
+    const int b1 = (int)rec[1], b2 = (int)rec[2];
+    const float cfm = rec[10], erp = rec[9];
+    for (int r = 0; r < 3; r++) rows[r].cfm = cfm;
+    const float *R1 = D.bscr + (size_t)(b0 + b1) * BSCR_STRIDE;
+    float an1[3] = {rec[3], rec[4], rec[5]}, an2[3] = {rec[6], rec[7], rec[8]}, a1[3], a2[3] = {0.f, 0.f, 0.f};
+    rows[0].J1[0] = 1.f; rows[1].J1[1] = 1.f; rows[2].J1[2] = 1.f;
+    mul0_331(a1, R1, an1);
+    rows[0].J1[4] = a1[2]; rows[0].J1[5] = -a1[1];
+    rows[1].J1[3] = -a1[2]; rows[1].J1[5] = a1[0];
+    rows[2].J1[3] = a1[1]; rows[2].J1[4] = -a1[0];
+    float4 p1 = D.pos[b0 + b1];
+    float x1[3] = {p1.x, p1.y, p1.z};
+    const float k = (1.f / P.h) * erp;
+    if (b2 >= 0) {
+        const float *R2 = D.bscr + (size_t)(b0 + b2) * BSCR_STRIDE;
+        rows[0].J2[0] = -1.f; rows[1].J2[1] = -1.f; rows[2].J2[2] = -1.f;
+        mul0_331(a2, R2, an2);
+        rows[0].J2[4] = -a2[2]; rows[0].J2[5] = a2[1];
+        rows[1].J2[3] = a2[2]; rows[1].J2[5] = -a2[0];
+        rows[2].J2[3] = -a2[1]; rows[2].J2[4] = a2[0];
+        float4 p2 = D.pos[b0 + b2];
+        float x2[3] = {p2.x, p2.y, p2.z};
+        for (int r = 0; r < 3; r++) rows[r].rhs = k * (a2[r] + x2[r] - a1[r] - x1[r]);
+    } else {
+        for (int r = 0; r < 3; r++) rows[r].rhs = k * (an2[r] - a1[r] - x1[r]);
+    }
+    return 3;
+}
+
+// Builds the rows of one joint (Stage2a scaling, Stage2c rhs, iMJ, Ad) and
+// writes them at world-local row index ri.
+__device__ int build_joint_rows(const DevWorlds &D, const StepParams &P, const float *rec, int b0, int r0, int ri) {
+    RowBuf rows[6];
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) { rows[r].J1[k] = 0.f; rows[r].J2[k] = 0.f; }
+        rows[r].rhs = 0.f;
+        rows[r].cfm = P.cfm;
+        rows[r].lo = -kInf;
+        rows[r].hi = kInf;
+        rows[r].findex = -1;
+    }
+    const int type = (int)rec[0];
+    const int m = type == 0 ? contact_info2(D, P, rec, b0, rows) : ball_info2(D, P, rec, b0, rows);
+    const int b1 = (int)rec[1], b2 = (int)rec[2];
+    const float fps = 1.f / P.h;
+    const float *s1 = D.bscr + (size_t)(b0 + b1) * BSCR_STRIDE;
+    const float *s2 = b2 >= 0 ? D.bscr + (size_t)(b0 + b2) * BSCR_STRIDE : nullptr;
+    for (int r = 0; r < m; r++) {
+        RowBuf &w = rows[r];
+        w.rhs *= fps;
+        w.cfm *= fps;
+        float sum = 0.f;
+#pragma unroll
+        for (int k = 0; k < 6; k++) sum += w.J1[k] * s1[24 + k];
+        if (s2) {
+#pragma unroll
+            for (int k = 0; k < 6; k++) sum += w.J2[k] * s2[24 + k];
+        }
+        w.rhs += sum;
+        float imj1[6], imj2[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        const float k1 = s1[30];
+        imj1[0] = k1 * w.J1[0]; imj1[1] = k1 * w.J1[1]; imj1[2] = k1 * w.J1[2];
+        mul0_331(imj1 + 3, s1 + 12, w.J1 + 3);
+        if (s2) {
+            const float k2 = s2[30];
+            imj2[0] = k2 * w.J2[0]; imj2[1] = k2 * w.J2[1]; imj2[2] = k2 * w.J2[2];
+            mul0_331(imj2 + 3, s2 + 12, w.J2 + 3);
+        }
+        float ssum = 0.f;
+#pragma unroll
+        for (int k = 0; k < 6; k++) ssum += imj1[k] * w.J1[k];
+        if (s2) {
+#pragma unroll
+            for (int k = 0; k < 6; k++) ssum += imj2[k] * w.J2[k];
+        }
+        const float Ad = P.sor_w / (ssum + w.cfm);
+        float *o = D.rows + (size_t)(r0 + ri + r) * ROW_STRIDE;
+        float4 *o4 = reinterpret_cast<float4 *>(o);
+        float J1s[6], J2s[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) { J1s[k] = w.J1[k] * Ad; J2s[k] = s2 ? w.J2[k] * Ad : w.J2[k]; }
+        o4[0] = make_float4(J1s[0], J1s[1], J1s[2], J1s[3]);
+        o4[1] = make_float4(J1s[4], J1s[5], w.rhs * Ad, w.cfm * Ad);
+        o4[2] = make_float4(J2s[0], J2s[1], J2s[2], J2s[3]);
+        o4[3] = make_float4(J2s[4], J2s[5], w.lo, w.hi);
+        o4[4] = make_float4(imj1[0], imj1[1], imj1[2], imj1[3]);
+        o4[5] = make_float4(imj1[4], imj1[5], imj2[0], imj2[1]);
+        o4[6] = make_float4(imj2[2], imj2[3], imj2[4], imj2[5]);
+        const int fidx = w.findex >= 0 ? ri + w.findex : -1;
+        o4[7] = make_float4(as_float(b1), as_float(b2), as_float(fidx), Ad);
+    }
+    return m;
+}
+
+// Warm-start feature vector of a contact (its first row at world-local ri).
+__device__ void contact_features(const DevWorlds &D, const float *rec, int b0, float *out) {
+    const int b1 = (int)rec[1], b2 = (int)rec[2];
+    float n[3] = {rec[6], rec[7], rec[8]}, t1[3], t2[3];
+    planeSpace(n, t1, t2);
+    float4 p1 = D.pos[b0 + b1], v1 = D.lvel[b0 + b1], w1 = D.avel[b0 + b1];
+    float c1[3] = {rec[3] - p1.x, rec[4] - p1.y, rec[5] - p1.z};
+    float w1a[3] = {w1.x, w1.y, w1.z}, u[3];
+    cross3(u, w1a, c1);
+    float vr[3] = {v1.x + u[0], v1.y + u[1], v1.z + u[2]};
+    float im2 = 0.f;
+    if (b2 >= 0) {
+        float4 p2 = D.pos[b0 + b2], v2 = D.lvel[b0 + b2], w2 = D.avel[b0 + b2];
+        float c2[3] = {rec[3] - p2.x, rec[4] - p2.y, rec[5] - p2.z};
+        float w2a[3] = {w2.x, w2.y, w2.z};
+        cross3(u, w2a, c2);
+        vr[0] -= v2.x + u[0]; vr[1] -= v2.y + u[1]; vr[2] -= v2.z + u[2];
+        im2 = D.bscr[(size_t)(b0 + b2) * BSCR_STRIDE + 30];
+    }
+    const float im1 = D.bscr[(size_t)(b0 + b1) * BSCR_STRIDE + 30];
+    out[0] = n[0]; out[1] = n[1]; out[2] = n[2];
+    out[3] = rec[9];
+    out[4] = dot3(vr, n); out[5] = dot3(vr, t1); out[6] = dot3(vr, t2);
+    out[7] = 0.f; out[8] = 0.f; out[9] = 0.f;   // previous impulses, filled by matching
+    out[10] = im1; out[11] = im2;
+    out[12] = 0.f;                               // matched flag
+    out[13] = rec[5] - p1.z;                     // contact height relative to body 1
+    out[14] = (float)(b2 >= 0);
+    out[15] = 1.f;
+}
+
+// ============================================================== kernel A ===
+__global__ __launch_bounds__(64) void k_prepare_small(DevWorlds D, StepParams P) {
+    const int w = blockIdx.x;
+    const int t = threadIdx.x;
+    __shared__ float s_ab[SW_MAXG][6];
+    __shared__ signed char s_gb[SW_MAXG];
+    __shared__ uint32_t s_pairs[SW_MAXPAIRS];
+    __shared__ unsigned char s_rb1[SW_MAXROWS], s_rb2[SW_MAXROWS], s_rfl[SW_MAXROWS];
+    __shared__ unsigned short s_lvl[SW_MAXROWS];
+    __shared__ unsigned short s_joff[SW_MAXROWS];
+    __shared__ unsigned char s_jm[SW_MAXROWS];
+    __shared__ int s_cnt[SW_MAXPH + 1];
+    __shared__ unsigned long long s_mask[SW_MAXB];
+    __shared__ int s_L;
+
+    const int b0 = D.body_start[w], nb = D.body_start[w + 1] - b0;
+    const int g0 = D.geom_start[w], ng = D.geom_start[w + 1] - g0;
+    const int l0 = D.ball_start[w], nball = D.ball_start[w + 1] - l0;
+    const int j0 = D.joint_start[w], jcap = D.joint_start[w + 1] - j0;
+    const int r0 = D.row_start[w], rcap = min(D.row_start[w + 1] - r0, SW_MAXROWS);
+
+    // S1: bodies
+    for (int i = t; i < nb; i += 64) body_prep(D, P, b0 + i);
+    __syncthreads();
+
+    int njoint, ncont;
+    if (P.collide) {
+        for (int k = t; k < nball * JOINT_STRIDE; k += 64) D.joints[(size_t)j0 * JOINT_STRIDE + k] = D.balls[(size_t)l0 * JOINT_STRIDE + k];
+        // S2: AABBs
+        for (int i = t; i < ng; i += 64) {
+            DGeom G;
+            load_geom(D, g0 + i, b0, G);
+            geom_aabb(G, s_ab[i]);
+            s_gb[i] = (signed char)G.body;
+        }
+        __syncthreads();
+        // S3: all-pairs broadphase in geom-index order (lower index first)
+        const int T = ng * (ng - 1) / 2;
+        int npairs = 0;
+        for (int base = 0; base < T; base += 64) {
+            const int k = base + t;
+            bool ok = false;
+            int gi = 0, gj = 0;
+            if (k < T) {
+                // row gi of the strict upper triangle holds ng-1-gi entries
+                float fn = (float)(2 * ng - 1);
+                gi = (int)floorf((fn - sqrtf(fn * fn - 8.f * (float)k)) * 0.5f);
+                if (gi < 0) gi = 0;
+                while (gi > 0 && gi * (2 * ng - gi - 1) / 2 > k) gi--;
+                while ((gi + 1) * (2 * ng - gi - 2) / 2 <= k) gi++;
+                gj = k - gi * (2 * ng - gi - 1) / 2 + gi + 1;
+                const int bi = s_gb[gi], bj = s_gb[gj];
+                if ((bi >= 0 || bj >= 0) && !(bi == bj)) {
+                    const float *x = s_ab[gi], *y = s_ab[gj];
+                    if (P.space == 0 && (x[1] == kInf || y[1] == kInf)) ok = true;
+                    else ok = !(x[0] > y[1] || x[1] < y[0] || x[2] > y[3] || x[3] < y[2] || x[4] > y[5] || x[5] < y[4]);
+                    if (ok && P.skip_connected && bi >= 0 && bj >= 0) {
+                        for (int q = 0; q < nball; q++) {
+                            const float *jr = D.balls + (size_t)(l0 + q) * JOINT_STRIDE;
+                            int x1 = (int)jr[1], x2 = (int)jr[2];
+                            if ((x1 == bi && x2 == bj) || (x1 == bj && x2 == bi)) ok = false;
+                        }
+                    }
+                }
+            }
+            const unsigned long long bal = __ballot(ok);
+            const int off = __popcll(bal & ((1ull << t) - 1ull));
+            if (ok && npairs + off < SW_MAXPAIRS) s_pairs[npairs + off] = ((uint32_t)gi << 16) | (uint32_t)gj;
+            npairs += __popcll(bal);
+        }
+        if (npairs > SW_MAXPAIRS) {
+            if (t == 0) atomicAdd(&D.overflow[0], npairs - SW_MAXPAIRS);
+            npairs = SW_MAXPAIRS;
+        }
+        __syncthreads();
+        // S4: narrowphase, contacts appended after the ball joints
+        const int ccap = jcap - nball;
+        ncont = 0;
+        for (int base = 0; base < npairs; base += 64) {
+            const int k = base + t;
+            int n = 0, gi = 0, gj = 0;
+            DContact c[ODX_MAXC];
+            if (k < npairs) {
+                gi = (int)(s_pairs[k] >> 16);
+                gj = (int)(s_pairs[k] & 0xffff);
+                DGeom A, B;
+                load_geom(D, g0 + gi, b0, A);
+                load_geom(D, g0 + gj, b0, B);
+                n = dcollide(A, B, min(P.maxc, ODX_MAXC), c);
+            }
+            const int incl = wave_incl_scan(n);
+            const int off = incl - n;
+            for (int q = 0; q < n; q++) {
+                const int slot = ncont + off + q;
+                if (slot < ccap) {
+                    int bb1 = s_gb[gi], bb2 = s_gb[gj];
+                    float sg = 1.f;
+                    if (bb1 < 0) { bb1 = bb2; bb2 = -1; sg = -1.f; }  // dJointAttach: reversed joint
+                    float *o = D.joints + (size_t)(j0 + nball + slot) * JOINT_STRIDE;
+                    float4 *o4 = reinterpret_cast<float4 *>(o);
+                    o4[0] = make_float4(0.f, (float)bb1, (float)bb2, c[q].pos[0]);
+                    o4[1] = make_float4(c[q].pos[1], c[q].pos[2], sg * c[q].normal[0], sg * c[q].normal[1]);
+                    o4[2] = make_float4(sg * c[q].normal[2], c[q].depth, 0.f, 0.f);
+                    o4[3] = make_float4(0.f, 0.f, (float)gi, (float)gj);
+                }
+            }
+            ncont += wave_bcast_last(incl);
+        }
+        if (ncont > ccap) {
+            if (t == 0) atomicAdd(&D.overflow[1], ncont - ccap);
+            ncont = ccap;
+        }
+        njoint = nball + ncont;
+        if (t == 0) { D.njoints[w] = njoint; D.ncontacts[w] = ncont; }
+    } else {
+        njoint = D.njoints[w];
+        ncont = D.ncontacts[w];
+    }
+    __syncthreads();
+
+    // S5: rows, in joint order
+    int mrow = 0;
+    for (int base = 0; base < njoint; base += 64) {
+        const int k = base + t;
+        int mj = 0;
+        const float *rec = D.joints + (size_t)(j0 + k) * JOINT_STRIDE;
+        if (k < njoint) mj = ((int)rec[0] == 0) ? contact_rows_m(D.surf[(int)rec[13]]) : 3;
+        const int incl = wave_incl_scan(mj);
+        const int ri = mrow + incl - mj;
+        if (k < njoint) {
+            if (ri + mj <= rcap && k < SW_MAXROWS) {
+                build_joint_rows(D, P, rec, b0, r0, ri);
+                const int bb1 = (int)rec[1], bb2 = (int)rec[2];
+                s_joff[k] = (unsigned short)ri;
+                s_jm[k] = (unsigned char)mj;
+                for (int r = 0; r < mj; r++) {
+                    s_rb1[ri + r] = (unsigned char)bb1;
+                    s_rb2[ri + r] = (unsigned char)(bb2 < 0 ? 255 : bb2);
+                }
+                // findex rows of a contact are its friction rows (Approx1)
+                if ((int)rec[0] == 0) {
+                    const int mode = D.surf[(int)rec[13]].mode;
+                    s_rfl[ri] = 0;
+                    int rr = 1;
+                    const Surface sf = D.surf[(int)rec[13]];
+                    if (mj > 1) {
+                        if (sf.mu > 0.f) { s_rfl[ri + rr] = (mode & 0x1000) ? 1 : 0; rr++; }
+                        if (rr < mj) s_rfl[ri + rr] = (mode & 0x2000) ? 1 : 0;
+                    }
+                } else {
+                    s_rfl[ri] = 0; s_rfl[ri + 1] = 0; s_rfl[ri + 2] = 0;
+                }
+                if (P.features && (int)rec[0] == 0)
+                    contact_features(D, rec, b0, D.feat + (size_t)(j0 + k) * FEAT_DIM);
+            } else if (k < SW_MAXROWS) {
+                s_jm[k] = 0;
+            }
+        }
+        mrow += wave_bcast_last(incl);
+    }
+    int nj_used = njoint;
+    if (mrow > rcap) {
+        if (t == 0) atomicAdd(&D.overflow[2], mrow - rcap);
+        // keep only joints whose rows fit
+        mrow = 0;
+        nj_used = 0;
+        for (int k = 0; k < njoint && k < SW_MAXROWS; k++) {
+            if (s_jm[k] == 0) break;
+            mrow += s_jm[k];
+            nj_used++;
+        }
+    }
+    __syncthreads();
+
+    // S6: schedule
+    int *ps = D.phase_start + r0 + w;
+    int *sched = D.sched + r0;
+    if (P.order == 0) {
+        // dependency levels over the ReorderPrep order (lane 0, LDS-resident)
+        if (t == 0) {
+            unsigned short *last = reinterpret_cast<unsigned short *>(s_mask);
+            for (int b = 0; b < nb; b++) last[b] = 0;
+            int L = 0;
+            for (int pass = 0; pass < 2; pass++) {
+                for (int r = 0; r < mrow; r++) {
+                    if (s_rfl[r] != pass) continue;
+                    const int a = s_rb1[r], bb = s_rb2[r];
+                    int lv = last[a];
+                    if (bb != 255 && last[bb] > lv) lv = last[bb];
+                    lv += 1;
+                    last[a] = (unsigned short)lv;
+                    if (bb != 255) last[bb] = (unsigned short)lv;
+                    s_lvl[r] = (unsigned short)(lv - 1);
+                    if (lv > L) L = lv;
+                }
+            }
+            s_L = L;
+        }
+        __syncthreads();
+        const int L = s_L;
+        for (int p = t; p <= L; p += 64) s_cnt[p] = 0;
+        __syncthreads();
+        for (int r = t; r < mrow; r += 64) atomicAdd(&s_cnt[s_lvl[r] + 1], 1);
+        __syncthreads();
+        if (t == 0) {
+            for (int p = 1; p <= L; p++) s_cnt[p] += s_cnt[p - 1];
+            for (int p = 0; p <= L; p++) ps[p] = s_cnt[p];
+        }
+        __syncthreads();
+        for (int r = t; r < mrow; r += 64) {
+            const int pos = atomicAdd(&s_cnt[s_lvl[r]], 1);
+            sched[pos] = (r << 3) | 1;
+        }
+        if (t == 0) D.nphases[w] = L;
+    } else {
+        // greedy joint colouring with per-body colour masks (lane 0)
+        if (t == 0) {
+            for (int b = 0; b < nb; b++) s_mask[b] = 0ull;
+            int L = 0, extra = 0;
+            for (int k = 0; k < nj_used; k++) {
+                const int r = s_joff[k];
+                const int a = s_rb1[r], bb = s_rb2[r];
+                unsigned long long used = s_mask[a] | (bb != 255 ? s_mask[bb] : 0ull);
+                int c;
+                if (~used) {
+                    c = __ffsll((long long)~used) - 1;
+                    const unsigned long long bit = 1ull << c;
+                    s_mask[a] |= bit;
+                    if (bb != 255) s_mask[bb] |= bit;
+                } else {
+                    c = 64 + extra++;  // no free colour: a phase of its own
+                }
+                s_lvl[k] = (unsigned short)c;
+                if (c + 1 > L) L = c + 1;
+            }
+            s_L = L;
+        }
+        __syncthreads();
+        const int L = s_L;
+        for (int p = t; p <= L; p += 64) s_cnt[p] = 0;
+        __syncthreads();
+        for (int k = t; k < nj_used; k += 64) atomicAdd(&s_cnt[s_lvl[k] + 1], 1);
+        __syncthreads();
+        if (t == 0) {
+            for (int p = 1; p <= L; p++) s_cnt[p] += s_cnt[p - 1];
+            for (int p = 0; p <= L; p++) ps[p] = s_cnt[p];
+        }
+        __syncthreads();
+        for (int k = t; k < nj_used; k += 64) {
+            const int pos = atomicAdd(&s_cnt[s_lvl[k]], 1);
+            sched[pos] = ((int)s_joff[k] << 3) | (int)s_jm[k];
+        }
+        if (t == 0) D.nphases[w] = L;
+    }
+    if (t == 0) D.nrows[w] = mrow;
+}
+
+// ============================================================== kernel C ===
+__device__ __forceinline__ void process_item(const DevWorlds &D, int r0, int item, float *s_fc) {
+    const int rs = item >> 3, nr = item & 7;
+    const float4 *row4 = reinterpret_cast<const float4 *>(D.rows + (size_t)(r0 + rs) * ROW_STRIDE);
+    const float4 ids = row4[7];
+    const int b1 = as_int(ids.x), b2 = as_int(ids.y);
+    float f1[6], f2[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 6; k++) f1[k] = s_fc[b1 * 6 + k];
+    if (b2 >= 0) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) f2[k] = s_fc[b2 * 6 + k];
+    }
+    float *lam = D.lambda + r0;
+    float lam_first = 0.f;
+    for (int q = 0; q < nr; q++) {
+        const float4 *rr = row4 + q * 8;
+        const float4 a0 = rr[0], a1 = rr[1], a2 = rr[2], a3 = rr[3], a4 = rr[4], a5 = rr[5], a6 = rr[6], a7 = rr[7];
+        const float old = lam[rs + q];
+        float delta = a1.z - old * a1.w;
+        delta -= f1[0] * a0.x + f1[1] * a0.y + f1[2] * a0.z + f1[3] * a0.w + f1[4] * a1.x + f1[5] * a1.y;
+        if (b2 >= 0) delta -= f2[0] * a2.x + f2[1] * a2.y + f2[2] * a2.z + f2[3] * a2.w + f2[4] * a3.x + f2[5] * a3.y;
+        float lo = a3.z, hi = a3.w;
+        const int fidx = as_int(a7.z);
+        if (fidx >= 0) {
+            const float lf = (fidx == rs) ? lam_first : lam[fidx];
+            hi = fabsf(hi * lf);
+            lo = -hi;
+        }
+        float nl = old + delta;
+        if (nl < lo) { delta = lo - old; nl = lo; }
+        else if (nl > hi) { delta = hi - old; nl = hi; }
+        lam[rs + q] = nl;
+        if (q == 0) lam_first = nl;
+        f1[0] += delta * a4.x; f1[1] += delta * a4.y; f1[2] += delta * a4.z;
+        f1[3] += delta * a4.w; f1[4] += delta * a5.x; f1[5] += delta * a5.y;
+        if (b2 >= 0) {
+            f2[0] += delta * a5.z; f2[1] += delta * a5.w; f2[2] += delta * a6.x;
+            f2[3] += delta * a6.y; f2[4] += delta * a6.z; f2[5] += delta * a6.w;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) s_fc[b1 * 6 + k] = f1[k];
+    if (b2 >= 0) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) s_fc[b2 * 6 + k] = f2[k];
+    }
+}
+
+// Residual of the current iterate: the un-relaxed projected correction of
+// each row relative to max |lambda| (SOR's Ad already carries the factor w).
+__device__ float world_residual(const DevWorlds &D, const StepParams &P, int r0, int m, const float *s_fc) {
+    float mx = 0.f, lmax = 0.f;
+    const float *lam = D.lambda + r0;
+    const float iw = 1.f / P.sor_w;
+    for (int r = lane_id(); r < m; r += 64) {
+        const float4 *rr = reinterpret_cast<const float4 *>(D.rows + (size_t)(r0 + r) * ROW_STRIDE);
+        const float4 a0 = rr[0], a1 = rr[1], a2 = rr[2], a3 = rr[3], a7 = rr[7];
+        const int b1 = as_int(a7.x), b2 = as_int(a7.y), fidx = as_int(a7.z);
+        const float *f1 = s_fc + b1 * 6;
+        const float l = lam[r];
+        float d = a1.z - l * a1.w;
+        d -= f1[0] * a0.x + f1[1] * a0.y + f1[2] * a0.z + f1[3] * a0.w + f1[4] * a1.x + f1[5] * a1.y;
+        if (b2 >= 0) {
+            const float *f2 = s_fc + b2 * 6;
+            d -= f2[0] * a2.x + f2[1] * a2.y + f2[2] * a2.z + f2[3] * a2.w + f2[4] * a3.x + f2[5] * a3.y;
+        }
+        d *= iw;
+        float lo = a3.z, hi = a3.w;
+        if (fidx >= 0) { hi = fabsf(hi * lam[fidx]); lo = -hi; }
+        float nl = fminf(fmaxf(l + d, lo), hi);
+        mx = fmaxf(mx, fabsf(nl - l));
+        lmax = fmaxf(lmax, fabsf(l));
+    }
+    mx = wave_max(mx);
+    lmax = wave_max(lmax);
+    return mx / fmaxf(lmax, P.lambda_floor);
+}
+
+__global__ __launch_bounds__(64) void k_solve_small(DevWorlds D, StepParams P) {
+    const int w = blockIdx.x;
+    const int t = threadIdx.x;
+    __shared__ float s_fc[SW_MAXB * 6];
+    const int b0 = D.body_start[w], nb = D.body_start[w + 1] - b0;
+    const int r0 = D.row_start[w];
+    const int m = D.nrows[w];
+    const int L = D.nphases[w];
+    const int *ps = D.phase_start + r0 + w;
+    const int *sched = D.sched + r0;
+    float *lam = D.lambda + r0;
+
+    for (int k = t; k < nb * 6; k += 64) s_fc[k] = 0.f;
+    if (P.warm == 1 && D.lambda0) {
+        // lambda0 clamped to the row bounds; friction bounds use the clamped normal
+        for (int pass = 0; pass < 2; pass++) {
+            for (int r = t; r < m; r += 64) {
+                const float4 *rr = reinterpret_cast<const float4 *>(D.rows + (size_t)(r0 + r) * ROW_STRIDE);
+                const float4 a3 = rr[3], a7 = rr[7];
+                const int fidx = as_int(a7.z);
+                if ((fidx >= 0) != (pass == 1)) continue;
+                float lo = a3.z, hi = a3.w;
+                if (fidx >= 0) { hi = fabsf(hi * lam[fidx]); lo = -hi; }
+                lam[r] = fminf(fmaxf(D.lambda0[r0 + r], lo), hi);
+            }
+            __syncthreads();
+        }
+        for (int r = t; r < m; r += 64) {
+            const float4 *rr = reinterpret_cast<const float4 *>(D.rows + (size_t)(r0 + r) * ROW_STRIDE);
+            const float4 a4 = rr[4], a5 = rr[5], a6 = rr[6], a7 = rr[7];
+            const int b1 = as_int(a7.x), b2 = as_int(a7.y);
+            const float l = lam[r];
+            if (l != 0.f) {
+                atomicAdd(&s_fc[b1 * 6 + 0], l * a4.x); atomicAdd(&s_fc[b1 * 6 + 1], l * a4.y);
+                atomicAdd(&s_fc[b1 * 6 + 2], l * a4.z); atomicAdd(&s_fc[b1 * 6 + 3], l * a4.w);
+                atomicAdd(&s_fc[b1 * 6 + 4], l * a5.x); atomicAdd(&s_fc[b1 * 6 + 5], l * a5.y);
+                if (b2 >= 0) {
+                    atomicAdd(&s_fc[b2 * 6 + 0], l * a5.z); atomicAdd(&s_fc[b2 * 6 + 1], l * a5.w);
+                    atomicAdd(&s_fc[b2 * 6 + 2], l * a6.x); atomicAdd(&s_fc[b2 * 6 + 3], l * a6.y);
+                    atomicAdd(&s_fc[b2 * 6 + 4], l * a6.z); atomicAdd(&s_fc[b2 * 6 + 5], l * a6.w);
+                }
+            }
+        }
+    } else {
+        for (int r = t; r < m; r += 64) lam[r] = 0.f;
+    }
+    __syncthreads();
+
+    int it = 0;
+    float res = -1.f;
+    if (m > 0) {
+        while (it < P.max_iters) {
+            for (int p = 0; p < L; p++) {
+                const int s = ps[p], e = ps[p + 1];
+                for (int k = s + t; k < e; k += 64) process_item(D, r0, sched[k], s_fc);
+                __syncthreads();
+            }
+            it++;
+            if (P.tol > 0.f && (it % P.check_every == 0 || it == P.max_iters)) {
+                res = world_residual(D, P, r0, m, s_fc);
+                if (res <= P.tol) break;
+            }
+        }
+        if (P.final_residual && res < 0.f) res = world_residual(D, P, r0, m, s_fc);
+    } else {
+        res = 0.f;
+    }
+    if (t == 0) { D.niters[w] = it; D.residual[w] = res; }
+
+    // Stage4b, Stage6a, Stage6b + dxStepBody (infinitesimal rotation)
+    const float h = P.h;
+    for (int i = t; i < nb; i += 64) {
+        const int b = b0 + i;
+        const float *s = D.bscr + (size_t)b * BSCR_STRIDE;
+        float4 p4 = D.pos[b], q4 = D.quat[b], lv = D.lvel[b], av = D.avel[b], fa = D.facc[b], ta = D.tacc[b];
+        const float *fc = s_fc + i * 6;
+        if (m > 0) {
+            lv.x += h * fc[0]; lv.y += h * fc[1]; lv.z += h * fc[2];
+            av.x += h * fc[3]; av.y += h * fc[4]; av.z += h * fc[5];
+        }
+        const float ims = h * s[30];
+        lv.x += ims * fa.x; lv.y += ims * fa.y; lv.z += ims * fa.z;
+        float tq[3] = {ta.x * h, ta.y * h, ta.z * h}, dw[3];
+        mul0_331(dw, s + 12, tq);
+        av.x += dw[0]; av.y += dw[1]; av.z += dw[2];
+        p4.x += h * lv.x; p4.y += h * lv.y; p4.z += h * lv.z;
+        float dq0 = 0.5f * (-av.x * q4.y - av.y * q4.z - av.z * q4.w);
+        float dq1 = 0.5f * (av.x * q4.x + av.y * q4.w - av.z * q4.z);
+        float dq2 = 0.5f * (-av.x * q4.w + av.y * q4.x + av.z * q4.y);
+        float dq3 = 0.5f * (av.x * q4.z - av.y * q4.y + av.z * q4.x);
+        q4.x += h * dq0; q4.y += h * dq1; q4.z += h * dq2; q4.w += h * dq3;
+        float l2 = q4.x * q4.x + q4.y * q4.y + q4.z * q4.z + q4.w * q4.w;
+        if (l2 > 0.f) {
+            const float l = 1.f / sqrtf(l2);
+            q4.x *= l; q4.y *= l; q4.z *= l; q4.w *= l;
+        } else {
+            q4 = make_float4(1.f, 0.f, 0.f, 0.f);
+        }
+        D.pos[b] = p4; D.quat[b] = q4; D.lvel[b] = lv; D.avel[b] = av;
+        D.facc[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+        D.tacc[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+}  // namespace odx
+
+// Host-side launchers (the kernels' only callers).
+namespace odx {
+void launch_prepare_small(const DevWorlds &D, const StepParams &P, hipStream_t s) {
+    hipLaunchKernelGGL(k_prepare_small, dim3(D.nworlds), dim3(64), 0, s, D, P);
+}
+void launch_solve_small(const DevWorlds &D, const StepParams &P, hipStream_t s) {
+    hipLaunchKernelGGL(k_solve_small, dim3(D.nworlds), dim3(64), 0, s, D, P);
+}
+}  // namespace odx

@@ -1,0 +1,205 @@
+"""ctypes binding of libodx.so (the HIP step path) and the batched-worlds API.
+
+The library is built in-tree by `python -m ode_amd.build`.  Loading fails
+loudly when it is missing; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libodx.so")
+
+BODY = 32
+GEOM = 16
+JOINT = 16
+ROW = 32
+
+_lib = None
+
+
+class OdxError(RuntimeError):
+    pass
+
+
+_MSG_CB_T = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_char_p)
+_messages = []
+
+
+@_MSG_CB_T
+def _on_message(err, msg):
+    _messages.append((err, msg.decode(errors="replace")))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OdxError("libodx.so is not built (run `python -m ode_amd.build`): " + LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, fp, dp, i = ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int
+        L.odxVersion.restype = ctypes.c_char_p
+        L.odxBatchCreate.restype = ctypes.c_void_p
+        L.odxBatchCreate.argtypes = [i, ip, fp, ip, fp, ip, fp, dp, i]
+        L.odxBatchDestroy.argtypes = [vp]
+        L.odxBatchSetSolver.argtypes = [vp, i, i, ctypes.c_float, i, i]
+        L.odxBatchStep.argtypes = [vp, i]
+        L.odxBatchSync.argtypes = [vp]
+        L.odxBatchTime.argtypes = [vp, i, fp]
+        L.odxBatchKernelTimes.argtypes = [vp, i, fp]
+        L.odxBatchGetBodies.argtypes = [vp, fp]
+        L.odxBatchSetBodies.argtypes = [vp, fp]
+        L.odxBatchGetStats.argtypes = [vp, ip, ip, ip, fp]
+        L.odxBatchGetJoints.argtypes = [vp, i, fp, i]
+        L.odxBatchGetLambda.argtypes = [vp, i, fp, i]
+        L.odxBatchGetRows.argtypes = [vp, i, fp, i]
+        L.odxBatchGetOverflow.argtypes = [vp, ip]
+        L.odxBatchRowStarts.argtypes = [vp, ip]
+        L.odxBatchSetLambda0.argtypes = [vp, fp, i]
+        L.odxSetMessageHandler(_on_message)
+        _lib = L
+    return _lib
+
+
+def last_messages(clear=True):
+    out = list(_messages)
+    if clear:
+        _messages.clear()
+    return out
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _check(ok, what):
+    if not ok:
+        raise OdxError("%s failed: %s" % (what, "; ".join(m for _, m in last_messages()) or "unknown error"))
+
+
+def params_vector(h=0.01, gravity=(0, 0, -9.81), erp=0.2, cfm=1e-5, w=1.3, iters=20, max_vel=np.inf, min_depth=0.0,
+                  mode=0x7000, mu=0.5, mu2=0.0, bounce=0.0, bounce_vel=0.0, soft_erp=0.0, soft_cfm=0.0, maxc=4,
+                  space=0, skip_connected=0, lambda_floor=1e-3):
+    """PARAMS vector of include/odx.h (dWorldCreate defaults for dSINGLE)."""
+    p = np.zeros(32, np.float64)
+    p[0] = h
+    p[1:4] = gravity
+    p[4], p[5], p[6], p[7] = erp, cfm, w, iters
+    p[8], p[9] = max_vel, min_depth
+    p[10], p[11], p[12], p[13], p[14], p[15], p[16] = mode, mu, mu2, bounce, bounce_vel, soft_erp, soft_cfm
+    p[17], p[18] = maxc, space
+    p[24], p[25] = skip_connected, lambda_floor
+    return p
+
+
+def pack_worlds(worlds):
+    """List of scenes.World -> concatenated record arrays + prefix offsets."""
+    bs, gs, ls = [0], [0], [0]
+    for w in worlds:
+        bs.append(bs[-1] + len(w.bodies))
+        gs.append(gs[-1] + len(w.geoms))
+        ls.append(ls[-1] + len(w.balls))
+    bodies = np.ascontiguousarray(np.vstack([w.bodies for w in worlds]), np.float32)
+    geoms = np.ascontiguousarray(np.vstack([w.geoms for w in worlds]), np.float32)
+    balls = [w.balls for w in worlds if len(w.balls)]
+    balls = np.ascontiguousarray(np.vstack(balls), np.float32) if balls else np.zeros((0, JOINT), np.float32)
+    return (np.asarray(bs, np.int32), bodies, np.asarray(gs, np.int32), geoms, np.asarray(ls, np.int32), balls)
+
+
+class Batch:
+    """Many independent worlds stepped together on the GPU (odxBatch*)."""
+
+    ORDER_REFERENCE = 0
+    ORDER_COLOURED = 1
+
+    def __init__(self, worlds, params, contacts_per_body=8):
+        L = lib()
+        self.worlds = worlds
+        bs, bodies, gs, geoms, ls, balls = pack_worlds(worlds)
+        self.body_start, self.geom_start, self.ball_start = bs, gs, ls
+        self.nworlds = len(worlds)
+        self.nbodies = int(bs[-1])
+        self._keep = (bodies, geoms, balls)
+        self.params = np.ascontiguousarray(params, np.float64)
+        h = L.odxBatchCreate(self.nworlds, _p(bs), _p(bodies), _p(gs), _p(geoms), _p(ls), _p(balls),
+                             _p(self.params), contacts_per_body)
+        _check(h, "odxBatchCreate")
+        self.h = h
+        rs = np.zeros(self.nworlds + 1, np.int32)
+        L.odxBatchRowStarts(self.h, _p(rs))
+        self.row_start = rs
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().odxBatchDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_solver(self, order=0, max_iters=20, tol=0.0, check_every=1, warm=0):
+        _check(lib().odxBatchSetSolver(self.h, order, max_iters, float(tol), check_every, warm), "odxBatchSetSolver")
+
+    def step(self, n=1):
+        _check(lib().odxBatchStep(self.h, n), "odxBatchStep")
+
+    def sync(self):
+        _check(lib().odxBatchSync(self.h), "odxBatchSync")
+
+    def time(self, n):
+        out = np.zeros(1, np.float32)
+        _check(lib().odxBatchTime(self.h, n, _p(out)), "odxBatchTime")
+        return float(out[0])
+
+    def kernel_times(self, n):
+        out = np.zeros(2, np.float32)
+        _check(lib().odxBatchKernelTimes(self.h, n, _p(out)), "odxBatchKernelTimes")
+        return float(out[0]), float(out[1])
+
+    def bodies(self):
+        out = np.zeros((self.nbodies, BODY), np.float32)
+        _check(lib().odxBatchGetBodies(self.h, _p(out)), "odxBatchGetBodies")
+        return out
+
+    def set_bodies(self, bodies):
+        b = np.ascontiguousarray(bodies, np.float32)
+        _check(lib().odxBatchSetBodies(self.h, _p(b)), "odxBatchSetBodies")
+
+    def world_bodies(self, w, bodies=None):
+        b = self.bodies() if bodies is None else bodies
+        return b[self.body_start[w]:self.body_start[w + 1]]
+
+    def stats(self):
+        n = self.nworlds
+        nc, nr, ni = (np.zeros(n, np.int32) for _ in range(3))
+        res = np.zeros(n, np.float32)
+        _check(lib().odxBatchGetStats(self.h, _p(nc), _p(nr), _p(ni), _p(res)), "odxBatchGetStats")
+        return {"contacts": nc, "rows": nr, "iters": ni, "residual": res}
+
+    def joints(self, w, cap=8192):
+        out = np.zeros((cap, JOINT), np.float32)
+        n = lib().odxBatchGetJoints(self.h, w, _p(out), cap)
+        return out[:n]
+
+    def lam(self, w, cap=8192):
+        out = np.zeros(cap, np.float32)
+        n = lib().odxBatchGetLambda(self.h, w, _p(out), cap)
+        return out[:n]
+
+    def rows(self, w, cap=8192):
+        out = np.zeros((cap, ROW), np.float32)
+        n = lib().odxBatchGetRows(self.h, w, _p(out), cap)
+        return out[:n]
+
+    def overflow(self):
+        out = np.zeros(4, np.int32)
+        _check(lib().odxBatchGetOverflow(self.h, _p(out)), "odxBatchGetOverflow")
+        return out
+
+    def set_lambda0(self, lam0):
+        a = np.ascontiguousarray(lam0, np.float32)
+        _check(lib().odxBatchSetLambda0(self.h, _p(a), len(a)), "odxBatchSetLambda0")

@@ -1,0 +1,250 @@
+"""Parity of the HIP batched step (libodx) with the CPU oracle.
+
+Deterministic mode (reference row order) is compared step by step against the
+oracle's sequential QuickStep: contact counts and geom pairs exactly,
+contact geometry, impulses and body state within the stated relative
+tolerance.  The coloured mode is a different Gauss-Seidel order and is judged
+by its LCP residual and the LCP's invariants.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from ode_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+# Tolerances (relative to the largest magnitude of the compared quantity).
+TOL_STATE = 1e-4     # positions, quaternions, velocities after one step
+TOL_LAMBDA = 1e-4    # constraint impulses after one step (float32 vs the float64 oracle)
+TOL_CONTACT = 1e-5   # contact positions / normals / depths
+
+
+def _odx():
+    from ode_amd import odx
+    return odx
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_joints(contacts, geoms):
+    """Oracle contacts -> JOINT layout with the dJointAttach reversal applied."""
+    j = O.contacts_to_joints(contacts, geoms)
+    for r in j:
+        if r[1] < 0:
+            r[1], r[2] = r[2], -1
+            r[6:9] = -r[6:9]
+    return j
+
+
+def relerr(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    if a.size == 0:
+        return 0.0
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-12))
+
+
+def run_one_step(worlds, prm, order=0):
+    odx = _odx()
+    B = odx.Batch(worlds, prm)
+    B.set_solver(order=order, max_iters=int(prm[7]), tol=0.0)
+    B.step(1)
+    return B
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_single_step_matches_oracle_reference_order(seed):
+    say("case: uneven worlds seed", seed)
+    rng = np.random.default_rng(100 + seed)
+    worlds = [S.uneven_world(rng, 8, 24) for _ in range(12)]
+    # sink a few bodies into the ground so every world has contacts
+    for w in worlds:
+        w.bodies[:, 2] = np.minimum(w.bodies[:, 2], 0.3)
+    prm = O.params()
+    odx = _odx()
+    B = run_one_step(worlds, prm)
+    gb = B.bodies()
+    st = B.stats()
+    assert B.overflow().sum() == 0
+    for wi, w in enumerate(worlds):
+        ref_b, ref_c, ref_l = O.world_step(w.bodies, w.geoms, prm, dtype=np.float64)
+        gj = B.joints(wi)
+        assert st["contacts"][wi] == len(ref_c), "contact count differs in world %d" % wi
+        rj = oracle_joints(ref_c, w.geoms)
+        # same geom pairs in the same (canonical) order
+        np.testing.assert_array_equal(gj[:, 14:16].astype(int), ref_c[:, 7:9].astype(int))
+        np.testing.assert_array_equal(gj[:, 1:3].astype(int), rj[:, 1:3].astype(int))
+        assert relerr(gj[:, 3:6], rj[:, 3:6]) < TOL_CONTACT
+        assert np.max(np.abs(gj[:, 6:9] - rj[:, 6:9])) < TOL_CONTACT
+        assert np.max(np.abs(gj[:, 9] - rj[:, 9])) < 1e-5
+        lam = B.lam(wi)
+        assert len(lam) == len(ref_l)
+        assert relerr(lam, ref_l) < TOL_LAMBDA, "lambda world %d: %g" % (wi, relerr(lam, ref_l))
+        wb = B.world_bodies(wi, gb)
+        assert relerr(wb[:, 0:3], ref_b[:, 0:3]) < TOL_STATE
+        assert relerr(wb[:, 4:8], ref_b[:, 4:8]) < TOL_STATE
+        vs = np.hstack([wb[:, 8:11], wb[:, 12:15]])
+        vr = np.hstack([ref_b[:, 8:11], ref_b[:, 12:15]])
+        assert np.max(np.abs(vs - vr)) / max(np.max(np.abs(vr)), 1.0) < TOL_STATE
+    B.close()
+
+
+def test_towers_single_step_and_rows():
+    say("case: towers single step")
+    rng = np.random.default_rng(7)
+    worlds = [S.tower_world(rng, levels=12) for _ in range(8)]
+    prm = O.params()
+    B = run_one_step(worlds, prm)
+    gb = B.bodies()
+    for wi, w in enumerate(worlds):
+        ref_b, ref_c, ref_l = O.world_step(w.bodies, w.geoms, prm, dtype=np.float64)
+        assert B.stats()["contacts"][wi] == len(ref_c)
+        lam = B.lam(wi)
+        assert relerr(lam, ref_l) < TOL_LAMBDA
+        wb = B.world_bodies(wi, gb)
+        assert relerr(wb[:, 0:3], ref_b[:, 0:3]) < TOL_STATE
+        rows = B.rows(wi)
+        # row layout: normal rows have lo = 0, hi = inf; friction rows carry mu and findex
+        m = len(rows)
+        assert m == 3 * len(ref_c)
+        normals = rows[0::3]
+        assert np.all(normals[:, 14] == 0) and np.all(np.isinf(normals[:, 15]))
+        fr = rows[1::3]
+        np.testing.assert_allclose(fr[:, 15], 0.5)
+        fidx = fr[:, 30].view(np.int32)
+        np.testing.assert_array_equal(fidx, np.arange(0, m, 3))
+    B.close()
+
+
+def test_multi_step_trajectory_tracks_float32_oracle():
+    say("case: 10-step trajectory")
+    rng = np.random.default_rng(21)
+    worlds = [S.tower_world(rng, levels=6) for _ in range(4)]
+    prm = O.params()
+    odx = _odx()
+    B = odx.Batch(worlds, prm)
+    B.set_solver(order=0, max_iters=20)
+    ref = [w.bodies.copy() for w in worlds]
+    for s in range(10):
+        B.step(1)
+        for wi, w in enumerate(worlds):
+            ref[wi], _, _ = O.world_step(ref[wi], w.geoms, prm, dtype=np.float32)
+    gb = B.bodies()
+    for wi in range(len(worlds)):
+        wb = B.world_bodies(wi, gb)
+        assert relerr(wb[:, 0:3], ref[wi][:, 0:3]) < 1e-3
+    B.close()
+
+
+def test_capsule_chain_with_ball_joints():
+    say("case: capsule chains")
+    rng = np.random.default_rng(3)
+    worlds = [S.chain_world(rng) for _ in range(4)]
+    prm = O.params(iters=40, skip_connected=1)
+    B = run_one_step(worlds, prm)
+    gb = B.bodies()
+    for wi, w in enumerate(worlds):
+        ref_b, ref_c, ref_l = O.world_step(w.bodies, w.geoms, prm, balls=w.balls, dtype=np.float64)
+        assert B.stats()["contacts"][wi] == len(ref_c)
+        lam = B.lam(wi)
+        assert len(lam) == len(ref_l)
+        assert relerr(lam, ref_l) < TOL_LAMBDA
+        wb = B.world_bodies(wi, gb)
+        assert relerr(wb[:, 0:3], ref_b[:, 0:3]) < TOL_STATE
+        vs = np.hstack([wb[:, 8:11], wb[:, 12:15]])
+        vr = np.hstack([ref_b[:, 8:11], ref_b[:, 12:15]])
+        assert np.max(np.abs(vs - vr)) / max(np.max(np.abs(vr)), 1.0) < TOL_STATE
+    B.close()
+
+
+def test_hash_space_semantics_same_contacts():
+    say("case: hash space")
+    rng = np.random.default_rng(5)
+    worlds = [S.uneven_world(rng, 8, 32) for _ in range(6)]
+    for w in worlds:
+        w.bodies[:, 2] = np.minimum(w.bodies[:, 2], 0.3)
+    prm = O.params(space=1)
+    B = run_one_step(worlds, prm)
+    for wi, w in enumerate(worlds):
+        ref_b, ref_c, ref_l = O.world_step(w.bodies, w.geoms, prm, dtype=np.float64)
+        assert B.stats()["contacts"][wi] == len(ref_c)
+        assert relerr(B.lam(wi), ref_l) < TOL_LAMBDA
+    B.close()
+
+
+def test_coloured_order_converges_and_respects_friction_cone():
+    say("case: coloured order residual")
+    rng = np.random.default_rng(9)
+    worlds = [S.tower_world(rng, levels=16) for _ in range(16)]
+    prm = O.params()
+    odx = _odx()
+    B = odx.Batch(worlds, prm)
+    B.set_solver(order=1, max_iters=2000, tol=1e-4, check_every=5)
+    B.step(1)
+    st = B.stats()
+    assert np.all(st["residual"] <= 1e-4), st["residual"]
+    assert np.all(st["iters"] < 2000)
+    for wi in range(len(worlds)):
+        lam = B.lam(wi).reshape(-1, 3)
+        assert np.all(lam[:, 0] >= 0)
+        assert np.all(np.abs(lam[:, 1:]) <= 0.5 * lam[:, 0:1] * (1 + 1e-6) + 1e-6)
+    # the GPU residual agrees with the float64 re-evaluation on the CPU
+    for wi, w in enumerate(worlds[:4]):
+        j = B.joints(wi).astype(np.float64)
+        r64 = O.lcp_residual(w.bodies, j, prm, B.lam(wi))
+        assert r64 < 2e-4, r64
+    B.close()
+
+
+def test_tolerance_mode_stops_early_and_reference_order_residual_matches_oracle():
+    say("case: tolerance mode")
+    rng = np.random.default_rng(11)
+    worlds = [S.tower_world(rng, levels=8) for _ in range(4)]
+    prm = O.params()
+    odx = _odx()
+    B = odx.Batch(worlds, prm)
+    B.set_solver(order=0, max_iters=5000, tol=1e-4, check_every=1)
+    B.step(1)
+    st = B.stats()
+    for wi, w in enumerate(worlds):
+        ref_b, ref_c, _ = O.world_step(w.bodies, w.geoms, prm, dtype=np.float64)
+        j = O.contacts_to_joints(ref_c, w.geoms)
+        it, res, _ = O.solve_to_residual(w.bodies, j, prm, 1e-4, 5000)
+        assert st["residual"][wi] <= 1e-4
+        # same Gauss-Seidel order: the sweep count to the target matches within
+        # float32 rounding effects
+        assert abs(int(st["iters"][wi]) - it) <= max(2, it // 10), (st["iters"][wi], it)
+    B.close()
+
+
+def test_warm_start_from_previous_lambda_cuts_sweeps():
+    say("case: warm start")
+    rng = np.random.default_rng(13)
+    worlds = [S.tower_world(rng, levels=12) for _ in range(8)]
+    prm = O.params()
+    odx = _odx()
+    B = odx.Batch(worlds, prm)
+    B.set_solver(order=1, max_iters=3000, tol=1e-4, check_every=2, warm=0)
+    B.step(1)
+    snap = B.bodies()
+    B.step(1)
+    cold = B.stats()["iters"].copy()
+    lam_prev = np.zeros(B.row_start[-1], np.float32)
+    for wi in range(len(worlds)):
+        l = B.lam(wi)
+        lam_prev[B.row_start[wi]:B.row_start[wi] + len(l)] = l
+    # re-run the same step warm-started from its own converged impulses
+    B.set_bodies(snap)
+    B.set_lambda0(lam_prev)
+    B.set_solver(order=1, max_iters=3000, tol=1e-4, check_every=2, warm=1)
+    B.step(1)
+    warm = B.stats()["iters"]
+    assert np.all(warm <= 2), warm
+    assert np.mean(cold) > 5 * np.mean(warm)
+    B.close()
