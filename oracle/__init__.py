@@ -21,7 +21,7 @@ JOINT = 16
 # params vector indices (oracle.h ORC_P_*)
 P_H, P_GX, P_GY, P_GZ, P_ERP, P_CFM, P_SOR_W, P_ITERS, P_MAX_VEL, P_MIN_DEPTH = range(10)
 P_MODE, P_MU, P_MU2, P_BOUNCE, P_BOUNCE_VEL, P_SOFT_ERP, P_SOFT_CFM, P_MAXC, P_SPACE = range(10, 19)
-P_MOTION1, P_MOTION2, P_MOTIONN, P_SLIP1, P_SLIP2, P_SKIP_CONNECTED, P_LAMBDA_FLOOR = range(19, 26)
+P_MOTION1, P_MOTION2, P_MOTIONN, P_SLIP1, P_SLIP2, P_SKIP_CONNECTED, P_LAMBDA_FLOOR, P_TOL, P_CHECK_EVERY = range(19, 28)
 
 
 def build():
@@ -72,6 +72,7 @@ def params(**kw):
         "min_depth": P_MIN_DEPTH, "mode": P_MODE, "mu": P_MU, "mu2": P_MU2, "bounce": P_BOUNCE,
         "bounce_vel": P_BOUNCE_VEL, "soft_erp": P_SOFT_ERP, "soft_cfm": P_SOFT_CFM, "maxc": P_MAXC,
         "space": P_SPACE, "skip_connected": P_SKIP_CONNECTED, "lambda_floor": P_LAMBDA_FLOOR,
+        "tol": P_TOL, "check_every": P_CHECK_EVERY,
     }
     for k, v in kw.items():
         if k == "gravity":
@@ -79,6 +80,10 @@ def params(**kw):
         else:
             p[names[k]] = v
     return p
+
+
+def last_sweeps():
+    return lib().orc_last_sweeps()
 
 
 def drand_seq(seed, n):

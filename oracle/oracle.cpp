@@ -183,6 +183,7 @@ template <class R> bool invertPD3(R *out /*3x4*/, const R *I /*3x4*/) {
 }
 
 const double kInf = std::numeric_limits<double>::infinity();
+int g_last_sweeps = 0;
 
 /* ------------------------------------------------------------- records -- */
 enum { CLS_SPHERE = 0, CLS_BOX = 1, CLS_CAPSULE = 2, CLS_PLANE = 4 };
@@ -1113,6 +1114,8 @@ template <class R> struct Params {
     Surface<R> s;
     int maxc, space, skip_connected;
     double lambda_floor;
+    double tol;
+    int check_every;
 };
 
 template <class R> Params<R> read_params(const double *P) {
@@ -1130,6 +1133,8 @@ template <class R> Params<R> read_params(const double *P) {
     p.maxc = (int)P[ORC_P_MAXC]; p.space = (int)P[ORC_P_SPACE];
     p.skip_connected = (int)P[ORC_P_SKIP_CONNECTED];
     p.lambda_floor = P[ORC_P_LAMBDA_FLOOR] > 0 ? P[ORC_P_LAMBDA_FLOOR] : 1e-3;
+    p.tol = P[ORC_P_TOL];
+    p.check_every = P[ORC_P_CHECK_EVERY] >= 1 ? (int)P[ORC_P_CHECK_EVERY] : 1;
     return p;
 }
 
@@ -1400,10 +1405,18 @@ template <class R> int quickstep(int nb, R *bodies, std::vector<Joint<R>> &joint
     System<R> S;
     assemble(nb, bodies, joints, p, S, invI);
     std::vector<R> lambda(S.m, 0), fc((size_t)nb * 6, 0);
+    int sweeps = 0;
     if (S.m > 0) {
-        for (int it = 0; it < p.iters; it++)
+        for (int it = 0; it < p.iters; it++) {
             for (int k = 0; k < S.m; k++) sor_row(S, S.order[k], lambda, fc);
+            sweeps = it + 1;
+            /* optional stopping rule: LCP residual <= tol, checked every check_every sweeps */
+            if (p.tol > 0 && (sweeps % p.check_every == 0 || sweeps == p.iters) &&
+                residual(S, lambda, fc, p.w, p.lambda_floor) <= p.tol)
+                break;
+        }
     }
+    g_last_sweeps = sweeps;
     integrate(nb, bodies, invI, S.m > 0 ? &fc : nullptr, p.h);
     if (lambda_out) for (int i = 0; i < S.m && i < lambda_cap; i++) lambda_out[i] = lambda[i];
     if (Sout) *Sout = S;
@@ -1675,5 +1688,7 @@ int orc_solve_to_residual(int nb, const double *bodies, int nball, const double 
 void orc_normalize3_f32(float *a) { normalize3(a); }
 void orc_normalize3_f64(double *a) { normalize3(a); }
 void orc_plane_space_f64(const double *n, double *p, double *q) { planeSpace(n, p, q); }
+
+int orc_last_sweeps(void) { return g_last_sweeps; }
 
 }  // extern "C"

@@ -42,7 +42,7 @@ enum {
     ORC_P_MODE, ORC_P_MU, ORC_P_MU2, ORC_P_BOUNCE, ORC_P_BOUNCE_VEL,
     ORC_P_SOFT_ERP, ORC_P_SOFT_CFM, ORC_P_MAXC, ORC_P_SPACE,
     ORC_P_MOTION1, ORC_P_MOTION2, ORC_P_MOTIONN, ORC_P_SLIP1, ORC_P_SLIP2,
-    ORC_P_SKIP_CONNECTED, ORC_P_LAMBDA_FLOOR,
+    ORC_P_SKIP_CONNECTED, ORC_P_LAMBDA_FLOOR, ORC_P_TOL, ORC_P_CHECK_EVERY,
     ORC_P__MAX = 32
 };
 
@@ -116,6 +116,11 @@ double orc_lcp_residual(int nb, const double *bodies, int nball, const double *b
 int orc_solve_to_residual(int nb, const double *bodies, int nball, const double *balls,
                           int nc, const double *contacts, const double *params,
                           double target, int maxit, double *lambda, double *res_out);
+
+/* Sweeps run by the last quickstep/world step (the stopping rule of
+ * ORC_P_TOL / ORC_P_CHECK_EVERY ends sweeps early once the LCP residual of the
+ * assembled system is <= tol). */
+int orc_last_sweeps(void);
 
 /* dxSafeNormalize3 with the dNormalize3 fallback; dxPlaneSpace. */
 void orc_normalize3_f32(float *a);
