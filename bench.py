@@ -1,14 +1,19 @@
 """Benchmark of the batched step path on one MI355X.
 
-Prints ONE JSON line.  A step is one pass of the hot path (collide -> assemble
--> PGS sweeps to the LCP residual target -> integrate) over every world of the
-workload; value = world-steps per second (whole job), inputs resident in HBM.
+Prints ONE JSON line.  A step is one pass of the hot path over every world of
+the workload (collide -> assemble -> [warm-start inference] -> SOR-LCP sweeps
+until the world's LCP residual is <= 1e-4 -> integrate); value = world-steps
+per second of the whole job with inputs resident in HBM.
 
 Workloads (seeded synthetic scenes, ode_amd/scenes.py):
-  uneven  4,096 worlds of 8-64 spheres/boxes dropped on a plane
-  towers  4,096 worlds of a 32-box tower plus a projectile sphere
+  towers  4,096 worlds of a 32-box tower plus a projectile sphere; the learned
+          warm start (trained first on the GPU on 2M samples of the same
+          configuration with other seeds) vs cold PGS at the same residual.
+          value = learned warm-start world-steps/s.            (configs[4])
+  uneven  4,096 worlds of 8-64 spheres/boxes dropped on a plane, cold PGS.
+                                                                 (configs[1])
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload uneven]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload towers]
 """
 import argparse
 import json
@@ -23,7 +28,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "world-steps/sec at LCP residual ≤1e-4: warm-start vs cold PGS, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
 ROW_SWEEP_BYTES = 228  # per row per PGS sweep: row 124 B, lambda r/w 8 B, fc gather+scatter 96 B
+MLP_FLOP_PER_CONTACT = 2 * (16 * 128 + 128 * 128 + 128 * 3)
 
 
 def build_worlds(workload, nworlds, seed):
@@ -36,8 +43,8 @@ def build_worlds(workload, nworlds, seed):
 
 
 def cpu_baseline(worlds, bodies_now, body_start, prm, budget_s=12.0, steps_per_world=5):
-    """The oracle's sequential float32 step (same stopping rule) on a sample of
-    the same worlds, from the GPU's current state, on one host core."""
+    """The oracle's sequential float32 step (cold start, same stopping rule) on
+    a sample of the same worlds from the GPU's state after warmup, one core."""
     import oracle as O
     done = 0
     t0 = time.perf_counter()
@@ -52,70 +59,128 @@ def cpu_baseline(worlds, bodies_now, body_start, prm, budget_s=12.0, steps_per_w
             break
     dt = time.perf_counter() - t0
     return {"value": done / dt, "unit": "world-steps/s", "cores": 1, "kind": "port",
-            "sample": "%d worlds x %d steps from the GPU state after warmup (%.1f s, oracle float32, same "
-                      "residual stopping rule)" % (nw, steps_per_world, dt)}
+            "sample": "%d worlds x %d steps from the GPU state after warmup (%.1f s; oracle float32, cold start, "
+                      "reference row order, same residual stopping rule)" % (nw, steps_per_world, dt)}
+
+
+def measure(B, args, warm, nworlds):
+    """Warmup, timed graph replays, then per-kernel hipEvent durations."""
+    B.set_solver(order=args.order, max_iters=args.max_iters, tol=args.tol, check_every=args.check_every, warm=warm)
+    B.step(args.warmup)
+    B.sync()
+    bodies_after_warmup = B.bodies()
+    total_ms = B.time(args.steps)
+    st = B.stats()
+    ach, dur, sweeps, contacts = [], [[], [], []], [], []
+    for _ in range(args.kernel_steps):
+        k = B.kernel_times(1)
+        s = B.stats()
+        row_sweeps = float(np.sum(s["iters"].astype(np.float64) * s["rows"]))
+        ach.append(row_sweeps * ROW_SWEEP_BYTES / (k[2] * 1e-3) / 1e9)
+        for i in range(3):
+            dur[i].append(k[i])
+        sweeps.append(float(np.mean(s["iters"])))
+        contacts.append(float(np.sum(s["contacts"])))
+    ms = total_ms / args.steps
+    return {
+        "world_steps_per_s": nworlds / (ms * 1e-3),
+        "ms_per_step": ms,
+        "mean_sweeps": float(np.mean(st["iters"])),
+        "max_sweeps": int(np.max(st["iters"])),
+        "max_residual": float(np.max(st["residual"])),
+        "worlds_at_target": float(np.mean(st["residual"] <= args.tol)),
+        "contacts_per_step": float(np.sum(st["contacts"])),
+        "kernel_ms": {"prepare": float(np.mean(dur[0])), "infer": float(np.mean(dur[1])),
+                      "solve": float(np.mean(dur[2]))},
+        "solve_gbs": float(np.mean(ach)),
+        "kernel_window_contacts": float(np.mean(contacts)),
+        "overflow": [int(x) for x in B.overflow()],
+        "_bodies": bodies_after_warmup,
+    }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="uneven")
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="towers")
     ap.add_argument("--worlds", type=int, default=4096)
-    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--tol", type=float, default=1e-4)
-    ap.add_argument("--max-iters", type=int, default=1000)
+    ap.add_argument("--max-iters", type=int, default=5000)
     ap.add_argument("--check-every", type=int, default=5)
     ap.add_argument("--order", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-steps", type=int, default=10)
+    ap.add_argument("--train-samples", type=int, default=2_000_000)
+    ap.add_argument("--train-steps", type=int, default=6000)
+    ap.add_argument("--modes", default="cold,plain,learned")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (worlds shard by index "
-              "across GPUs with no exchange, see DESIGN.md)" % args.gpus, file=sys.stderr)
+              "across GPUs with no exchange; see DESIGN.md)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     from ode_amd import odx
     import oracle as O
 
-    worlds = build_worlds(args.workload, args.worlds, args.seed)
     prm = odx.params_vector(iters=args.max_iters)
-    B = odx.Batch(worlds, prm)
-    B.set_solver(order=args.order, max_iters=args.max_iters, tol=args.tol, check_every=args.check_every, warm=0)
-    B.step(args.warmup)
-    B.sync()
-    bodies_after_warmup = B.bodies()
+    modes = args.modes.split(",") if args.workload == "towers" else ["cold"]
+    train_info = None
+    mlp = None
+    if "learned" in modes:
+        from ode_amd import warmstart as WS
+        t0 = time.perf_counter()
+        train_worlds = build_worlds(args.workload, args.worlds, args.seed + 1000)
+        Bt = odx.Batch(train_worlds, prm)
+        per_step = max(1, args.train_samples // 25)
+        xs, ts = [], []
+        Bt.set_mlp(None)
+        Bt.set_solver(order=args.order, max_iters=args.max_iters, tol=1e-5, check_every=args.check_every, warm=3)
+        rng = np.random.default_rng(0)
+        for s in range(100):
+            Bt.step(1)
+            if s % 4 == 3:
+                X, T = Bt.samples()
+                keep = rng.permutation(len(X))[:per_step]
+                xs.append(X[keep])
+                ts.append(T[keep])
+        Bt.close()
+        X, T = np.concatenate(xs), np.concatenate(ts)
+        t1 = time.perf_counter()
+        mlp, losses = WS.train(X, T, steps=args.train_steps, batch=4096)
+        t2 = time.perf_counter()
+        train_info = {"samples": int(len(X)), "collect_s": t1 - t0, "train_s": t2 - t1,
+                      "train_steps": args.train_steps, "loss_first": float(losses[:20].mean()),
+                      "loss_last": float(losses[-20:].mean())}
 
-    total_ms = B.time(args.steps)
-    st = B.stats()
+    worlds = build_worlds(args.workload, args.worlds, args.seed)
+    results = {}
+    bodies0 = None
+    for mode in modes:
+        B = odx.Batch(worlds, prm)
+        if mode == "learned":
+            B.set_mlp(mlp)
+        r = measure(B, args, {"cold": 0, "plain": 2, "learned": 3}[mode], args.worlds)
+        if mode == "cold":
+            bodies0 = r["_bodies"]
+            body_start = B.body_start
+        r.pop("_bodies")
+        results[mode] = r
+        B.close()
 
-    # per-kernel durations (hipEvents on the batch stream) and algorithmic bytes
-    ach, durs_a, durs_c, sweeps, rows = [], [], [], [], []
-    for _ in range(args.kernel_steps):
-        ka, kc = B.kernel_times(1)
-        s = B.stats()
-        row_sweeps = float(np.sum(s["iters"].astype(np.float64) * s["rows"]))
-        ach.append(row_sweeps * ROW_SWEEP_BYTES / (kc * 1e-3) / 1e9)
-        durs_a.append(ka)
-        durs_c.append(kc)
-        sweeps.append(float(np.mean(s["iters"])))
-        rows.append(float(np.mean(s["rows"])))
-    achieved = float(np.mean(ach))
-    ov = B.overflow()
-
-    ms_per_step = total_ms / args.steps
-    value = args.worlds / (ms_per_step * 1e-3)
-    contacts = float(np.sum(st["contacts"]))
+    head = "learned" if "learned" in results else modes[0]
+    R = results[head]
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": R["world_steps_per_s"],
         "unit": "world-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": R["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -124,33 +189,34 @@ def main():
         "config": {
             "workload": args.workload,
             "worlds": args.worlds,
-            "bodies": int(B.nbodies),
-            "solver": "cold PGS, coloured contact blocks, stop at LCP residual <= %g (checked every %d sweeps, "
-                      "cap %d)" % (args.tol, args.check_every, args.max_iters),
+            "solver": "%s start, coloured contact blocks, SOR w=1.3, sweeps until LCP residual <= %g "
+                      "(checked every %d sweeps, cap %d)" % (head, args.tol, args.check_every, args.max_iters),
             "h": 0.01, "mu": 0.5, "erp": 0.2, "cfm": 1e-5, "seed": args.seed,
         },
-        "contacts_per_s": contacts / (ms_per_step * 1e-3),
-        "contacts_per_step": contacts,
-        "mean_sweeps": float(np.mean(st["iters"])),
-        "max_residual": float(np.max(st["residual"])),
-        "worlds_at_target": float(np.mean(st["residual"] <= args.tol)),
-        "kernel_ms": {"prepare": float(np.mean(durs_a)), "solve": float(np.mean(durs_c))},
+        "modes": results,
         "roofline": {
             "bound": "hbm",
-            "achieved": achieved,
+            "achieved": R["solve_gbs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": R["solve_gbs"] / HBM_PEAK_GBS,
             "traffic": None,
             "kernel": "k_solve_small",
         },
-        "overflow": [int(x) for x in ov],
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(worlds, bodies_after_warmup, B.body_start,
-                                           O.params(iters=args.max_iters, tol=args.tol,
-                                                    check_every=args.check_every))
-    B.close()
+    if "cold" in results and head != "cold":
+        out["speedup_vs_cold"] = R["world_steps_per_s"] / results["cold"]["world_steps_per_s"]
+    if "plain" in results and head != "plain":
+        out["speedup_vs_plain_warm"] = R["world_steps_per_s"] / results["plain"]["world_steps_per_s"]
+    if train_info:
+        out["training"] = train_info
+        ki = R["kernel_ms"]["infer"]
+        if ki > 0:
+            out["mlp_infer_tflops"] = R["kernel_window_contacts"] * MLP_FLOP_PER_CONTACT / (ki * 1e-3) / 1e12
+            out["mlp_infer_frac_bf16_peak"] = out["mlp_infer_tflops"] / BF16_PEAK_TFLOPS
+    if not args.no_cpu and bodies0 is not None:
+        out["cpu_baseline"] = cpu_baseline(worlds, bodies0, body_start,
+                                           O.params(iters=args.max_iters, tol=args.tol, check_every=args.check_every))
     print(json.dumps(out), flush=True)
 
 

@@ -87,7 +87,8 @@ int odxBatchSync(odxBatch *b);
  * (total milliseconds). */
 int odxBatchTime(odxBatch *b, int nsteps, float *total_ms);
 /* Runs nsteps directly launched steps with hipEvents around each kernel and
- * returns the average durations [prepare, solve] in milliseconds. */
+ * returns the average durations [prepare, warm-start inference, solve] in
+ * milliseconds (kernel_ms[3]; inference is 0 when not used). */
 int odxBatchKernelTimes(odxBatch *b, int nsteps, float *kernel_ms);
 
 /* Readback.  bodies: [NB][32].  stats per world: contacts, rows, sweeps,
@@ -107,6 +108,38 @@ int odxBatchGetOverflow(odxBatch *b, int *counts);
 int odxBatchRowStarts(odxBatch *b, int *row_start);
 /* Warm-start values per row in the row-capacity layout (host array). */
 int odxBatchSetLambda0(odxBatch *b, const float *lambda0, int n);
+
+/* ------------------------------------------------ learned warm start (MLP) */
+/* A 16 -> 128 -> 128 -> 3 GELU network on bf16 MFMA with fp32 accumulation
+ * that predicts a contact's initial (normal, friction1, friction2) impulses
+ * from its features (layout in ode_amd/csrc/odx_device.h).  No ODE
+ * counterpart: the reference's warm start is compiled out
+ * (ode/src/quickstep.cpp:46-51).  Parameters are a flat float vector
+ * W1[128][16] b1[128] W2[128][128] b2[128] W3[3][128] b3[3] (19,075 floats). */
+typedef struct odxMlp odxMlp;
+odxMlp *odxMlpCreate(unsigned seed);
+void odxMlpDestroy(odxMlp *m);
+/* norm38: input mean[16], input scale[16], output mean[3], output scale[3]. */
+int odxMlpSetNormalization(odxMlp *m, const float *norm38);
+int odxMlpGetParams(odxMlp *m, float *params);
+int odxMlpSetParams(odxMlp *m, const float *params);
+/* Training set: features X[N][16] (raw) and targets T[N][3] (impulse / force scale). */
+int odxMlpSetData(odxMlp *m, const float *X, const float *T, int N);
+/* Loss gradient over the whole uploaded set (no update); grads[19075]. */
+int odxMlpGradients(odxMlp *m, float *grads, float *loss);
+/* nsteps Adam steps on random mini-batches; per-step losses (may be NULL). */
+int odxMlpTrain(odxMlp *m, int nsteps, int batch, float lr, unsigned seed, float *losses);
+/* Predicted impulses (force units) for features X[N][16] -> Y[N][3]. */
+int odxMlpPredict(odxMlp *m, const float *X, int N, float *Y);
+/* MFMA operand-layout probe: per-lane inputs a,b [64][8] -> d [64][4]. */
+int odxMfmaProbe(const float *a, const float *b, float *d);
+
+/* Use the network for warm mode 3 (NULL detaches it; mode 3 then warm-starts
+ * from matched previous-step impulses and only records features). */
+int odxBatchSetMlp(odxBatch *b, const odxMlp *mlp);
+/* Features and converged targets (impulse / force scale) of the last step's
+ * contacts recorded in warm mode 3; returns the count. */
+int odxBatchGetSamples(odxBatch *b, float *X, float *T, int cap);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,1096 @@
+// odx_api.cpp -- the ODE-compatible C API (include/ode.h) over the HIP kernels.
+//
+// Host objects keep ODE's observable structure: bodies and joints are
+// head-inserted into LIFO lists (ode.cpp:54-60, 1383-1439) so the island DFS
+// of dWorldQuickStep (util.cpp:724-855) visits joints in the reference's order,
+// and contact joints live in joint groups emptied LIFO (ode.cpp:1325-1366).
+// The numerical work runs on the GPU: dWorldQuickStep -> k_single_step (body
+// stage, rows, SOR sweeps, feedback, integration); dSpaceCollide -> k_aabbs +
+// k_pairs_all; dCollide -> k_collide_pairs, evaluated for every candidate pair
+// of the current dSpaceCollide at the first call and served from that batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../../include/ode.h"
+#include "odx_device.h"
+#include "odx_host.h"
+
+namespace odx {
+void launch_single_step(const DevWorlds &D, const StepParams &P, int nb, hipStream_t s);
+void launch_aabbs(const float *gviews, int n, float *aabb, hipStream_t s);
+void launch_pairs_all(const float *gviews, const float *aabb, const unsigned *bits, int n, int space, int *pairs,
+                      int *count, int cap, hipStream_t s);
+void launch_collide_pairs(const float *gviews, const int *pairs, int np, int maxc, float *out, int *counts,
+                          hipStream_t s);
+}  // namespace odx
+
+using namespace odx;
+
+// ------------------------------------------------------------ device arena
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    bool ensure(size_t bytes) {
+        if (bytes <= cap && p) return true;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = std::max<size_t>(bytes, 256);
+        want = want + want / 2;
+        if (!odx_check(hipMalloc(&p, want), "hipMalloc")) return false;
+        cap = want;
+        return true;
+    }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+hipStream_t g_stream = nullptr;
+bool ensure_stream() {
+    if (g_stream) return true;
+    return odx_check(hipStreamCreate(&g_stream), "hipStreamCreate");
+}
+
+template <class T> bool up(DevBuf &b, const T *h, size_t n) {
+    if (!b.ensure(n * sizeof(T))) return false;
+    if (n == 0) return true;
+    return odx_check(hipMemcpyAsync(b.p, h, n * sizeof(T), hipMemcpyHostToDevice, g_stream), "hipMemcpyAsync H2D");
+}
+template <class T> bool down(T *h, const DevBuf &b, size_t n) {
+    if (n == 0) return true;
+    return odx_check(hipMemcpyAsync(h, b.p, n * sizeof(T), hipMemcpyDeviceToHost, g_stream), "hipMemcpyAsync D2H");
+}
+
+void rfromq(float *R, const float *q) {
+    float qq1 = 2 * q[1] * q[1], qq2 = 2 * q[2] * q[2], qq3 = 2 * q[3] * q[3];
+    R[0] = 1 - qq2 - qq3; R[1] = 2 * (q[1] * q[2] - q[0] * q[3]); R[2] = 2 * (q[1] * q[3] + q[0] * q[2]); R[3] = 0;
+    R[4] = 2 * (q[1] * q[2] + q[0] * q[3]); R[5] = 1 - qq1 - qq3; R[6] = 2 * (q[2] * q[3] - q[0] * q[1]); R[7] = 0;
+    R[8] = 2 * (q[1] * q[3] - q[0] * q[2]); R[9] = 2 * (q[2] * q[3] + q[0] * q[1]); R[10] = 1 - qq1 - qq2; R[11] = 0;
+}
+
+// Quaternion of a rotation matrix (the trace / largest-diagonal branches of
+// dQfromR, rotation.cpp:258-303).
+void qfromr(float *q, const float *R) {
+    auto r = [&](int i, int j) { return R[i * 4 + j]; };
+    float tr = r(0, 0) + r(1, 1) + r(2, 2), s;
+    if (tr >= 0) {
+        s = std::sqrt(tr + 1);
+        q[0] = 0.5f * s;
+        s = 0.5f / s;
+        q[1] = (r(2, 1) - r(1, 2)) * s; q[2] = (r(0, 2) - r(2, 0)) * s; q[3] = (r(1, 0) - r(0, 1)) * s;
+        return;
+    }
+    int c = 0;
+    if (r(1, 1) > r(0, 0)) c = r(2, 2) > r(1, 1) ? 2 : 1;
+    else if (r(2, 2) > r(0, 0)) c = 2;
+    if (c == 0) {
+        s = std::sqrt((r(0, 0) - (r(1, 1) + r(2, 2))) + 1);
+        q[1] = 0.5f * s; s = 0.5f / s;
+        q[2] = (r(0, 1) + r(1, 0)) * s; q[3] = (r(2, 0) + r(0, 2)) * s; q[0] = (r(2, 1) - r(1, 2)) * s;
+    } else if (c == 1) {
+        s = std::sqrt((r(1, 1) - (r(2, 2) + r(0, 0))) + 1);
+        q[2] = 0.5f * s; s = 0.5f / s;
+        q[3] = (r(1, 2) + r(2, 1)) * s; q[1] = (r(0, 1) + r(1, 0)) * s; q[0] = (r(0, 2) - r(2, 0)) * s;
+    } else {
+        s = std::sqrt((r(2, 2) - (r(0, 0) + r(1, 1))) + 1);
+        q[3] = 0.5f * s; s = 0.5f / s;
+        q[1] = (r(2, 0) + r(0, 2)) * s; q[2] = (r(1, 2) + r(2, 1)) * s; q[0] = (r(1, 0) - r(0, 1)) * s;
+    }
+}
+
+void normq(float *q) {
+    float l = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    if (l > 0) { l = 1.f / std::sqrt(l); for (int i = 0; i < 4; i++) q[i] *= l; }
+    else { q[0] = 1; q[1] = q[2] = q[3] = 0; }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- objects
+struct dxJointNode {
+    dxJoint *joint;
+    dxBody *body;  // the other body (NULL = world)
+    dxJointNode *next;
+};
+
+struct dxBody {
+    dxWorld *world = nullptr;
+    dxBody *next = nullptr, *prev = nullptr;
+    dVector3 pos = {0, 0, 0, 0};
+    dQuaternion q = {1, 0, 0, 0};
+    dMatrix3 R = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    dVector3 lvel = {0, 0, 0, 0}, avel = {0, 0, 0, 0}, facc = {0, 0, 0, 0}, tacc = {0, 0, 0, 0};
+    dMass mass{};
+    bool gravity = true, gyro = true, enabled = true;
+    dxJointNode *firstjoint = nullptr;
+    std::vector<dxGeom *> geoms;
+    void *data = nullptr;
+    int tag = 0;
+    int uid = 0;
+};
+
+struct dxJoint {
+    dxWorld *world = nullptr;
+    int type = dJointTypeNone;
+    dxJointGroup *group = nullptr;
+    dxJointNode node[2] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    bool reverse = false, enabled = true;
+    dContact contact{};
+    dVector3 anchor1 = {0, 0, 0, 0}, anchor2 = {0, 0, 0, 0};
+    float erp = 0.2f, cfm = 1e-5f;
+    dJointFeedback *feedback = nullptr;
+    int tag = 0;
+};
+
+struct dxJointGroup {
+    std::vector<dxJoint *> joints;
+};
+
+struct dxSpace {
+    int kind = 2;  // 0 SAP, 1 hash, 2 simple
+    int minlevel = -3, maxlevel = 10;
+    int cleanup = 1;
+    std::vector<dxGeom *> geoms;
+    dxSpace *parent = nullptr;
+};
+
+struct dxGeom {
+    int cls = 0;
+    float p[4] = {0, 0, 0, 0};
+    dxBody *body = nullptr;
+    dxSpace *space = nullptr;
+    dVector3 pos = {0, 0, 0, 0};
+    dMatrix3 R = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    unsigned long cat = ~0ul, col = ~0ul;
+    bool enabled = true;
+    void *data = nullptr;
+};
+
+struct dxWorld {
+    dxBody *firstbody = nullptr;
+    int nbodies = 0, uid_next = 1;
+    std::vector<dxJoint *> free_joints;  // joints not in a group
+    dVector3 gravity = {0, 0, 0, 0};
+    float erp = 0.2f, cfm = 1e-5f, w = 1.3f;
+    int iterations = 20;
+    float max_vel = dInfinity, min_depth = 0.f;
+    int order = 0;
+    int last_sweeps = 0, last_rows = 0;
+    float last_res = -1.f;
+    // device buffers of the step
+    DevBuf pos, quat, lvel, avel, facc, tacc, mass, Ib, invIb, joints, surf, joff, rows, lambda, sched, phase, bscr,
+        counts, fb, bstart, residual, niters, scratch;
+};
+
+namespace {
+
+int g_seed = 0;  // dRand state (misc.cpp:35-48)
+
+void add_node(dxBody *b, dxJointNode *n) {
+    n->next = b->firstjoint;
+    b->firstjoint = n;
+}
+void remove_node(dxBody *b, dxJointNode *n) {
+    for (dxJointNode **pp = &b->firstjoint; *pp; pp = &(*pp)->next)
+        if (*pp == n) { *pp = n->next; n->next = nullptr; return; }
+}
+void detach(dxJoint *j) {
+    if (j->node[0].body) remove_node(j->node[0].body, &j->node[1]);
+    if (j->node[1].body) remove_node(j->node[1].body, &j->node[0]);
+    j->node[0].body = j->node[1].body = nullptr;
+}
+
+dxJoint *new_joint(dxWorld *w, dxJointGroup *g, int type) {
+    auto *j = new dxJoint();
+    j->world = w;
+    j->type = type;
+    j->group = g;
+    j->node[0].joint = j->node[1].joint = j;
+    j->erp = w->erp;
+    j->cfm = w->cfm;
+    if (g) g->joints.push_back(j);
+    else w->free_joints.push_back(j);
+    return j;
+}
+
+void destroy_joint(dxJoint *j) {
+    detach(j);
+    if (!j->group) {
+        auto &v = j->world->free_joints;
+        v.erase(std::remove(v.begin(), v.end(), j), v.end());
+    }
+    delete j;
+}
+
+bool joint_enabled(const dxJoint *j) {
+    // dxJoint::isEnabled joints/joint.cpp:73-78
+    if (!j->enabled || !j->node[0].body) return false;
+    return j->node[0].body->mass.mass > 0 || (j->node[1].body && j->node[1].body->mass.mass > 0);
+}
+
+void geom_pose(const dxGeom *g, float *pos, float *R) {
+    if (g->body) {
+        for (int k = 0; k < 3; k++) pos[k] = g->body->pos[k];
+        for (int k = 0; k < 12; k++) R[k] = g->body->R[k];
+    } else {
+        for (int k = 0; k < 3; k++) pos[k] = g->pos[k];
+        for (int k = 0; k < 12; k++) R[k] = g->R[k];
+    }
+}
+
+void geom_view(const dxGeom *g, float *v) {
+    v[0] = (float)g->cls;
+    v[1] = g->body ? (float)g->body->uid : -1.f;
+    for (int k = 0; k < 4; k++) v[2 + k] = g->p[k];
+    geom_pose(g, v + 6, v + 9);
+    v[21] = v[22] = v[23] = 0.f;
+}
+
+// Collision session of the current dSpaceCollide (dCollide serves from it).
+struct Session {
+    bool active = false;
+    std::vector<dxGeom *> geoms;
+    std::vector<int> pairs;  // ordered, 2 per pair
+    int current = -1;
+    int maxc = 0;
+    std::vector<float> contacts;  // [pair][4][7]
+    std::vector<int> counts;
+    DevBuf gviews, aabb, bits, dpairs, cnt, out, outcnt;
+};
+Session g_sess;
+
+bool gpu_collide_pairs(Session &S, int maxc) {
+    const int np = (int)S.pairs.size() / 2;
+    if (!S.dpairs.ensure((size_t)np * 2 * sizeof(int)) || !S.out.ensure((size_t)np * ODX_MAXC * 7 * sizeof(float)) ||
+        !S.outcnt.ensure((size_t)np * sizeof(int)))
+        return false;
+    if (np > 0) {
+        if (!odx_check(hipMemcpyAsync(S.dpairs.p, S.pairs.data(), (size_t)np * 2 * sizeof(int), hipMemcpyHostToDevice,
+                                      g_stream), "hipMemcpyAsync"))
+            return false;
+        launch_collide_pairs(S.gviews.as<float>(), S.dpairs.as<int>(), np, maxc, S.out.as<float>(), S.outcnt.as<int>(),
+                             g_stream);
+        if (!odx_check(hipGetLastError(), "k_collide_pairs")) return false;
+    }
+    S.contacts.resize((size_t)np * ODX_MAXC * 7);
+    S.counts.resize(np);
+    if (np > 0 &&
+        (!odx_check(hipMemcpyAsync(S.contacts.data(), S.out.p, S.contacts.size() * sizeof(float), hipMemcpyDeviceToHost,
+                                   g_stream), "hipMemcpyAsync") ||
+         !odx_check(hipMemcpyAsync(S.counts.data(), S.outcnt.p, np * sizeof(int), hipMemcpyDeviceToHost, g_stream),
+                    "hipMemcpyAsync")))
+        return false;
+    if (!odx_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize")) return false;
+    S.maxc = maxc;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ------------------------------------------------------------------ init
+void dInitODE(void) { (void)dInitODE2(0); }
+int dInitODE2(unsigned int) { return ensure_stream() ? 1 : 0; }
+void dCloseODE(void) {
+    if (g_stream) { (void)hipStreamDestroy(g_stream); g_stream = nullptr; }
+}
+
+// ----------------------------------------------------------------- world
+dWorldID dWorldCreate(void) {
+    if (!ensure_stream()) return nullptr;
+    return new dxWorld();
+}
+
+void dWorldDestroy(dWorldID w) {
+    if (!w) return;
+    for (dxJoint *j : std::vector<dxJoint *>(w->free_joints)) destroy_joint(j);
+    for (dxBody *b = w->firstbody; b;) {
+        dxBody *n = b->next;
+        for (dxGeom *g : b->geoms) g->body = nullptr;
+        delete b;
+        b = n;
+    }
+    delete w;
+}
+
+void dWorldSetGravity(dWorldID w, dReal x, dReal y, dReal z) { w->gravity[0] = x; w->gravity[1] = y; w->gravity[2] = z; }
+void dWorldGetGravity(dWorldID w, dVector3 g) { for (int k = 0; k < 3; k++) g[k] = w->gravity[k]; }
+void dWorldSetERP(dWorldID w, dReal erp) { w->erp = erp; }
+dReal dWorldGetERP(dWorldID w) { return w->erp; }
+void dWorldSetCFM(dWorldID w, dReal cfm) { w->cfm = cfm; }
+dReal dWorldGetCFM(dWorldID w) { return w->cfm; }
+void dWorldSetQuickStepNumIterations(dWorldID w, int num) { w->iterations = num; }
+int dWorldGetQuickStepNumIterations(dWorldID w) { return w->iterations; }
+void dWorldSetQuickStepDynamicIterationParameters(dWorldID, const dReal *, const dReal *, const dReal *) {
+    // The GPU solve runs exactly the configured iteration count (the
+    // reference's dynamic adjustment disabled); the parameters are accepted.
+}
+void dWorldSetQuickStepW(dWorldID w, dReal over_relaxation) { w->w = over_relaxation; }
+dReal dWorldGetQuickStepW(dWorldID w) { return w->w; }
+void dWorldSetContactMaxCorrectingVel(dWorldID w, dReal vel) { w->max_vel = vel; }
+void dWorldSetContactSurfaceLayer(dWorldID w, dReal depth) { w->min_depth = depth; }
+void odxWorldSetRowOrder(dWorldID w, int order) { w->order = order ? 1 : 0; }
+void odxWorldGetSolveStats(dWorldID w, int *sweeps, float *residual, int *rows) {
+    if (sweeps) *sweeps = w->last_sweeps;
+    if (residual) *residual = w->last_res;
+    if (rows) *rows = w->last_rows;
+}
+
+// ------------------------------------------------------------------ body
+dBodyID dBodyCreate(dWorldID w) {
+    auto *b = new dxBody();
+    b->world = w;
+    b->uid = w->uid_next++;
+    dMassSetParameters(&b->mass, 1, 0, 0, 0, 1, 1, 1, 0, 0, 0);
+    // head insertion, as addObjectToList (ode.cpp:54-60)
+    b->next = w->firstbody;
+    if (w->firstbody) w->firstbody->prev = b;
+    w->firstbody = b;
+    w->nbodies++;
+    return b;
+}
+
+void dBodyDestroy(dBodyID b) {
+    if (!b) return;
+    dxWorld *w = b->world;
+    for (dxGeom *g : b->geoms) g->body = nullptr;
+    // detach joints (ODE detaches group joints and destroys free ones' links)
+    while (b->firstjoint) detach(b->firstjoint->joint);
+    if (b->prev) b->prev->next = b->next;
+    else w->firstbody = b->next;
+    if (b->next) b->next->prev = b->prev;
+    w->nbodies--;
+    delete b;
+}
+
+void dBodySetData(dBodyID b, void *data) { b->data = data; }
+void *dBodyGetData(dBodyID b) { return b->data; }
+void dBodySetPosition(dBodyID b, dReal x, dReal y, dReal z) { b->pos[0] = x; b->pos[1] = y; b->pos[2] = z; }
+void dBodySetRotation(dBodyID b, const dMatrix3 R) {
+    std::memcpy(b->R, R, sizeof(dMatrix3));
+    qfromr(b->q, b->R);
+    normq(b->q);
+    rfromq(b->R, b->q);
+}
+void dBodySetQuaternion(dBodyID b, const dQuaternion q) {
+    for (int k = 0; k < 4; k++) b->q[k] = q[k];
+    normq(b->q);
+    rfromq(b->R, b->q);
+}
+void dBodySetLinearVel(dBodyID b, dReal x, dReal y, dReal z) { b->lvel[0] = x; b->lvel[1] = y; b->lvel[2] = z; }
+void dBodySetAngularVel(dBodyID b, dReal x, dReal y, dReal z) { b->avel[0] = x; b->avel[1] = y; b->avel[2] = z; }
+const dReal *dBodyGetPosition(dBodyID b) { return b->pos; }
+const dReal *dBodyGetRotation(dBodyID b) { return b->R; }
+const dReal *dBodyGetQuaternion(dBodyID b) { return b->q; }
+const dReal *dBodyGetLinearVel(dBodyID b) { return b->lvel; }
+const dReal *dBodyGetAngularVel(dBodyID b) { return b->avel; }
+void dBodySetMass(dBodyID b, const dMass *m) { b->mass = *m; }
+void dBodyGetMass(dBodyID b, dMass *m) { *m = b->mass; }
+void dBodyAddForce(dBodyID b, dReal fx, dReal fy, dReal fz) { b->facc[0] += fx; b->facc[1] += fy; b->facc[2] += fz; }
+void dBodyAddTorque(dBodyID b, dReal fx, dReal fy, dReal fz) { b->tacc[0] += fx; b->tacc[1] += fy; b->tacc[2] += fz; }
+void dBodyEnable(dBodyID b) { b->enabled = true; }
+void dBodyDisable(dBodyID b) { b->enabled = false; }
+int dBodyIsEnabled(dBodyID b) { return b->enabled ? 1 : 0; }
+void dBodySetGravityMode(dBodyID b, int mode) { b->gravity = mode != 0; }
+void dBodySetGyroscopicMode(dBodyID b, int enabled) { b->gyro = enabled != 0; }
+
+// ---------------------------------------------------------------- joints
+dJointID dJointCreateBall(dWorldID w, dJointGroupID g) { return new_joint(w, g, dJointTypeBall); }
+dJointID dJointCreateContact(dWorldID w, dJointGroupID g, const dContact *c) {
+    dxJoint *j = new_joint(w, g, dJointTypeContact);
+    j->contact = *c;
+    return j;
+}
+void dJointDestroy(dJointID j) {
+    if (!j) return;
+    if (j->group) {  // group joints are only detached; the group owns the memory
+        detach(j);
+        j->enabled = false;
+        return;
+    }
+    destroy_joint(j);
+}
+dJointGroupID dJointGroupCreate(int) { return new dxJointGroup(); }
+void dJointGroupDestroy(dJointGroupID g) {
+    if (!g) return;
+    dJointGroupEmpty(g);
+    delete g;
+}
+void dJointGroupEmpty(dJointGroupID g) {
+    // detached from the most recently created (ode.cpp:1325-1366)
+    for (size_t i = g->joints.size(); i-- > 0;) {
+        detach(g->joints[i]);
+        delete g->joints[i];
+    }
+    g->joints.clear();
+}
+void dJointAttach(dJointID j, dBodyID body1, dBodyID body2) {
+    // ode.cpp:1383-1439
+    if (j->node[0].body || j->node[1].body) detach(j);
+    if (!body1) { body1 = body2; body2 = nullptr; j->reverse = true; }
+    else j->reverse = false;
+    j->node[0].body = body1;
+    j->node[1].body = body2;
+    if (body1) add_node(body1, &j->node[1]);
+    if (body2) add_node(body2, &j->node[0]);
+}
+dJointType dJointGetType(dJointID j) { return (dJointType)j->type; }
+dBodyID dJointGetBody(dJointID j, int index) {
+    // reversed joints report their bodies in the caller's order
+    if (index == 0) return j->reverse ? j->node[1].body : j->node[0].body;
+    if (index == 1) return j->reverse ? j->node[0].body : j->node[1].body;
+    return nullptr;
+}
+void dJointSetFeedback(dJointID j, dJointFeedback *fb) { j->feedback = fb; }
+void dJointSetBallAnchor(dJointID j, dReal x, dReal y, dReal z) {
+    // setAnchors joints/joint.cpp:289-319
+    dxBody *b0 = j->node[0].body;
+    if (!b0) return;
+    float q[3] = {x - b0->pos[0], y - b0->pos[1], z - b0->pos[2]};
+    for (int i = 0; i < 3; i++) j->anchor1[i] = b0->R[i] * q[0] + b0->R[4 + i] * q[1] + b0->R[8 + i] * q[2];
+    dxBody *b1 = j->node[1].body;
+    if (b1) {
+        float r[3] = {x - b1->pos[0], y - b1->pos[1], z - b1->pos[2]};
+        for (int i = 0; i < 3; i++) j->anchor2[i] = b1->R[i] * r[0] + b1->R[4 + i] * r[1] + b1->R[8 + i] * r[2];
+    } else {
+        j->anchor2[0] = x; j->anchor2[1] = y; j->anchor2[2] = z;
+    }
+    j->anchor1[3] = j->anchor2[3] = 0;
+}
+void dJointSetBallAnchor2(dJointID j, dReal x, dReal y, dReal z) {
+    j->anchor2[0] = x; j->anchor2[1] = y; j->anchor2[2] = z; j->anchor2[3] = 0;
+}
+int dAreConnected(dBodyID a, dBodyID b) {
+    for (dxJointNode *n = a->firstjoint; n; n = n->next)
+        if (n->body == b) return 1;
+    return 0;
+}
+int dAreConnectedExcluding(dBodyID a, dBodyID b, int joint_type) {
+    for (dxJointNode *n = a->firstjoint; n; n = n->next)
+        if (n->body == b && n->joint->type != joint_type) return 1;
+    return 0;
+}
+
+// ------------------------------------------------------------------ mass
+void dMassSetZero(dMass *m) { std::memset(m, 0, sizeof(dMass)); }
+void dMassSetParameters(dMass *m, dReal themass, dReal cgx, dReal cgy, dReal cgz, dReal I11, dReal I22, dReal I33,
+                        dReal I12, dReal I13, dReal I23) {
+    dMassSetZero(m);
+    m->mass = themass;
+    m->c[0] = cgx; m->c[1] = cgy; m->c[2] = cgz;
+    m->I[0] = I11; m->I[5] = I22; m->I[10] = I33;
+    m->I[1] = m->I[4] = I12;
+    m->I[2] = m->I[8] = I13;
+    m->I[6] = m->I[9] = I23;
+}
+void dMassSetSphereTotal(dMass *m, dReal total, dReal r) {
+    float ii = 0.4f * total * r * r;
+    dMassSetParameters(m, total, 0, 0, 0, ii, ii, ii, 0, 0, 0);
+}
+void dMassSetSphere(dMass *m, dReal density, dReal r) {
+    dMassSetSphereTotal(m, (dReal)((4.0 / 3.0) * M_PI * r * r * r * density), r);
+}
+void dMassSetBoxTotal(dMass *m, dReal total, dReal lx, dReal ly, dReal lz) {
+    dMassSetParameters(m, total, 0, 0, 0, total / 12.f * (ly * ly + lz * lz), total / 12.f * (lx * lx + lz * lz),
+                       total / 12.f * (lx * lx + ly * ly), 0, 0, 0);
+}
+void dMassSetBox(dMass *m, dReal density, dReal lx, dReal ly, dReal lz) {
+    dMassSetBoxTotal(m, lx * ly * lz * density, lx, ly, lz);
+}
+void dMassSetCapsule(dMass *m, dReal density, int direction, dReal r, dReal len) {
+    // mass.cpp:130-152
+    dMassSetZero(m);
+    float M1 = (float)(M_PI * r * r * len * density);
+    float M2 = (float)((4.0 / 3.0) * M_PI * r * r * r * density);
+    m->mass = M1 + M2;
+    float Ia = M1 * (0.25f * r * r + (1.f / 12.f) * len * len) + M2 * (0.4f * r * r + 0.375f * r * len + 0.25f * len * len);
+    float Ib = (M1 * 0.5f + M2 * 0.4f) * r * r;
+    m->I[0] = m->I[5] = m->I[10] = Ia;
+    if (direction >= 1 && direction <= 3) m->I[(direction - 1) * 5] = Ib;
+}
+void dMassSetCapsuleTotal(dMass *m, dReal total, int direction, dReal r, dReal len) {
+    dMassSetCapsule(m, 1.f, direction, r, len);
+    float s = total / m->mass;
+    m->mass = total;
+    for (int k = 0; k < 12; k++) m->I[k] *= s;
+}
+
+// --------------------------------------------------------------- geoms
+static dxGeom *new_geom(dSpaceID space, int cls) {
+    auto *g = new dxGeom();
+    g->cls = cls;
+    if (space) { g->space = space; space->geoms.push_back(g); }
+    return g;
+}
+dGeomID dCreateSphere(dSpaceID space, dReal radius) {
+    dxGeom *g = new_geom(space, dSphereClass);
+    g->p[0] = radius;
+    return g;
+}
+dGeomID dCreateBox(dSpaceID space, dReal lx, dReal ly, dReal lz) {
+    dxGeom *g = new_geom(space, dBoxClass);
+    g->p[0] = lx; g->p[1] = ly; g->p[2] = lz;
+    return g;
+}
+dGeomID dCreatePlane(dSpaceID space, dReal a, dReal b, dReal c, dReal d) {
+    dxGeom *g = new_geom(space, dPlaneClass);
+    float l = std::sqrt(a * a + b * b + c * c);
+    if (l > 0) { a /= l; b /= l; c /= l; d /= l; }
+    else { a = 1; b = 0; c = 0; d = 0; }
+    g->p[0] = a; g->p[1] = b; g->p[2] = c; g->p[3] = d;
+    return g;
+}
+dGeomID dCreateCapsule(dSpaceID space, dReal radius, dReal length) {
+    dxGeom *g = new_geom(space, dCapsuleClass);
+    g->p[0] = radius; g->p[1] = length;
+    return g;
+}
+void dGeomDestroy(dGeomID g) {
+    if (!g) return;
+    if (g->space) {
+        auto &v = g->space->geoms;
+        v.erase(std::remove(v.begin(), v.end(), g), v.end());
+    }
+    if (g->body) {
+        auto &v = g->body->geoms;
+        v.erase(std::remove(v.begin(), v.end(), g), v.end());
+    }
+    delete g;
+}
+void dGeomSetData(dGeomID g, void *data) { g->data = data; }
+void *dGeomGetData(dGeomID g) { return g->data; }
+void dGeomSetBody(dGeomID g, dBodyID b) {
+    if (g->body) {
+        auto &v = g->body->geoms;
+        v.erase(std::remove(v.begin(), v.end(), g), v.end());
+    }
+    if (!b && g->body) {  // keeps the last pose, as dGeomSetBody(g, 0) does
+        geom_pose(g, g->pos, g->R);
+    }
+    g->body = b;
+    if (b) b->geoms.push_back(g);
+}
+dBodyID dGeomGetBody(dGeomID g) { return g->body; }
+void dGeomSetPosition(dGeomID g, dReal x, dReal y, dReal z) {
+    if (g->body) dBodySetPosition(g->body, x, y, z);
+    else { g->pos[0] = x; g->pos[1] = y; g->pos[2] = z; }
+}
+void dGeomSetRotation(dGeomID g, const dMatrix3 R) {
+    if (g->body) dBodySetRotation(g->body, R);
+    else std::memcpy(g->R, R, sizeof(dMatrix3));
+}
+void dGeomSetQuaternion(dGeomID g, const dQuaternion q) {
+    if (g->body) { dBodySetQuaternion(g->body, q); return; }
+    float qq[4] = {q[0], q[1], q[2], q[3]};
+    normq(qq);
+    rfromq(g->R, qq);
+}
+const dReal *dGeomGetPosition(dGeomID g) { return g->body ? g->body->pos : g->pos; }
+dSpaceID dGeomGetSpace(dGeomID g) { return g->space; }
+int dGeomGetClass(dGeomID g) { return g->cls; }
+void dGeomSetCategoryBits(dGeomID g, unsigned long bits) { g->cat = bits; }
+void dGeomSetCollideBits(dGeomID g, unsigned long bits) { g->col = bits; }
+void dGeomEnable(dGeomID g) { g->enabled = true; }
+void dGeomDisable(dGeomID g) { g->enabled = false; }
+dReal dGeomSphereGetRadius(dGeomID g) { return g->p[0]; }
+void dGeomBoxGetLengths(dGeomID g, dVector3 r) { r[0] = g->p[0]; r[1] = g->p[1]; r[2] = g->p[2]; }
+void dGeomPlaneGetParams(dGeomID g, dVector4 r) { for (int k = 0; k < 4; k++) r[k] = g->p[k]; }
+void dGeomCapsuleGetParams(dGeomID g, dReal *radius, dReal *length) { *radius = g->p[0]; *length = g->p[1]; }
+
+void dGeomGetAABB(dGeomID g, dReal aabb[6]) {
+    // host restatement of the computeAABB family for the query API
+    float pos[3], R[12];
+    geom_pose(g, pos, R);
+    const float inf = dInfinity;
+    switch (g->cls) {
+    case dSphereClass:
+        for (int k = 0; k < 3; k++) { aabb[2 * k] = pos[k] - g->p[0]; aabb[2 * k + 1] = pos[k] + g->p[0]; }
+        break;
+    case dBoxClass:
+        for (int k = 0; k < 3; k++) {
+            float r = 0.5f * (std::fabs(R[4 * k] * g->p[0]) + std::fabs(R[4 * k + 1] * g->p[1]) + std::fabs(R[4 * k + 2] * g->p[2]));
+            aabb[2 * k] = pos[k] - r; aabb[2 * k + 1] = pos[k] + r;
+        }
+        break;
+    case dCapsuleClass:
+        for (int k = 0; k < 3; k++) {
+            float r = std::fabs(R[4 * k + 2] * g->p[1]) * 0.5f + g->p[0];
+            aabb[2 * k] = pos[k] - r; aabb[2 * k + 1] = pos[k] + r;
+        }
+        break;
+    default:
+        for (int k = 0; k < 3; k++) { aabb[2 * k] = -inf; aabb[2 * k + 1] = inf; }
+        if (g->p[1] == 0 && g->p[2] == 0) { aabb[0] = g->p[0] > 0 ? -inf : -g->p[3]; aabb[1] = g->p[0] > 0 ? g->p[3] : inf; }
+        else if (g->p[0] == 0 && g->p[2] == 0) { aabb[2] = g->p[1] > 0 ? -inf : -g->p[3]; aabb[3] = g->p[1] > 0 ? g->p[3] : inf; }
+        else if (g->p[0] == 0 && g->p[1] == 0) { aabb[4] = g->p[2] > 0 ? -inf : -g->p[3]; aabb[5] = g->p[2] > 0 ? g->p[3] : inf; }
+    }
+}
+
+// ---------------------------------------------------------------- spaces
+dSpaceID dSimpleSpaceCreate(dSpaceID parent) { auto *s = new dxSpace(); s->kind = 2; s->parent = parent; return s; }
+dSpaceID dHashSpaceCreate(dSpaceID parent) { auto *s = new dxSpace(); s->kind = 1; s->parent = parent; return s; }
+dSpaceID dSweepAndPruneSpaceCreate(dSpaceID parent, int) { auto *s = new dxSpace(); s->kind = 0; s->parent = parent; return s; }
+void dSpaceDestroy(dSpaceID s) {
+    if (!s) return;
+    std::vector<dxGeom *> gs = s->geoms;
+    for (dxGeom *g : gs) {
+        if (s->cleanup) dGeomDestroy(g);
+        else g->space = nullptr;
+    }
+    delete s;
+}
+void dHashSpaceSetLevels(dSpaceID s, int minlevel, int maxlevel) { s->minlevel = minlevel; s->maxlevel = maxlevel; }
+void dSpaceSetCleanup(dSpaceID s, int mode) { s->cleanup = mode; }
+void dSpaceAdd(dSpaceID s, dGeomID g) {
+    if (g->space) {
+        auto &v = g->space->geoms;
+        v.erase(std::remove(v.begin(), v.end(), g), v.end());
+    }
+    g->space = s;
+    s->geoms.push_back(g);
+}
+int dSpaceGetNumGeoms(dSpaceID s) { return (int)s->geoms.size(); }
+
+// dSpaceCollide: candidate pairs on the GPU, ordered on the host as the space
+// would report them, then the callback per pair.
+void dSpaceCollide(dSpaceID space, void *data, dNearCallback *callback) {
+    if (!space || !callback || !ensure_stream()) return;
+    Session &S = g_sess;
+    S.geoms.clear();
+    for (dxGeom *g : space->geoms)
+        if (g->enabled) S.geoms.push_back(g);
+    const int n = (int)S.geoms.size();
+    S.pairs.clear();
+    S.maxc = 0;
+    if (n >= 2) {
+        std::vector<float> gv((size_t)n * GVIEW_STRIDE);
+        std::vector<unsigned> bits((size_t)n * 2);
+        for (int i = 0; i < n; i++) {
+            geom_view(S.geoms[i], &gv[(size_t)i * GVIEW_STRIDE]);
+            bits[2 * i] = (unsigned)S.geoms[i]->cat;
+            bits[2 * i + 1] = (unsigned)S.geoms[i]->col;
+        }
+        const long long T = (long long)n * (n - 1) / 2;
+        int cap = (int)std::min<long long>(T, 1 << 26);
+        int count = 0;
+        bool ok = up(S.gviews, gv.data(), gv.size()) && up(S.bits, bits.data(), bits.size()) &&
+                  S.aabb.ensure((size_t)n * 6 * sizeof(float)) && S.cnt.ensure(sizeof(int)) &&
+                  S.dpairs.ensure((size_t)cap * 2 * sizeof(int)) &&
+                  odx_check(hipMemsetAsync(S.cnt.p, 0, sizeof(int), g_stream), "hipMemsetAsync");
+        if (ok) {
+            launch_aabbs(S.gviews.as<float>(), n, S.aabb.as<float>(), g_stream);
+            launch_pairs_all(S.gviews.as<float>(), S.aabb.as<float>(), S.bits.as<unsigned>(), n, space->kind == 0 ? 0 : 1,
+                             S.dpairs.as<int>(), S.cnt.as<int>(), cap, g_stream);
+            ok = odx_check(hipGetLastError(), "k_pairs_all") && down(&count, S.cnt, 1) &&
+                 odx_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize");
+        }
+        if (!ok) return;
+        if (count > cap) {
+            odx_message(3, "dSpaceCollide: %d candidate pairs exceed the buffer (%d); extra pairs dropped", count, cap);
+            count = cap;
+        }
+        std::vector<int> raw((size_t)count * 2);
+        std::vector<float> ab((size_t)n * 6);
+        if (!down(raw.data(), S.dpairs, raw.size()) || !down(ab.data(), S.aabb, ab.size()) ||
+            !odx_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize"))
+            return;
+        // order: sweep-and-prune emits finite pairs by the x-min sort order of
+        // the first geom then the second (BoxPruning collision_sapspace.cpp:560-621),
+        // then infinite x infinite, then infinite x finite (:517-532); the other
+        // spaces report pairs in geom order.
+        std::vector<std::pair<long long, int>> keyed((size_t)count);
+        if (space->kind == 0) {
+            std::vector<int> fin, inf;
+            for (int i = 0; i < n; i++) (ab[(size_t)i * 6 + 1] == dInfinity ? inf : fin).push_back(i);
+            auto fkey = [&](int i) {
+                float f = ab[(size_t)i * 6];
+                uint32_t u;
+                std::memcpy(&u, &f, 4);
+                return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+            };
+            std::vector<int> order = fin;
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return fkey(a) < fkey(b); });
+            std::vector<int> rank(n, -1), irank(n, -1), frank(n, -1);
+            for (size_t k = 0; k < order.size(); k++) rank[order[k]] = (int)k;
+            for (size_t k = 0; k < inf.size(); k++) irank[inf[k]] = (int)k;
+            for (size_t k = 0; k < fin.size(); k++) frank[fin[k]] = (int)k;
+            const long long F = (long long)fin.size() + 1, I = (long long)inf.size() + 1;
+            for (int k = 0; k < count; k++) {
+                int a = raw[2 * k], b = raw[2 * k + 1];
+                long long key;
+                if (rank[a] >= 0 && rank[b] >= 0) {
+                    if (rank[a] > rank[b]) std::swap(a, b);
+                    key = rank[a] * F + rank[b];
+                    raw[2 * k] = a; raw[2 * k + 1] = b;
+                } else if (irank[a] >= 0 && irank[b] >= 0) {
+                    if (irank[a] > irank[b]) std::swap(a, b);
+                    key = F * F + (long long)irank[a] * (I + F) + irank[b];
+                    raw[2 * k] = a; raw[2 * k + 1] = b;
+                } else {
+                    if (irank[a] < 0) std::swap(a, b);  // a infinite
+                    key = F * F + (long long)irank[a] * (I + F) + I + frank[b];
+                    raw[2 * k] = a; raw[2 * k + 1] = b;
+                }
+                keyed[k] = {key, k};
+            }
+        } else {
+            for (int k = 0; k < count; k++) keyed[k] = {(long long)raw[2 * k] * n + raw[2 * k + 1], k};
+        }
+        std::sort(keyed.begin(), keyed.end());
+        S.pairs.resize((size_t)count * 2);
+        for (int k = 0; k < count; k++) {
+            S.pairs[2 * k] = raw[2 * keyed[k].second];
+            S.pairs[2 * k + 1] = raw[2 * keyed[k].second + 1];
+        }
+    }
+    S.active = true;
+    const int np = (int)S.pairs.size() / 2;
+    for (int k = 0; k < np; k++) {
+        S.current = k;
+        callback(data, S.geoms[S.pairs[2 * k]], S.geoms[S.pairs[2 * k + 1]]);
+    }
+    S.active = false;
+    S.current = -1;
+}
+
+// dCollide collision_kernel.cpp:292-329.  Inside dSpaceCollide the contacts of
+// every candidate pair are computed in one launch at the first call.
+int dCollide(dGeomID o1, dGeomID o2, int flags, dContactGeom *contact, int skip) {
+    if (!o1 || !o2 || !contact) return 0;
+    int maxc = flags & 0xffff;
+    if (maxc == 0) return 0;
+    if (o1 == o2) return 0;
+    if (o1->body && o1->body == o2->body) return 0;
+    if (skip < (int)sizeof(dContactGeom)) skip = (int)sizeof(dContactGeom);
+    if (maxc > ODX_MAXC) maxc = ODX_MAXC;
+    if (!ensure_stream()) return 0;
+    Session &S = g_sess;
+    const float *res = nullptr;
+    int n = 0;
+    std::vector<float> single;
+    if (S.active && S.current >= 0 && S.geoms[S.pairs[2 * S.current]] == o1 &&
+        S.geoms[S.pairs[2 * S.current + 1]] == o2) {
+        if (S.maxc != maxc && !gpu_collide_pairs(S, maxc)) return 0;
+        n = S.counts[S.current];
+        res = &S.contacts[(size_t)S.current * ODX_MAXC * 7];
+    } else {
+        // a lone pair: one launch on its own buffers
+        static Session one;
+        one.geoms = {o1, o2};
+        std::vector<float> gv(2 * GVIEW_STRIDE);
+        geom_view(o1, gv.data());
+        geom_view(o2, gv.data() + GVIEW_STRIDE);
+        one.pairs = {0, 1};
+        if (!up(one.gviews, gv.data(), gv.size()) || !gpu_collide_pairs(one, maxc)) return 0;
+        n = one.counts[0];
+        single = one.contacts;
+        res = single.data();
+    }
+    for (int i = 0; i < n; i++) {
+        dContactGeom *c = (dContactGeom *)((char *)contact + (size_t)i * skip);
+        const float *r = res + (size_t)i * 7;
+        c->pos[0] = r[0]; c->pos[1] = r[1]; c->pos[2] = r[2]; c->pos[3] = 0;
+        c->normal[0] = r[3]; c->normal[1] = r[4]; c->normal[2] = r[5]; c->normal[3] = 0;
+        c->depth = r[6];
+        c->g1 = o1;
+        c->g2 = o2;
+        c->side1 = -1;
+        c->side2 = -1;
+    }
+    return n;
+}
+
+// ------------------------------------------------------------------ step
+// dWorldQuickStep objects.h:427 / ode.cpp:1847: islands by the reference's
+// DFS, rows and sweeps on the GPU.  Returns 1, or 0 (state unchanged) when
+// the device work fails.
+int dWorldQuickStep(dWorldID w, dReal stepsize) {
+    if (!w || !(stepsize > 0) || !ensure_stream()) return 0;
+    // ---- islands (util.cpp:724-855): body list order, LIFO joint lists
+    std::vector<dxBody *> bodies;
+    std::vector<dxJoint *> joints;
+    for (dxBody *b = w->firstbody; b; b = b->next) b->tag = 0;
+    std::vector<dxBody *> stack;
+    std::vector<char> jtag;
+    auto tagged = [&](dxJoint *j) { return j->tag != 0; };
+    std::vector<dxJoint *> seen;
+    for (dxBody *bb = w->firstbody; bb; bb = bb->next) {
+        if (bb->tag || !bb->enabled) continue;
+        bb->tag = 1;
+        bodies.push_back(bb);
+        dxBody *b = bb;
+        while (true) {
+            for (dxJointNode *nd = b->firstjoint; nd; nd = nd->next) {
+                dxJoint *j = nd->joint;
+                if (tagged(j)) continue;
+                if (joint_enabled(j)) {
+                    j->tag = 1;
+                    seen.push_back(j);
+                    joints.push_back(j);
+                    dxBody *o = nd->body;
+                    if (o && o->tag <= 0) {
+                        o->tag = 1;
+                        o->enabled = true;
+                        stack.push_back(o);
+                    }
+                } else {
+                    j->tag = -1;
+                    seen.push_back(j);
+                }
+            }
+            if (stack.empty()) break;
+            b = stack.back();
+            stack.pop_back();
+            bodies.push_back(b);
+        }
+    }
+    for (dxJoint *j : seen) j->tag = 0;
+    const int nb = (int)bodies.size();
+    if (nb > SL_MAXB) {
+        odx_message(4, "dWorldQuickStep: %d enabled bodies exceed this build's single-world limit (%d)", nb, SL_MAXB);
+        return 0;
+    }
+    for (int i = 0; i < nb; i++) bodies[i]->tag = i;
+
+    // ---- records
+    std::vector<float4> pos(nb), quat(nb), lvel(nb), avel(nb), facc(nb), tacc(nb), mass(nb);
+    std::vector<float> Ib((size_t)nb * 12), invIb((size_t)nb * 12);
+    for (int i = 0; i < nb; i++) {
+        dxBody *b = bodies[i];
+        pos[i] = make_float4(b->pos[0], b->pos[1], b->pos[2], 0.f);
+        quat[i] = make_float4(b->q[0], b->q[1], b->q[2], b->q[3]);
+        lvel[i] = make_float4(b->lvel[0], b->lvel[1], b->lvel[2], 0.f);
+        avel[i] = make_float4(b->avel[0], b->avel[1], b->avel[2], 0.f);
+        facc[i] = make_float4(b->facc[0], b->facc[1], b->facc[2], 0.f);
+        tacc[i] = make_float4(b->tacc[0], b->tacc[1], b->tacc[2], 0.f);
+        const float flags = (float)((b->gravity ? 0 : 1) | (b->gyro ? 2 : 0));
+        mass[i] = make_float4(b->mass.mass, 1.f / b->mass.mass, flags, 0.f);
+        float *I = &Ib[(size_t)i * 12];
+        for (int k = 0; k < 12; k++) I[k] = b->mass.I[k];
+        I[3] = I[7] = I[11] = 0.f;
+        odx_invert_pd3(I, &invIb[(size_t)i * 12]);
+    }
+    const int nj = (int)joints.size();
+    std::vector<float> jrec((size_t)nj * JOINT_STRIDE, 0.f);
+    std::vector<Surface> surf;
+    std::vector<int> joff(nj), jm(nj);
+    std::vector<int> rb1, rb2, rfl;
+    int m = 0;
+    for (int k = 0; k < nj; k++) {
+        dxJoint *j = joints[k];
+        float *r = &jrec[(size_t)k * JOINT_STRIDE];
+        r[1] = (float)j->node[0].body->tag;
+        r[2] = j->node[1].body ? (float)j->node[1].body->tag : -1.f;
+        int mj;
+        if (j->type == dJointTypeContact) {
+            const dContact &c = j->contact;
+            r[0] = 0;
+            const float sg = j->reverse ? -1.f : 1.f;
+            for (int i = 0; i < 3; i++) { r[3 + i] = c.geom.pos[i]; r[6 + i] = sg * c.geom.normal[i]; r[10 + i] = c.fdir1[i]; }
+            r[9] = c.geom.depth;
+            r[13] = (float)surf.size();
+            Surface s;
+            s.mode = c.surface.mode;
+            s.mu = c.surface.mu < 0 ? 0.f : c.surface.mu;
+            s.mu2 = c.surface.mu2 < 0 ? 0.f : c.surface.mu2;
+            s.bounce = c.surface.bounce; s.bounce_vel = c.surface.bounce_vel;
+            s.soft_erp = c.surface.soft_erp; s.soft_cfm = c.surface.soft_cfm;
+            s.motion1 = c.surface.motion1; s.motion2 = c.surface.motion2; s.motionN = c.surface.motionN;
+            s.slip1 = c.surface.slip1; s.slip2 = c.surface.slip2;
+            surf.push_back(s);
+            // getInfo1 contact.cpp:48-123
+            mj = (s.mode & dContactMu2) ? 1 + (s.mu > 0) + (s.mu2 > 0) : 1 + (s.mu > 0 ? 2 : 0);
+            // findex rows (friction with Approx1), in contact_info2's row order
+            rfl.push_back(0);
+            int row = 1;
+            if (mj > 1) {
+                if (s.mu > 0) { rfl.push_back((s.mode & dContactApprox1_1) ? 1 : 0); row++; }
+                if (row < mj) rfl.push_back((s.mode & dContactApprox1_2) ? 1 : 0);
+            }
+        } else {
+            r[0] = 1;
+            for (int i = 0; i < 3; i++) { r[3 + i] = j->anchor1[i]; r[6 + i] = j->anchor2[i]; }
+            r[9] = j->erp;
+            r[10] = j->cfm;
+            mj = 3;
+            rfl.push_back(0); rfl.push_back(0); rfl.push_back(0);
+        }
+        joff[k] = m;
+        jm[k] = mj;
+        for (int q = 0; q < mj; q++) { rb1.push_back((int)r[1]); rb2.push_back((int)r[2]); }
+        m += mj;
+    }
+    if (surf.empty()) surf.push_back(Surface{});
+    // ---- schedule (see odx_step.hip): dependency levels over the ReorderPrep
+    // order, or greedy colours of whole joints
+    std::vector<int> sched(m), phase;
+    {
+        std::vector<int> level;
+        int L = 0;
+        if (w->order == 0) {
+            level.assign(m, 0);
+            std::vector<int> last(nb, 0);
+            for (int pass = 0; pass < 2; pass++)
+                for (int r = 0; r < m; r++) {
+                    if (rfl[r] != pass) continue;
+                    int lv = last[rb1[r]];
+                    if (rb2[r] >= 0) lv = std::max(lv, last[rb2[r]]);
+                    lv++;
+                    last[rb1[r]] = lv;
+                    if (rb2[r] >= 0) last[rb2[r]] = lv;
+                    level[r] = lv - 1;
+                    L = std::max(L, lv);
+                }
+            std::vector<int> cnt(L + 1, 0);
+            for (int r = 0; r < m; r++) cnt[level[r] + 1]++;
+            for (int p = 1; p <= L; p++) cnt[p] += cnt[p - 1];
+            phase = cnt;
+            for (int r = 0; r < m; r++) sched[cnt[level[r]]++] = (r << 3) | 1;
+        } else {
+            level.assign(nj, 0);
+            std::vector<std::vector<char>> used(nb);
+            for (int k = 0; k < nj; k++) {
+                const int a = rb1[joff[k]], b = rb2[joff[k]];
+                int c = 0;
+                while (true) {
+                    bool ua = c < (int)used[a].size() && used[a][c];
+                    bool ub = b >= 0 && c < (int)used[b].size() && used[b][c];
+                    if (!ua && !ub) break;
+                    c++;
+                }
+                if ((int)used[a].size() <= c) used[a].resize(c + 1, 0);
+                used[a][c] = 1;
+                if (b >= 0) { if ((int)used[b].size() <= c) used[b].resize(c + 1, 0); used[b][c] = 1; }
+                level[k] = c;
+                L = std::max(L, c + 1);
+            }
+            std::vector<int> cnt(L + 1, 0);
+            for (int k = 0; k < nj; k++) cnt[level[k] + 1]++;
+            for (int p = 1; p <= L; p++) cnt[p] += cnt[p - 1];
+            phase = cnt;
+            sched.resize(nj);
+            for (int k = 0; k < nj; k++) sched[cnt[level[k]]++] = (joff[k] << 3) | jm[k];
+        }
+    }
+    const int L = (int)phase.size() - 1;
+
+    // ---- device
+    int bstart[2] = {0, nb};
+    bool ok = up(w->pos, pos.data(), nb) && up(w->quat, quat.data(), nb) && up(w->lvel, lvel.data(), nb) &&
+              up(w->avel, avel.data(), nb) && up(w->facc, facc.data(), nb) && up(w->tacc, tacc.data(), nb) &&
+              up(w->mass, mass.data(), nb) && up(w->Ib, Ib.data(), Ib.size()) && up(w->invIb, invIb.data(), invIb.size()) &&
+              up(w->joints, jrec.data(), jrec.size()) && up(w->surf, surf.data(), surf.size()) &&
+              up(w->joff, joff.data(), joff.size()) && up(w->sched, sched.data(), sched.size()) &&
+              up(w->phase, phase.data(), phase.size()) && up(w->bstart, bstart, 2) &&
+              w->rows.ensure((size_t)std::max(m, 1) * ROW_STRIDE * sizeof(float)) &&
+              w->lambda.ensure((size_t)std::max(m, 1) * sizeof(float)) &&
+              w->bscr.ensure((size_t)std::max(nb, 1) * BSCR_STRIDE * sizeof(float)) &&
+              w->fb.ensure((size_t)std::max(nj, 1) * 12 * sizeof(float)) && w->counts.ensure(4 * sizeof(int)) &&
+              w->residual.ensure(sizeof(float)) && w->niters.ensure(sizeof(int));
+    if (!ok) return 0;
+    int cnts[4] = {nj, 0, m, L};  // njoints, ncontacts (unused), nrows, nphases
+    ok = up(w->counts, cnts, 4);
+    if (!ok) return 0;
+    DevWorlds D{};
+    D.nworlds = 1;
+    D.body_start = w->bstart.as<int>();
+    D.pos = w->pos.as<float4>(); D.quat = w->quat.as<float4>(); D.lvel = w->lvel.as<float4>(); D.avel = w->avel.as<float4>();
+    D.facc = w->facc.as<float4>(); D.tacc = w->tacc.as<float4>(); D.mass = w->mass.as<float4>();
+    D.Ib = w->Ib.as<float>(); D.invIb = w->invIb.as<float>();
+    D.surf = w->surf.as<Surface>();
+    D.joints = w->joints.as<float>();
+    D.njoints = w->counts.as<int>();
+    D.ncontacts = w->counts.as<int>() + 1;
+    D.nrows = w->counts.as<int>() + 2;
+    D.nphases = w->counts.as<int>() + 3;
+    D.niters = w->niters.as<int>();
+    D.residual = w->residual.as<float>();
+    D.rows = w->rows.as<float>();
+    D.lambda = w->lambda.as<float>();
+    D.sched = w->sched.as<int>();
+    D.phase_start = w->phase.as<int>();
+    D.bscr = w->bscr.as<float>();
+    D.joff = w->joff.as<int>();
+    D.fb = w->fb.as<float>();
+    StepParams P{};
+    P.h = stepsize;
+    P.gx = w->gravity[0]; P.gy = w->gravity[1]; P.gz = w->gravity[2];
+    P.erp = w->erp; P.cfm = w->cfm; P.sor_w = w->w;
+    P.max_iters = w->iterations;
+    P.tol = 0.f;
+    P.check_every = 1;
+    P.max_vel = w->max_vel; P.min_depth = w->min_depth;
+    P.maxc = 4;
+    P.order = w->order;
+    P.lambda_floor = 1e-3f;
+    P.collide = 0;
+    bool want_fb = false;
+    for (dxJoint *j : joints) want_fb = want_fb || j->feedback;
+    P.feedback = want_fb ? 1 : 0;
+    launch_single_step(D, P, nb, g_stream);
+    if (!odx_check(hipGetLastError(), "k_single_step")) return 0;
+    std::vector<float> fbh(want_fb ? (size_t)nj * 12 : 0);
+    int it = 0;
+    float res = -1.f;
+    ok = down(pos.data(), w->pos, nb) && down(quat.data(), w->quat, nb) && down(lvel.data(), w->lvel, nb) &&
+         down(avel.data(), w->avel, nb) && down(&it, w->niters, 1) && down(&res, w->residual, 1) &&
+         (!want_fb || down(fbh.data(), w->fb, fbh.size())) &&
+         odx_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize");
+    if (!ok) return 0;
+    for (int i = 0; i < nb; i++) {
+        dxBody *b = bodies[i];
+        b->pos[0] = pos[i].x; b->pos[1] = pos[i].y; b->pos[2] = pos[i].z;
+        b->q[0] = quat[i].x; b->q[1] = quat[i].y; b->q[2] = quat[i].z; b->q[3] = quat[i].w;
+        rfromq(b->R, b->q);
+        b->lvel[0] = lvel[i].x; b->lvel[1] = lvel[i].y; b->lvel[2] = lvel[i].z;
+        b->avel[0] = avel[i].x; b->avel[1] = avel[i].y; b->avel[2] = avel[i].z;
+        for (int k = 0; k < 3; k++) { b->facc[k] = 0; b->tacc[k] = 0; }
+    }
+    if (want_fb) {
+        for (int k = 0; k < nj; k++) {
+            dJointFeedback *fb = joints[k]->feedback;
+            if (!fb) continue;
+            const float *f = &fbh[(size_t)k * 12];
+            for (int i = 0; i < 3; i++) { fb->f1[i] = f[i]; fb->t1[i] = f[3 + i]; }
+            if (joints[k]->node[1].body)
+                for (int i = 0; i < 3; i++) { fb->f2[i] = f[6 + i]; fb->t2[i] = f[9 + i]; }
+        }
+    }
+    w->last_sweeps = it;
+    w->last_res = res;
+    w->last_rows = m;
+    return 1;
+}
+
+// ------------------------------------------------------------------ misc
+unsigned long dRand(void) {
+    g_seed = (int)((1664525u * (uint32_t)g_seed + 1013904223u) & 0xffffffffu);
+    return (unsigned long)(uint32_t)g_seed;
+}
+unsigned long dRandGetSeed(void) { return (unsigned long)(uint32_t)g_seed; }
+void dRandSetSeed(unsigned long s) { g_seed = (int)(uint32_t)s; }
+int dRandInt(int n) {
+    // misc.cpp:78-130
+    uint32_t r = (uint32_t)dRand(), un = (uint32_t)n;
+    if (un <= 0x10u) {
+        r ^= (r >> 16); r ^= (r >> 8); r ^= (r >> 4);
+        if (un <= 0x2u) { r ^= (r >> 2); r ^= (r >> 1); return (int)(r & (un >> 1)); }
+        if (un <= 0x4u) { r ^= (r >> 2); return (int)(((r & 0x03u) * un) >> 2); }
+        return (int)(((r & 0x0Fu) * un) >> 4);
+    }
+    if (un <= 0x100u) { r ^= (r >> 16); r ^= (r >> 8); return (int)(((r & 0xFFu) * un) >> 8); }
+    if (un <= 0x10000u) { r ^= (r >> 16); return (int)(((r & 0xFFFFu) * un) >> 16); }
+    return (int)(((uint64_t)r * un) >> 32);
+}
+
+}  // extern "C"

@@ -19,6 +19,8 @@
 namespace odx {
 void launch_prepare_small(const DevWorlds &D, const StepParams &P, hipStream_t s);
 void launch_solve_small(const DevWorlds &D, const StepParams &P, hipStream_t s);
+void launch_mlp_infer(const odxMlp *m, const float *feat, const int *feat_row, const int *count, float *lambda0,
+                      int max_entries, hipStream_t s);
 }  // namespace odx
 
 using namespace odx;
@@ -120,9 +122,17 @@ struct odxBatch {
     void drop_graph() {
         if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }
     }
+    const odxMlp *mlp = nullptr;
+    bool infer_on() const { return P.warm == 3 && mlp; }
+    bool enqueue_infer() {
+        if (!infer_on()) return true;
+        launch_mlp_infer(mlp, D.feat, D.feat_row, D.feat_count, D.lambda0, jcap, stream);
+        return odx_check(hipGetLastError(), "k_mlp_infer launch");
+    }
     bool enqueue_step() {
         launch_prepare_small(D, P, stream);
         if (!odx_check(hipGetLastError(), "k_prepare_small launch")) return false;
+        if (!enqueue_infer()) return false;
         launch_solve_small(D, P, stream);
         return odx_check(hipGetLastError(), "k_solve_small launch");
     }
@@ -310,7 +320,8 @@ odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies
     ok = ok && B->alloc(&D.rows, (size_t)B->rcap * ROW_STRIDE) && B->alloc(&D.lambda, B->rcap);
     ok = ok && B->alloc(&D.sched, B->rcap) && B->alloc(&D.phase_start, (size_t)B->rcap + nworlds);
     ok = ok && B->alloc(&D.bscr, (size_t)nb * BSCR_STRIDE);
-    ok = ok && B->alloc(&D.feat, (size_t)B->jcap * FEAT_DIM);
+    ok = ok && B->alloc(&D.feat, (size_t)B->jcap * FEAT_DIM) && B->alloc(&D.feat_row, B->jcap) &&
+         B->alloc(&D.feat_count, 2) && B->alloc(&D.joff, B->jcap);
     ok = ok && B->alloc(&D.prev, (size_t)B->jcap * JOINT_STRIDE) && B->alloc(&D.nprev, nworlds);
     ok = ok && B->alloc(&D.overflow, 4);
     ok = ok && B->alloc(&B->lambda0, B->rcap);
@@ -320,6 +331,7 @@ odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies
              odx_check(hipMemset(D.ncontacts, 0, nworlds * sizeof(int)), "hipMemset") &&
              odx_check(hipMemset(D.niters, 0, nworlds * sizeof(int)), "hipMemset") &&
              odx_check(hipMemset(D.nprev, 0, nworlds * sizeof(int)), "hipMemset") &&
+             odx_check(hipMemset(D.feat_count, 0, 2 * sizeof(int)), "hipMemset") &&
              odx_check(hipMemset(B->lambda0, 0, (size_t)B->rcap * sizeof(float)), "hipMemset") &&
              odx_check(hipMemset(D.lambda, 0, (size_t)B->rcap * sizeof(float)), "hipMemset");
     }
@@ -336,7 +348,7 @@ odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies
 void odxBatchDestroy(odxBatch *b) { delete b; }
 
 int odxBatchSetSolver(odxBatch *b, int order, int max_iters, float tol, int check_every, int warm) {
-    if (!b || max_iters < 0 || check_every <= 0 || order < 0 || order > 1) {
+    if (!b || max_iters < 0 || check_every <= 0 || order < 0 || order > 1 || warm < 0 || warm > 3) {
         odx_message(1, "odxBatchSetSolver: bad arguments");
         return 0;
     }
@@ -394,31 +406,37 @@ int odxBatchTime(odxBatch *b, int nsteps, float *total_ms) {
 int odxBatchKernelTimes(odxBatch *b, int nsteps, float *kernel_ms) {
     if (!b || nsteps <= 0 || !kernel_ms) return 0;
     if (!odxBatchSync(b)) return 0;
-    std::vector<hipEvent_t> ev(2 * (size_t)nsteps + 1, nullptr);
+    // events: before prepare, before inference, before solve, per step; one final
+    std::vector<hipEvent_t> ev(3 * (size_t)nsteps + 1, nullptr);
     bool ok = true;
     for (auto &e : ev) ok = ok && odx_check(hipEventCreate(&e), "hipEventCreate");
     for (int s = 0; ok && s < nsteps; s++) {
-        ok = ok && odx_check(hipEventRecord(ev[2 * s], b->stream), "hipEventRecord");
+        ok = ok && odx_check(hipEventRecord(ev[3 * s], b->stream), "hipEventRecord");
         launch_prepare_small(b->D, b->P, b->stream);
         ok = ok && odx_check(hipGetLastError(), "k_prepare_small launch");
-        ok = ok && odx_check(hipEventRecord(ev[2 * s + 1], b->stream), "hipEventRecord");
+        ok = ok && odx_check(hipEventRecord(ev[3 * s + 1], b->stream), "hipEventRecord");
+        ok = ok && b->enqueue_infer();
+        ok = ok && odx_check(hipEventRecord(ev[3 * s + 2], b->stream), "hipEventRecord");
         launch_solve_small(b->D, b->P, b->stream);
         ok = ok && odx_check(hipGetLastError(), "k_solve_small launch");
     }
-    ok = ok && odx_check(hipEventRecord(ev[2 * (size_t)nsteps], b->stream), "hipEventRecord");
-    ok = ok && odx_check(hipEventSynchronize(ev[2 * (size_t)nsteps]), "hipEventSynchronize");
-    double ta = 0.0, tc = 0.0;
+    ok = ok && odx_check(hipEventRecord(ev[3 * (size_t)nsteps], b->stream), "hipEventRecord");
+    ok = ok && odx_check(hipEventSynchronize(ev[3 * (size_t)nsteps]), "hipEventSynchronize");
+    double ta = 0.0, ti = 0.0, tc = 0.0;
     for (int s = 0; ok && s < nsteps; s++) {
-        float a = 0.f, c = 0.f;
-        ok = ok && odx_check(hipEventElapsedTime(&a, ev[2 * s], ev[2 * s + 1]), "hipEventElapsedTime");
-        ok = ok && odx_check(hipEventElapsedTime(&c, ev[2 * s + 1], ev[2 * s + 2]), "hipEventElapsedTime");
+        float a = 0.f, i = 0.f, c = 0.f;
+        ok = ok && odx_check(hipEventElapsedTime(&a, ev[3 * s], ev[3 * s + 1]), "hipEventElapsedTime");
+        ok = ok && odx_check(hipEventElapsedTime(&i, ev[3 * s + 1], ev[3 * s + 2]), "hipEventElapsedTime");
+        ok = ok && odx_check(hipEventElapsedTime(&c, ev[3 * s + 2], ev[3 * s + 3]), "hipEventElapsedTime");
         ta += a;
+        ti += i;
         tc += c;
     }
     for (auto &e : ev) if (e) (void)hipEventDestroy(e);
     if (!ok) return 0;
     kernel_ms[0] = (float)(ta / nsteps);
-    kernel_ms[1] = (float)(tc / nsteps);
+    kernel_ms[1] = (float)(ti / nsteps);
+    kernel_ms[2] = (float)(tc / nsteps);
     return 1;
 }
 
@@ -521,6 +539,32 @@ int odxBatchRowStarts(odxBatch *b, int *row_start) {
     if (!b || !row_start) return 0;
     std::memcpy(row_start, b->row_start.data(), (b->nw + 1) * sizeof(int));
     return b->nw + 1;
+}
+
+int odxBatchSetMlp(odxBatch *b, const odxMlp *mlp) {
+    if (!b) return 0;
+    b->mlp = mlp;
+    b->drop_graph();
+    return 1;
+}
+
+int odxBatchGetSamples(odxBatch *b, float *X, float *T, int cap) {
+    if (!b || !X || !T || cap <= 0) return 0;
+    if (!odxBatchSync(b)) return 0;
+    int cnt[2] = {0, 0};
+    if (!download(cnt, b->D.feat_count, 2)) return 0;
+    const int n = cnt[1] < cap ? cnt[1] : cap;
+    if (n <= 0) return 0;
+    std::vector<int> rows(n);
+    std::vector<float> lam(b->rcap);
+    if (!download(X, b->D.feat, (size_t)n * FEAT_DIM) || !download(rows.data(), b->D.feat_row, n) ||
+        !download(lam.data(), b->D.lambda, b->rcap))
+        return 0;
+    for (int i = 0; i < n; i++) {
+        const float s = X[(size_t)i * FEAT_DIM + 15];
+        for (int o = 0; o < 3; o++) T[(size_t)i * 3 + o] = lam[rows[i] + o] / s;
+    }
+    return n;
 }
 
 int odxBatchSetLambda0(odxBatch *b, const float *lambda0, int n) {

@@ -19,6 +19,14 @@ constexpr int SW_MAXPAIRS = 1024; // candidate pairs after broadphase
 constexpr int SW_MAXROWS = 2048;  // constraint rows
 constexpr int SW_MAXPH = 2048;    // schedule phases
 
+// Limit of the single-world (dWorldQuickStep) kernel: bodies per island set.
+constexpr int SL_MAXB = 2048;
+
+// Contacts a device collider returns per geom pair at most.
+#ifndef ODX_MAXC
+#define ODX_MAXC 4
+#endif
+
 // Row record layout (floats): J1[0..5] rhs[6] cfm[7] J2[8..13] lo[14] hi[15]
 // iMJ1[16..21] iMJ2[22..27] b1[28] b2[29] findex[30] (int bits) Ad[31].
 enum { R_J1 = 0, R_RHS = 6, R_CFM = 7, R_J2 = 8, R_LO = 14, R_HI = 15, R_IMJ1 = 16, R_IMJ2 = 22,
@@ -43,11 +51,13 @@ struct StepParams {
     int space;           // 0 SAP semantics, 1 hash semantics
     int skip_connected;  // drop contacts between bodies joined by a ball joint
     int order;           // 0 reference row order (dependency levels), 1 coloured contact blocks
-    int warm;            // 0 cold start, 1 lambda0 per row, 2 lambda0 per contact (3 values)
+    int warm;            // 0 cold start, 1 host lambda0 per row, 2 matched previous-step impulses,
+                         // 3 learned (the MLP fills lambda0 from the contact features)
     float lambda_floor;
     int final_residual;  // 1: compute the residual after the last sweep
     int collide;         // 1: generate contacts; 0: joints were uploaded
     int features;        // 1: write warm-start features per contact
+    int feedback;        // 1: write joint forces J^T lambda per joint (dJointFeedback)
 };
 
 struct DevWorlds {
@@ -65,14 +75,32 @@ struct DevWorlds {
     float *residual;        // [W]
     float *rows;            // [row cap * 32]
     float *lambda;          // [row cap]
-    const float *lambda0;   // warm start values or null
+    float *lambda0;         // [row cap] warm-start values (host- or device-written)
     int *sched;             // [row cap] work items
     int *phase_start;       // [row cap + W] per-world phase offsets (row_start[w] + w based)
     float *bscr;            // [NB * 32]
-    float *feat;            // [joint cap * 16] features (contacts only)
-    float *prev;            // [joint cap * 16] previous-step contacts (for matching)
+    int *joff;              // [joint cap] world-local first row of each joint
+    float *feat;            // [joint cap * 16] compacted warm-start features (learned mode)
+    int *feat_row;          // [joint cap] global first row of each feature entry
+    int *feat_count;        // [1] entries written this step
+    float *prev;            // [joint cap * 16] previous-step contacts: g1 g2 plocal(3) n(3) lambda(3)
     int *nprev;             // [W]
     int *overflow;          // [4] pairs / contacts / rows / phases overflow counters
+    float *fb;              // [joint cap * 12] f1 t1 f2 t2 per joint (feedback mode)
 };
+
+// Geom view for the collision entry points of the single-world API:
+// cls, body identity (-1 none), params[4], pos[3], R[12] (3x4).
+constexpr int GVIEW_STRIDE = 24;
+
+// Previous-contact record (floats) kept by the solve kernel for matching.
+enum { PV_G1 = 0, PV_G2 = 1, PV_LOCAL = 2, PV_N = 5, PV_LAM = 8 };
+constexpr float MATCH_DIST = 0.05f;   // max body-local distance (m) of a matched contact point
+constexpr float GRAVITY_SCALE = 9.81f;  // force scale of the learned warm start: s = g / (im1 + im2)
+
+// Feature vector of a contact (learned warm start), raw units:
+// 0-2 normal, 3 depth, 4 v_n, 5 v_t1, 6 v_t2 (relative velocity of the contact
+// point), 7-9 matched previous impulses / s, 10 im1, 11 im2, 12 matched flag,
+// 13 contact height over body 1, 14 two-body flag, 15 s (force scale).
 
 }  // namespace odx

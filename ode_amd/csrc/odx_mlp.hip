@@ -1,0 +1,651 @@
+// odx_mlp.hip -- learned warm start: a 16 -> 128 -> 128 -> 3 GELU MLP on bf16
+// MFMA (v_mfma_f32_16x16x32_bf16, fp32 accumulate) for gfx950.
+//
+// It maps a contact's features (normal, depth, relative normal/tangent
+// velocity, matched previous-step impulses, inverse masses, ...) to initial
+// normal/friction impulses for the SOR-LCP solve.  The reference has no such
+// component: its warm start is compiled out (quickstep.cpp:46-51) and the
+// solve starts from lambda = 0 every step (:1929-1972).  Cold-start PGS stays
+// the fallback and the judge: sweeps always continue to the same residual.
+//
+// Kernels: k_mlp_infer (forward, writes lambda0 rows), k_mlp_train (forward +
+// backward over sampled mini-batches, fp32 gradient accumulation), k_mlp_adam
+// (Adam on fp32 master weights, refreshes the bf16 copies), k_mfma_probe
+// (operand-layout check of the MFMA builtin).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/odx.h"
+#include "odx_device.h"
+#include "odx_host.h"
+
+namespace odx {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int NIN = 16, NH = 128, NOUT = 3;
+constexpr int KIN = 32;     // layer-1 K padded to one MFMA K step
+constexpr int NOUTP = 16;   // layer-3 N padded to one MFMA N tile
+
+// Operand layout of v_mfma_f32_16x16x32_bf16 (wave64), lane l, element j:
+//   A (16 x 32): row = l % 16, k = 8 * (l / 16) + j      (j < 8)
+//   B (32 x 16): col = l % 16, k = 8 * (l / 16) + j      (j < 8)
+//   D (16 x 16): col = l % 16, row = 4 * (l / 16) + j    (j < 4)
+// (checked on the device by odxMfmaProbe / tests/test_gpu_mlp.py)
+
+struct MlpDev {
+    float *W1, *b1, *W2, *b2, *W3, *b3;        // fp32 masters, [out][in]
+    __bf16 *W1h, *W2h, *W3h;                  // bf16 copies: [128][32], [128][128], [16][128]
+    float *gW1, *gb1, *gW2, *gb2, *gW3, *gb3;  // gradients
+    float *mom, *vel;                          // Adam moments over the flat parameter vector
+    float *norm;                               // in_mu[16] in_sig[16] out_mu[3] out_sig[3]
+    float *loss;                               // [2] sum of squared error, samples
+};
+
+// parameter vector layout (flat offsets)
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NH * NIN, OFF_W2 = OFF_B1 + NH, OFF_B2 = OFF_W2 + NH * NH,
+              OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NOUT * NH, NPARAM = OFF_B3 + NOUT;
+
+__device__ __forceinline__ float gelu(float x, float &dgelu) {
+    // tanh form: 0.5 x (1 + tanh(c (x + a x^3)))
+    const float c = 0.7978845608028654f, a = 0.044715f;
+    const float u = c * (x + a * x * x * x);
+    const float th = tanhf(u);
+    dgelu = 0.5f * (1.f + th) + 0.5f * x * (1.f - th * th) * c * (1.f + 3.f * a * x * x);
+    return 0.5f * x * (1.f + th);
+}
+
+__device__ __forceinline__ f32x4 mfma(const bf16x8 &a, const bf16x8 &b, const f32x4 &c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x8 load8(const __bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
+
+// Per-wave scratch of one 16-sample tile (LDS).
+struct TileBuf {
+    __bf16 act1[16][NH];
+    __bf16 act2[16][NH];
+    __bf16 g1[16][NH];   // gelu'(z1), later dZ1
+    __bf16 g2[16][NH];   // gelu'(z2), later dZ2
+    float x[16][NIN];    // normalised input
+    float dz3[16][4];
+};
+
+// Forward pass of one 16-sample tile by one wave.  xr: raw features of the
+// tile's samples (invalid samples produce garbage rows that callers ignore).
+// Returns the layer-3 accumulator (D layout, cols 0..2 are the outputs).
+__device__ f32x4 forward_tile(const MlpDev &M, const __bf16 *W1h, const __bf16 *W2h, const __bf16 *W3h,
+                              const float *b1, const float *b2, const float *b3, const float *nrm, TileBuf &tb,
+                              const float *xr, int nvalid, bool keep_grad) {
+    const int l = threadIdx.x & 63;
+    const int row = l & 15, g = l >> 4;
+    // normalised inputs
+    for (int e = l; e < 16 * NIN; e += 64) {
+        const int s = e / NIN, f = e % NIN;
+        const float v = s < nvalid ? xr[(size_t)s * NIN + f] : 0.f;
+        tb.x[s][f] = (v - nrm[f]) / nrm[NIN + f];
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    bf16x8 a1;
+#pragma unroll
+    for (int j = 0; j < 8; j++) a1[j] = g < 2 ? (__bf16)tb.x[row][8 * g + j] : (__bf16)0.f;
+    // layer 1
+#pragma unroll
+    for (int t = 0; t < NH / 16; t++) {
+        const int n = 16 * t + row;
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mfma(a1, load8(W1h + (size_t)n * KIN + 8 * g), acc);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int r = 4 * g + j;
+            float d;
+            const float h = gelu(acc[j] + b1[n], d);
+            tb.act1[r][n] = (__bf16)h;
+            if (keep_grad) tb.g1[r][n] = (__bf16)d;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // layer 2
+    bf16x8 a2[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) a2[s] = load8(&tb.act1[row][32 * s + 8 * g]);
+#pragma unroll
+    for (int t = 0; t < NH / 16; t++) {
+        const int n = 16 * t + row;
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 4; s++) acc = mfma(a2[s], load8(W2h + (size_t)n * NH + 32 * s + 8 * g), acc);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int r = 4 * g + j;
+            float d;
+            const float h = gelu(acc[j] + b2[n], d);
+            tb.act2[r][n] = (__bf16)h;
+            if (keep_grad) tb.g2[r][n] = (__bf16)d;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // layer 3 (outputs padded to 16)
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+        acc = mfma(load8(&tb.act2[row][32 * s + 8 * g]), load8(W3h + (size_t)row * NH + 32 * s + 8 * g), acc);
+    const int n = row;
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[j] += n < NOUT ? b3[n] : 0.f;
+    return acc;
+}
+
+// ------------------------------------------------------------ inference --
+__global__ __launch_bounds__(256) void k_mlp_infer(MlpDev M, const float *feat, const int *feat_row,
+                                                   const int *count, float *lambda0) {
+    __shared__ __align__(16) __bf16 sW1[NH * KIN];
+    __shared__ __align__(16) __bf16 sW2[NH * NH];
+    __shared__ __align__(16) __bf16 sW3[NOUTP * NH];
+    __shared__ float sb[NH + NH + 4 + 38];
+    __shared__ TileBuf tb[4];
+    const int tid = threadIdx.x;
+    const int n = *count;
+    if (n <= 0) return;
+    for (int i = tid; i < NH * KIN / 8; i += 256) reinterpret_cast<bf16x8 *>(sW1)[i] = reinterpret_cast<const bf16x8 *>(M.W1h)[i];
+    for (int i = tid; i < NH * NH / 8; i += 256) reinterpret_cast<bf16x8 *>(sW2)[i] = reinterpret_cast<const bf16x8 *>(M.W2h)[i];
+    for (int i = tid; i < NOUTP * NH / 8; i += 256) reinterpret_cast<bf16x8 *>(sW3)[i] = reinterpret_cast<const bf16x8 *>(M.W3h)[i];
+    for (int i = tid; i < NH; i += 256) { sb[i] = M.b1[i]; sb[NH + i] = M.b2[i]; }
+    if (tid < NOUT) sb[2 * NH + tid] = M.b3[tid];
+    if (tid < 38) sb[2 * NH + 4 + tid] = M.norm[tid];
+    __syncthreads();
+    const int wv = tid >> 6, l = tid & 63;
+    const float *nrm = sb + 2 * NH + 4;
+    const int ntiles = (n + 15) / 16;
+    for (int tile = blockIdx.x * 4 + wv; tile < ntiles; tile += gridDim.x * 4) {
+        const int s0 = tile * 16;
+        const int nv = min(16, n - s0);
+        f32x4 y = forward_tile(M, sW1, sW2, sW3, sb, sb + NH, sb + 2 * NH, nrm, tb[wv], feat + (size_t)s0 * NIN, nv, false);
+        const int o = l & 15, g = l >> 4;
+        if (o < NOUT) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int smp = s0 + 4 * g + j;
+                if (smp < n) {
+                    const float scale = feat[(size_t)smp * NIN + 15];
+                    lambda0[feat_row[smp] + o] = scale * (y[j] * nrm[2 * NIN + NOUT + o] + nrm[2 * NIN + o]);
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------- training --
+__device__ __forceinline__ unsigned hash32(unsigned x) {
+    x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+    return x;
+}
+
+// Forward + backward over `chunks` 64-sample chunks per block.  Samples are
+// X[idx], T[idx] with idx drawn from (seed, position) unless seq != 0 (then
+// samples are taken in order from 0).  Gradients of the mean squared error of
+// the normalised outputs are accumulated into M.g* with one atomic add per
+// weight per block.
+__global__ __launch_bounds__(256) void k_mlp_train(MlpDev M, const float *X, const float *T, int N, int batch,
+                                                   unsigned seed, int seq) {
+    __shared__ __align__(16) __bf16 sW1[NH * KIN];
+    __shared__ __align__(16) __bf16 sW2[NH * NH];
+    __shared__ __align__(16) __bf16 sW3[NOUTP * NH];
+    __shared__ float sb[NH + NH + 4 + 38];
+    __shared__ TileBuf tb[4];
+    const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
+    const int row = l & 15, g = l >> 4;
+    for (int i = tid; i < NH * KIN / 8; i += 256) reinterpret_cast<bf16x8 *>(sW1)[i] = reinterpret_cast<const bf16x8 *>(M.W1h)[i];
+    for (int i = tid; i < NH * NH / 8; i += 256) reinterpret_cast<bf16x8 *>(sW2)[i] = reinterpret_cast<const bf16x8 *>(M.W2h)[i];
+    for (int i = tid; i < NOUTP * NH / 8; i += 256) reinterpret_cast<bf16x8 *>(sW3)[i] = reinterpret_cast<const bf16x8 *>(M.W3h)[i];
+    for (int i = tid; i < NH; i += 256) { sb[i] = M.b1[i]; sb[NH + i] = M.b2[i]; }
+    if (tid < NOUT) sb[2 * NH + tid] = M.b3[tid];
+    if (tid < 38) sb[2 * NH + 4 + tid] = M.norm[tid];
+    __syncthreads();
+    const float *nrm = sb + 2 * NH + 4;
+    const float inv_b = 1.f / (float)batch;
+
+    // register accumulators: dW2 tiles owned by this wave (mt 0..7, nt = 2*wv, 2*wv+1)
+    f32x4 accW2[8][2];
+#pragma unroll
+    for (int a = 0; a < 8; a++) for (int b = 0; b < 2; b++) accW2[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float accW1[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // (out = tid/2, in = 8*(tid&1)+j)
+    float accW3[2] = {0.f, 0.f};                                 // pairs (o,k) = tid, tid+256 of 384
+    float accb1 = 0.f, accb2 = 0.f, accb3 = 0.f, lsum = 0.f;
+    __shared__ float sx_raw[64][NIN];
+    __shared__ float st[64][NOUT];
+
+    const int nchunks = (batch + 63) / 64;
+    for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        // gather the chunk's samples
+        for (int e = tid; e < 64 * NIN; e += 256) {
+            const int s = e / NIN, f = e % NIN;
+            const int pos = c * 64 + s;
+            float v = 0.f;
+            if (pos < batch) {
+                const unsigned id = seq ? (unsigned)pos % (unsigned)N : hash32(seed * 0x9e3779b9u + (unsigned)pos) % (unsigned)N;
+                v = X[(size_t)id * NIN + f];
+                if (f < NOUT) st[s][f] = T[(size_t)id * NOUT + f];
+            }
+            sx_raw[s][f] = v;
+        }
+        __syncthreads();
+        const int base = wv * 16;
+        const int nv = max(0, min(16, batch - (c * 64 + base)));
+        f32x4 y = forward_tile(M, sW1, sW2, sW3, sb, sb + NH, sb + 2 * NH, nrm, tb[wv], &sx_raw[base][0], nv, true);
+        // dZ3 = 2 (y - t_norm) / batch for valid samples
+        {
+            const int o = row;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int r = 4 * g + j;
+                float d = 0.f;
+                if (o < NOUT && r < nv) {
+                    const float tn = (st[base + r][o] - nrm[2 * NIN + o]) / nrm[2 * NIN + NOUT + o];
+                    const float e = y[j] - tn;
+                    lsum += e * e;
+                    d = 2.f * e * inv_b;
+                }
+                if (o < 4) tb[wv].dz3[r][o] = d;
+            }
+        }
+        __syncthreads();
+        // dW3, db3 (VALU): pairs p = (o, k)
+        for (int q = 0; q < 2; q++) {
+            const int p = tid + 256 * q;
+            if (p < NOUT * NH) {
+                const int o = p / NH, k = p % NH;
+                float s = 0.f;
+                for (int smp = 0; smp < 64; smp++) s += tb[smp >> 4].dz3[smp & 15][o] * (float)tb[smp >> 4].act2[smp & 15][k];
+                accW3[q] += s;
+            }
+        }
+        if (tid < NOUT) {
+            float s = 0.f;
+            for (int smp = 0; smp < 64; smp++) s += tb[smp >> 4].dz3[smp & 15][tid];
+            accb3 += s;
+        }
+        // dZ2 = (dZ3 W3) * gelu'(z2), stored over g2 (bf16)
+        for (int e = tid; e < 64 * NH; e += 256) {
+            const int smp = e / NH, k = e % NH;
+            TileBuf &b = tb[smp >> 4];
+            float s = 0.f;
+#pragma unroll
+            for (int o = 0; o < NOUT; o++) s += b.dz3[smp & 15][o] * M.W3[o * NH + k];
+            b.g2[smp & 15][k] = (__bf16)(s * (float)b.g2[smp & 15][k]);
+        }
+        __syncthreads();
+        // dW2 += dZ2^T act1 (MFMA, K = 64 samples in two steps)
+#pragma unroll
+        for (int mt = 0; mt < 8; mt++) {
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const int nt = 2 * wv + q;
+#pragma unroll
+                for (int ks = 0; ks < 2; ks++) {
+                    bf16x8 a, b;
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        const int smp = 32 * ks + 8 * g + j;
+                        a[j] = tb[smp >> 4].g2[smp & 15][16 * mt + row];
+                        b[j] = tb[smp >> 4].act1[smp & 15][16 * nt + row];
+                    }
+                    accW2[mt][q] = mfma(a, b, accW2[mt][q]);
+                }
+            }
+        }
+        if (tid < NH) {
+            float s = 0.f;
+            for (int smp = 0; smp < 64; smp++) s += (float)tb[smp >> 4].g2[smp & 15][tid];
+            accb2 += s;
+        }
+        // dZ1 = (dZ2 W2) * gelu'(z1) per wave (MFMA), stored over g1
+        {
+            TileBuf &b = tb[wv];
+            bf16x8 a[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) a[s] = load8(&b.g2[row][32 * s + 8 * g]);
+#pragma unroll
+            for (int t = 0; t < NH / 16; t++) {
+                const int kin = 16 * t + row;
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    bf16x8 w;
+#pragma unroll
+                    for (int j = 0; j < 8; j++) w[j] = sW2[(size_t)(32 * s + 8 * g + j) * NH + kin];
+                    acc = mfma(a[s], w, acc);
+                }
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int r = 4 * g + j;
+                    b.g1[r][kin] = (__bf16)(acc[j] * (float)b.g1[r][kin]);
+                }
+            }
+        }
+        __syncthreads();
+        // dW1 += dZ1^T x, db1 (VALU)
+        {
+            const int o = tid >> 1, i0 = 8 * (tid & 1);
+            for (int smp = 0; smp < 64; smp++) {
+                const float dz = (float)tb[smp >> 4].g1[smp & 15][o];
+#pragma unroll
+                for (int j = 0; j < 8; j++) accW1[j] += dz * (float)(__bf16)tb[smp >> 4].x[smp & 15][i0 + j];
+            }
+        }
+        if (tid < NH) {
+            float s = 0.f;
+            for (int smp = 0; smp < 64; smp++) s += (float)tb[smp >> 4].g1[smp & 15][tid];
+            accb1 += s;
+        }
+        __syncthreads();
+    }
+    // flush accumulators
+#pragma unroll
+    for (int mt = 0; mt < 8; mt++)
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int nt = 2 * wv + q;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int out = 16 * mt + 4 * g + j, in = 16 * nt + row;
+                atomicAdd(&M.gW2[out * NH + in], accW2[mt][q][j]);
+            }
+        }
+    {
+        const int o = tid >> 1, i0 = 8 * (tid & 1);
+#pragma unroll
+        for (int j = 0; j < 8; j++) atomicAdd(&M.gW1[o * NIN + i0 + j], accW1[j]);
+    }
+    for (int q = 0; q < 2; q++) {
+        const int p = tid + 256 * q;
+        if (p < NOUT * NH) atomicAdd(&M.gW3[p], accW3[q]);
+    }
+    if (tid < NH) { atomicAdd(&M.gb1[tid], accb1); atomicAdd(&M.gb2[tid], accb2); }
+    if (tid < NOUT) atomicAdd(&M.gb3[tid], accb3);
+    // loss: reduce within the wave then one atomic per wave
+    for (int o = 32; o; o >>= 1) lsum += __shfl_xor(lsum, o, 64);
+    if (l == 0) atomicAdd(&M.loss[0], lsum);
+}
+
+// Adam on the fp32 masters (flat parameter vector), then refresh the bf16
+// copies and clear the gradients.
+__global__ void k_mlp_adam(MlpDev M, float lr, float b1, float b2, float eps, float bc1, float bc2, int update) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NPARAM) return;
+    float *p, *gp;
+    int k;
+    if (i < OFF_B1) { k = i - OFF_W1; p = M.W1 + k; gp = M.gW1 + k; }
+    else if (i < OFF_W2) { k = i - OFF_B1; p = M.b1 + k; gp = M.gb1 + k; }
+    else if (i < OFF_B2) { k = i - OFF_W2; p = M.W2 + k; gp = M.gW2 + k; }
+    else if (i < OFF_W3) { k = i - OFF_B2; p = M.b2 + k; gp = M.gb2 + k; }
+    else if (i < OFF_B3) { k = i - OFF_W3; p = M.W3 + k; gp = M.gW3 + k; }
+    else { k = i - OFF_B3; p = M.b3 + k; gp = M.gb3 + k; }
+    if (update) {
+        const float gr = *gp;
+        const float m = b1 * M.mom[i] + (1.f - b1) * gr;
+        const float v = b2 * M.vel[i] + (1.f - b2) * gr * gr;
+        M.mom[i] = m;
+        M.vel[i] = v;
+        *p -= lr * (m / bc1) / (sqrtf(v / bc2) + eps);
+        *gp = 0.f;
+    }
+    // bf16 copies
+    if (i < OFF_B1) {
+        const int o = k / NIN, in = k % NIN;
+        M.W1h[o * KIN + in] = (__bf16)*p;
+        M.W1h[o * KIN + NIN + in] = (__bf16)0.f;
+    } else if (i >= OFF_W2 && i < OFF_B2) {
+        M.W2h[k] = (__bf16)*p;
+    } else if (i >= OFF_W3 && i < OFF_B3) {
+        M.W3h[k] = (__bf16)*p;
+    }
+}
+
+// ------------------------------------------------------------ MFMA probe --
+__global__ void k_mfma_probe(const float *a, const float *b, float *d) {
+    const int l = threadIdx.x;
+    bf16x8 A, B;
+    for (int j = 0; j < 8; j++) { A[j] = (__bf16)a[l * 8 + j]; B[j] = (__bf16)b[l * 8 + j]; }
+    f32x4 c = {0.f, 0.f, 0.f, 0.f};
+    c = mfma(A, B, c);
+    for (int j = 0; j < 4; j++) d[l * 4 + j] = c[j];
+}
+
+}  // namespace odx
+
+using namespace odx;
+
+struct odxMlp {
+    MlpDev M{};
+    std::vector<void *> allocs;
+    hipStream_t stream = nullptr;
+    float *X = nullptr, *T = nullptr;
+    int N = 0, step = 0;
+    float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    template <class Tp> bool alloc(Tp **p, size_t n) {
+        *p = nullptr;
+        if (!odx_check(hipMalloc((void **)p, (n ? n : 1) * sizeof(Tp)), "hipMalloc")) return false;
+        allocs.push_back((void *)*p);
+        return true;
+    }
+    ~odxMlp() {
+        if (stream) (void)hipStreamDestroy(stream);
+        for (void *p : allocs) (void)hipFree(p);
+    }
+};
+
+namespace odx {
+// used by the batch runtime to run inference inside the step graph
+void launch_mlp_infer(const odxMlp *m, const float *feat, const int *feat_row, const int *count, float *lambda0,
+                      int max_entries, hipStream_t s) {
+    int tiles = (max_entries + 15) / 16;
+    int blocks = (tiles + 3) / 4;
+    if (blocks > 1024) blocks = 1024;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, s, m->M, feat, feat_row, count, lambda0);
+}
+}  // namespace odx
+
+static void mlp_refresh_copies(odxMlp *m) {
+    hipLaunchKernelGGL(k_mlp_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, m->stream, m->M, 0.f, 0.f, 0.f, 1.f, 1.f,
+                       1.f, 0);
+}
+
+extern "C" {
+
+int odxMfmaProbe(const float *a, const float *b, float *d) {
+    float *da = nullptr, *db = nullptr, *dd = nullptr;
+    bool ok = odx_check(hipMalloc((void **)&da, 512 * sizeof(float)), "hipMalloc") &&
+              odx_check(hipMalloc((void **)&db, 512 * sizeof(float)), "hipMalloc") &&
+              odx_check(hipMalloc((void **)&dd, 256 * sizeof(float)), "hipMalloc");
+    ok = ok && odx_check(hipMemcpy(da, a, 512 * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy") &&
+         odx_check(hipMemcpy(db, b, 512 * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+    if (ok) {
+        hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, 0, da, db, dd);
+        ok = odx_check(hipGetLastError(), "k_mfma_probe") &&
+             odx_check(hipMemcpy(d, dd, 256 * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+    }
+    if (da) (void)hipFree(da);
+    if (db) (void)hipFree(db);
+    if (dd) (void)hipFree(dd);
+    return ok ? 1 : 0;
+}
+
+odxMlp *odxMlpCreate(unsigned seed) {
+    auto *m = new odxMlp();
+    MlpDev &M = m->M;
+    bool ok = m->alloc(&M.W1, NH * NIN) && m->alloc(&M.b1, NH) && m->alloc(&M.W2, NH * NH) && m->alloc(&M.b2, NH) &&
+              m->alloc(&M.W3, NOUT * NH) && m->alloc(&M.b3, NOUT) && m->alloc(&M.W1h, NH * KIN) &&
+              m->alloc(&M.W2h, NH * NH) && m->alloc(&M.W3h, NOUTP * NH) && m->alloc(&M.gW1, NH * NIN) &&
+              m->alloc(&M.gb1, NH) && m->alloc(&M.gW2, NH * NH) && m->alloc(&M.gb2, NH) && m->alloc(&M.gW3, NOUT * NH) &&
+              m->alloc(&M.gb3, NOUT) && m->alloc(&M.mom, NPARAM) && m->alloc(&M.vel, NPARAM) && m->alloc(&M.norm, 38) &&
+              m->alloc(&M.loss, 2) && odx_check(hipStreamCreate(&m->stream), "hipStreamCreate");
+    if (!ok) { delete m; return nullptr; }
+    // He-style uniform init from a fixed LCG (host), zero biases
+    std::vector<float> P(NPARAM, 0.f);
+    unsigned s = seed ? seed : 1u;
+    auto rnd = [&]() { s = 1664525u * s + 1013904223u; return ((s >> 8) * (1.0f / 16777216.0f)) * 2.f - 1.f; };
+    for (int i = 0; i < NH * NIN; i++) P[OFF_W1 + i] = rnd() * std::sqrt(6.f / NIN);
+    for (int i = 0; i < NH * NH; i++) P[OFF_W2 + i] = rnd() * std::sqrt(6.f / NH);
+    for (int i = 0; i < NOUT * NH; i++) P[OFF_W3 + i] = rnd() * std::sqrt(1.f / NH);
+    std::vector<float> norm(38, 0.f);
+    for (int i = 0; i < NIN; i++) norm[NIN + i] = 1.f;
+    for (int i = 0; i < NOUT; i++) norm[2 * NIN + NOUT + i] = 1.f;
+    ok = odx_check(hipMemcpy(M.W1, &P[OFF_W1], NH * NIN * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+         odx_check(hipMemcpy(M.b1, &P[OFF_B1], NH * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+         odx_check(hipMemcpy(M.W2, &P[OFF_W2], NH * NH * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+         odx_check(hipMemcpy(M.b2, &P[OFF_B2], NH * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+         odx_check(hipMemcpy(M.W3, &P[OFF_W3], NOUT * NH * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+         odx_check(hipMemcpy(M.b3, &P[OFF_B3], NOUT * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+         odx_check(hipMemcpy(M.norm, norm.data(), 38 * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+         odx_check(hipMemset(M.W3h, 0, NOUTP * NH * sizeof(__bf16)), "hipMemset") &&
+         odx_check(hipMemset(M.gW1, 0, NH * NIN * 4), "hipMemset") && odx_check(hipMemset(M.gb1, 0, NH * 4), "hipMemset") &&
+         odx_check(hipMemset(M.gW2, 0, NH * NH * 4), "hipMemset") && odx_check(hipMemset(M.gb2, 0, NH * 4), "hipMemset") &&
+         odx_check(hipMemset(M.gW3, 0, NOUT * NH * 4), "hipMemset") &&
+         odx_check(hipMemset(M.gb3, 0, NOUT * 4), "hipMemset") &&
+         odx_check(hipMemset(M.mom, 0, NPARAM * 4), "hipMemset") &&
+         odx_check(hipMemset(M.vel, 0, NPARAM * 4), "hipMemset") && odx_check(hipMemset(M.loss, 0, 8), "hipMemset");
+    if (!ok) { delete m; return nullptr; }
+    mlp_refresh_copies(m);
+    if (!odx_check(hipStreamSynchronize(m->stream), "hipStreamSynchronize")) { delete m; return nullptr; }
+    return m;
+}
+
+void odxMlpDestroy(odxMlp *m) { delete m; }
+
+int odxMlpSetNormalization(odxMlp *m, const float *norm38) {
+    if (!m || !norm38) return 0;
+    for (int i = 0; i < NIN; i++)
+        if (!(norm38[NIN + i] > 0.f)) { odx_message(1, "odxMlpSetNormalization: input scale %d must be > 0", i); return 0; }
+    for (int i = 0; i < NOUT; i++)
+        if (!(norm38[2 * NIN + NOUT + i] > 0.f)) { odx_message(1, "odxMlpSetNormalization: output scale %d must be > 0", i); return 0; }
+    return odx_check(hipMemcpy(m->M.norm, norm38, 38 * 4, hipMemcpyHostToDevice), "hipMemcpy") ? 1 : 0;
+}
+
+int odxMlpGetParams(odxMlp *m, float *params) {
+    if (!m || !params) return 0;
+    const MlpDev &M = m->M;
+    bool ok = odx_check(hipStreamSynchronize(m->stream), "hipStreamSynchronize") &&
+              odx_check(hipMemcpy(params + OFF_W1, M.W1, NH * NIN * 4, hipMemcpyDeviceToHost), "hipMemcpy") &&
+              odx_check(hipMemcpy(params + OFF_B1, M.b1, NH * 4, hipMemcpyDeviceToHost), "hipMemcpy") &&
+              odx_check(hipMemcpy(params + OFF_W2, M.W2, NH * NH * 4, hipMemcpyDeviceToHost), "hipMemcpy") &&
+              odx_check(hipMemcpy(params + OFF_B2, M.b2, NH * 4, hipMemcpyDeviceToHost), "hipMemcpy") &&
+              odx_check(hipMemcpy(params + OFF_W3, M.W3, NOUT * NH * 4, hipMemcpyDeviceToHost), "hipMemcpy") &&
+              odx_check(hipMemcpy(params + OFF_B3, M.b3, NOUT * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    return ok ? NPARAM : 0;
+}
+
+int odxMlpSetParams(odxMlp *m, const float *params) {
+    if (!m || !params) return 0;
+    MlpDev &M = m->M;
+    bool ok = odx_check(hipMemcpy(M.W1, params + OFF_W1, NH * NIN * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+              odx_check(hipMemcpy(M.b1, params + OFF_B1, NH * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+              odx_check(hipMemcpy(M.W2, params + OFF_W2, NH * NH * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+              odx_check(hipMemcpy(M.b2, params + OFF_B2, NH * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+              odx_check(hipMemcpy(M.W3, params + OFF_W3, NOUT * NH * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+              odx_check(hipMemcpy(M.b3, params + OFF_B3, NOUT * 4, hipMemcpyHostToDevice), "hipMemcpy");
+    if (!ok) return 0;
+    mlp_refresh_copies(m);
+    return odx_check(hipStreamSynchronize(m->stream), "hipStreamSynchronize") ? NPARAM : 0;
+}
+
+int odxMlpSetData(odxMlp *m, const float *X, const float *T, int N) {
+    if (!m || !X || !T || N <= 0) return 0;
+    if (N > m->N) {
+        if (!m->alloc(&m->X, (size_t)N * NIN) || !m->alloc(&m->T, (size_t)N * NOUT)) return 0;
+    }
+    m->N = N;
+    bool ok = odx_check(hipMemcpy(m->X, X, (size_t)N * NIN * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+              odx_check(hipMemcpy(m->T, T, (size_t)N * NOUT * 4, hipMemcpyHostToDevice), "hipMemcpy");
+    return ok ? N : 0;
+}
+
+// Gradient of the loss over samples [0, N) of the uploaded data (no update).
+// grads: NPARAM floats in the parameter layout; loss: mean squared error.
+int odxMlpGradients(odxMlp *m, float *grads, float *loss) {
+    if (!m || m->N <= 0 || !grads) return 0;
+    MlpDev &M = m->M;
+    (void)hipMemsetAsync(M.loss, 0, 8, m->stream);
+    int chunks = (m->N + 63) / 64;
+    int blocks = chunks < 256 ? chunks : 256;
+    hipLaunchKernelGGL(k_mlp_train, dim3(blocks), dim3(256), 0, m->stream, M, m->X, m->T, m->N, m->N, 0u, 1);
+    if (!odx_check(hipGetLastError(), "k_mlp_train")) return 0;
+    bool ok = odx_check(hipStreamSynchronize(m->stream), "hipStreamSynchronize") &&
+              odx_check(hipMemcpy(grads + OFF_W1, M.gW1, NH * NIN * 4, hipMemcpyDeviceToHost), "hipMemcpy") &&
+              odx_check(hipMemcpy(grads + OFF_B1, M.gb1, NH * 4, hipMemcpyDeviceToHost), "hipMemcpy") &&
+              odx_check(hipMemcpy(grads + OFF_W2, M.gW2, NH * NH * 4, hipMemcpyDeviceToHost), "hipMemcpy") &&
+              odx_check(hipMemcpy(grads + OFF_B2, M.gb2, NH * 4, hipMemcpyDeviceToHost), "hipMemcpy") &&
+              odx_check(hipMemcpy(grads + OFF_W3, M.gW3, NOUT * NH * 4, hipMemcpyDeviceToHost), "hipMemcpy") &&
+              odx_check(hipMemcpy(grads + OFF_B3, M.gb3, NOUT * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    float l[2] = {0.f, 0.f};
+    ok = ok && odx_check(hipMemcpy(l, M.loss, 8, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (loss) *loss = l[0] / (float)m->N;
+    // clear the gradients for later training
+    ok = ok && odx_check(hipMemset(M.gW1, 0, NH * NIN * 4), "hipMemset") && odx_check(hipMemset(M.gb1, 0, NH * 4), "hipMemset") &&
+         odx_check(hipMemset(M.gW2, 0, NH * NH * 4), "hipMemset") && odx_check(hipMemset(M.gb2, 0, NH * 4), "hipMemset") &&
+         odx_check(hipMemset(M.gW3, 0, NOUT * NH * 4), "hipMemset") && odx_check(hipMemset(M.gb3, 0, NOUT * 4), "hipMemset");
+    return ok ? NPARAM : 0;
+}
+
+// nsteps Adam steps on mini-batches of `batch` samples drawn from the uploaded
+// data; losses[i] receives each step's mini-batch mean squared error (may be NULL).
+int odxMlpTrain(odxMlp *m, int nsteps, int batch, float lr, unsigned seed, float *losses) {
+    if (!m || m->N <= 0 || nsteps <= 0 || batch <= 0) return 0;
+    MlpDev &M = m->M;
+    int chunks = (batch + 63) / 64;
+    int blocks = chunks < 256 ? chunks : 256;
+    std::vector<float> l(2);
+    for (int s = 0; s < nsteps; s++) {
+        if (losses) (void)hipMemsetAsync(M.loss, 0, 8, m->stream);
+        hipLaunchKernelGGL(k_mlp_train, dim3(blocks), dim3(256), 0, m->stream, M, m->X, m->T, m->N, batch,
+                           seed + (unsigned)m->step * 7919u, 0);
+        m->step++;
+        const float bc1 = 1.f - std::pow(m->b1, (float)m->step), bc2 = 1.f - std::pow(m->b2, (float)m->step);
+        hipLaunchKernelGGL(k_mlp_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, m->stream, M, lr, m->b1, m->b2, m->eps,
+                           bc1, bc2, 1);
+        if (!odx_check(hipGetLastError(), "mlp train launch")) return 0;
+        if (losses) {
+            if (!odx_check(hipMemcpyAsync(l.data(), M.loss, 8, hipMemcpyDeviceToHost, m->stream), "hipMemcpyAsync") ||
+                !odx_check(hipStreamSynchronize(m->stream), "hipStreamSynchronize"))
+                return 0;
+            losses[s] = l[0] / (float)batch;
+        }
+    }
+    return odx_check(hipStreamSynchronize(m->stream), "hipStreamSynchronize") ? nsteps : 0;
+}
+
+// Forward pass on host features X [N][16] -> lambda [N][3] (force units).
+int odxMlpPredict(odxMlp *m, const float *X, int N, float *Y) {
+    if (!m || !X || !Y || N <= 0) return 0;
+    float *dX = nullptr, *dY = nullptr;
+    int *drow = nullptr, *dcnt = nullptr;
+    bool ok = odx_check(hipMalloc((void **)&dX, (size_t)N * NIN * 4), "hipMalloc") &&
+              odx_check(hipMalloc((void **)&dY, (size_t)N * NOUT * 4), "hipMalloc") &&
+              odx_check(hipMalloc((void **)&drow, (size_t)N * 4), "hipMalloc") &&
+              odx_check(hipMalloc((void **)&dcnt, 4), "hipMalloc");
+    std::vector<int> rows(N);
+    for (int i = 0; i < N; i++) rows[i] = NOUT * i;
+    ok = ok && odx_check(hipMemcpy(dX, X, (size_t)N * NIN * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+         odx_check(hipMemcpy(drow, rows.data(), (size_t)N * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+         odx_check(hipMemcpy(dcnt, &N, 4, hipMemcpyHostToDevice), "hipMemcpy");
+    if (ok) {
+        launch_mlp_infer(m, dX, drow, dcnt, dY, N, m->stream);
+        ok = odx_check(hipGetLastError(), "k_mlp_infer") && odx_check(hipStreamSynchronize(m->stream), "sync") &&
+             odx_check(hipMemcpy(Y, dY, (size_t)N * NOUT * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    }
+    if (dX) (void)hipFree(dX);
+    if (dY) (void)hipFree(dY);
+    if (drow) (void)hipFree(drow);
+    if (dcnt) (void)hipFree(dcnt);
+    return ok ? N : 0;
+}
+
+}  // extern "C"

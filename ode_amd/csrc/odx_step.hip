@@ -372,7 +372,54 @@ __device__ int build_joint_rows(const DevWorlds &D, const StepParams &P, const f
 }
 
 // Warm-start feature vector of a contact (its first row at world-local ri).
-__device__ void contact_features(const DevWorlds &D, const float *rec, int b0, float *out) {
+// Nearest previous-step contact of the same geom pair, compared in body-1
+// coordinates (contacts are rebuilt every step; the reference keeps no contact
+// identity, quickstep.cpp:46-51).  The previous friction impulse is carried
+// over as a tangential force vector and re-expressed in the new tangent basis.
+// Returns 1 and lam[3] = (normal, t1, t2) when a match is found.
+__device__ int match_previous(const DevWorlds &D, int j0, int nprev, const float *rec, int b0, float *lam) {
+    lam[0] = lam[1] = lam[2] = 0.f;
+    if (nprev <= 0) return 0;
+    const int g1 = (int)rec[14], g2 = (int)rec[15];
+    const unsigned key = ((unsigned)g1 << 16) | (unsigned)g2;
+    const float *pv = D.prev + (size_t)j0 * JOINT_STRIDE;
+    int lo = 0, hi = nprev;
+    while (lo < hi) {  // first record with pair key >= key (records are in pair order)
+        const int mid = (lo + hi) >> 1;
+        const float *r = pv + (size_t)mid * JOINT_STRIDE;
+        const unsigned km = ((unsigned)(int)r[PV_G1] << 16) | (unsigned)(int)r[PV_G2];
+        if (km < key) lo = mid + 1; else hi = mid;
+    }
+    const int b1 = (int)rec[1];
+    const float *s1 = D.bscr + (size_t)(b0 + b1) * BSCR_STRIDE;
+    float4 x1 = D.pos[b0 + b1];
+    float d[3] = {rec[3] - x1.x, rec[4] - x1.y, rec[5] - x1.z}, pl[3];
+    mul1_331(pl, s1, d);
+    int best = -1;
+    float bestd = MATCH_DIST * MATCH_DIST;
+    for (int i = lo; i < nprev && i < lo + 8; i++) {
+        const float *r = pv + (size_t)i * JOINT_STRIDE;
+        if ((int)r[PV_G1] != g1 || (int)r[PV_G2] != g2) break;
+        const float dx = r[PV_LOCAL] - pl[0], dy = r[PV_LOCAL + 1] - pl[1], dz = r[PV_LOCAL + 2] - pl[2];
+        const float dd = dx * dx + dy * dy + dz * dz;
+        if (dd <= bestd) { bestd = dd; best = i; }
+    }
+    if (best < 0) return 0;
+    const float *r = pv + (size_t)best * JOINT_STRIDE;
+    float no[3] = {r[PV_N], r[PV_N + 1], r[PV_N + 2]}, to1[3], to2[3];
+    planeSpace(no, to1, to2);
+    float f[3];
+    for (int k = 0; k < 3; k++) f[k] = r[PV_LAM + 1] * to1[k] + r[PV_LAM + 2] * to2[k];
+    float n[3] = {rec[6], rec[7], rec[8]}, t1[3], t2[3];
+    planeSpace(n, t1, t2);
+    lam[0] = r[PV_LAM];
+    lam[1] = dot3(f, t1);
+    lam[2] = dot3(f, t2);
+    return 1;
+}
+
+__device__ void contact_features(const DevWorlds &D, const float *rec, int b0, const float *lam_prev, int matched,
+                                 float *out) {
     const int b1 = (int)rec[1], b2 = (int)rec[2];
     float n[3] = {rec[6], rec[7], rec[8]}, t1[3], t2[3];
     planeSpace(n, t1, t2);
@@ -391,15 +438,32 @@ __device__ void contact_features(const DevWorlds &D, const float *rec, int b0, f
         im2 = D.bscr[(size_t)(b0 + b2) * BSCR_STRIDE + 30];
     }
     const float im1 = D.bscr[(size_t)(b0 + b1) * BSCR_STRIDE + 30];
+    const float s = GRAVITY_SCALE / (im1 + im2);
+    const float si = 1.f / s;
     out[0] = n[0]; out[1] = n[1]; out[2] = n[2];
     out[3] = rec[9];
     out[4] = dot3(vr, n); out[5] = dot3(vr, t1); out[6] = dot3(vr, t2);
-    out[7] = 0.f; out[8] = 0.f; out[9] = 0.f;   // previous impulses, filled by matching
+    out[7] = lam_prev[0] * si; out[8] = lam_prev[1] * si; out[9] = lam_prev[2] * si;
     out[10] = im1; out[11] = im2;
-    out[12] = 0.f;                               // matched flag
-    out[13] = rec[5] - p1.z;                     // contact height relative to body 1
+    out[12] = (float)matched;
+    out[13] = rec[5] - p1.z;
     out[14] = (float)(b2 >= 0);
-    out[15] = 1.f;
+    out[15] = s;
+}
+
+// Warm-start value of each row of a contact from (normal, t1, t2) impulses,
+// following the row order of contact_info2.
+__device__ void write_contact_lambda0(const DevWorlds &D, const float *rec, int r0, int ri, const float *lam) {
+    const Surface s = D.surf[(int)rec[13]];
+    const int m = contact_rows_m(s);
+    float *o = D.lambda0 + r0 + ri;
+    o[0] = lam[0];
+    int row = 1;
+    if (m > 1) {
+        if (s.mu > 0.f) o[row++] = lam[1];
+        const float mu2 = (s.mode & 0x001) ? s.mu2 : s.mu;
+        if (mu2 > 0.f && row < m) o[row++] = lam[2];
+    }
 }
 
 // ============================================================== kernel A ===
@@ -554,10 +618,38 @@ __global__ __launch_bounds__(64) void k_prepare_small(DevWorlds D, StepParams P)
                 } else {
                     s_rfl[ri] = 0; s_rfl[ri + 1] = 0; s_rfl[ri + 2] = 0;
                 }
-                if (P.features && (int)rec[0] == 0)
-                    contact_features(D, rec, b0, D.feat + (size_t)(j0 + k) * FEAT_DIM);
-            } else if (k < SW_MAXROWS) {
-                s_jm[k] = 0;
+                D.joff[j0 + k] = ri;
+            } else {
+                if (k < SW_MAXROWS) s_jm[k] = 0;
+                D.joff[j0 + k] = 1 << 30;  // no rows (capacity exceeded)
+            }
+        }
+        // warm start: matched previous impulses (mode 2) or features for the
+        // learned predictor (mode 3); ball joints keep last step's impulses
+        if (P.warm >= 2) {
+            const bool fits = k < njoint && ri + mj <= rcap && k < SW_MAXROWS;
+            const bool is_contact = fits && (int)rec[0] == 0;
+            float lp[3] = {0.f, 0.f, 0.f};
+            int matched = 0;
+            if (is_contact && P.collide) matched = match_previous(D, j0, D.nprev[w], rec, b0, lp);
+            if (fits && !is_contact)
+                for (int r = 0; r < mj; r++) D.lambda0[r0 + ri + r] = D.lambda[r0 + ri + r];
+            if (is_contact && P.warm == 2) write_contact_lambda0(D, rec, r0, ri, lp);
+            if (P.warm == 3) {
+                // the network predicts (normal, t1, t2): contacts with exactly those three rows
+                const bool learn = is_contact && mj == 3;
+                if (is_contact && !learn) write_contact_lambda0(D, rec, r0, ri, lp);
+                const unsigned long long bal = __ballot(learn);
+                int base = 0;
+                if (t == 0 && bal) base = atomicAdd(D.feat_count, __popcll(bal));
+                base = __shfl(base, 0, 64);
+                if (learn) {
+                    const int slot = base + __popcll(bal & ((1ull << t) - 1ull));
+                    contact_features(D, rec, b0, lp, matched, D.feat + (size_t)slot * FEAT_DIM);
+                    D.feat_row[slot] = r0 + ri;
+                    // rows beyond the predicted three (none for this surface) start at zero
+                    write_contact_lambda0(D, rec, r0, ri, lp);
+                }
             }
         }
         mrow += wave_bcast_last(incl);
@@ -751,7 +843,11 @@ __global__ __launch_bounds__(64) void k_solve_small(DevWorlds D, StepParams P) {
     float *lam = D.lambda + r0;
 
     for (int k = t; k < nb * 6; k += 64) s_fc[k] = 0.f;
-    if (P.warm == 1 && D.lambda0) {
+    if (w == 0 && t == 0 && D.feat_count) {  // feature entries were consumed; keep the count for readback
+        D.feat_count[1] = D.feat_count[0];
+        D.feat_count[0] = 0;
+    }
+    if (P.warm >= 1 && D.lambda0) {
         // lambda0 clamped to the row bounds; friction bounds use the clamped normal
         for (int pass = 0; pass < 2; pass++) {
             for (int r = t; r < m; r += 64) {
@@ -807,6 +903,46 @@ __global__ __launch_bounds__(64) void k_solve_small(DevWorlds D, StepParams P) {
     }
     if (t == 0) { D.niters[w] = it; D.residual[w] = res; }
 
+    // keep this step's contacts and impulses for next step's matching
+    // (body-1 frame at the pose the contacts were generated for)
+    if (P.warm >= 2 && P.collide) {
+        const int j0 = D.joint_start[w];
+        const int nball = D.ball_start[w + 1] - D.ball_start[w];
+        const int nj = D.njoints[w];
+        int nkeep = 0;
+        for (int k = nball + t; k < nj; k += 64) {
+            const float *rec = D.joints + (size_t)(j0 + k) * JOINT_STRIDE;
+            const int ri = D.joff[j0 + k];
+            float *o = D.prev + (size_t)(j0 + k - nball) * JOINT_STRIDE;
+            o[PV_G1] = rec[14]; o[PV_G2] = rec[15];
+            if (ri >= m) {  // a contact that got no rows: keeps the pair order, never matches
+                o[PV_LOCAL] = o[PV_LOCAL + 1] = o[PV_LOCAL + 2] = 1e30f;
+                o[PV_LAM] = o[PV_LAM + 1] = o[PV_LAM + 2] = 0.f;
+                continue;
+            }
+            const Surface s = D.surf[(int)rec[13]];
+            const int mj = contact_rows_m(s);
+            float lmb[3] = {lam[ri], 0.f, 0.f};
+            int row = 1;
+            if (mj > 1) {
+                if (s.mu > 0.f) lmb[1] = lam[ri + row++];
+                const float mu2 = (s.mode & 0x001) ? s.mu2 : s.mu;
+                if (mu2 > 0.f && row < mj) lmb[2] = lam[ri + row++];
+            }
+            const int b1 = (int)rec[1];
+            const float *s1 = D.bscr + (size_t)(b0 + b1) * BSCR_STRIDE;
+            float4 x1 = D.pos[b0 + b1];
+            float d[3] = {rec[3] - x1.x, rec[4] - x1.y, rec[5] - x1.z}, pl[3];
+            mul1_331(pl, s1, d);
+            o[PV_LOCAL] = pl[0]; o[PV_LOCAL + 1] = pl[1]; o[PV_LOCAL + 2] = pl[2];
+            o[PV_N] = rec[6]; o[PV_N + 1] = rec[7]; o[PV_N + 2] = rec[8];
+            o[PV_LAM] = lmb[0]; o[PV_LAM + 1] = lmb[1]; o[PV_LAM + 2] = lmb[2];
+        }
+        (void)nkeep;
+        if (t == 0) D.nprev[w] = nj - nball;
+        __syncthreads();
+    }
+
     // Stage4b, Stage6a, Stage6b + dxStepBody (infinitesimal rotation)
     const float h = P.h;
     for (int i = t; i < nb; i += 64) {
@@ -844,6 +980,209 @@ __global__ __launch_bounds__(64) void k_solve_small(DevWorlds D, StepParams P) {
 
 }  // namespace odx
 
+namespace odx {
+
+// ===================================================== single large world ===
+// dWorldQuickStep of one world of up to SL_MAXB bodies: one workgroup of 1024
+// threads.  The host builds the joint list in island order (the reference's
+// DFS, util.cpp:724-855), each joint's first row (joff) and the sweep schedule;
+// the kernel runs Stage0 bodies, the rows of every joint, the SOR sweeps with
+// fc in LDS, the optional residual, joint feedback (Stage4b,
+// quickstep.cpp:3105-3150) and the integration.
+constexpr int SL_BLOCK = 1024;
+static_assert(SL_MAXB * 6 * sizeof(float) <= 65536, "fc of the single-world kernel must fit 64 KB of LDS");
+
+__device__ float block_max(float v, float *s_red) {
+    v = wave_max(v);
+    const int wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane_id() == 0) s_red[wv] = v;
+    __syncthreads();
+    float r = 0.f;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) r = fmaxf(r, s_red[i]);
+    return r;
+}
+
+__global__ __launch_bounds__(SL_BLOCK) void k_single_step(DevWorlds D, StepParams P) {
+    extern __shared__ float s_fc[];  // nb * 6
+    __shared__ float s_red[32];
+    const int t = threadIdx.x;
+    const int nb = D.body_start[1];
+    const int nj = D.njoints[0];
+    const int m = D.nrows[0];
+    const int L = D.nphases[0];
+    float *lam = D.lambda;
+    for (int i = t; i < nb; i += SL_BLOCK) body_prep(D, P, i);
+    for (int k = t; k < nb * 6; k += SL_BLOCK) s_fc[k] = 0.f;
+    __syncthreads();
+    for (int j = t; j < nj; j += SL_BLOCK) build_joint_rows(D, P, D.joints + (size_t)j * JOINT_STRIDE, 0, 0, D.joff[j]);
+    __syncthreads();
+    for (int r = t; r < m; r += SL_BLOCK) lam[r] = 0.f;
+    __syncthreads();
+    int it = 0;
+    float res = -1.f;
+    if (m > 0) {
+        while (it < P.max_iters) {
+            for (int p = 0; p < L; p++) {
+                const int s = D.phase_start[p], e = D.phase_start[p + 1];
+                for (int k = s + t; k < e; k += SL_BLOCK) process_item(D, 0, D.sched[k], s_fc);
+                __syncthreads();
+            }
+            it++;
+            if (P.tol > 0.f && (it % P.check_every == 0 || it == P.max_iters)) {
+                // residual over all rows (block-wide)
+                float mx = 0.f, lmax = 0.f;
+                const float iw = 1.f / P.sor_w;
+                for (int r = t; r < m; r += SL_BLOCK) {
+                    const float4 *rr = reinterpret_cast<const float4 *>(D.rows + (size_t)r * ROW_STRIDE);
+                    const float4 a0 = rr[0], a1 = rr[1], a2 = rr[2], a3 = rr[3], a7 = rr[7];
+                    const int b1 = as_int(a7.x), b2 = as_int(a7.y), fidx = as_int(a7.z);
+                    const float *f1 = s_fc + b1 * 6;
+                    const float l = lam[r];
+                    float d = a1.z - l * a1.w;
+                    d -= f1[0] * a0.x + f1[1] * a0.y + f1[2] * a0.z + f1[3] * a0.w + f1[4] * a1.x + f1[5] * a1.y;
+                    if (b2 >= 0) {
+                        const float *f2 = s_fc + b2 * 6;
+                        d -= f2[0] * a2.x + f2[1] * a2.y + f2[2] * a2.z + f2[3] * a2.w + f2[4] * a3.x + f2[5] * a3.y;
+                    }
+                    d *= iw;
+                    float lo = a3.z, hi = a3.w;
+                    if (fidx >= 0) { hi = fabsf(hi * lam[fidx]); lo = -hi; }
+                    const float nl = fminf(fmaxf(l + d, lo), hi);
+                    mx = fmaxf(mx, fabsf(nl - l));
+                    lmax = fmaxf(lmax, fabsf(l));
+                }
+                mx = block_max(mx, s_red);
+                lmax = block_max(lmax, s_red);
+                res = mx / fmaxf(lmax, P.lambda_floor);
+                __syncthreads();
+                if (res <= P.tol) break;
+            }
+        }
+    } else {
+        res = -1.f;
+    }
+    if (t == 0) { D.niters[0] = it; D.residual[0] = res; }
+    // joint feedback: f = J^T lambda with the unscaled J (J_row / Ad)
+    if (P.feedback) {
+        for (int j = t; j < nj; j += SL_BLOCK) {
+            const float *rec = D.joints + (size_t)j * JOINT_STRIDE;
+            const int ri = D.joff[j];
+            const int mj = (int)rec[0] == 0 ? contact_rows_m(D.surf[(int)rec[13]]) : 3;
+            float f[12] = {0.f};
+            for (int q = 0; q < mj; q++) {
+                const float *rw = D.rows + (size_t)(ri + q) * ROW_STRIDE;
+                const float s = lam[ri + q] / rw[R_AD];
+                for (int k = 0; k < 6; k++) { f[k] += rw[R_J1 + k] * s; f[6 + k] += rw[R_J2 + k] * s; }
+            }
+            for (int k = 0; k < 12; k++) D.fb[(size_t)j * 12 + k] = f[k];
+        }
+    }
+    __syncthreads();
+    const float h = P.h;
+    for (int i = t; i < nb; i += SL_BLOCK) {
+        const float *s = D.bscr + (size_t)i * BSCR_STRIDE;
+        float4 p4 = D.pos[i], q4 = D.quat[i], lv = D.lvel[i], av = D.avel[i], fa = D.facc[i], ta = D.tacc[i];
+        const float *fc = s_fc + i * 6;
+        if (m > 0) {
+            lv.x += h * fc[0]; lv.y += h * fc[1]; lv.z += h * fc[2];
+            av.x += h * fc[3]; av.y += h * fc[4]; av.z += h * fc[5];
+        }
+        const float ims = h * s[30];
+        lv.x += ims * fa.x; lv.y += ims * fa.y; lv.z += ims * fa.z;
+        float tq[3] = {ta.x * h, ta.y * h, ta.z * h}, dw[3];
+        mul0_331(dw, s + 12, tq);
+        av.x += dw[0]; av.y += dw[1]; av.z += dw[2];
+        p4.x += h * lv.x; p4.y += h * lv.y; p4.z += h * lv.z;
+        float dq0 = 0.5f * (-av.x * q4.y - av.y * q4.z - av.z * q4.w);
+        float dq1 = 0.5f * (av.x * q4.x + av.y * q4.w - av.z * q4.z);
+        float dq2 = 0.5f * (-av.x * q4.w + av.y * q4.x + av.z * q4.y);
+        float dq3 = 0.5f * (av.x * q4.z - av.y * q4.y + av.z * q4.x);
+        q4.x += h * dq0; q4.y += h * dq1; q4.z += h * dq2; q4.w += h * dq3;
+        const float l2 = q4.x * q4.x + q4.y * q4.y + q4.z * q4.z + q4.w * q4.w;
+        if (l2 > 0.f) {
+            const float l = 1.f / sqrtf(l2);
+            q4.x *= l; q4.y *= l; q4.z *= l; q4.w *= l;
+        } else {
+            q4 = make_float4(1.f, 0.f, 0.f, 0.f);
+        }
+        D.pos[i] = p4; D.quat[i] = q4; D.lvel[i] = lv; D.avel[i] = av;
+        D.facc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        D.tacc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// -------------------------------------------- collision entry points -------
+__device__ void load_gview(const float *gv, DGeom &G) {
+    G.cls = (int)gv[0];
+    G.body = (int)gv[1];
+    for (int k = 0; k < 4; k++) G.p[k] = gv[2 + k];
+    for (int k = 0; k < 3; k++) G.pos[k] = gv[6 + k];
+    for (int k = 0; k < 12; k++) G.R[k] = gv[9 + k];
+}
+
+// All candidate pairs of n geoms (lower index first).  space 0: infinite
+// AABBs pair with everything untested (dxSAPSpace::collide,
+// collision_sapspace.cpp:479-532); space 1: every pair AABB-tested
+// (dxHashSpace, collision_space.cpp:421-614); space 2 (simple space): every
+// pair AABB-tested.  Geoms with body identity b share no pair with geoms of
+// the same non-negative b; category/collide bits filter as
+// collideAABBs (collision_space_internal.h:44-78).  Appends to pairs via an
+// atomic counter; the caller orders them.
+__global__ void k_pairs_all(const float *gviews, const float *aabb, const unsigned *bits, int n, int space,
+                            int *pairs, int *count, int cap) {
+    const long long T = (long long)n * (n - 1) / 2;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < T; k += (long long)gridDim.x * blockDim.x) {
+        const double fn = 2.0 * n - 1.0;
+        int i = (int)floor((fn - sqrt(fn * fn - 8.0 * (double)k)) * 0.5);
+        if (i < 0) i = 0;
+        while (i > 0 && (long long)i * (2 * n - i - 1) / 2 > k) i--;
+        while ((long long)(i + 1) * (2 * n - i - 2) / 2 <= k) i++;
+        const int j = (int)(k - (long long)i * (2 * n - i - 1) / 2) + i + 1;
+        const int bi = (int)gviews[(size_t)i * GVIEW_STRIDE + 1], bj = (int)gviews[(size_t)j * GVIEW_STRIDE + 1];
+        if (bi == bj && bi >= 0) continue;
+        const unsigned ci = bits[2 * i], li = bits[2 * i + 1], cj = bits[2 * j], lj = bits[2 * j + 1];
+        if (!((ci & lj) || (cj & li))) continue;
+        const float *x = aabb + (size_t)i * 6, *y = aabb + (size_t)j * 6;
+        bool ok;
+        if (space == 0 && (x[1] == kInf || y[1] == kInf)) ok = true;
+        else ok = !(x[0] > y[1] || x[1] < y[0] || x[2] > y[3] || x[3] < y[2] || x[4] > y[5] || x[5] < y[4]);
+        if (!ok) continue;
+        const int slot = atomicAdd(count, 1);
+        if (slot < cap) { pairs[2 * slot] = i; pairs[2 * slot + 1] = j; }
+    }
+}
+
+// AABBs of geom views (computeAABB of each class).
+__global__ void k_aabbs(const float *gviews, int n, float *aabb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    DGeom G;
+    load_gview(gviews + (size_t)i * GVIEW_STRIDE, G);
+    geom_aabb(G, aabb + (size_t)i * 6);
+}
+
+// dCollide for a list of geom-view pairs: up to maxc contacts per pair
+// (pos[3] normal[3] depth) into out[pair][maxc][7], counts[pair].
+__global__ void k_collide_pairs(const float *gviews, const int *pairs, int np, int maxc, float *out, int *counts) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    DGeom A, B;
+    load_gview(gviews + (size_t)pairs[2 * p] * GVIEW_STRIDE, A);
+    load_gview(gviews + (size_t)pairs[2 * p + 1] * GVIEW_STRIDE, B);
+    DContact c[ODX_MAXC];
+    const int n = dcollide(A, B, min(maxc, ODX_MAXC), c);
+    counts[p] = n;
+    for (int q = 0; q < n; q++) {
+        float *o = out + ((size_t)p * ODX_MAXC + q) * 7;
+        o[0] = c[q].pos[0]; o[1] = c[q].pos[1]; o[2] = c[q].pos[2];
+        o[3] = c[q].normal[0]; o[4] = c[q].normal[1]; o[5] = c[q].normal[2];
+        o[6] = c[q].depth;
+    }
+}
+
+}  // namespace odx
+
 // Host-side launchers (the kernels' only callers).
 namespace odx {
 void launch_prepare_small(const DevWorlds &D, const StepParams &P, hipStream_t s) {
@@ -851,5 +1190,28 @@ void launch_prepare_small(const DevWorlds &D, const StepParams &P, hipStream_t s
 }
 void launch_solve_small(const DevWorlds &D, const StepParams &P, hipStream_t s) {
     hipLaunchKernelGGL(k_solve_small, dim3(D.nworlds), dim3(64), 0, s, D, P);
+}
+}  // namespace odx
+
+namespace odx {
+void launch_single_step(const DevWorlds &D, const StepParams &P, int nb, hipStream_t s) {
+    hipLaunchKernelGGL(k_single_step, dim3(1), dim3(SL_BLOCK), (size_t)nb * 6 * sizeof(float), s, D, P);
+}
+void launch_aabbs(const float *gviews, int n, float *aabb, hipStream_t s) {
+    hipLaunchKernelGGL(k_aabbs, dim3((n + 255) / 256), dim3(256), 0, s, gviews, n, aabb);
+}
+void launch_pairs_all(const float *gviews, const float *aabb, const unsigned *bits, int n, int space, int *pairs,
+                      int *count, int cap, hipStream_t s) {
+    const long long T = (long long)n * (n - 1) / 2;
+    long long blocks = (T + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_pairs_all, dim3((unsigned)blocks), dim3(256), 0, s, gviews, aabb, bits, n, space, pairs, count,
+                       cap);
+}
+void launch_collide_pairs(const float *gviews, const int *pairs, int np, int maxc, float *out, int *counts,
+                          hipStream_t s) {
+    if (np <= 0) return;
+    hipLaunchKernelGGL(k_collide_pairs, dim3((np + 127) / 128), dim3(128), 0, s, gviews, pairs, np, maxc, out, counts);
 }
 }  // namespace odx

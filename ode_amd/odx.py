@@ -156,9 +156,10 @@ class Batch:
         return float(out[0])
 
     def kernel_times(self, n):
-        out = np.zeros(2, np.float32)
+        """Average [prepare, warm-start inference, solve] kernel milliseconds."""
+        out = np.zeros(3, np.float32)
         _check(lib().odxBatchKernelTimes(self.h, n, _p(out)), "odxBatchKernelTimes")
-        return float(out[0]), float(out[1])
+        return float(out[0]), float(out[1]), float(out[2])
 
     def bodies(self):
         out = np.zeros((self.nbodies, BODY), np.float32)
@@ -203,3 +204,112 @@ class Batch:
     def set_lambda0(self, lam0):
         a = np.ascontiguousarray(lam0, np.float32)
         _check(lib().odxBatchSetLambda0(self.h, _p(a), len(a)), "odxBatchSetLambda0")
+
+
+NPARAM = 19075
+FEAT = 16
+
+
+def _setup_mlp_api(L):
+    vp, i, fp = ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p
+    L.odxMlpCreate.restype = ctypes.c_void_p
+    L.odxMlpCreate.argtypes = [ctypes.c_uint]
+    L.odxMlpDestroy.argtypes = [vp]
+    L.odxMlpSetNormalization.argtypes = [vp, fp]
+    L.odxMlpGetParams.argtypes = [vp, fp]
+    L.odxMlpSetParams.argtypes = [vp, fp]
+    L.odxMlpSetData.argtypes = [vp, fp, fp, i]
+    L.odxMlpGradients.argtypes = [vp, fp, fp]
+    L.odxMlpTrain.argtypes = [vp, i, i, ctypes.c_float, ctypes.c_uint, fp]
+    L.odxMlpPredict.argtypes = [vp, fp, i, fp]
+    L.odxMfmaProbe.argtypes = [fp, fp, fp]
+    L.odxBatchSetMlp.argtypes = [vp, vp]
+    L.odxBatchGetSamples.argtypes = [vp, fp, fp, i]
+
+
+def mfma_probe(a_lanes, b_lanes):
+    L = lib()
+    _setup_mlp_api(L)
+    a = np.ascontiguousarray(a_lanes, np.float32).reshape(64, 8)
+    b = np.ascontiguousarray(b_lanes, np.float32).reshape(64, 8)
+    d = np.zeros((64, 4), np.float32)
+    _check(L.odxMfmaProbe(_p(a), _p(b), _p(d)), "odxMfmaProbe")
+    return d
+
+
+class Mlp:
+    """The learned warm-start network (odxMlp*): bf16 MFMA, fp32 accumulate."""
+
+    def __init__(self, seed=1):
+        L = lib()
+        _setup_mlp_api(L)
+        self.h = L.odxMlpCreate(seed)
+        _check(self.h, "odxMlpCreate")
+        self.norm = np.concatenate([np.zeros(16), np.ones(16), np.zeros(3), np.ones(3)]).astype(np.float32)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().odxMlpDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_normalization(self, norm38):
+        self.norm = np.ascontiguousarray(norm38, np.float32)
+        _check(lib().odxMlpSetNormalization(self.h, _p(self.norm)), "odxMlpSetNormalization")
+
+    def params(self):
+        out = np.zeros(NPARAM, np.float32)
+        _check(lib().odxMlpGetParams(self.h, _p(out)), "odxMlpGetParams")
+        return out
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, np.float32)
+        _check(lib().odxMlpSetParams(self.h, _p(a)), "odxMlpSetParams")
+
+    def set_data(self, X, T):
+        self._X = np.ascontiguousarray(X, np.float32)
+        self._T = np.ascontiguousarray(T, np.float32)
+        _check(lib().odxMlpSetData(self.h, _p(self._X), _p(self._T), len(self._X)), "odxMlpSetData")
+
+    def gradients(self):
+        g = np.zeros(NPARAM, np.float32)
+        loss = np.zeros(1, np.float32)
+        _check(lib().odxMlpGradients(self.h, _p(g), _p(loss)), "odxMlpGradients")
+        return g, float(loss[0])
+
+    def train(self, steps, batch=4096, lr=1e-3, seed=7):
+        losses = np.zeros(steps, np.float32)
+        _check(lib().odxMlpTrain(self.h, steps, batch, float(lr), seed, _p(losses)), "odxMlpTrain")
+        return losses
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((len(X), 3), np.float32)
+        _check(lib().odxMlpPredict(self.h, _p(X), len(X), _p(Y)), "odxMlpPredict")
+        return Y
+
+
+def _batch_set_mlp(self, mlp):
+    L = lib()
+    _setup_mlp_api(L)
+    self._mlp = mlp
+    _check(L.odxBatchSetMlp(self.h, mlp.h if mlp is not None else None), "odxBatchSetMlp")
+
+
+def _batch_samples(self, cap=None):
+    L = lib()
+    _setup_mlp_api(L)
+    cap = cap or int(self.row_start[-1])
+    X = np.zeros((cap, FEAT), np.float32)
+    T = np.zeros((cap, 3), np.float32)
+    n = L.odxBatchGetSamples(self.h, _p(X), _p(T), cap)
+    return X[:n], T[:n]
+
+
+Batch.set_mlp = _batch_set_mlp
+Batch.samples = _batch_samples

@@ -1,0 +1,231 @@
+"""The ODE-compatible C API (include/ode.h) driven through a near-callback
+frame loop, checked against the oracle: candidate pairs, dCollide contacts,
+and dWorldQuickStep in the reference's island row order."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle.islands import island_order
+from ode_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+class Frame:
+    """Builds a world through the C API and records what the callback did."""
+
+    def __init__(self, world_scene, space="sap", mu=0.5, iters=20):
+        from ode_amd import ode as A
+        self.A = A
+        L = A.api()
+        self.L = L
+        self.w = L.dWorldCreate()
+        L.dWorldSetGravity(self.w, 0, 0, -9.81)
+        L.dWorldSetQuickStepNumIterations(self.w, iters)
+        self.space = {"sap": lambda: L.dSweepAndPruneSpaceCreate(None, A.dSAP_AXES_XYZ),
+                      "hash": lambda: L.dHashSpaceCreate(None),
+                      "simple": lambda: L.dSimpleSpaceCreate(None)}[space]()
+        self.group = L.dJointGroupCreate(0)
+        self.mu = mu
+        self.scene = world_scene
+        self.bodies, self.geoms, self.geom_index = [], [], {}
+        for rec in world_scene.bodies:
+            b = L.dBodyCreate(self.w)
+            L.dBodySetPosition(b, *[float(x) for x in rec[0:3]])
+            q = (A.dReal * 4)(*[float(x) for x in rec[4:8]])
+            L.dBodySetQuaternion(b, q)
+            L.dBodySetLinearVel(b, *[float(x) for x in rec[8:11]])
+            L.dBodySetAngularVel(b, *[float(x) for x in rec[12:15]])
+            m = A.dMass()
+            L.dMassSetParameters(ctypes.byref(m), float(rec[24]), 0, 0, 0, float(rec[25]), float(rec[26]), float(rec[27]),
+                                 float(rec[28]), float(rec[29]), float(rec[30]))
+            L.dBodySetMass(b, ctypes.byref(m))
+            self.bodies.append(b)
+        for gi, g in enumerate(world_scene.geoms):
+            cls = int(g[0])
+            if cls == S.SPHERE:
+                h = L.dCreateSphere(self.space, float(g[2]))
+            elif cls == S.BOX:
+                h = L.dCreateBox(self.space, float(g[2]), float(g[3]), float(g[4]))
+            elif cls == S.CAPSULE:
+                h = L.dCreateCapsule(self.space, float(g[2]), float(g[3]))
+            else:
+                h = L.dCreatePlane(self.space, float(g[2]), float(g[3]), float(g[4]), float(g[5]))
+            if int(g[1]) >= 0:
+                L.dGeomSetBody(h, self.bodies[int(g[1])])
+            self.geoms.append(h)
+            self.geom_index[h] = gi
+        self.body_index = {b: i for i, b in enumerate(self.bodies)}
+
+    def collide(self):
+        A, L = self.A, self.L
+        self.pairs, self.contacts = [], []
+        buf = (A.dContact * 4)()
+
+        def near(data, o1, o2):
+            self.pairs.append((self.geom_index[o1], self.geom_index[o2]))
+            n = L.dCollide(o1, o2, 4, ctypes.cast(ctypes.byref(buf[0], A.dContact.geom.offset),
+                                                   ctypes.POINTER(A.dContactGeom)), ctypes.sizeof(A.dContact))
+            b1, b2 = L.dGeomGetBody(o1), L.dGeomGetBody(o2)
+            for i in range(n):
+                c = buf[i]
+                c.surface.mode = A.dContactApprox1
+                c.surface.mu = self.mu
+                j = L.dJointCreateContact(self.w, self.group, ctypes.byref(c))
+                L.dJointAttach(j, b1, b2)
+                i1 = self.body_index.get(b1, -1) if b1 else -1
+                i2 = self.body_index.get(b2, -1) if b2 else -1
+                self.contacts.append((i1, i2, list(c.geom.pos)[:3], list(c.geom.normal)[:3], c.geom.depth,
+                                      self.geom_index[o1], self.geom_index[o2]))
+
+        self._cb = A.NearCallback(near)
+        L.dSpaceCollide(self.space, None, self._cb)
+
+    def state(self):
+        A, L = self.A, self.L
+        out = np.zeros((len(self.bodies), 32))
+        for i, b in enumerate(self.bodies):
+            out[i, 0:3] = A.vec(L.dBodyGetPosition(b))
+            out[i, 4:8] = A.vec(L.dBodyGetQuaternion(b), 4)
+            out[i, 8:11] = A.vec(L.dBodyGetLinearVel(b))
+            out[i, 12:15] = A.vec(L.dBodyGetAngularVel(b))
+            out[i, 24:32] = self.scene.bodies[i, 24:32]
+        return out
+
+    def close(self):
+        L = self.L
+        L.dJointGroupDestroy(self.group)
+        L.dSpaceDestroy(self.space)
+        L.dWorldDestroy(self.w)
+
+
+def settle_scene(seed, n=14):
+    rng = np.random.default_rng(seed)
+    w = S.uneven_world(rng, n, n)
+    w.bodies[:, 2] = np.minimum(w.bodies[:, 2], 0.35) + rng.uniform(0, 0.05, size=len(w.bodies))
+    return w
+
+
+@pytest.mark.parametrize("space", ["sap", "hash"])
+def test_space_collide_pairs_and_contacts_match_oracle(space):
+    say("case: dSpaceCollide", space)
+    scene = settle_scene(3, 20)
+    F = Frame(scene, space=space)
+    F.collide()
+    ab = O.aabbs(scene.geoms, scene.bodies, np.float32)
+    body = scene.geoms[:, 1].astype(np.int32)
+    want = {tuple(sorted(p)) for p in O.pairs(ab, body, space)}
+    got = {tuple(sorted(p)) for p in F.pairs}
+    assert got == want
+    assert len(F.pairs) == len(want)
+    # every callback's dCollide equals the oracle's dCollide of that ordered pair
+    by_pair = {}
+    for c in F.contacts:
+        by_pair.setdefault((c[5], c[6]), []).append(c)
+    for g1, g2 in F.pairs:
+        bi, bj = int(scene.geoms[g1, 1]), int(scene.geoms[g2, 1])
+        ref = O.collide(scene.geoms[g1], scene.geoms[g2], scene.bodies[bi] if bi >= 0 else None,
+                        scene.bodies[bj] if bj >= 0 else None, maxc=4)
+        got_c = by_pair.get((g1, g2), [])
+        assert len(got_c) == len(ref)
+        for gc, rc in zip(got_c, ref):
+            np.testing.assert_allclose(gc[2], rc[0:3], atol=2e-5)
+            np.testing.assert_allclose(gc[3], rc[3:6], atol=2e-5)
+            assert abs(gc[4] - rc[6]) < 2e-5
+    F.close()
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_quickstep_matches_oracle_in_island_order(seed):
+    say("case: dWorldQuickStep seed", seed)
+    scene = settle_scene(seed)
+    F = Frame(scene)
+    F.collide()
+    before = scene.bodies.copy()
+    assert F.L.dWorldQuickStep(F.w, 0.01) == 1
+    got = F.state()
+    joints = [(c[0], c[1]) for c in F.contacts]
+    # dJointAttach with a world geom first: body1 NULL -> reversed joint
+    recs = []
+    for (i1, i2, pos, nrm, depth, g1, g2) in F.contacts:
+        r = np.zeros(16)
+        r[1], r[2] = i1, i2
+        r[3:6], r[6:9], r[9] = pos, nrm, depth
+        recs.append(r)
+    norm_pairs = [(a, b) if a >= 0 else (b, -1) for a, b in joints]
+    _, order = island_order(len(scene.bodies), norm_pairs)
+    ref_b, ref_l = O.quickstep(before, np.array([recs[k] for k in order]).reshape(-1, 16), O.params(), dtype=np.float64)
+    err_x = np.max(np.abs(got[:, 0:3] - ref_b[:, 0:3])) / np.max(np.abs(ref_b[:, 0:3]))
+    err_v = np.max(np.abs(got[:, 8:15] - ref_b[:, 8:15])) / max(np.max(np.abs(ref_b[:, 8:15])), 1.0)
+    say("  contacts", len(recs), "err_x", err_x, "err_v", err_v)
+    assert err_x < 1e-4 and err_v < 1e-4
+    F.L.dJointGroupEmpty(F.group)
+    F.close()
+
+
+def test_frame_loop_runs_and_stays_above_ground():
+    say("case: frame loop")
+    scene = settle_scene(4)
+    F = Frame(scene)
+    for _ in range(100):
+        F.collide()
+        assert F.L.dWorldQuickStep(F.w, 0.01) == 1
+        F.L.dJointGroupEmpty(F.group)
+    st = F.state()
+    assert np.all(np.isfinite(st[:, 0:15]))
+    assert np.min(st[:, 2]) > -0.05
+    F.close()
+
+
+def test_ball_joint_feedback_carries_the_weight():
+    say("case: joint feedback")
+    from ode_amd import ode as A
+    L = A.api()
+    w = L.dWorldCreate()
+    L.dWorldSetGravity(w, 0, 0, -9.81)
+    L.dWorldSetQuickStepNumIterations(w, 200)
+    b = L.dBodyCreate(w)
+    L.dBodySetPosition(b, 0, 0, 1)
+    m = A.dMass()
+    L.dMassSetSphere(ctypes.byref(m), 1000.0, 0.1)
+    L.dBodySetMass(b, ctypes.byref(m))
+    j = L.dJointCreateBall(w, None)
+    L.dJointAttach(j, b, None)
+    L.dJointSetBallAnchor(j, 0, 0, 1.2)
+    fb = A.dJointFeedback()
+    L.dJointSetFeedback(j, ctypes.byref(fb))
+    for _ in range(50):
+        assert L.dWorldQuickStep(w, 0.01) == 1
+    # the joint holds the hanging body: force on body 1 balances gravity
+    assert fb.f1[2] == pytest.approx(m.mass * 9.81, rel=0.02)
+    pos = A.vec(L.dBodyGetPosition(b))
+    assert abs(pos[2] - 1.0) < 0.01
+    L.dWorldDestroy(w)
+
+
+def test_dcollide_alone_and_drand():
+    say("case: dCollide single pair, dRand")
+    from ode_amd import ode as A
+    L = A.api()
+    s1 = L.dCreateSphere(None, 0.5)
+    pl = L.dCreatePlane(None, 0, 0, 1, 0)
+    L.dGeomSetPosition(s1, 1, 2, 0.4)
+    c = (A.dContactGeom * 4)()
+    n = L.dCollide(s1, pl, 4, c, ctypes.sizeof(A.dContactGeom))
+    assert n == 1
+    assert c[0].depth == pytest.approx(0.1, abs=1e-6)
+    assert list(c[0].normal)[:3] == pytest.approx([0, 0, 1])
+    n = L.dCollide(pl, s1, 4, c, ctypes.sizeof(A.dContactGeom))
+    assert list(c[0].normal)[:3] == pytest.approx([0, 0, -1])
+    L.dRandSetSeed(0)
+    assert [L.dRand() for _ in range(5)] == O.drand_seq(0, 5)
+    L.dGeomDestroy(s1)
+    L.dGeomDestroy(pl)

@@ -364,3 +364,16 @@ def test_float_and_double_oracles_agree_single_step():
     assert len(c64) == len(c32)
     np.testing.assert_allclose(b32[:, 0:3], b64[:, 0:3], rtol=1e-5, atol=1e-5)
     np.testing.assert_allclose(b32[:, 8:15], b64[:, 8:15], rtol=1e-3, atol=1e-3)
+
+
+# ------------------------------------------------------------ island order
+def test_island_order_hand_checked():
+    from oracle.islands import island_order
+    # bodies 0..3 (world list 3,2,1,0); joints in attach order:
+    # j0 (0,1), j1 (1,2), j2 (3,-1), j3 (0,2)
+    bodies, joints = island_order(4, [(0, 1), (1, 2), (3, -1), (0, 2)])
+    # body 3 first (list head), its joint j2 (world) -> island {3}
+    # then body 2: nodes (most recent first) j3->0, j1->1: push 0, push 1; pop 1
+    # body 1: nodes j1 (tagged), j0->0 (0 already tagged); pop 0; body 0: all tagged
+    assert bodies == [3, 2, 1, 0]
+    assert joints == [2, 3, 1, 0]
