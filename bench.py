@@ -64,9 +64,27 @@ def cpu_baseline(worlds, bodies_now, body_start, prm, budget_s=12.0, steps_per_w
 
 
 def measure(B, args, warm, nworlds):
-    """Warmup, timed graph replays, then per-kernel hipEvent durations."""
+    """Warmup, timed graph replays, then per-kernel hipEvent durations.  The
+    LDS row budget is sized during warmup to the largest world seen (rounded
+    up to 16 rows, within the 64 KB limit) so that every world sweeps from LDS."""
     B.set_solver(order=args.order, max_iters=args.max_iters, tol=args.tol, check_every=args.check_every, warm=warm)
-    B.step(args.warmup)
+    B.set_solve_options(restart_after=args.restart_after, lds_rows=args.lds_rows if args.lds_rows >= 0 else 400)
+    half = args.warmup // 2
+    B.step(half)
+    max_rows = int(B.stats()["rows"].max())
+    for _ in range(half, args.warmup):
+        B.step(1)
+        max_rows = max(max_rows, int(B.stats()["rows"].max()))
+    lds_rows = args.lds_rows if args.lds_rows >= 0 else 400
+    if args.lds_rows < 0:
+        want = -(-int(max_rows * 1.1) // 16) * 16
+        while want > max_rows:
+            try:
+                B.set_solve_options(restart_after=args.restart_after, lds_rows=want)
+                lds_rows = want
+                break
+            except Exception:
+                want -= 16
     B.sync()
     bodies_after_warmup = B.bodies()
     total_ms = B.time(args.steps)
@@ -95,6 +113,8 @@ def measure(B, args, warm, nworlds):
         "solve_gbs": float(np.mean(ach)),
         "kernel_window_contacts": float(np.mean(contacts)),
         "overflow": [int(x) for x in B.overflow()],
+        "lds_rows": lds_rows,
+        "worlds_over_lds_budget": int(np.sum(st["rows"] > lds_rows)),
         "_bodies": bodies_after_warmup,
     }
 
@@ -116,6 +136,9 @@ def main():
     ap.add_argument("--train-samples", type=int, default=2_000_000)
     ap.add_argument("--train-steps", type=int, default=6000)
     ap.add_argument("--modes", default="cold,plain,learned")
+    ap.add_argument("--restart-after", type=int, default=600,
+                    help="sweeps after which an unconverged warm start restarts from lambda = 0")
+    ap.add_argument("--lds-rows", type=int, default=-1, help="LDS row budget per world (-1: sized during warmup)")
     args = ap.parse_args()
 
     if args.gpus != 1:

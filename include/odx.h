@@ -77,6 +77,13 @@ void odxBatchDestroy(odxBatch *b);
  * capped at max_iters.  warm: 0 cold start, 1 lambda0 from odxBatchSetLambda0. */
 int odxBatchSetSolver(odxBatch *b, int order, int max_iters, float tol, int check_every, int warm);
 
+/* restart_after > 0: a warm-started world that has not reached tol after that
+ * many sweeps restarts from lambda = 0 (0 = never).  lds_rows: worlds with at
+ * most this many rows sweep with rows and bodies held in LDS (64 B per row);
+ * larger worlds stream rows from global memory; 0 disables the LDS path.
+ * Fails when lds_rows does not fit the kernel's 64 KB LDS budget. */
+int odxBatchSetSolveOptions(odxBatch *b, int restart_after, int lds_rows);
+
 /* Runs nsteps steps on the batch's stream (captured once into a hipGraph and
  * replayed).  Returns 1 on success. */
 int odxBatchStep(odxBatch *b, int nsteps);
@@ -121,6 +128,11 @@ odxMlp *odxMlpCreate(unsigned seed);
 void odxMlpDestroy(odxMlp *m);
 /* norm38: input mean[16], input scale[16], output mean[3], output scale[3]. */
 int odxMlpSetNormalization(odxMlp *m, const float *norm38);
+/* gate 1: contacts matched to a previous-step contact keep its impulses and
+ * the network predicts only new contacts; gate 0 (default): the network's
+ * predicted change is added for every contact.  Takes effect for batches
+ * attached (odxBatchSetMlp) afterwards. */
+int odxMlpSetGate(odxMlp *m, int gate);
 int odxMlpGetParams(odxMlp *m, float *params);
 int odxMlpSetParams(odxMlp *m, const float *params);
 /* Training set: features X[N][16] (raw) and targets T[N][3] (impulse / force scale). */

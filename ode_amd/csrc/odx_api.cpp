@@ -974,12 +974,30 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
                 level[k] = c;
                 L = std::max(L, c + 1);
             }
-            std::vector<int> cnt(L + 1, 0);
-            for (int k = 0; k < nj; k++) cnt[level[k] + 1]++;
-            for (int p = 1; p <= L; p++) cnt[p] += cnt[p - 1];
-            phase = cnt;
-            sched.resize(nj);
-            for (int k = 0; k < nj; k++) sched[cnt[level[k]]++] = (joff[k] << 3) | jm[k];
+            // two passes like ReorderPrep (quickstep.cpp:2329-2355): each joint's
+            // leading non-findex rows in colour phases first, then its findex
+            // (friction-cone) rows in a second set of colour phases
+            std::vector<int> na(nj), nf(nj);
+            for (int k = 0; k < nj; k++) split_joint_rows(&rfl[joff[k]], jm[k], na[k], nf[k]);
+            std::vector<int> cnt(2 * L + 1, 0);
+            for (int k = 0; k < nj; k++) {
+                cnt[level[k] + 1]++;
+                if (nf[k]) cnt[L + level[k] + 1]++;
+            }
+            std::vector<int> start(2 * L, 0);
+            phase.clear();
+            int run = 0;
+            for (int p = 0; p < 2 * L; p++) {
+                start[p] = run;
+                if (cnt[p + 1]) phase.push_back(run);
+                run += cnt[p + 1];
+            }
+            phase.push_back(run);
+            sched.resize(run);
+            for (int k = 0; k < nj; k++) {
+                sched[start[level[k]]++] = (joff[k] << 3) | na[k];
+                if (nf[k]) sched[start[L + level[k]]++] = ((joff[k] + na[k]) << 3) | nf[k];
+            }
         }
     }
     const int L = (int)phase.size() - 1;

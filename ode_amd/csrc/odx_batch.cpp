@@ -6,6 +6,7 @@
 // here throws across the C ABI.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -138,6 +139,19 @@ struct odxBatch {
     }
 };
 
+// Rows per world the batched solve keeps in LDS by default: 400 rows (64 B
+// each plus lambda, work item and phase start: 74 B) and 65 bodies take
+// 33 KB, so four to five worlds share a CU's 160 KB of LDS.
+constexpr int ODX_DEFAULT_LDS_ROWS = 400;
+
+// Largest row count <= want whose LDS footprint fits the kernel's limit
+// (same accounting as solve_small_lds_bytes).
+static int odx_lds_rows_fit(int lds_bodies, int want) {
+    const int per_row = (16 + 2) * (int)sizeof(float) + (int)sizeof(unsigned short);
+    const int room = (SOLVE_LDS_MAX - lds_bodies * 16 * (int)sizeof(float) - 2) / per_row;
+    return std::max(0, std::min(want, std::min(room, SW_MAXROWS)));
+}
+
 static void params_from(const double *p, StepParams &P, Surface &S) {
     P.h = (float)p[0];
     P.gx = (float)p[1]; P.gy = (float)p[2]; P.gz = (float)p[3];
@@ -234,6 +248,11 @@ odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies
     B->jcap = B->joint_start[nworlds];
     B->rcap = B->row_start[nworlds];
     params_from(params, B->P, B->surf);
+    int maxnb = 0;
+    for (int w = 0; w < nworlds; w++) maxnb = std::max(maxnb, B->body_start[w + 1] - B->body_start[w]);
+    B->P.lds_bodies = maxnb + 1;
+    B->P.lds_rows = odx_lds_rows_fit(B->P.lds_bodies, ODX_DEFAULT_LDS_ROWS);
+    B->P.restart_after = 0;
 
     // host-side SoA conversion
     const int nb = B->nb, ng = B->ng;
@@ -323,7 +342,7 @@ odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies
     ok = ok && B->alloc(&D.feat, (size_t)B->jcap * FEAT_DIM) && B->alloc(&D.feat_row, B->jcap) &&
          B->alloc(&D.feat_count, 2) && B->alloc(&D.joff, B->jcap);
     ok = ok && B->alloc(&D.prev, (size_t)B->jcap * JOINT_STRIDE) && B->alloc(&D.nprev, nworlds);
-    ok = ok && B->alloc(&D.overflow, 4);
+    ok = ok && B->alloc(&D.overflow, 4) && B->alloc(&D.order, nworlds);
     ok = ok && B->alloc(&B->lambda0, B->rcap);
     if (ok) {
         ok = odx_check(hipMemset(D.overflow, 0, 4 * sizeof(int)), "hipMemset") &&
@@ -357,6 +376,23 @@ int odxBatchSetSolver(odxBatch *b, int order, int max_iters, float tol, int chec
     b->P.tol = tol;
     b->P.check_every = check_every;
     b->P.warm = warm;
+    b->drop_graph();
+    return 1;
+}
+
+int odxBatchSetSolveOptions(odxBatch *b, int restart_after, int lds_rows) {
+    if (!b || restart_after < 0 || lds_rows < 0) {
+        odx_message(1, "odxBatchSetSolveOptions: bad arguments");
+        return 0;
+    }
+    const int fit = odx_lds_rows_fit(b->P.lds_bodies, lds_rows);
+    if (fit < lds_rows) {
+        odx_message(1, "odxBatchSetSolveOptions: %d LDS rows exceed the %d-byte budget (at most %d)", lds_rows,
+                    SOLVE_LDS_MAX, fit);
+        return 0;
+    }
+    b->P.restart_after = restart_after;
+    b->P.lds_rows = lds_rows;
     b->drop_graph();
     return 1;
 }

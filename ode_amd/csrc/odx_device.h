@@ -17,7 +17,9 @@ constexpr int SW_MAXB = 64;       // bodies
 constexpr int SW_MAXG = 96;       // geoms
 constexpr int SW_MAXPAIRS = 1024; // candidate pairs after broadphase
 constexpr int SW_MAXROWS = 2048;  // constraint rows
-constexpr int SW_MAXPH = 2048;    // schedule phases
+// Schedule keys: colours are < 64 + joints <= 64 + SW_MAXROWS (a joint whose
+// bodies have used all 64 mask colours gets a phase of its own).
+constexpr int SW_MAXPH = SW_MAXROWS + 65;
 
 // Limit of the single-world (dWorldQuickStep) kernel: bodies per island set.
 constexpr int SL_MAXB = 2048;
@@ -58,7 +60,28 @@ struct StepParams {
     int collide;         // 1: generate contacts; 0: joints were uploaded
     int features;        // 1: write warm-start features per contact
     int feedback;        // 1: write joint forces J^T lambda per joint (dJointFeedback)
+    int restart_after;   // > 0: a warm-started world not at tol after this many sweeps restarts from lambda = 0
+    int lds_rows;        // batched solve: worlds with at most this many rows sweep from LDS (0: never)
+    int lds_bodies;      // LDS body records per world (max bodies per world + 1 zero body)
 };
+
+// Largest dynamic LDS of the batched solve kernel (bytes per workgroup).
+constexpr int SOLVE_LDS_MAX = 65536;
+
+// Split of a joint's rows for the coloured schedule, which keeps the
+// reference's two passes (ReorderPrep, quickstep.cpp:2329-2355: rows without
+// findex first, findex rows after): rfl[r] = 1 for a findex row.  A joint
+// whose rows are n_a non-findex rows followed by n_f findex rows gives
+// (n_a, n_f); any other pattern stays one item in the first pass (n_f = 0).
+template <class Flag>
+__host__ __device__ inline void split_joint_rows(const Flag *rfl, int mj, int &n_a, int &n_f) {
+    int a = 0;
+    while (a < mj && rfl[a] == 0) a++;
+    int f = a;
+    while (f < mj && rfl[f] == 1) f++;
+    if (a > 0 && f == mj) { n_a = a; n_f = mj - a; }
+    else { n_a = mj; n_f = 0; }
+}
 
 struct DevWorlds {
     int nworlds;
@@ -87,6 +110,7 @@ struct DevWorlds {
     int *nprev;             // [W]
     int *overflow;          // [4] pairs / contacts / rows / phases overflow counters
     float *fb;              // [joint cap * 12] f1 t1 f2 t2 per joint (feedback mode)
+    int *order;             // [W] solve dispatch order (longest previous solve first); null = index order
 };
 
 // Geom view for the collision entry points of the single-world API:

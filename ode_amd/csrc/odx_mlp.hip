@@ -44,6 +44,7 @@ struct MlpDev {
     float *mom, *vel;                          // Adam moments over the flat parameter vector
     float *norm;                               // in_mu[16] in_sig[16] out_mu[3] out_sig[3]
     float *loss;                               // [2] sum of squared error, samples
+    int gate;                                  // 1: matched contacts keep their previous impulses
 };
 
 // parameter vector layout (flat offsets)
@@ -51,10 +52,12 @@ constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NH * NIN, OFF_W2 = OFF_B1 + NH, OFF_
               OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NOUT * NH, NPARAM = OFF_B3 + NOUT;
 
 __device__ __forceinline__ float gelu(float x, float &dgelu) {
-    // tanh form: 0.5 x (1 + tanh(c (x + a x^3)))
+    // tanh form: 0.5 x (1 + tanh(c (x + a x^3))), tanh(u) = 1 - 2 / (1 + e^{2u})
+    // (e^{2u} overflows to +inf for large u, which gives 1; the result feeds
+    // bf16 activations, so the few-ulp error of the exp/rcp pair is invisible)
     const float c = 0.7978845608028654f, a = 0.044715f;
     const float u = c * (x + a * x * x * x);
-    const float th = tanhf(u);
+    const float th = 1.f - 2.f * __frcp_rn(1.f + __expf(2.f * u));
     dgelu = 0.5f * (1.f + th) + 0.5f * x * (1.f - th * th) * c * (1.f + 3.f * a * x * x);
     return 0.5f * x * (1.f + th);
 }
@@ -75,25 +78,41 @@ struct TileBuf {
     float dz3[16][4];
 };
 
+// Inference scratch of one tile: activations only.
+struct InferBuf {
+    __bf16 act1[16][NH];
+    __bf16 act2[16][NH];
+};
+
 // Forward pass of one 16-sample tile by one wave.  xr: raw features of the
 // tile's samples (invalid samples produce garbage rows that callers ignore).
 // Returns the layer-3 accumulator (D layout, cols 0..2 are the outputs).
-__device__ f32x4 forward_tile(const MlpDev &M, const __bf16 *W1h, const __bf16 *W2h, const __bf16 *W3h,
-                              const float *b1, const float *b2, const float *b3, const float *nrm, TileBuf &tb,
-                              const float *xr, int nvalid, bool keep_grad) {
+// keep_grad (training, Buf = TileBuf) also stores gelu' of both layers.
+template <bool keep_grad, class Buf>
+__device__ f32x4 forward_tile(const __bf16 *W1h, const __bf16 *W2h, const __bf16 *W3h, const float *b1,
+                              const float *b2, const float *b3, const float *nrm, Buf &tb, const float *xr,
+                              int nvalid) {
     const int l = threadIdx.x & 63;
     const int row = l & 15, g = l >> 4;
-    // normalised inputs
-    for (int e = l; e < 16 * NIN; e += 64) {
-        const int s = e / NIN, f = e % NIN;
-        const float v = s < nvalid ? xr[(size_t)s * NIN + f] : 0.f;
-        tb.x[s][f] = (v - nrm[f]) / nrm[NIN + f];
+    // normalised inputs: lane (row, g < 2) holds features 8g..8g+7 of sample row
+    float xn[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int f = 8 * (g & 1) + j;
+        const float v = row < nvalid ? xr[(size_t)row * NIN + f] : 0.f;
+        xn[j] = (v - nrm[f]) / nrm[NIN + f];
     }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if constexpr (keep_grad) {
+        if (g < 2) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) tb.x[row][8 * g + j] = xn[j];
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
     bf16x8 a1;
 #pragma unroll
-    for (int j = 0; j < 8; j++) a1[j] = g < 2 ? (__bf16)tb.x[row][8 * g + j] : (__bf16)0.f;
+    for (int j = 0; j < 8; j++) a1[j] = g < 2 ? (__bf16)xn[j] : (__bf16)0.f;
     // layer 1
 #pragma unroll
     for (int t = 0; t < NH / 16; t++) {
@@ -106,7 +125,7 @@ __device__ f32x4 forward_tile(const MlpDev &M, const __bf16 *W1h, const __bf16 *
             float d;
             const float h = gelu(acc[j] + b1[n], d);
             tb.act1[r][n] = (__bf16)h;
-            if (keep_grad) tb.g1[r][n] = (__bf16)d;
+            if constexpr (keep_grad) tb.g1[r][n] = (__bf16)d;
         }
     }
     __builtin_amdgcn_wave_barrier();
@@ -127,7 +146,7 @@ __device__ f32x4 forward_tile(const MlpDev &M, const __bf16 *W1h, const __bf16 *
             float d;
             const float h = gelu(acc[j] + b2[n], d);
             tb.act2[r][n] = (__bf16)h;
-            if (keep_grad) tb.g2[r][n] = (__bf16)d;
+            if constexpr (keep_grad) tb.g2[r][n] = (__bf16)d;
         }
     }
     __builtin_amdgcn_wave_barrier();
@@ -150,7 +169,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(MlpDev M, const float *feat, 
     __shared__ __align__(16) __bf16 sW2[NH * NH];
     __shared__ __align__(16) __bf16 sW3[NOUTP * NH];
     __shared__ float sb[NH + NH + 4 + 38];
-    __shared__ TileBuf tb[4];
+    __shared__ InferBuf tb[4];
     const int tid = threadIdx.x;
     const int n = *count;
     if (n <= 0) return;
@@ -167,15 +186,20 @@ __global__ __launch_bounds__(256) void k_mlp_infer(MlpDev M, const float *feat, 
     for (int tile = blockIdx.x * 4 + wv; tile < ntiles; tile += gridDim.x * 4) {
         const int s0 = tile * 16;
         const int nv = min(16, n - s0);
-        f32x4 y = forward_tile(M, sW1, sW2, sW3, sb, sb + NH, sb + 2 * NH, nrm, tb[wv], feat + (size_t)s0 * NIN, nv, false);
+        f32x4 y = forward_tile<false>(sW1, sW2, sW3, sb, sb + NH, sb + 2 * NH, nrm, tb[wv], feat + (size_t)s0 * NIN, nv);
         const int o = l & 15, g = l >> 4;
         if (o < NOUT) {
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const int smp = s0 + 4 * g + j;
                 if (smp < n) {
-                    const float scale = feat[(size_t)smp * NIN + 15];
-                    lambda0[feat_row[smp] + o] = scale * (y[j] * nrm[2 * NIN + NOUT + o] + nrm[2 * NIN + o]);
+                    // the network predicts the change from the matched previous impulse
+                    // (features 7-9, zero when unmatched), in units of the force scale s;
+                    // with the gate on, matched contacts (feature 12) keep that impulse
+                    const float *x = feat + (size_t)smp * NIN;
+                    const float use = (M.gate && x[12] > 0.5f) ? 0.f : 1.f;
+                    lambda0[feat_row[smp] + o] =
+                        x[15] * (x[7 + o] + use * (y[j] * nrm[2 * NIN + NOUT + o] + nrm[2 * NIN + o]));
                 }
             }
         }
@@ -239,7 +263,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpDev M, const float *X, con
         __syncthreads();
         const int base = wv * 16;
         const int nv = max(0, min(16, batch - (c * 64 + base)));
-        f32x4 y = forward_tile(M, sW1, sW2, sW3, sb, sb + NH, sb + 2 * NH, nrm, tb[wv], &sx_raw[base][0], nv, true);
+        f32x4 y = forward_tile<true>(sW1, sW2, sW3, sb, sb + NH, sb + 2 * NH, nrm, tb[wv], &sx_raw[base][0], nv);
         // dZ3 = 2 (y - t_norm) / batch for valid samples
         {
             const int o = row;
@@ -520,6 +544,12 @@ odxMlp *odxMlpCreate(unsigned seed) {
 }
 
 void odxMlpDestroy(odxMlp *m) { delete m; }
+
+int odxMlpSetGate(odxMlp *m, int gate) {
+    if (!m || gate < 0 || gate > 1) return 0;
+    m->M.gate = gate;
+    return 1;
+}
 
 int odxMlpSetNormalization(odxMlp *m, const float *norm38) {
     if (!m || !norm38) return 0;

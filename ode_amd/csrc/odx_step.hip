@@ -732,21 +732,44 @@ __global__ __launch_bounds__(64) void k_prepare_small(DevWorlds D, StepParams P)
             s_L = L;
         }
         __syncthreads();
+        // Two passes like ReorderPrep (quickstep.cpp:2329-2355): each joint's
+        // non-findex rows in the colour phases [0, L), then its findex
+        // (friction-cone) rows in phases [L, 2L); empty phases are dropped.
+        // Interleaving a contact's friction rows with its normal row instead
+        // can settle into a limit cycle short of the residual target.
         const int L = s_L;
-        for (int p = t; p <= L; p += 64) s_cnt[p] = 0;
+        const int K = 2 * L <= SW_MAXPH ? 2 * L : L;
+        for (int p = t; p <= K; p += 64) s_cnt[p] = 0;
         __syncthreads();
-        for (int k = t; k < nj_used; k += 64) atomicAdd(&s_cnt[s_lvl[k] + 1], 1);
+        for (int k = t; k < nj_used; k += 64) {
+            int na = s_jm[k], nf = 0;
+            if (K > L) split_joint_rows(s_rfl + s_joff[k], (int)s_jm[k], na, nf);
+            atomicAdd(&s_cnt[s_lvl[k] + 1], 1);
+            if (nf) atomicAdd(&s_cnt[L + s_lvl[k] + 1], 1);
+        }
         __syncthreads();
         if (t == 0) {
-            for (int p = 1; p <= L; p++) s_cnt[p] += s_cnt[p - 1];
-            for (int p = 0; p <= L; p++) ps[p] = s_cnt[p];
+            int run = 0, q = 0;
+            for (int p = 0; p < K; p++) {
+                const int c = s_cnt[p + 1];
+                s_cnt[p] = run;  // insertion cursor of key p
+                if (c > 0) ps[q++] = run;
+                run += c;
+            }
+            ps[q] = run;
+            D.nphases[w] = q;
         }
         __syncthreads();
         for (int k = t; k < nj_used; k += 64) {
+            int na = s_jm[k], nf = 0;
+            if (K > L) split_joint_rows(s_rfl + s_joff[k], (int)s_jm[k], na, nf);
             const int pos = atomicAdd(&s_cnt[s_lvl[k]], 1);
-            sched[pos] = ((int)s_joff[k] << 3) | (int)s_jm[k];
+            sched[pos] = ((int)s_joff[k] << 3) | na;
+            if (nf) {
+                const int pf = atomicAdd(&s_cnt[L + s_lvl[k]], 1);
+                sched[pf] = (((int)s_joff[k] + na) << 3) | nf;
+            }
         }
-        if (t == 0) D.nphases[w] = L;
     }
     if (t == 0) D.nrows[w] = mrow;
 }
@@ -830,10 +853,206 @@ __device__ float world_residual(const DevWorlds &D, const StepParams &P, int r0,
     return mx / fmaxf(lmax, P.lambda_floor);
 }
 
-__global__ __launch_bounds__(64) void k_solve_small(DevWorlds D, StepParams P) {
-    const int w = blockIdx.x;
+// ---- LDS-resident sweeps -------------------------------------------------
+// A world whose rows fit the launch's LDS budget is solved from LDS: each
+// sweep then costs LDS latency instead of a round trip to HBM per row.
+// Row (16 floats): 0-2 J1 linear, 3-5 J1 angular, 6-8 J2 angular, 9 rhs,
+// 10 cfm (J, rhs and cfm Ad-scaled as in the global row), 11 1/Ad,
+// 12 lo, 13 hi, 14 ids = b1 | b2 << 8 | (findex + 1) << 16, 15 unused.
+// J2 linear is -J1 linear for every contact and ball row (checked when the
+// rows are staged).  The body-side terms of iMJ = M^-1 J^T are formed from
+// the body record, iMJ = (1/Ad) [invMass J_lin, invI_w J_ang] with J
+// Ad-scaled, so a row needs 64 B instead of 128 B.
+// Body (16 floats): 0-5 fc, 6 invMass, 7-15 invI_w (3x3 row-major).  A row
+// without a second body points at the zero body at index nb.
+constexpr int LDS_ROW = 16, LDS_BODY = 16;
+
+__device__ __forceinline__ void lds_body(const float *s_body, int b, float *fc, float &im, float *I) {
+    const float4 *p = reinterpret_cast<const float4 *>(s_body + b * LDS_BODY);
+    const float4 a = p[0], c = p[1], d = p[2], e = p[3];
+    fc[0] = a.x; fc[1] = a.y; fc[2] = a.z; fc[3] = a.w; fc[4] = c.x; fc[5] = c.y;
+    im = c.z;
+    I[0] = c.w; I[1] = d.x; I[2] = d.y; I[3] = d.z; I[4] = d.w; I[5] = e.x; I[6] = e.y; I[7] = e.z; I[8] = e.w;
+}
+
+__device__ __forceinline__ void lds_store_fc(float *s_body, int b, const float *fc) {
+    float4 *p = reinterpret_cast<float4 *>(s_body + b * LDS_BODY);
+    p[0] = make_float4(fc[0], fc[1], fc[2], fc[3]);
+    *reinterpret_cast<float2 *>(p + 1) = make_float2(fc[4], fc[5]);
+}
+
+// LDS work item: first row (11 bits), row count (3 bits), body 1 and body 2
+// (7 bits each), so the body and row loads of an item issue together.
+__device__ __forceinline__ int lds_item(int rs, int nr, int b1, int b2) {
+    return rs | (nr << 11) | (b1 << 14) | (b2 << 21);
+}
+
+__device__ __forceinline__ void process_item_lds(const float *s_rows, float *s_lam, float *s_body, int nb, int item) {
+    const int rs = item & 2047, nr = (item >> 11) & 7;
+    const int b1 = (item >> 14) & 127, b2 = (item >> 21) & 127;
+    const float4 *row4 = reinterpret_cast<const float4 *>(s_rows) + rs * 4;
+    float f1[6], f2[6], I1[9], I2[9], im1, im2;
+    lds_body(s_body, b1, f1, im1, I1);
+    lds_body(s_body, b2, f2, im2, I2);
+    float lam_first = 0.f;
+    for (int q = 0; q < nr; q++) {
+        const float4 a0 = row4[4 * q], a1 = row4[4 * q + 1], a2 = row4[4 * q + 2], a3 = row4[4 * q + 3];
+        const float old = s_lam[rs + q];
+        float delta = a2.y - old * a2.z;
+        delta -= a0.x * (f1[0] - f2[0]) + a0.y * (f1[1] - f2[1]) + a0.z * (f1[2] - f2[2]);
+        delta -= a0.w * f1[3] + a1.x * f1[4] + a1.y * f1[5];
+        delta -= a1.z * f2[3] + a1.w * f2[4] + a2.x * f2[5];
+        float lo = a3.x, hi = a3.y;
+        const int fidx = (as_int(a3.z) >> 16) - 1;
+        if (fidx >= 0) {
+            const float lf = (fidx == rs) ? lam_first : s_lam[fidx];
+            hi = fabsf(hi * lf);
+            lo = -hi;
+        }
+        float nl = old + delta;
+        if (nl < lo) { delta = lo - old; nl = lo; }
+        else if (nl > hi) { delta = hi - old; nl = hi; }
+        s_lam[rs + q] = nl;
+        if (q == 0) lam_first = nl;
+        const float e = delta * a2.w, e1 = e * im1, e2 = e * im2;
+        f1[0] += e1 * a0.x; f1[1] += e1 * a0.y; f1[2] += e1 * a0.z;
+        f2[0] -= e2 * a0.x; f2[1] -= e2 * a0.y; f2[2] -= e2 * a0.z;
+        f1[3] += e * (I1[0] * a0.w + I1[1] * a1.x + I1[2] * a1.y);
+        f1[4] += e * (I1[3] * a0.w + I1[4] * a1.x + I1[5] * a1.y);
+        f1[5] += e * (I1[6] * a0.w + I1[7] * a1.x + I1[8] * a1.y);
+        f2[3] += e * (I2[0] * a1.z + I2[1] * a1.w + I2[2] * a2.x);
+        f2[4] += e * (I2[3] * a1.z + I2[4] * a1.w + I2[5] * a2.x);
+        f2[5] += e * (I2[6] * a1.z + I2[7] * a1.w + I2[8] * a2.x);
+    }
+    lds_store_fc(s_body, b1, f1);
+    if (b2 != nb) lds_store_fc(s_body, b2, f2);
+}
+
+__device__ float world_residual_lds(const float *s_rows, const float *s_lam, const float *s_body, int m,
+                                    const StepParams &P) {
+    float mx = 0.f, lmax = 0.f;
+    const float iw = 1.f / P.sor_w;
+    for (int r = lane_id(); r < m; r += 64) {
+        const float4 *rr = reinterpret_cast<const float4 *>(s_rows) + r * 4;
+        const float4 a0 = rr[0], a1 = rr[1], a2 = rr[2], a3 = rr[3];
+        const int ids = as_int(a3.z);
+        const float *f1 = s_body + (ids & 255) * LDS_BODY, *f2 = s_body + ((ids >> 8) & 255) * LDS_BODY;
+        const int fidx = (ids >> 16) - 1;
+        const float l = s_lam[r];
+        float d = a2.y - l * a2.z;
+        d -= a0.x * (f1[0] - f2[0]) + a0.y * (f1[1] - f2[1]) + a0.z * (f1[2] - f2[2]);
+        d -= a0.w * f1[3] + a1.x * f1[4] + a1.y * f1[5];
+        d -= a1.z * f2[3] + a1.w * f2[4] + a2.x * f2[5];
+        d *= iw;
+        float lo = a3.x, hi = a3.y;
+        if (fidx >= 0) { hi = fabsf(hi * s_lam[fidx]); lo = -hi; }
+        const float nl = fminf(fmaxf(l + d, lo), hi);
+        mx = fmaxf(mx, fabsf(nl - l));
+        lmax = fmaxf(lmax, fabsf(l));
+    }
+    mx = wave_max(mx);
+    lmax = wave_max(lmax);
+    return mx / fmaxf(lmax, P.lambda_floor);
+}
+
+// fc += lambda iMJ for the rows' current lambda (warm start), LDS atomics.
+__device__ void lds_accumulate_fc(const float *s_rows, const float *s_lam, float *s_body, int nb, int m) {
+    for (int r = lane_id(); r < m; r += 64) {
+        const float l = s_lam[r];
+        if (l == 0.f) continue;
+        const float4 *rr = reinterpret_cast<const float4 *>(s_rows) + r * 4;
+        const float4 a0 = rr[0], a1 = rr[1], a2 = rr[2], a3 = rr[3];
+        const int ids = as_int(a3.z);
+        const int b1 = ids & 255, b2 = (ids >> 8) & 255;
+        float f[6], I[9], im;
+        const float e = l * a2.w;
+        lds_body(s_body, b1, f, im, I);
+        float *o = s_body + b1 * LDS_BODY;
+        atomicAdd(o + 0, e * im * a0.x); atomicAdd(o + 1, e * im * a0.y); atomicAdd(o + 2, e * im * a0.z);
+        atomicAdd(o + 3, e * (I[0] * a0.w + I[1] * a1.x + I[2] * a1.y));
+        atomicAdd(o + 4, e * (I[3] * a0.w + I[4] * a1.x + I[5] * a1.y));
+        atomicAdd(o + 5, e * (I[6] * a0.w + I[7] * a1.x + I[8] * a1.y));
+        if (b2 != nb) {
+            lds_body(s_body, b2, f, im, I);
+            o = s_body + b2 * LDS_BODY;
+            atomicAdd(o + 0, -e * im * a0.x); atomicAdd(o + 1, -e * im * a0.y); atomicAdd(o + 2, -e * im * a0.z);
+            atomicAdd(o + 3, e * (I[0] * a1.z + I[1] * a1.w + I[2] * a2.x));
+            atomicAdd(o + 4, e * (I[3] * a1.z + I[4] * a1.w + I[5] * a2.x));
+            atomicAdd(o + 5, e * (I[6] * a1.z + I[7] * a1.w + I[8] * a2.x));
+        }
+    }
+}
+
+// Stages the world's bodies, rows and sweep schedule (packed items, phase
+// starts) into LDS.  Returns false (wave-uniform) when a row breaks the
+// J2 linear = -J1 linear form; the caller then solves from global memory.
+__device__ bool lds_stage(const DevWorlds &D, int b0, int nb, int r0, int m, int L, const int *ps,
+                          const int *sched, float *s_body, float *s_rows, int *s_sched, unsigned short *s_ps) {
+    const int t = lane_id();
+    for (int p = t; p <= L; p += 64) s_ps[p] = (unsigned short)ps[p];
+    const int nitems = ps[L];
+    for (int k = t; k < nitems; k += 64) {
+        const int it = sched[k], rs = it >> 3;
+        const int2 ids = *reinterpret_cast<const int2 *>(D.rows + (size_t)(r0 + rs) * ROW_STRIDE + R_B1);
+        s_sched[k] = lds_item(rs, it & 7, ids.x, ids.y >= 0 ? ids.y : nb);
+    }
+    for (int i = t; i <= nb; i += 64) {
+        float4 *o = reinterpret_cast<float4 *>(s_body + i * LDS_BODY);
+        if (i < nb) {
+            const float *s = D.bscr + (size_t)(b0 + i) * BSCR_STRIDE;
+            o[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+            o[1] = make_float4(0.f, 0.f, s[30], s[12]);
+            o[2] = make_float4(s[13], s[14], s[16], s[17]);
+            o[3] = make_float4(s[18], s[20], s[21], s[22]);
+        } else {
+            o[0] = o[1] = o[2] = o[3] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+    int bad = 0;
+    for (int r = t; r < m; r += 64) {
+        const float4 *g = reinterpret_cast<const float4 *>(D.rows + (size_t)(r0 + r) * ROW_STRIDE);
+        const float4 a0 = g[0], a1 = g[1], a2 = g[2], a3 = g[3], a7 = g[7];
+        const int b1 = as_int(a7.x), b2 = as_int(a7.y), fidx = as_int(a7.z);
+        if (b2 >= 0 && (a2.x != -a0.x || a2.y != -a0.y || a2.z != -a0.z)) bad = 1;
+        const int ids = b1 | ((b2 >= 0 ? b2 : nb) << 8) | ((fidx + 1) << 16);
+        float4 *o = reinterpret_cast<float4 *>(s_rows) + r * 4;
+        o[0] = make_float4(a0.x, a0.y, a0.z, a0.w);
+        o[1] = make_float4(a1.x, a1.y, a2.w, a3.x);
+        o[2] = make_float4(a3.y, a1.z, a1.w, 1.f / a7.w);
+        o[3] = make_float4(a3.z, a3.w, as_float(ids), 0.f);
+    }
+    return !__any(bad);
+}
+
+// Dispatch order of the solve: worlds by descending sweep count of their
+// previous solve (a counting sort over 1024 buckets of 4 sweeps), so the
+// longest solves start first and short ones fill in behind them.  Only the
+// assignment of worlds to workgroups changes, never a world's arithmetic.
+constexpr int ORDER_BUCKETS = 1024;
+__global__ __launch_bounds__(1024) void k_order_worlds(const int *niters, int nworlds, int *order) {
+    __shared__ int s_h[ORDER_BUCKETS + 1];
     const int t = threadIdx.x;
-    __shared__ float s_fc[SW_MAXB * 6];
+    for (int i = t; i <= ORDER_BUCKETS; i += blockDim.x) s_h[i] = 0;
+    __syncthreads();
+    for (int w = t; w < nworlds; w += blockDim.x) {
+        const int key = ORDER_BUCKETS - 1 - min(max(niters[w], 0) >> 2, ORDER_BUCKETS - 1);
+        atomicAdd(&s_h[key + 1], 1);
+    }
+    __syncthreads();
+    if (t == 0)
+        for (int i = 1; i <= ORDER_BUCKETS; i++) s_h[i] += s_h[i - 1];
+    __syncthreads();
+    for (int w = t; w < nworlds; w += blockDim.x) {
+        const int key = ORDER_BUCKETS - 1 - min(max(niters[w], 0) >> 2, ORDER_BUCKETS - 1);
+        order[atomicAdd(&s_h[key], 1)] = w;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_solve_small(DevWorlds D, StepParams P) {
+    const int w = D.order ? D.order[blockIdx.x] : (int)blockIdx.x;
+    const int t = threadIdx.x;
+    extern __shared__ float4 s_dyn4[];
+    float *s_dyn = reinterpret_cast<float *>(s_dyn4);
     const int b0 = D.body_start[w], nb = D.body_start[w + 1] - b0;
     const int r0 = D.row_start[w];
     const int m = D.nrows[w];
@@ -841,65 +1060,135 @@ __global__ __launch_bounds__(64) void k_solve_small(DevWorlds D, StepParams P) {
     const int *ps = D.phase_start + r0 + w;
     const int *sched = D.sched + r0;
     float *lam = D.lambda + r0;
+    const bool warm = P.warm >= 1 && D.lambda0;
 
-    for (int k = t; k < nb * 6; k += 64) s_fc[k] = 0.f;
     if (w == 0 && t == 0 && D.feat_count) {  // feature entries were consumed; keep the count for readback
         D.feat_count[1] = D.feat_count[0];
         D.feat_count[0] = 0;
     }
-    if (P.warm >= 1 && D.lambda0) {
-        // lambda0 clamped to the row bounds; friction bounds use the clamped normal
-        for (int pass = 0; pass < 2; pass++) {
-            for (int r = t; r < m; r += 64) {
-                const float4 *rr = reinterpret_cast<const float4 *>(D.rows + (size_t)(r0 + r) * ROW_STRIDE);
-                const float4 a3 = rr[3], a7 = rr[7];
-                const int fidx = as_int(a7.z);
-                if ((fidx >= 0) != (pass == 1)) continue;
-                float lo = a3.z, hi = a3.w;
-                if (fidx >= 0) { hi = fabsf(hi * lam[fidx]); lo = -hi; }
-                lam[r] = fminf(fmaxf(D.lambda0[r0 + r], lo), hi);
-            }
-            __syncthreads();
-        }
-        for (int r = t; r < m; r += 64) {
-            const float4 *rr = reinterpret_cast<const float4 *>(D.rows + (size_t)(r0 + r) * ROW_STRIDE);
-            const float4 a4 = rr[4], a5 = rr[5], a6 = rr[6], a7 = rr[7];
-            const int b1 = as_int(a7.x), b2 = as_int(a7.y);
-            const float l = lam[r];
-            if (l != 0.f) {
-                atomicAdd(&s_fc[b1 * 6 + 0], l * a4.x); atomicAdd(&s_fc[b1 * 6 + 1], l * a4.y);
-                atomicAdd(&s_fc[b1 * 6 + 2], l * a4.z); atomicAdd(&s_fc[b1 * 6 + 3], l * a4.w);
-                atomicAdd(&s_fc[b1 * 6 + 4], l * a5.x); atomicAdd(&s_fc[b1 * 6 + 5], l * a5.y);
-                if (b2 >= 0) {
-                    atomicAdd(&s_fc[b2 * 6 + 0], l * a5.z); atomicAdd(&s_fc[b2 * 6 + 1], l * a5.w);
-                    atomicAdd(&s_fc[b2 * 6 + 2], l * a6.x); atomicAdd(&s_fc[b2 * 6 + 3], l * a6.y);
-                    atomicAdd(&s_fc[b2 * 6 + 4], l * a6.z); atomicAdd(&s_fc[b2 * 6 + 5], l * a6.w);
-                }
-            }
-        }
-    } else {
-        for (int r = t; r < m; r += 64) lam[r] = 0.f;
+    // LDS path: bodies [lds_bodies][16], rows [lds_rows][16], lambda [lds_rows],
+    // items [lds_rows], phase starts [lds_rows + 1] (16-bit)
+    float *s_body = s_dyn;
+    float *s_rows = s_dyn + P.lds_bodies * LDS_BODY;
+    float *s_lam = s_rows + P.lds_rows * LDS_ROW;
+    int *s_sched = reinterpret_cast<int *>(s_lam + P.lds_rows);
+    unsigned short *s_ps = reinterpret_cast<unsigned short *>(s_sched + P.lds_rows);
+    bool lds = m > 0 && m <= P.lds_rows && nb < P.lds_bodies;
+    if (lds) {
+        lds = lds_stage(D, b0, nb, r0, m, L, ps, sched, s_body, s_rows, s_sched, s_ps);
+        __syncthreads();
     }
-    __syncthreads();
-
     int it = 0;
     float res = -1.f;
-    if (m > 0) {
+    if (lds) {
+        if (warm) {
+            for (int pass = 0; pass < 2; pass++) {
+                for (int r = t; r < m; r += 64) {
+                    const float4 a3 = reinterpret_cast<const float4 *>(s_rows)[r * 4 + 3];
+                    const int fidx = (as_int(a3.z) >> 16) - 1;
+                    if ((fidx >= 0) != (pass == 1)) continue;
+                    float lo = a3.x, hi = a3.y;
+                    if (fidx >= 0) { hi = fabsf(hi * s_lam[fidx]); lo = -hi; }
+                    s_lam[r] = fminf(fmaxf(D.lambda0[r0 + r], lo), hi);
+                }
+                __syncthreads();
+            }
+            lds_accumulate_fc(s_rows, s_lam, s_body, nb, m);
+        } else {
+            for (int r = t; r < m; r += 64) s_lam[r] = 0.f;
+        }
+        __syncthreads();
+        bool restarted = false;
         while (it < P.max_iters) {
             for (int p = 0; p < L; p++) {
-                const int s = ps[p], e = ps[p + 1];
-                for (int k = s + t; k < e; k += 64) process_item(D, r0, sched[k], s_fc);
+                const int s = s_ps[p], e = s_ps[p + 1];
+                for (int k = s + t; k < e; k += 64) process_item_lds(s_rows, s_lam, s_body, nb, s_sched[k]);
                 __syncthreads();
             }
             it++;
             if (P.tol > 0.f && (it % P.check_every == 0 || it == P.max_iters)) {
-                res = world_residual(D, P, r0, m, s_fc);
+                res = world_residual_lds(s_rows, s_lam, s_body, m, P);
                 if (res <= P.tol) break;
+                if (warm && !restarted && P.restart_after > 0 && it >= P.restart_after) {
+                    // a warm start that has not converged restarts from lambda = 0
+                    restarted = true;
+                    for (int r = t; r < m; r += 64) s_lam[r] = 0.f;
+                    for (int i = t; i < nb; i += 64) {
+                        const float z[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                        lds_store_fc(s_body, i, z);
+                    }
+                    __syncthreads();
+                }
             }
         }
-        if (P.final_residual && res < 0.f) res = world_residual(D, P, r0, m, s_fc);
-    } else {
-        res = 0.f;
+        if (P.final_residual && res < 0.f) res = world_residual_lds(s_rows, s_lam, s_body, m, P);
+        for (int r = t; r < m; r += 64) lam[r] = s_lam[r];
+        __syncthreads();
+    }
+    // global-memory path: rows streamed from HBM, fc in LDS
+    float *s_fc = s_dyn;
+    if (!lds) {
+        for (int k = t; k < nb * 6; k += 64) s_fc[k] = 0.f;
+        if (warm) {
+            // lambda0 clamped to the row bounds; friction bounds use the clamped normal
+            for (int pass = 0; pass < 2; pass++) {
+                for (int r = t; r < m; r += 64) {
+                    const float4 *rr = reinterpret_cast<const float4 *>(D.rows + (size_t)(r0 + r) * ROW_STRIDE);
+                    const float4 a3 = rr[3], a7 = rr[7];
+                    const int fidx = as_int(a7.z);
+                    if ((fidx >= 0) != (pass == 1)) continue;
+                    float lo = a3.z, hi = a3.w;
+                    if (fidx >= 0) { hi = fabsf(hi * lam[fidx]); lo = -hi; }
+                    lam[r] = fminf(fmaxf(D.lambda0[r0 + r], lo), hi);
+                }
+                __syncthreads();
+            }
+            for (int r = t; r < m; r += 64) {
+                const float4 *rr = reinterpret_cast<const float4 *>(D.rows + (size_t)(r0 + r) * ROW_STRIDE);
+                const float4 a4 = rr[4], a5 = rr[5], a6 = rr[6], a7 = rr[7];
+                const int b1 = as_int(a7.x), b2 = as_int(a7.y);
+                const float l = lam[r];
+                if (l != 0.f) {
+                    atomicAdd(&s_fc[b1 * 6 + 0], l * a4.x); atomicAdd(&s_fc[b1 * 6 + 1], l * a4.y);
+                    atomicAdd(&s_fc[b1 * 6 + 2], l * a4.z); atomicAdd(&s_fc[b1 * 6 + 3], l * a4.w);
+                    atomicAdd(&s_fc[b1 * 6 + 4], l * a5.x); atomicAdd(&s_fc[b1 * 6 + 5], l * a5.y);
+                    if (b2 >= 0) {
+                        atomicAdd(&s_fc[b2 * 6 + 0], l * a5.z); atomicAdd(&s_fc[b2 * 6 + 1], l * a5.w);
+                        atomicAdd(&s_fc[b2 * 6 + 2], l * a6.x); atomicAdd(&s_fc[b2 * 6 + 3], l * a6.y);
+                        atomicAdd(&s_fc[b2 * 6 + 4], l * a6.z); atomicAdd(&s_fc[b2 * 6 + 5], l * a6.w);
+                    }
+                }
+            }
+        } else {
+            for (int r = t; r < m; r += 64) lam[r] = 0.f;
+        }
+        __syncthreads();
+
+        if (m > 0) {
+            bool restarted = false;
+            while (it < P.max_iters) {
+                for (int p = 0; p < L; p++) {
+                    const int s = ps[p], e = ps[p + 1];
+                    for (int k = s + t; k < e; k += 64) process_item(D, r0, sched[k], s_fc);
+                    __syncthreads();
+                }
+                it++;
+                if (P.tol > 0.f && (it % P.check_every == 0 || it == P.max_iters)) {
+                    res = world_residual(D, P, r0, m, s_fc);
+                    if (res <= P.tol) break;
+                    if (warm && !restarted && P.restart_after > 0 && it >= P.restart_after) {
+                        // a warm start that has not converged restarts from lambda = 0
+                        restarted = true;
+                        for (int r = t; r < m; r += 64) lam[r] = 0.f;
+                        for (int k = t; k < nb * 6; k += 64) s_fc[k] = 0.f;
+                        __syncthreads();
+                    }
+                }
+            }
+            if (P.final_residual && res < 0.f) res = world_residual(D, P, r0, m, s_fc);
+        } else {
+            res = 0.f;
+        }
     }
     if (t == 0) { D.niters[w] = it; D.residual[w] = res; }
 
@@ -949,7 +1238,7 @@ __global__ __launch_bounds__(64) void k_solve_small(DevWorlds D, StepParams P) {
         const int b = b0 + i;
         const float *s = D.bscr + (size_t)b * BSCR_STRIDE;
         float4 p4 = D.pos[b], q4 = D.quat[b], lv = D.lvel[b], av = D.avel[b], fa = D.facc[b], ta = D.tacc[b];
-        const float *fc = s_fc + i * 6;
+        const float *fc = lds ? s_body + i * LDS_BODY : s_fc + i * 6;
         if (m > 0) {
             lv.x += h * fc[0]; lv.y += h * fc[1]; lv.z += h * fc[2];
             av.x += h * fc[3]; av.y += h * fc[4]; av.z += h * fc[5];
@@ -1188,8 +1477,15 @@ namespace odx {
 void launch_prepare_small(const DevWorlds &D, const StepParams &P, hipStream_t s) {
     hipLaunchKernelGGL(k_prepare_small, dim3(D.nworlds), dim3(64), 0, s, D, P);
 }
+size_t solve_small_lds_bytes(const StepParams &P) {
+    const size_t lds = ((size_t)P.lds_bodies * LDS_BODY + (size_t)P.lds_rows * (LDS_ROW + 2)) * sizeof(float) +
+                       ((size_t)P.lds_rows + 1) * sizeof(unsigned short);
+    const size_t fc = (size_t)SW_MAXB * 6 * sizeof(float);
+    return lds > fc ? lds : fc;
+}
 void launch_solve_small(const DevWorlds &D, const StepParams &P, hipStream_t s) {
-    hipLaunchKernelGGL(k_solve_small, dim3(D.nworlds), dim3(64), 0, s, D, P);
+    if (D.order) hipLaunchKernelGGL(k_order_worlds, dim3(1), dim3(1024), 0, s, D.niters, D.nworlds, D.order);
+    hipLaunchKernelGGL(k_solve_small, dim3(D.nworlds), dim3(64), solve_small_lds_bytes(P), s, D, P);
 }
 }  // namespace odx
 

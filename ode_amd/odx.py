@@ -44,6 +44,7 @@ def lib():
         L.odxBatchCreate.argtypes = [i, ip, fp, ip, fp, ip, fp, dp, i]
         L.odxBatchDestroy.argtypes = [vp]
         L.odxBatchSetSolver.argtypes = [vp, i, i, ctypes.c_float, i, i]
+        L.odxBatchSetSolveOptions.argtypes = [vp, i, i]
         L.odxBatchStep.argtypes = [vp, i]
         L.odxBatchSync.argtypes = [vp]
         L.odxBatchTime.argtypes = [vp, i, fp]
@@ -144,6 +145,11 @@ class Batch:
     def set_solver(self, order=0, max_iters=20, tol=0.0, check_every=1, warm=0):
         _check(lib().odxBatchSetSolver(self.h, order, max_iters, float(tol), check_every, warm), "odxBatchSetSolver")
 
+    def set_solve_options(self, restart_after=0, lds_rows=400):
+        """restart_after: sweeps after which an unconverged warm start restarts
+        cold (0 = never); lds_rows: row budget of the LDS-resident sweeps."""
+        _check(lib().odxBatchSetSolveOptions(self.h, restart_after, lds_rows), "odxBatchSetSolveOptions")
+
     def step(self, n=1):
         _check(lib().odxBatchStep(self.h, n), "odxBatchStep")
 
@@ -216,6 +222,7 @@ def _setup_mlp_api(L):
     L.odxMlpCreate.argtypes = [ctypes.c_uint]
     L.odxMlpDestroy.argtypes = [vp]
     L.odxMlpSetNormalization.argtypes = [vp, fp]
+    L.odxMlpSetGate.argtypes = [vp, i]
     L.odxMlpGetParams.argtypes = [vp, fp]
     L.odxMlpSetParams.argtypes = [vp, fp]
     L.odxMlpSetData.argtypes = [vp, fp, fp, i]
@@ -261,6 +268,11 @@ class Mlp:
     def set_normalization(self, norm38):
         self.norm = np.ascontiguousarray(norm38, np.float32)
         _check(lib().odxMlpSetNormalization(self.h, _p(self.norm)), "odxMlpSetNormalization")
+
+    def set_gate(self, gate):
+        """1: matched contacts keep their previous impulses (the network predicts
+        new contacts only).  Takes effect for batches attached afterwards."""
+        _check(lib().odxMlpSetGate(self.h, int(gate)), "odxMlpSetGate")
 
     def params(self):
         out = np.zeros(NPARAM, np.float32)

@@ -74,11 +74,13 @@ def forward(p, X, norm):
 
 
 def predict(p, X, norm):
-    """Impulses in force units: s * (y * out_scale + out_mean), s = X[:, 15]."""
+    """Impulses in force units: s * (base + y * out_scale + out_mean), with
+    s = X[:, 15] and base = X[:, 7:10] (matched previous impulses / s)."""
     F = _f()
     norm = np.asarray(norm, F)
     y, _ = forward(p, X, norm)
-    return np.asarray(X, F)[:, 15:16] * (y * norm[35:38] + norm[32:35])
+    X = np.asarray(X, F)
+    return X[:, 15:16] * (X[:, 7:10] + y * norm[35:38] + norm[32:35])
 
 
 def gradients(p, X, T, norm):

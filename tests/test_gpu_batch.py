@@ -248,3 +248,63 @@ def test_warm_start_from_previous_lambda_cuts_sweeps():
     assert np.all(warm <= 2), warm
     assert np.mean(cold) > 5 * np.mean(warm)
     B.close()
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_lds_and_global_row_paths_agree(order):
+    """Rows swept from LDS and rows streamed from global memory (LDS budget 0)
+    give the same step within float32 rounding; a budget of 96 rows splits the
+    batch between the two paths."""
+    say("case: LDS vs global rows, order", order)
+    rng = np.random.default_rng(31 + order)
+    worlds = [S.uneven_world(rng, 8, 40) for _ in range(16)]
+    for w in worlds:
+        w.bodies[:, 2] = np.minimum(w.bodies[:, 2], 0.3)
+    prm = O.params()
+    odx = _odx()
+    runs = {}
+    for budget in (0, 96, 400):
+        B = odx.Batch(worlds, prm)
+        B.set_solver(order=order, max_iters=20, tol=0.0)
+        B.set_solve_options(lds_rows=budget)
+        B.step(1)
+        runs[budget] = (B.bodies(), [B.lam(wi) for wi in range(len(worlds))], B.stats()["rows"])
+        B.close()
+    rows = runs[0][2]
+    assert (rows > 96).any() and (rows <= 96).any(), rows
+    ref_b, ref_l, _ = runs[0]
+    for budget in (96, 400):
+        b, l, r = runs[budget]
+        np.testing.assert_array_equal(r, rows)
+        assert relerr(b[:, 0:3], ref_b[:, 0:3]) < 1e-6
+        v, vr = np.hstack([b[:, 8:11], b[:, 12:15]]), np.hstack([ref_b[:, 8:11], ref_b[:, 12:15]])
+        assert np.max(np.abs(v - vr)) / max(np.max(np.abs(vr)), 1.0) < 1e-5
+        for wi in range(len(worlds)):
+            assert relerr(l[wi], ref_l[wi]) < 1e-4, (budget, wi, relerr(l[wi], ref_l[wi]))
+
+
+@pytest.mark.parametrize("lds_rows", [0, 400])
+def test_unconverged_warm_start_restarts_cold(lds_rows):
+    """A warm start that misses the residual target by sweep 6 restarts from
+    lambda = 0 and then repeats the cold solve exactly: 6 more sweeps and the
+    same impulses."""
+    say("case: warm-start restart, lds_rows", lds_rows)
+    rng = np.random.default_rng(17)
+    worlds = [S.tower_world(rng, levels=8) for _ in range(6)]
+    prm = O.params()
+    odx = _odx()
+    out = {}
+    for mode in ("cold", "restart"):
+        B = odx.Batch(worlds, prm)
+        B.set_solve_options(restart_after=6 if mode == "restart" else 0, lds_rows=lds_rows)
+        if mode == "restart":
+            B.set_lambda0(np.random.default_rng(3).uniform(-50, 50, size=B.row_start[-1]).astype(np.float32))
+        B.set_solver(order=1, max_iters=3000, tol=1e-4, check_every=2, warm=1 if mode == "restart" else 0)
+        B.step(1)
+        st = B.stats()
+        assert np.all(st["residual"] <= 1e-4)
+        out[mode] = (st["iters"].copy(), [B.lam(wi) for wi in range(len(worlds))])
+        B.close()
+    np.testing.assert_array_equal(out["restart"][0], out["cold"][0] + 6)
+    for a, b in zip(out["restart"][1], out["cold"][1]):
+        np.testing.assert_array_equal(a, b)
