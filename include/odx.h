@@ -84,8 +84,14 @@ int odxBatchSetSolver(odxBatch *b, int order, int max_iters, float tol, int chec
  * Fails when lds_rows does not fit the kernel's 64 KB LDS budget. */
 int odxBatchSetSolveOptions(odxBatch *b, int restart_after, int lds_rows);
 
-/* Runs nsteps steps on the batch's stream (captured once into a hipGraph and
- * replayed).  Returns 1 on success. */
+/* Splits the worlds into nsplit (1..8) equal index ranges that step on streams
+ * of their own inside one hipGraph of graph_steps steps: the worlds are
+ * independent, so while one range finishes its slowest solve the others keep
+ * the GPU busy with their next steps.  Results are those of nsplit = 1. */
+int odxBatchSetSplits(odxBatch *b, int nsplit, int graph_steps);
+
+/* Runs nsteps steps on the batch's stream (captured once into a hipGraph of
+ * graph_steps steps and replayed).  Returns 1 on success. */
 int odxBatchStep(odxBatch *b, int nsteps);
 /* Blocks until queued work is done. */
 int odxBatchSync(odxBatch *b);
@@ -111,6 +117,9 @@ int odxBatchGetLambda(odxBatch *b, int w, float *lambda, int cap);
 int odxBatchGetRows(odxBatch *b, int w, float *rows, int cap);
 /* Overflow counters: [pairs, contacts, rows, phases] dropped since creation. */
 int odxBatchGetOverflow(odxBatch *b, int *counts);
+/* Start and end (microseconds from the earliest start) of each world's solve
+ * in the last step, [nworlds][2], from the device wall clock. */
+int odxBatchGetSolveTimes(odxBatch *b, double *start_end_us);
 /* Row-capacity offsets [nworlds+1] (the lambda0 layout). */
 int odxBatchRowStarts(odxBatch *b, int *row_start);
 /* Warm-start values per row in the row-capacity layout (host array). */

@@ -97,17 +97,30 @@ template <class T> static bool download(T *h, const T *d, size_t n) {
     return odx_check(hipMemcpy(h, d, n * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy D2H");
 }
 
+// The batch's worlds may be split into ranges stepped on streams of their own
+// inside one graph: worlds are independent, so while one range finishes its
+// slowest solve the others keep the GPU busy with their next steps.
+struct odxSplit {
+    int w0 = 0, w1 = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+};
+
 struct odxBatch {
     int nw = 0, nb = 0, ng = 0, nball = 0, jcap = 0, rcap = 0;
     std::vector<int> body_start, geom_start, ball_start, joint_start, row_start;
     std::vector<void *> allocs;
     DevWorlds D{};
-    StepParams P{};
+    StepParams P{};  // solver settings; wbase / wcount / split / feat_base are set per launch
     Surface surf{};
     hipStream_t stream = nullptr;
-    hipGraphExec_t graph = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<odxSplit> splits;  // empty: the whole batch on `stream`
+    int graph_steps = 1;           // steps per graph launch
+    hipGraphExec_t graph = nullptr, graph1 = nullptr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     float *lambda0 = nullptr;
+    int samples_split = 0;  // counter pair of the launch mode that ran last (for odxBatchGetSamples)
 
     template <class T> bool alloc(T **p, size_t n) {
         if (!dalloc(p, n)) return false;
@@ -115,27 +128,85 @@ struct odxBatch {
         return true;
     }
     ~odxBatch() {
-        if (graph) (void)hipGraphExecDestroy(graph);
+        drop_graph();
+        drop_splits();
         for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
         if (stream) (void)hipStreamDestroy(stream);
         for (void *p : allocs) (void)hipFree(p);
     }
     void drop_graph() {
         if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }
+        if (graph1) { (void)hipGraphExecDestroy(graph1); graph1 = nullptr; }
+    }
+    void drop_splits() {
+        for (auto &s : splits) {
+            if (s.done) (void)hipEventDestroy(s.done);
+            if (s.stream) (void)hipStreamDestroy(s.stream);
+        }
+        splits.clear();
+    }
+    // launch parameters of worlds [w0, w1) using counter pair `split`
+    StepParams launch_params(int w0, int w1, int split) const {
+        StepParams q = P;
+        q.wbase = w0;
+        q.wcount = w1 - w0;
+        q.split = split;
+        q.feat_base = joint_start[w0];
+        return q;
     }
     const odxMlp *mlp = nullptr;
     bool infer_on() const { return P.warm == 3 && mlp; }
-    bool enqueue_infer() {
+    bool enqueue_infer(const StepParams &q, hipStream_t s) {
         if (!infer_on()) return true;
-        launch_mlp_infer(mlp, D.feat, D.feat_row, D.feat_count, D.lambda0, jcap, stream);
+        launch_mlp_infer(mlp, D.feat + (size_t)q.feat_base * FEAT_DIM, D.feat_row + q.feat_base, D.feat_count + 2 * q.split,
+                         D.lambda0, joint_start[q.wbase + q.wcount] - q.feat_base, s);
         return odx_check(hipGetLastError(), "k_mlp_infer launch");
     }
-    bool enqueue_step() {
-        launch_prepare_small(D, P, stream);
+    bool enqueue_step(const StepParams &q, hipStream_t s) {
+        launch_prepare_small(D, q, s);
         if (!odx_check(hipGetLastError(), "k_prepare_small launch")) return false;
-        if (!enqueue_infer()) return false;
-        launch_solve_small(D, P, stream);
+        if (!enqueue_infer(q, s)) return false;
+        launch_solve_small(D, q, s);
         return odx_check(hipGetLastError(), "k_solve_small launch");
+    }
+    // nsteps steps of every world, captured on `stream` (forked over the splits)
+    bool enqueue_steps(int nsteps) {
+        if (splits.empty()) {
+            const StepParams q = launch_params(0, nw, 0);
+            for (int k = 0; k < nsteps; k++)
+                if (!enqueue_step(q, stream)) return false;
+            samples_split = 0;
+            return true;
+        }
+        if (!odx_check(hipEventRecord(fork, stream), "hipEventRecord")) return false;
+        for (size_t i = 0; i < splits.size(); i++) {
+            odxSplit &sp = splits[i];
+            if (!odx_check(hipStreamWaitEvent(sp.stream, fork, 0), "hipStreamWaitEvent")) return false;
+            const StepParams q = launch_params(sp.w0, sp.w1, (int)i);
+            for (int k = 0; k < nsteps; k++)
+                if (!enqueue_step(q, sp.stream)) return false;
+            if (!odx_check(hipEventRecord(sp.done, sp.stream), "hipEventRecord") ||
+                !odx_check(hipStreamWaitEvent(stream, sp.done, 0), "hipStreamWaitEvent"))
+                return false;
+        }
+        samples_split = -1;
+        return true;
+    }
+    bool capture(int nsteps, hipGraphExec_t *out) {
+        hipGraph_t g = nullptr;
+        if (!odx_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
+            return false;
+        bool ok = enqueue_steps(nsteps);
+        hipError_t e = hipStreamEndCapture(stream, &g);
+        if (!ok || !odx_check(e, "hipStreamEndCapture")) {
+            if (g) (void)hipGraphDestroy(g);
+            return false;
+        }
+        e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (!odx_check(e, "hipGraphInstantiate")) { *out = nullptr; return false; }
+        return true;
     }
 };
 
@@ -338,11 +409,12 @@ odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies
          B->alloc(&D.nphases, nworlds) && B->alloc(&D.niters, nworlds) && B->alloc(&D.residual, nworlds);
     ok = ok && B->alloc(&D.rows, (size_t)B->rcap * ROW_STRIDE) && B->alloc(&D.lambda, B->rcap);
     ok = ok && B->alloc(&D.sched, B->rcap) && B->alloc(&D.phase_start, (size_t)B->rcap + nworlds);
+    ok = ok && B->alloc(&D.perm, B->rcap) && B->alloc(&D.inv, B->rcap);
     ok = ok && B->alloc(&D.bscr, (size_t)nb * BSCR_STRIDE);
     ok = ok && B->alloc(&D.feat, (size_t)B->jcap * FEAT_DIM) && B->alloc(&D.feat_row, B->jcap) &&
-         B->alloc(&D.feat_count, 2) && B->alloc(&D.joff, B->jcap);
+         B->alloc(&D.feat_count, 2 * (ODX_MAX_SPLITS + 1)) && B->alloc(&D.joff, B->jcap);
     ok = ok && B->alloc(&D.prev, (size_t)B->jcap * JOINT_STRIDE) && B->alloc(&D.nprev, nworlds);
-    ok = ok && B->alloc(&D.overflow, 4) && B->alloc(&D.order, nworlds);
+    ok = ok && B->alloc(&D.overflow, 4) && B->alloc(&D.order, nworlds) && B->alloc(&D.wtime, 2 * (size_t)nworlds);
     ok = ok && B->alloc(&B->lambda0, B->rcap);
     if (ok) {
         ok = odx_check(hipMemset(D.overflow, 0, 4 * sizeof(int)), "hipMemset") &&
@@ -350,13 +422,14 @@ odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies
              odx_check(hipMemset(D.ncontacts, 0, nworlds * sizeof(int)), "hipMemset") &&
              odx_check(hipMemset(D.niters, 0, nworlds * sizeof(int)), "hipMemset") &&
              odx_check(hipMemset(D.nprev, 0, nworlds * sizeof(int)), "hipMemset") &&
-             odx_check(hipMemset(D.feat_count, 0, 2 * sizeof(int)), "hipMemset") &&
+             odx_check(hipMemset(D.feat_count, 0, 2 * (ODX_MAX_SPLITS + 1) * sizeof(int)), "hipMemset") &&
              odx_check(hipMemset(B->lambda0, 0, (size_t)B->rcap * sizeof(float)), "hipMemset") &&
              odx_check(hipMemset(D.lambda, 0, (size_t)B->rcap * sizeof(float)), "hipMemset");
     }
     D.lambda0 = B->lambda0;
     ok = ok && odx_check(hipStreamCreate(&B->stream), "hipStreamCreate");
     for (auto &e : B->ev) ok = ok && odx_check(hipEventCreate(&e), "hipEventCreate");
+    ok = ok && odx_check(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming), "hipEventCreate");
     if (!ok) {
         delete B;
         return nullptr;
@@ -397,25 +470,42 @@ int odxBatchSetSolveOptions(odxBatch *b, int restart_after, int lds_rows) {
     return 1;
 }
 
+int odxBatchSetSplits(odxBatch *b, int nsplit, int graph_steps) {
+    if (!b || nsplit < 1 || nsplit > ODX_MAX_SPLITS || nsplit > b->nw || graph_steps < 1) {
+        odx_message(1, "odxBatchSetSplits: bad arguments (1 <= nsplit <= %d and <= worlds, graph_steps >= 1)",
+                    ODX_MAX_SPLITS);
+        return 0;
+    }
+    if (!odxBatchSync(b)) return 0;
+    b->drop_graph();
+    b->drop_splits();
+    b->graph_steps = graph_steps;
+    if (nsplit == 1) return 1;
+    for (int i = 0; i < nsplit; i++) {
+        odxSplit s;
+        s.w0 = (int)((long long)b->nw * i / nsplit);
+        s.w1 = (int)((long long)b->nw * (i + 1) / nsplit);
+        if (!odx_check(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate") ||
+            !odx_check(hipEventCreateWithFlags(&s.done, hipEventDisableTiming), "hipEventCreate")) {
+            if (s.stream) (void)hipStreamDestroy(s.stream);
+            b->drop_splits();
+            return 0;
+        }
+        b->splits.push_back(s);
+    }
+    return 1;
+}
+
 int odxBatchStep(odxBatch *b, int nsteps) {
     if (!b || nsteps < 0) return 0;
     if (nsteps == 0) return 1;
-    if (!b->graph) {
-        hipGraph_t g = nullptr;
-        if (!odx_check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
-            return 0;
-        bool ok = b->enqueue_step();
-        hipError_t e = hipStreamEndCapture(b->stream, &g);
-        if (!ok || !odx_check(e, "hipStreamEndCapture")) {
-            if (g) (void)hipGraphDestroy(g);
-            return 0;
-        }
-        e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (!odx_check(e, "hipGraphInstantiate")) { b->graph = nullptr; return 0; }
-    }
-    for (int s = 0; s < nsteps; s++)
+    const int K = b->graph_steps;
+    if (nsteps >= K && !b->graph && !b->capture(K, &b->graph)) return 0;
+    if (nsteps % K && !b->graph1 && !b->capture(1, &b->graph1)) return 0;
+    for (int s = 0; s + K <= nsteps; s += K)
         if (!odx_check(hipGraphLaunch(b->graph, b->stream), "hipGraphLaunch")) return 0;
+    for (int s = 0; s < nsteps % K; s++)
+        if (!odx_check(hipGraphLaunch(b->graph1, b->stream), "hipGraphLaunch")) return 0;
     return 1;
 }
 
@@ -426,9 +516,11 @@ int odxBatchSync(odxBatch *b) {
 
 int odxBatchTime(odxBatch *b, int nsteps, float *total_ms) {
     if (!b || nsteps <= 0) return 0;
-    if (!odxBatchStep(b, 0) || !odxBatchSync(b)) return 0;
-    if (!b->graph && !odxBatchStep(b, 1)) return 0;  // instantiate outside the timed region
     if (!odxBatchSync(b)) return 0;
+    // instantiate the graphs outside the timed region
+    const int K = b->graph_steps;
+    if (nsteps >= K && !b->graph && !b->capture(K, &b->graph)) return 0;
+    if (nsteps % K && !b->graph1 && !b->capture(1, &b->graph1)) return 0;
     if (!odx_check(hipEventRecord(b->ev[0], b->stream), "hipEventRecord")) return 0;
     if (!odxBatchStep(b, nsteps)) return 0;
     if (!odx_check(hipEventRecord(b->ev[1], b->stream), "hipEventRecord")) return 0;
@@ -446,16 +538,20 @@ int odxBatchKernelTimes(odxBatch *b, int nsteps, float *kernel_ms) {
     std::vector<hipEvent_t> ev(3 * (size_t)nsteps + 1, nullptr);
     bool ok = true;
     for (auto &e : ev) ok = ok && odx_check(hipEventCreate(&e), "hipEventCreate");
+    // whole-batch launches (the kernels' full grids) on the batch stream, with
+    // a counter pair of their own
+    const StepParams q = b->launch_params(0, b->nw, ODX_MAX_SPLITS);
     for (int s = 0; ok && s < nsteps; s++) {
         ok = ok && odx_check(hipEventRecord(ev[3 * s], b->stream), "hipEventRecord");
-        launch_prepare_small(b->D, b->P, b->stream);
+        launch_prepare_small(b->D, q, b->stream);
         ok = ok && odx_check(hipGetLastError(), "k_prepare_small launch");
         ok = ok && odx_check(hipEventRecord(ev[3 * s + 1], b->stream), "hipEventRecord");
-        ok = ok && b->enqueue_infer();
+        ok = ok && b->enqueue_infer(q, b->stream);
         ok = ok && odx_check(hipEventRecord(ev[3 * s + 2], b->stream), "hipEventRecord");
-        launch_solve_small(b->D, b->P, b->stream);
+        launch_solve_small(b->D, q, b->stream);
         ok = ok && odx_check(hipGetLastError(), "k_solve_small launch");
     }
+    b->samples_split = ODX_MAX_SPLITS;
     ok = ok && odx_check(hipEventRecord(ev[3 * (size_t)nsteps], b->stream), "hipEventRecord");
     ok = ok && odx_check(hipEventSynchronize(ev[3 * (size_t)nsteps]), "hipEventSynchronize");
     double ta = 0.0, ti = 0.0, tc = 0.0;
@@ -571,6 +667,21 @@ int odxBatchGetOverflow(odxBatch *b, int *counts) {
     return download(counts, b->D.overflow, 4) ? 4 : 0;
 }
 
+int odxBatchGetSolveTimes(odxBatch *b, double *start_end_us) {
+    if (!b || !start_end_us) return 0;
+    if (!odxBatchSync(b)) return 0;
+    std::vector<long long> t(2 * (size_t)b->nw);
+    if (!download(t.data(), b->D.wtime, t.size())) return 0;
+    int dev = 0, khz = 0;
+    if (!odx_check(hipGetDevice(&dev), "hipGetDevice") ||
+        !odx_check(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "hipDeviceGetAttribute") || khz <= 0)
+        return 0;
+    long long t0 = t[0];
+    for (long long v : t) t0 = std::min(t0, v);
+    for (size_t i = 0; i < t.size(); i++) start_end_us[i] = (double)(t[i] - t0) * 1e3 / (double)khz;
+    return b->nw;
+}
+
 int odxBatchRowStarts(odxBatch *b, int *row_start) {
     if (!b || !row_start) return 0;
     std::memcpy(row_start, b->row_start.data(), (b->nw + 1) * sizeof(int));
@@ -587,18 +698,28 @@ int odxBatchSetMlp(odxBatch *b, const odxMlp *mlp) {
 int odxBatchGetSamples(odxBatch *b, float *X, float *T, int cap) {
     if (!b || !X || !T || cap <= 0) return 0;
     if (!odxBatchSync(b)) return 0;
-    int cnt[2] = {0, 0};
-    if (!download(cnt, b->D.feat_count, 2)) return 0;
-    const int n = cnt[1] < cap ? cnt[1] : cap;
-    if (n <= 0) return 0;
-    std::vector<int> rows(n);
+    std::vector<int> cnt(2 * (ODX_MAX_SPLITS + 1), 0);
+    if (!download(cnt.data(), b->D.feat_count, cnt.size())) return 0;
+    // feature ranges (first slot, count) of the launches that ran last
+    std::vector<std::pair<int, int>> ranges;
+    if (b->samples_split >= 0) ranges.push_back({0, cnt[2 * b->samples_split + 1]});
+    else
+        for (size_t i = 0; i < b->splits.size(); i++) ranges.push_back({b->joint_start[b->splits[i].w0], cnt[2 * i + 1]});
     std::vector<float> lam(b->rcap);
-    if (!download(X, b->D.feat, (size_t)n * FEAT_DIM) || !download(rows.data(), b->D.feat_row, n) ||
-        !download(lam.data(), b->D.lambda, b->rcap))
-        return 0;
-    for (int i = 0; i < n; i++) {
-        const float s = X[(size_t)i * FEAT_DIM + 15];
-        for (int o = 0; o < 3; o++) T[(size_t)i * 3 + o] = lam[rows[i] + o] / s;
+    if (!download(lam.data(), b->D.lambda, b->rcap)) return 0;
+    int n = 0;
+    for (const auto &rg : ranges) {
+        const int k = std::min(rg.second, cap - n);
+        if (k <= 0) break;
+        std::vector<int> rows(k);
+        if (!download(X + (size_t)n * FEAT_DIM, b->D.feat + (size_t)rg.first * FEAT_DIM, (size_t)k * FEAT_DIM) ||
+            !download(rows.data(), b->D.feat_row + rg.first, k))
+            return 0;
+        for (int i = 0; i < k; i++) {
+            const float s = X[(size_t)(n + i) * FEAT_DIM + 15];
+            for (int o = 0; o < 3; o++) T[(size_t)(n + i) * 3 + o] = lam[rows[i] + o] / s;
+        }
+        n += k;
     }
     return n;
 }

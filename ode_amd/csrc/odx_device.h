@@ -63,7 +63,13 @@ struct StepParams {
     int restart_after;   // > 0: a warm-started world not at tol after this many sweeps restarts from lambda = 0
     int lds_rows;        // batched solve: worlds with at most this many rows sweep from LDS (0: never)
     int lds_bodies;      // LDS body records per world (max bodies per world + 1 zero body)
+    int wbase, wcount;   // batched launch covers worlds [wbase, wbase + wcount)
+    int split;           // feature counter pair of this launch: feat_count[2 * split], [2 * split + 1]
+    int feat_base;       // first feature slot of this launch (joint_start[wbase])
 };
+
+// Independent world ranges a batch may be split into (one stream each).
+constexpr int ODX_MAX_SPLITS = 8;
 
 // Largest dynamic LDS of the batched solve kernel (bytes per workgroup).
 constexpr int SOLVE_LDS_MAX = 65536;
@@ -110,7 +116,9 @@ struct DevWorlds {
     int *nprev;             // [W]
     int *overflow;          // [4] pairs / contacts / rows / phases overflow counters
     float *fb;              // [joint cap * 12] f1 t1 f2 t2 per joint (feedback mode)
+    int *perm, *inv;        // [row cap] sweep position of each row and its inverse (per world, row_start based)
     int *order;             // [W] solve dispatch order (longest previous solve first); null = index order
+    long long *wtime;       // [W][2] solve start / end of each world (device wall clock), may be null
 };
 
 // Geom view for the collision entry points of the single-world API:

@@ -31,6 +31,18 @@ __device__ __forceinline__ int lane_id() { return (int)threadIdx.x & 63; }
 __device__ __forceinline__ int as_int(float f) { return __float_as_int(f); }
 __device__ __forceinline__ float as_float(int i) { return __int_as_float(i); }
 
+// Sweep work item of the batched solve: first row in sweep order (11 bits),
+// row count (4 bits), body 1 and body 2 (7 bits each; body 2 = nb when the
+// rows have no second body).  Its rows are consecutive in sweep order.
+constexpr int ITEM_MAXROWS = 15;
+__device__ __forceinline__ int pack_item(int start, int nrow, int b1, int b2) {
+    return start | (nrow << 11) | (b1 << 15) | (b2 << 22);
+}
+__device__ __forceinline__ int item_start(int it) { return it & 2047; }
+__device__ __forceinline__ int item_rows(int it) { return (it >> 11) & 15; }
+__device__ __forceinline__ int item_b1(int it) { return (it >> 15) & 127; }
+__device__ __forceinline__ int item_b2(int it) { return (it >> 22) & 127; }
+
 __device__ __forceinline__ int wave_incl_scan(int v) {
     const int l = lane_id();
 #pragma unroll
@@ -468,7 +480,7 @@ __device__ void write_contact_lambda0(const DevWorlds &D, const float *rec, int 
 
 // ============================================================== kernel A ===
 __global__ __launch_bounds__(64) void k_prepare_small(DevWorlds D, StepParams P) {
-    const int w = blockIdx.x;
+    const int w = P.wbase + (int)blockIdx.x;
     const int t = threadIdx.x;
     __shared__ float s_ab[SW_MAXG][6];
     __shared__ signed char s_gb[SW_MAXG];
@@ -479,7 +491,9 @@ __global__ __launch_bounds__(64) void k_prepare_small(DevWorlds D, StepParams P)
     __shared__ unsigned char s_jm[SW_MAXROWS];
     __shared__ int s_cnt[SW_MAXPH + 1];
     __shared__ unsigned long long s_mask[SW_MAXB];
-    __shared__ int s_L;
+    __shared__ int s_L, s_nblk;
+    static_assert(SW_MAXPAIRS * sizeof(uint32_t) >= SW_MAXROWS * sizeof(unsigned short), "block starts alias s_pairs");
+    static_assert(SW_MAXG * 6 * sizeof(float) >= SW_MAXROWS, "block lengths alias s_ab");
 
     const int b0 = D.body_start[w], nb = D.body_start[w + 1] - b0;
     const int g0 = D.geom_start[w], ng = D.geom_start[w + 1] - g0;
@@ -641,10 +655,10 @@ __global__ __launch_bounds__(64) void k_prepare_small(DevWorlds D, StepParams P)
                 if (is_contact && !learn) write_contact_lambda0(D, rec, r0, ri, lp);
                 const unsigned long long bal = __ballot(learn);
                 int base = 0;
-                if (t == 0 && bal) base = atomicAdd(D.feat_count, __popcll(bal));
+                if (t == 0 && bal) base = atomicAdd(D.feat_count + 2 * P.split, __popcll(bal));
                 base = __shfl(base, 0, 64);
                 if (learn) {
-                    const int slot = base + __popcll(bal & ((1ull << t) - 1ull));
+                    const int slot = P.feat_base + base + __popcll(bal & ((1ull << t) - 1ull));
                     contact_features(D, rec, b0, lp, matched, D.feat + (size_t)slot * FEAT_DIM);
                     D.feat_row[slot] = r0 + ri;
                     // rows beyond the predicted three (none for this surface) start at zero
@@ -668,9 +682,14 @@ __global__ __launch_bounds__(64) void k_prepare_small(DevWorlds D, StepParams P)
     }
     __syncthreads();
 
-    // S6: schedule
+    // S6: schedule.  Work items are packed (first row, row count, body 1,
+    // body 2; pack_item) and address rows in "sweep order": row r of the world
+    // sits at sweep position perm[r] (inverse: inv), the order in which the
+    // solve kernel stages rows into LDS, so that every item's rows are
+    // contiguous.  Order 0 keeps rows in place (perm = identity).
     int *ps = D.phase_start + r0 + w;
     int *sched = D.sched + r0;
+    int *perm = D.perm + r0, *inv = D.inv + r0;
     if (P.order == 0) {
         // dependency levels over the ReorderPrep order (lane 0, LDS-resident)
         if (t == 0) {
@@ -705,18 +724,47 @@ __global__ __launch_bounds__(64) void k_prepare_small(DevWorlds D, StepParams P)
         __syncthreads();
         for (int r = t; r < mrow; r += 64) {
             const int pos = atomicAdd(&s_cnt[s_lvl[r]], 1);
-            sched[pos] = (r << 3) | 1;
+            sched[pos] = pack_item(r, 1, s_rb1[r], s_rb2[r] == 255 ? nb : s_rb2[r]);
+            perm[r] = r;
+            inv[r] = r;
         }
         if (t == 0) D.nphases[w] = L;
     } else {
-        // greedy joint colouring with per-body colour masks (lane 0)
+        // Contact blocks: consecutive joints on the same body pair (the up to
+        // four contacts of one geom pair) form one work item per pass, so a
+        // sweep has a few long items instead of many short phases.  Two passes
+        // like ReorderPrep (quickstep.cpp:2329-2355): each block's non-findex
+        // rows (normals, ball rows) in colour phases first, then its findex
+        // (friction-cone) rows in a second set of colour phases; interleaving
+        // a contact's friction rows with its normal row instead can settle into
+        // a limit cycle short of the residual target.
+        unsigned short *s_bstart = reinterpret_cast<unsigned short *>(s_pairs);  // free after S4
+        unsigned char *s_blen = reinterpret_cast<unsigned char *>(&s_ab[0][0]);
         if (t == 0) {
-            for (int b = 0; b < nb; b++) s_mask[b] = 0ull;
-            int L = 0, extra = 0;
+            int nblk = 0, pa = -2, pb = -2, acc_a = 0, acc_f = 0;
             for (int k = 0; k < nj_used; k++) {
                 const int r = s_joff[k];
                 const int a = s_rb1[r], bb = s_rb2[r];
-                unsigned long long used = s_mask[a] | (bb != 255 ? s_mask[bb] : 0ull);
+                int na, nf;
+                split_joint_rows(s_rfl + r, (int)s_jm[k], na, nf);
+                if (nblk > 0 && a == pa && bb == pb && acc_a + na <= ITEM_MAXROWS && acc_f + nf <= ITEM_MAXROWS) {
+                    s_blen[nblk - 1]++;
+                    acc_a += na;
+                    acc_f += nf;
+                } else {
+                    s_bstart[nblk] = (unsigned short)k;
+                    s_blen[nblk] = 1;
+                    nblk++;
+                    pa = a; pb = bb; acc_a = na; acc_f = nf;
+                }
+            }
+            // greedy block colouring with per-body colour masks
+            for (int b = 0; b < nb; b++) s_mask[b] = 0ull;
+            int L = 0, extra = 0;
+            for (int q = 0; q < nblk; q++) {
+                const int r = s_joff[s_bstart[q]];
+                const int a = s_rb1[r], bb = s_rb2[r];
+                const unsigned long long used = s_mask[a] | (bb != 255 ? s_mask[bb] : 0ull);
                 int c;
                 if (~used) {
                     c = __ffsll((long long)~used) - 1;
@@ -726,49 +774,58 @@ __global__ __launch_bounds__(64) void k_prepare_small(DevWorlds D, StepParams P)
                 } else {
                     c = 64 + extra++;  // no free colour: a phase of its own
                 }
-                s_lvl[k] = (unsigned short)c;
+                s_lvl[q] = (unsigned short)c;
                 if (c + 1 > L) L = c + 1;
             }
             s_L = L;
+            s_nblk = nblk;
         }
         __syncthreads();
-        // Two passes like ReorderPrep (quickstep.cpp:2329-2355): each joint's
-        // non-findex rows in the colour phases [0, L), then its findex
-        // (friction-cone) rows in phases [L, 2L); empty phases are dropped.
-        // Interleaving a contact's friction rows with its normal row instead
-        // can settle into a limit cycle short of the residual target.
-        const int L = s_L;
-        const int K = 2 * L <= SW_MAXPH ? 2 * L : L;
-        for (int p = t; p <= K; p += 64) s_cnt[p] = 0;
-        __syncthreads();
-        for (int k = t; k < nj_used; k += 64) {
-            int na = s_jm[k], nf = 0;
-            if (K > L) split_joint_rows(s_rfl + s_joff[k], (int)s_jm[k], na, nf);
-            atomicAdd(&s_cnt[s_lvl[k] + 1], 1);
-            if (nf) atomicAdd(&s_cnt[L + s_lvl[k] + 1], 1);
+        const int L = s_L, nblk = s_nblk;
+        // Emit the items colour by colour (pass A then pass B); lanes take
+        // blocks in order, a wave scan gives each item its slot and rows.
+        int nitem = 0, npos = 0, nph = 0;
+        for (int pass = 0; pass < 2; pass++) {
+            for (int c = 0; c < L; c++) {
+                int phase_items = 0;
+                for (int base = 0; base < nblk; base += 64) {
+                    const int q = base + t;
+                    int nrow = 0;
+                    if (q < nblk && s_lvl[q] == c) {
+                        const int k0 = s_bstart[q], k1 = k0 + s_blen[q];
+                        for (int k = k0; k < k1; k++) {
+                            int na, nf;
+                            split_joint_rows(s_rfl + s_joff[k], (int)s_jm[k], na, nf);
+                            nrow += pass == 0 ? na : nf;
+                        }
+                    }
+                    const unsigned long long bal = __ballot(nrow > 0);
+                    const int incl = wave_incl_scan(nrow);
+                    if (nrow > 0) {
+                        const int item = nitem + __popcll(bal & ((1ull << t) - 1ull));
+                        const int start = npos + incl - nrow;
+                        const int r = s_joff[s_bstart[q]];
+                        sched[item] = pack_item(start, nrow, s_rb1[r], s_rb2[r] == 255 ? nb : s_rb2[r]);
+                        int p = start;
+                        const int k0 = s_bstart[q], k1 = k0 + s_blen[q];
+                        for (int k = k0; k < k1; k++) {
+                            int na, nf;
+                            split_joint_rows(s_rfl + s_joff[k], (int)s_jm[k], na, nf);
+                            const int rr = s_joff[k] + (pass == 0 ? 0 : na), n = pass == 0 ? na : nf;
+                            for (int i = 0; i < n; i++, p++) { perm[rr + i] = p; inv[p] = rr + i; }
+                        }
+                    }
+                    phase_items += __popcll(bal);
+                    nitem += __popcll(bal);
+                    npos += wave_bcast_last(incl);
+                }
+                if (phase_items > 0 && t == 0) ps[nph] = nitem - phase_items;
+                if (phase_items > 0) nph++;
+            }
         }
-        __syncthreads();
         if (t == 0) {
-            int run = 0, q = 0;
-            for (int p = 0; p < K; p++) {
-                const int c = s_cnt[p + 1];
-                s_cnt[p] = run;  // insertion cursor of key p
-                if (c > 0) ps[q++] = run;
-                run += c;
-            }
-            ps[q] = run;
-            D.nphases[w] = q;
-        }
-        __syncthreads();
-        for (int k = t; k < nj_used; k += 64) {
-            int na = s_jm[k], nf = 0;
-            if (K > L) split_joint_rows(s_rfl + s_joff[k], (int)s_jm[k], na, nf);
-            const int pos = atomicAdd(&s_cnt[s_lvl[k]], 1);
-            sched[pos] = ((int)s_joff[k] << 3) | na;
-            if (nf) {
-                const int pf = atomicAdd(&s_cnt[L + s_lvl[k]], 1);
-                sched[pf] = (((int)s_joff[k] + na) << 3) | nf;
-            }
+            ps[nph] = nitem;
+            D.nphases[w] = nph;
         }
     }
     if (t == 0) D.nrows[w] = mrow;
@@ -808,6 +865,53 @@ __device__ __forceinline__ void process_item(const DevWorlds &D, int r0, int ite
         else if (nl > hi) { delta = hi - old; nl = hi; }
         lam[rs + q] = nl;
         if (q == 0) lam_first = nl;
+        f1[0] += delta * a4.x; f1[1] += delta * a4.y; f1[2] += delta * a4.z;
+        f1[3] += delta * a4.w; f1[4] += delta * a5.x; f1[5] += delta * a5.y;
+        if (b2 >= 0) {
+            f2[0] += delta * a5.z; f2[1] += delta * a5.w; f2[2] += delta * a6.x;
+            f2[3] += delta * a6.y; f2[4] += delta * a6.z; f2[5] += delta * a6.w;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) s_fc[b1 * 6 + k] = f1[k];
+    if (b2 >= 0) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) s_fc[b2 * 6 + k] = f2[k];
+    }
+}
+
+// Batched solve, rows streamed from global memory: one packed item (its rows
+// in sweep order, mapped back to rows through inv).
+__device__ __forceinline__ void process_item_packed(const DevWorlds &D, int r0, const int *inv, int nb, int item,
+                                                    float *s_fc) {
+    const int rs = item_start(item), nr = item_rows(item);
+    const int b1 = item_b1(item), b2 = item_b2(item) == nb ? -1 : item_b2(item);
+    float f1[6], f2[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 6; k++) f1[k] = s_fc[b1 * 6 + k];
+    if (b2 >= 0) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) f2[k] = s_fc[b2 * 6 + k];
+    }
+    float *lam = D.lambda + r0;
+    for (int q = 0; q < nr; q++) {
+        const int g = inv[rs + q];
+        const float4 *rr = reinterpret_cast<const float4 *>(D.rows + (size_t)(r0 + g) * ROW_STRIDE);
+        const float4 a0 = rr[0], a1 = rr[1], a2 = rr[2], a3 = rr[3], a4 = rr[4], a5 = rr[5], a6 = rr[6], a7 = rr[7];
+        const float old = lam[g];
+        float delta = a1.z - old * a1.w;
+        delta -= f1[0] * a0.x + f1[1] * a0.y + f1[2] * a0.z + f1[3] * a0.w + f1[4] * a1.x + f1[5] * a1.y;
+        if (b2 >= 0) delta -= f2[0] * a2.x + f2[1] * a2.y + f2[2] * a2.z + f2[3] * a2.w + f2[4] * a3.x + f2[5] * a3.y;
+        float lo = a3.z, hi = a3.w;
+        const int fidx = as_int(a7.z);
+        if (fidx >= 0) {
+            hi = fabsf(hi * lam[fidx]);
+            lo = -hi;
+        }
+        float nl = old + delta;
+        if (nl < lo) { delta = lo - old; nl = lo; }
+        else if (nl > hi) { delta = hi - old; nl = hi; }
+        lam[g] = nl;
         f1[0] += delta * a4.x; f1[1] += delta * a4.y; f1[2] += delta * a4.z;
         f1[3] += delta * a4.w; f1[4] += delta * a5.x; f1[5] += delta * a5.y;
         if (b2 >= 0) {
@@ -867,36 +971,54 @@ __device__ float world_residual(const DevWorlds &D, const StepParams &P, int r0,
 // without a second body points at the zero body at index nb.
 constexpr int LDS_ROW = 16, LDS_BODY = 16;
 
+// 64-byte records start on one of only two 16-bank boundaries of LDS, so the
+// lanes of a 128-bit read that touch different records would pile onto the
+// same banks.  The four 16-byte slots of record r are therefore stored in the
+// order j ^ ((r >> 1) & 3): records r .. r + 7 then read slot j from eight
+// different bank groups.
+__device__ __forceinline__ int sw4(int r, int j) { return r * 4 + (j ^ ((r >> 1) & 3)); }
+__device__ __forceinline__ float4 ld4(const float *base, int r, int j) {
+    return reinterpret_cast<const float4 *>(base)[sw4(r, j)];
+}
+__device__ __forceinline__ void st4(float *base, int r, int j, float4 v) {
+    reinterpret_cast<float4 *>(base)[sw4(r, j)] = v;
+}
+// element k (0..15) of record r
+__device__ __forceinline__ float *rec_elem(float *base, int r, int k) { return base + 4 * sw4(r, k >> 2) + (k & 3); }
+
+__device__ __forceinline__ void lds_fc(const float *s_body, int b, float *fc) {
+    const float4 a = ld4(s_body, b, 0);
+    const float2 c = *reinterpret_cast<const float2 *>(s_body + 4 * sw4(b, 1));
+    fc[0] = a.x; fc[1] = a.y; fc[2] = a.z; fc[3] = a.w; fc[4] = c.x; fc[5] = c.y;
+}
+
 __device__ __forceinline__ void lds_body(const float *s_body, int b, float *fc, float &im, float *I) {
-    const float4 *p = reinterpret_cast<const float4 *>(s_body + b * LDS_BODY);
-    const float4 a = p[0], c = p[1], d = p[2], e = p[3];
+    const float4 a = ld4(s_body, b, 0), c = ld4(s_body, b, 1), d = ld4(s_body, b, 2), e = ld4(s_body, b, 3);
     fc[0] = a.x; fc[1] = a.y; fc[2] = a.z; fc[3] = a.w; fc[4] = c.x; fc[5] = c.y;
     im = c.z;
     I[0] = c.w; I[1] = d.x; I[2] = d.y; I[3] = d.z; I[4] = d.w; I[5] = e.x; I[6] = e.y; I[7] = e.z; I[8] = e.w;
 }
 
 __device__ __forceinline__ void lds_store_fc(float *s_body, int b, const float *fc) {
-    float4 *p = reinterpret_cast<float4 *>(s_body + b * LDS_BODY);
-    p[0] = make_float4(fc[0], fc[1], fc[2], fc[3]);
-    *reinterpret_cast<float2 *>(p + 1) = make_float2(fc[4], fc[5]);
-}
-
-// LDS work item: first row (11 bits), row count (3 bits), body 1 and body 2
-// (7 bits each), so the body and row loads of an item issue together.
-__device__ __forceinline__ int lds_item(int rs, int nr, int b1, int b2) {
-    return rs | (nr << 11) | (b1 << 14) | (b2 << 21);
+    st4(s_body, b, 0, make_float4(fc[0], fc[1], fc[2], fc[3]));
+    *reinterpret_cast<float2 *>(s_body + 4 * sw4(b, 1)) = make_float2(fc[4], fc[5]);
 }
 
 __device__ __forceinline__ void process_item_lds(const float *s_rows, float *s_lam, float *s_body, int nb, int item) {
-    const int rs = item & 2047, nr = (item >> 11) & 7;
-    const int b1 = (item >> 14) & 127, b2 = (item >> 21) & 127;
-    const float4 *row4 = reinterpret_cast<const float4 *>(s_rows) + rs * 4;
+    const int rs = item_start(item), nr = item_rows(item);
+    const int b1 = item_b1(item), b2 = item_b2(item);
     float f1[6], f2[6], I1[9], I2[9], im1, im2;
+    // rows are read one ahead (issued with the body records for the first), so
+    // each row's LDS latency overlaps the previous row's arithmetic
+    float4 a0 = ld4(s_rows, rs, 0), a1 = ld4(s_rows, rs, 1), a2 = ld4(s_rows, rs, 2), a3 = ld4(s_rows, rs, 3);
     lds_body(s_body, b1, f1, im1, I1);
     lds_body(s_body, b2, f2, im2, I2);
-    float lam_first = 0.f;
     for (int q = 0; q < nr; q++) {
-        const float4 a0 = row4[4 * q], a1 = row4[4 * q + 1], a2 = row4[4 * q + 2], a3 = row4[4 * q + 3];
+        float4 n0 = a0, n1 = a1, n2 = a2, n3 = a3;
+        if (q + 1 < nr) {
+            n0 = ld4(s_rows, rs + q + 1, 0); n1 = ld4(s_rows, rs + q + 1, 1);
+            n2 = ld4(s_rows, rs + q + 1, 2); n3 = ld4(s_rows, rs + q + 1, 3);
+        }
         const float old = s_lam[rs + q];
         float delta = a2.y - old * a2.z;
         delta -= a0.x * (f1[0] - f2[0]) + a0.y * (f1[1] - f2[1]) + a0.z * (f1[2] - f2[2]);
@@ -905,15 +1027,13 @@ __device__ __forceinline__ void process_item_lds(const float *s_rows, float *s_l
         float lo = a3.x, hi = a3.y;
         const int fidx = (as_int(a3.z) >> 16) - 1;
         if (fidx >= 0) {
-            const float lf = (fidx == rs) ? lam_first : s_lam[fidx];
-            hi = fabsf(hi * lf);
+            hi = fabsf(hi * s_lam[fidx]);
             lo = -hi;
         }
         float nl = old + delta;
         if (nl < lo) { delta = lo - old; nl = lo; }
         else if (nl > hi) { delta = hi - old; nl = hi; }
         s_lam[rs + q] = nl;
-        if (q == 0) lam_first = nl;
         const float e = delta * a2.w, e1 = e * im1, e2 = e * im2;
         f1[0] += e1 * a0.x; f1[1] += e1 * a0.y; f1[2] += e1 * a0.z;
         f2[0] -= e2 * a0.x; f2[1] -= e2 * a0.y; f2[2] -= e2 * a0.z;
@@ -923,6 +1043,7 @@ __device__ __forceinline__ void process_item_lds(const float *s_rows, float *s_l
         f2[3] += e * (I2[0] * a1.z + I2[1] * a1.w + I2[2] * a2.x);
         f2[4] += e * (I2[3] * a1.z + I2[4] * a1.w + I2[5] * a2.x);
         f2[5] += e * (I2[6] * a1.z + I2[7] * a1.w + I2[8] * a2.x);
+        a0 = n0; a1 = n1; a2 = n2; a3 = n3;
     }
     lds_store_fc(s_body, b1, f1);
     if (b2 != nb) lds_store_fc(s_body, b2, f2);
@@ -933,10 +1054,11 @@ __device__ float world_residual_lds(const float *s_rows, const float *s_lam, con
     float mx = 0.f, lmax = 0.f;
     const float iw = 1.f / P.sor_w;
     for (int r = lane_id(); r < m; r += 64) {
-        const float4 *rr = reinterpret_cast<const float4 *>(s_rows) + r * 4;
-        const float4 a0 = rr[0], a1 = rr[1], a2 = rr[2], a3 = rr[3];
+        const float4 a0 = ld4(s_rows, r, 0), a1 = ld4(s_rows, r, 1), a2 = ld4(s_rows, r, 2), a3 = ld4(s_rows, r, 3);
         const int ids = as_int(a3.z);
-        const float *f1 = s_body + (ids & 255) * LDS_BODY, *f2 = s_body + ((ids >> 8) & 255) * LDS_BODY;
+        float f1[6], f2[6];
+        lds_fc(s_body, ids & 255, f1);
+        lds_fc(s_body, (ids >> 8) & 255, f2);
         const int fidx = (ids >> 16) - 1;
         const float l = s_lam[r];
         float d = a2.y - l * a2.z;
@@ -960,52 +1082,51 @@ __device__ void lds_accumulate_fc(const float *s_rows, const float *s_lam, float
     for (int r = lane_id(); r < m; r += 64) {
         const float l = s_lam[r];
         if (l == 0.f) continue;
-        const float4 *rr = reinterpret_cast<const float4 *>(s_rows) + r * 4;
-        const float4 a0 = rr[0], a1 = rr[1], a2 = rr[2], a3 = rr[3];
+        const float4 a0 = ld4(s_rows, r, 0), a1 = ld4(s_rows, r, 1), a2 = ld4(s_rows, r, 2), a3 = ld4(s_rows, r, 3);
         const int ids = as_int(a3.z);
         const int b1 = ids & 255, b2 = (ids >> 8) & 255;
         float f[6], I[9], im;
         const float e = l * a2.w;
         lds_body(s_body, b1, f, im, I);
-        float *o = s_body + b1 * LDS_BODY;
-        atomicAdd(o + 0, e * im * a0.x); atomicAdd(o + 1, e * im * a0.y); atomicAdd(o + 2, e * im * a0.z);
-        atomicAdd(o + 3, e * (I[0] * a0.w + I[1] * a1.x + I[2] * a1.y));
-        atomicAdd(o + 4, e * (I[3] * a0.w + I[4] * a1.x + I[5] * a1.y));
-        atomicAdd(o + 5, e * (I[6] * a0.w + I[7] * a1.x + I[8] * a1.y));
+        atomicAdd(rec_elem(s_body, b1, 0), e * im * a0.x);
+        atomicAdd(rec_elem(s_body, b1, 1), e * im * a0.y);
+        atomicAdd(rec_elem(s_body, b1, 2), e * im * a0.z);
+        atomicAdd(rec_elem(s_body, b1, 3), e * (I[0] * a0.w + I[1] * a1.x + I[2] * a1.y));
+        atomicAdd(rec_elem(s_body, b1, 4), e * (I[3] * a0.w + I[4] * a1.x + I[5] * a1.y));
+        atomicAdd(rec_elem(s_body, b1, 5), e * (I[6] * a0.w + I[7] * a1.x + I[8] * a1.y));
         if (b2 != nb) {
             lds_body(s_body, b2, f, im, I);
-            o = s_body + b2 * LDS_BODY;
-            atomicAdd(o + 0, -e * im * a0.x); atomicAdd(o + 1, -e * im * a0.y); atomicAdd(o + 2, -e * im * a0.z);
-            atomicAdd(o + 3, e * (I[0] * a1.z + I[1] * a1.w + I[2] * a2.x));
-            atomicAdd(o + 4, e * (I[3] * a1.z + I[4] * a1.w + I[5] * a2.x));
-            atomicAdd(o + 5, e * (I[6] * a1.z + I[7] * a1.w + I[8] * a2.x));
+            atomicAdd(rec_elem(s_body, b2, 0), -e * im * a0.x);
+            atomicAdd(rec_elem(s_body, b2, 1), -e * im * a0.y);
+            atomicAdd(rec_elem(s_body, b2, 2), -e * im * a0.z);
+            atomicAdd(rec_elem(s_body, b2, 3), e * (I[0] * a1.z + I[1] * a1.w + I[2] * a2.x));
+            atomicAdd(rec_elem(s_body, b2, 4), e * (I[3] * a1.z + I[4] * a1.w + I[5] * a2.x));
+            atomicAdd(rec_elem(s_body, b2, 5), e * (I[6] * a1.z + I[7] * a1.w + I[8] * a2.x));
         }
     }
 }
 
-// Stages the world's bodies, rows and sweep schedule (packed items, phase
-// starts) into LDS.  Returns false (wave-uniform) when a row breaks the
-// J2 linear = -J1 linear form; the caller then solves from global memory.
+// Stages the world's bodies, rows (each at its sweep position perm[r], with
+// findex mapped the same way) and sweep schedule into LDS.  Returns false
+// (wave-uniform) when a row breaks the J2 linear = -J1 linear form; the
+// caller then solves from global memory.
 __device__ bool lds_stage(const DevWorlds &D, int b0, int nb, int r0, int m, int L, const int *ps,
-                          const int *sched, float *s_body, float *s_rows, int *s_sched, unsigned short *s_ps) {
+                          const int *sched, const int *perm, float *s_body, float *s_rows, int *s_sched,
+                          unsigned short *s_ps) {
     const int t = lane_id();
     for (int p = t; p <= L; p += 64) s_ps[p] = (unsigned short)ps[p];
     const int nitems = ps[L];
-    for (int k = t; k < nitems; k += 64) {
-        const int it = sched[k], rs = it >> 3;
-        const int2 ids = *reinterpret_cast<const int2 *>(D.rows + (size_t)(r0 + rs) * ROW_STRIDE + R_B1);
-        s_sched[k] = lds_item(rs, it & 7, ids.x, ids.y >= 0 ? ids.y : nb);
-    }
+    for (int k = t; k < nitems; k += 64) s_sched[k] = sched[k];
     for (int i = t; i <= nb; i += 64) {
-        float4 *o = reinterpret_cast<float4 *>(s_body + i * LDS_BODY);
+        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
         if (i < nb) {
             const float *s = D.bscr + (size_t)(b0 + i) * BSCR_STRIDE;
-            o[0] = make_float4(0.f, 0.f, 0.f, 0.f);
-            o[1] = make_float4(0.f, 0.f, s[30], s[12]);
-            o[2] = make_float4(s[13], s[14], s[16], s[17]);
-            o[3] = make_float4(s[18], s[20], s[21], s[22]);
+            st4(s_body, i, 0, z);
+            st4(s_body, i, 1, make_float4(0.f, 0.f, s[30], s[12]));
+            st4(s_body, i, 2, make_float4(s[13], s[14], s[16], s[17]));
+            st4(s_body, i, 3, make_float4(s[18], s[20], s[21], s[22]));
         } else {
-            o[0] = o[1] = o[2] = o[3] = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int j = 0; j < 4; j++) st4(s_body, i, j, z);
         }
     }
     int bad = 0;
@@ -1014,12 +1135,12 @@ __device__ bool lds_stage(const DevWorlds &D, int b0, int nb, int r0, int m, int
         const float4 a0 = g[0], a1 = g[1], a2 = g[2], a3 = g[3], a7 = g[7];
         const int b1 = as_int(a7.x), b2 = as_int(a7.y), fidx = as_int(a7.z);
         if (b2 >= 0 && (a2.x != -a0.x || a2.y != -a0.y || a2.z != -a0.z)) bad = 1;
-        const int ids = b1 | ((b2 >= 0 ? b2 : nb) << 8) | ((fidx + 1) << 16);
-        float4 *o = reinterpret_cast<float4 *>(s_rows) + r * 4;
-        o[0] = make_float4(a0.x, a0.y, a0.z, a0.w);
-        o[1] = make_float4(a1.x, a1.y, a2.w, a3.x);
-        o[2] = make_float4(a3.y, a1.z, a1.w, 1.f / a7.w);
-        o[3] = make_float4(a3.z, a3.w, as_float(ids), 0.f);
+        const int ids = b1 | ((b2 >= 0 ? b2 : nb) << 8) | (((fidx >= 0 ? perm[fidx] : -1) + 1) << 16);
+        const int p = perm[r];
+        st4(s_rows, p, 0, make_float4(a0.x, a0.y, a0.z, a0.w));
+        st4(s_rows, p, 1, make_float4(a1.x, a1.y, a2.w, a3.x));
+        st4(s_rows, p, 2, make_float4(a3.y, a1.z, a1.w, 1.f / a7.w));
+        st4(s_rows, p, 3, make_float4(a3.z, a3.w, as_float(ids), 0.f));
     }
     return !__any(bad);
 }
@@ -1029,13 +1150,13 @@ __device__ bool lds_stage(const DevWorlds &D, int b0, int nb, int r0, int m, int
 // longest solves start first and short ones fill in behind them.  Only the
 // assignment of worlds to workgroups changes, never a world's arithmetic.
 constexpr int ORDER_BUCKETS = 1024;
-__global__ __launch_bounds__(1024) void k_order_worlds(const int *niters, int nworlds, int *order) {
+__global__ __launch_bounds__(1024) void k_order_worlds(const int *niters, int wbase, int nworlds, int *order) {
     __shared__ int s_h[ORDER_BUCKETS + 1];
     const int t = threadIdx.x;
     for (int i = t; i <= ORDER_BUCKETS; i += blockDim.x) s_h[i] = 0;
     __syncthreads();
     for (int w = t; w < nworlds; w += blockDim.x) {
-        const int key = ORDER_BUCKETS - 1 - min(max(niters[w], 0) >> 2, ORDER_BUCKETS - 1);
+        const int key = ORDER_BUCKETS - 1 - min(max(niters[wbase + w], 0) >> 2, ORDER_BUCKETS - 1);
         atomicAdd(&s_h[key + 1], 1);
     }
     __syncthreads();
@@ -1043,13 +1164,13 @@ __global__ __launch_bounds__(1024) void k_order_worlds(const int *niters, int nw
         for (int i = 1; i <= ORDER_BUCKETS; i++) s_h[i] += s_h[i - 1];
     __syncthreads();
     for (int w = t; w < nworlds; w += blockDim.x) {
-        const int key = ORDER_BUCKETS - 1 - min(max(niters[w], 0) >> 2, ORDER_BUCKETS - 1);
-        order[atomicAdd(&s_h[key], 1)] = w;
+        const int key = ORDER_BUCKETS - 1 - min(max(niters[wbase + w], 0) >> 2, ORDER_BUCKETS - 1);
+        order[wbase + atomicAdd(&s_h[key], 1)] = wbase + w;
     }
 }
 
 __global__ __launch_bounds__(64) void k_solve_small(DevWorlds D, StepParams P) {
-    const int w = D.order ? D.order[blockIdx.x] : (int)blockIdx.x;
+    const int w = D.order ? D.order[P.wbase + (int)blockIdx.x] : P.wbase + (int)blockIdx.x;
     const int t = threadIdx.x;
     extern __shared__ float4 s_dyn4[];
     float *s_dyn = reinterpret_cast<float *>(s_dyn4);
@@ -1059,12 +1180,14 @@ __global__ __launch_bounds__(64) void k_solve_small(DevWorlds D, StepParams P) {
     const int L = D.nphases[w];
     const int *ps = D.phase_start + r0 + w;
     const int *sched = D.sched + r0;
+    const int *perm = D.perm + r0, *inv = D.inv + r0;
     float *lam = D.lambda + r0;
     const bool warm = P.warm >= 1 && D.lambda0;
+    if (t == 0 && D.wtime) D.wtime[2 * w] = (long long)wall_clock64();
 
-    if (w == 0 && t == 0 && D.feat_count) {  // feature entries were consumed; keep the count for readback
-        D.feat_count[1] = D.feat_count[0];
-        D.feat_count[0] = 0;
+    if (blockIdx.x == 0 && t == 0 && D.feat_count) {  // features were consumed; keep the count for readback
+        D.feat_count[2 * P.split + 1] = D.feat_count[2 * P.split];
+        D.feat_count[2 * P.split] = 0;
     }
     // LDS path: bodies [lds_bodies][16], rows [lds_rows][16], lambda [lds_rows],
     // items [lds_rows], phase starts [lds_rows + 1] (16-bit)
@@ -1075,21 +1198,22 @@ __global__ __launch_bounds__(64) void k_solve_small(DevWorlds D, StepParams P) {
     unsigned short *s_ps = reinterpret_cast<unsigned short *>(s_sched + P.lds_rows);
     bool lds = m > 0 && m <= P.lds_rows && nb < P.lds_bodies;
     if (lds) {
-        lds = lds_stage(D, b0, nb, r0, m, L, ps, sched, s_body, s_rows, s_sched, s_ps);
+        lds = lds_stage(D, b0, nb, r0, m, L, ps, sched, perm, s_body, s_rows, s_sched, s_ps);
         __syncthreads();
     }
     int it = 0;
     float res = -1.f;
     if (lds) {
         if (warm) {
+            // lambda0 clamped to the row bounds (LDS holds rows in sweep order)
             for (int pass = 0; pass < 2; pass++) {
-                for (int r = t; r < m; r += 64) {
-                    const float4 a3 = reinterpret_cast<const float4 *>(s_rows)[r * 4 + 3];
+                for (int p = t; p < m; p += 64) {
+                    const float4 a3 = ld4(s_rows, p, 3);
                     const int fidx = (as_int(a3.z) >> 16) - 1;
                     if ((fidx >= 0) != (pass == 1)) continue;
                     float lo = a3.x, hi = a3.y;
                     if (fidx >= 0) { hi = fabsf(hi * s_lam[fidx]); lo = -hi; }
-                    s_lam[r] = fminf(fmaxf(D.lambda0[r0 + r], lo), hi);
+                    s_lam[p] = fminf(fmaxf(D.lambda0[r0 + inv[p]], lo), hi);
                 }
                 __syncthreads();
             }
@@ -1122,7 +1246,7 @@ __global__ __launch_bounds__(64) void k_solve_small(DevWorlds D, StepParams P) {
             }
         }
         if (P.final_residual && res < 0.f) res = world_residual_lds(s_rows, s_lam, s_body, m, P);
-        for (int r = t; r < m; r += 64) lam[r] = s_lam[r];
+        for (int r = t; r < m; r += 64) lam[r] = s_lam[perm[r]];
         __syncthreads();
     }
     // global-memory path: rows streamed from HBM, fc in LDS
@@ -1169,7 +1293,7 @@ __global__ __launch_bounds__(64) void k_solve_small(DevWorlds D, StepParams P) {
             while (it < P.max_iters) {
                 for (int p = 0; p < L; p++) {
                     const int s = ps[p], e = ps[p + 1];
-                    for (int k = s + t; k < e; k += 64) process_item(D, r0, sched[k], s_fc);
+                    for (int k = s + t; k < e; k += 64) process_item_packed(D, r0, inv, nb, sched[k], s_fc);
                     __syncthreads();
                 }
                 it++;
@@ -1238,7 +1362,9 @@ __global__ __launch_bounds__(64) void k_solve_small(DevWorlds D, StepParams P) {
         const int b = b0 + i;
         const float *s = D.bscr + (size_t)b * BSCR_STRIDE;
         float4 p4 = D.pos[b], q4 = D.quat[b], lv = D.lvel[b], av = D.avel[b], fa = D.facc[b], ta = D.tacc[b];
-        const float *fc = lds ? s_body + i * LDS_BODY : s_fc + i * 6;
+        float fc[6];
+        if (lds) lds_fc(s_body, i, fc);
+        else for (int k = 0; k < 6; k++) fc[k] = s_fc[i * 6 + k];
         if (m > 0) {
             lv.x += h * fc[0]; lv.y += h * fc[1]; lv.z += h * fc[2];
             av.x += h * fc[3]; av.y += h * fc[4]; av.z += h * fc[5];
@@ -1265,6 +1391,7 @@ __global__ __launch_bounds__(64) void k_solve_small(DevWorlds D, StepParams P) {
         D.facc[b] = make_float4(0.f, 0.f, 0.f, 0.f);
         D.tacc[b] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
+    if (t == 0 && D.wtime) D.wtime[2 * w + 1] = (long long)wall_clock64();
 }
 
 }  // namespace odx
@@ -1475,7 +1602,8 @@ __global__ void k_collide_pairs(const float *gviews, const int *pairs, int np, i
 // Host-side launchers (the kernels' only callers).
 namespace odx {
 void launch_prepare_small(const DevWorlds &D, const StepParams &P, hipStream_t s) {
-    hipLaunchKernelGGL(k_prepare_small, dim3(D.nworlds), dim3(64), 0, s, D, P);
+    if (P.wcount <= 0) return;
+    hipLaunchKernelGGL(k_prepare_small, dim3(P.wcount), dim3(64), 0, s, D, P);
 }
 size_t solve_small_lds_bytes(const StepParams &P) {
     const size_t lds = ((size_t)P.lds_bodies * LDS_BODY + (size_t)P.lds_rows * (LDS_ROW + 2)) * sizeof(float) +
@@ -1484,8 +1612,9 @@ size_t solve_small_lds_bytes(const StepParams &P) {
     return lds > fc ? lds : fc;
 }
 void launch_solve_small(const DevWorlds &D, const StepParams &P, hipStream_t s) {
-    if (D.order) hipLaunchKernelGGL(k_order_worlds, dim3(1), dim3(1024), 0, s, D.niters, D.nworlds, D.order);
-    hipLaunchKernelGGL(k_solve_small, dim3(D.nworlds), dim3(64), solve_small_lds_bytes(P), s, D, P);
+    if (P.wcount <= 0) return;
+    if (D.order) hipLaunchKernelGGL(k_order_worlds, dim3(1), dim3(1024), 0, s, D.niters, P.wbase, P.wcount, D.order);
+    hipLaunchKernelGGL(k_solve_small, dim3(P.wcount), dim3(64), solve_small_lds_bytes(P), s, D, P);
 }
 }  // namespace odx
 

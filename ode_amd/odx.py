@@ -45,6 +45,7 @@ def lib():
         L.odxBatchDestroy.argtypes = [vp]
         L.odxBatchSetSolver.argtypes = [vp, i, i, ctypes.c_float, i, i]
         L.odxBatchSetSolveOptions.argtypes = [vp, i, i]
+        L.odxBatchSetSplits.argtypes = [vp, i, i]
         L.odxBatchStep.argtypes = [vp, i]
         L.odxBatchSync.argtypes = [vp]
         L.odxBatchTime.argtypes = [vp, i, fp]
@@ -56,6 +57,7 @@ def lib():
         L.odxBatchGetLambda.argtypes = [vp, i, fp, i]
         L.odxBatchGetRows.argtypes = [vp, i, fp, i]
         L.odxBatchGetOverflow.argtypes = [vp, ip]
+        L.odxBatchGetSolveTimes.argtypes = [vp, dp]
         L.odxBatchRowStarts.argtypes = [vp, ip]
         L.odxBatchSetLambda0.argtypes = [vp, fp, i]
         L.odxSetMessageHandler(_on_message)
@@ -145,6 +147,11 @@ class Batch:
     def set_solver(self, order=0, max_iters=20, tol=0.0, check_every=1, warm=0):
         _check(lib().odxBatchSetSolver(self.h, order, max_iters, float(tol), check_every, warm), "odxBatchSetSolver")
 
+    def set_splits(self, nsplit=1, graph_steps=1):
+        """Step nsplit world ranges on streams of their own, graph_steps steps
+        per graph launch (results equal those of one range)."""
+        _check(lib().odxBatchSetSplits(self.h, nsplit, graph_steps), "odxBatchSetSplits")
+
     def set_solve_options(self, restart_after=0, lds_rows=400):
         """restart_after: sweeps after which an unconverged warm start restarts
         cold (0 = never); lds_rows: row budget of the LDS-resident sweeps."""
@@ -205,6 +212,12 @@ class Batch:
     def overflow(self):
         out = np.zeros(4, np.int32)
         _check(lib().odxBatchGetOverflow(self.h, _p(out)), "odxBatchGetOverflow")
+        return out
+
+    def solve_times(self):
+        """[nworlds, 2] start/end of each world's solve in the last step (us)."""
+        out = np.zeros((self.nworlds, 2), np.float64)
+        _check(lib().odxBatchGetSolveTimes(self.h, _p(out)), "odxBatchGetSolveTimes")
         return out
 
     def set_lambda0(self, lam0):

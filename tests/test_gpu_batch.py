@@ -308,3 +308,28 @@ def test_unconverged_warm_start_restarts_cold(lds_rows):
     np.testing.assert_array_equal(out["restart"][0], out["cold"][0] + 6)
     for a, b in zip(out["restart"][1], out["cold"][1]):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("warm", [0, 2])
+def test_split_streams_and_multi_step_graphs_change_nothing(warm):
+    """World ranges stepped on separate streams inside multi-step graphs give
+    bit-identical bodies, impulses and sweep counts to one range stepped one
+    graph launch per step (worlds are independent)."""
+    say("case: splits, warm", warm)
+    rng = np.random.default_rng(23)
+    worlds = [S.uneven_world(rng, 8, 40) for _ in range(37)]
+    prm = O.params()
+    odx = _odx()
+    out = []
+    for nsplit, gsteps in ((1, 1), (4, 3)):
+        B = odx.Batch(worlds, prm)
+        B.set_splits(nsplit, gsteps)
+        B.set_solver(order=1, max_iters=3000, tol=1e-4, check_every=2, warm=warm)
+        B.step(7)
+        st = B.stats()
+        out.append((B.bodies(), [B.lam(wi) for wi in range(len(worlds))], st["iters"].copy()))
+        B.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][2], out[1][2])
+    for a, b in zip(out[0][1], out[1][1]):
+        np.testing.assert_array_equal(a, b)
