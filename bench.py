@@ -77,7 +77,13 @@ def measure(B, args, warm, nworlds):
         max_rows = max(max_rows, int(B.stats()["rows"].max()))
     lds_rows = args.lds_rows if args.lds_rows >= 0 else 400
     if args.lds_rows < 0:
-        want = -(-int(max_rows * 1.1) // 16) * 16
+        # the largest budget that still fits five worlds in a CU's 160 KB of
+        # LDS (rows grow as towers fall), or 1.25x the largest world seen
+        nb_max = int(np.max(np.diff(B.body_start)))
+        def lds_bytes(r):  # solve_small_lds_bytes
+            return (nb_max + 1) * 64 + r * 72 + (r + 1) * 2
+        five = max(r for r in range(16, 2049) if 5 * lds_bytes(r) <= 160 * 1024)
+        want = max(five, -(-int(max_rows * 1.25) // 16) * 16)
         while want > max_rows:
             try:
                 B.set_solve_options(restart_after=args.restart_after, lds_rows=want)
@@ -139,6 +145,14 @@ def main():
     ap.add_argument("--restart-after", type=int, default=600,
                     help="sweeps after which an unconverged warm start restarts from lambda = 0")
     ap.add_argument("--lds-rows", type=int, default=-1, help="LDS row budget per world (-1: sized during warmup)")
+    ap.add_argument("--sor-w", type=float, default=1.0,
+                    help="SOR relaxation of every mode (ODE's default 1.3 sends some warm-started solves into "
+                         "limit cycles; cold convergence is the same at 1.0)")
+    ap.add_argument("--splits", type=int, default=4, help="world ranges stepped on parallel streams")
+    ap.add_argument("--graph-steps", type=int, default=10, help="steps per graph launch")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="finally copy MIB MiB device to device (a reference kernel of known traffic for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -149,7 +163,7 @@ def main():
     from ode_amd import odx
     import oracle as O
 
-    prm = odx.params_vector(iters=args.max_iters)
+    prm = odx.params_vector(iters=args.max_iters, w=args.sor_w)
     modes = args.modes.split(",") if args.workload == "towers" else ["cold"]
     train_info = None
     mlp = None
@@ -184,6 +198,7 @@ def main():
     bodies0 = None
     for mode in modes:
         B = odx.Batch(worlds, prm)
+        B.set_splits(args.splits, args.graph_steps)
         if mode == "learned":
             B.set_mlp(mlp)
         r = measure(B, args, {"cold": 0, "plain": 2, "learned": 3}[mode], args.worlds)
@@ -212,21 +227,33 @@ def main():
         "config": {
             "workload": args.workload,
             "worlds": args.worlds,
-            "solver": "%s start, coloured contact blocks, SOR w=1.3, sweeps until LCP residual <= %g "
-                      "(checked every %d sweeps, cap %d)" % (head, args.tol, args.check_every, args.max_iters),
+            "solver": "%s start, coloured contact blocks, SOR w=%g, sweeps until LCP residual <= %g "
+                      "(checked every %d sweeps, cap %d, warm starts restart cold after %d)"
+                      % (head, args.sor_w, args.tol, args.check_every, args.max_iters, args.restart_after),
+            "launch": "%d world ranges on parallel streams, %d steps per hipGraph launch" % (args.splits, args.graph_steps),
             "h": 0.01, "mu": 0.5, "erp": 0.2, "cfm": 1e-5, "seed": args.seed,
         },
         "modes": results,
-        "roofline": {
-            "bound": "hbm",
-            "achieved": R["solve_gbs"],
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": R["solve_gbs"] / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "k_solve_small",
-        },
     }
+    # roofline of the dominant kernel of the plain GPU path (cold PGS: the solve)
+    RR = results["cold"] if "cold" in results else R
+    out["roofline"] = {
+        "bound": "hbm",
+        "achieved": RR["solve_gbs"],
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": RR["solve_gbs"] / HBM_PEAK_GBS,
+        "traffic": None,
+        "kernel": "k_solve_small (cold mode, whole-batch launch)",
+        "note": "achieved = 228 B per row per sweep (rows streamed from HBM every sweep); this kernel reads each "
+                "row from HBM once per step and sweeps it in LDS, so achieved can exceed the HBM peak",
+    }
+    # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size)
+    tpath = os.path.join(ROOT, "profiles", "k_solve_small_%s4096_cold.json" % args.workload)
+    if args.worlds == 4096 and os.path.exists(tpath):
+        t = json.load(open(tpath))
+        out["roofline"]["traffic"] = t["traffic_bytes_per_launch"]
+        out["roofline"]["traffic_source"] = os.path.relpath(tpath, ROOT)
     if "cold" in results and head != "cold":
         out["speedup_vs_cold"] = R["world_steps_per_s"] / results["cold"]["world_steps_per_s"]
     if "plain" in results and head != "plain":
@@ -239,8 +266,26 @@ def main():
             out["mlp_infer_frac_bf16_peak"] = out["mlp_infer_tflops"] / BF16_PEAK_TFLOPS
     if not args.no_cpu and bodies0 is not None:
         out["cpu_baseline"] = cpu_baseline(worlds, bodies0, body_start,
-                                           O.params(iters=args.max_iters, tol=args.tol, check_every=args.check_every))
+                                           O.params(iters=args.max_iters, tol=args.tol, check_every=args.check_every, w=args.sor_w))
     print(json.dumps(out), flush=True)
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration << 20)
+
+
+def copy_calibration(nbytes):
+    """One device-to-device hipMemcpy of nbytes (the runtime's copy kernel
+    reads and writes nbytes), straight through the HIP runtime."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)):
+        raise SystemExit("copy calibration: hipMalloc failed")
+    hip.hipMemset(src, 1, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
 
 
 if __name__ == "__main__":

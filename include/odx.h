@@ -90,6 +90,13 @@ int odxBatchSetSolveOptions(odxBatch *b, int restart_after, int lds_rows);
  * the GPU busy with their next steps.  Results are those of nsplit = 1. */
 int odxBatchSetSplits(odxBatch *b, int nsplit, int graph_steps);
 
+/* steps_per_launch > 0: every world goes through that many complete steps
+ * (collide, rows, network, solve, integrate) inside one launch of a
+ * persistent fused kernel, one wave per world, so a world's slow solve delays
+ * only that world.  0 (default): one launch per stage per step.  Results are
+ * those of the staged launches. */
+int odxBatchSetFused(odxBatch *b, int steps_per_launch);
+
 /* Runs nsteps steps on the batch's stream (captured once into a hipGraph of
  * graph_steps steps and replayed).  Returns 1 on success. */
 int odxBatchStep(odxBatch *b, int nsteps);
@@ -120,6 +127,10 @@ int odxBatchGetOverflow(odxBatch *b, int *counts);
 /* Start and end (microseconds from the earliest start) of each world's solve
  * in the last step, [nworlds][2], from the device wall clock. */
 int odxBatchGetSolveTimes(odxBatch *b, double *start_end_us);
+/* Stage boundaries of each world's prepare stage in the last step,
+ * [nworlds][6] microseconds from the earliest stamp: start, after bodies,
+ * after broadphase, after narrowphase, after rows, after the schedule. */
+int odxBatchGetPrepareTimes(odxBatch *b, double *stamps_us);
 /* Row-capacity offsets [nworlds+1] (the lambda0 layout). */
 int odxBatchRowStarts(odxBatch *b, int *row_start);
 /* Warm-start values per row in the row-capacity layout (host array). */

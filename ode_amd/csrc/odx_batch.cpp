@@ -18,10 +18,13 @@
 #include "odx_host.h"
 
 namespace odx {
+struct MlpDev;
 void launch_prepare_small(const DevWorlds &D, const StepParams &P, hipStream_t s);
 void launch_solve_small(const DevWorlds &D, const StepParams &P, hipStream_t s);
+void launch_step_fused(const DevWorlds &D, const StepParams &P, const MlpDev *M, int nsteps, hipStream_t s);
 void launch_mlp_infer(const odxMlp *m, const float *feat, const int *feat_row, const int *count, float *lambda0,
                       int max_entries, hipStream_t s);
+const MlpDev *mlp_device_view(const odxMlp *m);
 }  // namespace odx
 
 using namespace odx;
@@ -170,27 +173,39 @@ struct odxBatch {
         launch_solve_small(D, q, s);
         return odx_check(hipGetLastError(), "k_solve_small launch");
     }
+    int fused = 0;  // > 0: steps per launch of the persistent fused kernel
+    // nsteps steps of worlds q.wbase.. on stream s
+    bool enqueue_range(StepParams q, int nsteps, hipStream_t s) {
+        if (fused <= 0) {
+            for (int k = 0; k < nsteps; k++)
+                if (!enqueue_step(q, s)) return false;
+            return true;
+        }
+        q.feat_local = 1;
+        const MlpDev *M = infer_on() ? mlp_device_view(mlp) : nullptr;
+        for (int k = 0; k < nsteps; k += fused) {
+            launch_step_fused(D, q, M, std::min(fused, nsteps - k), s);
+            if (!odx_check(hipGetLastError(), "k_step_fused launch")) return false;
+        }
+        return true;
+    }
     // nsteps steps of every world, captured on `stream` (forked over the splits)
     bool enqueue_steps(int nsteps) {
         if (splits.empty()) {
-            const StepParams q = launch_params(0, nw, 0);
-            for (int k = 0; k < nsteps; k++)
-                if (!enqueue_step(q, stream)) return false;
-            samples_split = 0;
+            if (!enqueue_range(launch_params(0, nw, 0), nsteps, stream)) return false;
+            samples_split = fused > 0 ? -2 : 0;
             return true;
         }
         if (!odx_check(hipEventRecord(fork, stream), "hipEventRecord")) return false;
         for (size_t i = 0; i < splits.size(); i++) {
             odxSplit &sp = splits[i];
             if (!odx_check(hipStreamWaitEvent(sp.stream, fork, 0), "hipStreamWaitEvent")) return false;
-            const StepParams q = launch_params(sp.w0, sp.w1, (int)i);
-            for (int k = 0; k < nsteps; k++)
-                if (!enqueue_step(q, sp.stream)) return false;
+            if (!enqueue_range(launch_params(sp.w0, sp.w1, (int)i), nsteps, sp.stream)) return false;
             if (!odx_check(hipEventRecord(sp.done, sp.stream), "hipEventRecord") ||
                 !odx_check(hipStreamWaitEvent(stream, sp.done, 0), "hipStreamWaitEvent"))
                 return false;
         }
-        samples_split = -1;
+        samples_split = fused > 0 ? -2 : -1;
         return true;
     }
     bool capture(int nsteps, hipGraphExec_t *out) {
@@ -413,8 +428,10 @@ odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies
     ok = ok && B->alloc(&D.bscr, (size_t)nb * BSCR_STRIDE);
     ok = ok && B->alloc(&D.feat, (size_t)B->jcap * FEAT_DIM) && B->alloc(&D.feat_row, B->jcap) &&
          B->alloc(&D.feat_count, 2 * (ODX_MAX_SPLITS + 1)) && B->alloc(&D.joff, B->jcap);
-    ok = ok && B->alloc(&D.prev, (size_t)B->jcap * JOINT_STRIDE) && B->alloc(&D.nprev, nworlds);
-    ok = ok && B->alloc(&D.overflow, 4) && B->alloc(&D.order, nworlds) && B->alloc(&D.wtime, 2 * (size_t)nworlds);
+    ok = ok && B->alloc(&D.prev, (size_t)B->jcap * JOINT_STRIDE) && B->alloc(&D.nprev, nworlds) &&
+         B->alloc(&D.nfeat, nworlds);
+    ok = ok && B->alloc(&D.overflow, 4) && B->alloc(&D.order, nworlds) && B->alloc(&D.wtime, 2 * (size_t)nworlds) &&
+         B->alloc(&D.ptime, PTIME_SLOTS * (size_t)nworlds);
     ok = ok && B->alloc(&B->lambda0, B->rcap);
     if (ok) {
         ok = odx_check(hipMemset(D.overflow, 0, 4 * sizeof(int)), "hipMemset") &&
@@ -493,6 +510,17 @@ int odxBatchSetSplits(odxBatch *b, int nsplit, int graph_steps) {
         }
         b->splits.push_back(s);
     }
+    return 1;
+}
+
+int odxBatchSetFused(odxBatch *b, int steps_per_launch) {
+    if (!b || steps_per_launch < 0) {
+        odx_message(1, "odxBatchSetFused: bad arguments");
+        return 0;
+    }
+    if (!odxBatchSync(b)) return 0;
+    b->fused = steps_per_launch;
+    b->drop_graph();
     return 1;
 }
 
@@ -667,19 +695,30 @@ int odxBatchGetOverflow(odxBatch *b, int *counts) {
     return download(counts, b->D.overflow, 4) ? 4 : 0;
 }
 
-int odxBatchGetSolveTimes(odxBatch *b, double *start_end_us) {
-    if (!b || !start_end_us) return 0;
-    if (!odxBatchSync(b)) return 0;
-    std::vector<long long> t(2 * (size_t)b->nw);
-    if (!download(t.data(), b->D.wtime, t.size())) return 0;
+// Device wall-clock stamps -> microseconds from the earliest stamp.
+static int odx_stamps_us(const long long *dev_stamps, size_t n, double *out) {
+    std::vector<long long> t(n);
+    if (!download(t.data(), dev_stamps, n)) return 0;
     int dev = 0, khz = 0;
     if (!odx_check(hipGetDevice(&dev), "hipGetDevice") ||
         !odx_check(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "hipDeviceGetAttribute") || khz <= 0)
         return 0;
     long long t0 = t[0];
     for (long long v : t) t0 = std::min(t0, v);
-    for (size_t i = 0; i < t.size(); i++) start_end_us[i] = (double)(t[i] - t0) * 1e3 / (double)khz;
-    return b->nw;
+    for (size_t i = 0; i < n; i++) out[i] = (double)(t[i] - t0) * 1e3 / (double)khz;
+    return 1;
+}
+
+int odxBatchGetSolveTimes(odxBatch *b, double *start_end_us) {
+    if (!b || !start_end_us) return 0;
+    if (!odxBatchSync(b)) return 0;
+    return odx_stamps_us(b->D.wtime, 2 * (size_t)b->nw, start_end_us) ? b->nw : 0;
+}
+
+int odxBatchGetPrepareTimes(odxBatch *b, double *stamps_us) {
+    if (!b || !stamps_us) return 0;
+    if (!odxBatchSync(b)) return 0;
+    return odx_stamps_us(b->D.ptime, PTIME_SLOTS * (size_t)b->nw, stamps_us) ? b->nw : 0;
 }
 
 int odxBatchRowStarts(odxBatch *b, int *row_start) {
@@ -702,15 +741,22 @@ int odxBatchGetSamples(odxBatch *b, float *X, float *T, int cap) {
     if (!download(cnt.data(), b->D.feat_count, cnt.size())) return 0;
     // feature ranges (first slot, count) of the launches that ran last
     std::vector<std::pair<int, int>> ranges;
-    if (b->samples_split >= 0) ranges.push_back({0, cnt[2 * b->samples_split + 1]});
-    else
+    if (b->samples_split >= 0) {
+        ranges.push_back({0, cnt[2 * b->samples_split + 1]});
+    } else if (b->samples_split == -1) {
         for (size_t i = 0; i < b->splits.size(); i++) ranges.push_back({b->joint_start[b->splits[i].w0], cnt[2 * i + 1]});
+    } else {  // fused kernel: world-local slots
+        std::vector<int> nf(b->nw);
+        if (!download(nf.data(), b->D.nfeat, b->nw)) return 0;
+        for (int w = 0; w < b->nw; w++) ranges.push_back({b->joint_start[w], nf[w]});
+    }
     std::vector<float> lam(b->rcap);
     if (!download(lam.data(), b->D.lambda, b->rcap)) return 0;
     int n = 0;
     for (const auto &rg : ranges) {
+        if (n >= cap) break;
         const int k = std::min(rg.second, cap - n);
-        if (k <= 0) break;
+        if (k <= 0) continue;
         std::vector<int> rows(k);
         if (!download(X + (size_t)n * FEAT_DIM, b->D.feat + (size_t)rg.first * FEAT_DIM, (size_t)k * FEAT_DIM) ||
             !download(rows.data(), b->D.feat_row + rg.first, k))

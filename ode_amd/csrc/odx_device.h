@@ -66,6 +66,7 @@ struct StepParams {
     int wbase, wcount;   // batched launch covers worlds [wbase, wbase + wcount)
     int split;           // feature counter pair of this launch: feat_count[2 * split], [2 * split + 1]
     int feat_base;       // first feature slot of this launch (joint_start[wbase])
+    int feat_local;      // 1: features go to slots joint_start[w] + i, in joint order (fused kernel)
 };
 
 // Independent world ranges a batch may be split into (one stream each).
@@ -111,7 +112,8 @@ struct DevWorlds {
     int *joff;              // [joint cap] world-local first row of each joint
     float *feat;            // [joint cap * 16] compacted warm-start features (learned mode)
     int *feat_row;          // [joint cap] global first row of each feature entry
-    int *feat_count;        // [1] entries written this step
+    int *feat_count;        // [2 * (ODX_MAX_SPLITS + 1)] entries written per launch range: this step, last step
+    int *nfeat;             // [W] entries of each world (world-local slots, fused kernel)
     float *prev;            // [joint cap * 16] previous-step contacts: g1 g2 plocal(3) n(3) lambda(3)
     int *nprev;             // [W]
     int *overflow;          // [4] pairs / contacts / rows / phases overflow counters
@@ -119,7 +121,9 @@ struct DevWorlds {
     int *perm, *inv;        // [row cap] sweep position of each row and its inverse (per world, row_start based)
     int *order;             // [W] solve dispatch order (longest previous solve first); null = index order
     long long *wtime;       // [W][2] solve start / end of each world (device wall clock), may be null
+    long long *ptime;       // [W][PTIME_SLOTS] prepare stage boundaries (device wall clock), may be null
 };
+constexpr int PTIME_SLOTS = 6;
 
 // Geom view for the collision entry points of the single-world API:
 // cls, body identity (-1 none), params[4], pos[3], R[12] (3x4).

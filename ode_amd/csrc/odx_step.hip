@@ -24,6 +24,7 @@
 #include "odx_collide.h"
 #include "odx_device.h"
 #include "odx_math.h"
+#include "odx_mlp_dev.h"
 
 namespace odx {
 
@@ -479,21 +480,56 @@ __device__ void write_contact_lambda0(const DevWorlds &D, const float *rec, int 
 }
 
 // ============================================================== kernel A ===
-__global__ __launch_bounds__(64) void k_prepare_small(DevWorlds D, StepParams P) {
-    const int w = P.wbase + (int)blockIdx.x;
+// LDS of the prepare stage of one world (one wave).  Laid out by
+// prep_lds_carve over one buffer so that a fused kernel can reuse the space
+// for the solve stage.
+struct PrepLds {
+    unsigned long long *mask;  // [SW_MAXB]
+    float (*ab)[6];            // [SW_MAXG]
+    uint32_t *pairs;           // [SW_MAXPAIRS]
+    int *cnt;                  // [SW_MAXPH + 1]
+    unsigned short *lvl, *joff;  // [SW_MAXROWS]
+    unsigned char *rb1, *rb2, *rfl, *jm;  // [SW_MAXROWS]
+    signed char *gb;           // [SW_MAXG]
+    int *L, *nblk;             // scalars
+};
+constexpr size_t PREP_LDS_BYTES = SW_MAXB * 8 + SW_MAXG * 6 * 4 + SW_MAXPAIRS * 4 + (SW_MAXPH + 1) * 4 +
+                                  2 * SW_MAXROWS * 2 + 4 * SW_MAXROWS + SW_MAXG + 2 * 4 + 16;
+__device__ inline PrepLds prep_lds_carve(void *base) {
+    char *p = reinterpret_cast<char *>(base);  // 16-byte aligned
+    PrepLds S;
+    S.mask = reinterpret_cast<unsigned long long *>(p); p += SW_MAXB * 8;
+    S.ab = reinterpret_cast<float (*)[6]>(p); p += SW_MAXG * 6 * 4;
+    S.pairs = reinterpret_cast<uint32_t *>(p); p += SW_MAXPAIRS * 4;
+    S.cnt = reinterpret_cast<int *>(p); p += (SW_MAXPH + 1) * 4;
+    S.L = reinterpret_cast<int *>(p); p += 4;
+    S.nblk = reinterpret_cast<int *>(p); p += 4;
+    S.lvl = reinterpret_cast<unsigned short *>(p); p += SW_MAXROWS * 2;
+    S.joff = reinterpret_cast<unsigned short *>(p); p += SW_MAXROWS * 2;
+    S.rb1 = reinterpret_cast<unsigned char *>(p); p += SW_MAXROWS;
+    S.rb2 = reinterpret_cast<unsigned char *>(p); p += SW_MAXROWS;
+    S.rfl = reinterpret_cast<unsigned char *>(p); p += SW_MAXROWS;
+    S.jm = reinterpret_cast<unsigned char *>(p); p += SW_MAXROWS;
+    S.gb = reinterpret_cast<signed char *>(p);
+    return S;
+}
+
+// Prepare stage of world w: bodies, AABBs, pairs, contacts, rows, schedule.
+// Returns the number of warm-start feature entries it wrote when features go
+// to world-local slots (P.feat_local), else 0.
+__device__ __forceinline__ int prepare_world(const DevWorlds &D, const StepParams &P, int w, const PrepLds &S) {
     const int t = threadIdx.x;
-    __shared__ float s_ab[SW_MAXG][6];
-    __shared__ signed char s_gb[SW_MAXG];
-    __shared__ uint32_t s_pairs[SW_MAXPAIRS];
-    __shared__ unsigned char s_rb1[SW_MAXROWS], s_rb2[SW_MAXROWS], s_rfl[SW_MAXROWS];
-    __shared__ unsigned short s_lvl[SW_MAXROWS];
-    __shared__ unsigned short s_joff[SW_MAXROWS];
-    __shared__ unsigned char s_jm[SW_MAXROWS];
-    __shared__ int s_cnt[SW_MAXPH + 1];
-    __shared__ unsigned long long s_mask[SW_MAXB];
-    __shared__ int s_L, s_nblk;
+    float (*s_ab)[6] = S.ab;
+    signed char *s_gb = S.gb;
+    uint32_t *s_pairs = S.pairs;
+    unsigned char *s_rb1 = S.rb1, *s_rb2 = S.rb2, *s_rfl = S.rfl, *s_jm = S.jm;
+    unsigned short *s_lvl = S.lvl, *s_joff = S.joff;
+    int *s_cnt = S.cnt;
+    unsigned long long *s_mask = S.mask;
+    int &s_L = *S.L, &s_nblk = *S.nblk;
     static_assert(SW_MAXPAIRS * sizeof(uint32_t) >= SW_MAXROWS * sizeof(unsigned short), "block starts alias s_pairs");
     static_assert(SW_MAXG * 6 * sizeof(float) >= SW_MAXROWS, "block lengths alias s_ab");
+    int nfeat = 0;
 
     const int b0 = D.body_start[w], nb = D.body_start[w + 1] - b0;
     const int g0 = D.geom_start[w], ng = D.geom_start[w + 1] - g0;
@@ -501,9 +537,14 @@ __global__ __launch_bounds__(64) void k_prepare_small(DevWorlds D, StepParams P)
     const int j0 = D.joint_start[w], jcap = D.joint_start[w + 1] - j0;
     const int r0 = D.row_start[w], rcap = min(D.row_start[w + 1] - r0, SW_MAXROWS);
 
+    // stage timestamps (device wall clock): start, S1, S3, S4, S5, S6
+    long long *pt = D.ptime ? D.ptime + (size_t)w * PTIME_SLOTS : nullptr;
+    if (pt && t == 0) pt[0] = (long long)wall_clock64();
+
     // S1: bodies
     for (int i = t; i < nb; i += 64) body_prep(D, P, b0 + i);
     __syncthreads();
+    if (pt && t == 0) pt[1] = (long long)wall_clock64();
 
     int njoint, ncont;
     if (P.collide) {
@@ -555,6 +596,7 @@ __global__ __launch_bounds__(64) void k_prepare_small(DevWorlds D, StepParams P)
             npairs = SW_MAXPAIRS;
         }
         __syncthreads();
+        if (pt && t == 0) pt[2] = (long long)wall_clock64();
         // S4: narrowphase, contacts appended after the ball joints
         const int ccap = jcap - nball;
         ncont = 0;
@@ -599,6 +641,7 @@ __global__ __launch_bounds__(64) void k_prepare_small(DevWorlds D, StepParams P)
         ncont = D.ncontacts[w];
     }
     __syncthreads();
+    if (pt && t == 0) pt[3] = (long long)wall_clock64();
 
     // S5: rows, in joint order
     int mrow = 0;
@@ -655,10 +698,17 @@ __global__ __launch_bounds__(64) void k_prepare_small(DevWorlds D, StepParams P)
                 if (is_contact && !learn) write_contact_lambda0(D, rec, r0, ri, lp);
                 const unsigned long long bal = __ballot(learn);
                 int base = 0;
-                if (t == 0 && bal) base = atomicAdd(D.feat_count + 2 * P.split, __popcll(bal));
-                base = __shfl(base, 0, 64);
+                int fbase;
+                if (P.feat_local) {
+                    // world-local slots from the world's first joint, in joint order
+                    fbase = j0 + nfeat;
+                    nfeat += __popcll(bal);
+                } else {
+                    if (t == 0 && bal) base = atomicAdd(D.feat_count + 2 * P.split, __popcll(bal));
+                    fbase = P.feat_base + __shfl(base, 0, 64);
+                }
                 if (learn) {
-                    const int slot = P.feat_base + base + __popcll(bal & ((1ull << t) - 1ull));
+                    const int slot = fbase + __popcll(bal & ((1ull << t) - 1ull));
                     contact_features(D, rec, b0, lp, matched, D.feat + (size_t)slot * FEAT_DIM);
                     D.feat_row[slot] = r0 + ri;
                     // rows beyond the predicted three (none for this surface) start at zero
@@ -681,6 +731,7 @@ __global__ __launch_bounds__(64) void k_prepare_small(DevWorlds D, StepParams P)
         }
     }
     __syncthreads();
+    if (pt && t == 0) pt[4] = (long long)wall_clock64();
 
     // S6: schedule.  Work items are packed (first row, row count, body 1,
     // body 2; pack_item) and address rows in "sweep order": row r of the world
@@ -829,6 +880,14 @@ __global__ __launch_bounds__(64) void k_prepare_small(DevWorlds D, StepParams P)
         }
     }
     if (t == 0) D.nrows[w] = mrow;
+    if (t == 0 && P.feat_local) D.nfeat[w] = nfeat;
+    if (pt && t == 0) pt[5] = (long long)wall_clock64();
+    return nfeat;
+}
+
+__global__ __launch_bounds__(64) void k_prepare_small(DevWorlds D, StepParams P) {
+    __shared__ __align__(16) char s_prep[PREP_LDS_BYTES];
+    prepare_world(D, P, P.wbase + (int)blockIdx.x, prep_lds_carve(s_prep));
 }
 
 // ============================================================== kernel C ===
@@ -1004,47 +1063,59 @@ __device__ __forceinline__ void lds_store_fc(float *s_body, int b, const float *
     *reinterpret_cast<float2 *>(s_body + 4 * sw4(b, 1)) = make_float2(fc[4], fc[5]);
 }
 
+// One SOR row update (IterationStep, quickstep.cpp:2917-3033) on an LDS row
+// a0..a3 at sweep position p, with the two bodies' fc and inverse mass terms
+// in registers.  Branch-free: the friction-cone bound's lambda is read for
+// every row (position 0 stands in when there is no findex) and selected.
+__device__ __forceinline__ void row_update(const float4 a0, const float4 a1, const float4 a2, const float4 a3, int p,
+                                           float *s_lam, float *f1, float *f2, const float *I1, const float *I2,
+                                           float im1, float im2) {
+    const int fidx = (as_int(a3.z) >> 16) - 1;
+    const float old = s_lam[p];
+    const float lf = s_lam[fidx >= 0 ? fidx : 0];
+    float delta = a2.y - old * a2.z;
+    delta -= a0.x * (f1[0] - f2[0]) + a0.y * (f1[1] - f2[1]) + a0.z * (f1[2] - f2[2]);
+    delta -= a0.w * f1[3] + a1.x * f1[4] + a1.y * f1[5];
+    delta -= a1.z * f2[3] + a1.w * f2[4] + a2.x * f2[5];
+    const float hf = fabsf(a3.y * lf);
+    const float hi = fidx >= 0 ? hf : a3.y, lo = fidx >= 0 ? -hf : a3.x;
+    const float raw = old + delta;
+    const float nl = fminf(fmaxf(raw, lo), hi);
+    if (nl != raw) delta = nl - old;  // clamped: delta to the bound, as the reference
+    s_lam[p] = nl;
+    const float e = delta * a2.w, e1 = e * im1, e2 = e * im2;
+    f1[0] += e1 * a0.x; f1[1] += e1 * a0.y; f1[2] += e1 * a0.z;
+    f2[0] -= e2 * a0.x; f2[1] -= e2 * a0.y; f2[2] -= e2 * a0.z;
+    f1[3] += e * (I1[0] * a0.w + I1[1] * a1.x + I1[2] * a1.y);
+    f1[4] += e * (I1[3] * a0.w + I1[4] * a1.x + I1[5] * a1.y);
+    f1[5] += e * (I1[6] * a0.w + I1[7] * a1.x + I1[8] * a1.y);
+    f2[3] += e * (I2[0] * a1.z + I2[1] * a1.w + I2[2] * a2.x);
+    f2[4] += e * (I2[3] * a1.z + I2[4] * a1.w + I2[5] * a2.x);
+    f2[5] += e * (I2[6] * a1.z + I2[7] * a1.w + I2[8] * a2.x);
+}
+
 __device__ __forceinline__ void process_item_lds(const float *s_rows, float *s_lam, float *s_body, int nb, int item) {
     const int rs = item_start(item), nr = item_rows(item);
     const int b1 = item_b1(item), b2 = item_b2(item);
     float f1[6], f2[6], I1[9], I2[9], im1, im2;
-    // rows are read one ahead (issued with the body records for the first), so
-    // each row's LDS latency overlaps the previous row's arithmetic
+    // two row buffers in turn: the next row's LDS reads are in flight while
+    // the current row computes (issued with the body records for the first)
     float4 a0 = ld4(s_rows, rs, 0), a1 = ld4(s_rows, rs, 1), a2 = ld4(s_rows, rs, 2), a3 = ld4(s_rows, rs, 3);
+    float4 c0, c1, c2, c3;
     lds_body(s_body, b1, f1, im1, I1);
     lds_body(s_body, b2, f2, im2, I2);
-    for (int q = 0; q < nr; q++) {
-        float4 n0 = a0, n1 = a1, n2 = a2, n3 = a3;
-        if (q + 1 < nr) {
-            n0 = ld4(s_rows, rs + q + 1, 0); n1 = ld4(s_rows, rs + q + 1, 1);
-            n2 = ld4(s_rows, rs + q + 1, 2); n3 = ld4(s_rows, rs + q + 1, 3);
+    int q = 0;
+    for (; q + 1 < nr; q += 2) {
+        c0 = ld4(s_rows, rs + q + 1, 0); c1 = ld4(s_rows, rs + q + 1, 1);
+        c2 = ld4(s_rows, rs + q + 1, 2); c3 = ld4(s_rows, rs + q + 1, 3);
+        row_update(a0, a1, a2, a3, rs + q, s_lam, f1, f2, I1, I2, im1, im2);
+        if (q + 2 < nr) {
+            a0 = ld4(s_rows, rs + q + 2, 0); a1 = ld4(s_rows, rs + q + 2, 1);
+            a2 = ld4(s_rows, rs + q + 2, 2); a3 = ld4(s_rows, rs + q + 2, 3);
         }
-        const float old = s_lam[rs + q];
-        float delta = a2.y - old * a2.z;
-        delta -= a0.x * (f1[0] - f2[0]) + a0.y * (f1[1] - f2[1]) + a0.z * (f1[2] - f2[2]);
-        delta -= a0.w * f1[3] + a1.x * f1[4] + a1.y * f1[5];
-        delta -= a1.z * f2[3] + a1.w * f2[4] + a2.x * f2[5];
-        float lo = a3.x, hi = a3.y;
-        const int fidx = (as_int(a3.z) >> 16) - 1;
-        if (fidx >= 0) {
-            hi = fabsf(hi * s_lam[fidx]);
-            lo = -hi;
-        }
-        float nl = old + delta;
-        if (nl < lo) { delta = lo - old; nl = lo; }
-        else if (nl > hi) { delta = hi - old; nl = hi; }
-        s_lam[rs + q] = nl;
-        const float e = delta * a2.w, e1 = e * im1, e2 = e * im2;
-        f1[0] += e1 * a0.x; f1[1] += e1 * a0.y; f1[2] += e1 * a0.z;
-        f2[0] -= e2 * a0.x; f2[1] -= e2 * a0.y; f2[2] -= e2 * a0.z;
-        f1[3] += e * (I1[0] * a0.w + I1[1] * a1.x + I1[2] * a1.y);
-        f1[4] += e * (I1[3] * a0.w + I1[4] * a1.x + I1[5] * a1.y);
-        f1[5] += e * (I1[6] * a0.w + I1[7] * a1.x + I1[8] * a1.y);
-        f2[3] += e * (I2[0] * a1.z + I2[1] * a1.w + I2[2] * a2.x);
-        f2[4] += e * (I2[3] * a1.z + I2[4] * a1.w + I2[5] * a2.x);
-        f2[5] += e * (I2[6] * a1.z + I2[7] * a1.w + I2[8] * a2.x);
-        a0 = n0; a1 = n1; a2 = n2; a3 = n3;
+        row_update(c0, c1, c2, c3, rs + q + 1, s_lam, f1, f2, I1, I2, im1, im2);
     }
+    if (q < nr) row_update(a0, a1, a2, a3, rs + q, s_lam, f1, f2, I1, I2, im1, im2);
     lds_store_fc(s_body, b1, f1);
     if (b2 != nb) lds_store_fc(s_body, b2, f2);
 }
@@ -1169,11 +1240,10 @@ __global__ __launch_bounds__(1024) void k_order_worlds(const int *niters, int wb
     }
 }
 
-__global__ __launch_bounds__(64) void k_solve_small(DevWorlds D, StepParams P) {
-    const int w = D.order ? D.order[P.wbase + (int)blockIdx.x] : P.wbase + (int)blockIdx.x;
+// Solve stage of world w (sweeps, residual stop, warm-start records,
+// integration); s_dyn: the launch's dynamic LDS (solve_small_lds_bytes).
+__device__ __forceinline__ void solve_world(const DevWorlds &D, const StepParams &P, int w, float *s_dyn) {
     const int t = threadIdx.x;
-    extern __shared__ float4 s_dyn4[];
-    float *s_dyn = reinterpret_cast<float *>(s_dyn4);
     const int b0 = D.body_start[w], nb = D.body_start[w + 1] - b0;
     const int r0 = D.row_start[w];
     const int m = D.nrows[w];
@@ -1184,11 +1254,6 @@ __global__ __launch_bounds__(64) void k_solve_small(DevWorlds D, StepParams P) {
     float *lam = D.lambda + r0;
     const bool warm = P.warm >= 1 && D.lambda0;
     if (t == 0 && D.wtime) D.wtime[2 * w] = (long long)wall_clock64();
-
-    if (blockIdx.x == 0 && t == 0 && D.feat_count) {  // features were consumed; keep the count for readback
-        D.feat_count[2 * P.split + 1] = D.feat_count[2 * P.split];
-        D.feat_count[2 * P.split] = 0;
-    }
     // LDS path: bodies [lds_bodies][16], rows [lds_rows][16], lambda [lds_rows],
     // items [lds_rows], phase starts [lds_rows + 1] (16-bit)
     float *s_body = s_dyn;
@@ -1392,6 +1457,65 @@ __global__ __launch_bounds__(64) void k_solve_small(DevWorlds D, StepParams P) {
         D.tacc[b] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     if (t == 0 && D.wtime) D.wtime[2 * w + 1] = (long long)wall_clock64();
+}
+
+__global__ __launch_bounds__(64) void k_solve_small(DevWorlds D, StepParams P) {
+    extern __shared__ float4 s_dyn4[];
+    const int w = D.order ? D.order[P.wbase + (int)blockIdx.x] : P.wbase + (int)blockIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && D.feat_count) {  // features were consumed; keep the count
+        D.feat_count[2 * P.split + 1] = D.feat_count[2 * P.split];
+        D.feat_count[2 * P.split] = 0;
+    }
+    solve_world(D, P, w, reinterpret_cast<float *>(s_dyn4));
+}
+
+// Warm-start network over world w's nfeat feature entries (world-local slots
+// from joint_start[w]): tiles of 16 contacts on MFMA, weights read from
+// global memory (L2-resident), activations in the wave's LDS.  Writes
+// lambda0 = s (previous + predicted change) like k_mlp_infer.
+__device__ __forceinline__ void infer_world(const DevWorlds &D, const MlpDev &M, int w, int nfeat, InferBuf &tb) {
+    const int j0 = D.joint_start[w];
+    const int l = threadIdx.x & 63, o = l & 15, g = l >> 4;
+    const float *nrm = M.norm;
+    for (int s0 = 0; s0 < nfeat; s0 += 16) {
+        const int nv = min(16, nfeat - s0);
+        const float *xr = D.feat + (size_t)(j0 + s0) * FEAT_DIM;
+        const f32x4 y = forward_tile<false>(M.W1h, M.W2h, M.W3h, M.b1, M.b2, M.b3, nrm, tb, xr, nv);
+        if (o < NOUT) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int smp = 4 * g + j;
+                if (smp < nv) {
+                    const float *x = xr + (size_t)smp * FEAT_DIM;
+                    const float use = (M.gate && x[12] > 0.5f) ? 0.f : 1.f;
+                    D.lambda0[D.feat_row[j0 + s0 + smp] + o] =
+                        x[15] * (x[7 + o] + use * (y[j] * nrm[2 * NIN + NOUT + o] + nrm[2 * NIN + o]));
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Persistent fused step: each workgroup (one wave) takes one world through
+// `nsteps` complete steps -- prepare (collide, rows, schedule), the network
+// when one is attached, solve and integrate -- with no kernel boundary in
+// between, so a world that needs many sweeps in one step delays only itself.
+// LDS holds one stage at a time (prepare arrays, then the network's
+// activations, then the solve's rows).
+__global__ __launch_bounds__(64) void k_step_fused(DevWorlds D, StepParams P, MlpDev M, int use_mlp, int nsteps) {
+    extern __shared__ float4 s_dyn4[];
+    const int w = D.order ? D.order[P.wbase + (int)blockIdx.x] : P.wbase + (int)blockIdx.x;
+    for (int k = 0; k < nsteps; k++) {
+        const int nfeat = prepare_world(D, P, w, prep_lds_carve(s_dyn4));
+        __syncthreads();
+        if (use_mlp) {
+            infer_world(D, M, w, nfeat, *reinterpret_cast<InferBuf *>(s_dyn4));
+            __syncthreads();
+        }
+        solve_world(D, P, w, reinterpret_cast<float *>(s_dyn4));
+        __syncthreads();
+    }
 }
 
 }  // namespace odx
@@ -1615,6 +1739,20 @@ void launch_solve_small(const DevWorlds &D, const StepParams &P, hipStream_t s) 
     if (P.wcount <= 0) return;
     if (D.order) hipLaunchKernelGGL(k_order_worlds, dim3(1), dim3(1024), 0, s, D.niters, P.wbase, P.wcount, D.order);
     hipLaunchKernelGGL(k_solve_small, dim3(P.wcount), dim3(64), solve_small_lds_bytes(P), s, D, P);
+}
+size_t step_fused_lds_bytes(const StepParams &P) {
+    size_t b = solve_small_lds_bytes(P);
+    if (PREP_LDS_BYTES > b) b = PREP_LDS_BYTES;
+    if (sizeof(InferBuf) > b) b = sizeof(InferBuf);
+    return b;
+}
+// P.feat_local must be 1 (features in world-local slots); M may be null.
+void launch_step_fused(const DevWorlds &D, const StepParams &P, const MlpDev *M, int nsteps, hipStream_t s) {
+    if (P.wcount <= 0 || nsteps <= 0) return;
+    if (D.order) hipLaunchKernelGGL(k_order_worlds, dim3(1), dim3(1024), 0, s, D.niters, P.wbase, P.wcount, D.order);
+    const MlpDev none{};
+    hipLaunchKernelGGL(k_step_fused, dim3(P.wcount), dim3(64), step_fused_lds_bytes(P), s, D, P, M ? *M : none,
+                       M ? 1 : 0, nsteps);
 }
 }  // namespace odx
 

@@ -46,6 +46,7 @@ def lib():
         L.odxBatchSetSolver.argtypes = [vp, i, i, ctypes.c_float, i, i]
         L.odxBatchSetSolveOptions.argtypes = [vp, i, i]
         L.odxBatchSetSplits.argtypes = [vp, i, i]
+        L.odxBatchSetFused.argtypes = [vp, i]
         L.odxBatchStep.argtypes = [vp, i]
         L.odxBatchSync.argtypes = [vp]
         L.odxBatchTime.argtypes = [vp, i, fp]
@@ -58,6 +59,7 @@ def lib():
         L.odxBatchGetRows.argtypes = [vp, i, fp, i]
         L.odxBatchGetOverflow.argtypes = [vp, ip]
         L.odxBatchGetSolveTimes.argtypes = [vp, dp]
+        L.odxBatchGetPrepareTimes.argtypes = [vp, dp]
         L.odxBatchRowStarts.argtypes = [vp, ip]
         L.odxBatchSetLambda0.argtypes = [vp, fp, i]
         L.odxSetMessageHandler(_on_message)
@@ -152,6 +154,11 @@ class Batch:
         per graph launch (results equal those of one range)."""
         _check(lib().odxBatchSetSplits(self.h, nsplit, graph_steps), "odxBatchSetSplits")
 
+    def set_fused(self, steps_per_launch):
+        """> 0: that many complete steps per launch of the persistent fused
+        kernel (0: staged launches)."""
+        _check(lib().odxBatchSetFused(self.h, steps_per_launch), "odxBatchSetFused")
+
     def set_solve_options(self, restart_after=0, lds_rows=400):
         """restart_after: sweeps after which an unconverged warm start restarts
         cold (0 = never); lds_rows: row budget of the LDS-resident sweeps."""
@@ -218,6 +225,13 @@ class Batch:
         """[nworlds, 2] start/end of each world's solve in the last step (us)."""
         out = np.zeros((self.nworlds, 2), np.float64)
         _check(lib().odxBatchGetSolveTimes(self.h, _p(out)), "odxBatchGetSolveTimes")
+        return out
+
+    def prepare_times(self):
+        """[nworlds, 6] prepare-stage boundaries in the last step (us): start,
+        bodies, broadphase, narrowphase, rows, schedule."""
+        out = np.zeros((self.nworlds, 6), np.float64)
+        _check(lib().odxBatchGetPrepareTimes(self.h, _p(out)), "odxBatchGetPrepareTimes")
         return out
 
     def set_lambda0(self, lam0):

@@ -310,26 +310,43 @@ def test_unconverged_warm_start_restarts_cold(lds_rows):
         np.testing.assert_array_equal(a, b)
 
 
-@pytest.mark.parametrize("warm", [0, 2])
-def test_split_streams_and_multi_step_graphs_change_nothing(warm):
-    """World ranges stepped on separate streams inside multi-step graphs give
-    bit-identical bodies, impulses and sweep counts to one range stepped one
-    graph launch per step (worlds are independent)."""
-    say("case: splits, warm", warm)
+@pytest.mark.parametrize("warm", [0, 2, 3])
+def test_split_streams_fused_kernel_and_multi_step_graphs_change_nothing(warm):
+    """World ranges stepped on separate streams inside multi-step graphs, and
+    the persistent fused kernel (several whole steps per launch, network run
+    inside it), give bit-identical bodies, impulses and sweep counts to one
+    range stepped one graph launch per step (worlds are independent)."""
+    say("case: splits / fused, warm", warm)
     rng = np.random.default_rng(23)
     worlds = [S.uneven_world(rng, 8, 40) for _ in range(37)]
     prm = O.params()
     odx = _odx()
+    mlp = None
+    if warm == 3:
+        mlp = odx.Mlp(11)
+        X = np.random.default_rng(1).normal(size=(64, 16)).astype(np.float32)
+        norm = np.concatenate([np.zeros(16), np.ones(16), np.zeros(3), np.full(3, 0.05)]).astype(np.float32)
+        mlp.set_normalization(norm)
     out = []
-    for nsplit, gsteps in ((1, 1), (4, 3)):
+    for nsplit, gsteps, fused in ((1, 1, 0), (4, 3, 0), (1, 1, 3), (2, 4, 2)):
         B = odx.Batch(worlds, prm)
         B.set_splits(nsplit, gsteps)
+        B.set_fused(fused)
+        if mlp is not None:
+            B.set_mlp(mlp)
         B.set_solver(order=1, max_iters=3000, tol=1e-4, check_every=2, warm=warm)
         B.step(7)
         st = B.stats()
-        out.append((B.bodies(), [B.lam(wi) for wi in range(len(worlds))], st["iters"].copy()))
+        out.append((B.bodies(), [B.lam(wi) for wi in range(len(worlds))], st["iters"].copy(),
+                    B.samples()[0] if warm == 3 else None))
         B.close()
-    np.testing.assert_array_equal(out[0][0], out[1][0])
-    np.testing.assert_array_equal(out[0][2], out[1][2])
-    for a, b in zip(out[0][1], out[1][1]):
-        np.testing.assert_array_equal(a, b)
+    for k in range(1, len(out)):
+        np.testing.assert_array_equal(out[0][0], out[k][0])
+        np.testing.assert_array_equal(out[0][2], out[k][2])
+        for a, b in zip(out[0][1], out[k][1]):
+            np.testing.assert_array_equal(a, b)
+        if warm == 3:
+            # the same feature vectors, gathered in a different slot order
+            A, Bf = out[0][3], out[k][3]
+            assert A.shape == Bf.shape
+            np.testing.assert_array_equal(np.sort(A, axis=0), np.sort(Bf, axis=0))
