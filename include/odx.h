@@ -136,6 +136,54 @@ int odxBatchRowStarts(odxBatch *b, int *row_start);
 /* Warm-start values per row in the row-capacity layout (host array). */
 int odxBatchSetLambda0(odxBatch *b, const float *lambda0, int n);
 
+/* ------------------------------------------------------- one large world */
+/* One world of up to millions of bodies stepped across the whole GPU:
+ * dSpaceCollide over a dHashSpace (ode/src/collision_space.cpp:421-614; the
+ * pair set is every pair of geoms on different bodies whose closed AABBs
+ * overlap, planes tested against all), dCollide per pair
+ * (collision_kernel.cpp:292-329), one contact joint per contact
+ * (dJointCreateContact ode.cpp:1192-1200 with dJointAttach's reversal), and
+ * dWorldQuickStep (quickstep.cpp) with the rows swept colour by colour: a
+ * colour is a set of contacts with disjoint bodies, so a phase runs in
+ * parallel and a sweep is a Gauss-Seidel pass in colour order (normal rows
+ * first, then friction rows, as ReorderPrep quickstep.cpp:2329-2355).
+ * Contacts only (no ball joints).  bodies [nb][32], geoms [ng][16], params as
+ * odxBatchCreate (params[7] = sweeps per step).  pair_cap / contact_cap bound
+ * the candidate pairs and contacts of one step (<= 0: 12 and 8 per geom).
+ * Returns NULL on failure. */
+typedef struct odxLarge odxLarge;
+odxLarge *odxLargeCreate(int nb, const float *bodies, int ng, const float *geoms, const double *params, int pair_cap,
+                         int contact_cap);
+void odxLargeDestroy(odxLarge *w);
+/* Sweeps per step (dWorldSetQuickStepNumIterations, objects.h:462). */
+int odxLargeSetIterations(odxLarge *w, int iters);
+/* Queues nsteps steps (dSpaceCollide + contacts + dWorldQuickStep +
+ * dJointGroupEmpty each) on the world's stream.  Returns 1 when queued. */
+int odxLargeStep(odxLarge *w, int nsteps);
+/* Waits for the queued steps.  Returns 0 (and reports why) when a step since
+ * the last sync dropped pairs or contacts past the capacities, left contacts
+ * uncoloured or failed a grid barrier. */
+int odxLargeSync(odxLarge *w);
+/* Times nsteps steps with hipEvents (total milliseconds). */
+int odxLargeTime(odxLarge *w, int nsteps, float *total_ms);
+/* Average [prepare (bodies, broadphase, narrowphase, rows), colouring, solve]
+ * milliseconds over nsteps steps (ms[3]). */
+int odxLargeKernelTimes(odxLarge *w, int nsteps, float *ms);
+int odxLargeGetBodies(odxLarge *w, float *bodies);
+int odxLargeSetBodies(odxLarge *w, const float *bodies);
+/* Counters of the last step [16]: pairs, contacts, rows, colours, dropped
+ * pairs, dropped contacts, uncoloured contacts, barrier failure, colouring
+ * rounds. */
+int odxLargeGetCounts(odxLarge *w, int *counts);
+/* Last step's candidate pairs [n][2] (lower geom first), contact JOINT
+ * records [n][16] (14, 15 = geoms), rows [m][32] (contact k's rows are
+ * consecutive, in contact order), lambda [m] and contact colours [n]. */
+int odxLargeGetPairs(odxLarge *w, int *pairs, int cap);
+int odxLargeGetJoints(odxLarge *w, float *joints, int cap);
+int odxLargeGetRows(odxLarge *w, float *rows, int cap);
+int odxLargeGetLambda(odxLarge *w, float *lambda, int cap);
+int odxLargeGetColours(odxLarge *w, int *colour, int cap);
+
 /* ------------------------------------------------ learned warm start (MLP) */
 /* A 16 -> 128 -> 128 -> 3 GELU network on bf16 MFMA with fp32 accumulation
  * that predicts a contact's initial (normal, friction1, friction2) impulses

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libodx.so")
 
-SOURCES = ["odx_step.hip", "odx_mlp.hip", "odx_batch.cpp", "odx_api.cpp"]
+SOURCES = ["odx_step.hip", "odx_mlp.hip", "odx_batch.cpp", "odx_api.cpp", "odx_large.cpp"]
 
 
 def sources():

@@ -100,6 +100,117 @@ template <class T> static bool download(T *h, const T *d, size_t n) {
     return odx_check(hipMemcpy(h, d, n * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy D2H");
 }
 
+// Body and geom records -> device SoA of D (positions, velocities, mass,
+// inertia and its inverse, geom parameters and static poses); every
+// allocation is appended to allocs.
+bool odx_state_upload(DevWorlds &D, int nb, const float *bodies, int ng, const float *geoms,
+                      std::vector<void *> &allocs) {
+    auto alloc = [&](auto **p, size_t n) {
+        if (!dalloc(p, n)) return false;
+        allocs.push_back((void *)*p);
+        return true;
+    };
+    std::vector<float4> pos(nb), quat(nb), lvel(nb), avel(nb), facc(nb), tacc(nb), mass(nb);
+    std::vector<float> Ib((size_t)nb * 12), invIb((size_t)nb * 12);
+    for (int i = 0; i < nb; i++) {
+        const float *r = bodies + (size_t)i * BODY_STRIDE;
+        pos[i] = make_float4(r[0], r[1], r[2], 0.f);
+        float q[4] = {r[4], r[5], r[6], r[7]};
+        float l = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+        if (l > 0.f) { l = 1.f / std::sqrt(l); for (float &x : q) x *= l; }
+        else { q[0] = 1.f; q[1] = q[2] = q[3] = 0.f; }
+        quat[i] = make_float4(q[0], q[1], q[2], q[3]);
+        lvel[i] = make_float4(r[8], r[9], r[10], 0.f);
+        avel[i] = make_float4(r[12], r[13], r[14], 0.f);
+        facc[i] = make_float4(r[16], r[17], r[18], 0.f);
+        tacc[i] = make_float4(r[20], r[21], r[22], 0.f);
+        mass[i] = make_float4(r[24], 1.f / r[24], r[31], 0.f);
+        float *I = &Ib[(size_t)i * 12];
+        I[0] = r[25]; I[5] = r[26]; I[10] = r[27];
+        I[1] = I[4] = r[28]; I[2] = I[8] = r[29]; I[6] = I[9] = r[30];
+        I[3] = I[7] = I[11] = 0.f;
+        odx_invert_pd3(I, &invIb[(size_t)i * 12]);
+    }
+    std::vector<float4> g0(ng), g1(ng), gp(ng), gq(ng);
+    for (int i = 0; i < ng; i++) {
+        const float *r = geoms + (size_t)i * GEOM_STRIDE;
+        g0[i] = make_float4(r[0], r[1], r[2], r[3]);
+        g1[i] = make_float4(r[4], r[5], 0.f, 0.f);
+        gp[i] = make_float4(r[6], r[7], r[8], 0.f);
+        float q[4] = {r[9], r[10], r[11], r[12]};
+        float l = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+        if (l > 0.f) { l = 1.f / std::sqrt(l); for (float &x : q) x *= l; }
+        else { q[0] = 1.f; q[1] = q[2] = q[3] = 0.f; }
+        gq[i] = make_float4(q[0], q[1], q[2], q[3]);
+    }
+    bool ok = alloc(&D.pos, nb) && upload(D.pos, pos.data(), nb);
+    ok = ok && alloc(&D.quat, nb) && upload(D.quat, quat.data(), nb);
+    ok = ok && alloc(&D.lvel, nb) && upload(D.lvel, lvel.data(), nb);
+    ok = ok && alloc(&D.avel, nb) && upload(D.avel, avel.data(), nb);
+    ok = ok && alloc(&D.facc, nb) && upload(D.facc, facc.data(), nb);
+    ok = ok && alloc(&D.tacc, nb) && upload(D.tacc, tacc.data(), nb);
+    float4 *m4 = nullptr;
+    ok = ok && alloc(&m4, nb) && upload(m4, mass.data(), nb);
+    D.mass = m4;
+    float *f = nullptr;
+    ok = ok && alloc(&f, (size_t)nb * 12) && upload(f, invIb.data(), (size_t)nb * 12);
+    D.invIb = f;
+    ok = ok && alloc(&f, (size_t)nb * 12) && upload(f, Ib.data(), (size_t)nb * 12);
+    D.Ib = f;
+    float4 *g4 = nullptr;
+    ok = ok && alloc(&g4, ng) && upload(g4, g0.data(), ng);
+    D.geom0 = g4;
+    ok = ok && alloc(&g4, ng) && upload(g4, g1.data(), ng);
+    D.geom1 = g4;
+    ok = ok && alloc(&g4, ng) && upload(g4, gp.data(), ng);
+    D.gpos = g4;
+    ok = ok && alloc(&g4, ng) && upload(g4, gq.data(), ng);
+    D.gquat = g4;
+    return ok;
+}
+
+// device body state -> body records [nb][32]
+bool odx_bodies_download(const DevWorlds &D, int nb, float *out) {
+    std::vector<float4> pos(nb), quat(nb), lvel(nb), avel(nb), facc(nb), tacc(nb), mass(nb);
+    std::vector<float> Ib((size_t)nb * 12);
+    bool ok = download(pos.data(), D.pos, nb) && download(quat.data(), D.quat, nb) &&
+              download(lvel.data(), D.lvel, nb) && download(avel.data(), D.avel, nb) &&
+              download(facc.data(), D.facc, nb) && download(tacc.data(), D.tacc, nb) &&
+              download(mass.data(), (float4 *)D.mass, nb) && download(Ib.data(), (float *)D.Ib, (size_t)nb * 12);
+    if (!ok) return false;
+    for (int i = 0; i < nb; i++) {
+        float *r = out + (size_t)i * BODY_STRIDE;
+        std::memset(r, 0, BODY_STRIDE * sizeof(float));
+        r[0] = pos[i].x; r[1] = pos[i].y; r[2] = pos[i].z;
+        r[4] = quat[i].x; r[5] = quat[i].y; r[6] = quat[i].z; r[7] = quat[i].w;
+        r[8] = lvel[i].x; r[9] = lvel[i].y; r[10] = lvel[i].z;
+        r[12] = avel[i].x; r[13] = avel[i].y; r[14] = avel[i].z;
+        r[16] = facc[i].x; r[17] = facc[i].y; r[18] = facc[i].z;
+        r[20] = tacc[i].x; r[21] = tacc[i].y; r[22] = tacc[i].z;
+        r[24] = mass[i].x;
+        const float *I = &Ib[(size_t)i * 12];
+        r[25] = I[0]; r[26] = I[5]; r[27] = I[10]; r[28] = I[1]; r[29] = I[2]; r[30] = I[6];
+        r[31] = mass[i].z;
+    }
+    return true;
+}
+
+// body records -> device pose, velocities and accumulators (mass unchanged)
+bool odx_bodies_upload(const DevWorlds &D, int nb, const float *bodies) {
+    std::vector<float4> pos(nb), quat(nb), lvel(nb), avel(nb), facc(nb), tacc(nb);
+    for (int i = 0; i < nb; i++) {
+        const float *r = bodies + (size_t)i * BODY_STRIDE;
+        pos[i] = make_float4(r[0], r[1], r[2], 0.f);
+        quat[i] = make_float4(r[4], r[5], r[6], r[7]);
+        lvel[i] = make_float4(r[8], r[9], r[10], 0.f);
+        avel[i] = make_float4(r[12], r[13], r[14], 0.f);
+        facc[i] = make_float4(r[16], r[17], r[18], 0.f);
+        tacc[i] = make_float4(r[20], r[21], r[22], 0.f);
+    }
+    return upload(D.pos, pos.data(), nb) && upload(D.quat, quat.data(), nb) && upload(D.lvel, lvel.data(), nb) &&
+           upload(D.avel, avel.data(), nb) && upload(D.facc, facc.data(), nb) && upload(D.tacc, tacc.data(), nb);
+}
+
 // The batch's worlds may be split into ranges stepped on streams of their own
 // inside one graph: worlds are independent, so while one range finishes its
 // slowest solve the others keep the GPU busy with their next steps.
@@ -238,7 +349,7 @@ static int odx_lds_rows_fit(int lds_bodies, int want) {
     return std::max(0, std::min(want, std::min(room, SW_MAXROWS)));
 }
 
-static void params_from(const double *p, StepParams &P, Surface &S) {
+void odx_params_from(const double *p, StepParams &P, Surface &S) {
     P.h = (float)p[0];
     P.gx = (float)p[1]; P.gy = (float)p[2]; P.gz = (float)p[3];
     P.erp = (float)p[4]; P.cfm = (float)p[5]; P.sor_w = (float)p[6];
@@ -333,53 +444,17 @@ odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies
     }
     B->jcap = B->joint_start[nworlds];
     B->rcap = B->row_start[nworlds];
-    params_from(params, B->P, B->surf);
+    odx_params_from(params, B->P, B->surf);
     int maxnb = 0;
     for (int w = 0; w < nworlds; w++) maxnb = std::max(maxnb, B->body_start[w + 1] - B->body_start[w]);
     B->P.lds_bodies = maxnb + 1;
     B->P.lds_rows = odx_lds_rows_fit(B->P.lds_bodies, ODX_DEFAULT_LDS_ROWS);
     B->P.restart_after = 0;
 
-    // host-side SoA conversion
-    const int nb = B->nb, ng = B->ng;
-    std::vector<float4> pos(nb), quat(nb), lvel(nb), avel(nb), facc(nb), tacc(nb), mass(nb);
-    std::vector<float> Ib((size_t)nb * 12), invIb((size_t)nb * 12);
-    for (int i = 0; i < nb; i++) {
-        const float *r = bodies + (size_t)i * BODY_STRIDE;
-        pos[i] = make_float4(r[0], r[1], r[2], 0.f);
-        float q[4] = {r[4], r[5], r[6], r[7]};
-        float l = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
-        if (l > 0.f) { l = 1.f / std::sqrt(l); for (float &x : q) x *= l; }
-        else { q[0] = 1.f; q[1] = q[2] = q[3] = 0.f; }
-        quat[i] = make_float4(q[0], q[1], q[2], q[3]);
-        lvel[i] = make_float4(r[8], r[9], r[10], 0.f);
-        avel[i] = make_float4(r[12], r[13], r[14], 0.f);
-        facc[i] = make_float4(r[16], r[17], r[18], 0.f);
-        tacc[i] = make_float4(r[20], r[21], r[22], 0.f);
-        mass[i] = make_float4(r[24], 1.f / r[24], r[31], 0.f);
-        float *I = &Ib[(size_t)i * 12];
-        I[0] = r[25]; I[5] = r[26]; I[10] = r[27];
-        I[1] = I[4] = r[28]; I[2] = I[8] = r[29]; I[6] = I[9] = r[30];
-        I[3] = I[7] = I[11] = 0.f;
-        odx_invert_pd3(I, &invIb[(size_t)i * 12]);
-    }
-    std::vector<float4> g0(ng), g1(ng), gp(ng), gq(ng);
-    for (int i = 0; i < ng; i++) {
-        const float *r = geoms + (size_t)i * GEOM_STRIDE;
-        g0[i] = make_float4(r[0], r[1], r[2], r[3]);
-        g1[i] = make_float4(r[4], r[5], 0.f, 0.f);
-        gp[i] = make_float4(r[6], r[7], r[8], 0.f);
-        float q[4] = {r[9], r[10], r[11], r[12]};
-        float l = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
-        if (l > 0.f) { l = 1.f / std::sqrt(l); for (float &x : q) x *= l; }
-        else { q[0] = 1.f; q[1] = q[2] = q[3] = 0.f; }
-        gq[i] = make_float4(q[0], q[1], q[2], q[3]);
-    }
-
     DevWorlds &D = B->D;
     D.nworlds = nworlds;
     int *ds = nullptr;
-    bool ok = true;
+    bool ok = odx_state_upload(D, B->nb, bodies, B->ng, geoms, B->allocs);
     ok = ok && B->alloc(&ds, nworlds + 1) && upload(ds, B->body_start.data(), nworlds + 1);
     D.body_start = ds;
     ok = ok && B->alloc(&ds, nworlds + 1) && upload(ds, B->geom_start.data(), nworlds + 1);
@@ -390,29 +465,7 @@ odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies
     D.joint_start = ds;
     ok = ok && B->alloc(&ds, nworlds + 1) && upload(ds, B->row_start.data(), nworlds + 1);
     D.row_start = ds;
-    ok = ok && B->alloc(&D.pos, nb) && upload(D.pos, pos.data(), nb);
-    ok = ok && B->alloc(&D.quat, nb) && upload(D.quat, quat.data(), nb);
-    ok = ok && B->alloc(&D.lvel, nb) && upload(D.lvel, lvel.data(), nb);
-    ok = ok && B->alloc(&D.avel, nb) && upload(D.avel, avel.data(), nb);
-    ok = ok && B->alloc(&D.facc, nb) && upload(D.facc, facc.data(), nb);
-    ok = ok && B->alloc(&D.tacc, nb) && upload(D.tacc, tacc.data(), nb);
-    float4 *m4 = nullptr;
-    ok = ok && B->alloc(&m4, nb) && upload(m4, mass.data(), nb);
-    D.mass = m4;
     float *f = nullptr;
-    ok = ok && B->alloc(&f, (size_t)nb * 12) && upload(f, invIb.data(), (size_t)nb * 12);
-    D.invIb = f;
-    ok = ok && B->alloc(&f, (size_t)nb * 12) && upload(f, Ib.data(), (size_t)nb * 12);
-    D.Ib = f;
-    float4 *g4 = nullptr;
-    ok = ok && B->alloc(&g4, ng) && upload(g4, g0.data(), ng);
-    D.geom0 = g4;
-    ok = ok && B->alloc(&g4, ng) && upload(g4, g1.data(), ng);
-    D.geom1 = g4;
-    ok = ok && B->alloc(&g4, ng) && upload(g4, gp.data(), ng);
-    D.gpos = g4;
-    ok = ok && B->alloc(&g4, ng) && upload(g4, gq.data(), ng);
-    D.gquat = g4;
     ok = ok && B->alloc(&f, (size_t)B->nball * JOINT_STRIDE) &&
          upload(f, balls ? balls : (const float *)nullptr, balls ? (size_t)B->nball * JOINT_STRIDE : 0);
     D.balls = f;
@@ -425,7 +478,7 @@ odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies
     ok = ok && B->alloc(&D.rows, (size_t)B->rcap * ROW_STRIDE) && B->alloc(&D.lambda, B->rcap);
     ok = ok && B->alloc(&D.sched, B->rcap) && B->alloc(&D.phase_start, (size_t)B->rcap + nworlds);
     ok = ok && B->alloc(&D.perm, B->rcap) && B->alloc(&D.inv, B->rcap);
-    ok = ok && B->alloc(&D.bscr, (size_t)nb * BSCR_STRIDE);
+    ok = ok && B->alloc(&D.bscr, (size_t)B->nb * BSCR_STRIDE);
     ok = ok && B->alloc(&D.feat, (size_t)B->jcap * FEAT_DIM) && B->alloc(&D.feat_row, B->jcap) &&
          B->alloc(&D.feat_count, 2 * (ODX_MAX_SPLITS + 1)) && B->alloc(&D.joff, B->jcap);
     ok = ok && B->alloc(&D.prev, (size_t)B->jcap * JOINT_STRIDE) && B->alloc(&D.nprev, nworlds) &&
@@ -603,49 +656,13 @@ int odxBatchKernelTimes(odxBatch *b, int nsteps, float *kernel_ms) {
 int odxBatchGetBodies(odxBatch *b, float *out) {
     if (!b || !out) return 0;
     if (!odxBatchSync(b)) return 0;
-    const int nb = b->nb;
-    std::vector<float4> pos(nb), quat(nb), lvel(nb), avel(nb), facc(nb), tacc(nb), mass(nb);
-    std::vector<float> Ib((size_t)nb * 12);
-    bool ok = download(pos.data(), b->D.pos, nb) && download(quat.data(), b->D.quat, nb) &&
-              download(lvel.data(), b->D.lvel, nb) && download(avel.data(), b->D.avel, nb) &&
-              download(facc.data(), b->D.facc, nb) && download(tacc.data(), b->D.tacc, nb) &&
-              download(mass.data(), (float4 *)b->D.mass, nb) && download(Ib.data(), (float *)b->D.Ib, (size_t)nb * 12);
-    if (!ok) return 0;
-    for (int i = 0; i < nb; i++) {
-        float *r = out + (size_t)i * BODY_STRIDE;
-        std::memset(r, 0, BODY_STRIDE * sizeof(float));
-        r[0] = pos[i].x; r[1] = pos[i].y; r[2] = pos[i].z;
-        r[4] = quat[i].x; r[5] = quat[i].y; r[6] = quat[i].z; r[7] = quat[i].w;
-        r[8] = lvel[i].x; r[9] = lvel[i].y; r[10] = lvel[i].z;
-        r[12] = avel[i].x; r[13] = avel[i].y; r[14] = avel[i].z;
-        r[16] = facc[i].x; r[17] = facc[i].y; r[18] = facc[i].z;
-        r[20] = tacc[i].x; r[21] = tacc[i].y; r[22] = tacc[i].z;
-        r[24] = mass[i].x;
-        const float *I = &Ib[(size_t)i * 12];
-        r[25] = I[0]; r[26] = I[5]; r[27] = I[10]; r[28] = I[1]; r[29] = I[2]; r[30] = I[6];
-        r[31] = mass[i].z;
-    }
-    return nb;
+    return odx_bodies_download(b->D, b->nb, out) ? b->nb : 0;
 }
 
 int odxBatchSetBodies(odxBatch *b, const float *bodies) {
     if (!b || !bodies) return 0;
     if (!odxBatchSync(b)) return 0;
-    const int nb = b->nb;
-    std::vector<float4> pos(nb), quat(nb), lvel(nb), avel(nb), facc(nb), tacc(nb);
-    for (int i = 0; i < nb; i++) {
-        const float *r = bodies + (size_t)i * BODY_STRIDE;
-        pos[i] = make_float4(r[0], r[1], r[2], 0.f);
-        quat[i] = make_float4(r[4], r[5], r[6], r[7]);
-        lvel[i] = make_float4(r[8], r[9], r[10], 0.f);
-        avel[i] = make_float4(r[12], r[13], r[14], 0.f);
-        facc[i] = make_float4(r[16], r[17], r[18], 0.f);
-        tacc[i] = make_float4(r[20], r[21], r[22], 0.f);
-    }
-    bool ok = upload(b->D.pos, pos.data(), nb) && upload(b->D.quat, quat.data(), nb) &&
-              upload(b->D.lvel, lvel.data(), nb) && upload(b->D.avel, avel.data(), nb) &&
-              upload(b->D.facc, facc.data(), nb) && upload(b->D.tacc, tacc.data(), nb);
-    return ok ? nb : 0;
+    return odx_bodies_upload(b->D, b->nb, bodies) ? b->nb : 0;
 }
 
 int odxBatchGetStats(odxBatch *b, int *ncontacts, int *nrows, int *niters, float *residual) {

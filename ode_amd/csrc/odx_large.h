@@ -1,0 +1,54 @@
+// odx_large.h -- device view of one large world (hash-space broadphase,
+// colour-parallel SOR-LCP over the whole chip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace odx {
+
+// Device counters of a large-world step.
+enum {
+    LG_PAIRS = 0,        // candidate pairs kept
+    LG_CONTACTS,         // contacts kept
+    LG_ROWS,             // constraint rows
+    LG_NCOL,             // colours of the sweep schedule
+    LG_PAIR_OVF,         // pairs beyond the pair capacity (dropped)
+    LG_CONTACT_OVF,      // contacts beyond the contact capacity (dropped)
+    LG_COLOUR_OVF,       // contacts left uncoloured after LG_MAXCOL colours
+    LG_BARRIER_ERR,      // a grid barrier wait ran out (the step is invalid)
+    LG_SUBROUNDS,        // claim/commit rounds the colouring took
+    LG_NCOUNTERS = 16
+};
+
+constexpr int LG_MAXCOL = 128;        // colours of the sweep schedule
+constexpr int LG_MAXSUB = 32;         // claim/commit rounds per colour
+constexpr long long LG_SPIN_LIMIT = 1ll << 21;  // polls of one barrier wait (~1 s)
+constexpr int LG_BLOCK = 256;         // threads per workgroup of the cooperative kernels
+
+struct LargeDev {
+    int nb, ng, nbig, hmask;          // bodies, geoms, big geoms, hash buckets - 1 (power of two)
+    float inv_cell;                   // 1 / cell edge; the edge bounds every finite AABB
+    int pair_cap, contact_cap;
+    float *aabb;                      // [ng * 6]
+    const int *isbig;                 // [ng] 1: infinite or oversized AABB, tested against every geom
+    const int *big;                   // [nbig] geom index of each big geom (ascending)
+    int4 *gcell;                      // [ng] cell x, y, z and hash bucket of each finite geom
+    int *bucket;                      // [hmask + 2] counts -> starts -> ends
+    int *bucket_geom;                 // [ng] finite geoms grouped by bucket, ascending index in a bucket
+    int *pair_cnt;                    // [ng + 1] pairs per geom -> first pair of each geom
+    int2 *pairs;                      // [pair_cap] (lower geom, higher geom)
+    int *contact_cnt;                 // [pair_cap + 1] contacts per pair -> first contact of each pair
+    int *row_cnt;                     // [contact_cap + 1] rows per contact -> first row of each contact
+    int *counters;                    // [LG_NCOUNTERS]
+    int2 *cbody;                      // [contact_cap] bodies of each contact (body 2 = -1: static)
+    int *colour;                      // [contact_cap]
+    int *taken;                       // [nb] last colour that holds the body
+    unsigned long long *owner;        // [nb] claim of the current round (stamp | priority | contact)
+    int *elig;                        // [LG_MAXCOL * LG_MAXSUB] contacts taking part in each round
+    int *col_start;                   // [LG_MAXCOL + 1] contacts per colour -> first slot of each colour
+    int *corder;                      // [contact_cap] contacts in colour order
+    unsigned *bar;                    // [2 + LG_MAXCOL] grid barrier (arrivals, generation), colour fill
+    float *fc;                        // [nb * 6] body accumulators of the solve
+    int *scan_tmp;                    // block sums of the scans
+};
+
+}  // namespace odx

@@ -1242,6 +1242,41 @@ __global__ __launch_bounds__(1024) void k_order_worlds(const int *niters, int wb
 
 // Solve stage of world w (sweeps, residual stop, warm-start records,
 // integration); s_dyn: the launch's dynamic LDS (solve_small_lds_bytes).
+// Stage4b (v += h fc), Stage6a (v += h M^-1 f), Stage6b + dxStepBody
+// (x += h v, q += h dq(w, q), normalised; f = t = 0) of body b,
+// quickstep.cpp:3082-3102, 3299-3337, util.cpp:583-699 (infinitesimal rotation).
+__device__ __forceinline__ void integrate_body(const DevWorlds &D, const StepParams &P, int b, const float *fc,
+                                               bool rows) {
+    const float h = P.h;
+    const float *s = D.bscr + (size_t)b * BSCR_STRIDE;
+    float4 p4 = D.pos[b], q4 = D.quat[b], lv = D.lvel[b], av = D.avel[b], fa = D.facc[b], ta = D.tacc[b];
+    if (rows) {
+        lv.x += h * fc[0]; lv.y += h * fc[1]; lv.z += h * fc[2];
+        av.x += h * fc[3]; av.y += h * fc[4]; av.z += h * fc[5];
+    }
+    const float ims = h * s[30];
+    lv.x += ims * fa.x; lv.y += ims * fa.y; lv.z += ims * fa.z;
+    float tq[3] = {ta.x * h, ta.y * h, ta.z * h}, dw[3];
+    mul0_331(dw, s + 12, tq);
+    av.x += dw[0]; av.y += dw[1]; av.z += dw[2];
+    p4.x += h * lv.x; p4.y += h * lv.y; p4.z += h * lv.z;
+    float dq0 = 0.5f * (-av.x * q4.y - av.y * q4.z - av.z * q4.w);
+    float dq1 = 0.5f * (av.x * q4.x + av.y * q4.w - av.z * q4.z);
+    float dq2 = 0.5f * (-av.x * q4.w + av.y * q4.x + av.z * q4.y);
+    float dq3 = 0.5f * (av.x * q4.z - av.y * q4.y + av.z * q4.x);
+    q4.x += h * dq0; q4.y += h * dq1; q4.z += h * dq2; q4.w += h * dq3;
+    float l2 = q4.x * q4.x + q4.y * q4.y + q4.z * q4.z + q4.w * q4.w;
+    if (l2 > 0.f) {
+        const float l = 1.f / sqrtf(l2);
+        q4.x *= l; q4.y *= l; q4.z *= l; q4.w *= l;
+    } else {
+        q4 = make_float4(1.f, 0.f, 0.f, 0.f);
+    }
+    D.pos[b] = p4; D.quat[b] = q4; D.lvel[b] = lv; D.avel[b] = av;
+    D.facc[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+    D.tacc[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 __device__ __forceinline__ void solve_world(const DevWorlds &D, const StepParams &P, int w, float *s_dyn) {
     const int t = threadIdx.x;
     const int b0 = D.body_start[w], nb = D.body_start[w + 1] - b0;
@@ -1422,39 +1457,11 @@ __device__ __forceinline__ void solve_world(const DevWorlds &D, const StepParams
     }
 
     // Stage4b, Stage6a, Stage6b + dxStepBody (infinitesimal rotation)
-    const float h = P.h;
     for (int i = t; i < nb; i += 64) {
-        const int b = b0 + i;
-        const float *s = D.bscr + (size_t)b * BSCR_STRIDE;
-        float4 p4 = D.pos[b], q4 = D.quat[b], lv = D.lvel[b], av = D.avel[b], fa = D.facc[b], ta = D.tacc[b];
         float fc[6];
         if (lds) lds_fc(s_body, i, fc);
         else for (int k = 0; k < 6; k++) fc[k] = s_fc[i * 6 + k];
-        if (m > 0) {
-            lv.x += h * fc[0]; lv.y += h * fc[1]; lv.z += h * fc[2];
-            av.x += h * fc[3]; av.y += h * fc[4]; av.z += h * fc[5];
-        }
-        const float ims = h * s[30];
-        lv.x += ims * fa.x; lv.y += ims * fa.y; lv.z += ims * fa.z;
-        float tq[3] = {ta.x * h, ta.y * h, ta.z * h}, dw[3];
-        mul0_331(dw, s + 12, tq);
-        av.x += dw[0]; av.y += dw[1]; av.z += dw[2];
-        p4.x += h * lv.x; p4.y += h * lv.y; p4.z += h * lv.z;
-        float dq0 = 0.5f * (-av.x * q4.y - av.y * q4.z - av.z * q4.w);
-        float dq1 = 0.5f * (av.x * q4.x + av.y * q4.w - av.z * q4.z);
-        float dq2 = 0.5f * (-av.x * q4.w + av.y * q4.x + av.z * q4.y);
-        float dq3 = 0.5f * (av.x * q4.z - av.y * q4.y + av.z * q4.x);
-        q4.x += h * dq0; q4.y += h * dq1; q4.z += h * dq2; q4.w += h * dq3;
-        float l2 = q4.x * q4.x + q4.y * q4.y + q4.z * q4.z + q4.w * q4.w;
-        if (l2 > 0.f) {
-            const float l = 1.f / sqrtf(l2);
-            q4.x *= l; q4.y *= l; q4.z *= l; q4.w *= l;
-        } else {
-            q4 = make_float4(1.f, 0.f, 0.f, 0.f);
-        }
-        D.pos[b] = p4; D.quat[b] = q4; D.lvel[b] = lv; D.avel[b] = av;
-        D.facc[b] = make_float4(0.f, 0.f, 0.f, 0.f);
-        D.tacc[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+        integrate_body(D, P, b0 + i, fc, m > 0);
     }
     if (t == 0 && D.wtime) D.wtime[2 * w + 1] = (long long)wall_clock64();
 }
@@ -1776,5 +1783,566 @@ void launch_collide_pairs(const float *gviews, const int *pairs, int np, int max
                           hipStream_t s) {
     if (np <= 0) return;
     hipLaunchKernelGGL(k_collide_pairs, dim3((np + 127) / 128), dim3(128), 0, s, gviews, pairs, np, maxc, out, counts);
+}
+}  // namespace odx
+
+// ========================================================= one large world ===
+// A single world of up to millions of bodies (hash-space broadphase).  The
+// per-body, per-geom, per-pair and per-contact device functions of the batched
+// path run one thread per item over the whole world; colouring and the solve
+// are cooperative persistent kernels whose phases are separated by grid
+// barriers.  The solve sweeps colour phases (contacts whose bodies are
+// disjoint), rows streamed from HBM, body accumulators fc in global memory.
+#include "odx_large.h"
+
+namespace odx {
+
+__device__ __forceinline__ int lg_hash(int x, int y, int z, int mask) {
+    unsigned h = (unsigned)x * 73856093u ^ (unsigned)y * 19349663u ^ (unsigned)z * 83492791u;
+    h ^= h >> 15; h *= 0x2c1b3c6du; h ^= h >> 12;
+    return (int)(h & (unsigned)mask);
+}
+
+__global__ void k_lg_fill(int *p, int n, int v) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Exclusive scan of one 1024-thread block's values; returns the exclusive
+// prefix of v, *total gets the block sum.
+__device__ int lg_block_scan(int v, int *s_tmp, int *total) {
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6, nw = blockDim.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_tmp[wv] = x;
+    __syncthreads();
+    if (t < 64) {
+        int w = t < nw ? s_tmp[t] : 0;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(w, o, 64);
+            if (lane >= o) w += y;
+        }
+        if (t < nw) s_tmp[t] = w;
+    }
+    __syncthreads();
+    const int r = (wv > 0 ? s_tmp[wv - 1] : 0) + x - v;
+    *total = s_tmp[nw - 1];
+    __syncthreads();
+    return r;
+}
+
+// In-place exclusive scan of data[0, n), n = min(*dn + 1, cap) when dn is
+// given (a count the device wrote; entry *dn is zero), else cap.  Blocks of
+// 2048 entries; k_lg_scan_sums scans the block sums in chunks of 2048.
+__device__ __forceinline__ int lg_scan_n(const int *dn, int cap) { return dn ? min(*dn + 1, cap) : cap; }
+
+__global__ __launch_bounds__(1024) void k_lg_scan_blocks(int *data, int cap, const int *dn, int *sums) {
+    __shared__ int s_tmp[64];
+    const int n = lg_scan_n(dn, cap);
+    const int i0 = blockIdx.x * 2048 + 2 * threadIdx.x;
+    const int a = i0 < n ? data[i0] : 0, b = i0 + 1 < n ? data[i0 + 1] : 0;
+    int total;
+    const int ex = lg_block_scan(a + b, s_tmp, &total);
+    if (i0 < n) data[i0] = ex;
+    if (i0 + 1 < n) data[i0 + 1] = ex + a;
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_lg_scan_sums(int *sums, int nblk) {
+    __shared__ int s_tmp[64];
+    int carry = 0;
+    for (int base = 0; base < nblk; base += 2048) {
+        const int i0 = base + 2 * threadIdx.x;
+        const int a = i0 < nblk ? sums[i0] : 0, b = i0 + 1 < nblk ? sums[i0 + 1] : 0;
+        int total;
+        const int ex = lg_block_scan(a + b, s_tmp, &total);
+        if (i0 < nblk) sums[i0] = carry + ex;
+        if (i0 + 1 < nblk) sums[i0 + 1] = carry + ex + a;
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_lg_scan_add(int *data, int cap, const int *dn, const int *sums) {
+    const int n = lg_scan_n(dn, cap);
+    const int add = sums[blockIdx.x];
+    const int i0 = blockIdx.x * 2048 + 2 * threadIdx.x;
+    if (i0 < n) data[i0] += add;
+    if (i0 + 1 < n) data[i0 + 1] += add;
+}
+
+// scan of at most cap entries (the grid covers cap; blocks past n do nothing)
+void lg_scan(int *data, int cap, const int *dn, int *sums, hipStream_t s) {
+    const int nblk = (cap + 2047) / 2048;
+    hipLaunchKernelGGL(k_lg_scan_blocks, dim3(nblk), dim3(1024), 0, s, data, cap, dn, sums);
+    hipLaunchKernelGGL(k_lg_scan_sums, dim3(1), dim3(1024), 0, s, sums, nblk);
+    hipLaunchKernelGGL(k_lg_scan_add, dim3(nblk), dim3(1024), 0, s, data, cap, dn, sums);
+}
+
+__global__ void k_lg_bodies(DevWorlds D, StepParams P, int nb) {
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) body_prep(D, P, b);
+}
+
+// AABBs; each finite geom gets the cell of its AABB centre and counts into
+// that cell's hash bucket
+__global__ void k_lg_aabb(DevWorlds D, LargeDev L) {
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < L.ng; g += gridDim.x * blockDim.x) {
+        DGeom G;
+        load_geom(D, g, 0, G);
+        float ab[6];
+        geom_aabb(G, ab);
+        for (int k = 0; k < 6; k++) L.aabb[(size_t)g * 6 + k] = ab[k];
+        if (L.isbig[g]) continue;
+        const int cx = (int)floorf(0.5f * (ab[0] + ab[1]) * L.inv_cell);
+        const int cy = (int)floorf(0.5f * (ab[2] + ab[3]) * L.inv_cell);
+        const int cz = (int)floorf(0.5f * (ab[4] + ab[5]) * L.inv_cell);
+        const int h = lg_hash(cx, cy, cz, L.hmask);
+        L.gcell[g] = make_int4(cx, cy, cz, h);
+        atomicAdd(&L.bucket[h], 1);
+    }
+}
+
+// after the exclusive scan bucket[h] is h's start; the scatter advances it to
+// h's end, so bucket h spans [h ? bucket[h - 1] : 0, bucket[h])
+__global__ void k_lg_scatter(LargeDev L) {
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < L.ng; g += gridDim.x * blockDim.x) {
+        if (L.isbig[g]) continue;
+        L.bucket_geom[atomicAdd(&L.bucket[L.gcell[g].w], 1)] = g;
+    }
+}
+
+// geoms of a bucket in index order (deterministic pair order)
+__global__ void k_lg_sort_buckets(LargeDev L) {
+    for (int h = blockIdx.x * blockDim.x + threadIdx.x; h <= L.hmask; h += gridDim.x * blockDim.x) {
+        const int s = h ? L.bucket[h - 1] : 0, e = L.bucket[h];
+        for (int i = s + 1; i < e; i++) {
+            const int v = L.bucket_geom[i];
+            int j = i - 1;
+            while (j >= s && L.bucket_geom[j] > v) { L.bucket_geom[j + 1] = L.bucket_geom[j]; j--; }
+            L.bucket_geom[j + 1] = v;
+        }
+    }
+}
+
+__device__ __forceinline__ bool lg_overlap(const float *x, const float *y) {
+    return !(x[0] > y[1] || x[1] < y[0] || x[2] > y[3] || x[3] < y[2] || x[4] > y[5] || x[5] < y[4]);
+}
+
+// pair filter of collideAABBs (collision_space_internal.h:44-78): not the same
+// body; pairs of two static geoms make no rows (their joint is in no island)
+// and are left out, as in the batched path
+__device__ __forceinline__ bool lg_bodies_ok(int bi, int bj) { return (bi >= 0 || bj >= 0) && bi != bj; }
+
+// Candidate pairs of the hash space (collision_space.cpp:421-614: every pair
+// whose closed AABBs overlap).  From each finite geom i: geoms j > i of the 27
+// cells around it (the cell edge bounds every finite AABB, so the centres of
+// overlapping AABBs lie in adjacent cells; a bucket's geoms count only when
+// their cell matches exactly), then every big geom.  A big geom i pairs with
+// the big geoms after it.  write = 0 counts into pair_cnt[i]; write = 1 stores
+// at the scanned offsets.
+__global__ void k_lg_pairs(DevWorlds D, LargeDev L, int write) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < L.ng; i += gridDim.x * blockDim.x) {
+        const float *ai = L.aabb + (size_t)i * 6;
+        const int bi = (int)D.geom0[i].y;
+        const bool ibig = L.isbig[i] != 0;
+        const int off = write ? L.pair_cnt[i] : 0;
+        int n = 0;
+        if (!ibig) {
+            const int4 c = L.gcell[i];
+            for (int dz = -1; dz <= 1; dz++)
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dx = -1; dx <= 1; dx++) {
+                        const int x = c.x + dx, y = c.y + dy, z = c.z + dz;
+                        const int h = lg_hash(x, y, z, L.hmask);
+                        const int s = h ? L.bucket[h - 1] : 0, e = L.bucket[h];
+                        for (int k = s; k < e; k++) {
+                            const int j = L.bucket_geom[k];
+                            if (j <= i) continue;
+                            const int4 cj = L.gcell[j];
+                            if (cj.x != x || cj.y != y || cj.z != z) continue;
+                            if (!lg_bodies_ok(bi, (int)D.geom0[j].y)) continue;
+                            if (!lg_overlap(ai, L.aabb + (size_t)j * 6)) continue;
+                            if (write && off + n < L.pair_cap) L.pairs[off + n] = make_int2(i, j);
+                            n++;
+                        }
+                    }
+        }
+        for (int q = 0; q < L.nbig; q++) {
+            const int j = L.big[q];
+            if (ibig && j <= i) continue;
+            if (!lg_bodies_ok(bi, (int)D.geom0[j].y)) continue;
+            if (!lg_overlap(ai, L.aabb + (size_t)j * 6)) continue;
+            if (write && off + n < L.pair_cap) L.pairs[off + n] = make_int2(min(i, j), max(i, j));
+            n++;
+        }
+        if (!write) L.pair_cnt[i] = n;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (!write) {
+            L.pair_cnt[L.ng] = 0;
+        } else {
+            const int np = L.pair_cnt[L.ng];
+            L.counters[LG_PAIRS] = min(np, L.pair_cap);
+            if (np > L.pair_cap) L.counters[LG_PAIR_OVF] = np - L.pair_cap;
+        }
+    }
+}
+
+// dCollide of every pair (collision_kernel.cpp:292-329); contacts become JOINT
+// records, the two bodies swapped and the normal negated when body 1 is
+// static (dJointAttach ode.cpp:1383-1439, contact.cpp:160-167).
+__global__ void k_lg_contacts(DevWorlds D, StepParams P, LargeDev L, int write) {
+    const int np = L.counters[LG_PAIRS];
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p <= np; p += gridDim.x * blockDim.x) {
+        if (p == np) {
+            if (!write) L.contact_cnt[np] = 0;
+            continue;
+        }
+        const int2 pr = L.pairs[p];
+        DGeom A, B;
+        load_geom(D, pr.x, 0, A);
+        load_geom(D, pr.y, 0, B);
+        DContact c[ODX_MAXC];
+        const int n = (A.body < 0 && B.body < 0) ? 0 : dcollide(A, B, min(P.maxc, ODX_MAXC), c);
+        if (!write) { L.contact_cnt[p] = n; continue; }
+        const int off = L.contact_cnt[p];
+        int b1 = A.body, b2 = B.body;
+        float sg = 1.f;
+        if (b1 < 0) { b1 = b2; b2 = -1; sg = -1.f; }
+        for (int q = 0; q < n && off + q < L.contact_cap; q++) {
+            float4 *o4 = reinterpret_cast<float4 *>(D.joints + (size_t)(off + q) * JOINT_STRIDE);
+            o4[0] = make_float4(0.f, (float)b1, (float)b2, c[q].pos[0]);
+            o4[1] = make_float4(c[q].pos[1], c[q].pos[2], sg * c[q].normal[0], sg * c[q].normal[1]);
+            o4[2] = make_float4(sg * c[q].normal[2], c[q].depth, 0.f, 0.f);
+            o4[3] = make_float4(0.f, 0.f, (float)pr.x, (float)pr.y);
+            L.cbody[off + q] = make_int2(b1, b2);
+        }
+    }
+    if (write && blockIdx.x == 0 && threadIdx.x == 0) {
+        const int nc = L.contact_cnt[np];
+        L.counters[LG_CONTACTS] = min(nc, L.contact_cap);
+        if (nc > L.contact_cap) L.counters[LG_CONTACT_OVF] = nc - L.contact_cap;
+    }
+}
+
+__global__ void k_lg_rowcount(DevWorlds D, LargeDev L) {
+    const int nc = L.counters[LG_CONTACTS];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k <= nc; k += gridDim.x * blockDim.x)
+        L.row_cnt[k] = k < nc ? contact_rows_m(D.surf[(int)D.joints[(size_t)k * JOINT_STRIDE + 13]]) : 0;
+}
+
+// getInfo2, Stage2a/2c, compute_invM_JT and Ad of every contact, at the
+// contact's scanned first row (rows of contact k are consecutive; a friction
+// row's findex points at its normal row)
+__global__ void k_lg_rows(DevWorlds D, StepParams P, LargeDev L) {
+    const int nc = L.counters[LG_CONTACTS];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x)
+        build_joint_rows(D, P, D.joints + (size_t)k * JOINT_STRIDE, 0, 0, L.row_cnt[k]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) L.counters[LG_ROWS] = L.row_cnt[nc];
+}
+
+// Grid barrier of the cooperative kernels: arrivals counter + generation.  A
+// wait that runs out sets LG_BARRIER_ERR; every block then leaves at its next
+// barrier, the host reports the step as failed.  Returns false on error.
+// `read`, when given, is loaded by one thread after the barrier and returned
+// to every thread in *val.
+__device__ bool lg_grid_sync(const LargeDev &L, unsigned &gen, const int *read = nullptr, int *val = nullptr) {
+    __shared__ int s_ok, s_val;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        __threadfence();
+        const unsigned arrived = atomicAdd(&L.bar[0], 1u) + 1u;
+        if (arrived == gridDim.x) {
+            atomicExch(&L.bar[0], 0u);
+            __threadfence();
+            atomicAdd(&L.bar[1], 1u);
+        } else {
+            long long spins = 0;
+            while (__hip_atomic_load(&L.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                if (++spins > LG_SPIN_LIMIT ||
+                    __hip_atomic_load(&L.counters[LG_BARRIER_ERR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    atomicExch(&L.counters[LG_BARRIER_ERR], 1);
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        gen++;
+        __threadfence();
+        if (__hip_atomic_load(&L.counters[LG_BARRIER_ERR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = 0;
+        if (read) s_val = __hip_atomic_load(read, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_ok = ok;
+    }
+    __syncthreads();
+    if (read) *val = s_val;
+    return s_ok != 0;
+}
+
+// sum over the block (every thread gets it)
+__device__ int lg_block_sum(int v, int *s_red) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if (lane == 0) s_red[wv] = v;
+    __syncthreads();
+    int s = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += s_red[i];
+    return s;
+}
+
+// Claim key of contact k in claim round `stamp`: rounds increase, so a claim
+// always beats those left by earlier rounds and owner[] needs no reset.
+__device__ __forceinline__ unsigned long long lg_key(int k, int stamp) {
+    unsigned h = (unsigned)k * 2654435761u ^ (unsigned)(stamp + 1) * 0x9e3779b9u;
+    h ^= h >> 16; h *= 0x7feb352du; h ^= h >> 15;
+    return ((unsigned long long)stamp << 50) | ((unsigned long long)(h & 0x3ffffffu) << 24) |
+           (unsigned long long)(k & 0xffffff);
+}
+
+// Colouring (cooperative): colour c is a maximal set of contacts with
+// disjoint dynamic bodies, built by claim/commit rounds (Luby): every
+// uncoloured contact whose bodies no contact of colour c holds claims both
+// with its key (atomicMax); one holding both claims joins colour c.  Then the
+// contacts are grouped by colour: corder[col_start[c] ..) are colour c's.
+// Keys hash the contact index (< 2^24) and the round, so the result is
+// deterministic.
+__global__ __launch_bounds__(LG_BLOCK) void k_lg_colour(DevWorlds D, LargeDev L) {
+    __shared__ int s_red[LG_BLOCK / 64];
+    __shared__ int s_hist[LG_MAXCOL], s_base[LG_MAXCOL], s_start[LG_MAXCOL + 1];
+    unsigned gen = 0;
+    const int nc = L.counters[LG_CONTACTS];
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x, gs = gridDim.x * blockDim.x;
+    for (int k = gt; k < nc; k += gs) L.colour[k] = -1;
+    for (int b = gt; b < L.nb; b += gs) { L.taken[b] = -1; L.owner[b] = 0ull; }
+    bool ok = lg_grid_sync(L, gen);
+    int ncol = 0, rounds = 0, left = nc;
+    for (int c = 0; ok && left > 0 && c < LG_MAXCOL; c++) {
+        for (int sub = 0; ok && sub < LG_MAXSUB; sub++) {
+            const int stamp = c * LG_MAXSUB + sub;
+            int e = 0;
+            for (int k = gt; k < nc; k += gs) {
+                if (L.colour[k] >= 0) continue;
+                const int2 bb = L.cbody[k];
+                if (L.taken[bb.x] == c || (bb.y >= 0 && L.taken[bb.y] == c)) continue;
+                const unsigned long long key = lg_key(k, stamp);
+                atomicMax(&L.owner[bb.x], key);
+                if (bb.y >= 0) atomicMax(&L.owner[bb.y], key);
+                e++;
+            }
+            e = lg_block_sum(e, s_red);
+            if (threadIdx.x == 0 && e) atomicAdd(&L.elig[stamp], e);
+            int elig = 0;
+            ok = lg_grid_sync(L, gen, &L.elig[stamp], &elig);
+            if (!ok) break;
+            rounds++;
+            if (elig == 0) break;
+            int won = 0;
+            for (int k = gt; k < nc; k += gs) {
+                if (L.colour[k] >= 0) continue;
+                const int2 bb = L.cbody[k];
+                const unsigned long long key = lg_key(k, stamp);
+                if (L.owner[bb.x] != key || (bb.y >= 0 && L.owner[bb.y] != key)) continue;
+                L.colour[k] = c;
+                L.taken[bb.x] = c;
+                if (bb.y >= 0) L.taken[bb.y] = c;
+                won++;
+            }
+            won = lg_block_sum(won, s_red);
+            if (threadIdx.x == 0 && won) atomicAdd(&L.col_start[c], won);
+            ok = lg_grid_sync(L, gen);
+        }
+        if (!ok) break;
+        ncol = c + 1;
+        int nmem = 0;   // contacts of colour c (complete: its last round ended with a barrier)
+        __syncthreads();
+        if (threadIdx.x == 0) s_hist[0] = __hip_atomic_load(&L.col_start[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        nmem = s_hist[0];
+        left -= nmem;
+    }
+    if (!ok) return;
+    // group by colour: per-block histogram, block base per colour, scan
+    for (int c = threadIdx.x; c < LG_MAXCOL; c += blockDim.x) s_hist[c] = 0;
+    __syncthreads();
+    for (int k = gt; k < nc; k += gs) {
+        const int c = L.colour[k];
+        if (c >= 0) atomicAdd(&s_hist[c], 1);
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < ncol; c += blockDim.x) s_base[c] = s_hist[c] ? (int)atomicAdd(&L.bar[2 + c], (unsigned)s_hist[c]) : 0;
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int c = 0; c < ncol; c++) {
+            s_start[c] = acc;
+            acc += __hip_atomic_load(&L.col_start[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_start[ncol] = acc;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < LG_MAXCOL; c += blockDim.x) s_hist[c] = 0;
+    __syncthreads();
+    for (int k = gt; k < nc; k += gs) {
+        const int c = L.colour[k];
+        if (c >= 0) L.corder[s_start[c] + s_base[c] + atomicAdd(&s_hist[c], 1)] = k;
+    }
+    ok = lg_grid_sync(L, gen);   // every block has read col_start[] counts
+    if (!ok) return;
+    if (blockIdx.x == 0) {
+        for (int c = threadIdx.x; c <= ncol; c += blockDim.x) L.col_start[c] = s_start[c];
+        if (threadIdx.x == 0) {
+            L.counters[LG_NCOL] = ncol;
+            L.counters[LG_SUBROUNDS] = rounds;
+            if (left > 0) L.counters[LG_COLOUR_OVF] = left;
+        }
+    }
+}
+
+// rows [r, r + n) of one contact with fc in global memory (within a colour
+// phase no other thread touches the contact's bodies)
+__device__ __forceinline__ void lg_rows(const DevWorlds &D, float *fc, int r, int n) {
+    const float4 *row4 = reinterpret_cast<const float4 *>(D.rows + (size_t)r * ROW_STRIDE);
+    const float4 ids = row4[7];
+    const int b1 = as_int(ids.x), b2 = as_int(ids.y);
+    float f1[6], f2[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < 6; k++) f1[k] = fc[(size_t)b1 * 6 + k];
+    if (b2 >= 0)
+        for (int k = 0; k < 6; k++) f2[k] = fc[(size_t)b2 * 6 + k];
+    for (int q = 0; q < n; q++) {
+        const float4 *rr = row4 + q * 8;
+        const float4 a0 = rr[0], a1 = rr[1], a2 = rr[2], a3 = rr[3], a4 = rr[4], a5 = rr[5], a6 = rr[6], a7 = rr[7];
+        const float old = D.lambda[r + q];
+        float delta = a1.z - old * a1.w;
+        delta -= f1[0] * a0.x + f1[1] * a0.y + f1[2] * a0.z + f1[3] * a0.w + f1[4] * a1.x + f1[5] * a1.y;
+        if (b2 >= 0) delta -= f2[0] * a2.x + f2[1] * a2.y + f2[2] * a2.z + f2[3] * a2.w + f2[4] * a3.x + f2[5] * a3.y;
+        float lo = a3.z, hi = a3.w;
+        const int fidx = as_int(a7.z);
+        if (fidx >= 0) { hi = fabsf(hi * D.lambda[fidx]); lo = -hi; }
+        float nl = old + delta;
+        if (nl < lo) { delta = lo - old; nl = lo; }
+        else if (nl > hi) { delta = hi - old; nl = hi; }
+        D.lambda[r + q] = nl;
+        f1[0] += delta * a4.x; f1[1] += delta * a4.y; f1[2] += delta * a4.z;
+        f1[3] += delta * a4.w; f1[4] += delta * a5.x; f1[5] += delta * a5.y;
+        if (b2 >= 0) {
+            f2[0] += delta * a5.z; f2[1] += delta * a5.w; f2[2] += delta * a6.x;
+            f2[3] += delta * a6.y; f2[4] += delta * a6.z; f2[5] += delta * a6.w;
+        }
+    }
+    for (int k = 0; k < 6; k++) fc[(size_t)b1 * 6 + k] = f1[k];
+    if (b2 >= 0)
+        for (int k = 0; k < 6; k++) fc[(size_t)b2 * 6 + k] = f2[k];
+}
+
+// Split of a contact's rows into the two passes of ReorderPrep
+// (quickstep.cpp:2329-2355): the normal row has no findex; friction rows with
+// findex follow it (a contact without Approx1 keeps all rows in pass one).
+__device__ __forceinline__ int lg_pass_a(const DevWorlds &D, int r, int n) {
+    return (n > 1 && as_int(D.rows[(size_t)(r + 1) * ROW_STRIDE + R_FINDEX]) >= 0) ? 1 : n;
+}
+
+// The solve (cooperative): lambda = 0, `iters` sweeps, each the contacts'
+// first-pass rows colour by colour, then their findex rows colour by colour,
+// then Stage4b/6 and dxStepBody of every body.
+__global__ __launch_bounds__(LG_BLOCK) void k_lg_solve(DevWorlds D, StepParams P, LargeDev L, int iters) {
+    unsigned gen = 0;
+    const int nc = L.counters[LG_CONTACTS];
+    const int m = L.counters[LG_ROWS];
+    const int ncol = L.counters[LG_NCOL];
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x, gs = gridDim.x * blockDim.x;
+    for (int i = gt; i < L.nb * 6; i += gs) L.fc[i] = 0.f;
+    for (int i = gt; i < m; i += gs) D.lambda[i] = 0.f;
+    bool ok = lg_grid_sync(L, gen);
+    for (int it = 0; ok && it < iters && nc > 0; it++) {
+        for (int pass = 0; ok && pass < 2; pass++) {
+            for (int c = 0; ok && c < ncol; c++) {
+                const int s = L.col_start[c], e = L.col_start[c + 1];
+                for (int idx = s + gt; idx < e; idx += gs) {
+                    const int k = L.corder[idx];
+                    const int r = L.row_cnt[k], n = L.row_cnt[k + 1] - r;
+                    const int na = lg_pass_a(D, r, n);
+                    if (pass == 0) lg_rows(D, L.fc, r, na);
+                    else if (n > na) lg_rows(D, L.fc, r + na, n - na);
+                }
+                ok = lg_grid_sync(L, gen);
+            }
+        }
+    }
+    if (!ok) return;
+    for (int b = gt; b < L.nb; b += gs) integrate_body(D, P, b, L.fc + (size_t)b * 6, nc > 0);
+}
+
+// largest value each counter reached since the host last cleared sticky
+__global__ void k_lg_fold(const int *counters, int *sticky) {
+    const int i = threadIdx.x;
+    if (i < LG_NCOUNTERS) sticky[i] = max(sticky[i], counters[i]);
+}
+
+// ---- launchers ----
+void lg_enqueue_fold(const LargeDev &L, int *sticky, hipStream_t s) {
+    hipLaunchKernelGGL(k_lg_fold, dim3(1), dim3(64), 0, s, L.counters, sticky);
+}
+
+// workgroups of a cooperative launch: every one resident at once
+int lg_coop_blocks(const void *kernel) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, LG_BLOCK, 0) != hipSuccess) return 0;
+    return cus * (per < 2 ? per : 2);
+}
+int lg_colour_blocks() { return lg_coop_blocks((const void *)k_lg_colour); }
+int lg_solve_blocks() { return lg_coop_blocks((const void *)k_lg_solve); }
+
+// The prepare stage of a step: bodies, broadphase, narrowphase, rows.
+// Capacities: ng <= pair scan cap etc. are checked by the host.
+void lg_enqueue_prepare(const DevWorlds &D, const StepParams &P, const LargeDev &L, hipStream_t s) {
+    const int G = 1024, T = 256;
+    hipLaunchKernelGGL(k_lg_fill, dim3(G), dim3(T), 0, s, L.bucket, L.hmask + 2, 0);
+    hipLaunchKernelGGL(k_lg_fill, dim3(1), dim3(T), 0, s, L.counters, LG_NCOUNTERS, 0);
+    hipLaunchKernelGGL(k_lg_bodies, dim3(G), dim3(T), 0, s, D, P, L.nb);
+    hipLaunchKernelGGL(k_lg_aabb, dim3(G), dim3(T), 0, s, D, L);
+    lg_scan(L.bucket, L.hmask + 2, nullptr, L.scan_tmp, s);
+    hipLaunchKernelGGL(k_lg_scatter, dim3(G), dim3(T), 0, s, L);
+    hipLaunchKernelGGL(k_lg_sort_buckets, dim3(G), dim3(T), 0, s, L);
+    hipLaunchKernelGGL(k_lg_pairs, dim3(G), dim3(T), 0, s, D, L, 0);
+    lg_scan(L.pair_cnt, L.ng + 1, nullptr, L.scan_tmp, s);
+    hipLaunchKernelGGL(k_lg_pairs, dim3(G), dim3(T), 0, s, D, L, 1);
+    hipLaunchKernelGGL(k_lg_contacts, dim3(G), dim3(T), 0, s, D, P, L, 0);
+    lg_scan(L.contact_cnt, L.pair_cap + 1, L.counters + LG_PAIRS, L.scan_tmp, s);
+    hipLaunchKernelGGL(k_lg_contacts, dim3(G), dim3(T), 0, s, D, P, L, 1);
+    hipLaunchKernelGGL(k_lg_rowcount, dim3(G), dim3(T), 0, s, D, L);
+    lg_scan(L.row_cnt, L.contact_cap + 1, L.counters + LG_CONTACTS, L.scan_tmp, s);
+    hipLaunchKernelGGL(k_lg_rows, dim3(G), dim3(T), 0, s, D, P, L);
+}
+
+bool lg_enqueue_colour(const DevWorlds &D, const LargeDev &L, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_lg_fill, dim3(16), dim3(256), 0, s, L.elig, LG_MAXCOL * LG_MAXSUB, 0);
+    hipLaunchKernelGGL(k_lg_fill, dim3(1), dim3(256), 0, s, L.col_start, LG_MAXCOL + 1, 0);
+    hipLaunchKernelGGL(k_lg_fill, dim3(1), dim3(256), 0, s, reinterpret_cast<int *>(L.bar), 2 + LG_MAXCOL, 0);
+    if (hipGetLastError() != hipSuccess) return false;
+    DevWorlds d = D;
+    LargeDev l = L;
+    void *args[] = {(void *)&d, (void *)&l};
+    return hipLaunchCooperativeKernel((const void *)k_lg_colour, dim3(blocks), dim3(LG_BLOCK), args, 0, s) ==
+           hipSuccess;
+}
+
+bool lg_enqueue_solve(const DevWorlds &D, const StepParams &P, const LargeDev &L, int iters, int blocks,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_lg_fill, dim3(1), dim3(256), 0, s, reinterpret_cast<int *>(L.bar), 2, 0);
+    if (hipGetLastError() != hipSuccess) return false;
+    DevWorlds d = D;
+    StepParams p = P;
+    LargeDev l = L;
+    int n = iters;
+    void *args[] = {(void *)&d, (void *)&p, (void *)&l, (void *)&n};
+    return hipLaunchCooperativeKernel((const void *)k_lg_solve, dim3(blocks), dim3(LG_BLOCK), args, 0, s) ==
+           hipSuccess;
 }
 }  // namespace odx

@@ -62,6 +62,19 @@ def lib():
         L.odxBatchGetPrepareTimes.argtypes = [vp, dp]
         L.odxBatchRowStarts.argtypes = [vp, ip]
         L.odxBatchSetLambda0.argtypes = [vp, fp, i]
+        L.odxLargeCreate.restype = ctypes.c_void_p
+        L.odxLargeCreate.argtypes = [i, fp, i, fp, dp, i, i]
+        L.odxLargeDestroy.argtypes = [vp]
+        for name in ("odxLargeSetIterations", "odxLargeStep"):
+            getattr(L, name).argtypes = [vp, i]
+        L.odxLargeSync.argtypes = [vp]
+        for name in ("odxLargeTime", "odxLargeKernelTimes"):
+            getattr(L, name).argtypes = [vp, i, fp]
+        for name in ("odxLargeGetBodies", "odxLargeSetBodies", "odxLargeGetCounts"):
+            getattr(L, name).argtypes = [vp, fp]
+        for name in ("odxLargeGetPairs", "odxLargeGetJoints", "odxLargeGetRows", "odxLargeGetLambda",
+                     "odxLargeGetColours"):
+            getattr(L, name).argtypes = [vp, fp, i]
         L.odxSetMessageHandler(_on_message)
         _lib = L
     return _lib
@@ -237,6 +250,90 @@ class Batch:
     def set_lambda0(self, lam0):
         a = np.ascontiguousarray(lam0, np.float32)
         _check(lib().odxBatchSetLambda0(self.h, _p(a), len(a)), "odxBatchSetLambda0")
+
+
+class Large:
+    """One large world (hash-space broadphase, colour-parallel QuickStep over
+    the whole GPU; odxLarge*)."""
+
+    COUNTERS = ("pairs", "contacts", "rows", "colours", "pair_overflow", "contact_overflow", "uncoloured",
+                "barrier_error", "colour_rounds")
+
+    def __init__(self, world, params, pair_cap=0, contact_cap=0):
+        L = lib()
+        self.bodies0 = np.ascontiguousarray(world.bodies, np.float32)
+        self.geoms = np.ascontiguousarray(world.geoms, np.float32)
+        self.nbodies, self.ngeoms = len(self.bodies0), len(self.geoms)
+        self.params = np.ascontiguousarray(params, np.float64)
+        h = L.odxLargeCreate(self.nbodies, _p(self.bodies0), self.ngeoms, _p(self.geoms), _p(self.params), pair_cap,
+                             contact_cap)
+        _check(h, "odxLargeCreate")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().odxLargeDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_iterations(self, iters):
+        _check(lib().odxLargeSetIterations(self.h, iters), "odxLargeSetIterations")
+
+    def step(self, n=1):
+        _check(lib().odxLargeStep(self.h, n), "odxLargeStep")
+
+    def sync(self):
+        _check(lib().odxLargeSync(self.h), "odxLargeSync")
+
+    def time(self, n):
+        out = np.zeros(1, np.float32)
+        _check(lib().odxLargeTime(self.h, n, _p(out)), "odxLargeTime")
+        return float(out[0])
+
+    def kernel_times(self, n):
+        """Average [prepare, colouring, solve] milliseconds per step."""
+        out = np.zeros(3, np.float32)
+        _check(lib().odxLargeKernelTimes(self.h, n, _p(out)), "odxLargeKernelTimes")
+        return float(out[0]), float(out[1]), float(out[2])
+
+    def bodies(self):
+        out = np.zeros((self.nbodies, BODY), np.float32)
+        _check(lib().odxLargeGetBodies(self.h, _p(out)), "odxLargeGetBodies")
+        return out
+
+    def set_bodies(self, bodies):
+        b = np.ascontiguousarray(bodies, np.float32)
+        _check(lib().odxLargeSetBodies(self.h, _p(b)), "odxLargeSetBodies")
+
+    def counts(self):
+        out = np.zeros(16, np.int32)
+        _check(lib().odxLargeGetCounts(self.h, _p(out)), "odxLargeGetCounts")
+        return {k: int(out[i]) for i, k in enumerate(self.COUNTERS)}
+
+    def _get(self, fn, shape, dtype, n):
+        out = np.zeros((max(n, 1),) + shape, dtype)
+        k = getattr(lib(), fn)(self.h, _p(out), n)
+        return out[:k]
+
+    def pairs(self):
+        return self._get("odxLargeGetPairs", (2,), np.int32, self.counts()["pairs"])
+
+    def joints(self):
+        return self._get("odxLargeGetJoints", (JOINT,), np.float32, self.counts()["contacts"])
+
+    def rows(self):
+        return self._get("odxLargeGetRows", (ROW,), np.float32, self.counts()["rows"])
+
+    def lam(self):
+        return self._get("odxLargeGetLambda", (), np.float32, self.counts()["rows"])
+
+    def colours(self):
+        return self._get("odxLargeGetColours", (), np.int32, self.counts()["contacts"])
 
 
 NPARAM = 19075

@@ -233,3 +233,41 @@ def sphere_pour(n=200000, rmin=0.05, rmax=0.15, half=5.0, density=1000.0, seed=4
     geoms.append(plane(0, 1, 0, -half))
     geoms.append(plane(0, -1, 0, -half))
     return World(bodies, geoms)
+
+
+def mixed_pile(n=1500, spacing=0.2, seed=6, density=1000.0, walls=True):
+    """A packed pile of spheres (r 0.05-0.15 m), boxes (sides 0.1-0.2 m) and
+    capsules (r 0.05 m, length 0.15 m) on a jittered cubic lattice of the given
+    spacing, so that neighbours touch or overlap and the bottom layer rests on
+    the z=0 plane; four walls bound the lattice's columns."""
+    rng = np.random.default_rng(seed)
+    per = max(2, int(round(n ** (1.0 / 3.0))))
+    half = 0.5 * per * spacing
+    bodies, geoms = [], []
+    for i in range(n):
+        lx, ly, lz = i % per, (i // per) % per, i // (per * per)
+        pos = (-half + spacing * (lx + 0.5) + rng.uniform(-0.01, 0.01),
+               -half + spacing * (ly + 0.5) + rng.uniform(-0.01, 0.01),
+               0.09 + spacing * lz + rng.uniform(-0.01, 0.01))
+        kind = rng.integers(0, 3)
+        if kind == 0:
+            r = rng.uniform(0.05, 0.15)
+            m, I = sphere_mass(density, r)
+            bodies.append(_body(pos, (1, 0, 0, 0), m, I))
+            geoms.append(geom(SPHERE, i, (r,)))
+        elif kind == 1:
+            s = rng.uniform(0.1, 0.2, size=3)
+            m, I = box_mass(density, *s)
+            bodies.append(_body(pos, _randq(rng), m, I))
+            geoms.append(geom(BOX, i, tuple(s)))
+        else:
+            m, I = capsule_mass(density, 0.05, 0.15)
+            bodies.append(_body(pos, _randq(rng), m, I))
+            geoms.append(geom(CAPSULE, i, (0.05, 0.15)))
+    geoms.append(plane(0, 0, 1, 0))
+    if walls:
+        geoms.append(plane(1, 0, 0, -half))
+        geoms.append(plane(-1, 0, 0, -half))
+        geoms.append(plane(0, 1, 0, -half))
+        geoms.append(plane(0, -1, 0, -half))
+    return World(bodies, geoms)

@@ -1,0 +1,240 @@
+"""Parity of the HIP large-world step (odxLarge*) with the CPU oracle.
+
+The hash-space pair set and the contacts are compared with the oracle's
+(dxHashSpace semantics, dCollide).  The solve sweeps contacts colour by
+colour, normal rows first and friction rows after; giving the oracle's
+QuickStep the contacts in the GPU's colour order makes its ReorderPrep order
+(non-findex rows, then findex rows, each in joint order) the same
+Gauss-Seidel order, so impulses and body states after the step must match the
+float64 oracle within the stated tolerance.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from ode_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+TOL_STATE = 1e-4     # positions, quaternions, velocities after one step (relative)
+TOL_LAMBDA = 1e-4    # impulses after one step, float32 GPU vs float64 oracle (relative to max |lambda|)
+TOL_CONTACT_F64 = 1e-4  # contact positions / normals / depths, float32 GPU vs the float64 oracle
+AABB_EPS = 1e-5      # pairs whose AABB overlap margin is below this may differ (float32 rounding of R)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def relerr(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    if a.size == 0:
+        return 0.0
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-12))
+
+
+def make(world, prm, **kw):
+    from ode_amd import odx
+    return odx.Large(world, prm, **kw)
+
+
+def overlap_margin(ab, i, j):
+    x, y = ab[i], ab[j]
+    return min(min(x[1], y[1]) - max(x[0], y[0]), min(x[3], y[3]) - max(x[2], y[2]),
+               min(x[5], y[5]) - max(x[4], y[4]))
+
+
+def colour_order(colours):
+    return np.argsort(colours, kind="stable")
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_pairs_contacts_and_coloured_step_match_oracle(seed):
+    say("case: large mixed pile seed", seed)
+    w = S.mixed_pile(1500, seed=60 + seed)
+    prm = O.params(space=1, w=1.0)
+    L = make(w, prm)
+    L.step(1)
+    L.sync()
+    cnt = L.counts()
+    assert cnt["pair_overflow"] == 0 and cnt["contact_overflow"] == 0 and cnt["uncoloured"] == 0
+    assert cnt["barrier_error"] == 0
+
+    # broadphase: the hash-space pair set (pairs of two static geoms make no rows and are left out)
+    body = w.geoms[:, 1].astype(np.int32)
+    ab = O.aabbs(w.geoms, w.bodies, dtype=np.float32).astype(np.float64)
+    ref = O.pairs(ab.astype(np.float32), body, space="hash")
+    ref = {(int(a), int(b)) for a, b in ref if body[a] >= 0 or body[b] >= 0}
+    got = [tuple(map(int, p)) for p in L.pairs()]
+    assert len(got) == len(set(got)), "duplicate pairs"
+    got = set(got)
+    for a, b in got:
+        assert a < b
+    diff = got ^ ref
+    assert all(abs(overlap_margin(ab, a, b)) < AABB_EPS for a, b in diff), sorted(diff)[:5]
+    assert len(diff) <= max(2, len(ref) // 1000)
+
+    # narrowphase: the same contacts per pair (oracle contacts in float64)
+    ref_b, ref_c, _ = O.world_step(w.bodies, w.geoms, prm, dtype=np.float64)
+    gj = L.joints()
+    assert len(gj) == cnt["contacts"]
+    key = lambda g1, g2: (int(g1), int(g2))
+    by_pair = {}
+    for r in ref_c:
+        by_pair.setdefault(key(r[7], r[8]), []).append(r)
+    gpu_pair = {}
+    for r in gj:
+        gpu_pair.setdefault(key(r[14], r[15]), []).append(r)
+    assert set(by_pair) == set(gpu_pair)
+    err = np.zeros(3)
+    for k, rs in by_pair.items():
+        gs = gpu_pair[k]
+        assert len(gs) == len(rs), k
+        for g, r in zip(gs, rs):
+            sg = -1.0 if body[k[0]] < 0 else 1.0   # dJointAttach reversal negates the normal
+            err = np.maximum(err, [np.max(np.abs(g[3:6] - r[0:3])) / max(1.0, np.max(np.abs(r[0:3]))),
+                                   np.max(np.abs(g[6:9] - sg * r[3:6])), abs(g[9] - r[6])])
+    say("contact errors (pos, normal, depth) vs float64:", err)
+    assert np.all(err < TOL_CONTACT_F64), err
+
+    # colouring: every contact coloured, no dynamic body twice in one colour
+    col = L.colours()
+    assert len(col) == len(gj) and col.min() >= 0 and col.max() < cnt["colours"]
+    b1 = gj[:, 1].astype(int)
+    b2 = gj[:, 2].astype(int)
+    for c in range(cnt["colours"]):
+        sel = col == c
+        bs = np.concatenate([b1[sel], b2[sel][b2[sel] >= 0]])
+        assert len(bs) == len(np.unique(bs)), "colour %d shares a body" % c
+    say("contacts", len(gj), "colours", cnt["colours"], "rounds", cnt["colour_rounds"])
+
+    # solve + integration: the oracle's QuickStep with the contacts in colour order
+    order = colour_order(col)
+    joints = gj[order].astype(np.float64)
+    joints[:, 0] = 0
+    ob, olam = O.quickstep(w.bodies, joints, prm, dtype=np.float64)
+    lam = L.lam()
+    rc = np.zeros(len(gj) + 1, int)
+    rc[1:] = np.cumsum(np.full(len(gj), 3))
+    assert len(lam) == rc[-1] == len(olam)
+    glam = np.concatenate([lam[rc[k]:rc[k + 1]] for k in order])
+    assert relerr(glam, olam) < TOL_LAMBDA, relerr(glam, olam)
+    gb = L.bodies()
+    assert relerr(gb[:, 0:3], ob[:, 0:3]) < TOL_STATE
+    assert relerr(gb[:, 4:8], ob[:, 4:8]) < TOL_STATE
+    vs = np.hstack([gb[:, 8:11], gb[:, 12:15]])
+    vr = np.hstack([ob[:, 8:11], ob[:, 12:15]])
+    assert np.max(np.abs(vs - vr)) / max(np.max(np.abs(vr)), 1.0) < TOL_STATE
+    L.close()
+
+
+def test_rows_match_oracle_contact_rows():
+    say("case: large rows")
+    w = S.mixed_pile(300, seed=70)
+    prm = O.params(space=1)
+    L = make(w, prm)
+    L.step(1)
+    L.sync()
+    gj = L.joints()
+    rows = L.rows()
+    assert len(rows) == 3 * len(gj)
+    # normal rows: lo 0, hi inf; friction rows: mu bound, findex -> the contact's normal row
+    np.testing.assert_array_equal(rows[0::3, 14], 0)
+    assert np.all(np.isinf(rows[0::3, 15]))
+    np.testing.assert_allclose(rows[1::3, 15], 0.5)
+    np.testing.assert_array_equal(rows[1::3, 30].view(np.int32), np.arange(0, len(rows), 3))
+    np.testing.assert_array_equal(rows[2::3, 30].view(np.int32), np.arange(0, len(rows), 3))
+    np.testing.assert_array_equal(rows[0::3, 28].view(np.int32), gj[:, 1].astype(np.int32))
+    np.testing.assert_array_equal(rows[0::3, 29].view(np.int32), gj[:, 2].astype(np.int32))
+    # Jacobians (stored times Ad, Ad in the last slot) against the oracle's getInfo2 rows
+    surf = np.zeros(16)
+    surf[0], surf[1] = prm[O.P_MODE], prm[O.P_MU]
+    bodies = w.bodies.astype(np.float64)
+    for k in range(0, len(gj), max(1, len(gj) // 40)):
+        g = gj[k].astype(np.float64)
+        b1, b2 = int(g[1]), int(g[2])
+        contact = np.concatenate([g[3:6], g[6:9], [g[9]]])
+        J, fi = O.contact_rows(contact, surf, bodies[b1], bodies[b2] if b2 >= 0 else None, fps=1.0 / prm[O.P_H],
+                               erp=prm[O.P_ERP])
+        assert len(J) == 3
+        for q in range(3):
+            row = rows[3 * k + q].astype(np.float64)
+            ad = row[31]
+            np.testing.assert_allclose(row[0:6] / ad, J[q][0:6], rtol=1e-4, atol=1e-5)
+            if b2 >= 0:
+                np.testing.assert_allclose(row[8:14] / ad, J[q][8:14], rtol=1e-4, atol=1e-5)
+    L.close()
+
+
+def test_step_is_deterministic():
+    say("case: large determinism")
+    w = S.mixed_pile(800, seed=71)
+    prm = O.params(space=1, iters=10)
+    out = []
+    for _ in range(2):
+        L = make(w, prm)
+        L.step(3)
+        L.sync()
+        out.append((L.bodies(), L.colours(), L.lam()))
+        L.close()
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_more_sweeps_lower_the_lcp_residual():
+    say("case: large residual")
+    w = S.mixed_pile(600, seed=72)
+    res = []
+    for iters in (5, 60):
+        prm = O.params(space=1, iters=iters, w=1.0)
+        L = make(w, prm)
+        L.step(1)
+        L.sync()
+        gj = L.joints().astype(np.float64)
+        col = L.colours()
+        lam = L.lam().astype(np.float64)
+        # natural row order of the joint list as given
+        r = O.lcp_residual(w.bodies, gj, prm, lam)
+        res.append(r)
+        lamr = lam.reshape(-1, 3)
+        assert np.all(lamr[:, 0] >= 0)
+        assert np.all(np.abs(lamr[:, 1:]) <= 0.5 * lamr[:, 0:1] * (1 + 1e-5) + 1e-6)
+        assert len(col) == len(gj)
+        L.close()
+    say("residual 5 / 60 sweeps", res)
+    assert res[1] < 0.5 * res[0]
+
+
+def test_capacity_overflow_fails_the_sync():
+    say("case: large overflow")
+    from ode_amd import odx
+    w = S.mixed_pile(300, seed=73)
+    prm = O.params(space=1)
+    L = make(w, prm, contact_cap=16)
+    L.step(1)
+    with pytest.raises(odx.OdxError, match="contacts"):
+        L.sync()
+    assert L.counts()["contact_overflow"] > 0
+    L.close()
+
+
+def test_sphere_pour_steps_and_settles():
+    say("case: sphere pour 20000")
+    w = S.sphere_pour(20000, seed=4)
+    prm = O.params(space=1, mu=0.3, iters=30, w=1.0)
+    L = make(w, prm)
+    L.step(40)
+    L.sync()
+    cnt = L.counts()
+    b = L.bodies()
+    assert np.all(np.isfinite(b[:, :15]))
+    assert cnt["contacts"] > 500 and cnt["uncoloured"] == 0
+    # nothing leaves the open box through the walls or the floor
+    assert np.all(b[:, 2] > -0.05)
+    assert np.all(np.abs(b[:, 0:2]) < 5.0 + 0.05)
+    say("counts", cnt)
+    L.close()
