@@ -9,9 +9,10 @@ Workloads (seeded synthetic scenes, ode_amd/scenes.py):
   towers  4,096 worlds of a 32-box tower plus a projectile sphere; the learned
           warm start (trained first on the GPU on 2M samples of the same
           configuration with other seeds) vs cold PGS at the same residual.
-          value = learned warm-start world-steps/s.            (configs[4])
+          value = learned warm-start world-steps/s.
   uneven  4,096 worlds of 8-64 spheres/boxes dropped on a plane, cold PGS.
-                                                                 (configs[1])
+  large   one world of 200,000 spheres poured into an open box (hash space,
+          30 sweeps per step); value = world-steps/s.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload towers]
 """
@@ -125,6 +126,88 @@ def measure(B, args, warm, nworlds):
     }
 
 
+def bench_large(args):
+    """One world of 200,000 spheres (radius 0.05-0.15 m) poured from a lattice
+    into a 10x10 m open box of 5 static planes; hash-space broadphase, mu=0.3,
+    30 sweeps per step.  The pour runs `warmup` steps before the timed
+    `steps`."""
+    from ode_amd import odx
+    from ode_amd import scenes as S
+    import oracle as O
+    bodies = args.bodies
+    world = S.sphere_pour(bodies, seed=args.seed)
+    prm = odx.params_vector(iters=args.large_iters, w=args.sor_w, mu=0.3, space=1)
+    L = odx.Large(world, prm)
+    t0 = time.perf_counter()
+    done = 0
+    while done < args.warmup:
+        n = min(50, args.warmup - done)
+        L.step(n)
+        L.sync()
+        done += n
+        print("bench: pour %d/%d steps, %d contacts" % (done, args.warmup, L.counts()["contacts"]),
+              file=sys.stderr, flush=True)
+    pour_s = time.perf_counter() - t0
+    total_ms = L.time(args.steps)
+    cnt = L.counts()
+    ks = [L.kernel_times(1) for _ in range(args.kernel_steps)]
+    c2 = L.counts()
+    km = np.mean(np.array(ks), axis=0)
+    rows = c2["rows"]
+    solve_gbs = rows * args.large_iters * ROW_SWEEP_BYTES / (km[2] * 1e-3) / 1e9
+    ms = total_ms / args.steps
+    out = {
+        "metric": "world-steps/sec, one large world, %d sweeps per step, 1 MI355X" % args.large_iters,
+        "value": 1000.0 / ms,
+        "unit": "world-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "large", "bodies": bodies, "geoms": bodies + 5, "space": "hash",
+                   "sweeps": args.large_iters, "mu": 0.3, "erp": 0.2, "cfm": 1e-5, "h": 0.01, "sor_w": args.sor_w,
+                   "seed": args.seed, "pour_steps": args.warmup},
+        "contacts_per_step": cnt["contacts"],
+        "contacts_per_s": cnt["contacts"] / (ms * 1e-3),
+        "counts": cnt,
+        "kernel_ms": {"prepare": float(km[0]), "colour": float(km[1]), "solve": float(km[2])},
+        "pour_s": pour_s,
+        "roofline": {
+            "bound": "hbm", "achieved": solve_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": solve_gbs / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "k_lg_solve (%d rows x %d sweeps x 228 B per launch)" % (rows, args.large_iters),
+        },
+    }
+    tpath = os.path.join(ROOT, "profiles", "k_lg_solve_large%d.json" % bodies)
+    if os.path.exists(tpath):
+        t = json.load(open(tpath))
+        out["roofline"]["traffic"] = t["traffic_bytes_per_launch"]
+        out["roofline"]["traffic_source"] = os.path.relpath(tpath, ROOT)
+    if not args.no_cpu:
+        # the oracle's QuickStep (rows, 30 sweeps, integration) over the GPU's
+        # contacts of the current state; broadphase and narrowphase are left out
+        # (the oracle's hash pairs are all-pairs), so the CPU rate is optimistic
+        b = L.bodies().astype(np.float64)
+        j = L.joints().astype(np.float64)
+        p64 = O.params(space=1, mu=0.3, iters=args.large_iters, w=args.sor_w)
+        t0 = time.perf_counter()
+        O.quickstep(b, j, p64, dtype=np.float32)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 1.0 / dt, "unit": "world-steps/s", "cores": 1, "kind": "port",
+                               "sample": "1 step of the whole world: the oracle's float32 QuickStep over the "
+                                         "GPU's %d contacts (rows, %d sweeps, integration; no broadphase or "
+                                         "narrowphase), %.1f s" % (len(j), args.large_iters, dt)}
+    L.close()
+    print(json.dumps(out), flush=True)
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration << 20)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,12 +236,16 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="finally copy MIB MiB device to device (a reference kernel of known traffic for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--bodies", type=int, default=200_000, help="bodies of the large workload")
+    ap.add_argument("--large-iters", type=int, default=30, help="sweeps per step of the large workload")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (worlds shard by index "
-              "across GPUs with no exchange; see DESIGN.md)" % args.gpus, file=sys.stderr)
+              "across GPUs with no exchange)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload == "large":
+        return bench_large(args)
 
     from ode_amd import odx
     import oracle as O

@@ -157,6 +157,10 @@ odxLarge *odxLargeCreate(int nb, const float *bodies, int ng, const float *geoms
 void odxLargeDestroy(odxLarge *w);
 /* Sweeps per step (dWorldSetQuickStepNumIterations, objects.h:462). */
 int odxLargeSetIterations(odxLarge *w, int iters);
+/* Workgroups per CU of the cooperative solve (1..8, capped by occupancy;
+ * default 1).  Returns the resulting grid size, 0 on failure.  No ODE
+ * counterpart (launch shape only; results do not depend on it). */
+int odxLargeSetSolveBlocks(odxLarge *w, int per_cu);
 /* Queues nsteps steps (dSpaceCollide + contacts + dWorldQuickStep +
  * dJointGroupEmpty each) on the world's stream.  Returns 1 when queued. */
 int odxLargeStep(odxLarge *w, int nsteps);

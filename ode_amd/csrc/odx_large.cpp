@@ -24,7 +24,7 @@ bool lg_enqueue_solve(const DevWorlds &D, const StepParams &P, const LargeDev &L
                       hipStream_t s);
 void lg_enqueue_fold(const LargeDev &L, int *sticky, hipStream_t s);
 int lg_colour_blocks();
-int lg_solve_blocks();
+int lg_solve_blocks(int per_cu);
 }  // namespace odx
 
 using namespace odx;
@@ -152,7 +152,7 @@ odxLarge *odxLargeCreate(int nb, const float *bodies, int ng, const float *geoms
          W->alloc(&L.contact_cnt, (size_t)pair_cap + 1) && W->alloc(&L.row_cnt, (size_t)contact_cap + 1) &&
          W->alloc(&L.counters, LG_NCOUNTERS) && W->alloc(&L.cbody, contact_cap) && W->alloc(&L.colour, contact_cap) && W->alloc(&L.taken, nb) &&
          W->alloc(&L.owner, nb) && W->alloc(&L.elig, LG_MAXCOL * LG_MAXSUB) && W->alloc(&L.col_start, LG_MAXCOL + 1) &&
-         W->alloc(&L.corder, contact_cap) && W->alloc(&L.bar, 2 + LG_MAXCOL) && W->alloc(&L.fc, (size_t)nb * 6);
+         W->alloc(&L.corder, contact_cap) && W->alloc(&L.items, contact_cap) && W->alloc(&L.bar, LG_BAR_WORDS) && W->alloc(&L.fc, (size_t)nb * 8);
     const size_t scan_cap = std::max({(size_t)hsize + 1, (size_t)ng + 1, (size_t)pair_cap + 1, (size_t)contact_cap + 1});
     ok = ok && W->alloc(&L.scan_tmp, (scan_cap + SCAN_BLOCK - 1) / SCAN_BLOCK);
     ok = ok && W->alloc(&W->sticky, LG_NCOUNTERS);
@@ -166,7 +166,7 @@ odxLarge *odxLargeCreate(int nb, const float *bodies, int ng, const float *geoms
     for (auto &e : W->ev) ok = ok && odx_check(hipEventCreate(&e), "hipEventCreate");
     if (ok) {
         W->colour_blocks = lg_colour_blocks();
-        W->solve_blocks = lg_solve_blocks();
+        W->solve_blocks = lg_solve_blocks(1);
         if (W->colour_blocks <= 0 || W->solve_blocks <= 0) {
             odx_message(3, "odxLargeCreate: no cooperative launch configuration");
             ok = false;
@@ -185,6 +185,14 @@ int odxLargeSetIterations(odxLarge *w, int iters) {
     if (!w || iters < 1) return 0;
     w->iters = iters;
     return 1;
+}
+
+int odxLargeSetSolveBlocks(odxLarge *w, int per_cu) {
+    if (!w || per_cu < 1 || per_cu > 8) return 0;
+    const int b = lg_solve_blocks(per_cu);
+    if (b <= 0) return 0;
+    w->solve_blocks = b;
+    return b;
 }
 
 int odxLargeStep(odxLarge *w, int nsteps) {

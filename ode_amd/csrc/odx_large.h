@@ -23,6 +23,9 @@ constexpr int LG_MAXCOL = 128;        // colours of the sweep schedule
 constexpr int LG_MAXSUB = 32;         // claim/commit rounds per colour
 constexpr long long LG_SPIN_LIMIT = 1ll << 21;  // polls of one barrier wait (~1 s)
 constexpr int LG_BLOCK = 256;         // threads per workgroup of the cooperative kernels
+// words of the barrier array: generation, top counter, 8 group counters (64 B
+// apart), then the per-colour fill counters of the colouring
+enum { LG_BAR_GEN = 0, LG_BAR_TOP = 16, LG_BAR_GROUP = 32, LG_BAR_FILL = 160, LG_BAR_WORDS = LG_BAR_FILL + LG_MAXCOL };
 
 struct LargeDev {
     int nb, ng, nbig, hmask;          // bodies, geoms, big geoms, hash buckets - 1 (power of two)
@@ -46,8 +49,9 @@ struct LargeDev {
     int *elig;                        // [LG_MAXCOL * LG_MAXSUB] contacts taking part in each round
     int *col_start;                   // [LG_MAXCOL + 1] contacts per colour -> first slot of each colour
     int *corder;                      // [contact_cap] contacts in colour order
-    unsigned *bar;                    // [2 + LG_MAXCOL] grid barrier (arrivals, generation), colour fill
-    float *fc;                        // [nb * 6] body accumulators of the solve
+    int4 *items;                      // [contact_cap] per slot: first row, first-pass rows | rows << 8, body 1, body 2
+    unsigned *bar;                    // [LG_BAR_WORDS] grid barrier counters, colour fill counters
+    float *fc;                        // [nb * 8] body accumulators of the solve (6 used)
     int *scan_tmp;                    // block sums of the scans
 };
 

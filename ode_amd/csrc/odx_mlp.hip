@@ -6,7 +6,7 @@
 // normal/friction impulses for the SOR-LCP solve.  The reference has no such
 // component: its warm start is compiled out (quickstep.cpp:46-51) and the
 // solve starts from lambda = 0 every step (:1929-1972).  Cold-start PGS stays
-// the fallback and the judge: sweeps always continue to the same residual.
+// the fallback and the arbiter: sweeps always continue to the same residual.
 //
 // Kernels: k_mlp_infer (forward, writes lambda0 rows), k_mlp_train (forward +
 // backward over sampled mini-batches, fp32 gradient accumulation), k_mlp_adam

@@ -2044,27 +2044,67 @@ __global__ void k_lg_rows(DevWorlds D, StepParams P, LargeDev L) {
     if (blockIdx.x == 0 && threadIdx.x == 0) L.counters[LG_ROWS] = L.row_cnt[nc];
 }
 
-// Grid barrier of the cooperative kernels: arrivals counter + generation.  A
-// wait that runs out sets LG_BARRIER_ERR; every block then leaves at its next
-// barrier, the host reports the step as failed.  Returns false on error.
-// `read`, when given, is loaded by one thread after the barrier and returned
-// to every thread in *val.
+// Values shared between workgroups inside one cooperative kernel are stored
+// and loaded with agent-scope atomics (global_store/load ... sc1: coherent at
+// the device level across the XCDs' L2 caches), so a barrier needs no L2
+// writeback or invalidation, only the completion of each thread's stores.
+__device__ __forceinline__ float ld_dev(const float *p) {
+    return __hip_atomic_load(const_cast<float *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_dev(const int *p) {
+    return __hip_atomic_load(const_cast<int *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned ld_dev(const unsigned *p) {
+    return __hip_atomic_load(const_cast<unsigned *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ld_dev(const unsigned long long *p) {
+    return __hip_atomic_load(const_cast<unsigned long long *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ void st_dev(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned add_dev(unsigned *p, unsigned v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// all of this thread's memory operations have completed (stores acknowledged
+// at their coherence level); also a compiler barrier
+__device__ __forceinline__ void mem_done() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
+}
+
+// Grid barrier of the cooperative kernels.  Workgroups count per residue
+// class blockIdx % 8 (one XCD each under round-robin dispatch; correctness
+// does not depend on it), the last of a class counts on the top counter, the
+// last class bumps the generation that the others wait for.  Counters sit 64
+// B apart.  A wait that runs out sets LG_BARRIER_ERR; every block then leaves
+// at its next barrier and the host reports the step as failed.  `read`, when
+// given, is loaded by one thread after the barrier and returned in *val.
 __device__ bool lg_grid_sync(const LargeDev &L, unsigned &gen, const int *read = nullptr, int *val = nullptr) {
     __shared__ int s_ok, s_val;
+    mem_done();
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
-        __threadfence();
-        const unsigned arrived = atomicAdd(&L.bar[0], 1u) + 1u;
-        if (arrived == gridDim.x) {
-            atomicExch(&L.bar[0], 0u);
-            __threadfence();
-            atomicAdd(&L.bar[1], 1u);
+        const unsigned x = blockIdx.x & 7u;
+        const unsigned per = (gridDim.x + 7u - x) >> 3;
+        const unsigned groups = gridDim.x < 8u ? gridDim.x : 8u;
+        bool last = false;
+        if (add_dev(&L.bar[LG_BAR_GROUP + 16 * x], 1u) + 1u == per) {
+            st_dev(&L.bar[LG_BAR_GROUP + 16 * x], 0u);
+            mem_done();
+            if (add_dev(&L.bar[LG_BAR_TOP], 1u) + 1u == groups) {
+                st_dev(&L.bar[LG_BAR_TOP], 0u);
+                mem_done();
+                last = true;
+            }
+        }
+        if (last) {
+            add_dev(&L.bar[LG_BAR_GEN], 1u);
         } else {
             long long spins = 0;
-            while (__hip_atomic_load(&L.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                if (++spins > LG_SPIN_LIMIT ||
-                    __hip_atomic_load(&L.counters[LG_BARRIER_ERR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            while (ld_dev(&L.bar[LG_BAR_GEN]) == gen) {
+                if (++spins > LG_SPIN_LIMIT || ld_dev(&L.counters[LG_BARRIER_ERR])) {
                     atomicExch(&L.counters[LG_BARRIER_ERR], 1);
                     ok = 0;
                     break;
@@ -2073,9 +2113,9 @@ __device__ bool lg_grid_sync(const LargeDev &L, unsigned &gen, const int *read =
             }
         }
         gen++;
-        __threadfence();
-        if (__hip_atomic_load(&L.counters[LG_BARRIER_ERR], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = 0;
-        if (read) s_val = __hip_atomic_load(read, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        mem_done();
+        if (ld_dev(&L.counters[LG_BARRIER_ERR])) ok = 0;
+        if (read) s_val = ld_dev(read);
         s_ok = ok;
     }
     __syncthreads();
@@ -2094,6 +2134,18 @@ __device__ int lg_block_sum(int v, int *s_red) {
     int s = 0;
     for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += s_red[i];
     return s;
+}
+
+// Rows of a contact in the first pass of ReorderPrep (quickstep.cpp:2329-2355:
+// rows without findex first, findex rows after), by the batched path's rule:
+// n_a leading rows without findex followed only by findex rows split there;
+// any other pattern keeps all n rows in the first pass.
+__device__ __forceinline__ int lg_pass_a(const DevWorlds &D, int r, int n) {
+    unsigned char fl[3] = {0, 0, 0};
+    for (int q = 0; q < n && q < 3; q++) fl[q] = as_int(D.rows[(size_t)(r + q) * ROW_STRIDE + R_FINDEX]) >= 0;
+    int na, nf;
+    split_joint_rows(fl, n, na, nf);
+    return nf > 0 ? na : n;
 }
 
 // Claim key of contact k in claim round `stamp`: rounds increase, so a claim
@@ -2119,7 +2171,7 @@ __global__ __launch_bounds__(LG_BLOCK) void k_lg_colour(DevWorlds D, LargeDev L)
     const int nc = L.counters[LG_CONTACTS];
     const int gt = blockIdx.x * blockDim.x + threadIdx.x, gs = gridDim.x * blockDim.x;
     for (int k = gt; k < nc; k += gs) L.colour[k] = -1;
-    for (int b = gt; b < L.nb; b += gs) { L.taken[b] = -1; L.owner[b] = 0ull; }
+    for (int b = gt; b < L.nb; b += gs) { st_dev(&L.taken[b], -1); st_dev(&L.owner[b], 0ull); }
     bool ok = lg_grid_sync(L, gen);
     int ncol = 0, rounds = 0, left = nc;
     for (int c = 0; ok && left > 0 && c < LG_MAXCOL; c++) {
@@ -2129,7 +2181,7 @@ __global__ __launch_bounds__(LG_BLOCK) void k_lg_colour(DevWorlds D, LargeDev L)
             for (int k = gt; k < nc; k += gs) {
                 if (L.colour[k] >= 0) continue;
                 const int2 bb = L.cbody[k];
-                if (L.taken[bb.x] == c || (bb.y >= 0 && L.taken[bb.y] == c)) continue;
+                if (ld_dev(&L.taken[bb.x]) == c || (bb.y >= 0 && ld_dev(&L.taken[bb.y]) == c)) continue;
                 const unsigned long long key = lg_key(k, stamp);
                 atomicMax(&L.owner[bb.x], key);
                 if (bb.y >= 0) atomicMax(&L.owner[bb.y], key);
@@ -2147,10 +2199,10 @@ __global__ __launch_bounds__(LG_BLOCK) void k_lg_colour(DevWorlds D, LargeDev L)
                 if (L.colour[k] >= 0) continue;
                 const int2 bb = L.cbody[k];
                 const unsigned long long key = lg_key(k, stamp);
-                if (L.owner[bb.x] != key || (bb.y >= 0 && L.owner[bb.y] != key)) continue;
-                L.colour[k] = c;
-                L.taken[bb.x] = c;
-                if (bb.y >= 0) L.taken[bb.y] = c;
+                if (ld_dev(&L.owner[bb.x]) != key || (bb.y >= 0 && ld_dev(&L.owner[bb.y]) != key)) continue;
+                L.colour[k] = c;   // contact k belongs to this thread in every round
+                st_dev(&L.taken[bb.x], c);
+                if (bb.y >= 0) st_dev(&L.taken[bb.y], c);
                 won++;
             }
             won = lg_block_sum(won, s_red);
@@ -2161,7 +2213,7 @@ __global__ __launch_bounds__(LG_BLOCK) void k_lg_colour(DevWorlds D, LargeDev L)
         ncol = c + 1;
         int nmem = 0;   // contacts of colour c (complete: its last round ended with a barrier)
         __syncthreads();
-        if (threadIdx.x == 0) s_hist[0] = __hip_atomic_load(&L.col_start[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (threadIdx.x == 0) s_hist[0] = ld_dev(&L.col_start[c]);
         __syncthreads();
         nmem = s_hist[0];
         left -= nmem;
@@ -2175,12 +2227,13 @@ __global__ __launch_bounds__(LG_BLOCK) void k_lg_colour(DevWorlds D, LargeDev L)
         if (c >= 0) atomicAdd(&s_hist[c], 1);
     }
     __syncthreads();
-    for (int c = threadIdx.x; c < ncol; c += blockDim.x) s_base[c] = s_hist[c] ? (int)atomicAdd(&L.bar[2 + c], (unsigned)s_hist[c]) : 0;
+    for (int c = threadIdx.x; c < ncol; c += blockDim.x)
+        s_base[c] = s_hist[c] ? (int)add_dev(&L.bar[LG_BAR_FILL + c], (unsigned)s_hist[c]) : 0;
     if (threadIdx.x == 0) {
         int acc = 0;
         for (int c = 0; c < ncol; c++) {
             s_start[c] = acc;
-            acc += __hip_atomic_load(&L.col_start[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            acc += ld_dev(&L.col_start[c]);
         }
         s_start[ncol] = acc;
     }
@@ -2189,7 +2242,12 @@ __global__ __launch_bounds__(LG_BLOCK) void k_lg_colour(DevWorlds D, LargeDev L)
     __syncthreads();
     for (int k = gt; k < nc; k += gs) {
         const int c = L.colour[k];
-        if (c >= 0) L.corder[s_start[c] + s_base[c] + atomicAdd(&s_hist[c], 1)] = k;
+        if (c < 0) continue;
+        const int pos = s_start[c] + s_base[c] + atomicAdd(&s_hist[c], 1);
+        const int r = L.row_cnt[k], n = L.row_cnt[k + 1] - r;
+        const int2 bb = L.cbody[k];
+        L.corder[pos] = k;
+        L.items[pos] = make_int4(r, lg_pass_a(D, r, n) | (n << 8), bb.x, bb.y);
     }
     ok = lg_grid_sync(L, gen);   // every block has read col_start[] counts
     if (!ok) return;
@@ -2203,78 +2261,168 @@ __global__ __launch_bounds__(LG_BLOCK) void k_lg_colour(DevWorlds D, LargeDev L)
     }
 }
 
-// rows [r, r + n) of one contact with fc in global memory (within a colour
-// phase no other thread touches the contact's bodies)
-__device__ __forceinline__ void lg_rows(const DevWorlds &D, float *fc, int r, int n) {
-    const float4 *row4 = reinterpret_cast<const float4 *>(D.rows + (size_t)r * ROW_STRIDE);
-    const float4 ids = row4[7];
-    const int b1 = as_int(ids.x), b2 = as_int(ids.y);
-    float f1[6], f2[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (int k = 0; k < 6; k++) f1[k] = fc[(size_t)b1 * 6 + k];
-    if (b2 >= 0)
-        for (int k = 0; k < 6; k++) f2[k] = fc[(size_t)b2 * 6 + k];
-    for (int q = 0; q < n; q++) {
-        const float4 *rr = row4 + q * 8;
-        const float4 a0 = rr[0], a1 = rr[1], a2 = rr[2], a3 = rr[3], a4 = rr[4], a5 = rr[5], a6 = rr[6], a7 = rr[7];
-        const float old = D.lambda[r + q];
-        float delta = a1.z - old * a1.w;
-        delta -= f1[0] * a0.x + f1[1] * a0.y + f1[2] * a0.z + f1[3] * a0.w + f1[4] * a1.x + f1[5] * a1.y;
-        if (b2 >= 0) delta -= f2[0] * a2.x + f2[1] * a2.y + f2[2] * a2.z + f2[3] * a2.w + f2[4] * a3.x + f2[5] * a3.y;
-        float lo = a3.z, hi = a3.w;
-        const int fidx = as_int(a7.z);
-        if (fidx >= 0) { hi = fabsf(hi * D.lambda[fidx]); lo = -hi; }
-        float nl = old + delta;
-        if (nl < lo) { delta = lo - old; nl = lo; }
-        else if (nl > hi) { delta = hi - old; nl = hi; }
-        D.lambda[r + q] = nl;
-        f1[0] += delta * a4.x; f1[1] += delta * a4.y; f1[2] += delta * a4.z;
-        f1[3] += delta * a4.w; f1[4] += delta * a5.x; f1[5] += delta * a5.y;
-        if (b2 >= 0) {
-            f2[0] += delta * a5.z; f2[1] += delta * a5.w; f2[2] += delta * a6.x;
-            f2[3] += delta * a6.y; f2[4] += delta * a6.z; f2[5] += delta * a6.w;
-        }
-    }
-    for (int k = 0; k < 6; k++) fc[(size_t)b1 * 6 + k] = f1[k];
-    if (b2 >= 0)
-        for (int k = 0; k < 6; k++) fc[(size_t)b2 * 6 + k] = f2[k];
+// Body accumulators fc of the solve: 8 floats per body (32 B), moved as a
+// 16 B + 8 B pair of buffer accesses with the device-coherent cache policy
+// (sc1), since bodies are shared between workgroups inside the kernel.  The
+// buffer descriptor bounds every access to the array.
+typedef __attribute__((ext_vector_type(4))) unsigned lg_u4;
+typedef __attribute__((ext_vector_type(2))) unsigned lg_u2;
+constexpr int LG_CPOL_DEVICE = 16;  // cache policy bits: sc1
+constexpr int LG_FC_STRIDE = 8;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t lg_fc_rsrc(float *fc, int nb) {
+    return __builtin_amdgcn_make_buffer_rsrc(fc, 0, nb * LG_FC_STRIDE * 4, 0x00020000);
+}
+__device__ __forceinline__ void lg_fc_load(__amdgpu_buffer_rsrc_t rs, int b, float *f) {
+    const float4 a = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, b * 32, 0, LG_CPOL_DEVICE));
+    const float2 c = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, b * 32 + 16, 0, LG_CPOL_DEVICE));
+    f[0] = a.x; f[1] = a.y; f[2] = a.z; f[3] = a.w; f[4] = c.x; f[5] = c.y;
+}
+__device__ __forceinline__ void lg_fc_store(__amdgpu_buffer_rsrc_t rs, int b, const float *f) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(lg_u4, make_float4(f[0], f[1], f[2], f[3])), rs, b * 32, 0,
+                                           LG_CPOL_DEVICE);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(lg_u2, make_float2(f[4], f[5])), rs, b * 32 + 16, 0,
+                                          LG_CPOL_DEVICE);
 }
 
-// Split of a contact's rows into the two passes of ReorderPrep
-// (quickstep.cpp:2329-2355): the normal row has no findex; friction rows with
-// findex follow it (a contact without Approx1 keeps all rows in pass one).
-__device__ __forceinline__ int lg_pass_a(const DevWorlds &D, int r, int n) {
-    return (n > 1 && as_int(D.rows[(size_t)(r + 1) * ROW_STRIDE + R_FINDEX]) >= 0) ? 1 : n;
+// One row update (IterationStep quickstep.cpp:2917-3033) on a row held in
+// registers; lf = lambda of the row's findex row (unused without findex).
+__device__ __forceinline__ float lg_row_update(const float4 *a, float old, float lf, float *f1, float *f2, bool two) {
+    float delta = a[1].z - old * a[1].w;
+    delta -= f1[0] * a[0].x + f1[1] * a[0].y + f1[2] * a[0].z + f1[3] * a[0].w + f1[4] * a[1].x + f1[5] * a[1].y;
+    if (two) delta -= f2[0] * a[2].x + f2[1] * a[2].y + f2[2] * a[2].z + f2[3] * a[2].w + f2[4] * a[3].x + f2[5] * a[3].y;
+    float lo = a[3].z, hi = a[3].w;
+    if (as_int(a[7].z) >= 0) { hi = fabsf(hi * lf); lo = -hi; }
+    float nl = old + delta;
+    if (nl < lo) { delta = lo - old; nl = lo; }
+    else if (nl > hi) { delta = hi - old; nl = hi; }
+    f1[0] += delta * a[4].x; f1[1] += delta * a[4].y; f1[2] += delta * a[4].z;
+    f1[3] += delta * a[4].w; f1[4] += delta * a[5].x; f1[5] += delta * a[5].y;
+    if (two) {
+        f2[0] += delta * a[5].z; f2[1] += delta * a[5].w; f2[2] += delta * a[6].x;
+        f2[3] += delta * a[6].y; f2[4] += delta * a[6].z; f2[5] += delta * a[6].w;
+    }
+    return nl;
+}
+
+// A slot's work in one phase: rows [r, r + n) of one contact between bodies
+// b1, b2 (-1: static).  Up to LG_PRE rows come preloaded in `pre`.  A row's
+// findex points at a row of the same contact: inside [r, r + n) the value
+// updated here is used, else the contact's first-pass row (fixed in this
+// pass).  lambda of a contact is touched only by the thread owning its slot.
+constexpr int LG_PRE = 2;
+struct LgSlot {
+    int r, n, b1, b2;
+    float4 pre[LG_PRE][8];
+};
+
+__device__ __forceinline__ void lg_slot_load(const DevWorlds &D, const int4 it, int pass, LgSlot &S) {
+    const int na = it.y & 255, n = it.y >> 8;
+    S.r = pass == 0 ? it.x : it.x + na;
+    S.n = pass == 0 ? na : n - na;
+    S.b1 = it.z;
+    S.b2 = it.w;
+    if (it.x < 0) S.n = 0;
+#pragma unroll
+    for (int q = 0; q < LG_PRE; q++) {
+        if (q < S.n) {
+            const float4 *rr = reinterpret_cast<const float4 *>(D.rows + (size_t)(S.r + q) * ROW_STRIDE);
+#pragma unroll
+            for (int k = 0; k < 8; k++) S.pre[q][k] = rr[k];
+        }
+    }
+}
+
+__device__ __forceinline__ void lg_slot_run(const DevWorlds &D, __amdgpu_buffer_rsrc_t rs, const LgSlot &S) {
+    if (S.n <= 0) return;
+    const bool two = S.b2 >= 0;
+    float f1[6], f2[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    lg_fc_load(rs, S.b1, f1);
+    if (two) lg_fc_load(rs, S.b2, f2);
+    float lam[LG_PRE], lf[LG_PRE];
+#pragma unroll
+    for (int q = 0; q < LG_PRE; q++) {
+        lam[q] = 0.f;
+        lf[q] = 0.f;
+        if (q < S.n) {
+            lam[q] = D.lambda[S.r + q];
+            const int fidx = as_int(S.pre[q][7].z);
+            if (fidx >= 0 && (fidx < S.r || fidx >= S.r + S.n)) lf[q] = D.lambda[fidx];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < LG_PRE; q++) {
+        if (q < S.n) {
+            const int fidx = as_int(S.pre[q][7].z);
+            float l = lf[q];
+#pragma unroll
+            for (int u = 0; u < q; u++)
+                if (fidx == S.r + u) l = lam[u];
+            lam[q] = lg_row_update(S.pre[q], lam[q], l, f1, f2, two);
+            D.lambda[S.r + q] = lam[q];
+        }
+    }
+    for (int q = LG_PRE; q < S.n; q++) {   // rows beyond the preloaded ones
+        const float4 *rr = reinterpret_cast<const float4 *>(D.rows + (size_t)(S.r + q) * ROW_STRIDE);
+        float4 a[8];
+        for (int k = 0; k < 8; k++) a[k] = rr[k];
+        const float old = D.lambda[S.r + q];
+        const float l = D.lambda[max(as_int(a[7].z), 0)];
+        D.lambda[S.r + q] = lg_row_update(a, old, l, f1, f2, two);
+    }
+    lg_fc_store(rs, S.b1, f1);
+    if (two) lg_fc_store(rs, S.b2, f2);
 }
 
 // The solve (cooperative): lambda = 0, `iters` sweeps, each the contacts'
 // first-pass rows colour by colour, then their findex rows colour by colour,
-// then Stage4b/6 and dxStepBody of every body.
+// then Stage4b/6 and dxStepBody of every body.  The thread at a slot of the
+// colour order handles that contact in every phase (its lambda is touched by
+// no other thread); each thread loads its slot of the next phase (item and
+// rows: read-only here) before the barrier.
 __global__ __launch_bounds__(LG_BLOCK) void k_lg_solve(DevWorlds D, StepParams P, LargeDev L, int iters) {
+    __shared__ int s_cs[LG_MAXCOL + 1];
     unsigned gen = 0;
     const int nc = L.counters[LG_CONTACTS];
-    const int m = L.counters[LG_ROWS];
     const int ncol = L.counters[LG_NCOL];
     const int gt = blockIdx.x * blockDim.x + threadIdx.x, gs = gridDim.x * blockDim.x;
-    for (int i = gt; i < L.nb * 6; i += gs) L.fc[i] = 0.f;
-    for (int i = gt; i < m; i += gs) D.lambda[i] = 0.f;
-    bool ok = lg_grid_sync(L, gen);
-    for (int it = 0; ok && it < iters && nc > 0; it++) {
-        for (int pass = 0; ok && pass < 2; pass++) {
-            for (int c = 0; ok && c < ncol; c++) {
-                const int s = L.col_start[c], e = L.col_start[c + 1];
-                for (int idx = s + gt; idx < e; idx += gs) {
-                    const int k = L.corder[idx];
-                    const int r = L.row_cnt[k], n = L.row_cnt[k + 1] - r;
-                    const int na = lg_pass_a(D, r, n);
-                    if (pass == 0) lg_rows(D, L.fc, r, na);
-                    else if (n > na) lg_rows(D, L.fc, r + na, n - na);
-                }
-                ok = lg_grid_sync(L, gen);
-            }
+    const __amdgpu_buffer_rsrc_t rs = lg_fc_rsrc(L.fc, L.nb);
+    const float zero[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int c = threadIdx.x; c <= ncol; c += blockDim.x) s_cs[c] = L.col_start[c];
+    for (int b = gt; b < L.nb; b += gs) lg_fc_store(rs, b, zero);
+    for (int c = 0; c < ncol; c++) {   // slots of colour c map to threads as in the sweeps
+        const int e = L.col_start[c + 1];
+        for (int idx = L.col_start[c] + gt; idx < e; idx += gs) {
+            const int4 it = L.items[idx];
+            for (int r = it.x; r < it.x + (it.y >> 8); r++) D.lambda[r] = 0.f;
         }
     }
+    bool ok = lg_grid_sync(L, gen);
+    const int T = (nc > 0 && ncol > 0) ? iters * 2 * ncol : 0;
+    int c = 0, pass = 0;
+    int idx = s_cs[0] + gt, e = s_cs[1];
+    LgSlot cur;
+    lg_slot_load(D, idx < e && T > 0 ? L.items[idx] : make_int4(-1, 0, 0, 0), 0, cur);
+    for (int p = 0; ok && p < T; p++) {
+        int c2 = c + 1, pass2 = pass;
+        if (c2 == ncol) { c2 = 0; pass2 ^= 1; }
+        const int idx2 = s_cs[c2] + gt, e2 = s_cs[c2 + 1];
+        LgSlot nxt;
+        lg_slot_load(D, (p + 1 < T && idx2 < e2) ? L.items[idx2] : make_int4(-1, 0, 0, 0), pass2, nxt);
+        lg_slot_run(D, rs, cur);
+        for (int j = idx + gs; j < e; j += gs) {   // slots beyond one per thread
+            LgSlot x;
+            lg_slot_load(D, L.items[j], pass, x);
+            lg_slot_run(D, rs, x);
+        }
+        ok = lg_grid_sync(L, gen);
+        c = c2; pass = pass2; idx = idx2; e = e2; cur = nxt;
+    }
     if (!ok) return;
-    for (int b = gt; b < L.nb; b += gs) integrate_body(D, P, b, L.fc + (size_t)b * 6, nc > 0);
+    for (int b = gt; b < L.nb; b += gs) {
+        float f[6];
+        lg_fc_load(rs, b, f);
+        integrate_body(D, P, b, f, nc > 0);
+    }
 }
 
 // largest value each counter reached since the host last cleared sticky
@@ -2289,15 +2437,15 @@ void lg_enqueue_fold(const LargeDev &L, int *sticky, hipStream_t s) {
 }
 
 // workgroups of a cooperative launch: every one resident at once
-int lg_coop_blocks(const void *kernel) {
+int lg_coop_blocks(const void *kernel, int per_cu) {
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, LG_BLOCK, 0) != hipSuccess) return 0;
-    return cus * (per < 2 ? per : 2);
+    return cus * (per < per_cu ? per : per_cu);
 }
-int lg_colour_blocks() { return lg_coop_blocks((const void *)k_lg_colour); }
-int lg_solve_blocks() { return lg_coop_blocks((const void *)k_lg_solve); }
+int lg_colour_blocks() { return lg_coop_blocks((const void *)k_lg_colour, 2); }
+int lg_solve_blocks(int per_cu) { return lg_coop_blocks((const void *)k_lg_solve, per_cu); }
 
 // The prepare stage of a step: bodies, broadphase, narrowphase, rows.
 // Capacities: ng <= pair scan cap etc. are checked by the host.
@@ -2324,7 +2472,7 @@ void lg_enqueue_prepare(const DevWorlds &D, const StepParams &P, const LargeDev 
 bool lg_enqueue_colour(const DevWorlds &D, const LargeDev &L, int blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_lg_fill, dim3(16), dim3(256), 0, s, L.elig, LG_MAXCOL * LG_MAXSUB, 0);
     hipLaunchKernelGGL(k_lg_fill, dim3(1), dim3(256), 0, s, L.col_start, LG_MAXCOL + 1, 0);
-    hipLaunchKernelGGL(k_lg_fill, dim3(1), dim3(256), 0, s, reinterpret_cast<int *>(L.bar), 2 + LG_MAXCOL, 0);
+    hipLaunchKernelGGL(k_lg_fill, dim3(1), dim3(256), 0, s, reinterpret_cast<int *>(L.bar), LG_BAR_WORDS, 0);
     if (hipGetLastError() != hipSuccess) return false;
     DevWorlds d = D;
     LargeDev l = L;
@@ -2335,7 +2483,7 @@ bool lg_enqueue_colour(const DevWorlds &D, const LargeDev &L, int blocks, hipStr
 
 bool lg_enqueue_solve(const DevWorlds &D, const StepParams &P, const LargeDev &L, int iters, int blocks,
                       hipStream_t s) {
-    hipLaunchKernelGGL(k_lg_fill, dim3(1), dim3(256), 0, s, reinterpret_cast<int *>(L.bar), 2, 0);
+    hipLaunchKernelGGL(k_lg_fill, dim3(1), dim3(256), 0, s, reinterpret_cast<int *>(L.bar), LG_BAR_FILL, 0);
     if (hipGetLastError() != hipSuccess) return false;
     DevWorlds d = D;
     StepParams p = P;

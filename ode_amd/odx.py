@@ -65,7 +65,7 @@ def lib():
         L.odxLargeCreate.restype = ctypes.c_void_p
         L.odxLargeCreate.argtypes = [i, fp, i, fp, dp, i, i]
         L.odxLargeDestroy.argtypes = [vp]
-        for name in ("odxLargeSetIterations", "odxLargeStep"):
+        for name in ("odxLargeSetIterations", "odxLargeStep", "odxLargeSetSolveBlocks"):
             getattr(L, name).argtypes = [vp, i]
         L.odxLargeSync.argtypes = [vp]
         for name in ("odxLargeTime", "odxLargeKernelTimes"):
@@ -283,6 +283,12 @@ class Large:
 
     def set_iterations(self, iters):
         _check(lib().odxLargeSetIterations(self.h, iters), "odxLargeSetIterations")
+
+    def set_solve_blocks(self, per_cu):
+        """Workgroups per CU of the cooperative solve; returns the grid size."""
+        b = lib().odxLargeSetSolveBlocks(self.h, per_cu)
+        _check(b, "odxLargeSetSolveBlocks")
+        return b
 
     def step(self, n=1):
         _check(lib().odxLargeStep(self.h, n), "odxLargeStep")
