@@ -11,8 +11,13 @@ Workloads (seeded synthetic scenes, ode_amd/scenes.py):
           configuration with other seeds) vs cold PGS at the same residual.
           value = learned warm-start world-steps/s.
   uneven  4,096 worlds of 8-64 spheres/boxes dropped on a plane, cold PGS.
+  chains  1,024 worlds of a 16-capsule chain on ball joints (anchored at one
+          end) with 8 free boxes, 40 sweeps per step (--tol 0 --max-iters 40
+          unless given), cold PGS.
   large   one world of 200,000 spheres poured into an open box (hash space,
           30 sweeps per step); value = world-steps/s.
+  boxes   one world of 1,000 boxes dropped on a plane, stepped through the
+          drop-in ODE API by a C++ frame loop; value = world-steps/s.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload towers]
 """
@@ -40,6 +45,8 @@ def build_worlds(workload, nworlds, seed):
         return S.uneven_batch(nworlds, seed=seed)
     if workload == "towers":
         return S.tower_batch(nworlds, seed=seed)
+    if workload == "chains":
+        return S.chain_batch(nworlds, seed=seed)
     raise SystemExit("unknown workload %r" % workload)
 
 
@@ -208,6 +215,81 @@ def bench_large(args):
         copy_calibration(args.copy_calibration << 20)
 
 
+def bench_boxes(args):
+    """One world of 1,000 boxes (side 0.5 m, density 1000) dropped from a
+    20x20x20 m volume onto a plane, stepped through the drop-in ODE API by a
+    C++ frame loop (ode_amd/examples/frame_loop: sweep-and-prune space, near
+    callback with dCollide and dJointCreateContact, dWorldQuickStep with 20
+    iterations, dJointGroupEmpty).  The timed region includes the host side
+    of the API (island order, callbacks) and its PCIe transfers."""
+    import subprocess
+    import tempfile
+    from ode_amd import scenes as S
+    import oracle as O
+    world = S.boxes_on_plane(args.bodies, seed=args.seed)
+    exe = os.path.join(ROOT, "ode_amd", "examples", "frame_loop")
+    if not os.path.exists(exe):
+        raise SystemExit("ode_amd/examples/frame_loop is not built (python -m ode_amd.build)")
+    with tempfile.NamedTemporaryFile(suffix=".scene", delete=False) as f:
+        np.array([len(world.bodies), len(world.geoms)], np.int32).tofile(f)
+        np.ascontiguousarray(world.bodies, np.float32).tofile(f)
+        np.ascontiguousarray(world.geoms, np.float32).tofile(f)
+        path = f.name
+    try:
+        res = subprocess.run([exe, path, str(args.warmup), str(args.steps), str(args.large_iters), "0.5",
+                              str(args.row_order)],
+                             capture_output=True, text=True, timeout=900)
+    finally:
+        os.unlink(path)
+    if res.returncode != 0:
+        raise SystemExit("frame_loop failed (%d): %s" % (res.returncode, res.stderr[-2000:]))
+    r = json.loads(res.stdout.strip().splitlines()[-1])
+    ms = r["ms_per_step"]
+    kms = r["kernel_ms_per_step"]
+    gbs = r["row_sweeps_per_step"] * ROW_SWEEP_BYTES / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    out = {
+        "metric": "world-steps/sec, one %d-box world through the ODE API, %d iterations, 1 MI355X"
+                  % (args.bodies, args.large_iters),
+        "value": 1000.0 / ms,
+        "unit": "world-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "boxes", "bodies": args.bodies, "space": "sweep-and-prune (XYZ)",
+                   "iterations": args.large_iters, "mu": 0.5, "erp": 0.2, "cfm": 1e-5, "h": 0.01, "sor_w": 1.3,
+                   "row_order": "reference (dependency levels)" if args.row_order == 0 else "colour phases",
+                   "seed": args.seed},
+        "frame": r,
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_single_step (row-sweeps x 228 B per launch, hipEvents inside dWorldQuickStep)"},
+    }
+    if not args.no_cpu:
+        # the oracle's float32 canonical step (SAP pairs, dCollide, QuickStep in
+        # island order with 20 iterations) over the same warmup + timed steps
+        prm = O.params(iters=args.large_iters, space=0)
+        b = np.ascontiguousarray(world.bodies, np.float32)
+        for _ in range(args.warmup):
+            b, _, _ = O.world_step(b, world.geoms, prm, dtype=np.float32, ccap=16 * args.bodies)
+        t0 = time.perf_counter()
+        n = 0
+        while n < args.steps and time.perf_counter() - t0 < 30.0:
+            b, _, _ = O.world_step(b, world.geoms, prm, dtype=np.float32, ccap=16 * args.bodies)
+            n += 1
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n / dt, "unit": "world-steps/s", "cores": 1, "kind": "port",
+                               "sample": "steps %d..%d of the same scene: the oracle's float32 step (SAP pairs, "
+                                         "dCollide, QuickStep, %d iterations), %.1f s" % (args.warmup, args.warmup + n,
+                                                                                         args.large_iters, dt)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,16 +318,34 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="finally copy MIB MiB device to device (a reference kernel of known traffic for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
-    ap.add_argument("--bodies", type=int, default=200_000, help="bodies of the large workload")
-    ap.add_argument("--large-iters", type=int, default=30, help="sweeps per step of the large workload")
+    ap.add_argument("--row-order", type=int, default=0,
+                    help="boxes workload: 0 the reference's row order, 1 colour phases")
+    ap.add_argument("--bodies", type=int, default=None, help="bodies of the large (200,000) or boxes (1,000) workload")
+    ap.add_argument("--large-iters", type=int, default=None,
+                    help="sweeps per step of the large (30) or boxes (20) workload")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (worlds shard by index "
               "across GPUs with no exchange)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload in ("large", "boxes"):
+        large = args.workload == "large"
+        if args.bodies is None:
+            args.bodies = 200_000 if large else 1000
+        if args.large_iters is None:
+            args.large_iters = 30 if large else 20
     if args.workload == "large":
         return bench_large(args)
+    if args.workload == "boxes":
+        return bench_boxes(args)
+    if args.workload == "chains":
+        if "--worlds" not in sys.argv:
+            args.worlds = 1024
+        if "--tol" not in sys.argv:
+            args.tol = 0.0
+        if "--max-iters" not in sys.argv:
+            args.max_iters = 40
 
     from ode_amd import odx
     import oracle as O
@@ -298,8 +398,14 @@ def main():
 
     head = "learned" if "learned" in results else modes[0]
     R = results[head]
+    metric = METRIC if args.tol > 0 else "world-steps/sec, %d sweeps per step, 1 MI355X" % args.max_iters
+    solver = ("%s start, coloured contact blocks, SOR w=%g, sweeps until LCP residual <= %g (checked every %d sweeps, "
+              "cap %d, warm starts restart cold after %d)" % (head, args.sor_w, args.tol, args.check_every, args.max_iters,
+                                                             args.restart_after)
+              if args.tol > 0 else
+              "%s start, coloured contact blocks, SOR w=%g, %d sweeps per step" % (head, args.sor_w, args.max_iters))
     out = {
-        "metric": METRIC,
+        "metric": metric,
         "value": R["world_steps_per_s"],
         "unit": "world-steps/s",
         "n_gpus": 1,
@@ -314,9 +420,7 @@ def main():
         "config": {
             "workload": args.workload,
             "worlds": args.worlds,
-            "solver": "%s start, coloured contact blocks, SOR w=%g, sweeps until LCP residual <= %g "
-                      "(checked every %d sweeps, cap %d, warm starts restart cold after %d)"
-                      % (head, args.sor_w, args.tol, args.check_every, args.max_iters, args.restart_after),
+            "solver": solver,
             "launch": "%d world ranges on parallel streams, %d steps per hipGraph launch" % (args.splits, args.graph_steps),
             "h": 0.01, "mu": 0.5, "erp": 0.2, "cfm": 1e-5, "seed": args.seed,
         },

@@ -136,6 +136,14 @@ int odxBatchRowStarts(odxBatch *b, int *row_start);
 /* Warm-start values per row in the row-capacity layout (host array). */
 int odxBatchSetLambda0(odxBatch *b, const float *lambda0, int n);
 
+/* Kernel time of the device step inside dWorldQuickStep (include/ode.h):
+ * odxApiSetKernelTiming(1) clears the totals and brackets every following
+ * step's kernel with hipEvents; odxApiGetKernelTiming fills stats[3] = steps,
+ * total kernel milliseconds, total row-sweeps (rows x sweeps).  No ODE
+ * counterpart (measurement only). */
+void odxApiSetKernelTiming(int on);
+int odxApiGetKernelTiming(double *stats);
+
 /* ------------------------------------------------------- one large world */
 /* One world of up to millions of bodies stepped across the whole GPU:
  * dSpaceCollide over a dHashSpace (ode/src/collision_space.cpp:421-614; the

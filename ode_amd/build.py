@@ -13,6 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libodx.so")
 
 SOURCES = ["odx_step.hip", "odx_mlp.hip", "odx_batch.cpp", "odx_api.cpp", "odx_large.cpp"]
+EXAMPLES = ["frame_loop"]
 
 
 def sources():
@@ -39,7 +40,22 @@ def build(verbose=True, extra=()):
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, OUT)
+    build_examples(verbose)
     return OUT
+
+
+def build_examples(verbose=True):
+    """Programs over the drop-in API (ode_amd/examples), linked against the
+    in-tree libodx.so."""
+    for name in EXAMPLES:
+        src = os.path.join(HERE, "examples", name + ".cpp")
+        out = os.path.join(HERE, "examples", name)
+        cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(HERE, "..", "include"), src, "-o", out + ".tmp",
+               "-L", HERE, "-lodx", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+        os.replace(out + ".tmp", out)
 
 
 if __name__ == "__main__":

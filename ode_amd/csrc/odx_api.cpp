@@ -268,6 +268,15 @@ struct Session {
 };
 Session g_sess;
 
+// hipEvent timing of dWorldQuickStep's device step, on request
+struct KernelTiming {
+    bool on = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    long long steps = 0;
+    double ms = 0.0, row_sweeps = 0.0;
+};
+KernelTiming g_ktime;
+
 bool gpu_collide_pairs(Session &S, int maxc) {
     const int np = (int)S.pairs.size() / 2;
     if (!S.dpairs.ensure((size_t)np * 2 * sizeof(int)) || !S.out.ensure((size_t)np * ODX_MAXC * 7 * sizeof(float)) ||
@@ -1055,8 +1064,15 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     bool want_fb = false;
     for (dxJoint *j : joints) want_fb = want_fb || j->feedback;
     P.feedback = want_fb ? 1 : 0;
+    if (g_ktime.on && !g_ktime.ev[0]) {
+        if (!odx_check(hipEventCreate(&g_ktime.ev[0]), "hipEventCreate") ||
+            !odx_check(hipEventCreate(&g_ktime.ev[1]), "hipEventCreate"))
+            return 0;
+    }
+    if (g_ktime.on && !odx_check(hipEventRecord(g_ktime.ev[0], g_stream), "hipEventRecord")) return 0;
     launch_single_step(D, P, nb, g_stream);
     if (!odx_check(hipGetLastError(), "k_single_step")) return 0;
+    if (g_ktime.on && !odx_check(hipEventRecord(g_ktime.ev[1], g_stream), "hipEventRecord")) return 0;
     std::vector<float> fbh(want_fb ? (size_t)nj * 12 : 0);
     int it = 0;
     float res = -1.f;
@@ -1087,6 +1103,29 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     w->last_sweeps = it;
     w->last_res = res;
     w->last_rows = m;
+    if (g_ktime.on) {
+        float ms = 0.f;
+        if (!odx_check(hipEventElapsedTime(&ms, g_ktime.ev[0], g_ktime.ev[1]), "hipEventElapsedTime")) return 0;
+        g_ktime.steps += 1;
+        g_ktime.ms += ms;
+        g_ktime.row_sweeps += (double)m * it;
+    }
+    return 1;
+}
+
+// Kernel time of the device step of dWorldQuickStep (no ODE counterpart).
+void odxApiSetKernelTiming(int on) {
+    g_ktime.on = on != 0;
+    g_ktime.steps = 0;
+    g_ktime.ms = 0.0;
+    g_ktime.row_sweeps = 0.0;
+}
+
+int odxApiGetKernelTiming(double *stats) {
+    if (!stats) return 0;
+    stats[0] = (double)g_ktime.steps;
+    stats[1] = g_ktime.ms;
+    stats[2] = g_ktime.row_sweeps;
     return 1;
 }
 

@@ -155,13 +155,15 @@ def quickstep(bodies, joints, prm, dtype=np.float64):
     return bodies, lam[:m]
 
 
-def world_step(bodies, geoms, prm, balls=None, dtype=np.float32):
-    """One canonical world step; returns (new_bodies, contacts[nc,12], lambda[m])."""
+def world_step(bodies, geoms, prm, balls=None, dtype=np.float32, ccap=None):
+    """One canonical world step; returns (new_bodies, contacts[nc,12], lambda[m]).
+    ccap bounds the contacts (default: 4 per geom pair)."""
     bodies = np.ascontiguousarray(bodies, dtype).copy()
     geoms = np.ascontiguousarray(geoms, dtype)
     balls = np.zeros((0, JOINT), dtype) if balls is None else np.ascontiguousarray(balls, dtype)
     ng = len(geoms)
-    ccap = 4 * ng * ng + 64
+    if ccap is None:
+        ccap = 4 * ng * ng + 64
     contacts = np.zeros((ccap, CONTACT), dtype)
     lam = np.zeros(3 * ccap + 3 * len(balls) + 8, dtype)
     m = ctypes.c_int(0)
