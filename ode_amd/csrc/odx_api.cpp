@@ -21,7 +21,8 @@
 #include "odx_host.h"
 
 namespace odx {
-void launch_single_step(const DevWorlds &D, const StepParams &P, int nb, hipStream_t s);
+void launch_single_step(const DevWorlds &D, const StepParams &P, int nb, int m, int nitems, int nphases,
+                        hipStream_t s);
 void launch_aabbs(const float *gviews, int n, float *aabb, hipStream_t s);
 void launch_pairs_all(const float *gviews, const float *aabb, const unsigned *bits, int n, int space, int *pairs,
                       int *count, int cap, hipStream_t s);
@@ -1070,7 +1071,7 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
             return 0;
     }
     if (g_ktime.on && !odx_check(hipEventRecord(g_ktime.ev[0], g_stream), "hipEventRecord")) return 0;
-    launch_single_step(D, P, nb, g_stream);
+    launch_single_step(D, P, nb, m, (int)sched.size(), L, g_stream);
     if (!odx_check(hipGetLastError(), "k_single_step")) return 0;
     if (g_ktime.on && !odx_check(hipEventRecord(g_ktime.ev[1], g_stream), "hipEventRecord")) return 0;
     std::vector<float> fbh(want_fb ? (size_t)nj * 12 : 0);

@@ -1550,37 +1550,145 @@ __device__ float block_max(float v, float *s_red) {
     return r;
 }
 
-__global__ __launch_bounds__(SL_BLOCK) void k_single_step(DevWorlds D, StepParams P) {
-    extern __shared__ float s_fc[];  // nb * 6
+// One work item of the single-world sweeps (first row << 3 | rows) with lambda
+// at `lam` (LDS or global); `pre`, when given, holds the item's first row.
+__device__ __forceinline__ void process_item_l(const DevWorlds &D, const float4 *pre, int item, float *lam,
+                                               float *s_fc) {
+    const int rs = item >> 3, nr = item & 7;
+    const float4 *row4 = reinterpret_cast<const float4 *>(D.rows + (size_t)rs * ROW_STRIDE);
+    float4 a[8];
+    if (pre) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) a[k] = pre[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; k++) a[k] = row4[k];
+    }
+    const int b1 = as_int(a[7].x), b2 = as_int(a[7].y);
+    float f1[6], f2[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 6; k++) f1[k] = s_fc[b1 * 6 + k];
+    if (b2 >= 0) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) f2[k] = s_fc[b2 * 6 + k];
+    }
+    float lam_first = 0.f;
+    for (int q = 0; q < nr; q++) {
+        if (q > 0) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) a[k] = row4[q * 8 + k];
+        }
+        const float old = lam[rs + q];
+        float delta = a[1].z - old * a[1].w;
+        delta -= f1[0] * a[0].x + f1[1] * a[0].y + f1[2] * a[0].z + f1[3] * a[0].w + f1[4] * a[1].x + f1[5] * a[1].y;
+        if (b2 >= 0)
+            delta -= f2[0] * a[2].x + f2[1] * a[2].y + f2[2] * a[2].z + f2[3] * a[2].w + f2[4] * a[3].x + f2[5] * a[3].y;
+        float lo = a[3].z, hi = a[3].w;
+        const int fidx = as_int(a[7].z);
+        if (fidx >= 0) {
+            const float lf = (fidx == rs) ? lam_first : lam[fidx];
+            hi = fabsf(hi * lf);
+            lo = -hi;
+        }
+        float nl = old + delta;
+        if (nl < lo) { delta = lo - old; nl = lo; }
+        else if (nl > hi) { delta = hi - old; nl = hi; }
+        lam[rs + q] = nl;
+        if (q == 0) lam_first = nl;
+        f1[0] += delta * a[4].x; f1[1] += delta * a[4].y; f1[2] += delta * a[4].z;
+        f1[3] += delta * a[4].w; f1[4] += delta * a[5].x; f1[5] += delta * a[5].y;
+        if (b2 >= 0) {
+            f2[0] += delta * a[5].z; f2[1] += delta * a[5].w; f2[2] += delta * a[6].x;
+            f2[3] += delta * a[6].y; f2[4] += delta * a[6].z; f2[5] += delta * a[6].w;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) s_fc[b1 * 6 + k] = f1[k];
+    if (b2 >= 0) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) s_fc[b2 * 6 + k] = f2[k];
+    }
+}
+
+// The single-world step.  STAGED = false: fc in LDS, lambda, the schedule and
+// the rows in global memory.  STAGED = true: lambda, the work items and the
+// phase starts are copied into LDS next to fc, and every thread loads its item
+// and first row of the next phase before the phase barrier, so a phase costs
+// LDS latency instead of dependent global loads.
+template <bool STAGED, int NT>
+__global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P) {
+    extern __shared__ float s_dyn[];
     __shared__ float s_red[32];
     const int t = threadIdx.x;
     const int nb = D.body_start[1];
     const int nj = D.njoints[0];
     const int m = D.nrows[0];
     const int L = D.nphases[0];
-    float *lam = D.lambda;
-    for (int i = t; i < nb; i += SL_BLOCK) body_prep(D, P, i);
-    for (int k = t; k < nb * 6; k += SL_BLOCK) s_fc[k] = 0.f;
+    const int nit = STAGED ? D.phase_start[L] : 0;
+    float *s_fc = s_dyn;
+    float *lam = STAGED ? s_dyn + nb * 6 : D.lambda;
+    const int *sched = STAGED ? reinterpret_cast<const int *>(s_dyn + nb * 6 + m) : D.sched;
+    const int *ph = STAGED ? reinterpret_cast<const int *>(s_dyn + nb * 6 + m + nit) : D.phase_start;
+    for (int i = t; i < nb; i += NT) body_prep(D, P, i);
+    for (int k = t; k < nb * 6; k += NT) s_fc[k] = 0.f;
+    if (STAGED) {
+        int *ss = reinterpret_cast<int *>(s_dyn + nb * 6 + m);
+        for (int k = t; k < nit; k += NT) ss[k] = D.sched[k];
+        for (int k = t; k <= L; k += NT) ss[nit + k] = D.phase_start[k];
+    }
     __syncthreads();
-    for (int j = t; j < nj; j += SL_BLOCK) build_joint_rows(D, P, D.joints + (size_t)j * JOINT_STRIDE, 0, 0, D.joff[j]);
-    __syncthreads();
-    for (int r = t; r < m; r += SL_BLOCK) lam[r] = 0.f;
+    for (int j = t; j < nj; j += NT) build_joint_rows(D, P, D.joints + (size_t)j * JOINT_STRIDE, 0, 0, D.joff[j]);
+    for (int r = t; r < m; r += NT) lam[r] = 0.f;
     __syncthreads();
     int it = 0;
     float res = -1.f;
     if (m > 0) {
         while (it < P.max_iters) {
-            for (int p = 0; p < L; p++) {
-                const int s = D.phase_start[p], e = D.phase_start[p + 1];
-                for (int k = s + t; k < e; k += SL_BLOCK) process_item(D, 0, D.sched[k], s_fc);
-                __syncthreads();
+            if (STAGED) {
+                int s = ph[0], e = ph[1];
+                int cur = s + t < e ? sched[s + t] : -1;
+                float4 pre[8], npre[8];
+                if (cur >= 0) {
+                    const float4 *rr = reinterpret_cast<const float4 *>(D.rows + (size_t)(cur >> 3) * ROW_STRIDE);
+#pragma unroll
+                    for (int k = 0; k < 8; k++) pre[k] = rr[k];
+                }
+                for (int p = 0; p < L; p++) {
+                    int nxt = -1, s2 = 0, e2 = 0;
+                    if (p + 1 < L) {
+                        s2 = ph[p + 1];
+                        e2 = ph[p + 2];
+                        if (s2 + t < e2) {
+                            nxt = sched[s2 + t];
+                            const float4 *rr = reinterpret_cast<const float4 *>(D.rows + (size_t)(nxt >> 3) * ROW_STRIDE);
+#pragma unroll
+                            for (int k = 0; k < 8; k++) npre[k] = rr[k];
+                        }
+                    }
+                    if (cur >= 0) process_item_l(D, pre, cur, lam, s_fc);
+                    for (int k = s + t + NT; k < e; k += NT) process_item_l(D, nullptr, sched[k], lam, s_fc);
+                    __syncthreads();
+                    s = s2;
+                    e = e2;
+                    cur = nxt;
+                    if (nxt >= 0) {
+#pragma unroll
+                        for (int k = 0; k < 8; k++) pre[k] = npre[k];
+                    }
+                }
+            } else {
+                for (int p = 0; p < L; p++) {
+                    const int s = ph[p], e = ph[p + 1];
+                    for (int k = s + t; k < e; k += NT) process_item_l(D, nullptr, sched[k], lam, s_fc);
+                    __syncthreads();
+                }
             }
             it++;
             if (P.tol > 0.f && (it % P.check_every == 0 || it == P.max_iters)) {
                 // residual over all rows (block-wide)
                 float mx = 0.f, lmax = 0.f;
                 const float iw = 1.f / P.sor_w;
-                for (int r = t; r < m; r += SL_BLOCK) {
+                for (int r = t; r < m; r += NT) {
                     const float4 *rr = reinterpret_cast<const float4 *>(D.rows + (size_t)r * ROW_STRIDE);
                     const float4 a0 = rr[0], a1 = rr[1], a2 = rr[2], a3 = rr[3], a7 = rr[7];
                     const int b1 = as_int(a7.x), b2 = as_int(a7.y), fidx = as_int(a7.z);
@@ -1610,53 +1718,27 @@ __global__ __launch_bounds__(SL_BLOCK) void k_single_step(DevWorlds D, StepParam
         res = -1.f;
     }
     if (t == 0) { D.niters[0] = it; D.residual[0] = res; }
+    if (STAGED) {
+        for (int r = t; r < m; r += NT) D.lambda[r] = lam[r];
+        __syncthreads();
+    }
     // joint feedback: f = J^T lambda with the unscaled J (J_row / Ad)
     if (P.feedback) {
-        for (int j = t; j < nj; j += SL_BLOCK) {
+        for (int j = t; j < nj; j += NT) {
             const float *rec = D.joints + (size_t)j * JOINT_STRIDE;
             const int ri = D.joff[j];
             const int mj = (int)rec[0] == 0 ? contact_rows_m(D.surf[(int)rec[13]]) : 3;
             float f[12] = {0.f};
             for (int q = 0; q < mj; q++) {
                 const float *rw = D.rows + (size_t)(ri + q) * ROW_STRIDE;
-                const float s = lam[ri + q] / rw[R_AD];
-                for (int k = 0; k < 6; k++) { f[k] += rw[R_J1 + k] * s; f[6 + k] += rw[R_J2 + k] * s; }
+                const float sc = D.lambda[ri + q] / rw[R_AD];
+                for (int k = 0; k < 6; k++) { f[k] += rw[R_J1 + k] * sc; f[6 + k] += rw[R_J2 + k] * sc; }
             }
             for (int k = 0; k < 12; k++) D.fb[(size_t)j * 12 + k] = f[k];
         }
     }
     __syncthreads();
-    const float h = P.h;
-    for (int i = t; i < nb; i += SL_BLOCK) {
-        const float *s = D.bscr + (size_t)i * BSCR_STRIDE;
-        float4 p4 = D.pos[i], q4 = D.quat[i], lv = D.lvel[i], av = D.avel[i], fa = D.facc[i], ta = D.tacc[i];
-        const float *fc = s_fc + i * 6;
-        if (m > 0) {
-            lv.x += h * fc[0]; lv.y += h * fc[1]; lv.z += h * fc[2];
-            av.x += h * fc[3]; av.y += h * fc[4]; av.z += h * fc[5];
-        }
-        const float ims = h * s[30];
-        lv.x += ims * fa.x; lv.y += ims * fa.y; lv.z += ims * fa.z;
-        float tq[3] = {ta.x * h, ta.y * h, ta.z * h}, dw[3];
-        mul0_331(dw, s + 12, tq);
-        av.x += dw[0]; av.y += dw[1]; av.z += dw[2];
-        p4.x += h * lv.x; p4.y += h * lv.y; p4.z += h * lv.z;
-        float dq0 = 0.5f * (-av.x * q4.y - av.y * q4.z - av.z * q4.w);
-        float dq1 = 0.5f * (av.x * q4.x + av.y * q4.w - av.z * q4.z);
-        float dq2 = 0.5f * (-av.x * q4.w + av.y * q4.x + av.z * q4.y);
-        float dq3 = 0.5f * (av.x * q4.z - av.y * q4.y + av.z * q4.x);
-        q4.x += h * dq0; q4.y += h * dq1; q4.z += h * dq2; q4.w += h * dq3;
-        const float l2 = q4.x * q4.x + q4.y * q4.y + q4.z * q4.z + q4.w * q4.w;
-        if (l2 > 0.f) {
-            const float l = 1.f / sqrtf(l2);
-            q4.x *= l; q4.y *= l; q4.z *= l; q4.w *= l;
-        } else {
-            q4 = make_float4(1.f, 0.f, 0.f, 0.f);
-        }
-        D.pos[i] = p4; D.quat[i] = q4; D.lvel[i] = lv; D.avel[i] = av;
-        D.facc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        D.tacc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
+    for (int i = t; i < nb; i += NT) integrate_body(D, P, i, s_fc + i * 6, m > 0);
 }
 
 // -------------------------------------------- collision entry points -------
@@ -1764,8 +1846,28 @@ void launch_step_fused(const DevWorlds &D, const StepParams &P, const MlpDev *M,
 }  // namespace odx
 
 namespace odx {
-void launch_single_step(const DevWorlds &D, const StepParams &P, int nb, hipStream_t s) {
-    hipLaunchKernelGGL(k_single_step, dim3(1), dim3(SL_BLOCK), (size_t)nb * 6 * sizeof(float), s, D, P);
+// The staged kernel when fc, lambda, the items and the phase starts fit the
+// device's LDS per workgroup; otherwise the global-memory kernel.
+void launch_single_step(const DevWorlds &D, const StepParams &P, int nb, int m, int nitems, int nphases,
+                        hipStream_t s) {
+    static int lds_max = -1;
+    if (lds_max < 0) {
+        int dev = 0, v = 0;
+        lds_max = (hipGetDevice(&dev) == hipSuccess &&
+                   hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess)
+                      ? v
+                      : 65536;
+    }
+    const size_t staged = ((size_t)nb * 6 + m + nitems + nphases + 1) * sizeof(float);
+    if (staged + 256 <= (size_t)lds_max &&
+        hipFuncSetAttribute((const void *)k_single_step_t<true, 512>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)staged) == hipSuccess) {
+        hipLaunchKernelGGL((k_single_step_t<true, 512>), dim3(1), dim3(512), staged, s, D, P);
+        return;
+    }
+    (void)hipGetLastError();
+    hipLaunchKernelGGL((k_single_step_t<false, SL_BLOCK>), dim3(1), dim3(SL_BLOCK), (size_t)nb * 6 * sizeof(float), s,
+                       D, P);
 }
 void launch_aabbs(const float *gviews, int n, float *aabb, hipStream_t s) {
     hipLaunchKernelGGL(k_aabbs, dim3((n + 255) / 256), dim3(256), 0, s, gviews, n, aabb);
