@@ -237,7 +237,7 @@ def bench_boxes(args):
         path = f.name
     try:
         res = subprocess.run([exe, path, str(args.warmup), str(args.steps), str(args.large_iters), "0.5",
-                              str(args.row_order)],
+                              str(args.row_order), "0"],
                              capture_output=True, text=True, timeout=900)
     finally:
         os.unlink(path)
@@ -264,6 +264,7 @@ def bench_boxes(args):
         "config": {"workload": "boxes", "bodies": args.bodies, "space": "sweep-and-prune (XYZ)",
                    "iterations": args.large_iters, "mu": 0.5, "erp": 0.2, "cfm": 1e-5, "h": 0.01, "sor_w": 1.3,
                    "row_order": "reference (dependency levels)" if args.row_order == 0 else "colour phases",
+                   "dynamic_iterations": "off (fixed count)",
                    "seed": args.seed},
         "frame": r,
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",

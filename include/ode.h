@@ -7,7 +7,10 @@
  * libode.  Each declaration names the reference declaration it replaces
  * (include/ode/<file>:<line> of thomasmarsh/ODE).  Semantics kept: normal into
  * body 1, reversed attach negates the normal, world ERP/CFM defaults 0.2/1e-5,
- * SOR w = 1.3, gyroscopic torque on, QuickStep row order with findex rows last.
+ * SOR w = 1.3, gyroscopic torque on, QuickStep row order with findex rows last,
+ * the dynamic iteration count per island (premature exit 1e-8, up to
+ * iterations x 1.0 extra sweeps while an adjustment is >= 1e-2; all three set
+ * to 0 give a fixed count).
  * Differences: the solve is deterministic (no random row reordering); bodies
  * may carry sphere/box/capsule/plane geoms; joints are contact and ball.
  */
@@ -105,6 +108,9 @@ int dWorldGetQuickStepNumIterations(dWorldID w);               /* objects.h:470 
 void dWorldSetQuickStepDynamicIterationParameters(dWorldID w, const dReal *premature_exit_delta,
                                                   const dReal *max_num_extra_factor,
                                                   const dReal *extra_iteration_requirement_delta); /* objects.h:509 */
+void dWorldGetQuickStepDynamicIterationParameters(dWorldID w, dReal *premature_exit_delta,
+                                                  dReal *max_num_extra_factor,
+                                                  dReal *extra_iteration_requirement_delta); /* objects.h:528 */
 void dWorldSetQuickStepW(dWorldID w, dReal over_relaxation);   /* objects.h:584 */
 dReal dWorldGetQuickStepW(dWorldID w);                         /* objects.h:591 */
 void dWorldSetContactMaxCorrectingVel(dWorldID w, dReal vel);  /* objects.h:603 */

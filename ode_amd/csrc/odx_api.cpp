@@ -183,14 +183,22 @@ struct dxWorld {
     dVector3 gravity = {0, 0, 0, 0};
     float erp = 0.2f, cfm = 1e-5f, w = 1.3f;
     int iterations = 20;
+    // dynamic iteration count adjustment (objects.h:121-203; dWorldCreate
+    // defaults include/ode/objects.h:473-475): premature exit delta, maximal
+    // extra iteration factor, extra iteration requirement delta
+    float dyn_premature = 1e-8f, dyn_extra_factor = 1.0f, dyn_extra_delta = 1e-2f;
     float max_vel = dInfinity, min_depth = 0.f;
     int order = 0;
     int last_sweeps = 0, last_rows = 0;
     float last_res = -1.f;
     // device buffers of the step
     DevBuf pos, quat, lvel, avel, facc, tacc, mass, Ib, invIb, joints, surf, joff, rows, lambda, sched, phase, bscr,
-        counts, fb, bstart, residual, niters, scratch;
+        counts, fb, bstart, residual, niters, scratch, island;
 };
+
+// Extra sweeps of the dynamic iteration count are capped here (the reference
+// allows up to UINT_MAX); a factor asking for more gets this many.
+constexpr int ODX_MAX_EXTRA_SWEEPS = 100000;
 
 namespace {
 
@@ -341,9 +349,22 @@ void dWorldSetCFM(dWorldID w, dReal cfm) { w->cfm = cfm; }
 dReal dWorldGetCFM(dWorldID w) { return w->cfm; }
 void dWorldSetQuickStepNumIterations(dWorldID w, int num) { w->iterations = num; }
 int dWorldGetQuickStepNumIterations(dWorldID w) { return w->iterations; }
-void dWorldSetQuickStepDynamicIterationParameters(dWorldID, const dReal *, const dReal *, const dReal *) {
-    // The GPU solve runs exactly the configured iteration count (the
-    // reference's dynamic adjustment disabled); the parameters are accepted.
+// ode.cpp:2068-2088 (negative values clamp to 0)
+void dWorldSetQuickStepDynamicIterationParameters(dWorldID w, const dReal *premature_exit_delta,
+                                                  const dReal *max_num_extra_factor,
+                                                  const dReal *extra_iteration_requirement_delta) {
+    if (!w) return;
+    if (premature_exit_delta) w->dyn_premature = std::max(*premature_exit_delta, 0.f);
+    if (extra_iteration_requirement_delta) w->dyn_extra_delta = std::max(*extra_iteration_requirement_delta, 0.f);
+    if (max_num_extra_factor) w->dyn_extra_factor = std::max(*max_num_extra_factor, 0.f);
+}
+// ode.cpp:2091-2108
+void dWorldGetQuickStepDynamicIterationParameters(dWorldID w, dReal *premature_exit_delta, dReal *max_num_extra_factor,
+                                                  dReal *extra_iteration_requirement_delta) {
+    if (!w) return;
+    if (premature_exit_delta) *premature_exit_delta = w->dyn_premature;
+    if (extra_iteration_requirement_delta) *extra_iteration_requirement_delta = w->dyn_extra_delta;
+    if (max_num_extra_factor) *max_num_extra_factor = w->dyn_extra_factor;
 }
 void dWorldSetQuickStepW(dWorldID w, dReal over_relaxation) { w->w = over_relaxation; }
 dReal dWorldGetQuickStepW(dWorldID w) { return w->w; }
@@ -834,10 +855,13 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     std::vector<char> jtag;
     auto tagged = [&](dxJoint *j) { return j->tag != 0; };
     std::vector<dxJoint *> seen;
+    std::vector<int> island;  // island of each listed body
+    int nisl = 0;
     for (dxBody *bb = w->firstbody; bb; bb = bb->next) {
         if (bb->tag || !bb->enabled) continue;
         bb->tag = 1;
         bodies.push_back(bb);
+        island.push_back(nisl);
         dxBody *b = bb;
         while (true) {
             for (dxJointNode *nd = b->firstjoint; nd; nd = nd->next) {
@@ -862,7 +886,9 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
             b = stack.back();
             stack.pop_back();
             bodies.push_back(b);
+            island.push_back(nisl);
         }
+        nisl++;
     }
     for (dxJoint *j : seen) j->tag = 0;
     const int nb = (int)bodies.size();
@@ -1065,6 +1091,19 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     bool want_fb = false;
     for (dxJoint *j : joints) want_fb = want_fb || j->feedback;
     P.feedback = want_fb ? 1 : 0;
+    // dynamic iteration count (objects.h:177-189): extra sweeps = iterations x
+    // factor; on when that is non-zero or the premature delta is
+    {
+        const float xc = (float)w->iterations * w->dyn_extra_factor;
+        const int max_extra = xc < (float)ODX_MAX_EXTRA_SWEEPS ? (int)xc : ODX_MAX_EXTRA_SWEEPS;
+        P.dyn = (max_extra != 0 || w->dyn_premature != 0.f) ? 1 : 0;
+        P.dyn_max_extra = max_extra;
+        P.dyn_premature = w->dyn_premature;
+        P.dyn_extra_delta = w->dyn_extra_delta;
+        P.nislands = nisl;
+        if (P.dyn && !(up(w->island, island.data(), island.size()))) return 0;
+        D.body_island = P.dyn ? w->island.as<int>() : nullptr;
+    }
     if (g_ktime.on && !g_ktime.ev[0]) {
         if (!odx_check(hipEventCreate(&g_ktime.ev[0]), "hipEventCreate") ||
             !odx_check(hipEventCreate(&g_ktime.ev[1]), "hipEventCreate"))

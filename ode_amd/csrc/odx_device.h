@@ -67,6 +67,11 @@ struct StepParams {
     int split;           // feature counter pair of this launch: feat_count[2 * split], [2 * split + 1]
     int feat_base;       // first feature slot of this launch (joint_start[wbase])
     int feat_local;      // 1: features go to slots joint_start[w] + i, in joint order (fused kernel)
+    // dynamic iteration count per island (drop-in world, quickstep.cpp:1820-1853)
+    int dyn;             // 1: on (then max_iters is the base count)
+    int dyn_max_extra;   // extra sweeps allowed after max_iters
+    float dyn_premature, dyn_extra_delta;
+    int nislands;        // islands of the drop-in world (body_island values)
 };
 
 // Independent world ranges a batch may be split into (one stream each).
@@ -122,6 +127,7 @@ struct DevWorlds {
     int *order;             // [W] solve dispatch order (longest previous solve first); null = index order
     long long *wtime;       // [W][2] solve start / end of each world (device wall clock), may be null
     long long *ptime;       // [W][PTIME_SLOTS] prepare stage boundaries (device wall clock), may be null
+    const int *body_island; // [NB] island of each body (drop-in world with the dynamic iteration count), may be null
 };
 constexpr int PTIME_SLOTS = 6;
 

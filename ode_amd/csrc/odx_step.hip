@@ -1552,8 +1552,13 @@ __device__ float block_max(float v, float *s_red) {
 
 // One work item of the single-world sweeps (first row << 3 | rows) with lambda
 // at `lam` (LDS or global); `pre`, when given, holds the item's first row.
+// With the dynamic iteration count (fa non-null): items of islands that have
+// stopped are skipped, and every non-zero delta adds delta * max|iMJ side| to
+// the body's positive or negative force adjustment (quickstep.cpp:3001-3031;
+// iMJ is stored unscaled, as compute_invM_JT's maxabs reads it).
 __device__ __forceinline__ void process_item_l(const DevWorlds &D, const float4 *pre, int item, float *lam,
-                                               float *s_fc) {
+                                               float *s_fc, float *fa = nullptr, const int *isl_b = nullptr,
+                                               const int *ist = nullptr) {
     const int rs = item >> 3, nr = item & 7;
     const float4 *row4 = reinterpret_cast<const float4 *>(D.rows + (size_t)rs * ROW_STRIDE);
     float4 a[8];
@@ -1565,6 +1570,7 @@ __device__ __forceinline__ void process_item_l(const DevWorlds &D, const float4 
         for (int k = 0; k < 8; k++) a[k] = row4[k];
     }
     const int b1 = as_int(a[7].x), b2 = as_int(a[7].y);
+    if (fa && !ist[isl_b[b1] * 4 + 2]) return;
     float f1[6], f2[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int k = 0; k < 6; k++) f1[k] = s_fc[b1 * 6 + k];
@@ -1601,6 +1607,17 @@ __device__ __forceinline__ void process_item_l(const DevWorlds &D, const float4 
             f2[0] += delta * a[5].z; f2[1] += delta * a[5].w; f2[2] += delta * a[6].x;
             f2[3] += delta * a[6].y; f2[4] += delta * a[6].z; f2[5] += delta * a[6].w;
         }
+        if (fa && delta != 0.f) {
+            const int sg = delta > 0.f ? 1 : 0;
+            const float m1 = fmaxf(fmaxf(fmaxf(fabsf(a[4].x), fabsf(a[4].y)), fmaxf(fabsf(a[4].z), fabsf(a[4].w))),
+                                   fmaxf(fabsf(a[5].x), fabsf(a[5].y)));
+            fa[b1 * 2 + sg] += delta * m1;
+            if (b2 >= 0) {
+                const float m2 = fmaxf(fmaxf(fmaxf(fabsf(a[5].z), fabsf(a[5].w)), fmaxf(fabsf(a[6].x), fabsf(a[6].y))),
+                                       fmaxf(fabsf(a[6].z), fabsf(a[6].w)));
+                fa[b2 * 2 + sg] += delta * m2;
+            }
+        }
     }
 #pragma unroll
     for (int k = 0; k < 6; k++) s_fc[b1 * 6 + k] = f1[k];
@@ -1629,6 +1646,13 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
     float *lam = STAGED ? s_dyn + nb * 6 : D.lambda;
     const int *sched = STAGED ? reinterpret_cast<const int *>(s_dyn + nb * 6 + m) : D.sched;
     const int *ph = STAGED ? reinterpret_cast<const int *>(s_dyn + nb * 6 + m + nit) : D.phase_start;
+    // dynamic iteration count: force adjustments [nb][2], island of each body,
+    // per island (sweeps, extra sweeps, active, not converged), limit per island
+    const int nisl = P.dyn ? P.nislands : 0;
+    float *fa = P.dyn ? s_dyn + nb * 6 + (STAGED ? m + nit + L + 1 : 0) : nullptr;
+    int *isl_b = P.dyn ? reinterpret_cast<int *>(fa + nb * 2) : nullptr;
+    int *ist = P.dyn ? isl_b + nb : nullptr;
+    float *ilim = P.dyn ? reinterpret_cast<float *>(ist + nisl * 4) : nullptr;
     for (int i = t; i < nb; i += NT) body_prep(D, P, i);
     for (int k = t; k < nb * 6; k += NT) s_fc[k] = 0.f;
     if (STAGED) {
@@ -1642,8 +1666,8 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
     __syncthreads();
     int it = 0;
     float res = -1.f;
-    if (m > 0) {
-        while (it < P.max_iters) {
+    // one sweep over the phases (items of stopped islands skipped when fa is set)
+    auto sweep = [&]() {
             if (STAGED) {
                 int s = ph[0], e = ph[1];
                 int cur = s + t < e ? sched[s + t] : -1;
@@ -1665,8 +1689,9 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
                             for (int k = 0; k < 8; k++) npre[k] = rr[k];
                         }
                     }
-                    if (cur >= 0) process_item_l(D, pre, cur, lam, s_fc);
-                    for (int k = s + t + NT; k < e; k += NT) process_item_l(D, nullptr, sched[k], lam, s_fc);
+                    if (cur >= 0) process_item_l(D, pre, cur, lam, s_fc, fa, isl_b, ist);
+                    for (int k = s + t + NT; k < e; k += NT)
+                        process_item_l(D, nullptr, sched[k], lam, s_fc, fa, isl_b, ist);
                     __syncthreads();
                     s = s2;
                     e = e2;
@@ -1679,10 +1704,62 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
             } else {
                 for (int p = 0; p < L; p++) {
                     const int s = ph[p], e = ph[p + 1];
-                    for (int k = s + t; k < e; k += NT) process_item_l(D, nullptr, sched[k], lam, s_fc);
+                    for (int k = s + t; k < e; k += NT) process_item_l(D, nullptr, sched[k], lam, s_fc, fa, isl_b, ist);
                     __syncthreads();
                 }
             }
+    };
+    if (m > 0 && P.dyn) {
+        // the iteration loop of dxQuickStepIsland (quickstep.cpp:1820-1853) per
+        // island: after each sweep an island that has run max_iters (+ its extra
+        // sweeps) stops; at max_iters it may switch to up to dyn_max_extra extra
+        // sweeps with the extra-iteration delta as its limit; it stops early once
+        // every body's positive and negative adjustment is below its limit
+        // (CheckForMaximumToBeLessThanLimitAndResetMaxAdjustments :3253-3285)
+        for (int i = t; i < nisl; i += NT) {
+            ist[i * 4 + 0] = 0; ist[i * 4 + 1] = 0; ist[i * 4 + 2] = 0; ist[i * 4 + 3] = 0;
+            ilim[i] = P.dyn_premature;
+        }
+        for (int b = t; b < nb; b += NT) { fa[b * 2] = 0.f; fa[b * 2 + 1] = 0.f; isl_b[b] = D.body_island[b]; }
+        __syncthreads();
+        for (int r = t; r < m; r += NT) ist[isl_b[as_int(D.rows[(size_t)r * ROW_STRIDE + R_B1])] * 4 + 2] = 1;
+        __syncthreads();
+        int any = 1;
+        while (any) {
+            sweep();
+            for (int i = t; i < nisl; i += NT) {
+                int *S = ist + i * 4;
+                if (!S[2]) continue;
+                S[0]++;
+                if (S[0] - S[1] == P.max_iters) {
+                    if (S[1] != 0 || P.dyn_max_extra == 0) { S[2] = 0; continue; }
+                    S[1] = P.dyn_max_extra;
+                    ilim[i] = P.dyn_extra_delta;
+                }
+                S[3] = ilim[i] == 0.f ? 1 : 0;
+            }
+            __syncthreads();
+            for (int b = t; b < nb; b += NT) {
+                const int i = isl_b[b];
+                if (i >= 0 && ist[i * 4 + 2] && (!(fa[b * 2 + 1] < ilim[i]) || !(-fa[b * 2] < ilim[i]))) ist[i * 4 + 3] = 1;
+                fa[b * 2] = 0.f;
+                fa[b * 2 + 1] = 0.f;
+            }
+            __syncthreads();
+            int mine = 0;
+            for (int i = t; i < nisl; i += NT) {
+                int *S = ist + i * 4;
+                if (S[2] && !S[3]) S[2] = 0;
+                mine |= S[2];
+            }
+            any = __syncthreads_or(mine);
+        }
+        int mx = 0;
+        for (int i = t; i < nisl; i += NT) mx = max(mx, ist[i * 4]);
+        it = (int)block_max((float)mx, s_red);
+    } else if (m > 0) {
+        while (it < P.max_iters) {
+            sweep();
             it++;
             if (P.tol > 0.f && (it % P.check_every == 0 || it == P.max_iters)) {
                 // residual over all rows (block-wide)
@@ -1858,7 +1935,8 @@ void launch_single_step(const DevWorlds &D, const StepParams &P, int nb, int m, 
                       ? v
                       : 65536;
     }
-    const size_t staged = ((size_t)nb * 6 + m + nitems + nphases + 1) * sizeof(float);
+    const size_t dynw = P.dyn ? (size_t)nb * 3 + (size_t)P.nislands * 5 : 0;  // fa, island of body, island state
+    const size_t staged = ((size_t)nb * 6 + m + nitems + nphases + 1 + dynw) * sizeof(float);
     if (staged + 256 <= (size_t)lds_max &&
         hipFuncSetAttribute((const void *)k_single_step_t<true, 512>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)staged) == hipSuccess) {
@@ -1866,8 +1944,11 @@ void launch_single_step(const DevWorlds &D, const StepParams &P, int nb, int m, 
         return;
     }
     (void)hipGetLastError();
-    hipLaunchKernelGGL((k_single_step_t<false, SL_BLOCK>), dim3(1), dim3(SL_BLOCK), (size_t)nb * 6 * sizeof(float), s,
-                       D, P);
+    const size_t plain = ((size_t)nb * 6 + dynw) * sizeof(float);
+    if (plain > 65536)
+        (void)hipFuncSetAttribute((const void *)k_single_step_t<false, SL_BLOCK>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)plain);
+    hipLaunchKernelGGL((k_single_step_t<false, SL_BLOCK>), dim3(1), dim3(SL_BLOCK), plain, s, D, P);
 }
 void launch_aabbs(const float *gviews, int n, float *aabb, hipStream_t s) {
     hipLaunchKernelGGL(k_aabbs, dim3((n + 255) / 256), dim3(256), 0, s, gviews, n, aabb);

@@ -4,10 +4,12 @@
 // dJointGroupEmpty (the loop of ode/demo/demo_space_stress.cpp:76-107,
 // 363-369).  Used by bench.py to time one world through libodx.
 //
-//   frame_loop SCENE WARMUP STEPS ITERS [MU [ORDER]]
+//   frame_loop SCENE WARMUP STEPS ITERS [MU [ORDER [DYNAMIC]]]
 //
 // ORDER 0 (default) sweeps the rows in the reference's order; 1 in colour
-// phases (odxWorldSetRowOrder).
+// phases (odxWorldSetRowOrder).  DYNAMIC 0 (default) runs exactly ITERS sweeps
+// (dWorldSetQuickStepDynamicIterationParameters with zeros); 1 keeps
+// dWorldCreate's dynamic iteration count.
 // SCENE is a binary file: int32 nb, int32 ng, float bodies[nb][32],
 // float geoms[ng][16] (the record layouts of include/odx.h).  The geoms go
 // into a sweep-and-prune space (axes XYZ).  Prints one JSON object.
@@ -46,12 +48,13 @@ void near_callback(void *data, dGeomID o1, dGeomID o2) {
 
 int main(int argc, char **argv) {
     if (argc < 5) {
-        fprintf(stderr, "usage: %s SCENE WARMUP STEPS ITERS [MU [ORDER]]\n", argv[0]);
+        fprintf(stderr, "usage: %s SCENE WARMUP STEPS ITERS [MU [ORDER [DYNAMIC]]]\n", argv[0]);
         return 2;
     }
     const int warmup = atoi(argv[2]), steps = atoi(argv[3]), iters = atoi(argv[4]);
     const float mu = argc > 5 ? (float)atof(argv[5]) : 0.5f;
     const int order = argc > 6 ? atoi(argv[6]) : 0;
+    const int dynamic = argc > 7 ? atoi(argv[7]) : 0;
     FILE *f = fopen(argv[1], "rb");
     if (!f) { perror("scene"); return 2; }
     int nb = 0, ng = 0;
@@ -71,6 +74,10 @@ int main(int argc, char **argv) {
     dWorldSetGravity(c.world, 0, 0, -9.81f);
     dWorldSetQuickStepNumIterations(c.world, iters);
     odxWorldSetRowOrder(c.world, order);
+    if (!dynamic) {
+        const dReal zero = 0;
+        dWorldSetQuickStepDynamicIterationParameters(c.world, &zero, &zero, &zero);
+    }
     dSpaceID space = dSweepAndPruneSpaceCreate(nullptr, dSAP_AXES_XYZ);
     std::vector<dBodyID> bodies(nb);
     for (int i = 0; i < nb; i++) {
@@ -118,9 +125,9 @@ int main(int argc, char **argv) {
     for (dBodyID b : bodies) zmin = dBodyGetPosition(b)[2] < zmin ? dBodyGetPosition(b)[2] : zmin;
     printf("{\"steps\": %d, \"warmup\": %d, \"total_ms\": %.6f, \"ms_per_step\": %.6f, \"pairs_per_step\": %.2f, "
            "\"contacts_per_step\": %.2f, \"min_z\": %.6f, \"kernel_ms_per_step\": %.6f, \"row_sweeps_per_step\": %.1f, "
-           "\"row_order\": %d}\n",
+           "\"row_order\": %d, \"dynamic_iterations\": %d}\n",
            steps, warmup, ms, ms / steps, (double)c.pairs / steps, (double)c.contacts / steps, zmin,
-           kt[0] > 0 ? kt[1] / kt[0] : 0.0, kt[0] > 0 ? kt[2] / kt[0] : 0.0, order);
+           kt[0] > 0 ? kt[1] / kt[0] : 0.0, kt[0] > 0 ? kt[2] / kt[0] : 0.0, order, dynamic);
     dWorldDestroy(c.world);
     dCloseODE();
     return 0;

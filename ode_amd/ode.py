@@ -64,6 +64,8 @@ def api():
         "dWorldGetQuickStepNumIterations": (i, [vp]), "dWorldSetQuickStepW": (None, [vp, f]),
         "dWorldSetContactMaxCorrectingVel": (None, [vp, f]), "dWorldSetContactSurfaceLayer": (None, [vp, f]),
         "odxWorldSetRowOrder": (None, [vp, i]), "odxWorldGetSolveStats": (None, [vp, P(i), P(ctypes.c_float), P(i)]),
+        "dWorldSetQuickStepDynamicIterationParameters": (None, [vp, P(dReal), P(dReal), P(dReal)]),
+        "dWorldGetQuickStepDynamicIterationParameters": (None, [vp, P(dReal), P(dReal), P(dReal)]),
         "dBodyCreate": (vp, [vp]), "dBodyDestroy": (None, [vp]),
         "dBodySetPosition": (None, [vp, f, f, f]), "dBodySetQuaternion": (None, [vp, P(dReal)]),
         "dBodySetRotation": (None, [vp, P(dReal)]),

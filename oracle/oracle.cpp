@@ -15,6 +15,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <functional>
 #include <vector>
 
 namespace {
@@ -1116,6 +1117,10 @@ template <class R> struct Params {
     double lambda_floor;
     double tol;
     int check_every;
+    /* dynamic iteration count adjustment (objects.h:121-203, quickstep.cpp:1809-1860) */
+    bool dyn;
+    unsigned max_extra;
+    R premature_delta, extra_delta;
 };
 
 template <class R> Params<R> read_params(const double *P) {
@@ -1135,6 +1140,14 @@ template <class R> Params<R> read_params(const double *P) {
     p.lambda_floor = P[ORC_P_LAMBDA_FLOOR] > 0 ? P[ORC_P_LAMBDA_FLOOR] : 1e-3;
     p.tol = P[ORC_P_TOL];
     p.check_every = P[ORC_P_CHECK_EVERY] >= 1 ? (int)P[ORC_P_CHECK_EVERY] : 1;
+    /* DeriveExtraIterationCount objects.h:177-184 and the enabled rule of
+     * UpdateDynamicIterationCountAdjustmentEnabledState objects.h:186-189 */
+    const double fac = P[ORC_P_DYN_EXTRA_FACTOR] > 0 ? P[ORC_P_DYN_EXTRA_FACTOR] : 0.0;
+    const double xc = (double)p.iters * (double)(R)fac;
+    p.max_extra = xc < 4294967295.0 ? (unsigned)xc : 4294967295u;
+    p.premature_delta = (R)(P[ORC_P_DYN_PREMATURE] > 0 ? P[ORC_P_DYN_PREMATURE] : 0.0);
+    p.extra_delta = (R)(P[ORC_P_DYN_EXTRA_DELTA] > 0 ? P[ORC_P_DYN_EXTRA_DELTA] : 0.0);
+    p.dyn = p.max_extra != 0 || p.premature_delta != 0;
     return p;
 }
 
@@ -1301,8 +1314,12 @@ template <class R> void assemble(int nb, R *bodies, std::vector<Joint<R>> &joint
     for (int i = 0; i < m; i++) if (S.findex[i] != -1) S.order.push_back(i);
 }
 
-/* dxQuickStepIsland_Stage4LCP_IterationStep :2917-3033 */
-template <class R> void sor_row(const System<R> &S, int index, std::vector<R> &lambda, std::vector<R> &fc) {
+/* dxQuickStepIsland_Stage4LCP_IterationStep :2917-3033.  fa, when given,
+ * receives the force adjustments of the dynamic iteration count: per body,
+ * delta * max|iMJ side| summed separately for positive and negative delta
+ * (:3001-3031; the maxabs of compute_invM_JT :859-897). */
+template <class R> void sor_row(const System<R> &S, int index, std::vector<R> &lambda, std::vector<R> &fc,
+                                std::vector<R> *fa = nullptr) {
     const Row<R> &row = S.J[index];
     R old = lambda[index];
     R delta = row.rhs - old * row.cfm;
@@ -1321,7 +1338,55 @@ template <class R> void sor_row(const System<R> &S, int index, std::vector<R> &l
         const R *o = &S.iMJ[(size_t)index * 12];
         for (int k = 0; k < 6; k++) f1[k] += delta * o[k];
         if (f2) for (int k = 0; k < 6; k++) f2[k] += delta * o[6 + k];
+        if (fa) {
+            const int sgn = delta > 0 ? 1 : 0;  /* [negative, positive] */
+            R m1 = std::fabs(o[0]);
+            for (int k = 1; k < 6; k++) m1 = std::max(m1, (R)std::fabs(o[k]));
+            (*fa)[(size_t)S.jb1[index] * 2 + sgn] += delta * m1;
+            if (f2) {
+                R m2 = std::fabs(o[6]);
+                for (int k = 7; k < 12; k++) m2 = std::max(m2, (R)std::fabs(o[k]));
+                (*fa)[(size_t)S.jb2[index] * 2 + sgn] += delta * m2;
+            }
+        }
     }
+}
+
+/* CheckForMaximumToBeLessThanLimitAndResetMaxAdjustments :3253-3285 over the
+ * bodies of one island: true when every body's positive and negative
+ * adjustment is strictly below limit (never for limit 0); all reset to 0. */
+template <class R> bool adjustments_below(std::vector<R> &fa, const std::vector<int> &bodies, R limit) {
+    bool ok = limit != 0;
+    for (int b : bodies) {
+        if (ok && (!(fa[(size_t)b * 2 + 1] < limit) || !(-fa[(size_t)b * 2] < limit))) ok = false;
+        fa[(size_t)b * 2] = 0;
+        fa[(size_t)b * 2 + 1] = 0;
+    }
+    return ok;
+}
+
+/* Islands of the assembled rows: connected components of the bodies through
+ * the rows, numbered by first appearance in row order (the island builder,
+ * util.cpp:724-855, lists each island's joints contiguously).  Returns the
+ * island of every row; bodies[i] lists island i's bodies. */
+template <class R> std::vector<int> row_islands(const System<R> &S, std::vector<std::vector<int>> &bodies) {
+    std::vector<int> parent(S.nb);
+    for (int i = 0; i < S.nb; i++) parent[i] = i;
+    std::function<int(int)> find = [&](int x) { return parent[x] == x ? x : parent[x] = find(parent[x]); };
+    for (int r = 0; r < S.m; r++)
+        if (S.jb2[r] >= 0) parent[find(S.jb1[r])] = find(S.jb2[r]);
+    std::vector<int> id(S.nb, -1), isl(S.m);
+    bodies.clear();
+    for (int r = 0; r < S.m; r++) {
+        const int root = find(S.jb1[r]);
+        if (id[root] < 0) { id[root] = (int)bodies.size(); bodies.emplace_back(); }
+        isl[r] = id[root];
+    }
+    for (int b = 0; b < S.nb; b++) {
+        const int root = find(b);
+        if (id[root] >= 0) bodies[id[root]].push_back(b);
+    }
+    return isl;
 }
 
 /* LCP residual of the assembled system at (lambda, fc = M^-1 J^T lambda):
@@ -1406,7 +1471,33 @@ template <class R> int quickstep(int nb, R *bodies, std::vector<Joint<R>> &joint
     assemble(nb, bodies, joints, p, S, invI);
     std::vector<R> lambda(S.m, 0), fc((size_t)nb * 6, 0);
     int sweeps = 0;
-    if (S.m > 0) {
+    if (S.m > 0 && p.dyn && !(p.tol > 0)) {
+        /* the iteration loop of dxQuickStepIsland (quickstep.cpp:1820-1853),
+         * run per island with its own count: premature exit once every body's
+         * adjustment is below the premature delta; after `iters`, up to
+         * max_extra more while some adjustment is at least extra_delta */
+        std::vector<std::vector<int>> ib;
+        const std::vector<int> isl = row_islands(S, ib);
+        std::vector<R> fa((size_t)nb * 2, 0);
+        for (size_t i = 0; i < ib.size(); i++) {
+            std::vector<int> rows;
+            for (int k = 0; k < S.m; k++)
+                if (isl[S.order[k]] == (int)i) rows.push_back(S.order[k]);
+            R limit = p.premature_delta;
+            unsigned iteration = 0, extra = 0;
+            for (;;) {
+                for (int r : rows) sor_row(S, r, lambda, fc, &fa);
+                ++iteration;
+                if (iteration - extra == (unsigned)p.iters) {
+                    if (extra != 0 || p.max_extra == 0) break;
+                    extra = p.max_extra;
+                    limit = p.extra_delta;
+                }
+                if (adjustments_below(fa, ib[i], limit)) break;
+            }
+            sweeps = std::max(sweeps, (int)iteration);
+        }
+    } else if (S.m > 0) {
         for (int it = 0; it < p.iters; it++) {
             for (int k = 0; k < S.m; k++) sor_row(S, S.order[k], lambda, fc);
             sweeps = it + 1;

@@ -43,6 +43,11 @@ enum {
     ORC_P_SOFT_ERP, ORC_P_SOFT_CFM, ORC_P_MAXC, ORC_P_SPACE,
     ORC_P_MOTION1, ORC_P_MOTION2, ORC_P_MOTIONN, ORC_P_SLIP1, ORC_P_SLIP2,
     ORC_P_SKIP_CONNECTED, ORC_P_LAMBDA_FLOOR, ORC_P_TOL, ORC_P_CHECK_EVERY,
+    /* dynamic iteration count adjustment (dWorldSetQuickStepDynamicIterationParameters):
+     * premature exit delta, max extra iteration factor, extra iteration
+     * requirement delta.  Enabled when factor * iterations >= 1 or the
+     * premature delta is non-zero (all zero: a fixed iteration count). */
+    ORC_P_DYN_PREMATURE, ORC_P_DYN_EXTRA_FACTOR, ORC_P_DYN_EXTRA_DELTA,
     ORC_P__MAX = 32
 };
 

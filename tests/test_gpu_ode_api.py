@@ -22,7 +22,7 @@ def say(*a):
 class Frame:
     """Builds a world through the C API and records what the callback did."""
 
-    def __init__(self, world_scene, space="sap", mu=0.5, iters=20):
+    def __init__(self, world_scene, space="sap", mu=0.5, iters=20, dynamic=False):
         from ode_amd import ode as A
         self.A = A
         L = A.api()
@@ -30,6 +30,10 @@ class Frame:
         self.w = L.dWorldCreate()
         L.dWorldSetGravity(self.w, 0, 0, -9.81)
         L.dWorldSetQuickStepNumIterations(self.w, iters)
+        if not dynamic:
+            # a fixed iteration count (the dynamic adjustment off: all three zero)
+            z = A.dReal(0.0)
+            L.dWorldSetQuickStepDynamicIterationParameters(self.w, ctypes.byref(z), ctypes.byref(z), ctypes.byref(z))
         self.space = {"sap": lambda: L.dSweepAndPruneSpaceCreate(None, A.dSAP_AXES_XYZ),
                       "hash": lambda: L.dHashSpaceCreate(None),
                       "simple": lambda: L.dSimpleSpaceCreate(None)}[space]()
@@ -143,15 +147,18 @@ def test_space_collide_pairs_and_contacts_match_oracle(space):
     F.close()
 
 
+@pytest.mark.parametrize("dynamic", [False, True])
 @pytest.mark.parametrize("seed", [1, 2])
-def test_quickstep_matches_oracle_in_island_order(seed):
-    say("case: dWorldQuickStep seed", seed)
+def test_quickstep_matches_oracle_in_island_order(seed, dynamic):
+    say("case: dWorldQuickStep seed", seed, "dynamic iterations", dynamic)
     scene = settle_scene(seed)
-    F = Frame(scene)
+    F = Frame(scene, dynamic=dynamic)
     F.collide()
     before = scene.bodies.copy()
     assert F.L.dWorldQuickStep(F.w, 0.01) == 1
     got = F.state()
+    sweeps, res, rows = ctypes.c_int(0), ctypes.c_float(0), ctypes.c_int(0)
+    F.L.odxWorldGetSolveStats(F.w, ctypes.byref(sweeps), ctypes.byref(res), ctypes.byref(rows))
     joints = [(c[0], c[1]) for c in F.contacts]
     # dJointAttach with a world geom first: body1 NULL -> reversed joint
     recs = []
@@ -162,11 +169,22 @@ def test_quickstep_matches_oracle_in_island_order(seed):
         recs.append(r)
     norm_pairs = [(a, b) if a >= 0 else (b, -1) for a, b in joints]
     _, order = island_order(len(scene.bodies), norm_pairs)
-    ref_b, ref_l = O.quickstep(before, np.array([recs[k] for k in order]).reshape(-1, 16), O.params(), dtype=np.float64)
+    # dWorldCreate's dynamic-iteration defaults (include/ode/objects.h:473-475)
+    prm = O.params(**(dict(dyn_premature=1e-8, dyn_extra_factor=1.0, dyn_extra_delta=1e-2) if dynamic else {}))
+    jr = np.array([recs[k] for k in order]).reshape(-1, 16)
+    ref_b, ref_l = O.quickstep(before, jr, prm, dtype=np.float64)
     err_x = np.max(np.abs(got[:, 0:3] - ref_b[:, 0:3])) / np.max(np.abs(ref_b[:, 0:3]))
     err_v = np.max(np.abs(got[:, 8:15] - ref_b[:, 8:15])) / max(np.max(np.abs(ref_b[:, 8:15])), 1.0)
-    say("  contacts", len(recs), "err_x", err_x, "err_v", err_v)
+    # the sweep count (largest over islands) against the float32 oracle: float32
+    # adjustments stall near rounding level, as in a dSINGLE build
+    O.quickstep(before, jr, prm, dtype=np.float32)
+    want = O.last_sweeps()
+    say("  contacts", len(recs), "err_x", err_x, "err_v", err_v, "sweeps", sweeps.value, "oracle f32", want)
     assert err_x < 1e-4 and err_v < 1e-4
+    if dynamic:
+        assert abs(sweeps.value - want) <= 1, (sweeps.value, want)
+    else:
+        assert sweeps.value == 20
     F.L.dJointGroupEmpty(F.group)
     F.close()
 
@@ -229,3 +247,46 @@ def test_dcollide_alone_and_drand():
     assert [L.dRand() for _ in range(5)] == O.drand_seq(0, 5)
     L.dGeomDestroy(s1)
     L.dGeomDestroy(pl)
+
+
+def _dyn_frame(bodies, geoms, premature=1e-4):
+    F = Frame(S.World(bodies, geoms), dynamic=True)
+    A = F.A
+    p, fac, xd = A.dReal(premature), A.dReal(1.0), A.dReal(1e-2)
+    F.L.dWorldSetQuickStepDynamicIterationParameters(F.w, ctypes.byref(p), ctypes.byref(fac), ctypes.byref(xd))
+    F.collide()
+    assert F.L.dWorldQuickStep(F.w, 0.01) == 1
+    sweeps, res, rows = ctypes.c_int(0), ctypes.c_float(0), ctypes.c_int(0)
+    F.L.odxWorldGetSolveStats(F.w, ctypes.byref(sweeps), ctypes.byref(res), ctypes.byref(rows))
+    st = F.state()
+    F.L.dJointGroupEmpty(F.group)
+    F.close()
+    return sweeps.value, st
+
+
+def test_dynamic_iterations_exit_per_island_at_the_closed_form_sweep():
+    # one contact per sphere: the SOR iterate is scalar, lambda_k = L (1 - (1 - w)^k),
+    # and the sweep-k adjustment is L w |1 - w|^(k-1) / m (w = 1.3); with the
+    # premature delta 1e-4 a resting sphere stops after 11 sweeps and one falling
+    # at 1 m/s (L 11x larger) after 13; together each island stops on its own
+    say("case: dynamic iteration count per island")
+    m, r = 10.0, 0.5
+    I = 0.4 * m * r * r
+
+    def sphere(x, vz):
+        b = np.zeros(32)
+        b[0:3] = (x, 0, r)
+        b[4] = 1.0
+        b[10] = vz
+        b[24], b[25:28], b[31] = m, (I, I, I), 2
+        return b
+
+    plane = S.plane(0, 0, 1, 0)
+    n_rest, st_rest = _dyn_frame([sphere(-3, 0)], [S.geom(S.SPHERE, 0, (r,)), plane])
+    n_fall, st_fall = _dyn_frame([sphere(3, -1.0)], [S.geom(S.SPHERE, 0, (r,)), plane])
+    n_both, st_both = _dyn_frame([sphere(-3, 0), sphere(3, -1.0)],
+                                 [S.geom(S.SPHERE, 0, (r,)), S.geom(S.SPHERE, 1, (r,)), plane])
+    say("  sweeps", n_rest, n_fall, n_both)
+    assert (n_rest, n_fall, n_both) == (11, 13, 13)
+    np.testing.assert_allclose(st_both[0, :15], st_rest[0, :15], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(st_both[1, :15], st_fall[0, :15], rtol=0, atol=1e-6)

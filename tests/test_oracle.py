@@ -377,3 +377,84 @@ def test_island_order_hand_checked():
     # body 1: nodes j1 (tagged), j0->0 (0 already tagged); pop 0; body 0: all tagged
     assert bodies == [3, 2, 1, 0]
     assert joints == [2, 3, 1, 0]
+
+
+# ------------------------------------------- dynamic iteration count (A26)
+# dxQuickStepIsland's loop (quickstep.cpp:1820-1853) with the force adjustments
+# of IterationStep (:3001-3031) and CheckForMaximumToBeLessThanLimit... (:3253-3285).
+# For one contact on a resting (or falling) sphere the SOR iterate is scalar:
+# lambda_k = L (1 - (1 - w)^k) with L = rhs / (J iMJ + cfm), so the adjustment
+# of sweep k is |L w (1 - w)^(k-1)| * max|iMJ| with max|iMJ| = 1/m (the angular
+# part of iMJ is zero: the contact normal passes through the centre).
+DYN = dict(dyn_premature=1e-8, dyn_extra_factor=1.0, dyn_extra_delta=1e-2)  # dWorldCreate defaults
+
+
+def _sphere_world(vz=0.0, x=0.0, m=10.0, r=0.5):
+    return [body((x, 0, r), mass=m, I=(0.4 * m * r * r,) * 3, lvel=(0, 0, vz))], [S.geom(S.SPHERE, 0, (r,))]
+
+
+def _exit_sweep(L, w, m, limit, start=1):
+    k = start
+    while abs(L * w * (1 - w) ** (k - 1)) / m >= limit:
+        k += 1
+    return k
+
+
+def test_dynamic_iterations_premature_exit_closed_form():
+    m, h, cfm = 10.0, 0.01, 1e-5
+    bodies, geoms = _sphere_world(m=m)
+    w = S.World(bodies, geoms + [S.plane(0, 0, 1, 0)])
+    L = 9.81 / (1 / m + cfm / h)
+    for wsor, premature in ((1.3, 1e-8), (1.3, 1e-6), (1.0, 1e-8)):
+        p = O.params(iters=20, w=wsor, **dict(DYN, dyn_premature=premature))
+        b, c, lam = O.world_step(w.bodies, w.geoms, p, dtype=np.float64)
+        want = 2 if wsor == 1.0 else _exit_sweep(L, wsor, m, premature)
+        assert O.last_sweeps() == want, (wsor, premature, O.last_sweeps(), want)
+        assert lam[0] == pytest.approx(L * (1 - (1 - wsor) ** want), rel=1e-12)
+
+
+def test_dynamic_iterations_extra_sweeps_and_their_cap():
+    m, h, cfm, wsor = 10.0, 0.01, 1e-5, 1.3
+    bodies, geoms = _sphere_world(m=m)
+    w = S.World(bodies, geoms + [S.plane(0, 0, 1, 0)])
+    L = 9.81 / (1 / m + cfm / h)
+    # after `iters` sweeps the limit becomes the extra-iteration delta (1e-2);
+    # sweeps continue while some adjustment reaches it, at most iters * factor more
+    k_extra = _exit_sweep(L, wsor, m, 1e-2)
+    assert k_extra == 7
+    for factor, want in ((1.0, 4), (5.0, 7), (0.0, 2)):
+        p = O.params(iters=2, w=wsor, **dict(DYN, dyn_premature=0.0, dyn_extra_factor=factor))
+        O.world_step(w.bodies, w.geoms, p, dtype=np.float64)
+        assert O.last_sweeps() == want, (factor, O.last_sweeps())
+    # premature delta 0 never exits early; all zero is a fixed count
+    p = O.params(iters=20, w=wsor, **dict(DYN, dyn_premature=0.0))
+    O.world_step(w.bodies, w.geoms, p, dtype=np.float64)
+    assert O.last_sweeps() == 20
+    p = O.params(iters=20, w=wsor)
+    O.world_step(w.bodies, w.geoms, p, dtype=np.float64)
+    assert O.last_sweeps() == 20
+
+
+def test_dynamic_iterations_run_per_island():
+    # a resting and a falling sphere: two islands that stop at different sweeps;
+    # each ends exactly as it does alone (the count is the larger one)
+    m, wsor = 10.0, 1.3
+    b1, g1 = _sphere_world(m=m, x=-3.0)
+    b2, g2 = _sphere_world(m=m, x=3.0, vz=-1.0)
+    g2 = [gg.copy() for gg in g2]
+    g2[0][1] = 1
+    plane = [S.plane(0, 0, 1, 0)]
+    p = O.params(iters=20, w=wsor, **dict(DYN, dyn_premature=1e-6))
+    both = S.World(b1 + b2, g1 + g2 + plane)
+    bb, _, lam = O.world_step(both.bodies, both.geoms, p, dtype=np.float64)
+    n_both = O.last_sweeps()
+    a, _, la = O.world_step(np.array(b1), np.array(g1 + plane), p, dtype=np.float64)
+    n_a = O.last_sweeps()
+    g2s = [gg.copy() for gg in g2]
+    g2s[0][1] = 0
+    c, _, lc = O.world_step(np.array(b2), np.array(g2s + plane), p, dtype=np.float64)
+    n_c = O.last_sweeps()
+    assert n_a == 15 and n_c == 17 and n_both == 17
+    np.testing.assert_array_equal(bb[0], a[0])
+    np.testing.assert_array_equal(bb[1], c[0])
+    np.testing.assert_array_equal(lam, np.concatenate([la, lc]))
