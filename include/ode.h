@@ -11,8 +11,9 @@
  * the dynamic iteration count per island (premature exit 1e-8, up to
  * iterations x 1.0 extra sweeps while an adjustment is >= 1e-2; all three set
  * to 0 give a fixed count).
- * Differences: the solve is deterministic (no random row reordering); bodies
- * may carry sphere/box/capsule/plane geoms; joints are contact and ball.
+ * Differences: the solve is deterministic unless odxWorldSetRowOrder(w, 2)
+ * selects the reference's random reordering; bodies may carry
+ * sphere/box/capsule/plane geoms; joints are contact and ball.
  */
 #ifndef ODX_ODE_H
 #define ODX_ODE_H
@@ -214,7 +215,11 @@ int dRandInt(int n);                                         /* misc.h:51 */
 /* ------------------------------------------- libodx controls (no ODE twin) */
 /* Row order of dWorldQuickStep: 0 = the reference's order (findex rows last,
  * joints in island order, no random reordering; default), 1 = coloured contact
- * blocks.  Applies to subsequent steps of the world. */
+ * blocks, 2 = the reference's default build (REORDERING_METHOD__RANDOMLY,
+ * quickstep.cpp:54-85): the order of 0, shuffled per island with dRandInt
+ * before iterations 8, 16, ... (quickstep.cpp:2589-2611), drawing from the
+ * dRand state of dRandSetSeed island by island.  Applies to subsequent steps
+ * of the world. */
 void odxWorldSetRowOrder(dWorldID w, int order);
 /* Sweeps / LCP residual of the last dWorldQuickStep (residual < 0 when no rows). */
 void odxWorldGetSolveStats(dWorldID w, int *sweeps, float *residual, int *rows);

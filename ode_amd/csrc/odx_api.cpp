@@ -193,7 +193,7 @@ struct dxWorld {
     float last_res = -1.f;
     // device buffers of the step
     DevBuf pos, quat, lvel, avel, facc, tacc, mass, Ib, invIb, joints, surf, joff, rows, lambda, sched, phase, bscr,
-        counts, fb, bstart, residual, niters, scratch, island;
+        counts, fb, bstart, residual, niters, scratch, island, segtab, isl_sweeps;
 };
 
 // Extra sweeps of the dynamic iteration count are capped here (the reference
@@ -203,6 +203,24 @@ constexpr int ODX_MAX_EXTRA_SWEEPS = 100000;
 namespace {
 
 int g_seed = 0;  // dRand state (misc.cpp:35-48)
+
+// dRand (misc.cpp:35-48) and dRandInt (misc.cpp:78-130) on an explicit state
+uint32_t rand_next(uint32_t &seed) {
+    seed = (uint32_t)((1664525u * (uint64_t)seed + 1013904223u) & 0xffffffffu);
+    return seed;
+}
+int rand_int(uint32_t &seed, int n) {
+    uint32_t r = rand_next(seed), un = (uint32_t)n;
+    if (un <= 0x10u) {
+        r ^= (r >> 16); r ^= (r >> 8); r ^= (r >> 4);
+        if (un <= 0x2u) { r ^= (r >> 2); r ^= (r >> 1); return (int)(r & (un >> 1)); }
+        if (un <= 0x4u) { r ^= (r >> 2); return (int)(((r & 0x03u) * un) >> 2); }
+        return (int)(((r & 0x0Fu) * un) >> 4);
+    }
+    if (un <= 0x100u) { r ^= (r >> 16); r ^= (r >> 8); return (int)(((r & 0xFFu) * un) >> 8); }
+    if (un <= 0x10000u) { r ^= (r >> 16); return (int)(((r & 0xFFFFu) * un) >> 16); }
+    return (int)(((uint64_t)r * un) >> 32);
+}
 
 void add_node(dxBody *b, dxJointNode *n) {
     n->next = b->firstjoint;
@@ -370,7 +388,7 @@ void dWorldSetQuickStepW(dWorldID w, dReal over_relaxation) { w->w = over_relaxa
 dReal dWorldGetQuickStepW(dWorldID w) { return w->w; }
 void dWorldSetContactMaxCorrectingVel(dWorldID w, dReal vel) { w->max_vel = vel; }
 void dWorldSetContactSurfaceLayer(dWorldID w, dReal depth) { w->min_depth = depth; }
-void odxWorldSetRowOrder(dWorldID w, int order) { w->order = order ? 1 : 0; }
+void odxWorldSetRowOrder(dWorldID w, int order) { w->order = (order == 1 || order == 2) ? order : 0; }
 void odxWorldGetSolveStats(dWorldID w, int *sweeps, float *residual, int *rows) {
     if (sweeps) *sweeps = w->last_sweeps;
     if (residual) *residual = w->last_res;
@@ -973,7 +991,7 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     {
         std::vector<int> level;
         int L = 0;
-        if (w->order == 0) {
+        if (w->order == 0 || w->order == 2) {
             level.assign(m, 0);
             std::vector<int> last(nb, 0);
             for (int pass = 0; pass < 2; pass++)
@@ -1036,7 +1054,117 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
             }
         }
     }
-    const int L = (int)phase.size() - 1;
+    int L = (int)phase.size() - 1;
+
+    // ---- random reordering (order 2: REORDERING_METHOD__RANDOMLY, the
+    // reference's default build, quickstep.cpp:54-85, 1080-1109, 2589-2611).
+    // Each island's order starts as its ReorderPrep order and is shuffled
+    // (index 1..m-1 swapped with dRandInt(index + 1)) before sweeps 8, 16, ...;
+    // islands draw from dRand one after the other in island order.  Sweeps
+    // 8s..8s+7 use schedule segment s (dependency levels of the orders after s
+    // shuffles).  An island's draws start where the earlier islands' shuffles
+    // ended, so the schedules assume each island's shuffle count; with the
+    // dynamic iteration count the counts are known only after the solve, and
+    // the step is run again with the counts it reported until they agree.
+    const unsigned seed0 = (unsigned)g_seed;
+    const int n_iter_max = [&] {
+        const float xc = (float)w->iterations * w->dyn_extra_factor;
+        const int xm = xc < (float)ODX_MAX_EXTRA_SWEEPS ? (int)xc : ODX_MAX_EXTRA_SWEEPS;
+        const bool dyn = xm != 0 || w->dyn_premature != 0.f;
+        return w->iterations + (dyn ? xm : 0);
+    }();
+    const int nseg = w->order == 2 ? std::max(1, (n_iter_max - 1) / 8 + 1) : 0;
+    std::vector<std::vector<int>> isl_rows;  // island -> its rows in ReorderPrep order
+    if (w->order == 2) {
+        isl_rows.assign(nisl, {});
+        for (int pass = 0; pass < 2; pass++)
+            for (int r = 0; r < m; r++)
+                if (rfl[r] == pass) isl_rows[island[rb1[r]]].push_back(r);
+    }
+    std::vector<int> assumed(nisl, 0), seg_table;
+    for (int i = 0; i < (int)isl_rows.size(); i++)
+        if (!isl_rows[i].empty()) assumed[i] = std::min(std::max(0, (w->iterations - 1) / 8), nseg - 1);
+    std::vector<unsigned> seed_after;  // per island, state after each of its shuffles (nseg entries)
+    const auto pos0 = pos, quat0 = quat, lvel0 = lvel, avel0 = avel;
+    auto build_segments = [&]() {
+        std::vector<std::vector<std::vector<int>>> orders(nseg, std::vector<std::vector<int>>(nisl));
+        seed_after.assign((size_t)nisl * nseg, 0u);
+        uint32_t seed = seed0;
+        for (int i = 0; i < nisl; i++) {
+            std::vector<int> o = isl_rows[i];
+            orders[0][i] = o;
+            uint32_t si = seed;
+            seed_after[(size_t)i * nseg] = si;
+            for (int sg = 1; sg < nseg; sg++) {
+                for (size_t index = 1; index < o.size(); ++index) std::swap(o[index], o[rand_int(si, (int)index + 1)]);
+                orders[sg][i] = o;
+                seed_after[(size_t)i * nseg + sg] = si;
+            }
+            seed = seed_after[(size_t)i * nseg + std::min(assumed[i], nseg - 1)];
+        }
+        sched.clear();
+        phase.clear();
+        seg_table.clear();
+        std::vector<int> lvl(m), last(nb);
+        for (int sg = 0; sg < nseg; sg++) {
+            std::fill(last.begin(), last.end(), 0);
+            int Ls = 0;
+            for (int i = 0; i < nisl; i++)
+                for (int r : orders[sg][i]) {
+                    int lv = last[rb1[r]];
+                    if (rb2[r] >= 0) lv = std::max(lv, last[rb2[r]]);
+                    lv++;
+                    last[rb1[r]] = lv;
+                    if (rb2[r] >= 0) last[rb2[r]] = lv;
+                    lvl[r] = lv - 1;
+                    Ls = std::max(Ls, lv);
+                }
+            std::vector<int> cnt(Ls + 1, 0);
+            for (int r = 0; r < m; r++) cnt[lvl[r] + 1]++;
+            for (int q = 1; q <= Ls; q++) cnt[q] += cnt[q - 1];
+            seg_table.push_back((int)sched.size());
+            seg_table.push_back((int)phase.size());
+            seg_table.push_back(Ls);
+            const int base = (int)sched.size();
+            for (int q = 0; q <= Ls; q++) phase.push_back(cnt[q]);  // relative to the segment's first item
+            sched.resize(base + m);
+            for (int i = 0; i < nisl; i++)
+                for (int r : orders[sg][i]) sched[base + cnt[lvl[r]]++] = (r << 3) | 1;
+        }
+        L = seg_table[2];
+    };
+    std::vector<float> fbh;
+    int it = 0;
+    float res = -1.f;
+    bool want_fb = false;
+    for (dxJoint *j : joints) want_fb = want_fb || j->feedback;
+    std::vector<int> isl_sweeps(std::max(nisl, 1));
+    for (int attempt = 0;; attempt++) {
+    if (w->order == 2) {
+        build_segments();
+        pos = pos0; quat = quat0; lvel = lvel0; avel = avel0;
+    }
+
+    // ---- every index the kernel will follow stays inside its array: items
+    // address rows [r, r + n) < m, phase starts rise within the segment's items
+    {
+        auto bad = [&](const char *what) {
+            odx_message(4, "dWorldQuickStep: malformed schedule (%s)", what);
+            return 0;
+        };
+        std::vector<int> segs = w->order == 2 ? seg_table : std::vector<int>{0, 0, L};
+        for (size_t q = 0; q + 2 < segs.size(); q += 3) {
+            const int so = segs[q], po = segs[q + 1], Ls = segs[q + 2];
+            if (so < 0 || po < 0 || Ls < 0 || (size_t)po + Ls >= phase.size()) return bad("segment table");
+            for (int k = 0; k < Ls; k++)
+                if (phase[po + k] > phase[po + k + 1]) return bad("phase order");
+            if (phase[po] < 0 || (size_t)so + phase[po + Ls] > sched.size()) return bad("phase range");
+            for (int k = phase[po]; k < phase[po + Ls]; k++) {
+                const int item = sched[so + k], r = item >> 3, n = item & 7;
+                if (r < 0 || n < 1 || r + n > m) return bad("item rows");
+            }
+        }
+    }
 
     // ---- device
     int bstart[2] = {0, nb};
@@ -1088,8 +1216,6 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     P.order = w->order;
     P.lambda_floor = 1e-3f;
     P.collide = 0;
-    bool want_fb = false;
-    for (dxJoint *j : joints) want_fb = want_fb || j->feedback;
     P.feedback = want_fb ? 1 : 0;
     // dynamic iteration count (objects.h:177-189): extra sweeps = iterations x
     // factor; on when that is non-zero or the premature delta is
@@ -1104,6 +1230,14 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
         if (P.dyn && !(up(w->island, island.data(), island.size()))) return 0;
         D.body_island = P.dyn ? w->island.as<int>() : nullptr;
     }
+    if (w->order == 2) {
+        if (!up(w->segtab, seg_table.data(), seg_table.size()) ||
+            !w->isl_sweeps.ensure((size_t)std::max(nisl, 1) * sizeof(int)))
+            return 0;
+        D.seg_table = w->segtab.as<int>();
+        D.nseg = nseg;
+        D.island_sweeps = P.dyn ? w->isl_sweeps.as<int>() : nullptr;
+    }
     if (g_ktime.on && !g_ktime.ev[0]) {
         if (!odx_check(hipEventCreate(&g_ktime.ev[0]), "hipEventCreate") ||
             !odx_check(hipEventCreate(&g_ktime.ev[1]), "hipEventCreate"))
@@ -1113,14 +1247,31 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     launch_single_step(D, P, nb, m, (int)sched.size(), L, g_stream);
     if (!odx_check(hipGetLastError(), "k_single_step")) return 0;
     if (g_ktime.on && !odx_check(hipEventRecord(g_ktime.ev[1], g_stream), "hipEventRecord")) return 0;
-    std::vector<float> fbh(want_fb ? (size_t)nj * 12 : 0);
-    int it = 0;
-    float res = -1.f;
+    fbh.assign(want_fb ? (size_t)nj * 12 : 0, 0.f);
     ok = down(pos.data(), w->pos, nb) && down(quat.data(), w->quat, nb) && down(lvel.data(), w->lvel, nb) &&
          down(avel.data(), w->avel, nb) && down(&it, w->niters, 1) && down(&res, w->residual, 1) &&
          (!want_fb || down(fbh.data(), w->fb, fbh.size())) &&
+         (!(w->order == 2 && P.dyn) || down(isl_sweeps.data(), w->isl_sweeps, nisl)) &&
          odx_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize");
     if (!ok) return 0;
+    if (w->order != 2) break;
+    // shuffles each island ran: before sweeps 8, 16, ... that it reached
+    bool agree = true;
+    for (int i = 0; i < nisl; i++) {
+        const int sw = P.dyn ? isl_sweeps[i] : w->iterations;
+        const int ran = isl_rows[i].empty() ? 0 : std::min(std::max(0, (sw - 1) / 8), nseg - 1);
+        if (ran != assumed[i]) agree = false;
+        assumed[i] = ran;
+    }
+    if (agree) {
+        g_seed = (int)(nisl > 0 ? seed_after[(size_t)(nisl - 1) * nseg + assumed[nisl - 1]] : seed0);
+        break;
+    }
+    if (attempt > nisl + 1) {
+        odx_message(4, "dWorldQuickStep: the random reordering's shuffle counts did not settle");
+        return 0;
+    }
+    }  // attempts
     for (int i = 0; i < nb; i++) {
         dxBody *b = bodies[i];
         b->pos[0] = pos[i].x; b->pos[1] = pos[i].y; b->pos[2] = pos[i].z;
@@ -1171,23 +1322,18 @@ int odxApiGetKernelTiming(double *stats) {
 
 // ------------------------------------------------------------------ misc
 unsigned long dRand(void) {
-    g_seed = (int)((1664525u * (uint32_t)g_seed + 1013904223u) & 0xffffffffu);
-    return (unsigned long)(uint32_t)g_seed;
+    uint32_t sd = (uint32_t)g_seed;
+    rand_next(sd);
+    g_seed = (int)sd;
+    return (unsigned long)sd;
 }
 unsigned long dRandGetSeed(void) { return (unsigned long)(uint32_t)g_seed; }
 void dRandSetSeed(unsigned long s) { g_seed = (int)(uint32_t)s; }
 int dRandInt(int n) {
-    // misc.cpp:78-130
-    uint32_t r = (uint32_t)dRand(), un = (uint32_t)n;
-    if (un <= 0x10u) {
-        r ^= (r >> 16); r ^= (r >> 8); r ^= (r >> 4);
-        if (un <= 0x2u) { r ^= (r >> 2); r ^= (r >> 1); return (int)(r & (un >> 1)); }
-        if (un <= 0x4u) { r ^= (r >> 2); return (int)(((r & 0x03u) * un) >> 2); }
-        return (int)(((r & 0x0Fu) * un) >> 4);
-    }
-    if (un <= 0x100u) { r ^= (r >> 16); r ^= (r >> 8); return (int)(((r & 0xFFu) * un) >> 8); }
-    if (un <= 0x10000u) { r ^= (r >> 16); return (int)(((r & 0xFFFFu) * un) >> 16); }
-    return (int)(((uint64_t)r * un) >> 32);
+    uint32_t sd = (uint32_t)g_seed;
+    const int r = rand_int(sd, n);
+    g_seed = (int)sd;
+    return r;
 }
 
 }  // extern "C"

@@ -128,6 +128,12 @@ struct DevWorlds {
     long long *wtime;       // [W][2] solve start / end of each world (device wall clock), may be null
     long long *ptime;       // [W][PTIME_SLOTS] prepare stage boundaries (device wall clock), may be null
     const int *body_island; // [NB] island of each body (drop-in world with the dynamic iteration count), may be null
+    // drop-in world with random reordering: sweeps 8s .. 8s+7 use segment s of
+    // the schedule (seg_table[3s] = first item, [3s+1] = first phase start,
+    // [3s+2] = phases; the last segment serves any later sweep)
+    const int *seg_table;   // [3 * nseg], may be null
+    int nseg;
+    int *island_sweeps;     // [nislands] sweeps each island ran, may be null
 };
 constexpr int PTIME_SLOTS = 6;
 

@@ -1666,8 +1666,24 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
     __syncthreads();
     int it = 0;
     float res = -1.f;
-    // one sweep over the phases (items of stopped islands skipped when fa is set)
+    // one sweep over the phases (items of stopped islands skipped when fa is
+    // set); with a segmented schedule (random reordering) sweep nsw uses
+    // segment nsw / 8
+    int nsw = 0;
     auto sweep = [&]() {
+            if (!STAGED && D.seg_table) {
+                const int sg = min(nsw / 8, D.nseg - 1);
+                const int *sch = D.sched + D.seg_table[3 * sg], *phs = D.phase_start + D.seg_table[3 * sg + 1];
+                const int Ls = D.seg_table[3 * sg + 2];
+                for (int p = 0; p < Ls; p++) {
+                    const int s = phs[p], e = phs[p + 1];
+                    for (int k = s + t; k < e; k += NT) process_item_l(D, nullptr, sch[k], lam, s_fc, fa, isl_b, ist);
+                    __syncthreads();
+                }
+                nsw++;
+                return;
+            }
+            nsw++;
             if (STAGED) {
                 int s = ph[0], e = ph[1];
                 int cur = s + t < e ? sched[s + t] : -1;
@@ -1755,7 +1771,10 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
             any = __syncthreads_or(mine);
         }
         int mx = 0;
-        for (int i = t; i < nisl; i += NT) mx = max(mx, ist[i * 4]);
+        for (int i = t; i < nisl; i += NT) {
+            mx = max(mx, ist[i * 4]);
+            if (D.island_sweeps) D.island_sweeps[i] = ist[i * 4];
+        }
         it = (int)block_max((float)mx, s_red);
     } else if (m > 0) {
         while (it < P.max_iters) {
@@ -1937,7 +1956,7 @@ void launch_single_step(const DevWorlds &D, const StepParams &P, int nb, int m, 
     }
     const size_t dynw = P.dyn ? (size_t)nb * 3 + (size_t)P.nislands * 5 : 0;  // fa, island of body, island state
     const size_t staged = ((size_t)nb * 6 + m + nitems + nphases + 1 + dynw) * sizeof(float);
-    if (staged + 256 <= (size_t)lds_max &&
+    if (!D.seg_table && staged + 256 <= (size_t)lds_max &&
         hipFuncSetAttribute((const void *)k_single_step_t<true, 512>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)staged) == hipSuccess) {
         hipLaunchKernelGGL((k_single_step_t<true, 512>), dim3(1), dim3(512), staged, s, D, P);

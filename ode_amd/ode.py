@@ -91,6 +91,7 @@ def api():
         "dJointSetFeedback": (None, [vp, P(dJointFeedback)]), "dJointGetType": (i, [vp]),
         "dAreConnectedExcluding": (i, [vp, vp, i]),
         "dRand": (ctypes.c_ulong, []), "dRandSetSeed": (None, [ctypes.c_ulong]), "dRandInt": (i, [i]),
+        "dRandGetSeed": (ctypes.c_ulong, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -22,7 +22,7 @@ JOINT = 16
 P_H, P_GX, P_GY, P_GZ, P_ERP, P_CFM, P_SOR_W, P_ITERS, P_MAX_VEL, P_MIN_DEPTH = range(10)
 P_MODE, P_MU, P_MU2, P_BOUNCE, P_BOUNCE_VEL, P_SOFT_ERP, P_SOFT_CFM, P_MAXC, P_SPACE = range(10, 19)
 P_MOTION1, P_MOTION2, P_MOTIONN, P_SLIP1, P_SLIP2, P_SKIP_CONNECTED, P_LAMBDA_FLOOR, P_TOL, P_CHECK_EVERY = range(19, 28)
-P_DYN_PREMATURE, P_DYN_EXTRA_FACTOR, P_DYN_EXTRA_DELTA = range(28, 31)
+P_DYN_PREMATURE, P_DYN_EXTRA_FACTOR, P_DYN_EXTRA_DELTA, P_RANDOM_REORDER = range(28, 32)
 
 
 def build():
@@ -75,6 +75,7 @@ def params(**kw):
         "space": P_SPACE, "skip_connected": P_SKIP_CONNECTED, "lambda_floor": P_LAMBDA_FLOOR,
         "tol": P_TOL, "check_every": P_CHECK_EVERY,
         "dyn_premature": P_DYN_PREMATURE, "dyn_extra_factor": P_DYN_EXTRA_FACTOR, "dyn_extra_delta": P_DYN_EXTRA_DELTA,
+        "random_reorder": P_RANDOM_REORDER,
     }
     for k, v in kw.items():
         if k == "gravity":
@@ -86,6 +87,15 @@ def params(**kw):
 
 def last_sweeps():
     return lib().orc_last_sweeps()
+
+
+def rand_seed(seed=None):
+    """The oracle's dRand state used by the random reordering: sets it when
+    seed is given; returns the current value."""
+    if seed is not None:
+        lib().orc_rand_set_seed(ctypes.c_uint(seed))
+    lib().orc_rand_get_seed.restype = ctypes.c_uint
+    return int(lib().orc_rand_get_seed())
 
 
 def drand_seq(seed, n):

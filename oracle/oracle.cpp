@@ -185,6 +185,8 @@ template <class R> bool invertPD3(R *out /*3x4*/, const R *I /*3x4*/) {
 
 const double kInf = std::numeric_limits<double>::infinity();
 int g_last_sweeps = 0;
+unsigned int g_rand_seed = 0;  /* dRand's global state (misc.cpp:35) */
+extern "C" int orc_drand_int(unsigned int *seed, int n);
 
 /* ------------------------------------------------------------- records -- */
 enum { CLS_SPHERE = 0, CLS_BOX = 1, CLS_CAPSULE = 2, CLS_PLANE = 4 };
@@ -1121,6 +1123,7 @@ template <class R> struct Params {
     bool dyn;
     unsigned max_extra;
     R premature_delta, extra_delta;
+    bool random_reorder;  /* REORDERING_METHOD__RANDOMLY, frequency 8 (quickstep.cpp:54-85) */
 };
 
 template <class R> Params<R> read_params(const double *P) {
@@ -1148,6 +1151,7 @@ template <class R> Params<R> read_params(const double *P) {
     p.premature_delta = (R)(P[ORC_P_DYN_PREMATURE] > 0 ? P[ORC_P_DYN_PREMATURE] : 0.0);
     p.extra_delta = (R)(P[ORC_P_DYN_EXTRA_DELTA] > 0 ? P[ORC_P_DYN_EXTRA_DELTA] : 0.0);
     p.dyn = p.max_extra != 0 || p.premature_delta != 0;
+    p.random_reorder = P[ORC_P_RANDOM_REORDER] != 0;
     return p;
 }
 
@@ -1471,11 +1475,16 @@ template <class R> int quickstep(int nb, R *bodies, std::vector<Joint<R>> &joint
     assemble(nb, bodies, joints, p, S, invI);
     std::vector<R> lambda(S.m, 0), fc((size_t)nb * 6, 0);
     int sweeps = 0;
-    if (S.m > 0 && p.dyn && !(p.tol > 0)) {
+    if (S.m > 0 && (p.dyn || p.random_reorder) && !(p.tol > 0)) {
         /* the iteration loop of dxQuickStepIsland (quickstep.cpp:1820-1853),
-         * run per island with its own count: premature exit once every body's
-         * adjustment is below the premature delta; after `iters`, up to
-         * max_extra more while some adjustment is at least extra_delta */
+         * island by island (dxProcessIslands util.cpp:886-938) with its own
+         * count: random reordering of the island's whole order at iterations
+         * 8, 16, ... (IsSORConstraintsReorderRequiredForIteration :1080-1109,
+         * ConstraintsShuffling :2589-2611: for index 1..m-1 swap with
+         * dRandInt(index + 1)); with the dynamic count, a premature exit once
+         * every body's adjustment is below the premature delta, and after
+         * `iters` up to max_extra more while some adjustment is at least
+         * extra_delta */
         std::vector<std::vector<int>> ib;
         const std::vector<int> isl = row_islands(S, ib);
         std::vector<R> fa((size_t)nb * 2, 0);
@@ -1486,8 +1495,18 @@ template <class R> int quickstep(int nb, R *bodies, std::vector<Joint<R>> &joint
             R limit = p.premature_delta;
             unsigned iteration = 0, extra = 0;
             for (;;) {
-                for (int r : rows) sor_row(S, r, lambda, fc, &fa);
+                if (p.random_reorder && iteration >= 8 && iteration % 8 == 0) {
+                    for (size_t index = 1; index < rows.size(); ++index) {
+                        const int swap_index = orc_drand_int(&g_rand_seed, (int)index + 1);
+                        std::swap(rows[index], rows[swap_index]);
+                    }
+                }
+                for (int r : rows) sor_row(S, r, lambda, fc, p.dyn ? &fa : nullptr);
                 ++iteration;
+                if (!p.dyn) {
+                    if (iteration == (unsigned)p.iters) break;
+                    continue;
+                }
                 if (iteration - extra == (unsigned)p.iters) {
                     if (extra != 0 || p.max_extra == 0) break;
                     extra = p.max_extra;
@@ -1633,6 +1652,9 @@ unsigned int orc_drand(unsigned int *seed) {
     *seed = (unsigned int)((1664525u * (uint64_t)*seed + 1013904223u) & 0xffffffffu);
     return *seed;
 }
+
+void orc_rand_set_seed(unsigned int seed) { g_rand_seed = seed; }
+unsigned int orc_rand_get_seed(void) { return g_rand_seed; }
 
 int orc_drand_int(unsigned int *seed, int n) {
     uint32_t r = orc_drand(seed), un = (uint32_t)n;

@@ -48,12 +48,19 @@ enum {
      * requirement delta.  Enabled when factor * iterations >= 1 or the
      * premature delta is non-zero (all zero: a fixed iteration count). */
     ORC_P_DYN_PREMATURE, ORC_P_DYN_EXTRA_FACTOR, ORC_P_DYN_EXTRA_DELTA,
+    /* 1: random constraint reordering every 8 iterations (quickstep.cpp:72,
+     * the reference's default build), drawing from the oracle's dRand state */
+    ORC_P_RANDOM_REORDER,
     ORC_P__MAX = 32
 };
 
 /* dRand LCG (misc.cpp:35-48) and dRandInt (misc.cpp:78-130). */
 unsigned int orc_drand(unsigned int *seed);
 int orc_drand_int(unsigned int *seed, int n);
+/* The dRand state the QuickStep's random reordering draws from (the
+ * reference's global seed; dRandSetSeed / dRandGetSeed). */
+void orc_rand_set_seed(unsigned int seed);
+unsigned int orc_rand_get_seed(void);
 
 /* dCollide semantics for one geom pair; returns contact count. */
 int orc_collide_f32(const float *g1, const float *g2, const float *b1, const float *b2,

@@ -290,3 +290,47 @@ def test_dynamic_iterations_exit_per_island_at_the_closed_form_sweep():
     assert (n_rest, n_fall, n_both) == (11, 13, 13)
     np.testing.assert_allclose(st_both[0, :15], st_rest[0, :15], rtol=0, atol=1e-6)
     np.testing.assert_allclose(st_both[1, :15], st_fall[0, :15], rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("dynamic", [False, True])
+def test_random_reordering_matches_oracle_and_draws_as_the_reference(dynamic):
+    # row order 2: the reference's default build; the same dRand seed in the
+    # library and the oracle gives the same shuffles, iterate and final seed
+    say("case: random reordering, dynamic", dynamic)
+    scene = settle_scene(3, 16)
+    F = Frame(scene, dynamic=dynamic)
+    F.L.odxWorldSetRowOrder(F.w, 2)
+    F.collide()
+    before = scene.bodies.copy()
+    F.L.dRandSetSeed(4242)
+    assert F.L.dWorldQuickStep(F.w, 0.01) == 1
+    seed_gpu = int(F.L.dRandGetSeed())
+    got = F.state()
+    sweeps, res, rows = ctypes.c_int(0), ctypes.c_float(0), ctypes.c_int(0)
+    F.L.odxWorldGetSolveStats(F.w, ctypes.byref(sweeps), ctypes.byref(res), ctypes.byref(rows))
+    recs = []
+    for (i1, i2, pos, nrm, depth, g1, g2) in F.contacts:
+        r = np.zeros(16)
+        r[1], r[2] = i1, i2
+        r[3:6], r[6:9], r[9] = pos, nrm, depth
+        recs.append(r)
+    norm_pairs = [(a, b) if a >= 0 else (b, -1) for a, b in [(c[0], c[1]) for c in F.contacts]]
+    _, order = island_order(len(scene.bodies), norm_pairs)
+    jr = np.array([recs[k] for k in order]).reshape(-1, 16)
+    prm = O.params(random_reorder=1, **(dict(dyn_premature=1e-8, dyn_extra_factor=1.0, dyn_extra_delta=1e-2)
+                                        if dynamic else {}))
+    O.rand_seed(4242)
+    ref_b, ref_l = O.quickstep(before, jr, prm, dtype=np.float32)
+    seed_ref = O.rand_seed()
+    n_ref = O.last_sweeps()
+    O.rand_seed(4242)
+    ref64, _ = O.quickstep(before, jr, prm, dtype=np.float64)
+    err_x = np.max(np.abs(got[:, 0:3] - ref64[:, 0:3])) / np.max(np.abs(ref64[:, 0:3]))
+    err_v = np.max(np.abs(got[:, 8:15] - ref64[:, 8:15])) / max(np.max(np.abs(ref64[:, 8:15])), 1.0)
+    say("  contacts", len(recs), "sweeps", sweeps.value, n_ref, "seeds", seed_gpu, seed_ref, "err", err_x, err_v)
+    assert sweeps.value == n_ref
+    assert seed_gpu == seed_ref
+    assert seed_ref != 4242 or sweeps.value <= 8
+    assert err_x < 1e-4 and err_v < 1e-4
+    F.L.dJointGroupEmpty(F.group)
+    F.close()

@@ -458,3 +458,87 @@ def test_dynamic_iterations_run_per_island():
     np.testing.assert_array_equal(bb[0], a[0])
     np.testing.assert_array_equal(bb[1], c[0])
     np.testing.assert_array_equal(lam, np.concatenate([la, lc]))
+
+
+# ------------------------------------------ random constraint reordering (A24)
+# REORDERING_METHOD__RANDOMLY (quickstep.cpp:54-85, the reference's default
+# build): before iterations 8, 16, ... the island's whole order is shuffled with
+# dRandInt(index + 1) for index = 1..m-1, so one step draws
+# floor((sweeps - 1) / 8) * (m - 1) values from dRand, island by island.
+def _pile(seed=11, n=12):
+    rng = np.random.default_rng(seed)
+    w = S.uneven_world(rng, n, n)
+    w.bodies[:, 2] = np.minimum(w.bodies[:, 2], 0.3)
+    return w
+
+
+def _lcg_advance(seed, n):
+    for _ in range(n):
+        seed = (1664525 * seed + 1013904223) & 0xffffffff
+    return seed
+
+
+@pytest.mark.parametrize("iters", [8, 9, 16, 17, 20, 25])
+def test_random_reordering_draws_per_iteration_schedule(iters):
+    w = _pile()
+    p = O.params(iters=iters, random_reorder=1)
+    O.rand_seed(12345)
+    b, c, lam = O.world_step(w.bodies, w.geoms, p, dtype=np.float64)
+    j = O.contacts_to_joints(c, w.geoms)
+    # islands of the contact graph (dynamic bodies joined by contacts)
+    parent = list(range(len(w.bodies)))
+
+    def find(x):
+        while parent[x] != x:
+            parent[x] = parent[parent[x]]
+            x = parent[x]
+        return x
+    pairs = [(int(a), int(bb)) if a >= 0 else (int(bb), -1) for a, bb in j[:, 1:3]]
+    for a, bb in pairs:
+        if bb >= 0:
+            parent[find(a)] = find(bb)
+    rows = {}
+    for a, _ in pairs:
+        rows[find(a)] = rows.get(find(a), 0) + 3
+    draws = sum((iters - 1) // 8 * (m - 1) for m in rows.values())
+    assert O.rand_seed() == _lcg_advance(12345, draws)
+    assert O.last_sweeps() == iters
+
+
+def test_random_reordering_keeps_the_frictionless_solution():
+    # a frictionless resting pile has a unique LCP solution: any Gauss-Seidel
+    # order converges to it, shuffled or not
+    w = _pile(seed=5, n=10)
+    base = O.params(iters=400, mu=0.0, w=1.0)
+    _, _, lam0 = O.world_step(w.bodies, w.geoms, base, dtype=np.float64)
+    O.rand_seed(7)
+    _, _, lam1 = O.world_step(w.bodies, w.geoms, O.params(iters=400, mu=0.0, w=1.0, random_reorder=1),
+                              dtype=np.float64)
+    assert O.rand_seed() != 7
+    np.testing.assert_allclose(lam1, lam0, rtol=1e-6, atol=1e-6 * np.max(np.abs(lam0)))
+
+
+def test_random_reordering_with_dynamic_count_draws_for_the_sweeps_run():
+    w = _pile(seed=3, n=6)
+    p = O.params(iters=20, random_reorder=1, **DYN)
+    O.rand_seed(99)
+    b, c, lam = O.world_step(w.bodies, w.geoms, p, dtype=np.float64)
+    j = O.contacts_to_joints(c, w.geoms)
+    pairs = [(int(a), int(bb)) if a >= 0 else (int(bb), -1) for a, bb in j[:, 1:3]]
+    bodies_with_rows = {a for a, _ in pairs} | {bb for _, bb in pairs if bb >= 0}
+    # one island here (the pile is connected): draws follow that island's sweeps
+    parent = {x: x for x in bodies_with_rows}
+
+    def find(x):
+        while parent[x] != x:
+            x = parent[x]
+        return x
+    for a, bb in pairs:
+        if bb >= 0:
+            parent[find(a)] = find(bb)
+    roots = {find(x) for x in bodies_with_rows}
+    if len(roots) != 1:
+        pytest.skip("scene is not one island")
+    n = O.last_sweeps()
+    assert 1 <= n <= 40
+    assert O.rand_seed() == _lcg_advance(99, (n - 1) // 8 * (3 * len(pairs) - 1))
