@@ -116,6 +116,32 @@ void dWorldSetQuickStepW(dWorldID w, dReal over_relaxation);   /* objects.h:584 
 dReal dWorldGetQuickStepW(dWorldID w);                         /* objects.h:591 */
 void dWorldSetContactMaxCorrectingVel(dWorldID w, dReal vel);  /* objects.h:603 */
 void dWorldSetContactSurfaceLayer(dWorldID w, dReal depth);    /* objects.h:623 */
+/* Defaults that bodies take at dBodyCreate: auto-disable (objects.h:670-748),
+ * damping and the angular speed cap (objects.h:790-865).  Thresholds are
+ * speeds; the world keeps them squared, as the reference does. */
+dReal dWorldGetAutoDisableLinearThreshold(dWorldID w);                        /* objects.h:670 */
+void dWorldSetAutoDisableLinearThreshold(dWorldID w, dReal threshold);        /* objects.h:677 */
+dReal dWorldGetAutoDisableAngularThreshold(dWorldID w);                       /* objects.h:684 */
+void dWorldSetAutoDisableAngularThreshold(dWorldID w, dReal threshold);       /* objects.h:691 */
+int dWorldGetAutoDisableAverageSamplesCount(dWorldID w);                      /* objects.h:698 */
+void dWorldSetAutoDisableAverageSamplesCount(dWorldID w, unsigned int count); /* objects.h:706 */
+int dWorldGetAutoDisableSteps(dWorldID w);                                    /* objects.h:713 */
+void dWorldSetAutoDisableSteps(dWorldID w, int steps);                        /* objects.h:720 */
+dReal dWorldGetAutoDisableTime(dWorldID w);                                   /* objects.h:727 */
+void dWorldSetAutoDisableTime(dWorldID w, dReal time);                        /* objects.h:734 */
+int dWorldGetAutoDisableFlag(dWorldID w);                                     /* objects.h:741 */
+void dWorldSetAutoDisableFlag(dWorldID w, int do_auto_disable);               /* objects.h:748 */
+dReal dWorldGetLinearDampingThreshold(dWorldID w);                            /* objects.h:790 */
+void dWorldSetLinearDampingThreshold(dWorldID w, dReal threshold);            /* objects.h:798 */
+dReal dWorldGetAngularDampingThreshold(dWorldID w);                           /* objects.h:804 */
+void dWorldSetAngularDampingThreshold(dWorldID w, dReal threshold);           /* objects.h:812 */
+dReal dWorldGetLinearDamping(dWorldID w);                                     /* objects.h:818 */
+void dWorldSetLinearDamping(dWorldID w, dReal scale);                         /* objects.h:826 */
+dReal dWorldGetAngularDamping(dWorldID w);                                    /* objects.h:832 */
+void dWorldSetAngularDamping(dWorldID w, dReal scale);                        /* objects.h:840 */
+void dWorldSetDamping(dWorldID w, dReal linear_scale, dReal angular_scale);   /* objects.h:848 */
+dReal dWorldGetMaxAngularSpeed(dWorldID w);                                   /* objects.h:857 */
+void dWorldSetMaxAngularSpeed(dWorldID w, dReal max_speed);                   /* objects.h:865 */
 
 /* ---------------------------------------------------------------- body */
 dBodyID dBodyCreate(dWorldID w);                               /* objects.h:1011 */
@@ -141,6 +167,39 @@ void dBodyDisable(dBodyID b);                                  /* objects.h:1433
 int dBodyIsEnabled(dBodyID b);                                 /* objects.h:1440 */
 void dBodySetGravityMode(dBodyID b, int mode);                 /* objects.h:1449 */
 void dBodySetGyroscopicMode(dBodyID b, int enabled);           /* objects.h:1613 */
+/* Auto-disable (objects.h:908-994): dWorldQuickStep first samples the
+ * velocities of each enabled auto-disabling body that has joints; a body idle
+ * for the step and time budgets is disabled with zero velocities and left out
+ * of the islands until a joint to an enabled body re-enables it. */
+dReal dBodyGetAutoDisableLinearThreshold(dBodyID b);                          /* objects.h:908 */
+void dBodySetAutoDisableLinearThreshold(dBodyID b, dReal threshold);          /* objects.h:915 */
+dReal dBodyGetAutoDisableAngularThreshold(dBodyID b);                         /* objects.h:922 */
+void dBodySetAutoDisableAngularThreshold(dBodyID b, dReal threshold);         /* objects.h:929 */
+int dBodyGetAutoDisableAverageSamplesCount(dBodyID b);                        /* objects.h:936 */
+void dBodySetAutoDisableAverageSamplesCount(dBodyID b, unsigned int count);   /* objects.h:943 */
+int dBodyGetAutoDisableSteps(dBodyID b);                                      /* objects.h:951 */
+void dBodySetAutoDisableSteps(dBodyID b, int steps);                          /* objects.h:958 */
+dReal dBodyGetAutoDisableTime(dBodyID b);                                     /* objects.h:965 */
+void dBodySetAutoDisableTime(dBodyID b, dReal time);                          /* objects.h:972 */
+int dBodyGetAutoDisableFlag(dBodyID b);                                       /* objects.h:979 */
+void dBodySetAutoDisableFlag(dBodyID b, int do_auto_disable);                 /* objects.h:986 */
+void dBodySetAutoDisableDefaults(dBodyID b);                                  /* objects.h:994 */
+/* Damping and the angular speed cap, applied by the kernel's integration
+ * (dxStepBody, ode/src/util.cpp:583-699): the cap before the move, damping
+ * (velocity times 1 - scale when the squared speed exceeds the squared
+ * threshold) after it. */
+void dBodySetDampingDefaults(dBodyID b);                                      /* objects.h:1500 */
+dReal dBodyGetLinearDamping(dBodyID b);                                       /* objects.h:1506 */
+void dBodySetLinearDamping(dBodyID b, dReal scale);                           /* objects.h:1516 */
+dReal dBodyGetAngularDamping(dBodyID b);                                      /* objects.h:1524 */
+void dBodySetAngularDamping(dBodyID b, dReal scale);                          /* objects.h:1534 */
+void dBodySetDamping(dBodyID b, dReal linear_scale, dReal angular_scale);     /* objects.h:1543 */
+dReal dBodyGetLinearDampingThreshold(dBodyID b);                              /* objects.h:1549 */
+void dBodySetLinearDampingThreshold(dBodyID b, dReal threshold);              /* objects.h:1557 */
+dReal dBodyGetAngularDampingThreshold(dBodyID b);                             /* objects.h:1563 */
+void dBodySetAngularDampingThreshold(dBodyID b, dReal threshold);             /* objects.h:1571 */
+dReal dBodyGetMaxAngularSpeed(dBodyID b);                                     /* objects.h:1578 */
+void dBodySetMaxAngularSpeed(dBodyID b, dReal max_speed);                     /* objects.h:1588 */
 
 /* --------------------------------------------------------------- joints */
 dJointID dJointCreateBall(dWorldID w, dJointGroupID g);                    /* objects.h:1693 */

@@ -22,7 +22,13 @@
  * Record layouts (float arrays):
  *   BODY  [32]: 0-2 pos, 4-7 q (w,x,y,z), 8-10 lvel, 12-14 avel, 16-18 facc,
  *               20-22 tacc, 24 mass, 25-30 I (xx,yy,zz,xy,xz,yz),
- *               31 flags (bit0 no gravity, bit1 gyroscopic)
+ *               31 flags (bit0 no gravity, bit1 gyroscopic, bit2 linear
+ *               damping, bit3 angular damping, bit4 angular speed cap);
+ *               3 linear damping scale, 11 angular damping scale, 15 linear
+ *               damping threshold, 19 angular damping threshold, 23 maximum
+ *               angular speed (dBodySetDamping*, dBodySetMaxAngularSpeed,
+ *               include/ode/objects.h:1500-1588; applied by dxStepBody,
+ *               ode/src/util.cpp:583-699)
  *   GEOM  [16]: 0 class (0 sphere, 1 box, 2 capsule, 4 plane), 1 body (world-
  *               local index, -1 static), 2-5 params (sphere r | box lx,ly,lz |
  *               capsule r,len | plane a,b,c,d), 6-8 static pos, 9-12 static q

@@ -123,6 +123,22 @@ struct dxJointNode {
     dxJointNode *next;
 };
 
+// Auto-disable parameters (dxAutoDisable, ode/src/objects.h:87-96; defaults
+// objects.cpp:54-61); thresholds are kept squared, as the reference keeps them.
+struct AutoDisableParams {
+    float idle_time = 0.f;
+    int idle_steps = 10;
+    unsigned average_samples = 1;
+    float lin_thr2 = 0.01f * 0.01f, ang_thr2 = 0.01f * 0.01f;
+};
+
+// Damping parameters (dxDampingParameters, objects.h:100-108; defaults
+// objects.cpp:63-69); thresholds squared.
+struct DampingParams {
+    float lin_scale = 0.f, ang_scale = 0.f;
+    float lin_thr2 = 0.01f * 0.01f, ang_thr2 = 0.01f * 0.01f;
+};
+
 struct dxBody {
     dxWorld *world = nullptr;
     dxBody *next = nullptr, *prev = nullptr;
@@ -132,6 +148,18 @@ struct dxBody {
     dVector3 lvel = {0, 0, 0, 0}, avel = {0, 0, 0, 0}, facc = {0, 0, 0, 0}, tacc = {0, 0, 0, 0};
     dMass mass{};
     bool gravity = true, gyro = true, enabled = true;
+    // auto-disable state (objects.h:237-243)
+    bool auto_disable = false;
+    AutoDisableParams adis;
+    float adis_timeleft = 0.f;
+    int adis_stepsleft = 10;
+    std::vector<float> avg_lvel, avg_avel;  // [average_samples][3]
+    unsigned avg_counter = 0;
+    int avg_ready = 0;
+    // damping and angular speed cap (objects.h:246-247)
+    bool lin_damping = false, ang_damping = false, max_ang = false;
+    DampingParams dampingp;
+    float max_angular_speed = dInfinity;
     dxJointNode *firstjoint = nullptr;
     std::vector<dxGeom *> geoms;
     void *data = nullptr;
@@ -188,12 +216,18 @@ struct dxWorld {
     // extra iteration factor, extra iteration requirement delta
     float dyn_premature = 1e-8f, dyn_extra_factor = 1.0f, dyn_extra_delta = 1e-2f;
     float max_vel = dInfinity, min_depth = 0.f;
+    // defaults new bodies take (dxWorld::adis, body_flags, dampingp,
+    // max_angular_speed; objects.cpp:99-117)
+    AutoDisableParams adis;
+    bool body_auto_disable = false, body_lin_damping = false, body_ang_damping = false, body_max_ang = false;
+    DampingParams dampingp;
+    float max_angular_speed = dInfinity;
     int order = 0;
     int last_sweeps = 0, last_rows = 0;
     float last_res = -1.f;
     // device buffers of the step
     DevBuf pos, quat, lvel, avel, facc, tacc, mass, Ib, invIb, joints, surf, joff, rows, lambda, sched, phase, bscr,
-        counts, fb, bstart, residual, niters, scratch, island, segtab, isl_sweeps;
+        counts, fb, bstart, residual, niters, scratch, island, segtab, isl_sweeps, damp;
 };
 
 // Extra sweeps of the dynamic iteration count are capped here (the reference
@@ -406,6 +440,14 @@ dBodyID dBodyCreate(dWorldID w) {
     if (w->firstbody) w->firstbody->prev = b;
     w->firstbody = b;
     w->nbodies++;
+    // the world's auto-disable, damping and angular speed defaults (ode.cpp:267-283)
+    dBodySetAutoDisableDefaults(b);
+    b->adis_stepsleft = b->adis.idle_steps;
+    b->adis_timeleft = b->adis.idle_time;
+    dBodySetAutoDisableAverageSamplesCount(b, b->adis.average_samples);
+    dBodySetDampingDefaults(b);
+    b->max_ang = w->body_max_ang;
+    b->max_angular_speed = w->max_angular_speed;
     return b;
 }
 
@@ -447,9 +489,100 @@ void dBodySetMass(dBodyID b, const dMass *m) { b->mass = *m; }
 void dBodyGetMass(dBodyID b, dMass *m) { *m = b->mass; }
 void dBodyAddForce(dBodyID b, dReal fx, dReal fy, dReal fz) { b->facc[0] += fx; b->facc[1] += fy; b->facc[2] += fz; }
 void dBodyAddTorque(dBodyID b, dReal fx, dReal fy, dReal fz) { b->tacc[0] += fx; b->tacc[1] += fy; b->tacc[2] += fz; }
-void dBodyEnable(dBodyID b) { b->enabled = true; }
+void dBodyEnable(dBodyID b) {
+    b->enabled = true;
+    b->adis_stepsleft = b->adis.idle_steps;
+    b->adis_timeleft = b->adis.idle_time;
+}
 void dBodyDisable(dBodyID b) { b->enabled = false; }
 int dBodyIsEnabled(dBodyID b) { return b->enabled ? 1 : 0; }
+
+// ---- auto-disable (ode.cpp:891-1021, 1889-1969): thresholds squared on set,
+// square roots on get
+dReal dBodyGetAutoDisableLinearThreshold(dBodyID b) { return std::sqrt(b->adis.lin_thr2); }
+void dBodySetAutoDisableLinearThreshold(dBodyID b, dReal t) { b->adis.lin_thr2 = t * t; }
+dReal dBodyGetAutoDisableAngularThreshold(dBodyID b) { return std::sqrt(b->adis.ang_thr2); }
+void dBodySetAutoDisableAngularThreshold(dBodyID b, dReal t) { b->adis.ang_thr2 = t * t; }
+int dBodyGetAutoDisableAverageSamplesCount(dBodyID b) { return (int)b->adis.average_samples; }
+void dBodySetAutoDisableAverageSamplesCount(dBodyID b, unsigned int n) {
+    b->adis.average_samples = n;
+    b->avg_lvel.assign((size_t)n * 3, 0.f);
+    b->avg_avel.assign((size_t)n * 3, 0.f);
+    b->avg_counter = 0;
+    b->avg_ready = 0;
+}
+int dBodyGetAutoDisableSteps(dBodyID b) { return b->adis.idle_steps; }
+void dBodySetAutoDisableSteps(dBodyID b, int steps) { b->adis.idle_steps = steps; }
+dReal dBodyGetAutoDisableTime(dBodyID b) { return b->adis.idle_time; }
+void dBodySetAutoDisableTime(dBodyID b, dReal t) { b->adis.idle_time = t; }
+int dBodyGetAutoDisableFlag(dBodyID b) { return b->auto_disable ? 1 : 0; }
+void dBodySetAutoDisableFlag(dBodyID b, int on) {
+    if (!on) {
+        // clearing the flag also enables the body and takes the world's
+        // step/time counts and sample count again (ode.cpp:992-1010)
+        b->auto_disable = false;
+        b->enabled = true;
+        b->adis.idle_steps = b->world->adis.idle_steps;
+        b->adis.idle_time = b->world->adis.idle_time;
+        dBodySetAutoDisableAverageSamplesCount(b, b->world->adis.average_samples);
+    } else {
+        b->auto_disable = true;
+    }
+}
+void dBodySetAutoDisableDefaults(dBodyID b) {
+    b->adis = b->world->adis;
+    dBodySetAutoDisableFlag(b, b->world->body_auto_disable ? 1 : 0);
+}
+
+dReal dWorldGetAutoDisableLinearThreshold(dWorldID w) { return std::sqrt(w->adis.lin_thr2); }
+void dWorldSetAutoDisableLinearThreshold(dWorldID w, dReal t) { w->adis.lin_thr2 = t * t; }
+dReal dWorldGetAutoDisableAngularThreshold(dWorldID w) { return std::sqrt(w->adis.ang_thr2); }
+void dWorldSetAutoDisableAngularThreshold(dWorldID w, dReal t) { w->adis.ang_thr2 = t * t; }
+int dWorldGetAutoDisableAverageSamplesCount(dWorldID w) { return (int)w->adis.average_samples; }
+void dWorldSetAutoDisableAverageSamplesCount(dWorldID w, unsigned int n) { w->adis.average_samples = n; }
+int dWorldGetAutoDisableSteps(dWorldID w) { return w->adis.idle_steps; }
+void dWorldSetAutoDisableSteps(dWorldID w, int steps) { w->adis.idle_steps = steps; }
+dReal dWorldGetAutoDisableTime(dWorldID w) { return w->adis.idle_time; }
+void dWorldSetAutoDisableTime(dWorldID w, dReal t) { w->adis.idle_time = t; }
+int dWorldGetAutoDisableFlag(dWorldID w) { return w->body_auto_disable ? 1 : 0; }
+void dWorldSetAutoDisableFlag(dWorldID w, int on) { w->body_auto_disable = on != 0; }
+
+// ---- damping and angular speed cap (ode.cpp:1024-1110, 1971-2050): a
+// scale of 0 turns damping off, an infinite cap turns the cap off
+dReal dBodyGetLinearDamping(dBodyID b) { return b->dampingp.lin_scale; }
+void dBodySetLinearDamping(dBodyID b, dReal scale) { b->lin_damping = scale != 0; b->dampingp.lin_scale = scale; }
+dReal dBodyGetAngularDamping(dBodyID b) { return b->dampingp.ang_scale; }
+void dBodySetAngularDamping(dBodyID b, dReal scale) { b->ang_damping = scale != 0; b->dampingp.ang_scale = scale; }
+void dBodySetDamping(dBodyID b, dReal lin, dReal ang) {
+    dBodySetLinearDamping(b, lin);
+    dBodySetAngularDamping(b, ang);
+}
+dReal dBodyGetLinearDampingThreshold(dBodyID b) { return std::sqrt(b->dampingp.lin_thr2); }
+void dBodySetLinearDampingThreshold(dBodyID b, dReal t) { b->dampingp.lin_thr2 = t * t; }
+dReal dBodyGetAngularDampingThreshold(dBodyID b) { return std::sqrt(b->dampingp.ang_thr2); }
+void dBodySetAngularDampingThreshold(dBodyID b, dReal t) { b->dampingp.ang_thr2 = t * t; }
+void dBodySetDampingDefaults(dBodyID b) {
+    b->dampingp = b->world->dampingp;
+    b->lin_damping = b->world->body_lin_damping;
+    b->ang_damping = b->world->body_ang_damping;
+}
+dReal dBodyGetMaxAngularSpeed(dBodyID b) { return b->max_angular_speed; }
+void dBodySetMaxAngularSpeed(dBodyID b, dReal s) { b->max_ang = s < dInfinity; b->max_angular_speed = s; }
+
+dReal dWorldGetLinearDampingThreshold(dWorldID w) { return std::sqrt(w->dampingp.lin_thr2); }
+void dWorldSetLinearDampingThreshold(dWorldID w, dReal t) { w->dampingp.lin_thr2 = t * t; }
+dReal dWorldGetAngularDampingThreshold(dWorldID w) { return std::sqrt(w->dampingp.ang_thr2); }
+void dWorldSetAngularDampingThreshold(dWorldID w, dReal t) { w->dampingp.ang_thr2 = t * t; }
+dReal dWorldGetLinearDamping(dWorldID w) { return w->dampingp.lin_scale; }
+void dWorldSetLinearDamping(dWorldID w, dReal scale) { w->body_lin_damping = scale != 0; w->dampingp.lin_scale = scale; }
+dReal dWorldGetAngularDamping(dWorldID w) { return w->dampingp.ang_scale; }
+void dWorldSetAngularDamping(dWorldID w, dReal scale) { w->body_ang_damping = scale != 0; w->dampingp.ang_scale = scale; }
+void dWorldSetDamping(dWorldID w, dReal lin, dReal ang) {
+    dWorldSetLinearDamping(w, lin);
+    dWorldSetAngularDamping(w, ang);
+}
+dReal dWorldGetMaxAngularSpeed(dWorldID w) { return w->max_angular_speed; }
+void dWorldSetMaxAngularSpeed(dWorldID w, dReal s) { w->body_max_ang = s < dInfinity; w->max_angular_speed = s; }
 void dBodySetGravityMode(dBodyID b, int mode) { b->gravity = mode != 0; }
 void dBodySetGyroscopicMode(dBodyID b, int enabled) { b->gyro = enabled != 0; }
 
@@ -863,8 +996,55 @@ int dCollide(dGeomID o1, dGeomID o2, int flags, dContactGeom *contact, int skip)
 // dWorldQuickStep objects.h:427 / ode.cpp:1847: islands by the reference's
 // DFS, rows and sweeps on the GPU.  Returns 1, or 0 (state unchanged) when
 // the device work fails.
+// dInternalHandleAutoDisabling (util.cpp:427-565), run before the islands
+// are built: each enabled auto-disabling body with joints samples its
+// velocities into a ring of average_samples entries; once the ring is full,
+// the squared speeds of the averages are tested against the thresholds, idle
+// steps count the step and time budgets down (anything else resets them), and
+// a body whose budgets are both spent is disabled with its velocities zeroed.
+static void handle_auto_disabling(dxWorld *w, float h) {
+    for (dxBody *b = w->firstbody; b; b = b->next) {
+        if (!b->firstjoint) continue;  // never freeze a body in mid-air
+        if (!b->auto_disable || !b->enabled) continue;
+        const unsigned n = b->adis.average_samples;
+        if (n == 0) continue;
+        if (b->avg_counter >= n) { b->avg_ready = 0; b->avg_counter = 0; }
+        for (int k = 0; k < 3; k++) {
+            b->avg_lvel[3 * b->avg_counter + k] = b->lvel[k];
+            b->avg_avel[3 * b->avg_counter + k] = b->avel[k];
+        }
+        if (++b->avg_counter >= n) { b->avg_counter = 0; b->avg_ready = 1; }
+        bool idle = false;
+        if (b->avg_ready) {
+            idle = true;
+            float al[3], aa[3];
+            for (int k = 0; k < 3; k++) { al[k] = b->avg_lvel[k]; aa[k] = b->avg_avel[k]; }
+            if (n > 1) {
+                for (unsigned i = 1; i < n; i++)
+                    for (int k = 0; k < 3; k++) { al[k] += b->avg_lvel[3 * i + k]; aa[k] += b->avg_avel[3 * i + k]; }
+                const float r1 = 1.f / (float)n;
+                for (int k = 0; k < 3; k++) { al[k] *= r1; aa[k] *= r1; }
+            }
+            if (al[0] * al[0] + al[1] * al[1] + al[2] * al[2] > b->adis.lin_thr2) idle = false;
+            else if (aa[0] * aa[0] + aa[1] * aa[1] + aa[2] * aa[2] > b->adis.ang_thr2) idle = false;
+        }
+        if (idle) {
+            b->adis_stepsleft--;
+            b->adis_timeleft -= h;
+        } else {
+            b->adis_stepsleft = b->adis.idle_steps;
+            b->adis_timeleft = b->adis.idle_time;
+        }
+        if (b->adis_stepsleft <= 0 && b->adis_timeleft <= 0) {
+            b->enabled = false;
+            for (int k = 0; k < 3; k++) { b->lvel[k] = 0; b->avel[k] = 0; }
+        }
+    }
+}
+
 int dWorldQuickStep(dWorldID w, dReal stepsize) {
     if (!w || !(stepsize > 0) || !ensure_stream()) return 0;
+    handle_auto_disabling(w, stepsize);
     // ---- islands (util.cpp:724-855): body list order, LIFO joint lists
     std::vector<dxBody *> bodies;
     std::vector<dxJoint *> joints;
@@ -917,7 +1097,7 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     for (int i = 0; i < nb; i++) bodies[i]->tag = i;
 
     // ---- records
-    std::vector<float4> pos(nb), quat(nb), lvel(nb), avel(nb), facc(nb), tacc(nb), mass(nb);
+    std::vector<float4> pos(nb), quat(nb), lvel(nb), avel(nb), facc(nb), tacc(nb), mass(nb), damp;
     std::vector<float> Ib((size_t)nb * 12), invIb((size_t)nb * 12);
     for (int i = 0; i < nb; i++) {
         dxBody *b = bodies[i];
@@ -929,6 +1109,14 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
         tacc[i] = make_float4(b->tacc[0], b->tacc[1], b->tacc[2], 0.f);
         const float flags = (float)((b->gravity ? 0 : 1) | (b->gyro ? 2 : 0));
         mass[i] = make_float4(b->mass.mass, 1.f / b->mass.mass, flags, 0.f);
+        const int dbits = (b->lin_damping ? DAMP_LINEAR : 0) | (b->ang_damping ? DAMP_ANGULAR : 0) |
+                          (b->max_ang ? DAMP_MAX_ANGULAR_SPEED : 0);
+        if (dbits) {
+            if (damp.empty()) damp.assign((size_t)nb * 2, make_float4(0.f, 0.f, 0.f, 0.f));
+            damp[2 * i] = make_float4(b->dampingp.lin_scale, b->dampingp.ang_scale, b->dampingp.lin_thr2,
+                                      b->dampingp.ang_thr2);
+            damp[2 * i + 1] = make_float4(b->max_angular_speed, (float)dbits, 0.f, 0.f);
+        }
         float *I = &Ib[(size_t)i * 12];
         for (int k = 0; k < 12; k++) I[k] = b->mass.I[k];
         I[3] = I[7] = I[11] = 0.f;
@@ -1204,6 +1392,10 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     D.bscr = w->bscr.as<float>();
     D.joff = w->joff.as<int>();
     D.fb = w->fb.as<float>();
+    if (!damp.empty()) {
+        if (!up(w->damp, damp.data(), damp.size())) return 0;
+        D.damp = w->damp.as<float4>();
+    }
     StepParams P{};
     P.h = stepsize;
     P.gx = w->gravity[0]; P.gy = w->gravity[1]; P.gz = w->gravity[2];

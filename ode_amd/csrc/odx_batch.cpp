@@ -166,6 +166,22 @@ bool odx_state_upload(DevWorlds &D, int nb, const float *bodies, int ng, const f
     D.gpos = g4;
     ok = ok && alloc(&g4, ng) && upload(g4, gq.data(), ng);
     D.gquat = g4;
+    // damping / angular speed cap of bodies whose flags ask for them (record
+    // slots 3, 11, 15, 19, 23; thresholds squared as dBodySet*DampingThreshold)
+    std::vector<float4> damp;
+    for (int i = 0; i < nb; i++) {
+        const float *r = bodies + (size_t)i * BODY_STRIDE;
+        const int bits = ((int)r[31] >> 2) & 7;
+        if (!bits) continue;
+        if (damp.empty()) damp.assign((size_t)nb * 2, make_float4(0.f, 0.f, 0.f, 0.f));
+        damp[2 * i] = make_float4(r[3], r[11], r[15] * r[15], r[19] * r[19]);
+        damp[2 * i + 1] = make_float4(r[23], (float)bits, r[15], r[19]);
+    }
+    D.damp = nullptr;
+    if (ok && !damp.empty()) {
+        ok = alloc(&g4, damp.size()) && upload(g4, damp.data(), damp.size());
+        D.damp = g4;
+    }
     return ok;
 }
 
@@ -191,6 +207,16 @@ bool odx_bodies_download(const DevWorlds &D, int nb, float *out) {
         const float *I = &Ib[(size_t)i * 12];
         r[25] = I[0]; r[26] = I[5]; r[27] = I[10]; r[28] = I[1]; r[29] = I[2]; r[30] = I[6];
         r[31] = mass[i].z;
+    }
+    if (D.damp) {
+        std::vector<float4> damp((size_t)nb * 2);
+        if (!download(damp.data(), (float4 *)D.damp, damp.size())) return false;
+        for (int i = 0; i < nb; i++) {
+            float *r = out + (size_t)i * BODY_STRIDE;
+            if (!((int)damp[2 * i + 1].y)) continue;
+            r[3] = damp[2 * i].x; r[11] = damp[2 * i].y;
+            r[15] = damp[2 * i + 1].z; r[19] = damp[2 * i + 1].w; r[23] = damp[2 * i + 1].x;
+        }
     }
     return true;
 }

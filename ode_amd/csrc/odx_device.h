@@ -102,6 +102,11 @@ struct DevWorlds {
     const float4 *mass;                                // [NB] mass, invMass, flags, 0
     const float *invIb;                                // [NB*12] body-frame inverse inertia
     const float *Ib;                                   // [NB*12] body-frame inertia
+    // [NB*2] per-body damping and angular speed cap (util.cpp:583-594, 675-698),
+    // null when no body uses them: (linear scale, angular scale, linear
+    // threshold^2, angular threshold^2), (max angular speed, DAMP_* bits,
+    // linear threshold, angular threshold)
+    const float4 *damp;
     const float4 *geom0, *geom1, *gpos, *gquat;        // [NG] cls/body/p0/p1, p2/p3, static pose
     const float *balls;                                // [NBALL*16]
     const Surface *surf;
@@ -136,6 +141,9 @@ struct DevWorlds {
     int *island_sweeps;     // [nislands] sweeps each island ran, may be null
 };
 constexpr int PTIME_SLOTS = 6;
+
+// DAMP_* bits of a body's damping record (BODY record flags bits 2-4 >> 2)
+enum { DAMP_LINEAR = 1, DAMP_ANGULAR = 2, DAMP_MAX_ANGULAR_SPEED = 4 };
 
 // Geom view for the collision entry points of the single-world API:
 // cls, body identity (-1 none), params[4], pos[3], R[12] (3x4).

@@ -1243,8 +1243,9 @@ __global__ __launch_bounds__(1024) void k_order_worlds(const int *niters, int wb
 // Solve stage of world w (sweeps, residual stop, warm-start records,
 // integration); s_dyn: the launch's dynamic LDS (solve_small_lds_bytes).
 // Stage4b (v += h fc), Stage6a (v += h M^-1 f), Stage6b + dxStepBody
-// (x += h v, q += h dq(w, q), normalised; f = t = 0) of body b,
-// quickstep.cpp:3082-3102, 3299-3337, util.cpp:583-699 (infinitesimal rotation).
+// (angular speed cap, x += h v, q += h dq(w, q), normalised, damping;
+// f = t = 0) of body b, quickstep.cpp:3082-3102, 3299-3337,
+// util.cpp:583-699 (infinitesimal rotation).
 __device__ __forceinline__ void integrate_body(const DevWorlds &D, const StepParams &P, int b, const float *fc,
                                                bool rows) {
     const float h = P.h;
@@ -1259,6 +1260,20 @@ __device__ __forceinline__ void integrate_body(const DevWorlds &D, const StepPar
     float tq[3] = {ta.x * h, ta.y * h, ta.z * h}, dw[3];
     mul0_331(dw, s + 12, tq);
     av.x += dw[0]; av.y += dw[1]; av.z += dw[2];
+    float4 d0 = make_float4(0.f, 0.f, 0.f, 0.f), d1 = d0;
+    int dfl = 0;
+    if (D.damp) {
+        d0 = D.damp[2 * b];
+        d1 = D.damp[2 * b + 1];
+        dfl = (int)d1.y;
+    }
+    if (dfl & DAMP_MAX_ANGULAR_SPEED) {  // util.cpp:585-594
+        const float a2 = av.x * av.x + av.y * av.y + av.z * av.z;
+        if (a2 > d1.x * d1.x) {
+            const float c = d1.x / sqrtf(a2);
+            av.x *= c; av.y *= c; av.z *= c;
+        }
+    }
     p4.x += h * lv.x; p4.y += h * lv.y; p4.z += h * lv.z;
     float dq0 = 0.5f * (-av.x * q4.y - av.y * q4.z - av.z * q4.w);
     float dq1 = 0.5f * (av.x * q4.x + av.y * q4.w - av.z * q4.z);
@@ -1271,6 +1286,19 @@ __device__ __forceinline__ void integrate_body(const DevWorlds &D, const StepPar
         q4.x *= l; q4.y *= l; q4.z *= l; q4.w *= l;
     } else {
         q4 = make_float4(1.f, 0.f, 0.f, 0.f);
+    }
+    // damping after the move (util.cpp:675-698): scale a speed above its threshold
+    if (dfl & DAMP_LINEAR) {
+        if (lv.x * lv.x + lv.y * lv.y + lv.z * lv.z > d0.z) {
+            const float k = 1.f - d0.x;
+            lv.x *= k; lv.y *= k; lv.z *= k;
+        }
+    }
+    if (dfl & DAMP_ANGULAR) {
+        if (av.x * av.x + av.y * av.y + av.z * av.z > d0.w) {
+            const float k = 1.f - d0.y;
+            av.x *= k; av.y *= k; av.z *= k;
+        }
     }
     D.pos[b] = p4; D.quat[b] = q4; D.lvel[b] = lv; D.avel[b] = av;
     D.facc[b] = make_float4(0.f, 0.f, 0.f, 0.f);

@@ -92,7 +92,24 @@ def api():
         "dAreConnectedExcluding": (i, [vp, vp, i]),
         "dRand": (ctypes.c_ulong, []), "dRandSetSeed": (None, [ctypes.c_ulong]), "dRandInt": (i, [i]),
         "dRandGetSeed": (ctypes.c_ulong, []),
+        "dBodyEnable": (None, [vp]), "dBodyDisable": (None, [vp]), "dBodyIsEnabled": (i, [vp]),
     }
+    u = ctypes.c_uint
+    # auto-disable, damping and angular speed cap: world defaults and per body
+    for who in ("World", "Body"):
+        for name in ("AutoDisableLinearThreshold", "AutoDisableAngularThreshold", "AutoDisableTime",
+                     "LinearDampingThreshold", "AngularDampingThreshold", "LinearDamping", "AngularDamping",
+                     "MaxAngularSpeed"):
+            sig["d%sGet%s" % (who, name)] = (f, [vp])
+            sig["d%sSet%s" % (who, name)] = (None, [vp, f])
+        for name in ("AutoDisableSteps", "AutoDisableFlag"):
+            sig["d%sGet%s" % (who, name)] = (i, [vp])
+            sig["d%sSet%s" % (who, name)] = (None, [vp, i])
+        sig["d%sGetAutoDisableAverageSamplesCount" % who] = (i, [vp])
+        sig["d%sSetAutoDisableAverageSamplesCount" % who] = (None, [vp, u])
+        sig["d%sSetDamping" % who] = (None, [vp, f, f])
+    sig["dBodySetAutoDisableDefaults"] = (None, [vp])
+    sig["dBodySetDampingDefaults"] = (None, [vp])
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
         fn.restype = res

@@ -1418,7 +1418,11 @@ template <class R> double residual(const System<R> &S, const std::vector<R> &lam
 }
 
 /* Stages 4b, 6a, 6b: quickstep.cpp:3082-3102, 3299-3337, 3409-3439 and
- * dxStepBody util.cpp:583-699 (infinitesimal rotation, no caps/damping). */
+ * dxStepBody util.cpp:583-699 (infinitesimal rotation).  Body flags bit 4
+ * caps the angular speed at slot 23 before the move (util.cpp:585-594); bits
+ * 2 and 3 scale the linear / angular velocity by 1 - slot 3 / slot 11 after
+ * the move when its squared speed exceeds the square of slot 15 / slot 19
+ * (util.cpp:675-698). */
 template <class R> void integrate(int nb, R *bodies, const std::vector<R> &invI, const std::vector<R> *fc, R h) {
     for (int i = 0; i < nb; i++) {
         R *b = bodies + (size_t)i * ORC_BODY_STRIDE;
@@ -1438,11 +1442,23 @@ template <class R> void integrate(int nb, R *bodies, const std::vector<R> &invI,
     }
     for (int i = 0; i < nb; i++) {
         R *b = bodies + (size_t)i * ORC_BODY_STRIDE;
+        const int flags = (int)b[31];
+        if (flags & 16) {
+            const R cap = b[23], a2 = b[12] * b[12] + b[13] * b[13] + b[14] * b[14];
+            if (a2 > cap * cap) {
+                const R c = cap / std::sqrt(a2);
+                for (int k = 0; k < 3; k++) b[12 + k] *= c;
+            }
+        }
         for (int k = 0; k < 3; k++) b[k] += h * b[8 + k];
         R dq[4];
         DQfromW(dq, b + 12, b + 4);
         for (int k = 0; k < 4; k++) b[4 + k] += h * dq[k];
         normalize4(b + 4);
+        if ((flags & 4) && b[8] * b[8] + b[9] * b[9] + b[10] * b[10] > b[15] * b[15])
+            for (int k = 0; k < 3; k++) b[8 + k] *= 1 - b[3];
+        if ((flags & 8) && b[12] * b[12] + b[13] * b[13] + b[14] * b[14] > b[19] * b[19])
+            for (int k = 0; k < 3; k++) b[12 + k] *= 1 - b[11];
         for (int k = 0; k < 3; k++) { b[16 + k] = 0; b[20 + k] = 0; }
     }
 }

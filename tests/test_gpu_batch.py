@@ -58,18 +58,38 @@ def run_one_step(worlds, prm, order=0):
     return B
 
 
+@pytest.mark.parametrize("damped", [False, True])
 @pytest.mark.parametrize("seed", [0, 1])
-def test_single_step_matches_oracle_reference_order(seed):
-    say("case: uneven worlds seed", seed)
+def test_single_step_matches_oracle_reference_order(seed, damped):
+    say("case: uneven worlds seed", seed, "damping and angular speed cap", damped)
     rng = np.random.default_rng(100 + seed)
     worlds = [S.uneven_world(rng, 8, 24) for _ in range(12)]
     # sink a few bodies into the ground so every world has contacts
     for w in worlds:
         w.bodies[:, 2] = np.minimum(w.bodies[:, 2], 0.3)
+        if damped:
+            # record flags bits 2-4 with slots 3/11/15/19/23 (dxStepBody damping and cap)
+            nb = len(w.bodies)
+            w.bodies[:, 12:15] = rng.normal(0, 3.0, size=(nb, 3))
+            sel = rng.random(nb) < 0.6
+            bits = rng.integers(1, 8, size=nb) * sel
+            w.bodies[:, 31] = w.bodies[:, 31].astype(int) | (bits << 2)
+            w.bodies[:, 3] = rng.uniform(0.01, 0.3, nb)
+            w.bodies[:, 11] = rng.uniform(0.01, 0.3, nb)
+            w.bodies[:, 15] = rng.uniform(0.0, 0.5, nb)
+            w.bodies[:, 19] = rng.uniform(0.0, 2.0, nb)
+            w.bodies[:, 23] = rng.uniform(0.5, 2.0, nb)
     prm = O.params()
     odx = _odx()
     B = run_one_step(worlds, prm)
     gb = B.bodies()
+    if damped:
+        # the damping fields travel back with the records
+        allb = np.concatenate([w.bodies for w in worlds])
+        for k in (3, 11, 15, 19, 23):
+            sel = (allb[:, 31].astype(int) >> 2) != 0
+            np.testing.assert_allclose(gb[sel, k], allb[sel, k], rtol=1e-7)
+        np.testing.assert_array_equal(gb[:, 31], allb[:, 31].astype(np.float32))
     st = B.stats()
     assert B.overflow().sum() == 0
     for wi, w in enumerate(worlds):

@@ -334,3 +334,168 @@ def test_random_reordering_matches_oracle_and_draws_as_the_reference(dynamic):
     assert err_x < 1e-4 and err_v < 1e-4
     F.L.dJointGroupEmpty(F.group)
     F.close()
+
+
+def _oracle_step_of_frame(F, before, prm, dtype=np.float64):
+    """The oracle's QuickStep over the frame's contacts in the island order."""
+    recs, pairs = [], []
+    for (i1, i2, pos, nrm, depth, g1, g2) in F.contacts:
+        r = np.zeros(16)
+        r[1], r[2] = i1, i2
+        r[3:6], r[6:9], r[9] = pos, nrm, depth
+        recs.append(r)
+        pairs.append((i1, i2) if i1 >= 0 else (i2, -1))
+    _, order = island_order(len(before), pairs)
+    jr = np.array([recs[k] for k in order]).reshape(-1, 16)
+    return O.quickstep(before, jr, prm, dtype=dtype)[0]
+
+
+def test_damping_and_angular_speed_cap_match_oracle():
+    # dxStepBody's cap before the move and damping after it (util.cpp:583-699),
+    # per body through the API, against the oracle's record flags and slots
+    say("case: damping and angular speed cap")
+    scene = settle_scene(2)
+    rng = np.random.default_rng(9)
+    scene.bodies[:, 12:15] = rng.normal(0, 3.0, size=(len(scene.bodies), 3))
+    F = Frame(scene)
+    L = F.L
+    before = scene.bodies.astype(np.float64).copy()
+    for i, b in enumerate(F.bodies):
+        L.dBodySetAngularVel(b, *[float(x) for x in scene.bodies[i, 12:15]])
+        kind = i % 4
+        if kind == 0:
+            L.dBodySetDamping(b, 0.1, 0.2)
+            L.dBodySetLinearDampingThreshold(b, 0.0)
+            L.dBodySetAngularDampingThreshold(b, 0.5)
+            before[i, 31] = int(before[i, 31]) | 4 | 8
+            before[i, 3], before[i, 11], before[i, 15], before[i, 19] = 0.1, 0.2, 0.0, 0.5
+        elif kind == 1:
+            L.dBodySetMaxAngularSpeed(b, 1.5)
+            before[i, 31] = int(before[i, 31]) | 16
+            before[i, 23] = 1.5
+        elif kind == 2:
+            L.dBodySetMaxAngularSpeed(b, 1.0)
+            L.dBodySetAngularDamping(b, 0.3)
+            L.dBodySetAngularDampingThreshold(b, 0.0)
+            before[i, 31] = int(before[i, 31]) | 8 | 16
+            before[i, 11], before[i, 19], before[i, 23] = 0.3, 0.0, 1.0
+    # getters return what was set (thresholds through their squares)
+    assert L.dBodyGetLinearDamping(F.bodies[0]) == pytest.approx(0.1)
+    assert L.dBodyGetAngularDampingThreshold(F.bodies[0]) == pytest.approx(0.5)
+    assert L.dBodyGetMaxAngularSpeed(F.bodies[1]) == pytest.approx(1.5)
+    F.collide()
+    assert L.dWorldQuickStep(F.w, 0.01) == 1
+    got = F.state()
+    ref = _oracle_step_of_frame(F, before, O.params())
+    err_x = np.max(np.abs(got[:, 0:3] - ref[:, 0:3])) / np.max(np.abs(ref[:, 0:3]))
+    vel = np.r_[8:11, 12:15]  # slot 11 holds the angular damping scale, not a velocity
+    err_v = np.max(np.abs(got[:, vel] - ref[:, vel])) / max(np.max(np.abs(ref[:, vel])), 1.0)
+    say("  contacts", len(F.contacts), "err_x", err_x, "err_v", err_v)
+    assert err_x < 1e-4 and err_v < 1e-4
+    # capped bodies end at or under their cap (damping after it only lowers it)
+    for i in range(len(F.bodies)):
+        if i % 4 in (1, 2):
+            assert np.linalg.norm(got[i, 12:15]) <= (1.5 if i % 4 == 1 else 1.0) * (1 + 1e-6)
+    F.L.dJointGroupEmpty(F.group)
+    F.close()
+
+
+def test_world_damping_and_auto_disable_defaults_reach_new_bodies():
+    say("case: world defaults")
+    from ode_amd import ode as A
+    L = A.api()
+    w = L.dWorldCreate()
+    b0 = L.dBodyCreate(w)
+    # dWorldCreate defaults (objects.cpp:54-69, 99-117)
+    assert L.dBodyGetAutoDisableFlag(b0) == 0 and L.dBodyGetAutoDisableSteps(b0) == 10
+    assert L.dBodyGetAutoDisableAverageSamplesCount(b0) == 1
+    assert L.dBodyGetAutoDisableLinearThreshold(b0) == pytest.approx(0.01)
+    assert L.dBodyGetLinearDamping(b0) == 0 and L.dBodyGetMaxAngularSpeed(b0) == float("inf")
+    L.dWorldSetAutoDisableFlag(w, 1)
+    L.dWorldSetAutoDisableSteps(w, 7)
+    L.dWorldSetAutoDisableTime(w, 0.25)
+    L.dWorldSetAutoDisableAverageSamplesCount(w, 4)
+    L.dWorldSetDamping(w, 0.05, 0.125)
+    L.dWorldSetLinearDampingThreshold(w, 0.5)
+    L.dWorldSetMaxAngularSpeed(w, 3.0)
+    b1 = L.dBodyCreate(w)
+    assert L.dBodyGetAutoDisableFlag(b1) == 1 and L.dBodyGetAutoDisableSteps(b1) == 7
+    assert L.dBodyGetAutoDisableTime(b1) == pytest.approx(0.25)
+    assert L.dBodyGetAutoDisableAverageSamplesCount(b1) == 4
+    assert L.dBodyGetLinearDamping(b1) == pytest.approx(0.05)
+    assert L.dBodyGetAngularDamping(b1) == pytest.approx(0.125)
+    assert L.dBodyGetLinearDampingThreshold(b1) == pytest.approx(0.5)
+    assert L.dBodyGetMaxAngularSpeed(b1) == pytest.approx(3.0)
+    # bodies made earlier keep their own settings
+    assert L.dBodyGetAutoDisableFlag(b0) == 0 and L.dBodyGetLinearDamping(b0) == 0
+    # clearing a body's flag takes the world's counts again and enables it
+    L.dBodySetAutoDisableSteps(b1, 2)
+    L.dBodyDisable(b1)
+    L.dBodySetAutoDisableFlag(b1, 0)
+    assert L.dBodyIsEnabled(b1) == 1 and L.dBodyGetAutoDisableSteps(b1) == 7
+    L.dWorldDestroy(w)
+
+
+def test_auto_disable_matches_oracle_through_sleep_and_wake():
+    # four spheres resting on the plane fall asleep after 5 idle steps; a fifth
+    # rolls into the first, whose contact with the awake one wakes it; the
+    # library's enabled flags follow the oracle's auto-disable pass fed with
+    # the library's own velocities, step by step
+    from oracle import autodisable as AD
+    say("case: auto-disable sleep and wake")
+    m, r = 10.0, 0.25
+    I = 0.4 * m * r * r
+
+    def sphere(x, vx=0.0):
+        b = np.zeros(32)
+        b[0:3] = (x, 0.0, r)
+        b[4] = 1.0
+        b[8] = vx
+        b[24], b[25:28], b[31] = m, (I, I, I), 2
+        return b
+
+    bodies = [sphere(0.0), sphere(2.0), sphere(4.0), sphere(6.0), sphere(-0.6, 1.5)]
+    geoms = [S.geom(S.SPHERE, i, (r,)) for i in range(5)] + [S.plane(0, 0, 1, 0)]
+    F = Frame(S.World(np.array(bodies), np.array(geoms)))
+    L = F.L
+    # the step budget set after creation takes effect at the next reset
+    # (dBodyEnable restarts the countdown), as in the reference
+    ref = [AD.Body(idle_steps=10) for _ in F.bodies]
+    for b, rb in zip(F.bodies, ref):
+        L.dBodySetAutoDisableFlag(b, 1)
+        L.dBodySetAutoDisableSteps(b, 5)
+        L.dBodyEnable(b)
+        rb.set_idle_steps(5)
+        rb.enable()
+    n = len(F.bodies)
+    history = []
+    for step in range(1, 41):
+        F.collide()
+        lv = np.array([F.A.vec(L.dBodyGetLinearVel(b)) for b in F.bodies], np.float32)
+        av = np.array([F.A.vec(L.dBodyGetAngularVel(b)) for b in F.bodies], np.float32)
+        joints = [(c[0], c[1]) for c in F.contacts]
+        has = [any(i in j for j in joints) for i in range(n)]
+        AD.handle(ref, lv, av, has, 0.01)
+        AD.enabled_after_islands(ref, joints)
+        pos_before = F.state()[:, 0:3]
+        assert L.dWorldQuickStep(F.w, 0.01) == 1
+        got = [L.dBodyIsEnabled(b) for b in F.bodies]
+        want = [int(b.enabled) for b in ref]
+        assert got == want, (step, got, want)
+        st = F.state()
+        for i in range(n):
+            if not got[i]:
+                # asleep: zero velocities, pose untouched
+                assert np.all(st[i, 8:15] == 0)
+                np.testing.assert_array_equal(st[i, 0:3], pos_before[i])
+        history.append(got)
+        L.dJointGroupEmpty(F.group)
+    h = np.array(history)
+    say("  sphere 0 enabled per step:", "".join(map(str, h[:, 0])))
+    # the resting spheres fall asleep on the 5th step, exactly
+    assert np.all(h[:4, 1:4] == 1) and np.all(h[4:, 1:4] == 0)
+    # sphere 0 sleeps from step 5 until the rolling sphere reaches it
+    assert h[4, 0] == 0
+    woke = np.nonzero(h[5:, 0])[0]
+    assert len(woke) > 0, "the rolling sphere never woke sphere 0"
+    F.close()

@@ -1,0 +1,207 @@
+"""The oracle's step-adjacent semantics pinned by closed forms: damping and
+the angular speed cap of dxStepBody (ode/src/util.cpp:583-699), and
+auto-disabling (util.cpp:427-565, defaults ode/src/objects.cpp:54-61)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import autodisable as AD
+
+H = 0.125  # a step whose multiples are exact in binary
+
+
+def free_body(v=(0.0, 0.0, 0.0), w=(0.0, 0.0, 0.0), flags=0, m=2.0, I=0.5):
+    b = np.zeros(32)
+    b[0:3] = (1.0, -2.0, 3.0)
+    b[4] = 1.0
+    b[8:11] = v
+    b[12:15] = w
+    b[24], b[25:28] = m, (I, I, I)
+    b[31] = flags
+    return b
+
+
+def step_free(b, gz=-9.81, dtype=np.float64):
+    prm = O.params(h=H, gravity=(0.0, 0.0, gz))
+    out, _ = O.quickstep(b[None, :], np.zeros((0, 16)), prm, dtype=dtype)
+    return out[0]
+
+
+# ------------------------------------------------------------------ damping
+
+def test_linear_damping_scales_the_velocity_after_the_move():
+    # above the threshold: v' = (v + h g)(1 - s), while x moves with the undamped v + h g
+    b = free_body(v=(2.0, 0.0, 1.0), flags=4)
+    b[3], b[15] = 0.25, 0.5
+    out = step_free(b)
+    v = np.array([2.0, 0.0, 1.0 + H * -9.81])
+    np.testing.assert_allclose(out[8:11], v * 0.75, rtol=1e-14)
+    np.testing.assert_allclose(out[0:3], b[0:3] + H * v, rtol=1e-14)
+    # the same body without the flag keeps v + h g
+    b[31] = 0
+    np.testing.assert_allclose(step_free(b)[8:11], v, rtol=1e-14)
+
+
+def test_damping_leaves_speeds_at_or_below_the_threshold():
+    # speeds exactly at the thresholds (binary-exact: 0.375^2 + 0.5^2 = 0.625^2,
+    # 0.75^2 + 1 = 1.25^2): the test is strict (speed^2 > threshold^2)
+    for dtype in (np.float32, np.float64):
+        b = free_body(v=(0.375, 0.5, 0.0), w=(0.0, 0.75, 1.0), flags=4 | 8)
+        b[3], b[11], b[15], b[19] = 0.5, 0.5, 0.625, 1.25
+        out = step_free(b, gz=0.0, dtype=dtype)
+        np.testing.assert_array_equal(out[8:11], [0.375, 0.5, 0.0])
+        np.testing.assert_array_equal(out[12:15], [0.0, 0.75, 1.0])
+        # just under the speeds: both scale by one half
+        b[15], b[19] = 0.6171875, 1.2421875
+        out = step_free(b, gz=0.0, dtype=dtype)
+        np.testing.assert_array_equal(out[8:11], [0.1875, 0.25, 0.0])
+        np.testing.assert_array_equal(out[12:15], [0.0, 0.375, 0.5])
+
+
+def test_angular_damping_follows_the_rotation_with_the_undamped_rate():
+    # a sphere spinning about z at 2 rad/s: q advances by the undamped rate,
+    # the stored rate is scaled by 1 - 0.1
+    b = free_body(w=(0.0, 0.0, 2.0), flags=8)
+    b[11], b[19] = 0.1, 0.0
+    out = step_free(b, gz=0.0)
+    np.testing.assert_allclose(out[12:15], [0.0, 0.0, 1.8], rtol=1e-14)
+    # q' = normalize(q + h/2 w x q) with q = 1: (1, 0, 0, h)
+    q = np.array([1.0, 0.0, 0.0, H])
+    np.testing.assert_allclose(out[4:8], q / np.linalg.norm(q), rtol=1e-14)
+
+
+# --------------------------------------------------------- angular speed cap
+
+def test_angular_speed_cap_before_the_move():
+    # |w| = 5 capped at 1: w' = w / 5, and the rotation uses the capped rate
+    b = free_body(w=(3.0, 0.0, 4.0), flags=16)
+    b[23] = 1.0
+    out = step_free(b, gz=0.0)
+    np.testing.assert_allclose(out[12:15], [0.6, 0.0, 0.8], rtol=1e-14)
+    q = np.array([1.0, 0.5 * H * 0.6, 0.0, 0.5 * H * 0.8])
+    np.testing.assert_allclose(out[4:8], q / np.linalg.norm(q), rtol=1e-14)
+    # a rate under the cap is untouched
+    b[23] = 6.0
+    np.testing.assert_allclose(step_free(b, gz=0.0)[12:15], [3.0, 0.0, 4.0], rtol=1e-15)
+
+
+def test_cap_and_damping_compose_in_dxstepbody_order():
+    # cap (to 1) first, then damping (1 - 0.5) after the move
+    b = free_body(w=(0.0, 3.0, 0.0), flags=16 | 8)
+    b[23], b[11], b[19] = 1.0, 0.5, 0.0
+    out = step_free(b, gz=0.0)
+    np.testing.assert_allclose(out[12:15], [0.0, 0.5, 0.0], rtol=1e-14)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_damping_in_a_contact_step_matches_the_undamped_step_scaled(dtype):
+    # with contacts, damping only rescales the post-move velocity of the
+    # flagged body: positions and the other bodies equal the undamped step
+    rng = np.random.default_rng(3)
+    from ode_amd import scenes as S
+    w = S.uneven_world(rng, 8, 8)
+    w.bodies[:, 2] = np.minimum(w.bodies[:, 2], 0.3)
+    prm = O.params()
+    ref, c, _ = O.world_step(w.bodies, w.geoms, prm, dtype=dtype)
+    bd = w.bodies.copy()
+    bd[0, 31] = int(bd[0, 31]) | 4
+    bd[0, 3], bd[0, 15] = 0.2, 0.0
+    got, c2, _ = O.world_step(bd, w.geoms, prm, dtype=dtype)
+    assert len(c) == len(c2)
+    np.testing.assert_array_equal(got[:, 0:3], ref[:, 0:3])
+    np.testing.assert_array_equal(got[:, 4:8], ref[:, 4:8])
+    np.testing.assert_array_equal(got[1:, 8:15], ref[1:, 8:15])
+    np.testing.assert_allclose(got[0, 8:11], ref[0, 8:11] * dtype(0.8), rtol=1e-6)
+
+
+# ------------------------------------------------------------- auto-disable
+
+def run_idle(b, vels, joints=True, h=0.01):
+    """Calls handle once per velocity; returns the 1-based call that disabled b (or None)."""
+    for k, v in enumerate(vels):
+        lv = np.array([v], np.float32)
+        av = np.zeros((1, 3), np.float32)
+        if AD.handle([b], lv, av, [joints], h):
+            assert np.all(lv == 0) and np.all(av == 0)
+            return k + 1
+    return None
+
+
+def test_instantaneous_mode_disables_on_the_idle_steps_th_idle_call():
+    b = AD.Body(idle_steps=10)
+    assert run_idle(b, [(0.0, 0.0, 0.0)] * 30) == 10
+    assert not b.enabled
+
+
+def test_a_new_step_budget_waits_for_the_next_reset():
+    # created with the world's 10 steps, then set to 4: the countdown already
+    # running still counts from 10; dBodyEnable restarts it from 4
+    b = AD.Body(idle_steps=10)
+    b.set_idle_steps(4)
+    assert run_idle(b, [(0.0, 0.0, 0.0)] * 30) == 10
+    b = AD.Body(idle_steps=10)
+    b.set_idle_steps(4)
+    b.enable()
+    assert run_idle(b, [(0.0, 0.0, 0.0)] * 30) == 4
+    # a moving step resets it to the new budget as well
+    b = AD.Body(idle_steps=10)
+    b.set_idle_steps(4)
+    assert run_idle(b, [(1.0, 0.0, 0.0)] + [(0.0, 0.0, 0.0)] * 30) == 1 + 4
+
+
+def test_a_body_without_joints_never_sleeps():
+    b = AD.Body(idle_steps=1)
+    assert run_idle(b, [(0.0, 0.0, 0.0)] * 30, joints=False) is None
+    assert b.enabled
+
+
+def test_motion_resets_the_countdown():
+    # idle 6 calls, one fast call, then 4 more idle calls disable (steps = 4)
+    b = AD.Body(idle_steps=4)
+    vels = [(0.0, 0.0, 0.0)] * 3 + [(1.0, 0.0, 0.0)] + [(0.0, 0.0, 0.0)] * 10
+    assert run_idle(b, vels) == 3 + 1 + 4
+
+
+def test_time_budget_counts_exact_steps():
+    # idle_time 0.5 with h = 1/8 and no step budget: the 4th idle call spends it
+    b = AD.Body(idle_steps=0, idle_time=0.5)
+    assert run_idle(b, [(0.0, 0.0, 0.0)] * 10, h=H) == 4
+    # both budgets must be spent: 6 steps outlast 4 time units
+    b = AD.Body(idle_steps=6, idle_time=0.5)
+    assert run_idle(b, [(0.0, 0.0, 0.0)] * 10, h=H) == 6
+
+
+def test_averaged_velocity_of_an_oscillating_body_is_idle_once_the_ring_is_full():
+    # +-1 m/s alternating: every instantaneous sample is fast, the 2-sample
+    # average is zero; the first call fills half the ring (not idle), the
+    # next 3 are idle
+    b = AD.Body(idle_steps=3, average_samples=2)
+    vels = [(1.0, 0.0, 0.0), (-1.0, 0.0, 0.0)] * 10
+    assert run_idle(b, vels) == 1 + 3
+    # instantaneous mode never sleeps on the same motion
+    b = AD.Body(idle_steps=3, average_samples=1)
+    assert run_idle(b, vels) is None
+
+
+def test_threshold_is_strict_on_the_squared_speed():
+    # a speed exactly at the threshold is idle (binary-exact: 0.375^2 + 0.5^2
+    # = 0.625^2); just above it is not
+    b = AD.Body(idle_steps=2, linear_threshold=0.625)
+    assert run_idle(b, [(0.375, 0.5, 0.0)] * 5) == 2
+    b = AD.Body(idle_steps=2, linear_threshold=0.625)
+    assert run_idle(b, [(0.375, 0.5078125, 0.0)] * 5) is None
+    b = AD.Body(idle_steps=2, angular_threshold=0.25)
+    lv = np.zeros((1, 3), np.float32)
+    av = np.array([[0.0, 0.3, 0.0]], np.float32)
+    assert not AD.handle([b], lv, av, [True], 0.01) and not AD.handle([b], lv, av, [True], 0.01)
+
+
+def test_island_pass_re_enables_a_sleeping_body_touched_by_an_awake_one():
+    # 0 sleeps on the ground; 1 (awake) lands on it: 0 is re-enabled; 2
+    # sleeps on the ground alone and stays asleep
+    bodies = [AD.Body(), AD.Body(), AD.Body()]
+    bodies[0].enabled = False
+    bodies[2].enabled = False
+    stepped = AD.enabled_after_islands(bodies, [(0, -1), (1, 0), (2, -1)])
+    assert stepped == [0, 1]
+    assert [b.enabled for b in bodies] == [True, True, False]
