@@ -31,7 +31,9 @@
  *               ode/src/util.cpp:583-699)
  *   GEOM  [16]: 0 class (0 sphere, 1 box, 2 capsule, 4 plane), 1 body (world-
  *               local index, -1 static), 2-5 params (sphere r | box lx,ly,lz |
- *               capsule r,len | plane a,b,c,d), 6-8 static pos, 9-12 static q
+ *               capsule r,len | plane a,b,c,d), 6-8 static pos, 9-12 static q,
+ *               13 material, 14 category bits, 15 collide bits (bits 0-23;
+ *               read by odxBatchSetSurfaceTable's policy)
  *   JOINT [16]: 0 type (0 contact, 1 ball), 1 body1, 2 body2 (-1 none);
  *               contact: 3-5 pos 6-8 normal 9 depth 10-12 fdir1 13 surface 14 g1 15 g2
  *               ball:    3-5 anchor1 (body1 frame) 6-8 anchor2 (body2 frame or world) 9 erp 10 cfm
@@ -89,6 +91,20 @@ int odxBatchSetSolver(odxBatch *b, int order, int max_iters, float tol, int chec
  * larger worlds stream rows from global memory; 0 disables the LDS path.
  * Fails when lds_rows does not fit the kernel's 64 KB LDS budget. */
 int odxBatchSetSolveOptions(odxBatch *b, int restart_after, int lds_rows);
+
+/* The near-callback policy on the device, in place of a per-pair callback
+ * that sets dContact::surface before dJointCreateContact (as in
+ * ode/demo/demo_boxstack.cpp:131-160).  table: float[nmat][nmat][16], rows
+ * 0 mode, 1 mu, 2 mu2, 3 bounce, 4 bounce_vel, 5 soft_erp, 6 soft_cfm,
+ * 7 motion1, 8 motion2, 9 motionN, 10 slip1, 11 slip2 (dSurfaceParameters,
+ * include/ode/contact.h:55-71), 12 contacts per pair (1..4; 0 = the batch's
+ * count).  A contact of geoms g1 < g2 takes entry (material(g1),
+ * material(g2)); the table must be symmetric and cover every geom's
+ * material (GEOM slot 13).  category_filter != 0 drops pairs whose category
+ * and collide bits (GEOM slots 14, 15) fail the test of dSpaceCollide's
+ * collideAABBs, ode/src/collision_space_internal.h:44-79: (cat1 & col2) ||
+ * (cat2 & col1).  nmat = 0 returns to the one surface of the parameters. */
+int odxBatchSetSurfaceTable(odxBatch *b, int nmat, const float *table, int category_filter);
 
 /* Splits the worlds into nsplit (1..8) equal index ranges that step on streams
  * of their own inside one hipGraph of graph_steps steps: the worlds are

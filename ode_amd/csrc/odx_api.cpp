@@ -1141,7 +1141,7 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
             for (int i = 0; i < 3; i++) { r[3 + i] = c.geom.pos[i]; r[6 + i] = sg * c.geom.normal[i]; r[10 + i] = c.fdir1[i]; }
             r[9] = c.geom.depth;
             r[13] = (float)surf.size();
-            Surface s;
+            Surface s{};
             s.mode = c.surface.mode;
             s.mu = c.surface.mu < 0 ? 0.f : c.surface.mu;
             s.mu2 = c.surface.mu2 < 0 ? 0.f : c.surface.mu2;

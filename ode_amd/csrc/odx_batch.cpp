@@ -135,8 +135,8 @@ bool odx_state_upload(DevWorlds &D, int nb, const float *bodies, int ng, const f
     for (int i = 0; i < ng; i++) {
         const float *r = geoms + (size_t)i * GEOM_STRIDE;
         g0[i] = make_float4(r[0], r[1], r[2], r[3]);
-        g1[i] = make_float4(r[4], r[5], 0.f, 0.f);
-        gp[i] = make_float4(r[6], r[7], r[8], 0.f);
+        g1[i] = make_float4(r[4], r[5], r[13], r[14]);  // material, category bits
+        gp[i] = make_float4(r[6], r[7], r[8], r[15]);   // collide bits
         float q[4] = {r[9], r[10], r[11], r[12]};
         float l = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
         if (l > 0.f) { l = 1.f / std::sqrt(l); for (float &x : q) x *= l; }
@@ -253,6 +253,9 @@ struct odxBatch {
     DevWorlds D{};
     StepParams P{};  // solver settings; wbase / wcount / split / feat_base are set per launch
     Surface surf{};
+    std::vector<int> geom_mat;  // material of each geom (GEOM slot 13)
+    Surface *surf_default = nullptr;  // device copy of the parameters' surface
+    Surface *surf_table = nullptr;  // device material-pair table (nmat * nmat), owned
     hipStream_t stream = nullptr;
     hipEvent_t fork = nullptr;
     std::vector<odxSplit> splits;  // empty: the whole batch on `stream`
@@ -274,6 +277,7 @@ struct odxBatch {
         if (fork) (void)hipEventDestroy(fork);
         if (stream) (void)hipStreamDestroy(stream);
         for (void *p : allocs) (void)hipFree(p);
+        if (surf_table) (void)hipFree(surf_table);
     }
     void drop_graph() {
         if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }
@@ -427,6 +431,8 @@ odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies
     B->nb = B->body_start[nworlds];
     B->ng = B->geom_start[nworlds];
     B->nball = B->ball_start[nworlds];
+    B->geom_mat.resize(B->ng);
+    for (int g = 0; g < B->ng; g++) B->geom_mat[g] = (int)geoms[(size_t)g * GEOM_STRIDE + 13];
     if (contacts_per_body <= 0) contacts_per_body = 8;
     B->joint_start.resize(nworlds + 1);
     B->row_start.resize(nworlds + 1);
@@ -498,6 +504,7 @@ odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies
     Surface *sd = nullptr;
     ok = ok && B->alloc(&sd, 1) && upload(sd, &B->surf, 1);
     D.surf = sd;
+    B->surf_default = sd;
     ok = ok && B->alloc(&D.joints, (size_t)B->jcap * JOINT_STRIDE);
     ok = ok && B->alloc(&D.njoints, nworlds) && B->alloc(&D.ncontacts, nworlds) && B->alloc(&D.nrows, nworlds) &&
          B->alloc(&D.nphases, nworlds) && B->alloc(&D.niters, nworlds) && B->alloc(&D.residual, nworlds);
@@ -545,6 +552,52 @@ int odxBatchSetSolver(odxBatch *b, int order, int max_iters, float tol, int chec
     b->P.tol = tol;
     b->P.check_every = check_every;
     b->P.warm = warm;
+    b->drop_graph();
+    return 1;
+}
+
+int odxBatchSetSurfaceTable(odxBatch *b, int nmat, const float *table, int category_filter) {
+    if (!b || nmat < 0 || nmat > ODX_MAX_MATERIALS || (nmat > 0 && !table)) {
+        odx_message(1, "odxBatchSetSurfaceTable: bad arguments (0 <= nmat <= %d)", ODX_MAX_MATERIALS);
+        return 0;
+    }
+    std::vector<Surface> t((size_t)std::max(nmat * nmat, 1));
+    for (int a = 0; a < nmat; a++)
+        for (int c = 0; c < nmat; c++) {
+            const float *e = table + ((size_t)a * nmat + c) * 16, *f = table + ((size_t)c * nmat + a) * 16;
+            for (int k = 0; k < 13; k++)
+                if (!(e[k] == f[k]) && !(e[k] != e[k] && f[k] != f[k])) {
+                    odx_message(1, "odxBatchSetSurfaceTable: entry (%d, %d) differs from (%d, %d)", a, c, c, a);
+                    return 0;
+                }
+            if (e[12] < 0 || e[12] > ODX_MAXC) {
+                odx_message(1, "odxBatchSetSurfaceTable: max contacts of (%d, %d) outside 0..%d", a, c, ODX_MAXC);
+                return 0;
+            }
+            Surface &s = t[(size_t)a * nmat + c];
+            s.mode = (int)e[0];
+            s.mu = e[1] < 0 ? 0.f : e[1]; s.mu2 = e[2] < 0 ? 0.f : e[2];
+            s.bounce = e[3]; s.bounce_vel = e[4]; s.soft_erp = e[5]; s.soft_cfm = e[6];
+            s.motion1 = e[7]; s.motion2 = e[8]; s.motionN = e[9]; s.slip1 = e[10]; s.slip2 = e[11];
+            s.max_contacts = (int)e[12];
+        }
+    for (int g = 0; g < b->ng; g++)
+        if (nmat > 0 && (b->geom_mat[g] < 0 || b->geom_mat[g] >= nmat)) {
+            odx_message(1, "odxBatchSetSurfaceTable: geom %d has material %d, outside the table's %d", g,
+                        b->geom_mat[g], nmat);
+            return 0;
+        }
+    // no step in flight may still read the table being replaced
+    if (!odx_check(hipDeviceSynchronize(), "hipDeviceSynchronize")) return 0;
+    if (b->surf_table) { (void)hipFree(b->surf_table); b->surf_table = nullptr; }
+    if (nmat > 0) {
+        if (!dalloc(&b->surf_table, t.size()) || !upload(b->surf_table, t.data(), t.size())) return 0;
+        b->D.surf = b->surf_table;
+    } else {
+        b->D.surf = b->surf_default;
+    }
+    b->P.nmat = nmat;
+    b->P.cat_filter = category_filter ? 1 : 0;
     b->drop_graph();
     return 1;
 }

@@ -41,7 +41,11 @@ enum { R_J1 = 0, R_RHS = 6, R_CFM = 7, R_J2 = 8, R_LO = 14, R_HI = 15, R_IMJ1 = 
 struct Surface {
     int mode;
     float mu, mu2, bounce, bounce_vel, soft_erp, soft_cfm, motion1, motion2, motionN, slip1, slip2;
+    int max_contacts;  // batched collide: contacts per geom pair of this surface (0: StepParams::maxc)
 };
+
+// Material-pair surface tables of the batched path hold at most this many materials.
+constexpr int ODX_MAX_MATERIALS = 16;
 
 struct StepParams {
     float h, gx, gy, gz, erp, cfm, sor_w;
@@ -72,6 +76,10 @@ struct StepParams {
     int dyn_max_extra;   // extra sweeps allowed after max_iters
     float dyn_premature, dyn_extra_delta;
     int nislands;        // islands of the drop-in world (body_island values)
+    // near-callback policy of the batched collide: with nmat > 0 a contact of
+    // geoms g1 < g2 takes surface material(g1) * nmat + material(g2); with
+    // cat_filter a pair needs (cat1 & col2) | (cat2 & col1) != 0
+    int nmat, cat_filter;
 };
 
 // Independent world ranges a batch may be split into (one stream each).
@@ -107,7 +115,8 @@ struct DevWorlds {
     // threshold^2, angular threshold^2), (max angular speed, DAMP_* bits,
     // linear threshold, angular threshold)
     const float4 *damp;
-    const float4 *geom0, *geom1, *gpos, *gquat;        // [NG] cls/body/p0/p1, p2/p3, static pose
+    const float4 *geom0, *geom1, *gpos, *gquat;        // [NG] cls/body/p0/p1, p2/p3/material/category bits,
+                                                       // static pos + collide bits, static q
     const float *balls;                                // [NBALL*16]
     const Surface *surf;
     float *joints;          // [joint cap * 16]

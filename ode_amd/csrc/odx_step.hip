@@ -573,7 +573,13 @@ __device__ __forceinline__ int prepare_world(const DevWorlds &D, const StepParam
                 while ((gi + 1) * (2 * ng - gi - 2) / 2 <= k) gi++;
                 gj = k - gi * (2 * ng - gi - 1) / 2 + gi + 1;
                 const int bi = s_gb[gi], bj = s_gb[gj];
-                if ((bi >= 0 || bj >= 0) && !(bi == bj)) {
+                bool bits = true;
+                if (P.cat_filter) {  // collideAABBs' category test (collision_space_internal.h)
+                    const unsigned ci = (unsigned)D.geom1[g0 + gi].w, li = (unsigned)D.gpos[g0 + gi].w;
+                    const unsigned cj = (unsigned)D.geom1[g0 + gj].w, lj = (unsigned)D.gpos[g0 + gj].w;
+                    bits = ((ci & lj) | (cj & li)) != 0u;
+                }
+                if ((bi >= 0 || bj >= 0) && !(bi == bj) && bits) {
                     const float *x = s_ab[gi], *y = s_ab[gj];
                     if (P.space == 0 && (x[1] == kInf || y[1] == kInf)) ok = true;
                     else ok = !(x[0] > y[1] || x[1] < y[0] || x[2] > y[3] || x[3] < y[2] || x[4] > y[5] || x[5] < y[4]);
@@ -602,7 +608,7 @@ __device__ __forceinline__ int prepare_world(const DevWorlds &D, const StepParam
         ncont = 0;
         for (int base = 0; base < npairs; base += 64) {
             const int k = base + t;
-            int n = 0, gi = 0, gj = 0;
+            int n = 0, gi = 0, gj = 0, si = 0;
             DContact c[ODX_MAXC];
             if (k < npairs) {
                 gi = (int)(s_pairs[k] >> 16);
@@ -610,7 +616,12 @@ __device__ __forceinline__ int prepare_world(const DevWorlds &D, const StepParam
                 DGeom A, B;
                 load_geom(D, g0 + gi, b0, A);
                 load_geom(D, g0 + gj, b0, B);
-                n = dcollide(A, B, min(P.maxc, ODX_MAXC), c);
+                int maxc = P.maxc;
+                if (P.nmat > 0) {  // the material pair's surface and contact limit
+                    si = (int)D.geom1[g0 + gi].z * P.nmat + (int)D.geom1[g0 + gj].z;
+                    if (D.surf[si].max_contacts > 0) maxc = D.surf[si].max_contacts;
+                }
+                n = dcollide(A, B, min(maxc, ODX_MAXC), c);
             }
             const int incl = wave_incl_scan(n);
             const int off = incl - n;
@@ -625,7 +636,7 @@ __device__ __forceinline__ int prepare_world(const DevWorlds &D, const StepParam
                     o4[0] = make_float4(0.f, (float)bb1, (float)bb2, c[q].pos[0]);
                     o4[1] = make_float4(c[q].pos[1], c[q].pos[2], sg * c[q].normal[0], sg * c[q].normal[1]);
                     o4[2] = make_float4(sg * c[q].normal[2], c[q].depth, 0.f, 0.f);
-                    o4[3] = make_float4(0.f, 0.f, (float)gi, (float)gj);
+                    o4[3] = make_float4(0.f, (float)si, (float)gi, (float)gj);
                 }
             }
             ncont += wave_bcast_last(incl);

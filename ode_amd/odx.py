@@ -45,6 +45,7 @@ def lib():
         L.odxBatchDestroy.argtypes = [vp]
         L.odxBatchSetSolver.argtypes = [vp, i, i, ctypes.c_float, i, i]
         L.odxBatchSetSolveOptions.argtypes = [vp, i, i]
+        L.odxBatchSetSurfaceTable.argtypes = [vp, i, fp, i]
         L.odxBatchSetSplits.argtypes = [vp, i, i]
         L.odxBatchSetFused.argtypes = [vp, i]
         L.odxBatchStep.argtypes = [vp, i]
@@ -176,6 +177,18 @@ class Batch:
         """restart_after: sweeps after which an unconverged warm start restarts
         cold (0 = never); lds_rows: row budget of the LDS-resident sweeps."""
         _check(lib().odxBatchSetSolveOptions(self.h, restart_after, lds_rows), "odxBatchSetSolveOptions")
+
+    def set_surface_table(self, table, category_filter=False):
+        """Near-callback policy on the device: table [nmat, nmat, 16] (rows as
+        include/odx.h's odxBatchSetSurfaceTable), geoms' materials in slot 13
+        and category / collide bits in slots 14 / 15; None returns to the
+        parameters' surface."""
+        if table is None:
+            _check(lib().odxBatchSetSurfaceTable(self.h, 0, None, 0), "odxBatchSetSurfaceTable")
+            return
+        t = np.ascontiguousarray(table, np.float32)
+        _check(lib().odxBatchSetSurfaceTable(self.h, t.shape[0], _p(t), int(category_filter)),
+               "odxBatchSetSurfaceTable")
 
     def step(self, n=1):
         _check(lib().odxBatchStep(self.h, n), "odxBatchStep")

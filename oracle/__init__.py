@@ -167,9 +167,31 @@ def quickstep(bodies, joints, prm, dtype=np.float64):
     return bodies, lam[:m]
 
 
-def world_step(bodies, geoms, prm, balls=None, dtype=np.float32, ccap=None):
+def surface_table(nmat, entries, default=None):
+    """A symmetric material-pair surface table (double[nmat * nmat][16]):
+    entries maps (m1, m2) to a dict of surface fields (mode, mu, mu2, bounce,
+    bounce_vel, soft_erp, soft_cfm, motion1, motion2, motionN, slip1, slip2,
+    max_contacts); unspecified pairs take `default` (the parameters' surface
+    when None)."""
+    fields = ["mode", "mu", "mu2", "bounce", "bounce_vel", "soft_erp", "soft_cfm", "motion1", "motion2",
+              "motionN", "slip1", "slip2", "max_contacts"]
+    base = {"mode": 0x7000, "mu": 0.5}
+    base.update(default or {})
+    t = np.zeros((nmat, nmat, 16), np.float64)
+    for a in range(nmat):
+        for b in range(nmat):
+            e = dict(base)
+            e.update(entries.get((a, b), entries.get((b, a), {})))
+            for k, v in e.items():
+                t[a, b, fields.index(k)] = v
+    return t
+
+
+def world_step(bodies, geoms, prm, balls=None, dtype=np.float32, ccap=None, table=None, category_filter=False):
     """One canonical world step; returns (new_bodies, contacts[nc,12], lambda[m]).
-    ccap bounds the contacts (default: 4 per geom pair)."""
+    ccap bounds the contacts (default: 4 per geom pair).  table: a
+    surface_table; geoms then carry their material in slot 13 (and category /
+    collide bits in slots 14 / 15, tested when category_filter)."""
     bodies = np.ascontiguousarray(bodies, dtype).copy()
     geoms = np.ascontiguousarray(geoms, dtype)
     balls = np.zeros((0, JOINT), dtype) if balls is None else np.ascontiguousarray(balls, dtype)
@@ -179,6 +201,13 @@ def world_step(bodies, geoms, prm, balls=None, dtype=np.float32, ccap=None):
     contacts = np.zeros((ccap, CONTACT), dtype)
     lam = np.zeros(3 * ccap + 3 * len(balls) + 8, dtype)
     m = ctypes.c_int(0)
+    if table is not None:
+        t = np.ascontiguousarray(table, np.float64)
+        nc = getattr(lib(), "orc_world_step_table_" + _real(dtype))(
+            ctypes.c_int(len(bodies)), _p(bodies), ctypes.c_int(ng), _p(geoms), ctypes.c_int(len(balls)), _p(balls),
+            _p(np.ascontiguousarray(prm, np.float64)), ctypes.c_int(t.shape[0]), _p(t), ctypes.c_int(int(category_filter)),
+            _p(contacts), ctypes.c_int(ccap), _p(lam), ctypes.c_int(len(lam)), ctypes.byref(m))
+        return bodies, contacts[:nc], lam[: m.value]
     nc = getattr(lib(), "orc_world_step_" + _real(dtype))(
         ctypes.c_int(len(bodies)), _p(bodies), ctypes.c_int(ng), _p(geoms), ctypes.c_int(len(balls)), _p(balls),
         _p(np.ascontiguousarray(prm, np.float64)), _p(contacts), ctypes.c_int(ccap), _p(lam), ctypes.c_int(len(lam)),

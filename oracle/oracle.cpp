@@ -1126,6 +1126,42 @@ template <class R> struct Params {
     bool random_reorder;  /* REORDERING_METHOD__RANDOMLY, frequency 8 (quickstep.cpp:54-85) */
 };
 
+/* The near-callback policy as a table (what a program's callback decides
+ * per pair before dJointCreateContact, e.g. ode/demo/demo_boxstack.cpp:
+ * 131-160): each geom carries a material (GEOM slot 13) and category / collide
+ * bits (slots 14, 15); a pair passes the bit test of collideAABBs
+ * (collision_space_internal.h) when the filter is on, and its contacts take
+ * the surface of the material pair, at most that entry's contact count.
+ * Table rows (double[16]): 0 mode, 1 mu, 2 mu2, 3 bounce, 4 bounce_vel,
+ * 5 soft_erp, 6 soft_cfm, 7 motion1, 8 motion2, 9 motionN, 10 slip1, 11 slip2,
+ * 12 max contacts (0: the parameters' count). */
+template <class R> struct SurfaceTable {
+    int nmat = 0;
+    bool filter = false;
+    std::vector<Surface<R>> s;
+    std::vector<int> maxc;
+};
+
+template <class R> SurfaceTable<R> read_table(int nmat, const double *t, int filter) {
+    SurfaceTable<R> T;
+    T.nmat = nmat;
+    T.filter = filter != 0;
+    for (int k = 0; k < nmat * nmat; k++) {
+        const double *e = t + (size_t)k * 16;
+        Surface<R> x;
+        x.mode = (int)e[0];
+        x.mu = (R)e[1]; x.mu2 = (R)e[2]; x.bounce = (R)e[3]; x.bounce_vel = (R)e[4];
+        x.soft_erp = (R)e[5]; x.soft_cfm = (R)e[6];
+        x.motion1 = (R)e[7]; x.motion2 = (R)e[8]; x.motionN = (R)e[9];
+        x.slip1 = (R)e[10]; x.slip2 = (R)e[11];
+        T.s.push_back(x);
+        T.maxc.push_back((int)e[12]);
+    }
+    return T;
+}
+
+template <class R> inline int geom_material(const R *geoms, int g) { return (int)geoms[(size_t)g * ORC_GEOM_STRIDE + 13]; }
+
 template <class R> Params<R> read_params(const double *P) {
     Params<R> p;
     p.h = (R)P[ORC_P_H];
@@ -1560,7 +1596,8 @@ template <class R> int quickstep_api(int nb, R *bodies, int nj, const R *joints,
  * order, dCollide with max contacts, one contact joint per contact attached to
  * (body(g1), body(g2)) -- the loop of e.g. ode/demo/demo_space_stress.cpp:76-107. */
 template <class R> int gen_contacts(int nb, const R *bodies, int ng, const R *geoms, int nball, const R *balls,
-                                    const Params<R> &p, std::vector<Contact<R>> &out) {
+                                    const Params<R> &p, std::vector<Contact<R>> &out,
+                                    const SurfaceTable<R> *tab = nullptr) {
     std::vector<GeomView<R>> gv(ng);
     std::vector<R> aabb((size_t)ng * 6);
     std::vector<int> body(ng);
@@ -1571,8 +1608,16 @@ template <class R> int gen_contacts(int nb, const R *bodies, int ng, const R *ge
     }
     int cap = ng * ng + 16;
     std::vector<int> pairs((size_t)cap * 2);
-    int np = (p.space == 1) ? pairs_hash(ng, aabb.data(), body.data(), nullptr, nullptr, -3, 10, pairs.data(), cap)
-                            : pairs_sap(ng, aabb.data(), body.data(), nullptr, nullptr, pairs.data(), cap);
+    std::vector<unsigned> cat, col;
+    if (tab && tab->filter) {
+        for (int i = 0; i < ng; i++) {
+            cat.push_back((unsigned)geoms[(size_t)i * ORC_GEOM_STRIDE + 14]);
+            col.push_back((unsigned)geoms[(size_t)i * ORC_GEOM_STRIDE + 15]);
+        }
+    }
+    const unsigned *pc = cat.empty() ? nullptr : cat.data(), *pl = col.empty() ? nullptr : col.data();
+    int np = (p.space == 1) ? pairs_hash(ng, aabb.data(), body.data(), pc, pl, -3, 10, pairs.data(), cap)
+                            : pairs_sap(ng, aabb.data(), body.data(), pc, pl, pairs.data(), cap);
     std::vector<std::pair<int, int>> cp(np);
     for (int k = 0; k < np; k++) {
         int a = pairs[2 * k], b = pairs[2 * k + 1];
@@ -1592,15 +1637,21 @@ template <class R> int gen_contacts(int nb, const R *bodies, int ng, const R *ge
             }
             if (conn) continue;
         }
+        int maxc = p.maxc;
+        if (tab && tab->nmat) {
+            const int e = tab->maxc[geom_material(geoms, pr.first) * tab->nmat + geom_material(geoms, pr.second)];
+            if (e > 0) maxc = e;
+        }
         Contact<R> c[16];
-        int n = dcollide(a, pr.first, b, pr.second, p.maxc, c);
+        int n = dcollide(a, pr.first, b, pr.second, maxc, c);
         for (int i = 0; i < n; i++) out.push_back(c[i]);
     }
     return (int)out.size();
 }
 
 template <class R> void build_joints(const R *geoms, int nball, const R *balls, const std::vector<Contact<R>> &cs,
-                                     const Params<R> &p, std::vector<Joint<R>> &js) {
+                                     const Params<R> &p, std::vector<Joint<R>> &js,
+                                     const SurfaceTable<R> *tab = nullptr) {
     js.clear();
     for (int k = 0; k < nball; k++) {
         Joint<R> j;
@@ -1616,22 +1667,24 @@ template <class R> void build_joints(const R *geoms, int nball, const R *balls, 
         rec[9] = c.depth;
         Joint<R> j;
         read_joint(rec, p, j);
+        if (tab && tab->nmat) j.s = tab->s[geom_material(geoms, c.g1) * tab->nmat + geom_material(geoms, c.g2)];
         js.push_back(j);
     }
 }
 
 template <class R> int world_step(int nb, R *bodies, int ng, const R *geoms, int nball, const R *balls, const double *P,
-                                  R *contacts, int ccap, R *lambda, int lcap, int *m_out) {
+                                  R *contacts, int ccap, R *lambda, int lcap, int *m_out,
+                                  const SurfaceTable<R> *tab = nullptr) {
     Params<R> p = read_params<R>(P);
     std::vector<Contact<R>> cs;
-    int nc = gen_contacts(nb, bodies, ng, geoms, nball, balls, p, cs);
+    int nc = gen_contacts(nb, bodies, ng, geoms, nball, balls, p, cs, tab);
     for (int i = 0; i < nc && i < ccap; i++) {
         R *o = contacts + (size_t)i * ORC_CONTACT_STRIDE;
         for (int k = 0; k < 3; k++) { o[k] = cs[i].pos[k]; o[3 + k] = cs[i].normal[k]; }
         o[6] = cs[i].depth; o[7] = (R)cs[i].g1; o[8] = (R)cs[i].g2; o[9] = -1; o[10] = -1; o[11] = 0;
     }
     std::vector<Joint<R>> js;
-    build_joints(geoms, nball, balls, cs, p, js);
+    build_joints(geoms, nball, balls, cs, p, js, tab);
     int m = quickstep(nb, bodies, js, p, lambda, lcap);
     if (m_out) *m_out = m;
     return nc;
@@ -1773,6 +1826,16 @@ int orc_world_step_f64(int nb, double *bodies, int ng, const double *geoms, int 
                        double *contacts, int ccap, double *lambda, int lcap, int *m_out) {
     return world_step(nb, bodies, ng, geoms, nball, balls, params, contacts, ccap, lambda, lcap, m_out);
 }
+
+#define ORC_WORLD_STEP_TABLE(SUF, R)                                                                           \
+    int orc_world_step_table_##SUF(int nb, R *bodies, int ng, const R *geoms, int nball, const R *balls,        \
+                                   const double *params, int nmat, const double *table, int filter,            \
+                                   R *contacts, int ccap, R *lambda, int lcap, int *m_out) {                   \
+        const SurfaceTable<R> T = read_table<R>(nmat, table, filter);                                          \
+        return world_step(nb, bodies, ng, geoms, nball, balls, params, contacts, ccap, lambda, lcap, m_out, &T); \
+    }
+ORC_WORLD_STEP_TABLE(f32, float)
+ORC_WORLD_STEP_TABLE(f64, double)
 
 /* contacts here are JOINT records [16] (type 0, b1, b2, pos, normal, depth). */
 double orc_lcp_residual(int nb, const double *bodies, int nball, const double *balls, int nc, const double *contacts,

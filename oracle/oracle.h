@@ -115,6 +115,16 @@ int orc_world_step_f32(int nb, float *bodies, int ng, const float *geoms, int nb
 int orc_world_step_f64(int nb, double *bodies, int ng, const double *geoms, int nball,
                        const double *balls, const double *params, double *contacts,
                        int contact_cap, double *lambda, int lambda_cap, int *m_out);
+/* world_step with the near-callback policy as a table: materials and
+ * category / collide bits from GEOM slots 13-15, a surface and a contact
+ * limit per material pair (table: double[nmat * nmat][16], rows as in
+ * oracle.cpp's SurfaceTable), the category test when filter != 0. */
+int orc_world_step_table_f32(int nb, float *bodies, int ng, const float *geoms, int nball, const float *balls,
+                             const double *params, int nmat, const double *table, int filter, float *contacts,
+                             int ccap, float *lambda, int lcap, int *m_out);
+int orc_world_step_table_f64(int nb, double *bodies, int ng, const double *geoms, int nball, const double *balls,
+                             const double *params, int nmat, const double *table, int filter, double *contacts,
+                             int ccap, double *lambda, int lcap, int *m_out);
 
 /* LCP residual (double precision) of lambda for the world state *before* the
  * step (bodies as they were at step start) and the given contacts.  The rows

@@ -370,3 +370,62 @@ def test_split_streams_fused_kernel_and_multi_step_graphs_change_nothing(warm):
             A, Bf = out[0][3], out[k][3]
             assert A.shape == Bf.shape
             np.testing.assert_array_equal(np.sort(A, axis=0), np.sort(Bf, axis=0))
+
+
+def test_surface_table_and_category_bits_match_oracle():
+    # the near-callback policy on the device: each geom's material picks the
+    # pair's surface (friction, bounce, soft CFM, mode, contact limit) and its
+    # category / collide bits decide whether the pair collides at all
+    say("case: material-pair surface table with category bits")
+    rng = np.random.default_rng(77)
+    worlds = [S.uneven_world(rng, 8, 20) for _ in range(10)]
+    for w in worlds:
+        w.bodies[:, 2] = np.minimum(w.bodies[:, 2], 0.35)
+        w.bodies[:, 8:11] = rng.normal(0, 0.5, size=(len(w.bodies), 3))
+        body = w.geoms[:, 1] >= 0
+        w.geoms[body, 13] = rng.integers(1, 3, size=int(body.sum()))
+        w.geoms[~body, 13] = 0
+        grp = rng.integers(0, 2, size=len(w.geoms))
+        w.geoms[:, 14] = np.where(grp == 0, 1, 2)        # category: group bit
+        w.geoms[:, 15] = np.where(grp == 0, 1 | 4, 2 | 4)  # collide: own group and the ground's bit
+        w.geoms[~body, 14] = 4
+        w.geoms[~body, 15] = 1 | 2
+    approx = 0x1000 | 0x2000
+    t = O.surface_table(3, {
+        (0, 1): {"mode": approx, "mu": 0.2, "max_contacts": 2},
+        (0, 2): {"mode": approx | 0x004 | 0x010, "mu": 0.8, "bounce": 0.5, "bounce_vel": 0.1, "soft_cfm": 1e-3},
+        (1, 1): {"mode": approx, "mu": 0.0, "max_contacts": 1},
+        (1, 2): {"mode": approx | 0x008, "mu": 0.5, "soft_erp": 0.5, "max_contacts": 3},
+        (2, 2): {"mode": 0x001 | approx, "mu": 0.4, "mu2": 0.1},
+    })
+    prm = O.params()
+    odx = _odx()
+    B = odx.Batch(worlds, prm)
+    B.set_surface_table(t, category_filter=True)
+    B.set_solver(order=0, max_iters=int(prm[7]), tol=0.0)
+    B.step(1)
+    gb = B.bodies()
+    st = B.stats()
+    assert B.overflow().sum() == 0
+    dropped = 0
+    for wi, w in enumerate(worlds):
+        ref_b, ref_c, ref_l = O.world_step(w.bodies, w.geoms, prm, dtype=np.float64, table=t, category_filter=True)
+        _, all_c, _ = O.world_step(w.bodies, w.geoms, prm, dtype=np.float64)
+        dropped += len(all_c) - len(ref_c)
+        gj = B.joints(wi)
+        assert st["contacts"][wi] == len(ref_c), "contact count differs in world %d" % wi
+        np.testing.assert_array_equal(gj[:, 14:16].astype(int), ref_c[:, 7:9].astype(int))
+        # the surface index of each contact is the material pair's entry
+        mat = w.geoms[:, 13].astype(int)
+        np.testing.assert_array_equal(gj[:, 13].astype(int), mat[ref_c[:, 7].astype(int)] * 3 + mat[ref_c[:, 8].astype(int)])
+        lam = B.lam(wi)
+        assert len(lam) == len(ref_l)
+        assert relerr(lam, ref_l) < TOL_LAMBDA, "lambda world %d: %g" % (wi, relerr(lam, ref_l))
+        wb = B.world_bodies(wi, gb)
+        assert relerr(wb[:, 0:3], ref_b[:, 0:3]) < TOL_STATE
+        vs = np.hstack([wb[:, 8:11], wb[:, 12:15]])
+        vr = np.hstack([ref_b[:, 8:11], ref_b[:, 12:15]])
+        assert np.max(np.abs(vs - vr)) / max(np.max(np.abs(vr)), 1.0) < TOL_STATE
+    say("  contacts dropped by the policy (bits and limits):", dropped)
+    assert dropped > 0
+    B.close()

@@ -205,3 +205,101 @@ def test_island_pass_re_enables_a_sleeping_body_touched_by_an_awake_one():
     stepped = AD.enabled_after_islands(bodies, [(0, -1), (1, 0), (2, -1)])
     assert stepped == [0, 1]
     assert [b.enabled for b in bodies] == [True, True, False]
+
+
+# ------------------------------------------- near-callback policy as a table
+
+def _tagged(g, material=0, cat=None, col=None):
+    g = np.array(g, np.float64)
+    g[13] = material
+    if cat is not None:
+        g[14], g[15] = cat, col
+    return g
+
+
+def test_table_of_the_parameters_surface_changes_nothing():
+    from ode_amd import scenes as S
+    rng = np.random.default_rng(5)
+    w = S.uneven_world(rng, 10, 10)
+    w.bodies[:, 2] = np.minimum(w.bodies[:, 2], 0.3)
+    prm = O.params()
+    ref, c, lam = O.world_step(w.bodies, w.geoms, prm, dtype=np.float64)
+    t = O.surface_table(1, {})
+    got, c2, lam2 = O.world_step(w.bodies, w.geoms, prm, dtype=np.float64, table=t)
+    np.testing.assert_array_equal(c2, c)
+    np.testing.assert_array_equal(lam2, lam)
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_category_bits_decide_which_pairs_collide():
+    # collideAABBs: a pair passes when (cat1 & col2) || (cat2 & col1)
+    from ode_amd import scenes as S
+    m, I = S.sphere_mass(1000.0, 0.5)
+    bodies = [S._body((0, 0, 0), (1, 0, 0, 0), m, I), S._body((0.8, 0, 0), (1, 0, 0, 0), m, I)]
+    prm = O.params(gravity=(0.0, 0.0, 0.0))
+    t = O.surface_table(1, {})
+    cases = [((1, 2), (4, 8), 0), ((1, 2), (2, 8), 1), ((1, 2), (4, 1), 1), ((3, 0), (0, 3), 1), ((3, 0), (3, 0), 0),
+             ((5, 5), (5, 5), 1)]
+    for (c1, k1), (c2, k2), want in cases:
+        geoms = [_tagged(S.geom(S.SPHERE, 0, (0.5,)), 0, c1, k1), _tagged(S.geom(S.SPHERE, 1, (0.5,)), 0, c2, k2)]
+        _, cs, _ = O.world_step(np.array(bodies), np.array(geoms), prm, dtype=np.float64, table=t,
+                                category_filter=True)
+        assert len(cs) == want, ((c1, k1), (c2, k2))
+        # with the filter off every overlapping pair collides
+        _, cs, _ = O.world_step(np.array(bodies), np.array(geoms), prm, dtype=np.float64, table=t)
+        assert len(cs) == 1
+
+
+def test_contact_limit_per_material_pair():
+    # a box lying flat on the plane touches it at 4 corners; the pair's
+    # entry caps that at 1, 2, 3 or the parameters' 4
+    from ode_amd import scenes as S
+    m, I = S.box_mass(1000.0, 1.0, 1.0, 1.0)
+    bodies = np.array([S._body((0, 0, 0.49), (1, 0, 0, 0), m, I)])
+    for cap, want in ((1, 1), (2, 2), (3, 3), (0, 4)):
+        geoms = np.array([_tagged(S.geom(S.BOX, 0, (1.0, 1.0, 1.0)), 1), _tagged(S.plane(0, 0, 1, 0), 0)])
+        t = O.surface_table(2, {(0, 1): {"max_contacts": cap}})
+        _, cs, _ = O.world_step(bodies, geoms, O.params(), dtype=np.float64, table=t)
+        assert len(cs) == want
+
+
+def test_friction_follows_each_material_pair():
+    # two solid spheres sliding at 1 m/s on the plane: material 1 meets the
+    # ground frictionless (the speed stays exactly 1: the normal row has no x
+    # or angular part), material 2 with a friction bound far above what
+    # rolling needs (it starts to roll: v = v0 / (1 + 2/5) = 5/7 m/s)
+    from ode_amd import scenes as S
+    m, I = S.sphere_mass(1000.0, 0.5)
+    bodies = np.array([S._body((0, 0, 0.49), (1, 0, 0, 0), m, I, lvel=(1.0, 0, 0)),
+                       S._body((5, 0, 0.49), (1, 0, 0, 0), m, I, lvel=(1.0, 0, 0))])
+    geoms = np.array([_tagged(S.geom(S.SPHERE, 0, (0.5,)), 1), _tagged(S.geom(S.SPHERE, 1, (0.5,)), 2),
+                      _tagged(S.plane(0, 0, 1, 0), 0)])
+    prm = O.params(iters=200, w=1.0, cfm=0.0)  # rigid rows: the world CFM would soften the friction impulse
+    t = O.surface_table(3, {(0, 1): {"mu": 0.0}, (0, 2): {"mu": 1e6}})
+    out, cs, _ = O.world_step(bodies, geoms, prm, dtype=np.float64, table=t)
+    assert len(cs) == 2
+    assert out[0, 8] == 1.0 and out[0, 13] == 0.0
+    assert out[1, 8] == pytest.approx(5.0 / 7.0, rel=1e-9)
+    assert out[1, 13] == pytest.approx(out[1, 8] / 0.5, rel=1e-9)  # rolling: w r = v
+    # swapping the entries swaps the outcome
+    t = O.surface_table(3, {(0, 1): {"mu": 1e6}, (0, 2): {"mu": 0.0}})
+    out, _, _ = O.world_step(bodies, geoms, prm, dtype=np.float64, table=t)
+    assert out[0, 8] == pytest.approx(5.0 / 7.0, rel=1e-9) and out[1, 8] == 1.0
+
+
+def test_bounce_of_a_material_pair_reverses_the_approach_speed():
+    # a sphere touching the plane (depth 0) at -2 m/s with bounce 1 and
+    # bounce_vel 0 leaves at +2 m/s (dContactBounce, contact.cpp getInfo2);
+    # the same sphere on a pair without bounce stops
+    from ode_amd import scenes as S
+    m, I = S.sphere_mass(1000.0, 0.5)
+    bodies = np.array([S._body((0, 0, 0.5), (1, 0, 0, 0), m, I, lvel=(0, 0, -2.0)),
+                       S._body((5, 0, 0.5), (1, 0, 0, 0), m, I, lvel=(0, 0, -2.0))])
+    geoms = np.array([_tagged(S.geom(S.SPHERE, 0, (0.5,)), 1), _tagged(S.geom(S.SPHERE, 1, (0.5,)), 2),
+                      _tagged(S.plane(0, 0, 1, 0), 0)])
+    t = O.surface_table(3, {(0, 1): {"mode": 0x7000 | 0x004, "bounce": 1.0, "bounce_vel": 0.0}})
+    out, cs, _ = O.world_step(bodies, geoms, O.params(gravity=(0.0, 0.0, 0.0), iters=100, w=1.0, cfm=0.0),
+                              dtype=np.float64, table=t)
+    assert len(cs) == 2
+    assert out[0, 10] == pytest.approx(2.0, rel=1e-9)
+    assert abs(out[1, 10]) < 1e-9
