@@ -237,7 +237,7 @@ def bench_boxes(args):
         path = f.name
     try:
         res = subprocess.run([exe, path, str(args.warmup), str(args.steps), str(args.large_iters), "0.5",
-                              str(args.row_order), "0"],
+                              str(args.row_order), "0", str(args.sweep_mode)],
                              capture_output=True, text=True, timeout=900)
     finally:
         os.unlink(path)
@@ -321,6 +321,8 @@ def main():
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--row-order", type=int, default=0,
                     help="boxes workload: 0 the reference's row order, 1 colour phases")
+    ap.add_argument("--sweep-mode", type=int, default=0,
+                    help="boxes workload: 0 automatic, 1 workgroup-wide levels, 2 one wave (odxWorldSetSweepMode)")
     ap.add_argument("--bodies", type=int, default=None, help="bodies of the large (200,000) or boxes (1,000) workload")
     ap.add_argument("--large-iters", type=int, default=None,
                     help="sweeps per step of the large (30) or boxes (20) workload")

@@ -280,6 +280,12 @@ int dRandInt(int n);                                         /* misc.h:51 */
  * dRand state of dRandSetSeed island by island.  Applies to subsequent steps
  * of the world. */
 void odxWorldSetRowOrder(dWorldID w, int order);
+/* How the step kernel sweeps the dependency levels of rows (no counterpart in
+ * the reference, which sweeps on one CPU thread): 0 = automatic (one wave
+ * when levels hold at most 64 items on average; default), 1 = the whole
+ * workgroup per level, 2 = one wave.  The Gauss-Seidel order and the results
+ * are the same in every mode. */
+void odxWorldSetSweepMode(dWorldID w, int mode);
 /* Sweeps / LCP residual of the last dWorldQuickStep (residual < 0 when no rows). */
 void odxWorldGetSolveStats(dWorldID w, int *sweeps, float *residual, int *rows);
 

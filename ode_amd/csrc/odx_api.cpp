@@ -71,6 +71,57 @@ template <class T> bool down(T *h, const DevBuf &b, size_t n) {
     return odx_check(hipMemcpyAsync(h, b.p, n * sizeof(T), hipMemcpyDeviceToHost, g_stream), "hipMemcpyAsync D2H");
 }
 
+// Page-locked host memory, grown on demand.
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    bool ensure(size_t bytes) {
+        if (bytes <= cap && p) return true;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = std::max<size_t>(bytes, 256);
+        want = want + want / 2;
+        if (!odx_check(hipHostMalloc(&p, want, 0), "hipHostMalloc")) return false;
+        cap = want;
+        return true;
+    }
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+// One call's transfers packed into a page-locked host block and its device
+// twin, in 256-byte aligned slots: the inputs go up in one copy and the
+// outputs, laid out first, come back in one.  Every user synchronises its
+// stream before returning, so no copy is in flight when a block is reused.
+struct Staging {
+    DevBuf dev;
+    PinnedBuf host;
+    size_t used = 0;
+    void reset() { used = 0; }
+    size_t slot(size_t bytes) {
+        const size_t off = (used + 255) & ~(size_t)255;
+        used = off + bytes;
+        return off;
+    }
+    bool alloc() { return host.ensure(used) && dev.ensure(used); }
+    template <class T> T *h(size_t off) const { return reinterpret_cast<T *>(host.as<char>() + off); }
+    template <class T> T *d(size_t off) const { return reinterpret_cast<T *>(dev.as<char>() + off); }
+    template <class T> void put(size_t off, const T *src, size_t n) {
+        if (n) std::memcpy(host.as<char>() + off, src, n * sizeof(T));
+    }
+    bool send(size_t off, size_t bytes) {
+        return bytes == 0 || odx_check(hipMemcpyAsync(dev.as<char>() + off, host.as<char>() + off, bytes,
+                                                      hipMemcpyHostToDevice, g_stream), "hipMemcpyAsync H2D");
+    }
+    bool fetch(size_t off, size_t bytes) {
+        return bytes == 0 || odx_check(hipMemcpyAsync(host.as<char>() + off, dev.as<char>() + off, bytes,
+                                                      hipMemcpyDeviceToHost, g_stream), "hipMemcpyAsync D2H");
+    }
+};
+
 void rfromq(float *R, const float *q) {
     float qq1 = 2 * q[1] * q[1], qq2 = 2 * q[2] * q[2], qq3 = 2 * q[3] * q[3];
     R[0] = 1 - qq2 - qq3; R[1] = 2 * (q[1] * q[2] - q[0] * q[3]); R[2] = 2 * (q[1] * q[3] + q[0] * q[2]); R[3] = 0;
@@ -223,11 +274,13 @@ struct dxWorld {
     DampingParams dampingp;
     float max_angular_speed = dInfinity;
     int order = 0;
+    int sweep_mode = 0;  // odxWorldSetSweepMode
     int last_sweeps = 0, last_rows = 0;
     float last_res = -1.f;
-    // device buffers of the step
-    DevBuf pos, quat, lvel, avel, facc, tacc, mass, Ib, invIb, joints, surf, joff, rows, lambda, sched, phase, bscr,
-        counts, fb, bstart, residual, niters, scratch, island, segtab, isl_sweeps, damp;
+    // device buffers of the step: rows, impulses and body scratch stay on the
+    // device; everything else travels in `stage`
+    DevBuf rows, lambda, bscr;
+    Staging stage;  // the step's inputs and outputs (one copy each way)
 };
 
 // Extra sweeps of the dynamic iteration count are capped here (the reference
@@ -325,7 +378,11 @@ struct Session {
     int maxc = 0;
     std::vector<float> contacts;  // [pair][4][7]
     std::vector<int> counts;
-    DevBuf gviews, aabb, bits, dpairs, cnt, out, outcnt;
+    Staging space;                // geom views, bits and the counter in; counter and AABBs out
+    Staging coll;                 // pairs in; contact counts and contacts out
+    PinnedBuf hpairs;             // candidate pairs read back
+    DevBuf dpairs;                // candidate pairs (device)
+    const float *d_gviews = nullptr;
 };
 Session g_sess;
 
@@ -340,26 +397,22 @@ KernelTiming g_ktime;
 
 bool gpu_collide_pairs(Session &S, int maxc) {
     const int np = (int)S.pairs.size() / 2;
-    if (!S.dpairs.ensure((size_t)np * 2 * sizeof(int)) || !S.out.ensure((size_t)np * ODX_MAXC * 7 * sizeof(float)) ||
-        !S.outcnt.ensure((size_t)np * sizeof(int)))
-        return false;
+    Staging &C = S.coll;
+    C.reset();
+    const size_t o_cnt = C.slot((size_t)np * sizeof(int));
+    const size_t o_out = C.slot((size_t)np * ODX_MAXC * 7 * sizeof(float));
+    const size_t out_end = C.used;
+    const size_t o_pairs = C.slot((size_t)np * 2 * sizeof(int));
+    if (!C.alloc()) return false;
     if (np > 0) {
-        if (!odx_check(hipMemcpyAsync(S.dpairs.p, S.pairs.data(), (size_t)np * 2 * sizeof(int), hipMemcpyHostToDevice,
-                                      g_stream), "hipMemcpyAsync"))
-            return false;
-        launch_collide_pairs(S.gviews.as<float>(), S.dpairs.as<int>(), np, maxc, S.out.as<float>(), S.outcnt.as<int>(),
-                             g_stream);
-        if (!odx_check(hipGetLastError(), "k_collide_pairs")) return false;
+        C.put(o_pairs, S.pairs.data(), (size_t)np * 2);
+        if (!C.send(o_pairs, (size_t)np * 2 * sizeof(int))) return false;
+        launch_collide_pairs(S.d_gviews, C.d<int>(o_pairs), np, maxc, C.d<float>(o_out), C.d<int>(o_cnt), g_stream);
+        if (!odx_check(hipGetLastError(), "k_collide_pairs") || !C.fetch(0, out_end)) return false;
     }
-    S.contacts.resize((size_t)np * ODX_MAXC * 7);
-    S.counts.resize(np);
-    if (np > 0 &&
-        (!odx_check(hipMemcpyAsync(S.contacts.data(), S.out.p, S.contacts.size() * sizeof(float), hipMemcpyDeviceToHost,
-                                   g_stream), "hipMemcpyAsync") ||
-         !odx_check(hipMemcpyAsync(S.counts.data(), S.outcnt.p, np * sizeof(int), hipMemcpyDeviceToHost, g_stream),
-                    "hipMemcpyAsync")))
-        return false;
     if (!odx_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize")) return false;
+    S.counts.assign(C.h<int>(o_cnt), C.h<int>(o_cnt) + np);
+    S.contacts.assign(C.h<float>(o_out), C.h<float>(o_out) + (size_t)np * ODX_MAXC * 7);
     S.maxc = maxc;
     return true;
 }
@@ -423,6 +476,7 @@ dReal dWorldGetQuickStepW(dWorldID w) { return w->w; }
 void dWorldSetContactMaxCorrectingVel(dWorldID w, dReal vel) { w->max_vel = vel; }
 void dWorldSetContactSurfaceLayer(dWorldID w, dReal depth) { w->min_depth = depth; }
 void odxWorldSetRowOrder(dWorldID w, int order) { w->order = (order == 1 || order == 2) ? order : 0; }
+void odxWorldSetSweepMode(dWorldID w, int mode) { w->sweep_mode = (mode == 1 || mode == 2) ? mode : 0; }
 void odxWorldGetSolveStats(dWorldID w, int *sweeps, float *residual, int *rows) {
     if (sweeps) *sweeps = w->last_sweeps;
     if (residual) *residual = w->last_res;
@@ -864,28 +918,43 @@ void dSpaceCollide(dSpaceID space, void *data, dNearCallback *callback) {
         }
         const long long T = (long long)n * (n - 1) / 2;
         int cap = (int)std::min<long long>(T, 1 << 26);
-        int count = 0;
-        bool ok = up(S.gviews, gv.data(), gv.size()) && up(S.bits, bits.data(), bits.size()) &&
-                  S.aabb.ensure((size_t)n * 6 * sizeof(float)) && S.cnt.ensure(sizeof(int)) &&
-                  S.dpairs.ensure((size_t)cap * 2 * sizeof(int)) &&
-                  odx_check(hipMemsetAsync(S.cnt.p, 0, sizeof(int), g_stream), "hipMemsetAsync");
+        // one block: geom views and bits up with a zeroed counter; the counter
+        // and the AABBs come back in one copy
+        Staging &G = S.space;
+        G.reset();
+        const size_t o_gv = G.slot(gv.size() * sizeof(float)), o_bits = G.slot(bits.size() * sizeof(unsigned));
+        const size_t o_cnt = G.slot(sizeof(int)), o_ab = G.slot((size_t)n * 6 * sizeof(float));
+        bool ok = G.alloc() && S.dpairs.ensure((size_t)cap * 2 * sizeof(int));
+        if (!ok) return;
+        G.put(o_gv, gv.data(), gv.size());
+        G.put(o_bits, bits.data(), bits.size());
+        *G.h<int>(o_cnt) = 0;
+        S.d_gviews = G.d<float>(o_gv);
+        ok = G.send(0, o_cnt + sizeof(int));
         if (ok) {
-            launch_aabbs(S.gviews.as<float>(), n, S.aabb.as<float>(), g_stream);
-            launch_pairs_all(S.gviews.as<float>(), S.aabb.as<float>(), S.bits.as<unsigned>(), n, space->kind == 0 ? 0 : 1,
-                             S.dpairs.as<int>(), S.cnt.as<int>(), cap, g_stream);
-            ok = odx_check(hipGetLastError(), "k_pairs_all") && down(&count, S.cnt, 1) &&
+            launch_aabbs(S.d_gviews, n, G.d<float>(o_ab), g_stream);
+            launch_pairs_all(S.d_gviews, G.d<float>(o_ab), G.d<unsigned>(o_bits), n, space->kind == 0 ? 0 : 1,
+                             S.dpairs.as<int>(), G.d<int>(o_cnt), cap, g_stream);
+            ok = odx_check(hipGetLastError(), "k_pairs_all") && G.fetch(o_cnt, G.used - o_cnt) &&
                  odx_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize");
         }
         if (!ok) return;
+        int count = *G.h<int>(o_cnt);
         if (count > cap) {
             odx_message(3, "dSpaceCollide: %d candidate pairs exceed the buffer (%d); extra pairs dropped", count, cap);
             count = cap;
         }
         std::vector<int> raw((size_t)count * 2);
-        std::vector<float> ab((size_t)n * 6);
-        if (!down(raw.data(), S.dpairs, raw.size()) || !down(ab.data(), S.aabb, ab.size()) ||
-            !odx_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize"))
-            return;
+        const float *abp = G.h<float>(o_ab);
+        std::vector<float> ab(abp, abp + (size_t)n * 6);
+        if (count > 0) {
+            if (!S.hpairs.ensure(raw.size() * sizeof(int)) ||
+                !odx_check(hipMemcpyAsync(S.hpairs.p, S.dpairs.p, raw.size() * sizeof(int), hipMemcpyDeviceToHost,
+                                          g_stream), "hipMemcpyAsync D2H") ||
+                !odx_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize"))
+                return;
+            std::memcpy(raw.data(), S.hpairs.p, raw.size() * sizeof(int));
+        }
         // order: sweep-and-prune emits finite pairs by the x-min sort order of
         // the first geom then the second (BoxPruning collision_sapspace.cpp:560-621),
         // then infinite x infinite, then infinite x finite (:517-532); the other
@@ -973,7 +1042,13 @@ int dCollide(dGeomID o1, dGeomID o2, int flags, dContactGeom *contact, int skip)
         geom_view(o1, gv.data());
         geom_view(o2, gv.data() + GVIEW_STRIDE);
         one.pairs = {0, 1};
-        if (!up(one.gviews, gv.data(), gv.size()) || !gpu_collide_pairs(one, maxc)) return 0;
+        Staging &G = one.space;
+        G.reset();
+        const size_t o_gv = G.slot(gv.size() * sizeof(float));
+        if (!G.alloc()) return 0;
+        G.put(o_gv, gv.data(), gv.size());
+        one.d_gviews = G.d<float>(o_gv);
+        if (!G.send(o_gv, gv.size() * sizeof(float)) || !gpu_collide_pairs(one, maxc)) return 0;
         n = one.counts[0];
         single = one.contacts;
         res = single.data();
@@ -1354,48 +1429,68 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
         }
     }
 
-    // ---- device
+    // ---- device: every input in one staged copy; the state, the sweep count,
+    // the residual, joint feedback and island sweeps sit first in the block
+    // and come back in one copy
     int bstart[2] = {0, nb};
-    bool ok = up(w->pos, pos.data(), nb) && up(w->quat, quat.data(), nb) && up(w->lvel, lvel.data(), nb) &&
-              up(w->avel, avel.data(), nb) && up(w->facc, facc.data(), nb) && up(w->tacc, tacc.data(), nb) &&
-              up(w->mass, mass.data(), nb) && up(w->Ib, Ib.data(), Ib.size()) && up(w->invIb, invIb.data(), invIb.size()) &&
-              up(w->joints, jrec.data(), jrec.size()) && up(w->surf, surf.data(), surf.size()) &&
-              up(w->joff, joff.data(), joff.size()) && up(w->sched, sched.data(), sched.size()) &&
-              up(w->phase, phase.data(), phase.size()) && up(w->bstart, bstart, 2) &&
-              w->rows.ensure((size_t)std::max(m, 1) * ROW_STRIDE * sizeof(float)) &&
-              w->lambda.ensure((size_t)std::max(m, 1) * sizeof(float)) &&
-              w->bscr.ensure((size_t)std::max(nb, 1) * BSCR_STRIDE * sizeof(float)) &&
-              w->fb.ensure((size_t)std::max(nj, 1) * 12 * sizeof(float)) && w->counts.ensure(4 * sizeof(int)) &&
-              w->residual.ensure(sizeof(float)) && w->niters.ensure(sizeof(int));
-    if (!ok) return 0;
     int cnts[4] = {nj, 0, m, L};  // njoints, ncontacts (unused), nrows, nphases
-    ok = up(w->counts, cnts, 4);
+    const bool dyn_on = [&] {
+        const float xc = (float)w->iterations * w->dyn_extra_factor;
+        const int xm = xc < (float)ODX_MAX_EXTRA_SWEEPS ? (int)xc : ODX_MAX_EXTRA_SWEEPS;
+        return xm != 0 || w->dyn_premature != 0.f;
+    }();
+    Staging &G = w->stage;
+    G.reset();
+    const size_t f4 = (size_t)nb * sizeof(float4);
+    const size_t o_pos = G.slot(f4), o_quat = G.slot(f4), o_lvel = G.slot(f4), o_avel = G.slot(f4);
+    const size_t o_it = G.slot(2 * sizeof(int));
+    const size_t o_fb = G.slot(want_fb ? (size_t)nj * 12 * sizeof(float) : 0);
+    const size_t o_isl = G.slot(w->order == 2 ? (size_t)std::max(nisl, 1) * sizeof(int) : 0);
+    const size_t out_end = G.used;
+    const size_t o_facc = G.slot(f4), o_tacc = G.slot(f4), o_mass = G.slot(f4);
+    const size_t o_Ib = G.slot(Ib.size() * sizeof(float)), o_invIb = G.slot(invIb.size() * sizeof(float));
+    const size_t o_joints = G.slot(jrec.size() * sizeof(float)), o_surf = G.slot(surf.size() * sizeof(Surface));
+    const size_t o_joff = G.slot(joff.size() * sizeof(int)), o_sched = G.slot(sched.size() * sizeof(int));
+    const size_t o_phase = G.slot(phase.size() * sizeof(int)), o_bstart = G.slot(sizeof bstart);
+    const size_t o_counts = G.slot(sizeof cnts), o_damp = G.slot(damp.size() * sizeof(float4));
+    const size_t o_island = G.slot(dyn_on ? island.size() * sizeof(int) : 0);
+    const size_t o_seg = G.slot(seg_table.size() * sizeof(int));
+    bool ok = G.alloc() && w->rows.ensure((size_t)std::max(m, 1) * ROW_STRIDE * sizeof(float)) &&
+              w->lambda.ensure((size_t)std::max(m, 1) * sizeof(float)) &&
+              w->bscr.ensure((size_t)std::max(nb, 1) * BSCR_STRIDE * sizeof(float));
     if (!ok) return 0;
+    G.put(o_pos, pos.data(), nb); G.put(o_quat, quat.data(), nb); G.put(o_lvel, lvel.data(), nb);
+    G.put(o_avel, avel.data(), nb); G.put(o_facc, facc.data(), nb); G.put(o_tacc, tacc.data(), nb);
+    G.put(o_mass, mass.data(), nb); G.put(o_Ib, Ib.data(), Ib.size()); G.put(o_invIb, invIb.data(), invIb.size());
+    G.put(o_joints, jrec.data(), jrec.size()); G.put(o_surf, surf.data(), surf.size());
+    G.put(o_joff, joff.data(), joff.size()); G.put(o_sched, sched.data(), sched.size());
+    G.put(o_phase, phase.data(), phase.size()); G.put(o_bstart, bstart, 2); G.put(o_counts, cnts, 4);
+    G.put(o_damp, damp.data(), damp.size());
+    if (dyn_on) G.put(o_island, island.data(), island.size());
+    G.put(o_seg, seg_table.data(), seg_table.size());
+    if (!G.send(0, G.used)) return 0;
     DevWorlds D{};
     D.nworlds = 1;
-    D.body_start = w->bstart.as<int>();
-    D.pos = w->pos.as<float4>(); D.quat = w->quat.as<float4>(); D.lvel = w->lvel.as<float4>(); D.avel = w->avel.as<float4>();
-    D.facc = w->facc.as<float4>(); D.tacc = w->tacc.as<float4>(); D.mass = w->mass.as<float4>();
-    D.Ib = w->Ib.as<float>(); D.invIb = w->invIb.as<float>();
-    D.surf = w->surf.as<Surface>();
-    D.joints = w->joints.as<float>();
-    D.njoints = w->counts.as<int>();
-    D.ncontacts = w->counts.as<int>() + 1;
-    D.nrows = w->counts.as<int>() + 2;
-    D.nphases = w->counts.as<int>() + 3;
-    D.niters = w->niters.as<int>();
-    D.residual = w->residual.as<float>();
+    D.body_start = G.d<int>(o_bstart);
+    D.pos = G.d<float4>(o_pos); D.quat = G.d<float4>(o_quat); D.lvel = G.d<float4>(o_lvel); D.avel = G.d<float4>(o_avel);
+    D.facc = G.d<float4>(o_facc); D.tacc = G.d<float4>(o_tacc); D.mass = G.d<float4>(o_mass);
+    D.Ib = G.d<float>(o_Ib); D.invIb = G.d<float>(o_invIb);
+    D.surf = G.d<Surface>(o_surf);
+    D.joints = G.d<float>(o_joints);
+    D.njoints = G.d<int>(o_counts);
+    D.ncontacts = G.d<int>(o_counts) + 1;
+    D.nrows = G.d<int>(o_counts) + 2;
+    D.nphases = G.d<int>(o_counts) + 3;
+    D.niters = G.d<int>(o_it);
+    D.residual = G.d<float>(o_it) + 1;
     D.rows = w->rows.as<float>();
     D.lambda = w->lambda.as<float>();
-    D.sched = w->sched.as<int>();
-    D.phase_start = w->phase.as<int>();
+    D.sched = G.d<int>(o_sched);
+    D.phase_start = G.d<int>(o_phase);
     D.bscr = w->bscr.as<float>();
-    D.joff = w->joff.as<int>();
-    D.fb = w->fb.as<float>();
-    if (!damp.empty()) {
-        if (!up(w->damp, damp.data(), damp.size())) return 0;
-        D.damp = w->damp.as<float4>();
-    }
+    D.joff = G.d<int>(o_joff);
+    D.fb = want_fb ? G.d<float>(o_fb) : nullptr;
+    D.damp = damp.empty() ? nullptr : G.d<float4>(o_damp);
     StepParams P{};
     P.h = stepsize;
     P.gx = w->gravity[0]; P.gy = w->gravity[1]; P.gz = w->gravity[2];
@@ -1406,6 +1501,7 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     P.max_vel = w->max_vel; P.min_depth = w->min_depth;
     P.maxc = 4;
     P.order = w->order;
+    P.wave_sweep = w->sweep_mode;
     P.lambda_floor = 1e-3f;
     P.collide = 0;
     P.feedback = want_fb ? 1 : 0;
@@ -1419,16 +1515,12 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
         P.dyn_premature = w->dyn_premature;
         P.dyn_extra_delta = w->dyn_extra_delta;
         P.nislands = nisl;
-        if (P.dyn && !(up(w->island, island.data(), island.size()))) return 0;
-        D.body_island = P.dyn ? w->island.as<int>() : nullptr;
+        D.body_island = P.dyn ? G.d<int>(o_island) : nullptr;
     }
     if (w->order == 2) {
-        if (!up(w->segtab, seg_table.data(), seg_table.size()) ||
-            !w->isl_sweeps.ensure((size_t)std::max(nisl, 1) * sizeof(int)))
-            return 0;
-        D.seg_table = w->segtab.as<int>();
+        D.seg_table = G.d<int>(o_seg);
         D.nseg = nseg;
-        D.island_sweeps = P.dyn ? w->isl_sweeps.as<int>() : nullptr;
+        D.island_sweeps = P.dyn ? G.d<int>(o_isl) : nullptr;
     }
     if (g_ktime.on && !g_ktime.ev[0]) {
         if (!odx_check(hipEventCreate(&g_ktime.ev[0]), "hipEventCreate") ||
@@ -1439,13 +1531,17 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     launch_single_step(D, P, nb, m, (int)sched.size(), L, g_stream);
     if (!odx_check(hipGetLastError(), "k_single_step")) return 0;
     if (g_ktime.on && !odx_check(hipEventRecord(g_ktime.ev[1], g_stream), "hipEventRecord")) return 0;
-    fbh.assign(want_fb ? (size_t)nj * 12 : 0, 0.f);
-    ok = down(pos.data(), w->pos, nb) && down(quat.data(), w->quat, nb) && down(lvel.data(), w->lvel, nb) &&
-         down(avel.data(), w->avel, nb) && down(&it, w->niters, 1) && down(&res, w->residual, 1) &&
-         (!want_fb || down(fbh.data(), w->fb, fbh.size())) &&
-         (!(w->order == 2 && P.dyn) || down(isl_sweeps.data(), w->isl_sweeps, nisl)) &&
-         odx_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize");
+    ok = G.fetch(0, out_end) && odx_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize");
     if (!ok) return 0;
+    std::memcpy(pos.data(), G.h<float4>(o_pos), f4);
+    std::memcpy(quat.data(), G.h<float4>(o_quat), f4);
+    std::memcpy(lvel.data(), G.h<float4>(o_lvel), f4);
+    std::memcpy(avel.data(), G.h<float4>(o_avel), f4);
+    it = G.h<int>(o_it)[0];
+    res = G.h<float>(o_it)[1];
+    fbh.assign(want_fb ? (size_t)nj * 12 : 0, 0.f);
+    if (want_fb) std::memcpy(fbh.data(), G.h<float>(o_fb), fbh.size() * sizeof(float));
+    if (w->order == 2 && P.dyn) std::memcpy(isl_sweeps.data(), G.h<int>(o_isl), (size_t)nisl * sizeof(int));
     if (w->order != 2) break;
     // shuffles each island ran: before sweeps 8, 16, ... that it reached
     bool agree = true;

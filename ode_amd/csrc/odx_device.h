@@ -80,6 +80,7 @@ struct StepParams {
     // geoms g1 < g2 takes surface material(g1) * nmat + material(g2); with
     // cat_filter a pair needs (cat1 & col2) | (cat2 & col1) != 0
     int nmat, cat_filter;
+    int wave_sweep;      // drop-in world: 0 auto, 1 block-wide levels, 2 one wave (the kernel sees 0 or 1)
 };
 
 // Independent world ranges a batch may be split into (one stream each).

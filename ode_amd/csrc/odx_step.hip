@@ -1595,11 +1595,11 @@ __device__ float block_max(float v, float *s_red) {
 // stopped are skipped, and every non-zero delta adds delta * max|iMJ side| to
 // the body's positive or negative force adjustment (quickstep.cpp:3001-3031;
 // iMJ is stored unscaled, as compute_invM_JT's maxabs reads it).
-__device__ __forceinline__ void process_item_l(const DevWorlds &D, const float4 *pre, int item, float *lam,
+__device__ __forceinline__ void process_item_l(const float4 *rows4, const float4 *pre, int item, float *lam,
                                                float *s_fc, float *fa = nullptr, const int *isl_b = nullptr,
                                                const int *ist = nullptr) {
     const int rs = item >> 3, nr = item & 7;
-    const float4 *row4 = reinterpret_cast<const float4 *>(D.rows + (size_t)rs * ROW_STRIDE);
+    const float4 *row4 = rows4 + (size_t)rs * (ROW_STRIDE / 4);
     float4 a[8];
     if (pre) {
 #pragma unroll
@@ -1670,10 +1670,13 @@ __device__ __forceinline__ void process_item_l(const DevWorlds &D, const float4 
 // the rows in global memory.  STAGED = true: lambda, the work items and the
 // phase starts are copied into LDS next to fc, and every thread loads its item
 // and first row of the next phase before the phase barrier, so a phase costs
-// LDS latency instead of dependent global loads.
-template <bool STAGED, int NT>
+// LDS latency instead of dependent global loads.  ROWS_LDS (with STAGED): the
+// assembled rows are copied into LDS too (128 B each, first in the block), so
+// a level of one row -- a stack's long dependency chains -- costs LDS latency
+// rather than a global round trip.
+template <bool STAGED, int NT, bool ROWS_LDS = false>
 __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P) {
-    extern __shared__ float s_dyn[];
+    extern __shared__ float4 s_dyn4[];
     __shared__ float s_red[32];
     const int t = threadIdx.x;
     const int nb = D.body_start[1];
@@ -1681,6 +1684,8 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
     const int m = D.nrows[0];
     const int L = D.nphases[0];
     const int nit = STAGED ? D.phase_start[L] : 0;
+    float *s_dyn = reinterpret_cast<float *>(s_dyn4) + (ROWS_LDS ? (size_t)m * ROW_STRIDE : 0);
+    const float4 *rows4 = ROWS_LDS ? s_dyn4 : reinterpret_cast<const float4 *>(D.rows);
     float *s_fc = s_dyn;
     float *lam = STAGED ? s_dyn + nb * 6 : D.lambda;
     const int *sched = STAGED ? reinterpret_cast<const int *>(s_dyn + nb * 6 + m) : D.sched;
@@ -1703,6 +1708,12 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
     for (int j = t; j < nj; j += NT) build_joint_rows(D, P, D.joints + (size_t)j * JOINT_STRIDE, 0, 0, D.joff[j]);
     for (int r = t; r < m; r += NT) lam[r] = 0.f;
     __syncthreads();
+    if (ROWS_LDS) {
+        // the block's own rows, written above: visible after the barrier
+        const float4 *g4 = reinterpret_cast<const float4 *>(D.rows);
+        for (int k = t; k < m * (ROW_STRIDE / 4); k += NT) s_dyn4[k] = g4[k];
+        __syncthreads();
+    }
     int it = 0;
     float res = -1.f;
     // one sweep over the phases (items of stopped islands skipped when fa is
@@ -1716,7 +1727,7 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
                 const int Ls = D.seg_table[3 * sg + 2];
                 for (int p = 0; p < Ls; p++) {
                     const int s = phs[p], e = phs[p + 1];
-                    for (int k = s + t; k < e; k += NT) process_item_l(D, nullptr, sch[k], lam, s_fc, fa, isl_b, ist);
+                    for (int k = s + t; k < e; k += NT) process_item_l(rows4, nullptr, sch[k], lam, s_fc, fa, isl_b, ist);
                     __syncthreads();
                 }
                 nsw++;
@@ -1724,42 +1735,61 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
             }
             nsw++;
             if (STAGED) {
-                int s = ph[0], e = ph[1];
-                int cur = s + t < e ? sched[s + t] : -1;
-                float4 pre[8], npre[8];
-                if (cur >= 0) {
-                    const float4 *rr = reinterpret_cast<const float4 *>(D.rows + (size_t)(cur >> 3) * ROW_STRIDE);
+                // levels from LDS with the next level's rows prefetched; lane
+                // `ln` of `stride` takes items ln, ln + stride, ...; `sync` ends a level
+                auto levels = [&](const int ln, const int stride, auto sync) {
+                    int s = ph[0], e = ph[1];
+                    int cur = s + ln < e ? sched[s + ln] : -1;
+                    float4 pre[8], npre[8];
+                    if (cur >= 0) {
+                        const float4 *rr = rows4 + (size_t)(cur >> 3) * (ROW_STRIDE / 4);
 #pragma unroll
-                    for (int k = 0; k < 8; k++) pre[k] = rr[k];
-                }
-                for (int p = 0; p < L; p++) {
-                    int nxt = -1, s2 = 0, e2 = 0;
-                    if (p + 1 < L) {
-                        s2 = ph[p + 1];
-                        e2 = ph[p + 2];
-                        if (s2 + t < e2) {
-                            nxt = sched[s2 + t];
-                            const float4 *rr = reinterpret_cast<const float4 *>(D.rows + (size_t)(nxt >> 3) * ROW_STRIDE);
+                        for (int k = 0; k < 8; k++) pre[k] = rr[k];
+                    }
+                    for (int p = 0; p < L; p++) {
+                        int nxt = -1, s2 = 0, e2 = 0;
+                        if (p + 1 < L) {
+                            s2 = ph[p + 1];
+                            e2 = ph[p + 2];
+                            if (s2 + ln < e2) {
+                                nxt = sched[s2 + ln];
+                                const float4 *rr = rows4 + (size_t)(nxt >> 3) * (ROW_STRIDE / 4);
 #pragma unroll
-                            for (int k = 0; k < 8; k++) npre[k] = rr[k];
+                                for (int k = 0; k < 8; k++) npre[k] = rr[k];
+                            }
+                        }
+                        if (cur >= 0) process_item_l(rows4, pre, cur, lam, s_fc, fa, isl_b, ist);
+                        for (int k = s + ln + stride; k < e; k += stride)
+                            process_item_l(rows4, nullptr, sched[k], lam, s_fc, fa, isl_b, ist);
+                        sync();
+                        s = s2;
+                        e = e2;
+                        cur = nxt;
+                        if (nxt >= 0) {
+#pragma unroll
+                            for (int k = 0; k < 8; k++) pre[k] = npre[k];
                         }
                     }
-                    if (cur >= 0) process_item_l(D, pre, cur, lam, s_fc, fa, isl_b, ist);
-                    for (int k = s + t + NT; k < e; k += NT)
-                        process_item_l(D, nullptr, sched[k], lam, s_fc, fa, isl_b, ist);
+                };
+                if (P.wave_sweep) {
+                    // narrow levels: the first wave sweeps alone.  A wave's LDS
+                    // accesses complete in order, so a level needs no block
+                    // barrier, only a fence that keeps the compiler from moving
+                    // accesses across it; the block meets once per sweep.
+                    if (t < 64)
+                        levels(t, 64, [] {
+                            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                            __builtin_amdgcn_wave_barrier();
+                            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                        });
                     __syncthreads();
-                    s = s2;
-                    e = e2;
-                    cur = nxt;
-                    if (nxt >= 0) {
-#pragma unroll
-                        for (int k = 0; k < 8; k++) pre[k] = npre[k];
-                    }
+                } else {
+                    levels(t, NT, [] { __syncthreads(); });
                 }
             } else {
                 for (int p = 0; p < L; p++) {
                     const int s = ph[p], e = ph[p + 1];
-                    for (int k = s + t; k < e; k += NT) process_item_l(D, nullptr, sched[k], lam, s_fc, fa, isl_b, ist);
+                    for (int k = s + t; k < e; k += NT) process_item_l(rows4, nullptr, sched[k], lam, s_fc, fa, isl_b, ist);
                     __syncthreads();
                 }
             }
@@ -1983,8 +2013,12 @@ void launch_step_fused(const DevWorlds &D, const StepParams &P, const MlpDev *M,
 namespace odx {
 // The staged kernel when fc, lambda, the items and the phase starts fit the
 // device's LDS per workgroup; otherwise the global-memory kernel.
-void launch_single_step(const DevWorlds &D, const StepParams &P, int nb, int m, int nitems, int nphases,
+void launch_single_step(const DevWorlds &D, const StepParams &P0, int nb, int m, int nitems, int nphases,
                         hipStream_t s) {
+    // sweep mode: 1 the whole block per level, 2 the first wave alone, 0 the
+    // wave when levels hold at most 64 items on average
+    StepParams P = P0;
+    P.wave_sweep = P0.wave_sweep == 2 || (P0.wave_sweep == 0 && nitems <= 64 * (nphases > 0 ? nphases : 1)) ? 1 : 0;
     static int lds_max = -1;
     if (lds_max < 0) {
         int dev = 0, v = 0;
@@ -1995,6 +2029,15 @@ void launch_single_step(const DevWorlds &D, const StepParams &P, int nb, int m, 
     }
     const size_t dynw = P.dyn ? (size_t)nb * 3 + (size_t)P.nislands * 5 : 0;  // fa, island of body, island state
     const size_t staged = ((size_t)nb * 6 + m + nitems + nphases + 1 + dynw) * sizeof(float);
+    // rows in LDS as well when they fit next to the staged arrays
+    const size_t with_rows = staged + (size_t)m * ROW_STRIDE * sizeof(float);
+    if (!D.seg_table && with_rows + 256 <= (size_t)lds_max &&
+        hipFuncSetAttribute((const void *)k_single_step_t<true, 512, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)with_rows) == hipSuccess) {
+        hipLaunchKernelGGL((k_single_step_t<true, 512, true>), dim3(1), dim3(512), with_rows, s, D, P);
+        return;
+    }
+    (void)hipGetLastError();
     if (!D.seg_table && staged + 256 <= (size_t)lds_max &&
         hipFuncSetAttribute((const void *)k_single_step_t<true, 512>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)staged) == hipSuccess) {

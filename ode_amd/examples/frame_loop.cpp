@@ -4,12 +4,12 @@
 // dJointGroupEmpty (the loop of ode/demo/demo_space_stress.cpp:76-107,
 // 363-369).  Used by bench.py to time one world through libodx.
 //
-//   frame_loop SCENE WARMUP STEPS ITERS [MU [ORDER [DYNAMIC]]]
+//   frame_loop SCENE WARMUP STEPS ITERS [MU [ORDER [DYNAMIC [SWEEP_MODE]]]]
 //
 // ORDER 0 (default) sweeps the rows in the reference's order; 1 in colour
 // phases (odxWorldSetRowOrder).  DYNAMIC 0 (default) runs exactly ITERS sweeps
 // (dWorldSetQuickStepDynamicIterationParameters with zeros); 1 keeps
-// dWorldCreate's dynamic iteration count.
+// dWorldCreate's dynamic iteration count.  SWEEP_MODE: odxWorldSetSweepMode.
 // SCENE is a binary file: int32 nb, int32 ng, float bodies[nb][32],
 // float geoms[ng][16] (the record layouts of include/odx.h).  The geoms go
 // into a sweep-and-prune space (axes XYZ).  Prints one JSON object.
@@ -48,13 +48,14 @@ void near_callback(void *data, dGeomID o1, dGeomID o2) {
 
 int main(int argc, char **argv) {
     if (argc < 5) {
-        fprintf(stderr, "usage: %s SCENE WARMUP STEPS ITERS [MU [ORDER [DYNAMIC]]]\n", argv[0]);
+        fprintf(stderr, "usage: %s SCENE WARMUP STEPS ITERS [MU [ORDER [DYNAMIC [SWEEP_MODE]]]]\n", argv[0]);
         return 2;
     }
     const int warmup = atoi(argv[2]), steps = atoi(argv[3]), iters = atoi(argv[4]);
     const float mu = argc > 5 ? (float)atof(argv[5]) : 0.5f;
     const int order = argc > 6 ? atoi(argv[6]) : 0;
     const int dynamic = argc > 7 ? atoi(argv[7]) : 0;
+    const int sweep_mode = argc > 8 ? atoi(argv[8]) : 0;
     FILE *f = fopen(argv[1], "rb");
     if (!f) { perror("scene"); return 2; }
     int nb = 0, ng = 0;
@@ -74,6 +75,7 @@ int main(int argc, char **argv) {
     dWorldSetGravity(c.world, 0, 0, -9.81f);
     dWorldSetQuickStepNumIterations(c.world, iters);
     odxWorldSetRowOrder(c.world, order);
+    odxWorldSetSweepMode(c.world, sweep_mode);
     if (!dynamic) {
         const dReal zero = 0;
         dWorldSetQuickStepDynamicIterationParameters(c.world, &zero, &zero, &zero);
@@ -104,15 +106,22 @@ int main(int argc, char **argv) {
         }
         if ((int)g[1] >= 0) dGeomSetBody(h, bodies[(int)g[1]]);
     }
+    double collide_ms = 0, step_ms = 0;  // host wall time inside each call
     auto frame = [&]() {
+        auto a = std::chrono::steady_clock::now();
         dSpaceCollide(space, &c, &near_callback);
+        auto b = std::chrono::steady_clock::now();
         const int ok = dWorldQuickStep(c.world, 0.01f);
+        auto e = std::chrono::steady_clock::now();
         dJointGroupEmpty(c.group);
+        collide_ms += std::chrono::duration<double, std::milli>(b - a).count();
+        step_ms += std::chrono::duration<double, std::milli>(e - b).count();
         return ok;
     };
     for (int s = 0; s < warmup; s++)
         if (!frame()) { fprintf(stderr, "dWorldQuickStep failed\n"); return 1; }
     c.contacts = c.pairs = 0;
+    collide_ms = step_ms = 0;
     odxApiSetKernelTiming(1);
     auto t0 = std::chrono::steady_clock::now();
     for (int s = 0; s < steps; s++)
@@ -125,9 +134,11 @@ int main(int argc, char **argv) {
     for (dBodyID b : bodies) zmin = dBodyGetPosition(b)[2] < zmin ? dBodyGetPosition(b)[2] : zmin;
     printf("{\"steps\": %d, \"warmup\": %d, \"total_ms\": %.6f, \"ms_per_step\": %.6f, \"pairs_per_step\": %.2f, "
            "\"contacts_per_step\": %.2f, \"min_z\": %.6f, \"kernel_ms_per_step\": %.6f, \"row_sweeps_per_step\": %.1f, "
-           "\"row_order\": %d, \"dynamic_iterations\": %d}\n",
+           "\"row_order\": %d, \"dynamic_iterations\": %d, \"sweep_mode\": %d, \"collide_ms_per_step\": %.6f, "
+           "\"quickstep_ms_per_step\": %.6f}\n",
            steps, warmup, ms, ms / steps, (double)c.pairs / steps, (double)c.contacts / steps, zmin,
-           kt[0] > 0 ? kt[1] / kt[0] : 0.0, kt[0] > 0 ? kt[2] / kt[0] : 0.0, order, dynamic);
+           kt[0] > 0 ? kt[1] / kt[0] : 0.0, kt[0] > 0 ? kt[2] / kt[0] : 0.0, order, dynamic, sweep_mode,
+           collide_ms / steps, step_ms / steps);
     dWorldDestroy(c.world);
     dCloseODE();
     return 0;

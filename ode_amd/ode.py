@@ -63,7 +63,7 @@ def api():
         "dWorldQuickStep": (i, [vp, f]), "dWorldSetQuickStepNumIterations": (None, [vp, i]),
         "dWorldGetQuickStepNumIterations": (i, [vp]), "dWorldSetQuickStepW": (None, [vp, f]),
         "dWorldSetContactMaxCorrectingVel": (None, [vp, f]), "dWorldSetContactSurfaceLayer": (None, [vp, f]),
-        "odxWorldSetRowOrder": (None, [vp, i]), "odxWorldGetSolveStats": (None, [vp, P(i), P(ctypes.c_float), P(i)]),
+        "odxWorldSetRowOrder": (None, [vp, i]), "odxWorldSetSweepMode": (None, [vp, i]), "odxWorldGetSolveStats": (None, [vp, P(i), P(ctypes.c_float), P(i)]),
         "dWorldSetQuickStepDynamicIterationParameters": (None, [vp, P(dReal), P(dReal), P(dReal)]),
         "dWorldGetQuickStepDynamicIterationParameters": (None, [vp, P(dReal), P(dReal), P(dReal)]),
         "dBodyCreate": (vp, [vp]), "dBodyDestroy": (None, [vp]),

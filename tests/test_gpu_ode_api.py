@@ -499,3 +499,50 @@ def test_auto_disable_matches_oracle_through_sleep_and_wake():
     woke = np.nonzero(h[5:, 0])[0]
     assert len(woke) > 0, "the rolling sphere never woke sphere 0"
     F.close()
+
+
+@pytest.mark.parametrize("dynamic", [False, True])
+def test_sweep_modes_give_identical_steps(dynamic):
+    # the workgroup-wide and the one-wave level sweeps run the same
+    # Gauss-Seidel order (a level's items touch disjoint bodies): bit-equal
+    # states over a 30-step frame loop
+    say("case: sweep modes, dynamic", dynamic)
+    out = []
+    for mode in (1, 2):
+        scene = settle_scene(6, 30)
+        F = Frame(scene, dynamic=dynamic)
+        F.L.odxWorldSetSweepMode(F.w, mode)
+        for _ in range(30):
+            F.collide()
+            assert F.L.dWorldQuickStep(F.w, 0.01) == 1
+            F.L.dJointGroupEmpty(F.group)
+        out.append(F.state())
+        F.close()
+    np.testing.assert_array_equal(out[0], out[1])
+
+
+def test_quickstep_matches_oracle_with_rows_beyond_lds():
+    # ~2,000 rows (over 200 KB) do not fit in LDS next to the rest: the step
+    # streams rows from global memory; small worlds keep them in LDS
+    say("case: dWorldQuickStep, rows streamed from global memory")
+    rng = np.random.default_rng(12)
+    scene = S.uneven_world(rng, 300, 300)
+    scene.bodies[:, 0:2] *= 4.0
+    scene.bodies[:, 2] = np.minimum(scene.bodies[:, 2], 0.35) + rng.uniform(0, 0.05, size=len(scene.bodies))
+    F = Frame(scene)
+    F.collide()
+    before = scene.bodies.copy()
+    assert F.L.dWorldQuickStep(F.w, 0.01) == 1
+    got = F.state()
+    sweeps, res, rows = ctypes.c_int(0), ctypes.c_float(0), ctypes.c_int(0)
+    F.L.odxWorldGetSolveStats(F.w, ctypes.byref(sweeps), ctypes.byref(res), ctypes.byref(rows))
+    say("  rows", rows.value, "contacts", len(F.contacts))
+    assert rows.value * 128 > 160 * 1024
+    ref = _oracle_step_of_frame(F, before, O.params())
+    err_x = np.max(np.abs(got[:, 0:3] - ref[:, 0:3])) / np.max(np.abs(ref[:, 0:3]))
+    vel = np.r_[8:11, 12:15]
+    err_v = np.max(np.abs(got[:, vel] - ref[:, vel])) / max(np.max(np.abs(ref[:, vel])), 1.0)
+    say("  err_x", err_x, "err_v", err_v)
+    assert err_x < 1e-4 and err_v < 1e-4
+    F.L.dJointGroupEmpty(F.group)
+    F.close()
