@@ -18,6 +18,8 @@
 #ifndef ODX_ODE_H
 #define ODX_ODE_H
 
+#include <stdio.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -286,6 +288,12 @@ void odxWorldSetRowOrder(dWorldID w, int order);
  * workgroup per level, 2 = one wave.  The Gauss-Seidel order and the results
  * are the same in every mode. */
 void odxWorldSetSweepMode(dWorldID w, int mode);
+
+/* Writes the world in the Dynamics Interchange Format (export-dif.h:33,
+ * ode/src/export-dif.cpp:491-620): parameters, bodies in list order with the
+ * geoms attached to each, joints newest first (ball and contact here), reals
+ * with 7 significant digits as a dSINGLE build prints them. */
+void dWorldExportDIF(dWorldID w, FILE *file, const char *world_name);
 /* Sweeps / LCP residual of the last dWorldQuickStep (residual < 0 when no rows). */
 void odxWorldGetSolveStats(dWorldID w, int *sweeps, float *residual, int *rows);
 

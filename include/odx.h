@@ -106,6 +106,20 @@ int odxBatchSetSolveOptions(odxBatch *b, int restart_after, int lds_rows);
  * (cat2 & col1).  nmat = 0 returns to the one surface of the parameters. */
 int odxBatchSetSurfaceTable(odxBatch *b, int nmat, const float *table, int category_filter);
 
+/* World snapshot I/O (no counterpart in the reference, whose nearest is the
+ * text export dWorldExportDIF; for fixtures and reproducible benchmarks).
+ * odxBatchSaveSnapshot writes the batch to a binary file: the bodies as they
+ * are now, the geoms, balls and parameters it was created with, its surface
+ * table and its solver settings.  odxBatchLoadSnapshot creates a batch from
+ * such a file.  A cold-start step of the loaded batch equals the original's
+ * next step up to the quaternion renormalisation of the upload; warm-start
+ * history (previous contacts) is not saved, so warm modes start afresh.
+ * Both return 0 / NULL with a message on failure. */
+int odxBatchSaveSnapshot(odxBatch *b, const char *path);
+odxBatch *odxBatchLoadSnapshot(const char *path);
+/* Worlds, bodies and the world offsets (nworlds + 1 ints; may be NULL). */
+int odxBatchGetLayout(odxBatch *b, int *nworlds, int *nbodies, int *body_start);
+
 /* Splits the worlds into nsplit (1..8) equal index ranges that step on streams
  * of their own inside one hipGraph of graph_steps steps: the worlds are
  * independent, so while one range finishes its slowest solve the others keep

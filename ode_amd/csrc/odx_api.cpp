@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -229,6 +230,8 @@ struct dxJoint {
     float erp = 0.2f, cfm = 1e-5f;
     dJointFeedback *feedback = nullptr;
     int tag = 0;
+    dxJoint *wnext = nullptr, *wprev = nullptr;  // the world's joint list (newest first, as addObjectToList)
+    bool listed = false;
 };
 
 struct dxJointGroup {
@@ -257,6 +260,7 @@ struct dxGeom {
 
 struct dxWorld {
     dxBody *firstbody = nullptr;
+    dxJoint *firstjoint = nullptr;  // every live joint, newest first
     int nbodies = 0, uid_next = 1;
     std::vector<dxJoint *> free_joints;  // joints not in a group
     dVector3 gravity = {0, 0, 0, 0};
@@ -323,6 +327,23 @@ void detach(dxJoint *j) {
     j->node[0].body = j->node[1].body = nullptr;
 }
 
+// the world's joint list: head insertion, as addObjectToList (ode.cpp:54-60)
+void list_joint(dxWorld *w, dxJoint *j) {
+    j->wnext = w->firstjoint;
+    j->wprev = nullptr;
+    if (w->firstjoint) w->firstjoint->wprev = j;
+    w->firstjoint = j;
+    j->listed = true;
+}
+void unlist_joint(dxJoint *j) {
+    if (!j->listed) return;
+    if (j->wprev) j->wprev->wnext = j->wnext;
+    else j->world->firstjoint = j->wnext;
+    if (j->wnext) j->wnext->wprev = j->wprev;
+    j->wnext = j->wprev = nullptr;
+    j->listed = false;
+}
+
 dxJoint *new_joint(dxWorld *w, dxJointGroup *g, int type) {
     auto *j = new dxJoint();
     j->world = w;
@@ -333,10 +354,12 @@ dxJoint *new_joint(dxWorld *w, dxJointGroup *g, int type) {
     j->cfm = w->cfm;
     if (g) g->joints.push_back(j);
     else w->free_joints.push_back(j);
+    list_joint(w, j);
     return j;
 }
 
 void destroy_joint(dxJoint *j) {
+    unlist_joint(j);
     detach(j);
     if (!j->group) {
         auto &v = j->world->free_joints;
@@ -437,6 +460,13 @@ dWorldID dWorldCreate(void) {
 void dWorldDestroy(dWorldID w) {
     if (!w) return;
     for (dxJoint *j : std::vector<dxJoint *>(w->free_joints)) destroy_joint(j);
+    // group joints outlive the world: detach them from its bodies now
+    while (w->firstjoint) {
+        dxJoint *j = w->firstjoint;
+        unlist_joint(j);
+        detach(j);
+        j->enabled = false;
+    }
     for (dxBody *b = w->firstbody; b;) {
         dxBody *n = b->next;
         for (dxGeom *g : b->geoms) g->body = nullptr;
@@ -477,6 +507,184 @@ void dWorldSetContactMaxCorrectingVel(dWorldID w, dReal vel) { w->max_vel = vel;
 void dWorldSetContactSurfaceLayer(dWorldID w, dReal depth) { w->min_depth = depth; }
 void odxWorldSetRowOrder(dWorldID w, int order) { w->order = (order == 1 || order == 2) ? order : 0; }
 void odxWorldSetSweepMode(dWorldID w, int mode) { w->sweep_mode = (mode == 1 || mode == 2) ? mode : 0; }
+
+}  // extern "C"
+
+namespace {
+
+// Text in the Dynamics Interchange Format (export-dif.cpp): tab indents,
+// "name = value," lines, reals with 7 significant digits (a dSINGLE build),
+// +-inf spelled out.
+struct DifWriter {
+    FILE *f;
+    int indent = 1;
+    void tabs() const { for (int i = 0; i < indent; i++) std::fputc('\t', f); }
+    void real(float x) const {
+        if (x == dInfinity) std::fputs("inf", f);
+        else if (x == -dInfinity) std::fputs("-inf", f);
+        else std::fprintf(f, "%.*g", 7, (double)x);
+    }
+    void line(const char *name) const { tabs(); std::fprintf(f, "%s\n", name); }
+    void num(const char *name, int x) const { tabs(); std::fprintf(f, "%s = %d,\n", name, x); }
+    void num(const char *name, float x) const {
+        tabs();
+        std::fprintf(f, "%s = ", name);
+        real(x);
+        std::fputs(",\n", f);
+    }
+    void vec(const char *name, const float *x, int n = 3) const {
+        tabs();
+        std::fprintf(f, "%s = {", name);
+        for (int i = 0; i < n; i++) {
+            real(x[i]);
+            if (i < n - 1) std::fputc(',', f);
+        }
+        std::fputs("},\n", f);
+    }
+    void str(const char *name, const char *x) const { tabs(); std::fprintf(f, "%s = \"%s\",\n", name, x); }
+    // a vector is written only when all three components are non-zero, as the reference does
+    void vec_nonzero(const char *name, const float *x) const {
+        if (x[0] != 0 && x[1] != 0 && x[2] != 0) vec(name, x);
+    }
+};
+
+void dif_geom(DifWriter &c, const dxGeom *g) {
+    if (g->cat != ~0ul) { c.tabs(); std::fprintf(c.f, "category_bits = %lu\n", g->cat); }
+    if (g->col != ~0ul) { c.tabs(); std::fprintf(c.f, "collide_bits = %lu\n", g->col); }
+    if (!g->enabled) c.num("disabled", 1);
+    switch (g->cls) {
+    case dSphereClass: c.str("type", "sphere"); c.num("radius", g->p[0]); break;
+    case dBoxClass: c.str("type", "box"); c.vec("sides", g->p); break;
+    case dCapsuleClass: c.str("type", "capsule"); c.num("radius", g->p[0]); c.num("length", g->p[1]); break;
+    case dPlaneClass: c.str("type", "plane"); c.vec("normal", g->p); c.num("d", g->p[3]); break;
+    default: break;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// dWorldExportDIF (export-dif.cpp:491-620): the world's parameters, its
+// bodies in list order with the geoms attached to each (most recently
+// attached first) and its joints (newest first), for contact and ball joints.
+void dWorldExportDIF(dWorldID w, FILE *file, const char *prefix) {
+    if (!w || !file) return;
+    if (!prefix) prefix = "";
+    DifWriter c{file};
+    std::fprintf(file, "-- Dynamics Interchange Format v0.1\n\n%sworld = dynamics.world {\n", prefix);
+    c.vec("gravity", w->gravity);
+    c.line("ODE = {");
+    c.indent++;
+    c.num("ERP", w->erp);
+    c.num("CFM", w->cfm);
+    c.line("auto_disable = {");
+    c.indent++;
+    c.num("linear_threshold", w->adis.lin_thr2);
+    c.num("angular_threshold", w->adis.ang_thr2);
+    c.num("average_samples", (int)w->adis.average_samples);
+    c.num("idle_time", w->adis.idle_time);
+    c.num("idle_steps", w->adis.idle_steps);
+    std::fputs("\t\t},\n\t},\n}\n", file);
+    c.indent -= 3;
+    int num = 0;
+    std::fprintf(file, "%sbody = {}\n", prefix);
+    for (dxBody *b = w->firstbody; b; b = b->next) {
+        b->tag = num;
+        std::fprintf(file, "%sbody[%d] = dynamics.body {\n\tworld = %sworld,\n", prefix, num, prefix);
+        c.indent++;
+        c.vec("pos", b->pos);
+        c.vec("q", b->q, 4);
+        c.vec("lvel", b->lvel);
+        c.vec("avel", b->avel);
+        c.num("mass", b->mass.mass);
+        std::fputs("\tI = {{", file);
+        for (int i = 0; i < 3; i++) {
+            for (int k = 0; k < 3; k++) {
+                c.real(b->mass.I[i * 4 + k]);
+                if (k < 2) std::fputc(',', file);
+            }
+            if (i < 2) std::fputs("},{", file);
+        }
+        std::fputs("}},\n", file);
+        c.vec_nonzero("com", b->mass.c);
+        c.line("ODE = {");
+        c.indent++;
+        if (!b->enabled) c.num("disabled", 1);
+        if (!b->gravity) c.num("no_gravity", 1);
+        if (b->auto_disable) {
+            c.line("auto_disable = {");
+            c.indent++;
+            c.num("linear_threshold", b->adis.lin_thr2);
+            c.num("angular_threshold", b->adis.ang_thr2);
+            c.num("average_samples", (int)b->adis.average_samples);
+            c.num("idle_time", b->adis.idle_time);
+            c.num("idle_steps", b->adis.idle_steps);
+            c.num("time_left", b->adis_timeleft);
+            c.num("steps_left", b->adis_stepsleft);
+            c.indent--;
+            c.line("},");
+        }
+        c.vec_nonzero("facc", b->facc);
+        c.vec_nonzero("tacc", b->tacc);
+        c.indent--;
+        c.line("},");
+        if (!b->geoms.empty()) {
+            c.line("geometry = {");
+            c.indent++;
+            for (size_t k = b->geoms.size(); k-- > 0;) {
+                c.line("{");
+                c.indent++;
+                dif_geom(c, b->geoms[k]);
+                c.indent--;
+                c.line("},");
+            }
+            c.indent--;
+            c.line("},");
+        }
+        c.indent--;
+        c.line("}");
+        num++;
+    }
+    num = 0;
+    std::fprintf(file, "%sjoint = {}\n", prefix);
+    for (dxJoint *j = w->firstjoint; j; j = j->wnext) {
+        c.indent++;
+        const char *name = j->type == dJointTypeBall ? "ball" : j->type == dJointTypeContact ? "contact" : "unknown";
+        std::fprintf(file, "%sjoint[%d] = dynamics.%s_joint {\n\tworld = %sworld,\n\tbody = {", prefix, num, name, prefix);
+        if (j->node[0].body) std::fprintf(file, "%sbody[%d]", prefix, j->node[0].body->tag);
+        if (j->node[1].body) std::fprintf(file, ",%sbody[%d]", prefix, j->node[1].body->tag);
+        std::fputs("}\n", file);
+        if (j->type == dJointTypeBall) {
+            c.vec("anchor1", j->anchor1);
+            c.vec("anchor2", j->anchor2);
+        } else if (j->type == dJointTypeContact) {
+            const dContact &ct = j->contact;
+            const int mode = ct.surface.mode;
+            c.vec("pos", ct.geom.pos);
+            c.vec("normal", ct.geom.normal);
+            c.num("depth", ct.geom.depth);
+            c.num("mu", ct.surface.mu);
+            if (mode & dContactMu2) c.num("mu2", ct.surface.mu2);
+            if (mode & dContactBounce) c.num("bounce", ct.surface.bounce);
+            if (mode & dContactBounce) c.num("bounce_vel", ct.surface.bounce_vel);
+            if (mode & dContactSoftERP) c.num("soft_ERP", ct.surface.soft_erp);
+            if (mode & dContactSoftCFM) c.num("soft_CFM", ct.surface.soft_cfm);
+            if (mode & dContactMotion1) c.num("motion1", ct.surface.motion1);
+            if (mode & dContactMotion2) c.num("motion2", ct.surface.motion2);
+            if (mode & dContactSlip1) c.num("slip1", ct.surface.slip1);
+            if (mode & dContactSlip2) c.num("slip2", ct.surface.slip2);
+            const int fa = ((mode & dContactApprox1_1) ? 1 : 0) | ((mode & dContactApprox1_2) ? 2 : 0);
+            if (fa) c.num("friction_approximation", fa);
+            if (mode & dContactFDir1) c.vec("fdir1", ct.fdir1);
+        } else {
+            c.line("unknown joint");
+        }
+        c.indent--;
+        c.line("}");
+        num++;
+    }
+}
 void odxWorldGetSolveStats(dWorldID w, int *sweeps, float *residual, int *rows) {
     if (sweeps) *sweeps = w->last_sweeps;
     if (residual) *residual = w->last_res;
@@ -650,6 +858,7 @@ dJointID dJointCreateContact(dWorldID w, dJointGroupID g, const dContact *c) {
 void dJointDestroy(dJointID j) {
     if (!j) return;
     if (j->group) {  // group joints are only detached; the group owns the memory
+        unlist_joint(j);
         detach(j);
         j->enabled = false;
         return;
@@ -665,6 +874,7 @@ void dJointGroupDestroy(dJointGroupID g) {
 void dJointGroupEmpty(dJointGroupID g) {
     // detached from the most recently created (ode.cpp:1325-1366)
     for (size_t i = g->joints.size(); i-- > 0;) {
+        unlist_joint(g->joints[i]);
         detach(g->joints[i]);
         delete g->joints[i];
     }

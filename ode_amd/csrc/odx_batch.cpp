@@ -254,6 +254,11 @@ struct odxBatch {
     StepParams P{};  // solver settings; wbase / wcount / split / feat_base are set per launch
     Surface surf{};
     std::vector<int> geom_mat;  // material of each geom (GEOM slot 13)
+    // what a snapshot needs to re-create the batch: the records as created,
+    // the parameter vector, the surface table and the contact capacity
+    std::vector<float> geoms_h, balls_h, table_h;
+    double params_h[32] = {0};
+    int contacts_per_body = 8;
     Surface *surf_default = nullptr;  // device copy of the parameters' surface
     Surface *surf_table = nullptr;  // device material-pair table (nmat * nmat), owned
     hipStream_t stream = nullptr;
@@ -433,7 +438,11 @@ odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies
     B->nball = B->ball_start[nworlds];
     B->geom_mat.resize(B->ng);
     for (int g = 0; g < B->ng; g++) B->geom_mat[g] = (int)geoms[(size_t)g * GEOM_STRIDE + 13];
+    B->geoms_h.assign(geoms, geoms + (size_t)B->ng * GEOM_STRIDE);
+    if (balls) B->balls_h.assign(balls, balls + (size_t)B->nball * JOINT_STRIDE);
+    std::memcpy(B->params_h, params, sizeof B->params_h);
     if (contacts_per_body <= 0) contacts_per_body = 8;
+    B->contacts_per_body = contacts_per_body;
     B->joint_start.resize(nworlds + 1);
     B->row_start.resize(nworlds + 1);
     B->joint_start[0] = 0;
@@ -598,6 +607,7 @@ int odxBatchSetSurfaceTable(odxBatch *b, int nmat, const float *table, int categ
     }
     b->P.nmat = nmat;
     b->P.cat_filter = category_filter ? 1 : 0;
+    b->table_h.assign(table, table + (size_t)nmat * nmat * 16);
     b->drop_graph();
     return 1;
 }
@@ -742,6 +752,98 @@ int odxBatchSetBodies(odxBatch *b, const float *bodies) {
     if (!b || !bodies) return 0;
     if (!odxBatchSync(b)) return 0;
     return odx_bodies_upload(b->D, b->nb, bodies) ? b->nb : 0;
+}
+
+// ---- binary snapshot: "ODXSNAP1", then int32 counts (worlds, bodies, geoms,
+// balls, contacts per body, materials, category filter, order, max_iters,
+// check_every, warm, restart_after, lds_rows), float tol, the int32 world
+// offsets, the float records (bodies as they are now), double params[32] and
+// the float surface table
+namespace {
+constexpr char SNAP_MAGIC[8] = {'O', 'D', 'X', 'S', 'N', 'A', 'P', '1'};
+constexpr int SNAP_NINT = 13;
+}  // namespace
+
+int odxBatchSaveSnapshot(odxBatch *b, const char *path) {
+    if (!b || !path) {
+        odx_message(1, "odxBatchSaveSnapshot: bad arguments");
+        return 0;
+    }
+    std::vector<float> bodies((size_t)b->nb * BODY_STRIDE);
+    if (!odxBatchSync(b) || !odx_bodies_download(b->D, b->nb, bodies.data())) return 0;
+    FILE *f = std::fopen(path, "wb");
+    if (!f) {
+        odx_message(2, "odxBatchSaveSnapshot: cannot open %s", path);
+        return 0;
+    }
+    const int hdr[SNAP_NINT] = {b->nw, b->nb, b->ng, b->nball, b->contacts_per_body, b->P.nmat, b->P.cat_filter,
+                                b->P.order, b->P.max_iters, b->P.check_every, b->P.warm, b->P.restart_after,
+                                b->P.lds_rows};
+    const float tol = b->P.tol;
+    bool ok = std::fwrite(SNAP_MAGIC, 1, 8, f) == 8 && std::fwrite(hdr, sizeof(int), SNAP_NINT, f) == SNAP_NINT &&
+              std::fwrite(&tol, sizeof tol, 1, f) == 1 &&
+              std::fwrite(b->body_start.data(), sizeof(int), b->nw + 1, f) == (size_t)b->nw + 1 &&
+              std::fwrite(b->geom_start.data(), sizeof(int), b->nw + 1, f) == (size_t)b->nw + 1 &&
+              std::fwrite(b->ball_start.data(), sizeof(int), b->nw + 1, f) == (size_t)b->nw + 1 &&
+              std::fwrite(bodies.data(), sizeof(float), bodies.size(), f) == bodies.size() &&
+              std::fwrite(b->geoms_h.data(), sizeof(float), b->geoms_h.size(), f) == b->geoms_h.size() &&
+              std::fwrite(b->balls_h.data(), sizeof(float), b->balls_h.size(), f) == b->balls_h.size() &&
+              std::fwrite(b->params_h, sizeof(double), 32, f) == 32 &&
+              std::fwrite(b->table_h.data(), sizeof(float), b->table_h.size(), f) == b->table_h.size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) odx_message(2, "odxBatchSaveSnapshot: write to %s failed", path);
+    return ok ? 1 : 0;
+}
+
+int odxBatchGetLayout(odxBatch *b, int *nworlds, int *nbodies, int *body_start) {
+    if (!b) return 0;
+    if (nworlds) *nworlds = b->nw;
+    if (nbodies) *nbodies = b->nb;
+    if (body_start) std::memcpy(body_start, b->body_start.data(), (size_t)(b->nw + 1) * sizeof(int));
+    return 1;
+}
+
+odxBatch *odxBatchLoadSnapshot(const char *path) {
+    FILE *f = path ? std::fopen(path, "rb") : nullptr;
+    if (!f) {
+        odx_message(2, "odxBatchLoadSnapshot: cannot open %s", path ? path : "(null)");
+        return nullptr;
+    }
+    char magic[8];
+    int hdr[SNAP_NINT];
+    float tol = 0.f;
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, SNAP_MAGIC, 8) == 0 &&
+              std::fread(hdr, sizeof(int), SNAP_NINT, f) == SNAP_NINT && std::fread(&tol, sizeof tol, 1, f) == 1;
+    const int nw = ok ? hdr[0] : 0, nb = ok ? hdr[1] : 0, ng = ok ? hdr[2] : 0, nball = ok ? hdr[3] : 0;
+    const int nmat = ok ? hdr[5] : 0;
+    ok = ok && nw > 0 && nb >= 0 && ng >= 0 && nball >= 0 && nmat >= 0 && nmat <= ODX_MAX_MATERIALS;
+    std::vector<int> bs(ok ? nw + 1 : 0), gs(bs.size()), ls(bs.size());
+    std::vector<float> bodies(ok ? (size_t)nb * BODY_STRIDE : 0), geoms(ok ? (size_t)ng * GEOM_STRIDE : 0),
+        balls(ok ? (size_t)nball * JOINT_STRIDE : 0), table(ok ? (size_t)nmat * nmat * 16 : 0);
+    double params[32];
+    ok = ok && std::fread(bs.data(), sizeof(int), bs.size(), f) == bs.size() &&
+         std::fread(gs.data(), sizeof(int), gs.size(), f) == gs.size() &&
+         std::fread(ls.data(), sizeof(int), ls.size(), f) == ls.size() &&
+         std::fread(bodies.data(), sizeof(float), bodies.size(), f) == bodies.size() &&
+         std::fread(geoms.data(), sizeof(float), geoms.size(), f) == geoms.size() &&
+         std::fread(balls.data(), sizeof(float), balls.size(), f) == balls.size() &&
+         std::fread(params, sizeof(double), 32, f) == 32 &&
+         std::fread(table.data(), sizeof(float), table.size(), f) == table.size();
+    std::fclose(f);
+    ok = ok && bs[0] == 0 && bs[nw] == nb && gs[0] == 0 && gs[nw] == ng && ls[0] == 0 && ls[nw] == nball;
+    if (!ok) {
+        odx_message(2, "odxBatchLoadSnapshot: %s is not a complete snapshot", path);
+        return nullptr;
+    }
+    odxBatch *b = odxBatchCreate(nw, bs.data(), bodies.data(), gs.data(), geoms.data(), ls.data(),
+                                 nball ? balls.data() : nullptr, params, hdr[4]);
+    if (!b) return nullptr;
+    if (!odxBatchSetSolver(b, hdr[7], hdr[8], tol, hdr[9], hdr[10]) || !odxBatchSetSolveOptions(b, hdr[11], hdr[12]) ||
+        (nmat > 0 && !odxBatchSetSurfaceTable(b, nmat, table.data(), hdr[6]))) {
+        odxBatchDestroy(b);
+        return nullptr;
+    }
+    return b;
 }
 
 int odxBatchGetStats(odxBatch *b, int *ncontacts, int *nrows, int *niters, float *residual) {

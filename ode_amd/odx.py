@@ -46,6 +46,10 @@ def lib():
         L.odxBatchSetSolver.argtypes = [vp, i, i, ctypes.c_float, i, i]
         L.odxBatchSetSolveOptions.argtypes = [vp, i, i]
         L.odxBatchSetSurfaceTable.argtypes = [vp, i, fp, i]
+        L.odxBatchSaveSnapshot.argtypes = [vp, ctypes.c_char_p]
+        L.odxBatchLoadSnapshot.restype = ctypes.c_void_p
+        L.odxBatchLoadSnapshot.argtypes = [ctypes.c_char_p]
+        L.odxBatchGetLayout.argtypes = [vp, ip, ip, ip]
         L.odxBatchSetSplits.argtypes = [vp, i, i]
         L.odxBatchSetFused.argtypes = [vp, i]
         L.odxBatchStep.argtypes = [vp, i]
@@ -148,6 +152,28 @@ class Batch:
         rs = np.zeros(self.nworlds + 1, np.int32)
         L.odxBatchRowStarts(self.h, _p(rs))
         self.row_start = rs
+
+    @classmethod
+    def load_snapshot(cls, path):
+        """A batch re-created from odxBatchSaveSnapshot's file."""
+        L = lib()
+        h = L.odxBatchLoadSnapshot(str(path).encode())
+        _check(h, "odxBatchLoadSnapshot")
+        self = cls.__new__(cls)
+        self.h = h
+        nw, nb = ctypes.c_int(0), ctypes.c_int(0)
+        L.odxBatchGetLayout(h, ctypes.byref(nw), ctypes.byref(nb), None)
+        self.nworlds, self.nbodies = nw.value, nb.value
+        self.body_start = np.zeros(self.nworlds + 1, np.int32)
+        L.odxBatchGetLayout(h, None, None, _p(self.body_start))
+        self.worlds = None
+        rs = np.zeros(self.nworlds + 1, np.int32)
+        L.odxBatchRowStarts(h, _p(rs))
+        self.row_start = rs
+        return self
+
+    def save_snapshot(self, path):
+        _check(lib().odxBatchSaveSnapshot(self.h, str(path).encode()), "odxBatchSaveSnapshot")
 
     def close(self):
         if getattr(self, "h", None):

@@ -43,3 +43,18 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(odx.LIB_PATH)
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert not missing, "missing exports: %s" % missing
+
+
+def test_snapshot_loader_rejects_missing_and_foreign_files(tmp_path):
+    # host-side checks of odxBatchLoadSnapshot: no device work happens before them
+    from ode_amd import odx
+    L = odx.lib()
+    assert not L.odxBatchLoadSnapshot(str(tmp_path / "absent.snap").encode())
+    bad = tmp_path / "foreign.snap"
+    bad.write_bytes(b"NOTASNAP" + bytes(200))
+    assert not L.odxBatchLoadSnapshot(str(bad).encode())
+    short = tmp_path / "short.snap"
+    short.write_bytes(b"ODXSNAP1" + bytes(10))
+    assert not L.odxBatchLoadSnapshot(str(short).encode())
+    msgs = " ".join(m for _, m in odx.last_messages())
+    assert "cannot open" in msgs and "not a complete snapshot" in msgs

@@ -429,3 +429,40 @@ def test_surface_table_and_category_bits_match_oracle():
     say("  contacts dropped by the policy (bits and limits):", dropped)
     assert dropped > 0
     B.close()
+
+
+def test_snapshot_round_trip_continues_the_same_steps(tmp_path):
+    # odxBatchSaveSnapshot / odxBatchLoadSnapshot: the loaded batch holds the
+    # state, the records, the solver settings and the surface table, and
+    # continues as the original does (cold start)
+    say("case: batch snapshot round trip")
+    odx = _odx()
+    rng = np.random.default_rng(31)
+    worlds = [S.uneven_world(rng, 8, 20) for _ in range(16)]
+    for w in worlds:
+        w.bodies[:, 2] = np.minimum(w.bodies[:, 2], 0.6)
+        w.geoms[:, 13] = np.where(w.geoms[:, 1] >= 0, 1, 0)
+    t = O.surface_table(2, {(0, 1): {"mode": 0x1000 | 0x2000, "mu": 0.3, "max_contacts": 3},
+                            (1, 1): {"mode": 0x1000 | 0x2000, "mu": 0.7}})
+    B = odx.Batch(worlds, O.params())
+    B.set_surface_table(t)
+    B.set_solver(order=1, max_iters=30, tol=0.0)
+    B.step(5)
+    path = tmp_path / "batch.snap"
+    B.save_snapshot(path)
+    C = odx.Batch.load_snapshot(path)
+    assert C.nworlds == B.nworlds and C.nbodies == B.nbodies
+    np.testing.assert_array_equal(C.body_start, B.body_start)
+    b0, c0 = B.bodies(), C.bodies()
+    np.testing.assert_allclose(c0, b0, rtol=1e-6, atol=1e-7)  # quaternions renormalised on upload
+    B.step(3)
+    C.step(3)
+    sb, sc = B.stats(), C.stats()
+    np.testing.assert_array_equal(sb["contacts"], sc["contacts"])
+    np.testing.assert_array_equal(sb["iters"], sc["iters"])
+    gb, gc = B.bodies(), C.bodies()
+    assert relerr(gc[:, 0:3], gb[:, 0:3]) < TOL_STATE
+    vel = np.r_[8:11, 12:15]
+    assert np.max(np.abs(gc[:, vel] - gb[:, vel])) / max(np.max(np.abs(gb[:, vel])), 1.0) < TOL_STATE
+    B.close()
+    C.close()

@@ -546,3 +546,123 @@ def test_quickstep_matches_oracle_with_rows_beyond_lds():
     assert err_x < 1e-4 and err_v < 1e-4
     F.L.dJointGroupEmpty(F.group)
     F.close()
+
+
+def test_export_dif_matches_oracle_text(tmp_path):
+    # dWorldExportDIF of a world with flags, several geoms per body, a ball
+    # joint and contact joints (one reversed by a NULL body1), against the
+    # oracle's restatement of export-dif.cpp
+    from oracle import dif
+    from ode_amd import ode as A
+    say("case: dWorldExportDIF")
+    L = A.api()
+    L.dWorldExportDIF.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p]
+    L.dGeomSetCategoryBits.argtypes = [ctypes.c_void_p, ctypes.c_ulong]
+    L.dGeomSetCollideBits.argtypes = [ctypes.c_void_p, ctypes.c_ulong]
+    L.dGeomDisable.argtypes = [ctypes.c_void_p]
+    L.dBodyDisable.argtypes = [ctypes.c_void_p]
+    libc = ctypes.CDLL(None)
+    libc.fopen.restype = ctypes.c_void_p
+    libc.fopen.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+    libc.fclose.argtypes = [ctypes.c_void_p]
+    w = L.dWorldCreate()
+    L.dWorldSetGravity(w, 0, 0, -9.81)
+    space = L.dSimpleSpaceCreate(None)
+    group = L.dJointGroupCreate(0)
+    F32 = np.float32
+    specs = [((0.5, -1.25, 2.0), (1, 0, 0, 0), 2.5), ((0.1, 0.2, 0.7), (0.8, 0.6, 0, 0), 1.5), ((3, 3, 3), (1, 0, 0, 0), 4.0)]
+    bodies = []
+    for pos, q, mass in specs:
+        b = L.dBodyCreate(w)
+        L.dBodySetPosition(b, *pos)
+        L.dBodySetQuaternion(b, (A.dReal * 4)(*q))
+        L.dBodySetLinearVel(b, 0.25, -0.5, 1.0)
+        m = A.dMass()
+        L.dMassSetBox(ctypes.byref(m), 1000.0 * mass, 0.5, 0.25, 0.75)
+        L.dBodySetMass(b, ctypes.byref(m))
+        bodies.append((b, m))
+    L.dBodySetGravityMode(bodies[1][0], 0)
+    L.dBodySetAutoDisableFlag(bodies[2][0], 1)
+    L.dBodySetAutoDisableSteps(bodies[2][0], 4)
+    L.dBodyDisable(bodies[2][0])
+    L.dBodyAddTorque(bodies[0][0], 1.0, 2.0, 3.0)
+    g0 = L.dCreateSphere(space, 0.3)
+    L.dGeomSetBody(g0, bodies[0][0])
+    g1 = L.dCreateBox(space, 0.5, 0.25, 0.75)
+    L.dGeomSetBody(g1, bodies[0][0])
+    L.dGeomSetCategoryBits(g1, 5)
+    g2 = L.dCreateCapsule(space, 0.1, 0.6)
+    L.dGeomSetBody(g2, bodies[1][0])
+    L.dGeomDisable(g2)
+    L.dCreatePlane(space, 0, 0, 1, 0)
+    jb = L.dJointCreateBall(w, None)
+    L.dJointAttach(jb, bodies[0][0], bodies[1][0])
+    L.dJointSetBallAnchor(jb, 0.3, -0.5, 1.5)
+    cs = []
+    for k, (b1, b2, mode) in enumerate([(bodies[0][0], None, 0x1000 | 0x004 | 0x001),
+                                        (None, bodies[1][0], 0x1000 | 0x2000 | 0x010 | 0x002)]):
+        c = A.dContact()
+        c.surface.mode = mode
+        c.surface.mu = 0.5 + k
+        c.surface.mu2 = 0.25
+        c.surface.bounce, c.surface.bounce_vel, c.surface.soft_cfm = 0.3, 0.1, 1e-3
+        c.geom.pos[0], c.geom.pos[1], c.geom.pos[2] = 1.0, 2.0, 0.0
+        c.geom.normal[2] = 1.0
+        c.geom.depth = 0.015
+        c.fdir1[0] = 1.0
+        j = L.dJointCreateContact(w, group, ctypes.byref(c))
+        L.dJointAttach(j, b1, b2)
+        cs.append(c)
+    path = str(tmp_path / "world.dif").encode()
+    f = libc.fopen(path, b"w")
+    L.dWorldExportDIF(w, f, b"X")
+    libc.fclose(f)
+    got = open(path).read()
+
+    # the oracle's description of the same world: body list newest first
+    adis0 = {"lin_thr2": F32(0.01) * F32(0.01), "ang_thr2": F32(0.01) * F32(0.01), "samples": 1, "idle_time": 0.0,
+             "idle_steps": 10}
+    world = {"gravity": (0.0, 0.0, F32(-9.81)), "erp": 0.2, "cfm": 1e-5, "adis": adis0}
+    order = [2, 1, 0]
+    geoms = {0: [{"cls": 1, "p": (0.5, 0.25, 0.75, 0), "cat": 5, "col": dif.ALL_BITS, "enabled": True},
+                 {"cls": 0, "p": (0.3, 0, 0, 0), "cat": dif.ALL_BITS, "col": dif.ALL_BITS, "enabled": True}],
+             1: [{"cls": 2, "p": (0.1, 0.6, 0, 0), "cat": dif.ALL_BITS, "col": dif.ALL_BITS, "enabled": False}],
+             2: []}
+    desc = []
+    for i in order:
+        b, m = bodies[i]
+        d = {"pos": A.vec(L.dBodyGetPosition(b)), "q": A.vec(L.dBodyGetQuaternion(b), 4),
+             "lvel": A.vec(L.dBodyGetLinearVel(b)), "avel": A.vec(L.dBodyGetAngularVel(b)), "mass": m.mass,
+             "I": list(m.I), "c": list(m.c)[:3], "enabled": i != 2, "gravity": i != 1,
+             "adis": None, "facc": (0.0, 0.0, 0.0), "tacc": (1.0, 2.0, 3.0) if i == 0 else (0.0, 0.0, 0.0),
+             "geoms": geoms[i]}
+        if i == 2:
+            d["adis"] = dict(adis0, idle_steps=4, time_left=0.0, steps_left=10)
+        desc.append(d)
+    tag = {i: k for k, i in enumerate(order)}
+    joints = []
+    # joints newest first: the second contact (attached NULL, body 1 -> reversed to (1)), the first, the ball
+    c1, c0 = cs[1], cs[0]
+    joints.append({"type": "contact", "bodies": (tag[1], None), "pos": (1.0, 2.0, 0.0), "normal": (0.0, 0.0, 1.0),
+                   "depth": 0.015, "mode": c1.surface.mode, "mu": 1.5, "mu2": 0.25, "bounce": 0.3, "bounce_vel": 0.1,
+                   "soft_cfm": 1e-3, "fdir1": (1.0, 0.0, 0.0)})
+    joints.append({"type": "contact", "bodies": (tag[0], None), "pos": (1.0, 2.0, 0.0), "normal": (0.0, 0.0, 1.0),
+                   "depth": 0.015, "mode": c0.surface.mode, "mu": 0.5, "mu2": 0.25, "bounce": 0.3, "bounce_vel": 0.1,
+                   "soft_cfm": 1e-3, "fdir1": (1.0, 0.0, 0.0)})
+    ball_a1 = np.zeros(3)
+    ball_a2 = np.zeros(3)
+    # anchors in each body's frame: the library computed them from the world point
+    for line in got.split("\n"):
+        if line.startswith("\tanchor1 = "):
+            ball_a1 = [float(x) for x in line.split("{")[1].split("}")[0].split(",")]
+        if line.startswith("\tanchor2 = "):
+            ball_a2 = [float(x) for x in line.split("{")[1].split("}")[0].split(",")]
+    joints.append({"type": "ball", "bodies": (tag[0], tag[1]), "anchor1": ball_a1, "anchor2": ball_a2})
+    want = dif.export(world, desc, joints, prefix="X")
+    if got != want:
+        import difflib
+        say("".join(list(difflib.unified_diff(want.splitlines(True), got.splitlines(True)))[:60]))
+    assert got == want
+    L.dJointGroupDestroy(group)
+    L.dSpaceDestroy(space)
+    L.dWorldDestroy(w)
