@@ -983,7 +983,16 @@ template <class R> int pairs_hash(int ng, const R *aabb, const int *body, const 
 }
 
 /* ------------------------------------------------------------- joints --- */
-enum { JT_CONTACT = 0, JT_BALL = 1 };
+enum { JT_CONTACT = 0, JT_BALL = 1, JT_HINGE = 2, JT_SLIDER = 3, JT_FIXED = 4 };
+
+/* Extended joint record (32 reals) of hinge, slider and fixed joints: 0-2
+ * axis1 (body1 frame), 3-5 axis2 (body2 frame, or world), 6-9 qrel, 10-12
+ * offset, 14 the limit state the joint carries from its last getInfo1 (the
+ * hinge keeps it when its stops are not finite), 16 lostop, 17 histop, 18 vel,
+ * 19 fmax, 20 fudge factor, 21 normal cfm, 22 stop erp, 23 stop cfm,
+ * 24 bounce, 26 joint erp, 27 joint cfm (fixed joint).  dxJointLimitMotor
+ * joints/joint.h, dxJointHinge/Slider/Fixed joints/hinge.h, slider.h, fixed.h. */
+constexpr int ORC_JEXT_STRIDE = 32;
 
 template <class R> struct Surface {
     int mode;
@@ -996,6 +1005,10 @@ template <class R> struct Joint {
     R pos[3], normal[3], depth, fdir1[3];
     Surface<R> s;
     R anchor1[3], anchor2[3], erp, cfm;
+    /* hinge / slider / fixed */
+    R axis1[3], axis2[3], qrel[4], offset[3];
+    int limit;
+    R limit_err, lostop, histop, vel, fmax, fudge, normal_cfm, stop_erp, stop_cfm, bounce, jerp, jcfm;
 };
 
 struct BodyState;
@@ -1107,6 +1120,254 @@ template <class R> void ball_info2(const Joint<R> &j, const R *b0v, const R *b1v
         if (b1v) rows[r].rhs = k * (a2[r] + b1v[r] - a1[r] - b0v[r]);
         else rows[r].rhs = k * (j.anchor2[r] - a1[r] - b0v[r]);
     }
+}
+
+/* Quaternion products of rotation.cpp:41-107: qa = qb * qc with qb and / or
+ * qc conjugated (dQMultiply1: qb', dQMultiply2: qc', dQMultiply3: both). */
+template <class R> void qmul(R *qa, const R *qb, const R *qc, bool cb, bool cc) {
+    const R b[4] = {qb[0], cb ? -qb[1] : qb[1], cb ? -qb[2] : qb[2], cb ? -qb[3] : qb[3]};
+    const R c[4] = {qc[0], cc ? -qc[1] : qc[1], cc ? -qc[2] : qc[2], cc ? -qc[3] : qc[3]};
+    qa[0] = b[0] * c[0] - b[1] * c[1] - b[2] * c[2] - b[3] * c[3];
+    qa[1] = b[0] * c[1] + b[1] * c[0] + b[2] * c[3] - b[3] * c[2];
+    qa[2] = b[0] * c[2] + b[2] * c[0] + b[3] * c[1] - b[1] * c[3];
+    qa[3] = b[0] * c[3] + b[3] * c[0] + b[1] * c[2] - b[2] * c[1];
+}
+
+/* getHingeAngle joints/joint.cpp:421-490: the relative rotation angle about
+ * `axis` (body1 frame), in -pi..pi, zero at the orientation of qrel. */
+template <class R> R hinge_angle(const R *q1, const R *q2, const R *axis, const R *qrel) {
+    R qr[4];
+    if (q2) {
+        R qq[4];
+        qmul(qq, q1, q2, true, false);
+        qmul(qr, qq, qrel, false, true);
+    } else {
+        qmul(qr, q1, qrel, true, true);
+    }
+    const R cost2 = qr[0];
+    const R sint2 = std::sqrt(qr[1] * qr[1] + qr[2] * qr[2] + qr[3] * qr[3]);
+    R theta = dot3(qr + 1, axis) >= 0 ? 2 * std::atan2(sint2, cost2) : 2 * std::atan2(sint2, -cost2);
+    if (theta > (R)M_PI) theta -= (R)(2 * M_PI);
+    return -theta;
+}
+
+/* dJointGetSliderPosition joints/slider.cpp:45-82 */
+template <class R> R slider_position(const Joint<R> &j, const R *b0v, const R *b1v) {
+    R R0[12], ax1[3], q[3];
+    RfromQ(R0, b0v + 4);
+    mul0_331(ax1, R0, j.axis1);
+    if (b1v) {
+        R R1[12];
+        RfromQ(R1, b1v + 4);
+        mul0_331(q, R1, j.offset);
+        for (int i = 0; i < 3; i++) q[i] = b0v[i] - q[i] - b1v[i];
+    } else {
+        for (int i = 0; i < 3; i++) q[i] = b0v[i] - j.offset[i];
+        if (j.reverse) for (int i = 0; i < 3; i++) ax1[i] = -ax1[i];
+    }
+    return dot3(ax1, q);
+}
+
+/* getInfo1 of hinge (hinge.cpp:58-77), slider (slider.cpp:112-143), fixed
+ * (fixed.cpp:61-65): rows, and the limit state of the limit / motor. */
+template <class R> int joint_m(Joint<R> &j, const R *b0v, const R *b1v) {
+    if (j.type == JT_FIXED) return 6;
+    if (j.type == JT_HINGE) {
+        int m = j.fmax > 0 ? 6 : 5;
+        if ((j.lostop >= -(R)M_PI || j.histop <= (R)M_PI) && j.lostop <= j.histop) {
+            const R a = hinge_angle(b0v + 4, b1v ? b1v + 4 : (const R *)nullptr, j.axis1, j.qrel);
+            /* testRotationalLimit joint.cpp:574-595 */
+            if (a <= j.lostop) { j.limit = 1; j.limit_err = a - j.lostop; m = 6; }
+            else if (a >= j.histop) { j.limit = 2; j.limit_err = a - j.histop; m = 6; }
+            else j.limit = 0;
+        }
+        if (j.limit) m = 6;
+        return m;
+    }
+    if (j.type == JT_SLIDER) {
+        int m = j.fmax > 0 ? 6 : 5;
+        j.limit = 0;
+        if ((j.lostop > -kInf || j.histop < kInf) && j.lostop <= j.histop) {
+            const R pos = slider_position(j, b0v, b1v);
+            if (pos <= j.lostop) { j.limit = 1; j.limit_err = pos - j.lostop; m = 6; }
+            else if (pos >= j.histop) { j.limit = 2; j.limit_err = pos - j.histop; m = 6; }
+        }
+        return m;
+    }
+    return 3;
+}
+
+/* setFixedOrientation joints/joint.cpp:228-285: three rows making the
+ * relative rotation equal qrel. */
+template <class R> void fixed_orientation(const Joint<R> &j, const R *b0v, const R *b1v, R fps, R erp, Row<R> *rows) {
+    rows[0].J1[3] = 1; rows[1].J1[4] = 1; rows[2].J1[5] = 1;
+    if (b1v) { rows[0].J2[3] = -1; rows[1].J2[4] = -1; rows[2].J2[5] = -1; }
+    R qerr[4];
+    if (b1v) {
+        R qq[4];
+        qmul(qq, b0v + 4, b1v + 4, true, false);
+        qmul(qerr, qq, j.qrel, false, true);
+    } else {
+        qmul(qerr, b0v + 4, j.qrel, true, true);
+    }
+    if (qerr[0] < 0) { qerr[1] = -qerr[1]; qerr[2] = -qerr[2]; qerr[3] = -qerr[3]; }
+    R R0[12], e[3];
+    RfromQ(R0, b0v + 4);
+    mul0_331(e, R0, qerr + 1);
+    const R k2 = fps * erp * 2;
+    for (int r = 0; r < 3; r++) rows[r].rhs = k2 * e[r];
+}
+
+/* dxJointLimitMotor::addLimot joints/joint.cpp:598-786: the limit / motor row
+ * along ax1 (rotational or linear), and the force of a motor powered at its
+ * limit, added to the bodies' accumulators as the reference does. */
+template <class R> void limot_row(const Joint<R> &j, R *b0v, R *b1v, R fps, const R *ax1, bool rotational, Row<R> &row) {
+    int powered = j.fmax > 0;
+    if (!powered && !j.limit) return;
+    R *J1u = rotational ? row.J1 + 3 : row.J1, *J2u = rotational ? row.J2 + 3 : row.J2;
+    for (int k = 0; k < 3; k++) J1u[k] = ax1[k];
+    if (b1v) for (int k = 0; k < 3; k++) J2u[k] = -ax1[k];
+    R ltd[3] = {0, 0, 0};
+    if (!rotational && b1v) {
+        R c[3];
+        for (int k = 0; k < 3; k++) c[k] = (R)0.5 * (b1v[k] - b0v[k]);
+        cross3(ltd, c, ax1);
+        for (int k = 0; k < 3; k++) { row.J1[3 + k] = ltd[k]; row.J2[3 + k] = ltd[k]; }
+    }
+    if (j.limit && j.lostop == j.histop) powered = 0;
+    if (powered) {
+        row.cfm = j.normal_cfm;
+        if (!j.limit) {
+            row.rhs = j.vel;
+            row.lo = -j.fmax;
+            row.hi = j.fmax;
+        } else {
+            R fm = j.fmax;
+            if (j.vel > 0 || (j.vel == 0 && j.limit == 2)) fm = -fm;
+            if ((j.limit == 1 && j.vel > 0) || (j.limit == 2 && j.vel < 0)) fm *= j.fudge;
+            const R f[3] = {fm * ax1[0], fm * ax1[1], fm * ax1[2]};
+            if (rotational) {
+                if (b1v) for (int k = 0; k < 3; k++) b1v[20 + k] += f[k];
+                for (int k = 0; k < 3; k++) b0v[20 + k] -= f[k];
+            } else {
+                if (b1v) {
+                    for (int k = 0; k < 3; k++) { b0v[20 + k] += -fm * ltd[k]; b1v[20 + k] += -fm * ltd[k]; }
+                    for (int k = 0; k < 3; k++) b1v[16 + k] += f[k];
+                }
+                for (int k = 0; k < 3; k++) b0v[16 + k] -= f[k];
+            }
+        }
+    }
+    if (j.limit) {
+        const R k = fps * j.stop_erp;
+        row.rhs = -k * j.limit_err;
+        row.cfm = j.stop_cfm;
+        if (j.lostop == j.histop) {
+            row.lo = (R)-kInf;
+            row.hi = (R)kInf;
+        } else {
+            if (j.limit == 1) { row.lo = 0; row.hi = (R)kInf; }
+            else { row.lo = (R)-kInf; row.hi = 0; }
+            if (j.bounce > 0) {
+                R v;
+                if (rotational) {
+                    v = dot3(b0v + 12, ax1);
+                    if (b1v) v -= dot3(b1v + 12, ax1);
+                } else {
+                    v = dot3(b0v + 8, ax1);
+                    if (b1v) v -= dot3(b1v + 8, ax1);
+                }
+                if (j.limit == 1) {
+                    if (v < 0) { const R nc = -j.bounce * v; if (nc > row.rhs) row.rhs = nc; }
+                } else {
+                    if (v > 0) { const R nc = -j.bounce * v; if (nc < row.rhs) row.rhs = nc; }
+                }
+            }
+        }
+    }
+}
+
+/* getInfo2 of a hinge joint, hinge.cpp:80-153 */
+template <class R> void hinge_info2(const Joint<R> &j, R *b0v, R *b1v, R fps, R erp, Row<R> *rows) {
+    Joint<R> jb = j;
+    jb.erp = erp;
+    ball_info2(jb, b0v, b1v, fps, rows);
+    for (int r = 0; r < 3; r++) rows[r].cfm = j.cfm;
+    R R0[12], ax1[3], p[3], q[3], b[3];
+    RfromQ(R0, b0v + 4);
+    mul0_331(ax1, R0, j.axis1);
+    planeSpace(ax1, p, q);
+    for (int k = 0; k < 3; k++) { rows[3].J1[3 + k] = p[k]; rows[4].J1[3 + k] = q[k]; }
+    if (b1v) for (int k = 0; k < 3; k++) { rows[3].J2[3 + k] = -p[k]; rows[4].J2[3 + k] = -q[k]; }
+    if (b1v) {
+        R R1[12], ax2[3];
+        RfromQ(R1, b1v + 4);
+        mul0_331(ax2, R1, j.axis2);
+        cross3(b, ax1, ax2);
+    } else {
+        cross3(b, ax1, j.axis2);
+    }
+    const R k = fps * erp;
+    rows[3].rhs = k * dot3(b, p);
+    rows[4].rhs = k * dot3(b, q);
+    limot_row(j, b0v, b1v, fps, ax1, true, rows[5]);
+}
+
+/* getInfo2 of a slider joint, slider.cpp:146-250 */
+template <class R> void slider_info2(const Joint<R> &j, R *b0v, R *b1v, R fps, R erp, Row<R> *rows) {
+    fixed_orientation(j, b0v, b1v, fps, erp, rows);
+    R R0[12], ax1[3], p[3], q[3], c[3] = {0, 0, 0};
+    RfromQ(R0, b0v + 4);
+    mul0_331(ax1, R0, j.axis1);
+    planeSpace(ax1, p, q);
+    if (b1v) for (int k = 0; k < 3; k++) c[k] = b1v[k] - b0v[k];
+    for (int k = 0; k < 3; k++) { rows[3].J1[k] = p[k]; rows[4].J1[k] = q[k]; }
+    if (b1v) {
+        R t[3];
+        for (int k = 0; k < 3; k++) { rows[3].J2[k] = -p[k]; rows[4].J2[k] = -q[k]; }
+        cross3(t, c, p);
+        for (int k = 0; k < 3; k++) { rows[3].J1[3 + k] = (R)0.5 * t[k]; rows[3].J2[3 + k] = rows[3].J1[3 + k]; }
+        cross3(t, c, q);
+        for (int k = 0; k < 3; k++) { rows[4].J1[3 + k] = (R)0.5 * t[k]; rows[4].J2[3 + k] = rows[4].J1[3 + k]; }
+    }
+    const R k = fps * erp;
+    if (b1v) {
+        R R1[12], ofs[3];
+        RfromQ(R1, b1v + 4);
+        mul0_331(ofs, R1, j.offset);
+        for (int i = 0; i < 3; i++) c[i] += ofs[i];
+        rows[3].rhs = k * dot3(p, c);
+        rows[4].rhs = k * dot3(q, c);
+    } else {
+        R ofs[3];
+        for (int i = 0; i < 3; i++) ofs[i] = j.offset[i] - b0v[i];
+        rows[3].rhs = k * dot3(p, ofs);
+        rows[4].rhs = k * dot3(q, ofs);
+        if (j.reverse) for (int i = 0; i < 3; i++) ax1[i] = -ax1[i];
+    }
+    limot_row(j, b0v, b1v, fps, ax1, false, rows[5]);
+}
+
+/* getInfo2 of a fixed joint, fixed.cpp:68-105: the joint's own erp / cfm on
+ * the three position rows, the world's erp on the orientation rows. */
+template <class R> void fixed_info2(const Joint<R> &j, R *b0v, R *b1v, R fps, R erp, Row<R> *rows) {
+    fixed_orientation(j, b0v, b1v, fps, erp, rows + 3);
+    rows[0].J1[0] = 1; rows[1].J1[1] = 1; rows[2].J1[2] = 1;
+    const R k = fps * j.jerp;
+    R R0[12], ofs[3];
+    RfromQ(R0, b0v + 4);
+    mul0_331(ofs, R0, j.offset);
+    if (b1v) {
+        /* dSetCrossMatrixPlus(J1 + angular, ofs) */
+        rows[0].J1[4] = -ofs[2]; rows[0].J1[5] = ofs[1];
+        rows[1].J1[3] = ofs[2]; rows[1].J1[5] = -ofs[0];
+        rows[2].J1[3] = -ofs[1]; rows[2].J1[4] = ofs[0];
+        rows[0].J2[0] = -1; rows[1].J2[1] = -1; rows[2].J2[2] = -1;
+        for (int r = 0; r < 3; r++) rows[r].rhs = k * (b1v[r] - b0v[r] + ofs[r]);
+    } else {
+        for (int r = 0; r < 3; r++) rows[r].rhs = k * (j.offset[r] - b0v[r]);
+    }
+    for (int r = 0; r < 3; r++) rows[r].cfm = j.jcfm;
 }
 
 /* ---------------------------------------------------------- quickstep --- */
@@ -1262,7 +1523,10 @@ template <class R> void assemble(int nb, R *bodies, std::vector<Joint<R>> &joint
     std::vector<int> jm(joints.size());
     int m = 0;
     for (size_t i = 0; i < joints.size(); i++) {
-        jm[i] = joints[i].type == JT_CONTACT ? contact_m(joints[i].s) : 3;
+        Joint<R> &j = joints[i];
+        const R *b0v = bodies + (size_t)j.b1 * ORC_BODY_STRIDE;
+        const R *b1v = j.b2 >= 0 ? bodies + (size_t)j.b2 * ORC_BODY_STRIDE : nullptr;
+        jm[i] = j.type == JT_CONTACT ? contact_m(j.s) : joint_m(j, b0v, b1v);
         m += jm[i];
     }
     S.m = m;
@@ -1282,10 +1546,13 @@ template <class R> void assemble(int nb, R *bodies, std::vector<Joint<R>> &joint
             row.lo = (R)-kInf;
             row.hi = (R)kInf;
         }
-        const R *b0v = bodies + (size_t)j.b1 * ORC_BODY_STRIDE;
-        const R *b1v = j.b2 >= 0 ? bodies + (size_t)j.b2 * ORC_BODY_STRIDE : nullptr;
+        R *b0v = bodies + (size_t)j.b1 * ORC_BODY_STRIDE;
+        R *b1v = j.b2 >= 0 ? bodies + (size_t)j.b2 * ORC_BODY_STRIDE : nullptr;
         int fidx[6] = {-1, -1, -1, -1, -1, -1};
         if (j.type == JT_CONTACT) contact_info2(j, im, b0v, b1v, fps, p.erp, p.min_depth, p.max_vel, &S.J[ofs], fidx);
+        else if (j.type == JT_HINGE) hinge_info2(j, b0v, b1v, fps, p.erp, &S.J[ofs]);
+        else if (j.type == JT_SLIDER) slider_info2(j, b0v, b1v, fps, p.erp, &S.J[ofs]);
+        else if (j.type == JT_FIXED) fixed_info2(j, b0v, b1v, fps, p.erp, &S.J[ofs]);
         else ball_info2(j, b0v, b1v, fps, &S.J[ofs]);
         for (int r = 0; r < im; r++) {
             if (fidx[r] != -1) S.findex[ofs + r] = fidx[r] + ofs;
@@ -1517,6 +1784,16 @@ template <class R> void read_joint(const R *rec, const Params<R> &p, Joint<R> &j
     }
 }
 
+template <class R> void read_joint_ext(const R *x, Joint<R> &j) {
+    for (int i = 0; i < 3; i++) { j.axis1[i] = x[i]; j.axis2[i] = x[3 + i]; j.offset[i] = x[10 + i]; }
+    for (int i = 0; i < 4; i++) j.qrel[i] = x[6 + i];
+    j.limit = (int)x[14];
+    j.limit_err = 0;
+    j.lostop = x[16]; j.histop = x[17]; j.vel = x[18]; j.fmax = x[19]; j.fudge = x[20];
+    j.normal_cfm = x[21]; j.stop_erp = x[22]; j.stop_cfm = x[23]; j.bounce = x[24];
+    j.jerp = x[26]; j.jcfm = x[27];
+}
+
 /* dxQuickStepIsland quickstep.cpp:1112 (single-threaded path, deterministic
  * order: REORDERING_METHOD__DONT_REORDER, dynamic iteration count off). */
 template <class R> int quickstep(int nb, R *bodies, std::vector<Joint<R>> &joints, const Params<R> &p, R *lambda_out,
@@ -1585,10 +1862,14 @@ template <class R> int quickstep(int nb, R *bodies, std::vector<Joint<R>> &joint
     return S.m;
 }
 
-template <class R> int quickstep_api(int nb, R *bodies, int nj, const R *joints, const double *params, R *lambda, int cap) {
+template <class R> int quickstep_api(int nb, R *bodies, int nj, const R *joints, const double *params, R *lambda, int cap,
+                                     const R *jext = nullptr) {
     Params<R> p = read_params<R>(params);
     std::vector<Joint<R>> js(nj);
-    for (int i = 0; i < nj; i++) read_joint(joints + (size_t)i * ORC_JOINT_STRIDE, p, js[i]);
+    for (int i = 0; i < nj; i++) {
+        read_joint(joints + (size_t)i * ORC_JOINT_STRIDE, p, js[i]);
+        if (jext && js[i].type >= JT_HINGE) read_joint_ext(jext + (size_t)i * ORC_JEXT_STRIDE, js[i]);
+    }
     return quickstep(nb, bodies, js, p, lambda, cap);
 }
 
@@ -1816,6 +2097,14 @@ int orc_quickstep_f32(int nb, float *bodies, int nj, const float *joints, const 
 }
 int orc_quickstep_f64(int nb, double *bodies, int nj, const double *joints, const double *params, double *lambda, int cap) {
     return quickstep_api(nb, bodies, nj, joints, params, lambda, cap);
+}
+int orc_quickstep_ext_f32(int nb, float *bodies, int nj, const float *joints, const float *jext, const double *params,
+                          float *lambda, int cap) {
+    return quickstep_api(nb, bodies, nj, joints, params, lambda, cap, jext);
+}
+int orc_quickstep_ext_f64(int nb, double *bodies, int nj, const double *joints, const double *jext,
+                          const double *params, double *lambda, int cap) {
+    return quickstep_api(nb, bodies, nj, joints, params, lambda, cap, jext);
 }
 
 int orc_world_step_f32(int nb, float *bodies, int ng, const float *geoms, int nball, const float *balls, const double *params,

@@ -104,6 +104,12 @@ int orc_quickstep_f32(int nb, float *bodies, int nj, const float *joints,
                       const double *params, float *lambda, int lambda_cap);
 int orc_quickstep_f64(int nb, double *bodies, int nj, const double *joints,
                       const double *params, double *lambda, int lambda_cap);
+/* quickstep with hinge (type 2), slider (3) and fixed (4) joints: jext holds
+ * one 32-real extended record per joint (layout in oracle.cpp, ORC_JEXT_STRIDE). */
+int orc_quickstep_ext_f32(int nb, float *bodies, int nj, const float *joints, const float *jext, const double *params,
+                          float *lambda, int cap);
+int orc_quickstep_ext_f64(int nb, double *bodies, int nj, const double *joints, const double *jext,
+                          const double *params, double *lambda, int cap);
 
 /* Full canonical world step: AABBs -> pairs (geom index order) -> dCollide ->
  * contact joints -> QuickStep -> integrate.  Ball joints given in balls[nball][16]
