@@ -45,8 +45,16 @@ typedef enum {                               /* include/ode/common.h:382-400 */
     dJointTypeBall,
     dJointTypeHinge,
     dJointTypeSlider,
-    dJointTypeContact
+    dJointTypeContact,
+    dJointTypeUniversal,
+    dJointTypeHinge2,
+    dJointTypeFixed
 } dJointType;
+
+enum {                                       /* include/ode/common.h:439-456 (group 1) */
+    dParamLoStop = 0, dParamHiStop, dParamVel, dParamLoVel, dParamHiVel, dParamFMax, dParamFudgeFactor,
+    dParamBounce, dParamCFM, dParamStopERP, dParamStopCFM, dParamSuspensionERP, dParamSuspensionCFM, dParamERP
+};
 
 typedef struct dJointFeedback {              /* include/ode/common.h:516-521 */
     dVector3 f1, t1, f2, t2;
@@ -216,6 +224,29 @@ dBodyID dJointGetBody(dJointID j, int index);                              /* ob
 void dJointSetFeedback(dJointID j, dJointFeedback *fb);                    /* objects.h:1950 */
 void dJointSetBallAnchor(dJointID j, dReal x, dReal y, dReal z);           /* objects.h:1965 */
 void dJointSetBallAnchor2(dJointID j, dReal x, dReal y, dReal z);          /* objects.h:1971 */
+/* Hinge, slider and fixed joints with their limits and motors.  The rows are
+ * built in the step kernel; limit states are found at the start of
+ * dWorldQuickStep as getInfo1 does. */
+dJointID dJointCreateHinge(dWorldID w, dJointGroupID g);                   /* objects.h:1701 */
+dJointID dJointCreateSlider(dWorldID w, dJointGroupID g);                  /* objects.h:1709 */
+dJointID dJointCreateFixed(dWorldID w, dJointGroupID g);                   /* objects.h:1766 */
+void dJointSetHingeAnchor(dJointID j, dReal x, dReal y, dReal z);          /* objects.h:1983 */
+void dJointSetHingeAxis(dJointID j, dReal x, dReal y, dReal z);            /* objects.h:1991 */
+void dJointSetHingeParam(dJointID j, int parameter, dReal value);          /* objects.h:2027 */
+void dJointSetSliderAxis(dJointID j, dReal x, dReal y, dReal z);           /* objects.h:2043 */
+void dJointSetSliderParam(dJointID j, int parameter, dReal value);         /* objects.h:2054 */
+void dJointSetFixed(dJointID j);                                           /* objects.h:2451 */
+void dJointSetFixedParam(dJointID j, int parameter, dReal value);          /* objects.h:2458 */
+void dJointGetHingeAnchor(dJointID j, dVector3 result);                    /* objects.h:2582 */
+void dJointGetHingeAxis(dJointID j, dVector3 result);                      /* objects.h:2598 */
+dReal dJointGetHingeParam(dJointID j, int parameter);                      /* objects.h:2604 */
+dReal dJointGetHingeAngle(dJointID j);                                     /* objects.h:2618 */
+dReal dJointGetHingeAngleRate(dJointID j);                                 /* objects.h:2624 */
+dReal dJointGetSliderPosition(dJointID j);                                 /* objects.h:2637 */
+dReal dJointGetSliderPositionRate(dJointID j);                             /* objects.h:2643 */
+void dJointGetSliderAxis(dJointID j, dVector3 result);                     /* objects.h:2649 */
+dReal dJointGetSliderParam(dJointID j, int parameter);                     /* objects.h:2655 */
+dReal dJointGetFixedParam(dJointID j, int parameter);                      /* objects.h:3144 */
 int dAreConnected(dBodyID a, dBodyID b);                                   /* objects.h:3484 */
 int dAreConnectedExcluding(dBodyID a, dBodyID b, int joint_type);          /* objects.h:3499 */
 

@@ -232,6 +232,13 @@ struct dxJoint {
     int tag = 0;
     dxJoint *wnext = nullptr, *wprev = nullptr;  // the world's joint list (newest first, as addObjectToList)
     bool listed = false;
+    // hinge / slider / fixed (joints/hinge.h, slider.h, fixed.h)
+    dVector3 axis1 = {0, 0, 0, 0}, axis2 = {0, 0, 0, 0}, offset = {0, 0, 0, 0};
+    dQuaternion qrel = {1, 0, 0, 0};
+    // dxJointLimitMotor (joints/joint.h); set up by new_joint as init() does (joint.cpp:495-508)
+    float lostop = -dInfinity, histop = dInfinity, vel = 0, fmax = 0, fudge = 1, normal_cfm = 0, stop_erp = 0,
+          stop_cfm = 0, bounce = 0, limit_err = 0;
+    int limit = 0;
 };
 
 struct dxJointGroup {
@@ -352,6 +359,9 @@ dxJoint *new_joint(dxWorld *w, dxJointGroup *g, int type) {
     j->node[0].joint = j->node[1].joint = j;
     j->erp = w->erp;
     j->cfm = w->cfm;
+    j->normal_cfm = w->cfm;
+    j->stop_erp = w->erp;
+    j->stop_cfm = w->cfm;
     if (g) g->joints.push_back(j);
     else w->free_joints.push_back(j);
     list_joint(w, j);
@@ -417,6 +427,108 @@ struct KernelTiming {
     double ms = 0.0, row_sweeps = 0.0;
 };
 KernelTiming g_ktime;
+
+// dMultiply0_331 / dMultiply1_331 with a 3x4 rotation: R v and R^T v
+void rmul(float *r, const float *R, const float *v) {
+    for (int i = 0; i < 3; i++) r[i] = R[i * 4] * v[0] + R[i * 4 + 1] * v[1] + R[i * 4 + 2] * v[2];
+}
+void rtmul(float *r, const float *R, const float *v) {
+    for (int i = 0; i < 3; i++) r[i] = R[i] * v[0] + R[4 + i] * v[1] + R[8 + i] * v[2];
+}
+// qa = qb * qc with qb and / or qc conjugated (dQMultiply0..3, rotation.cpp:41-107)
+void qmul_h(float *qa, const float *qb, const float *qc, bool cb, bool cc) {
+    const float b1 = cb ? -qb[1] : qb[1], b2 = cb ? -qb[2] : qb[2], b3 = cb ? -qb[3] : qb[3];
+    const float c1 = cc ? -qc[1] : qc[1], c2 = cc ? -qc[2] : qc[2], c3 = cc ? -qc[3] : qc[3];
+    qa[0] = qb[0] * qc[0] - b1 * c1 - b2 * c2 - b3 * c3;
+    qa[1] = qb[0] * c1 + b1 * qc[0] + b2 * c3 - b3 * c2;
+    qa[2] = qb[0] * c2 + b2 * qc[0] + b3 * c1 - b1 * c3;
+    qa[3] = qb[0] * c3 + b3 * qc[0] + b1 * c2 - b2 * c1;
+}
+// computeInitialRelativeRotation of hinge / slider / fixed (hinge.cpp:376-394)
+void joint_qrel(dxJoint *j) {
+    dxBody *b0 = j->node[0].body, *b1 = j->node[1].body;
+    if (!b0) return;
+    if (b1) qmul_h(j->qrel, b0->q, b1->q, true, false);
+    else { j->qrel[0] = b0->q[0]; j->qrel[1] = -b0->q[1]; j->qrel[2] = -b0->q[2]; j->qrel[3] = -b0->q[3]; }
+}
+// getHingeAngle joints/joint.cpp:421-490
+float hinge_angle_h(const dxJoint *j) {
+    dxBody *b0 = j->node[0].body, *b1 = j->node[1].body;
+    float qr[4];
+    if (b1) { float qq[4]; qmul_h(qq, b0->q, b1->q, true, false); qmul_h(qr, qq, j->qrel, false, true); }
+    else qmul_h(qr, b0->q, j->qrel, true, true);
+    const float cost2 = qr[0], sint2 = std::sqrt(qr[1] * qr[1] + qr[2] * qr[2] + qr[3] * qr[3]);
+    const float d = qr[1] * j->axis1[0] + qr[2] * j->axis1[1] + qr[3] * j->axis1[2];
+    float theta = d >= 0 ? 2 * std::atan2(sint2, cost2) : 2 * std::atan2(sint2, -cost2);
+    if (theta > (float)M_PI) theta -= (float)(2 * M_PI);
+    return -theta;
+}
+// dJointGetSliderPosition joints/slider.cpp:45-82 (before the reverse sign of
+// the single-body case, which the caller applies)
+float slider_position_h(const dxJoint *j) {
+    dxBody *b0 = j->node[0].body, *b1 = j->node[1].body;
+    float ax1[3], q[3];
+    rmul(ax1, b0->R, j->axis1);
+    if (b1) {
+        rmul(q, b1->R, j->offset);
+        for (int i = 0; i < 3; i++) q[i] = b0->pos[i] - q[i] - b1->pos[i];
+    } else {
+        for (int i = 0; i < 3; i++) q[i] = b0->pos[i] - j->offset[i];
+        if (j->reverse) for (int i = 0; i < 3; i++) ax1[i] = -ax1[i];
+    }
+    return ax1[0] * q[0] + ax1[1] * q[1] + ax1[2] * q[2];
+}
+// getInfo1 of hinge (hinge.cpp:58-77) and slider (slider.cpp:112-143): rows,
+// and the limit state the limit / motor row uses
+int hinge_slider_m(dxJoint *j) {
+    int m = j->fmax > 0 ? 6 : 5;
+    if (j->type == dJointTypeHinge) {
+        if ((j->lostop >= -(float)M_PI || j->histop <= (float)M_PI) && j->lostop <= j->histop) {
+            const float a = hinge_angle_h(j);
+            if (a <= j->lostop) { j->limit = 1; j->limit_err = a - j->lostop; }
+            else if (a >= j->histop) { j->limit = 2; j->limit_err = a - j->histop; }
+            else j->limit = 0;
+        }
+    } else {
+        j->limit = 0;
+        if ((j->lostop > -dInfinity || j->histop < dInfinity) && j->lostop <= j->histop) {
+            const float pos = slider_position_h(j);
+            if (pos <= j->lostop) { j->limit = 1; j->limit_err = pos - j->lostop; }
+            else if (pos >= j->histop) { j->limit = 2; j->limit_err = pos - j->histop; }
+        }
+    }
+    return j->limit ? 6 : m;
+}
+// dxJointLimitMotor::set joints/joint.cpp:511-545 (limits and motors: the
+// group-1 parameters)
+void limot_set(dxJoint *j, int num, float v) {
+    switch (num) {
+    case 0: j->lostop = v; break;
+    case 1: j->histop = v; break;
+    case 2: j->vel = v; break;
+    case 5: if (v >= 0) j->fmax = v; break;
+    case 6: if (v >= 0 && v <= 1) j->fudge = v; break;
+    case 7: j->bounce = v; break;
+    case 8: j->normal_cfm = v; break;
+    case 9: j->stop_erp = v; break;
+    case 10: j->stop_cfm = v; break;
+    default: break;
+    }
+}
+float limot_get(const dxJoint *j, int num) {
+    switch (num) {
+    case 0: return j->lostop;
+    case 1: return j->histop;
+    case 2: return j->vel;
+    case 5: return j->fmax;
+    case 6: return j->fudge;
+    case 7: return j->bounce;
+    case 8: return j->normal_cfm;
+    case 9: return j->stop_erp;
+    case 10: return j->stop_cfm;
+    default: return 0;
+    }
+}
 
 bool gpu_collide_pairs(Session &S, int maxc) {
     const int np = (int)S.pairs.size() / 2;
@@ -561,6 +673,35 @@ void dif_geom(DifWriter &c, const dxGeom *g) {
     }
 }
 
+// printLimot export-dif.cpp:142-183
+void dif_limot(DifWriter &c, const dxJoint *j) {
+    c.line("limit = {");
+    c.indent++;
+    c.num("low_stop", j->lostop);
+    c.num("high_stop", j->histop);
+    if (j->bounce != 0) c.num("bounce", j->bounce);
+    c.line("ODE = {");
+    c.indent++;
+    if (j->stop_erp != 0) c.num("stop_erp", j->stop_erp);
+    if (j->stop_cfm != 0) c.num("stop_cfm", j->stop_cfm);
+    c.indent--;
+    c.line("},");
+    c.indent--;
+    c.line("},");
+    c.line("motor = {");
+    c.indent++;
+    if (j->vel != 0) c.num("vel", j->vel);
+    if (j->fmax != 0) c.num("fmax", j->fmax);
+    c.line("ODE = {");
+    c.indent++;
+    if (j->fudge != 0) c.num("fudge_factor", j->fudge);
+    if (j->normal_cfm != 0) c.num("normal_cfm", j->normal_cfm);
+    c.indent--;
+    c.line("},");
+    c.indent--;
+    c.line("},");
+}
+
 }  // namespace
 
 extern "C" {
@@ -650,7 +791,12 @@ void dWorldExportDIF(dWorldID w, FILE *file, const char *prefix) {
     std::fprintf(file, "%sjoint = {}\n", prefix);
     for (dxJoint *j = w->firstjoint; j; j = j->wnext) {
         c.indent++;
-        const char *name = j->type == dJointTypeBall ? "ball" : j->type == dJointTypeContact ? "contact" : "unknown";
+        const char *name = j->type == dJointTypeBall      ? "ball"
+                           : j->type == dJointTypeContact ? "contact"
+                           : j->type == dJointTypeHinge   ? "hinge"
+                           : j->type == dJointTypeSlider  ? "slider"
+                           : j->type == dJointTypeFixed   ? "fixed"
+                                                          : "unknown";
         std::fprintf(file, "%sjoint[%d] = dynamics.%s_joint {\n\tworld = %sworld,\n\tbody = {", prefix, num, name, prefix);
         if (j->node[0].body) std::fprintf(file, "%sbody[%d]", prefix, j->node[0].body->tag);
         if (j->node[1].body) std::fprintf(file, ",%sbody[%d]", prefix, j->node[1].body->tag);
@@ -677,6 +823,21 @@ void dWorldExportDIF(dWorldID w, FILE *file, const char *prefix) {
             const int fa = ((mode & dContactApprox1_1) ? 1 : 0) | ((mode & dContactApprox1_2) ? 2 : 0);
             if (fa) c.num("friction_approximation", fa);
             if (mode & dContactFDir1) c.vec("fdir1", ct.fdir1);
+        } else if (j->type == dJointTypeHinge) {  // printHinge :214-223
+            c.vec("anchor1", j->anchor1);
+            c.vec("anchor2", j->anchor2);
+            c.vec("axis1", j->axis1);
+            c.vec("axis2", j->axis2);
+            c.vec("qrel", j->qrel, 4);
+            dif_limot(c, j);
+        } else if (j->type == dJointTypeSlider) {  // printSlider :226-233
+            c.vec("axis1", j->axis1);
+            c.vec("qrel", j->qrel, 4);
+            c.vec("offset", j->offset);
+            dif_limot(c, j);
+        } else if (j->type == dJointTypeFixed) {  // printFixed :333-338 (three components of qrel)
+            c.vec("qrel", j->qrel);
+            c.vec("offset", j->offset);
         } else {
             c.line("unknown joint");
         }
@@ -915,6 +1076,118 @@ void dJointSetBallAnchor(dJointID j, dReal x, dReal y, dReal z) {
 }
 void dJointSetBallAnchor2(dJointID j, dReal x, dReal y, dReal z) {
     j->anchor2[0] = x; j->anchor2[1] = y; j->anchor2[2] = z; j->anchor2[3] = 0;
+}
+
+// ---- hinge, slider, fixed (joints/hinge.cpp, slider.cpp, fixed.cpp)
+dJointID dJointCreateHinge(dWorldID w, dJointGroupID g) { return new_joint(w, g, dJointTypeHinge); }
+dJointID dJointCreateSlider(dWorldID w, dJointGroupID g) { return new_joint(w, g, dJointTypeSlider); }
+dJointID dJointCreateFixed(dWorldID w, dJointGroupID g) { return new_joint(w, g, dJointTypeFixed); }
+
+void dJointSetHingeAnchor(dJointID j, dReal x, dReal y, dReal z) {
+    dJointSetBallAnchor(j, x, y, z);  // setAnchors
+    joint_qrel(j);
+}
+// setAxes joints/joint.cpp:325-360 (the direction normalised first)
+static void set_axes(dxJoint *j, float x, float y, float z, float *axis1, float *axis2) {
+    dxBody *b0 = j->node[0].body;
+    if (!b0) return;
+    float q[3] = {x, y, z};
+    const float l = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+    if (l > 0) for (int i = 0; i < 3; i++) q[i] /= l;
+    if (axis1) { rtmul(axis1, b0->R, q); axis1[3] = 0; }
+    if (axis2) {
+        if (j->node[1].body) rtmul(axis2, j->node[1].body->R, q);
+        else { axis2[0] = x; axis2[1] = y; axis2[2] = z; }
+        axis2[3] = 0;
+    }
+}
+void dJointSetHingeAxis(dJointID j, dReal x, dReal y, dReal z) {
+    set_axes(j, x, y, z, j->axis1, j->axis2);
+    joint_qrel(j);
+}
+void dJointGetHingeAnchor(dJointID j, dVector3 r) {
+    // getAnchor / getAnchor2 (joint.cpp:363-392), swapped for a reversed joint
+    dxBody *b0 = j->node[0].body, *b1 = j->node[1].body;
+    const bool second = j->reverse;
+    if (!second) {
+        if (!b0) return;
+        rmul(r, b0->R, j->anchor1);
+        for (int i = 0; i < 3; i++) r[i] += b0->pos[i];
+    } else if (b1) {
+        rmul(r, b1->R, j->anchor2);
+        for (int i = 0; i < 3; i++) r[i] += b1->pos[i];
+    } else {
+        for (int i = 0; i < 3; i++) r[i] = j->anchor2[i];
+    }
+}
+void dJointGetHingeAxis(dJointID j, dVector3 r) {
+    if (j->node[0].body) rmul(r, j->node[0].body->R, j->axis1);
+}
+void dJointSetHingeParam(dJointID j, int parameter, dReal value) { limot_set(j, parameter, value); }
+dReal dJointGetHingeParam(dJointID j, int parameter) { return limot_get(j, parameter); }
+dReal dJointGetHingeAngle(dJointID j) {
+    if (!j->node[0].body) return 0;
+    const float a = hinge_angle_h(j);
+    return j->reverse ? -a : a;
+}
+dReal dJointGetHingeAngleRate(dJointID j) {
+    dxBody *b0 = j->node[0].body, *b1 = j->node[1].body;
+    if (!b0) return 0;
+    float ax[3];
+    rmul(ax, b0->R, j->axis1);
+    float rate = ax[0] * b0->avel[0] + ax[1] * b0->avel[1] + ax[2] * b0->avel[2];
+    if (b1) rate -= ax[0] * b1->avel[0] + ax[1] * b1->avel[1] + ax[2] * b1->avel[2];
+    return j->reverse ? -rate : rate;
+}
+
+void dJointSetSliderAxis(dJointID j, dReal x, dReal y, dReal z) {
+    set_axes(j, x, y, z, j->axis1, nullptr);
+    // computeOffset slider.cpp:406-423
+    dxBody *b0 = j->node[0].body, *b1 = j->node[1].body;
+    if (b1) {
+        float c[3] = {b0->pos[0] - b1->pos[0], b0->pos[1] - b1->pos[1], b0->pos[2] - b1->pos[2]};
+        rtmul(j->offset, b1->R, c);
+    } else if (b0) {
+        for (int i = 0; i < 3; i++) j->offset[i] = b0->pos[i];
+    }
+    joint_qrel(j);
+}
+void dJointGetSliderAxis(dJointID j, dVector3 r) {
+    if (j->node[0].body) rmul(r, j->node[0].body->R, j->axis1);
+}
+void dJointSetSliderParam(dJointID j, int parameter, dReal value) { limot_set(j, parameter, value); }
+dReal dJointGetSliderParam(dJointID j, int parameter) { return limot_get(j, parameter); }
+dReal dJointGetSliderPosition(dJointID j) { return j->node[0].body ? slider_position_h(j) : 0; }
+dReal dJointGetSliderPositionRate(dJointID j) {
+    dxBody *b0 = j->node[0].body, *b1 = j->node[1].body;
+    if (!b0) return 0;
+    float ax[3];
+    rmul(ax, b0->R, j->axis1);
+    float rate = ax[0] * b0->lvel[0] + ax[1] * b0->lvel[1] + ax[2] * b0->lvel[2];
+    if (b1) return rate - (ax[0] * b1->lvel[0] + ax[1] * b1->lvel[1] + ax[2] * b1->lvel[2]);
+    return j->reverse ? -rate : rate;
+}
+
+void dJointSetFixed(dJointID j) {
+    // fixed.cpp:108-136: body 1's centre relative to body 2 in body 1's frame
+    dxBody *b0 = j->node[0].body, *b1 = j->node[1].body;
+    if (b0) {
+        if (b1) {
+            float ofs[3] = {b0->pos[0] - b1->pos[0], b0->pos[1] - b1->pos[1], b0->pos[2] - b1->pos[2]};
+            rtmul(j->offset, b0->R, ofs);
+        } else {
+            for (int i = 0; i < 3; i++) j->offset[i] = b0->pos[i];
+        }
+    }
+    joint_qrel(j);
+}
+void dJointSetFixedParam(dJointID j, int parameter, dReal value) {
+    // fixed.cpp:138-150: dParamCFM, dParamERP
+    if (parameter == 8) j->cfm = value;
+    else if (parameter == 13) j->erp = value;
+}
+dReal dJointGetFixedParam(dJointID j, int parameter) {
+    return parameter == 8 ? j->cfm : parameter == 13 ? j->erp : 0;
 }
 int dAreConnected(dBodyID a, dBodyID b) {
     for (dxJointNode *n = a->firstjoint; n; n = n->next)
@@ -1408,7 +1681,7 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
         odx_invert_pd3(I, &invIb[(size_t)i * 12]);
     }
     const int nj = (int)joints.size();
-    std::vector<float> jrec((size_t)nj * JOINT_STRIDE, 0.f);
+    std::vector<float> jrec((size_t)nj * JOINT_STRIDE, 0.f), jext;
     std::vector<Surface> surf;
     std::vector<int> joff(nj), jm(nj);
     std::vector<int> rb1, rb2, rfl;
@@ -1444,13 +1717,63 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
                 if (s.mu > 0) { rfl.push_back((s.mode & dContactApprox1_1) ? 1 : 0); row++; }
                 if (row < mj) rfl.push_back((s.mode & dContactApprox1_2) ? 1 : 0);
             }
-        } else {
+        } else if (j->type == dJointTypeBall) {
             r[0] = 1;
             for (int i = 0; i < 3; i++) { r[3 + i] = j->anchor1[i]; r[6 + i] = j->anchor2[i]; }
             r[9] = j->erp;
             r[10] = j->cfm;
             mj = 3;
             rfl.push_back(0); rfl.push_back(0); rfl.push_back(0);
+        } else {
+            // hinge / slider / fixed: getInfo1 here, getInfo2 in the kernel
+            r[0] = (float)(j->type == dJointTypeHinge ? JT_HINGE : j->type == dJointTypeSlider ? JT_SLIDER : JT_FIXED);
+            for (int i = 0; i < 3; i++) { r[3 + i] = j->anchor1[i]; r[6 + i] = j->anchor2[i]; }
+            r[9] = w->erp;
+            r[10] = w->cfm;  // the ball rows of a hinge keep the world's cfm
+            if (jext.empty()) jext.assign((size_t)nj * JEXT_STRIDE, 0.f);
+            float *x = &jext[(size_t)k * JEXT_STRIDE];
+            for (int i = 0; i < 3; i++) { x[JX_AXIS1 + i] = j->axis1[i]; x[JX_AXIS2 + i] = j->axis2[i]; x[JX_OFFSET + i] = j->offset[i]; }
+            for (int i = 0; i < 4; i++) x[JX_QREL + i] = j->qrel[i];
+            mj = j->type == dJointTypeFixed ? 6 : hinge_slider_m(j);
+            x[JX_LIMOT_ROW] = (j->type != dJointTypeFixed && mj == 6) ? 1.f : 0.f;
+            x[JX_LIMIT] = (float)j->limit;
+            x[JX_LIMIT_ERR] = j->limit_err;
+            x[JX_LOSTOP] = j->lostop; x[JX_HISTOP] = j->histop; x[JX_VEL] = j->vel; x[JX_FMAX] = j->fmax;
+            x[JX_FUDGE] = j->fudge; x[JX_NORMAL_CFM] = j->normal_cfm; x[JX_STOP_ERP] = j->stop_erp;
+            x[JX_STOP_CFM] = j->stop_cfm; x[JX_BOUNCE] = j->bounce; x[JX_REVERSE] = j->reverse ? 1.f : 0.f;
+            x[JX_ERP] = j->erp; x[JX_CFM] = j->cfm;
+            // a motor powered at its limit pushes with (a fraction of) fmax
+            // through the body accumulators (addLimot joint.cpp:640-705)
+            if (j->type != dJointTypeFixed && j->fmax > 0 && j->limit && j->lostop != j->histop) {
+                float fm = j->fmax;
+                if (j->vel > 0 || (j->vel == 0 && j->limit == 2)) fm = -fm;
+                if ((j->limit == 1 && j->vel > 0) || (j->limit == 2 && j->vel < 0)) fm *= j->fudge;
+                dxBody *b0 = j->node[0].body, *b1 = j->node[1].body;
+                float ax1[3];
+                rmul(ax1, b0->R, j->axis1);
+                const bool rot = j->type == dJointTypeHinge;
+                if (!rot && !b1 && j->reverse) for (int i = 0; i < 3; i++) ax1[i] = -ax1[i];
+                const float f[3] = {fm * ax1[0], fm * ax1[1], fm * ax1[2]};
+                float4 &t0 = tacc[b0->tag], &f0 = facc[b0->tag];
+                if (rot) {
+                    if (b1) { float4 &t1 = tacc[b1->tag]; t1.x += f[0]; t1.y += f[1]; t1.z += f[2]; }
+                    t0.x -= f[0]; t0.y -= f[1]; t0.z -= f[2];
+                } else {
+                    if (b1) {
+                        float c[3] = {0.5f * (b1->pos[0] - b0->pos[0]), 0.5f * (b1->pos[1] - b0->pos[1]),
+                                      0.5f * (b1->pos[2] - b0->pos[2])}, ltd[3];
+                        ltd[0] = c[1] * ax1[2] - c[2] * ax1[1];
+                        ltd[1] = c[2] * ax1[0] - c[0] * ax1[2];
+                        ltd[2] = c[0] * ax1[1] - c[1] * ax1[0];
+                        float4 &t1 = tacc[b1->tag], &f1 = facc[b1->tag];
+                        t0.x += -fm * ltd[0]; t0.y += -fm * ltd[1]; t0.z += -fm * ltd[2];
+                        t1.x += -fm * ltd[0]; t1.y += -fm * ltd[1]; t1.z += -fm * ltd[2];
+                        f1.x += f[0]; f1.y += f[1]; f1.z += f[2];
+                    }
+                    f0.x -= f[0]; f0.y -= f[1]; f0.z -= f[2];
+                }
+            }
+            for (int q = 0; q < mj; q++) rfl.push_back(0);
         }
         joff[k] = m;
         jm[k] = mj;
@@ -1665,6 +1988,7 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     const size_t o_counts = G.slot(sizeof cnts), o_damp = G.slot(damp.size() * sizeof(float4));
     const size_t o_island = G.slot(dyn_on ? island.size() * sizeof(int) : 0);
     const size_t o_seg = G.slot(seg_table.size() * sizeof(int));
+    const size_t o_jext = G.slot(jext.size() * sizeof(float));
     bool ok = G.alloc() && w->rows.ensure((size_t)std::max(m, 1) * ROW_STRIDE * sizeof(float)) &&
               w->lambda.ensure((size_t)std::max(m, 1) * sizeof(float)) &&
               w->bscr.ensure((size_t)std::max(nb, 1) * BSCR_STRIDE * sizeof(float));
@@ -1678,6 +2002,7 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     G.put(o_damp, damp.data(), damp.size());
     if (dyn_on) G.put(o_island, island.data(), island.size());
     G.put(o_seg, seg_table.data(), seg_table.size());
+    G.put(o_jext, jext.data(), jext.size());
     if (!G.send(0, G.used)) return 0;
     DevWorlds D{};
     D.nworlds = 1;
@@ -1701,6 +2026,7 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     D.joff = G.d<int>(o_joff);
     D.fb = want_fb ? G.d<float>(o_fb) : nullptr;
     D.damp = damp.empty() ? nullptr : G.d<float4>(o_damp);
+    D.jext = jext.empty() ? nullptr : G.d<float>(o_jext);
     StepParams P{};
     P.h = stepsize;
     P.gx = w->gravity[0]; P.gy = w->gravity[1]; P.gz = w->gravity[2];

@@ -146,11 +146,27 @@ struct DevWorlds {
     // drop-in world with random reordering: sweeps 8s .. 8s+7 use segment s of
     // the schedule (seg_table[3s] = first item, [3s+1] = first phase start,
     // [3s+2] = phases; the last segment serves any later sweep)
+    // drop-in world: one extended record per joint for hinge, slider and fixed
+    // joints (JEXT_* layout), may be null
+    const float *jext;      // [joint cap * JEXT_STRIDE]
     const int *seg_table;   // [3 * nseg], may be null
     int nseg;
     int *island_sweeps;     // [nislands] sweeps each island ran, may be null
 };
 constexpr int PTIME_SLOTS = 6;
+
+// JOINT record types: 0 contact, 1 ball, 2 hinge, 3 slider, 4 fixed.  Hinge,
+// slider and fixed joints carry an extended record of JEXT_STRIDE floats:
+// axis1 (body-1 frame), axis2 (body-2 frame or world), the initial relative
+// rotation qrel, the offset (slider / fixed), whether the limit / motor row is
+// present, the limit state and error from getInfo1, the limit / motor
+// parameters (dxJointLimitMotor, joints/joint.h) and the fixed joint's own
+// erp / cfm.
+constexpr int JEXT_STRIDE = 32;
+enum { JT_HINGE = 2, JT_SLIDER = 3, JT_FIXED = 4 };
+enum { JX_AXIS1 = 0, JX_AXIS2 = 3, JX_QREL = 6, JX_OFFSET = 10, JX_LIMOT_ROW = 13, JX_LIMIT = 14, JX_LIMIT_ERR = 15,
+       JX_LOSTOP = 16, JX_HISTOP = 17, JX_VEL = 18, JX_FMAX = 19, JX_FUDGE = 20, JX_NORMAL_CFM = 21,
+       JX_STOP_ERP = 22, JX_STOP_CFM = 23, JX_BOUNCE = 24, JX_REVERSE = 25, JX_ERP = 26, JX_CFM = 27 };
 
 // DAMP_* bits of a body's damping record (BODY record flags bits 2-4 >> 2)
 enum { DAMP_LINEAR = 1, DAMP_ANGULAR = 2, DAMP_MAX_ANGULAR_SPEED = 4 };

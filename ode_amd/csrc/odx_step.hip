@@ -317,6 +317,184 @@ __device__ int ball_info2(const DevWorlds &D, const StepParams &P, const float *
     return 3;
 }
 
+// qa = qb * qc with qb and / or qc conjugated (dQMultiply0..3, rotation.cpp:41-107)
+__device__ __forceinline__ void qmul(float *qa, const float *qb, const float *qc, bool cb, bool cc) {
+    const float b1 = cb ? -qb[1] : qb[1], b2 = cb ? -qb[2] : qb[2], b3 = cb ? -qb[3] : qb[3];
+    const float c1 = cc ? -qc[1] : qc[1], c2 = cc ? -qc[2] : qc[2], c3 = cc ? -qc[3] : qc[3];
+    qa[0] = qb[0] * qc[0] - b1 * c1 - b2 * c2 - b3 * c3;
+    qa[1] = qb[0] * c1 + b1 * qc[0] + b2 * c3 - b3 * c2;
+    qa[2] = qb[0] * c2 + b2 * qc[0] + b3 * c1 - b1 * c3;
+    qa[3] = qb[0] * c3 + b3 * qc[0] + b1 * c2 - b2 * c1;
+}
+
+// setFixedOrientation joints/joint.cpp:228-285: angular rows making the
+// relative rotation equal qrel; rows[0..2]
+__device__ void fixed_orientation_rows(const DevWorlds &D, int b0, int b1, int b2, const float *x, float fps, float erp,
+                                       RowBuf *rows) {
+    rows[0].J1[3] = 1.f; rows[1].J1[4] = 1.f; rows[2].J1[5] = 1.f;
+    if (b2 >= 0) { rows[0].J2[3] = -1.f; rows[1].J2[4] = -1.f; rows[2].J2[5] = -1.f; }
+    const float4 q1v = D.quat[b0 + b1];
+    const float q1[4] = {q1v.x, q1v.y, q1v.z, q1v.w};
+    float qerr[4];
+    if (b2 >= 0) {
+        const float4 q2v = D.quat[b0 + b2];
+        const float q2[4] = {q2v.x, q2v.y, q2v.z, q2v.w};
+        float qq[4];
+        qmul(qq, q1, q2, true, false);
+        qmul(qerr, qq, x + JX_QREL, false, true);
+    } else {
+        qmul(qerr, q1, x + JX_QREL, true, true);
+    }
+    if (qerr[0] < 0.f) { qerr[1] = -qerr[1]; qerr[2] = -qerr[2]; qerr[3] = -qerr[3]; }
+    float e[3];
+    mul0_331(e, D.bscr + (size_t)(b0 + b1) * BSCR_STRIDE, qerr + 1);
+    const float k2 = fps * erp * 2.f;
+    for (int r = 0; r < 3; r++) rows[r].rhs = k2 * e[r];
+}
+
+// dxJointLimitMotor::addLimot joints/joint.cpp:598-786 for the row along ax1;
+// the force of a motor powered at its limit is added to the body accumulators
+// on the host before the step
+__device__ void limot_row(const DevWorlds &D, int b0, int b1, int b2, const float *x, float fps, const float *ax1,
+                          bool rotational, RowBuf &row) {
+    const int limit = (int)x[JX_LIMIT];
+    bool powered = x[JX_FMAX] > 0.f;
+    float *J1u = rotational ? row.J1 + 3 : row.J1, *J2u = rotational ? row.J2 + 3 : row.J2;
+    for (int k = 0; k < 3; k++) J1u[k] = ax1[k];
+    if (b2 >= 0) for (int k = 0; k < 3; k++) J2u[k] = -ax1[k];
+    if (!rotational && b2 >= 0) {  // linear limot torque decoupling
+        const float4 p1 = D.pos[b0 + b1], p2 = D.pos[b0 + b2];
+        const float c[3] = {0.5f * (p2.x - p1.x), 0.5f * (p2.y - p1.y), 0.5f * (p2.z - p1.z)};
+        float ltd[3];
+        cross3(ltd, c, ax1);
+        for (int k = 0; k < 3; k++) { row.J1[3 + k] = ltd[k]; row.J2[3 + k] = ltd[k]; }
+    }
+    const float lo = x[JX_LOSTOP], hi = x[JX_HISTOP];
+    if (limit && lo == hi) powered = false;
+    if (powered) {
+        row.cfm = x[JX_NORMAL_CFM];
+        if (!limit) {
+            row.rhs = x[JX_VEL];
+            row.lo = -x[JX_FMAX];
+            row.hi = x[JX_FMAX];
+        }
+    }
+    if (limit) {
+        row.rhs = -fps * x[JX_STOP_ERP] * x[JX_LIMIT_ERR];
+        row.cfm = x[JX_STOP_CFM];
+        if (lo == hi) {
+            row.lo = -kInf;
+            row.hi = kInf;
+        } else {
+            if (limit == 1) { row.lo = 0.f; row.hi = kInf; }
+            else { row.lo = -kInf; row.hi = 0.f; }
+            const float bounce = x[JX_BOUNCE];
+            if (bounce > 0.f) {
+                float v;
+                if (rotational) {
+                    const float4 w1 = D.avel[b0 + b1];
+                    v = w1.x * ax1[0] + w1.y * ax1[1] + w1.z * ax1[2];
+                    if (b2 >= 0) { const float4 w2 = D.avel[b0 + b2]; v -= w2.x * ax1[0] + w2.y * ax1[1] + w2.z * ax1[2]; }
+                } else {
+                    const float4 v1 = D.lvel[b0 + b1];
+                    v = v1.x * ax1[0] + v1.y * ax1[1] + v1.z * ax1[2];
+                    if (b2 >= 0) { const float4 v2 = D.lvel[b0 + b2]; v -= v2.x * ax1[0] + v2.y * ax1[1] + v2.z * ax1[2]; }
+                }
+                if (limit == 1) {
+                    if (v < 0.f) { const float nc = -bounce * v; if (nc > row.rhs) row.rhs = nc; }
+                } else {
+                    if (v > 0.f) { const float nc = -bounce * v; if (nc < row.rhs) row.rhs = nc; }
+                }
+            }
+        }
+    }
+}
+
+// getInfo2 of hinge (hinge.cpp:80-153), slider (slider.cpp:146-250) and fixed
+// (fixed.cpp:68-105) joints
+__device__ int hinge_slider_fixed_info2(const DevWorlds &D, const StepParams &P, const float *rec, const float *x,
+                                        int b0, RowBuf *rows) {
+    const int type = (int)rec[0];
+    const int b1 = (int)rec[1], b2 = (int)rec[2];
+    const float fps = 1.f / P.h;
+    const float *R1 = D.bscr + (size_t)(b0 + b1) * BSCR_STRIDE;
+    const float4 p1v = D.pos[b0 + b1];
+    const float x1[3] = {p1v.x, p1v.y, p1v.z};
+    float ax1[3];
+    mul0_331(ax1, R1, x + JX_AXIS1);
+    if (type == JT_FIXED) {
+        fixed_orientation_rows(D, b0, b1, b2, x, fps, P.erp, rows + 3);
+        rows[0].J1[0] = 1.f; rows[1].J1[1] = 1.f; rows[2].J1[2] = 1.f;
+        const float k = fps * x[JX_ERP];
+        float ofs[3];
+        mul0_331(ofs, R1, x + JX_OFFSET);
+        if (b2 >= 0) {
+            rows[0].J1[4] = -ofs[2]; rows[0].J1[5] = ofs[1];
+            rows[1].J1[3] = ofs[2]; rows[1].J1[5] = -ofs[0];
+            rows[2].J1[3] = -ofs[1]; rows[2].J1[4] = ofs[0];
+            rows[0].J2[0] = -1.f; rows[1].J2[1] = -1.f; rows[2].J2[2] = -1.f;
+            const float4 p2v = D.pos[b0 + b2];
+            const float x2[3] = {p2v.x, p2v.y, p2v.z};
+            for (int r = 0; r < 3; r++) rows[r].rhs = k * (x2[r] - x1[r] + ofs[r]);
+        } else {
+            for (int r = 0; r < 3; r++) rows[r].rhs = k * (x[JX_OFFSET + r] - x1[r]);
+        }
+        for (int r = 0; r < 3; r++) rows[r].cfm = x[JX_CFM];
+        return 6;
+    }
+    float p[3], q[3];
+    planeSpace(ax1, p, q);
+    const float k = fps * P.erp;
+    if (type == JT_HINGE) {
+        // the ball rows with the world's erp, then the two axis rows
+        float brec[16];
+        for (int i = 0; i < 16; i++) brec[i] = rec[i];
+        brec[9] = P.erp;
+        ball_info2(D, P, brec, b0, rows);
+        for (int c = 0; c < 3; c++) { rows[3].J1[3 + c] = p[c]; rows[4].J1[3 + c] = q[c]; }
+        if (b2 >= 0) for (int c = 0; c < 3; c++) { rows[3].J2[3 + c] = -p[c]; rows[4].J2[3 + c] = -q[c]; }
+        float bv[3];
+        if (b2 >= 0) {
+            float ax2[3];
+            mul0_331(ax2, D.bscr + (size_t)(b0 + b2) * BSCR_STRIDE, x + JX_AXIS2);
+            cross3(bv, ax1, ax2);
+        } else {
+            cross3(bv, ax1, x + JX_AXIS2);
+        }
+        rows[3].rhs = k * dot3(bv, p);
+        rows[4].rhs = k * dot3(bv, q);
+        if (x[JX_LIMOT_ROW] != 0.f) { limot_row(D, b0, b1, b2, x, fps, ax1, true, rows[5]); return 6; }
+        return 5;
+    }
+    // slider
+    fixed_orientation_rows(D, b0, b1, b2, x, fps, P.erp, rows);
+    float c[3] = {0.f, 0.f, 0.f};
+    for (int i = 0; i < 3; i++) { rows[3].J1[i] = p[i]; rows[4].J1[i] = q[i]; }
+    if (b2 >= 0) {
+        const float4 p2v = D.pos[b0 + b2];
+        c[0] = p2v.x - x1[0]; c[1] = p2v.y - x1[1]; c[2] = p2v.z - x1[2];
+        float t[3];
+        for (int i = 0; i < 3; i++) { rows[3].J2[i] = -p[i]; rows[4].J2[i] = -q[i]; }
+        cross3(t, c, p);
+        for (int i = 0; i < 3; i++) { rows[3].J1[3 + i] = 0.5f * t[i]; rows[3].J2[3 + i] = rows[3].J1[3 + i]; }
+        cross3(t, c, q);
+        for (int i = 0; i < 3; i++) { rows[4].J1[3 + i] = 0.5f * t[i]; rows[4].J2[3 + i] = rows[4].J1[3 + i]; }
+        float ofs[3];
+        mul0_331(ofs, D.bscr + (size_t)(b0 + b2) * BSCR_STRIDE, x + JX_OFFSET);
+        for (int i = 0; i < 3; i++) c[i] += ofs[i];
+        rows[3].rhs = k * dot3(p, c);
+        rows[4].rhs = k * dot3(q, c);
+    } else {
+        float ofs[3];
+        for (int i = 0; i < 3; i++) ofs[i] = x[JX_OFFSET + i] - x1[i];
+        rows[3].rhs = k * dot3(p, ofs);
+        rows[4].rhs = k * dot3(q, ofs);
+        if (x[JX_REVERSE] != 0.f) for (int i = 0; i < 3; i++) ax1[i] = -ax1[i];
+    }
+    if (x[JX_LIMOT_ROW] != 0.f) { limot_row(D, b0, b1, b2, x, fps, ax1, false, rows[5]); return 6; }
+    return 5;
+}
+
 // Builds the rows of one joint (Stage2a scaling, Stage2c rhs, iMJ, Ad) and
 // writes them at world-local row index ri.
 __device__ int build_joint_rows(const DevWorlds &D, const StepParams &P, const float *rec, int b0, int r0, int ri) {
@@ -332,7 +510,11 @@ __device__ int build_joint_rows(const DevWorlds &D, const StepParams &P, const f
         rows[r].findex = -1;
     }
     const int type = (int)rec[0];
-    const int m = type == 0 ? contact_info2(D, P, rec, b0, rows) : ball_info2(D, P, rec, b0, rows);
+    int m;
+    if (type == 0) m = contact_info2(D, P, rec, b0, rows);
+    else if (type == 1) m = ball_info2(D, P, rec, b0, rows);
+    else m = hinge_slider_fixed_info2(D, P, rec, D.jext + (size_t)((rec - D.joints) / JOINT_STRIDE) * JEXT_STRIDE,
+                                      b0, rows);
     const int b1 = (int)rec[1], b2 = (int)rec[2];
     const float fps = 1.f / P.h;
     const float *s1 = D.bscr + (size_t)(b0 + b1) * BSCR_STRIDE;
@@ -1892,7 +2074,11 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
         for (int j = t; j < nj; j += NT) {
             const float *rec = D.joints + (size_t)j * JOINT_STRIDE;
             const int ri = D.joff[j];
-            const int mj = (int)rec[0] == 0 ? contact_rows_m(D.surf[(int)rec[13]]) : 3;
+            const int ty = (int)rec[0];
+            const int mj = ty == 0 ? contact_rows_m(D.surf[(int)rec[13]])
+                         : ty == 1 ? 3
+                         : ty == JT_FIXED ? 6
+                                          : 5 + (D.jext[(size_t)j * JEXT_STRIDE + JX_LIMOT_ROW] != 0.f ? 1 : 0);
             float f[12] = {0.f};
             for (int q = 0; q < mj; q++) {
                 const float *rw = D.rows + (size_t)(ri + q) * ROW_STRIDE;

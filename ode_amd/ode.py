@@ -73,6 +73,7 @@ def api():
         "dBodyGetPosition": (P(dReal), [vp]), "dBodyGetQuaternion": (P(dReal), [vp]),
         "dBodyGetRotation": (P(dReal), [vp]), "dBodyGetLinearVel": (P(dReal), [vp]),
         "dBodyGetAngularVel": (P(dReal), [vp]), "dBodySetMass": (None, [vp, P(dMass)]),
+        "dBodyGetMass": (None, [vp, P(dMass)]),
         "dBodyAddForce": (None, [vp, f, f, f]), "dBodyAddTorque": (None, [vp, f, f, f]),
         "dBodySetGravityMode": (None, [vp, i]), "dBodySetGyroscopicMode": (None, [vp, i]),
         "dMassSetBox": (None, [P(dMass), f, f, f, f]), "dMassSetSphere": (None, [P(dMass), f, f]),

@@ -14,10 +14,14 @@ thresholds printed as stored (squared).  The world is a plain description:
             geoms most recently attached first)
   geom   = {"cls": 0 sphere | 1 box | 2 capsule | 4 plane, "p": (4,),
             "cat": int, "col": int, "enabled": bool}
-  joints = [{"type": "ball" | "contact", "bodies": (i, j or None), ...}]
+  joints = [{"type": "ball" | "contact" | "hinge" | "slider" | "fixed",
+             "bodies": (i, j or None), ...}]
             (newest first; ball: anchor1, anchor2; contact: pos, normal, depth,
             mode, mu, mu2, bounce, bounce_vel, soft_erp, soft_cfm, motion1,
-            motion2, slip1, slip2, fdir1)
+            motion2, slip1, slip2, fdir1; hinge: anchor1, anchor2, axis1, axis2,
+            qrel, limot; slider: axis1, qrel, offset, limot; fixed: qrel,
+            offset; limot: lostop, histop, bounce, stop_erp, stop_cfm, vel,
+            fmax, fudge, normal_cfm)
 
 Parity note: the reference holds no DIF fixtures and its exporter needs the
 whole library to build, so this restatement is checked against the library
@@ -93,6 +97,43 @@ def _geom(c, g):
         c.str_("type", "plane")
         c.vec("normal", p)
         c.num("d", p[3])
+
+
+def _limot(c, lm):
+    """printLimot export-dif.cpp:142-183; lm: lostop, histop, bounce,
+    stop_erp, stop_cfm, vel, fmax, fudge, normal_cfm."""
+    c.line("limit = {")
+    c.indent += 1
+    c.num("low_stop", lm["lostop"])
+    c.num("high_stop", lm["histop"])
+    if lm["bounce"] != 0:
+        c.num("bounce", lm["bounce"])
+    c.line("ODE = {")
+    c.indent += 1
+    if lm["stop_erp"] != 0:
+        c.num("stop_erp", lm["stop_erp"])
+    if lm["stop_cfm"] != 0:
+        c.num("stop_cfm", lm["stop_cfm"])
+    c.indent -= 1
+    c.line("},")
+    c.indent -= 1
+    c.line("},")
+    c.line("motor = {")
+    c.indent += 1
+    if lm["vel"] != 0:
+        c.num("vel", lm["vel"])
+    if lm["fmax"] != 0:
+        c.num("fmax", lm["fmax"])
+    c.line("ODE = {")
+    c.indent += 1
+    if lm["fudge"] != 0:
+        c.num("fudge_factor", lm["fudge"])
+    if lm["normal_cfm"] != 0:
+        c.num("normal_cfm", lm["normal_cfm"])
+    c.indent -= 1
+    c.line("},")
+    c.indent -= 1
+    c.line("},")
 
 
 def export(world, bodies, joints, prefix=""):
@@ -175,6 +216,21 @@ def export(world, bodies, joints, prefix=""):
         if j["type"] == "ball":
             c.vec("anchor1", j["anchor1"])
             c.vec("anchor2", j["anchor2"])
+        elif j["type"] == "hinge":
+            c.vec("anchor1", j["anchor1"])
+            c.vec("anchor2", j["anchor2"])
+            c.vec("axis1", j["axis1"])
+            c.vec("axis2", j["axis2"])
+            c.vec("qrel", j["qrel"], 4)
+            _limot(c, j["limot"])
+        elif j["type"] == "slider":
+            c.vec("axis1", j["axis1"])
+            c.vec("qrel", j["qrel"], 4)
+            c.vec("offset", j["offset"])
+            _limot(c, j["limot"])
+        elif j["type"] == "fixed":
+            c.vec("qrel", j["qrel"])      # three components, as printFixed prints it
+            c.vec("offset", j["offset"])
         else:
             mode = j["mode"]
             c.vec("pos", j["pos"])
