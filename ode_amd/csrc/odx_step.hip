@@ -886,8 +886,9 @@ __device__ __forceinline__ int prepare_world(const DevWorlds &D, const StepParam
                 for (int r = 0; r < mj; r++) D.lambda0[r0 + ri + r] = D.lambda[r0 + ri + r];
             if (is_contact && P.warm == 2) write_contact_lambda0(D, rec, r0, ri, lp);
             if (P.warm == 3) {
-                // the network predicts (normal, t1, t2): contacts with exactly those three rows
-                const bool learn = is_contact && mj == 3;
+                // the network predicts (normal, t1, t2): contacts with exactly those three rows;
+                // a gated network leaves matched contacts at their previous impulses
+                const bool learn = is_contact && mj == 3 && !(P.gate && matched);
                 if (is_contact && !learn) write_contact_lambda0(D, rec, r0, ri, lp);
                 const unsigned long long bal = __ballot(learn);
                 int base = 0;
@@ -2843,12 +2844,19 @@ __device__ __forceinline__ void lg_slot_run(const DevWorlds &D, __amdgpu_buffer_
     if (two) lg_fc_store(rs, S.b2, f2);
 }
 
+// Colours of at most this many contacts at the end of the colour order run on
+// the first workgroup alone, one after another with workgroup barriers, in a
+// single grid phase: the greedy colouring ends in a few near-empty colours
+// whose grid barriers would cost as much as a full colour's.
+constexpr int LG_TAIL = 256;
+
 // The solve (cooperative): lambda = 0, `iters` sweeps, each the contacts'
 // first-pass rows colour by colour, then their findex rows colour by colour,
 // then Stage4b/6 and dxStepBody of every body.  The thread at a slot of the
 // colour order handles that contact in every phase (its lambda is touched by
 // no other thread); each thread loads its slot of the next phase (item and
-// rows: read-only here) before the barrier.
+// rows: read-only here) before the barrier.  Tail colours (LG_TAIL) map their
+// slots to the first workgroup's threads, in the sweeps and at the start.
 __global__ __launch_bounds__(LG_BLOCK) void k_lg_solve(DevWorlds D, StepParams P, LargeDev L, int iters) {
     __shared__ int s_cs[LG_MAXCOL + 1];
     unsigned gen = 0;
@@ -2858,31 +2866,49 @@ __global__ __launch_bounds__(LG_BLOCK) void k_lg_solve(DevWorlds D, StepParams P
     const __amdgpu_buffer_rsrc_t rs = lg_fc_rsrc(L.fc, L.nb);
     const float zero[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int c = threadIdx.x; c <= ncol; c += blockDim.x) s_cs[c] = L.col_start[c];
+    __syncthreads();
+    int ct = ncol;  // colours ct .. ncol-1 form the tail
+    while (ct > 0 && s_cs[ct] - s_cs[ct - 1] <= LG_TAIL) ct--;
     for (int b = gt; b < L.nb; b += gs) lg_fc_store(rs, b, zero);
     for (int c = 0; c < ncol; c++) {   // slots of colour c map to threads as in the sweeps
-        const int e = L.col_start[c + 1];
-        for (int idx = L.col_start[c] + gt; idx < e; idx += gs) {
+        const int e = s_cs[c + 1];
+        const int first = c < ct ? s_cs[c] + gt : (blockIdx.x == 0 ? s_cs[c] + (int)threadIdx.x : e);
+        const int step = c < ct ? gs : (int)blockDim.x;
+        for (int idx = first; idx < e; idx += step) {
             const int4 it = L.items[idx];
             for (int r = it.x; r < it.x + (it.y >> 8); r++) D.lambda[r] = 0.f;
         }
     }
     bool ok = lg_grid_sync(L, gen);
-    const int T = (nc > 0 && ncol > 0) ? iters * 2 * ncol : 0;
+    const int nph = ct + (ct < ncol ? 1 : 0);  // grid phases per pass
+    const int T = (nc > 0 && ncol > 0) ? iters * 2 * nph : 0;
     int c = 0, pass = 0;
-    int idx = s_cs[0] + gt, e = s_cs[1];
+    int idx = s_cs[0] + gt, e = ct > 0 ? s_cs[1] : 0;
     LgSlot cur;
     lg_slot_load(D, idx < e && T > 0 ? L.items[idx] : make_int4(-1, 0, 0, 0), 0, cur);
     for (int p = 0; ok && p < T; p++) {
-        int c2 = c + 1, pass2 = pass;
-        if (c2 == ncol) { c2 = 0; pass2 ^= 1; }
-        const int idx2 = s_cs[c2] + gt, e2 = s_cs[c2 + 1];
+        const int c2 = (p + 1) % nph, pass2 = ((p + 1) / nph) & 1;
+        const bool grid2 = p + 1 < T && c2 < ct;
+        const int idx2 = grid2 ? s_cs[c2] + gt : 0, e2 = grid2 ? s_cs[c2 + 1] : 0;
         LgSlot nxt;
-        lg_slot_load(D, (p + 1 < T && idx2 < e2) ? L.items[idx2] : make_int4(-1, 0, 0, 0), pass2, nxt);
-        lg_slot_run(D, rs, cur);
-        for (int j = idx + gs; j < e; j += gs) {   // slots beyond one per thread
-            LgSlot x;
-            lg_slot_load(D, L.items[j], pass, x);
-            lg_slot_run(D, rs, x);
+        lg_slot_load(D, idx2 < e2 ? L.items[idx2] : make_int4(-1, 0, 0, 0), pass2, nxt);
+        if (c < ct) {
+            lg_slot_run(D, rs, cur);
+            for (int j = idx + gs; j < e; j += gs) {   // slots beyond one per thread
+                LgSlot x;
+                lg_slot_load(D, L.items[j], pass, x);
+                lg_slot_run(D, rs, x);
+            }
+        } else if (blockIdx.x == 0) {
+            for (int tc = ct; tc < ncol; tc++) {
+                for (int j = s_cs[tc] + (int)threadIdx.x; j < s_cs[tc + 1]; j += blockDim.x) {
+                    LgSlot x;
+                    lg_slot_load(D, L.items[j], pass, x);
+                    lg_slot_run(D, rs, x);
+                }
+                mem_done();
+                __syncthreads();
+            }
         }
         ok = lg_grid_sync(L, gen);
         c = c2; pass = pass2; idx = idx2; e = e2; cur = nxt;
