@@ -8,7 +8,9 @@ per second of the whole job with inputs resident in HBM.
 Workloads (seeded synthetic scenes, ode_amd/scenes.py):
   towers  4,096 worlds of a 32-box tower plus a projectile sphere; the learned
           warm start (trained first on the GPU on 2M samples of the same
-          configuration with other seeds) vs cold PGS at the same residual.
+          configuration with other seeds; gated: contacts matched to a
+          previous-step contact keep its impulses, the network predicts the
+          new ones) vs cold PGS at the same residual.
           value = learned warm-start world-steps/s.
   uneven  4,096 worlds of 8-64 spheres/boxes dropped on a plane, cold PGS.
   chains  1,024 worlds of a 16-capsule chain on ball joints (anchored at one
@@ -308,6 +310,9 @@ def main():
     ap.add_argument("--train-samples", type=int, default=2_000_000)
     ap.add_argument("--train-steps", type=int, default=6000)
     ap.add_argument("--modes", default="cold,plain,learned")
+    ap.add_argument("--gate", type=int, default=1,
+                    help="learned mode: 1 keeps matched contacts at their previous impulses and runs the "
+                         "network on new contacts only (odxMlpSetGate); 0 adds its predicted change everywhere")
     ap.add_argument("--restart-after", type=int, default=600,
                     help="sweeps after which an unconverged warm start restarts from lambda = 0")
     ap.add_argument("--lds-rows", type=int, default=-1, help="LDS row budget per world (-1: sized during warmup)")
@@ -378,9 +383,10 @@ def main():
         X, T = np.concatenate(xs), np.concatenate(ts)
         t1 = time.perf_counter()
         mlp, losses = WS.train(X, T, steps=args.train_steps, batch=4096)
+        mlp.set_gate(args.gate)
         t2 = time.perf_counter()
         train_info = {"samples": int(len(X)), "collect_s": t1 - t0, "train_s": t2 - t1,
-                      "train_steps": args.train_steps, "loss_first": float(losses[:20].mean()),
+                      "train_steps": args.train_steps, "gate": args.gate, "loss_first": float(losses[:20].mean()),
                       "loss_last": float(losses[-20:].mean())}
 
     worlds = build_worlds(args.workload, args.worlds, args.seed)

@@ -247,7 +247,9 @@ int odxMlpSetNormalization(odxMlp *m, const float *norm38);
 /* gate 1: contacts matched to a previous-step contact keep its impulses and
  * the network predicts only new contacts; gate 0 (default): the network's
  * predicted change is added for every contact.  Takes effect for batches
- * attached (odxBatchSetMlp) afterwards. */
+ * attached (odxBatchSetMlp) afterwards: such a batch writes features (and
+ * runs the network) for its unmatched contacts only, so odxBatchGetSamples
+ * returns those. */
 int odxMlpSetGate(odxMlp *m, int gate);
 int odxMlpGetParams(odxMlp *m, float *params);
 int odxMlpSetParams(odxMlp *m, const float *params);

@@ -25,6 +25,7 @@ void launch_step_fused(const DevWorlds &D, const StepParams &P, const MlpDev *M,
 void launch_mlp_infer(const odxMlp *m, const float *feat, const int *feat_row, const int *count, float *lambda0,
                       int max_entries, hipStream_t s);
 const MlpDev *mlp_device_view(const odxMlp *m);
+int mlp_gate(const odxMlp *m);
 }  // namespace odx
 
 using namespace odx;
@@ -928,6 +929,7 @@ int odxBatchRowStarts(odxBatch *b, int *row_start) {
 int odxBatchSetMlp(odxBatch *b, const odxMlp *mlp) {
     if (!b) return 0;
     b->mlp = mlp;
+    b->P.gate = mlp_gate(mlp);
     b->drop_graph();
     return 1;
 }

@@ -71,6 +71,8 @@ struct StepParams {
     int split;           // feature counter pair of this launch: feat_count[2 * split], [2 * split + 1]
     int feat_base;       // first feature slot of this launch (joint_start[wbase])
     int feat_local;      // 1: features go to slots joint_start[w] + i, in joint order (fused kernel)
+    int gate;            // warm 3 with a gated network: matched contacts take their previous impulses
+                         // directly and only unmatched contacts get feature slots
     // dynamic iteration count per island (drop-in world, quickstep.cpp:1820-1853)
     int dyn;             // 1: on (then max_iters is the base count)
     int dyn_max_extra;   // extra sweeps allowed after max_iters

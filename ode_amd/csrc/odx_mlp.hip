@@ -35,7 +35,10 @@ __global__ __launch_bounds__(256) void k_mlp_infer(MlpDev M, const float *feat, 
     __shared__ InferBuf tb[4];
     const int tid = threadIdx.x;
     const int n = *count;
-    if (n <= 0) return;
+    const int ntiles = (n + 15) / 16;
+    // the grid is sized for every contact; workgroups with no tile (the count is
+    // known only on the device) leave before staging the weights
+    if (n <= 0 || (int)blockIdx.x * 4 >= ntiles) return;
     for (int i = tid; i < NH * KIN / 8; i += 256) reinterpret_cast<bf16x8 *>(sW1)[i] = reinterpret_cast<const bf16x8 *>(M.W1h)[i];
     for (int i = tid; i < NH * NH / 8; i += 256) reinterpret_cast<bf16x8 *>(sW2)[i] = reinterpret_cast<const bf16x8 *>(M.W2h)[i];
     for (int i = tid; i < NOUTP * NH / 8; i += 256) reinterpret_cast<bf16x8 *>(sW3)[i] = reinterpret_cast<const bf16x8 *>(M.W3h)[i];
@@ -45,7 +48,6 @@ __global__ __launch_bounds__(256) void k_mlp_infer(MlpDev M, const float *feat, 
     __syncthreads();
     const int wv = tid >> 6, l = tid & 63;
     const float *nrm = sb + 2 * NH + 4;
-    const int ntiles = (n + 15) / 16;
     for (int tile = blockIdx.x * 4 + wv; tile < ntiles; tile += gridDim.x * 4) {
         const int s0 = tile * 16;
         const int nv = min(16, n - s0);
@@ -341,6 +343,7 @@ void launch_mlp_infer(const odxMlp *m, const float *feat, const int *feat_row, c
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, s, m->M, feat, feat_row, count, lambda0);
 }
+int mlp_gate(const odxMlp *m) { return m ? m->M.gate : 0; }
 }  // namespace odx
 
 static void mlp_refresh_copies(odxMlp *m) {

@@ -142,3 +142,34 @@ def test_learned_warm_start_cuts_sweeps_vs_cold_at_equal_residual():
     # learned ~1.6x fewer than cold, about as many as the previous-impulse
     # warm start "plain"; the 3x target is not reached.)
     assert res["learned"] <= 0.8 * res["cold"], res
+
+
+def test_gated_network_leaves_matched_contacts_at_previous_impulses():
+    say("case: gated network, matched contacts")
+    import oracle as O
+    from ode_amd import odx, scenes as S
+    prm = O.params(iters=3000)
+    worlds = S.tower_batch(32, seed=7, levels=8)
+    # a network with zero weights and a zero output mean predicts a zero impulse
+    # for every new contact: gated, it must reproduce the previous-impulse warm start
+    m = odx.Mlp(3)
+    m.set_params(np.zeros(odx.NPARAM, np.float32))
+    norm = np.concatenate([np.zeros(16), np.ones(16), np.zeros(3), np.ones(3)]).astype(np.float32)
+    m.set_normalization(norm)
+    m.set_gate(1)
+    out = {}
+    for mode in ("plain", "gated"):
+        B = odx.Batch(worlds, prm)
+        if mode == "gated":
+            B.set_mlp(m)
+        B.set_solver(order=1, max_iters=3000, tol=1e-4, check_every=2, warm=2 if mode == "plain" else 3)
+        B.step(12)
+        if mode == "gated":
+            X, _ = B.samples()
+            say("  feature rows of the last step", len(X))
+            assert np.all(X[:, 12] == 0.0), "a matched contact got a feature slot"
+        out[mode] = (B.bodies().copy(), B.stats()["iters"].copy())
+        B.close()
+    m.close()
+    assert np.array_equal(out["plain"][1], out["gated"][1])
+    assert np.array_equal(out["plain"][0], out["gated"][0])
