@@ -2547,10 +2547,15 @@ __device__ __forceinline__ void mem_done() {
 // Grid barrier of the cooperative kernels.  Workgroups count per residue
 // class blockIdx % 8 (one XCD each under round-robin dispatch; correctness
 // does not depend on it), the last of a class counts on the top counter, the
-// last class bumps the generation that the others wait for.  Counters sit 64
-// B apart.  A wait that runs out sets LG_BARRIER_ERR; every block then leaves
-// at its next barrier and the host reports the step as failed.  `read`, when
-// given, is loaded by one thread after the barrier and returned in *val.
+// last class bumps the generation that the others wait for.  The counters are
+// zeroed before each launch and only grow inside it: at barrier g a class is
+// complete when its counter reaches per * (g + 1), so no reset store (and no
+// wait for one) sits on the barrier's path.  Counters sit 64 B apart.  A wait
+// that runs out sets LG_BARRIER_ERR and bit 31 of the generation word; every
+// block sees the bit in the generation it reads at its next barrier, leaves
+// with false, and the host reports the step as failed.  `read`, when given, is
+// loaded by one thread after the barrier and returned in *val.
+constexpr unsigned LG_GEN_ERR = 0x80000000u;
 __device__ bool lg_grid_sync(const LargeDev &L, unsigned &gen, const int *read = nullptr, int *val = nullptr) {
     __shared__ int s_ok, s_val;
     mem_done();
@@ -2561,31 +2566,26 @@ __device__ bool lg_grid_sync(const LargeDev &L, unsigned &gen, const int *read =
         const unsigned per = (gridDim.x + 7u - x) >> 3;
         const unsigned groups = gridDim.x < 8u ? gridDim.x : 8u;
         bool last = false;
-        if (add_dev(&L.bar[LG_BAR_GROUP + 16 * x], 1u) + 1u == per) {
-            st_dev(&L.bar[LG_BAR_GROUP + 16 * x], 0u);
-            mem_done();
-            if (add_dev(&L.bar[LG_BAR_TOP], 1u) + 1u == groups) {
-                st_dev(&L.bar[LG_BAR_TOP], 0u);
-                mem_done();
-                last = true;
-            }
-        }
+        unsigned seen = 0;
+        if (add_dev(&L.bar[LG_BAR_GROUP + 16 * x], 1u) + 1u == per * (gen + 1u))
+            last = add_dev(&L.bar[LG_BAR_TOP], 1u) + 1u == groups * (gen + 1u);
         if (last) {
-            add_dev(&L.bar[LG_BAR_GEN], 1u);
+            seen = add_dev(&L.bar[LG_BAR_GEN], 1u);
         } else {
             long long spins = 0;
-            while (ld_dev(&L.bar[LG_BAR_GEN]) == gen) {
-                if (++spins > LG_SPIN_LIMIT || ld_dev(&L.counters[LG_BARRIER_ERR])) {
+            while ((seen = ld_dev(&L.bar[LG_BAR_GEN])) == gen) {
+                if (++spins > LG_SPIN_LIMIT) {
                     atomicExch(&L.counters[LG_BARRIER_ERR], 1);
-                    ok = 0;
+                    __hip_atomic_fetch_or(&L.bar[LG_BAR_GEN], LG_GEN_ERR, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    seen = LG_GEN_ERR;
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
         }
+        if (seen & LG_GEN_ERR) ok = 0;
         gen++;
         mem_done();
-        if (ld_dev(&L.counters[LG_BARRIER_ERR])) ok = 0;
         if (read) s_val = ld_dev(read);
         s_ok = ok;
     }
