@@ -147,6 +147,7 @@ def bench_large(args):
     world = S.sphere_pour(bodies, seed=args.seed)
     prm = odx.params_vector(iters=args.large_iters, w=args.sor_w, mu=0.3, space=1)
     L = odx.Large(world, prm)
+    L.set_solve_blocks(args.solve_blocks)
     t0 = time.perf_counter()
     done = 0
     while done < args.warmup:
@@ -329,6 +330,8 @@ def main():
     ap.add_argument("--sweep-mode", type=int, default=0,
                     help="boxes workload: 0 automatic, 1 workgroup-wide levels, 2 one wave (odxWorldSetSweepMode)")
     ap.add_argument("--bodies", type=int, default=None, help="bodies of the large (200,000) or boxes (1,000) workload")
+    ap.add_argument("--solve-blocks", type=int, default=1,
+                    help="large workload: workgroups per CU of the cooperative solve (odxLargeSetSolveBlocks)")
     ap.add_argument("--large-iters", type=int, default=None,
                     help="sweeps per step of the large (30) or boxes (20) workload")
     args = ap.parse_args()

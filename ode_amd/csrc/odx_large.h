@@ -23,9 +23,9 @@ constexpr int LG_MAXCOL = 128;        // colours of the sweep schedule
 constexpr int LG_MAXSUB = 32;         // claim/commit rounds per colour
 constexpr long long LG_SPIN_LIMIT = 1ll << 21;  // polls of one barrier wait (~1 s)
 constexpr int LG_BLOCK = 256;         // threads per workgroup of the cooperative kernels
-// words of the barrier array: generation, top counter, 8 group counters (64 B
-// apart), then the per-colour fill counters of the colouring
-enum { LG_BAR_GEN = 0, LG_BAR_TOP = 16, LG_BAR_GROUP = 32, LG_BAR_FILL = 160, LG_BAR_WORDS = LG_BAR_FILL + LG_MAXCOL };
+// words of the barrier array: generation, 8 group counters (64 B apart), then
+// the per-colour fill counters of the colouring
+enum { LG_BAR_GEN = 0, LG_BAR_GROUP = 32, LG_BAR_FILL = 160, LG_BAR_WORDS = LG_BAR_FILL + LG_MAXCOL };
 
 struct LargeDev {
     int nb, ng, nbig, hmask;          // bodies, geoms, big geoms, hash buckets - 1 (power of two)

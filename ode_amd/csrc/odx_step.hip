@@ -2546,48 +2546,44 @@ __device__ __forceinline__ void mem_done() {
 
 // Grid barrier of the cooperative kernels.  Workgroups count per residue
 // class blockIdx % 8 (one XCD each under round-robin dispatch; correctness
-// does not depend on it), the last of a class counts on the top counter, the
-// last class bumps the generation that the others wait for.  The counters are
-// zeroed before each launch and only grow inside it: at barrier g a class is
-// complete when its counter reaches per * (g + 1), so no reset store (and no
-// wait for one) sits on the barrier's path.  Counters sit 64 B apart.  A wait
-// that runs out sets LG_BARRIER_ERR and bit 31 of the generation word; every
-// block sees the bit in the generation it reads at its next barrier, leaves
-// with false, and the host reports the step as failed.  `read`, when given, is
-// loaded by one thread after the barrier and returned in *val.
+// does not depend on it); the last of a class adds one to the generation word,
+// and every workgroup waits until that word has counted all classes.  The
+// counters are zeroed before each launch and only grow inside it: at barrier g
+// a class is complete when its counter reaches per * (g + 1), and the barrier
+// when the generation word reaches classes * (g + 1), so no reset store (and
+// no wait for one) sits on the barrier's path.  Counters sit 64 B apart.  A
+// wait that runs out sets LG_BARRIER_ERR and bit 31 of the generation word;
+// every block sees the bit at its next barrier, leaves with false, and the
+// host reports the step as failed.  `read`, when given, is loaded by one
+// thread after the barrier and returned in *val.
 constexpr unsigned LG_GEN_ERR = 0x80000000u;
 __device__ bool lg_grid_sync(const LargeDev &L, unsigned &gen, const int *read = nullptr, int *val = nullptr) {
     __shared__ int s_ok, s_val;
     mem_done();
     __syncthreads();
     if (threadIdx.x == 0) {
-        int ok = 1;
         const unsigned x = blockIdx.x & 7u;
         const unsigned per = (gridDim.x + 7u - x) >> 3;
         const unsigned groups = gridDim.x < 8u ? gridDim.x : 8u;
-        bool last = false;
+        const unsigned target = groups * (gen + 1u);
         unsigned seen = 0;
         if (add_dev(&L.bar[LG_BAR_GROUP + 16 * x], 1u) + 1u == per * (gen + 1u))
-            last = add_dev(&L.bar[LG_BAR_TOP], 1u) + 1u == groups * (gen + 1u);
-        if (last) {
-            seen = add_dev(&L.bar[LG_BAR_GEN], 1u);
-        } else {
-            long long spins = 0;
-            while ((seen = ld_dev(&L.bar[LG_BAR_GEN])) == gen) {
-                if (++spins > LG_SPIN_LIMIT) {
-                    atomicExch(&L.counters[LG_BARRIER_ERR], 1);
-                    __hip_atomic_fetch_or(&L.bar[LG_BAR_GEN], LG_GEN_ERR, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    seen = LG_GEN_ERR;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
+            seen = add_dev(&L.bar[LG_BAR_GEN], 1u) + 1u;
+        long long spins = 0;
+        while (seen < target) {   // an error bit makes seen >= target
+            if (++spins > LG_SPIN_LIMIT) {
+                atomicExch(&L.counters[LG_BARRIER_ERR], 1);
+                __hip_atomic_fetch_or(&L.bar[LG_BAR_GEN], LG_GEN_ERR, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                seen = LG_GEN_ERR;
+                break;
             }
+            if (spins > 1) __builtin_amdgcn_s_sleep(1);
+            seen = ld_dev(&L.bar[LG_BAR_GEN]);
         }
-        if (seen & LG_GEN_ERR) ok = 0;
         gen++;
         mem_done();
         if (read) s_val = ld_dev(read);
-        s_ok = ok;
+        s_ok = (seen & LG_GEN_ERR) ? 0 : 1;
     }
     __syncthreads();
     if (read) *val = s_val;
