@@ -205,6 +205,12 @@ int odxLargeSetIterations(odxLarge *w, int iters);
  * default 1).  Returns the resulting grid size, 0 on failure.  No ODE
  * counterpart (launch shape only; results do not depend on it). */
 int odxLargeSetSolveBlocks(odxLarge *w, int per_cu);
+/* Polls one grid-barrier wait may make before the step fails (at most and by
+ * default 2^21, about a second).  No ODE counterpart: it exists so that a test
+ * can force the barrier-timeout path from the host (0: every wait fails at
+ * once); every workgroup still leaves the kernel normally, and the next
+ * odxLargeSync reports the failed step. */
+int odxLargeSetSpinLimit(odxLarge *w, int polls);
 /* Queues nsteps steps (dSpaceCollide + contacts + dWorldQuickStep +
  * dJointGroupEmpty each) on the world's stream.  Returns 1 when queued. */
 int odxLargeStep(odxLarge *w, int nsteps);

@@ -136,6 +136,7 @@ odxLarge *odxLargeCreate(int nb, const float *bodies, int ng, const float *geoms
     int hsize = 1024;
     while (hsize < 2 * ng) hsize <<= 1;
     L.hmask = hsize - 1;
+    L.spin_limit = (int)LG_SPIN_LIMIT;
     cell = cell > 0.f ? cell * 1.001f : 1.f;  // slack keeps floor() of overlapping centres within one cell
     L.inv_cell = 1.f / cell;
     L.pair_cap = pair_cap;
@@ -193,6 +194,12 @@ int odxLargeSetSolveBlocks(odxLarge *w, int per_cu) {
     if (b <= 0) return 0;
     w->solve_blocks = b;
     return b;
+}
+
+int odxLargeSetSpinLimit(odxLarge *w, int polls) {
+    if (!w || polls < 0) return 0;
+    w->L.spin_limit = polls == 0 ? 0 : (polls < (int)LG_SPIN_LIMIT ? polls : (int)LG_SPIN_LIMIT);
+    return 1;
 }
 
 int odxLargeStep(odxLarge *w, int nsteps) {

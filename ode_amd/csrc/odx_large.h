@@ -53,6 +53,7 @@ struct LargeDev {
     unsigned *bar;                    // [LG_BAR_WORDS] grid barrier counters, colour fill counters
     float *fc;                        // [nb * 8] body accumulators of the solve (6 used)
     int *scan_tmp;                    // block sums of the scans
+    int spin_limit;                   // polls of one barrier wait before it fails (LG_SPIN_LIMIT)
 };
 
 }  // namespace odx

@@ -2571,7 +2571,7 @@ __device__ bool lg_grid_sync(const LargeDev &L, unsigned &gen, const int *read =
             seen = add_dev(&L.bar[LG_BAR_GEN], 1u) + 1u;
         long long spins = 0;
         while (seen < target) {   // an error bit makes seen >= target
-            if (++spins > LG_SPIN_LIMIT) {
+            if (++spins > L.spin_limit) {
                 atomicExch(&L.counters[LG_BARRIER_ERR], 1);
                 __hip_atomic_fetch_or(&L.bar[LG_BAR_GEN], LG_GEN_ERR, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 seen = LG_GEN_ERR;

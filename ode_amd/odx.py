@@ -70,7 +70,7 @@ def lib():
         L.odxLargeCreate.restype = ctypes.c_void_p
         L.odxLargeCreate.argtypes = [i, fp, i, fp, dp, i, i]
         L.odxLargeDestroy.argtypes = [vp]
-        for name in ("odxLargeSetIterations", "odxLargeStep", "odxLargeSetSolveBlocks"):
+        for name in ("odxLargeSetIterations", "odxLargeStep", "odxLargeSetSolveBlocks", "odxLargeSetSpinLimit"):
             getattr(L, name).argtypes = [vp, i]
         L.odxLargeSync.argtypes = [vp]
         for name in ("odxLargeTime", "odxLargeKernelTimes"):
@@ -328,6 +328,10 @@ class Large:
         b = lib().odxLargeSetSolveBlocks(self.h, per_cu)
         _check(b, "odxLargeSetSolveBlocks")
         return b
+
+    def set_spin_limit(self, polls=1 << 21):
+        """Polls one grid-barrier wait may make before the step fails."""
+        _check(lib().odxLargeSetSpinLimit(self.h, polls), "odxLargeSetSpinLimit")
 
     def step(self, n=1):
         _check(lib().odxLargeStep(self.h, n), "odxLargeStep")

@@ -222,6 +222,30 @@ def test_capacity_overflow_fails_the_sync():
     L.close()
 
 
+def test_barrier_timeout_fails_the_step_and_the_world_recovers():
+    # the timeout path of the grid barrier, forced from the host: with a spin
+    # limit of 0 every wait fails at once, every workgroup still leaves both
+    # cooperative kernels, and the next sync reports the failed step
+    say("case: large barrier timeout (spin limit 0)")
+    from ode_amd import odx
+    w = S.sphere_pour(2000, seed=6)
+    prm = O.params(space=1, mu=0.3, iters=10, w=1.0)
+    L = make(w, prm)
+    L.step(3)
+    L.sync()
+    L.set_spin_limit(0)
+    L.step(1)
+    with pytest.raises(odx.OdxError, match="barrier 1"):
+        L.sync()
+    L.set_spin_limit()
+    L.step(2)
+    L.sync()
+    cnt = L.counts()
+    assert cnt["barrier_error"] == 0 and cnt["uncoloured"] == 0
+    assert np.all(np.isfinite(L.bodies()[:, :15]))
+    L.close()
+
+
 def test_sphere_pour_steps_and_settles():
     say("case: sphere pour 20000")
     w = S.sphere_pour(20000, seed=4)
