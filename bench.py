@@ -102,6 +102,11 @@ def measure(B, args, warm, nworlds):
             except Exception:
                 want -= 16
     B.sync()
+    schedule = {"splits": args.splits, "graph_steps": args.graph_steps, "fused": 0, "autotuned": False}
+    if not args.no_autotune:
+        # the launch schedule is chosen by timing the candidates on these worlds
+        sch, _ = B.autotune(steps=args.tune_steps, reps=2)
+        schedule = dict(sch, autotuned=True)
     bodies_after_warmup = B.bodies()
     total_ms = B.time(args.steps)
     st = B.stats()
@@ -130,6 +135,7 @@ def measure(B, args, warm, nworlds):
         "kernel_window_contacts": float(np.mean(contacts)),
         "overflow": [int(x) for x in B.overflow()],
         "lds_rows": lds_rows,
+        "schedule": schedule,
         "worlds_over_lds_budget": int(np.sum(st["rows"] > lds_rows)),
         "_bodies": bodies_after_warmup,
     }
@@ -320,7 +326,11 @@ def main():
     ap.add_argument("--sor-w", type=float, default=1.0,
                     help="SOR relaxation of every mode (ODE's default 1.3 sends some warm-started solves into "
                          "limit cycles; cold convergence is the same at 1.0)")
-    ap.add_argument("--splits", type=int, default=4, help="world ranges stepped on parallel streams")
+    ap.add_argument("--splits", type=int, default=4,
+                    help="world ranges stepped on parallel streams (with --no-autotune)")
+    ap.add_argument("--tune-steps", type=int, default=40, help="steps per autotuner trial")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep --splits and staged launches instead of timing the launch schedules after warmup")
     ap.add_argument("--graph-steps", type=int, default=10, help="steps per graph launch")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="finally copy MIB MiB device to device (a reference kernel of known traffic for "
@@ -433,7 +443,8 @@ def main():
             "workload": args.workload,
             "worlds": args.worlds,
             "solver": solver,
-            "launch": "%d world ranges on parallel streams, %d steps per hipGraph launch" % (args.splits, args.graph_steps),
+            "launch": ("schedule autotuned per mode after warmup (modes[*].schedule)" if not args.no_autotune else
+                       "%d world ranges on parallel streams, %d steps per hipGraph launch" % (args.splits, args.graph_steps)),
             "h": 0.01, "mu": 0.5, "erp": 0.2, "cfm": 1e-5, "seed": args.seed,
         },
         "modes": results,

@@ -133,6 +133,17 @@ int odxBatchSetSplits(odxBatch *b, int nsplit, int graph_steps);
  * those of the staged launches. */
 int odxBatchSetFused(odxBatch *b, int steps_per_launch);
 
+/* Autotuner of the launch schedule (no ODE counterpart).  Times every
+ * candidate -- nsplit 1, 2, 4, 8 (odxBatchSetSplits, graph_steps as set) by
+ * staged or fused (odxBatchSetFused(b, graph_steps)) launches -- over `steps`
+ * steps of this batch, `reps` times each, and keeps the fastest; the schedule
+ * in effect at the call is kept unless a candidate beats it by more than 2 %.
+ * Every run starts from the state at the call (bodies, previous-step contacts
+ * and impulses, sweep counts), and that state is put back at the end.
+ * schedule (may be NULL): [nsplit, graph_steps, steps per fused launch or 0];
+ * ms_per_step (may be NULL): the winner's mean time per step. */
+int odxBatchAutotune(odxBatch *b, int steps, int reps, int *schedule, float *ms_per_step);
+
 /* Runs nsteps steps on the batch's stream (captured once into a hipGraph of
  * graph_steps steps and replayed).  Returns 1 on success. */
 int odxBatchStep(odxBatch *b, int nsteps);

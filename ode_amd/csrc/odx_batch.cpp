@@ -667,6 +667,81 @@ int odxBatchSetFused(odxBatch *b, int steps_per_launch) {
     return 1;
 }
 
+// Launch-schedule autotuner: every candidate (world splits 1, 2, 4, 8 times
+// staged launches or fused launches of graph_steps steps, with the graph size
+// in effect) steps the batch `steps` steps `reps` times from
+// the same body state; the fastest mean wins, where the schedule in effect at
+// the call counts 2 % faster than measured (a change must pay for itself
+// beyond run-to-run noise).  Schedules do not change results, so only time is
+// compared.
+int odxBatchAutotune(odxBatch *b, int steps, int reps, int *schedule, float *ms_per_step) {
+    if (!b || steps < 1 || reps < 1) {
+        odx_message(1, "odxBatchAutotune: bad arguments (steps >= 1, reps >= 1)");
+        return 0;
+    }
+    std::vector<float> saved((size_t)b->nb * ODX_BODY_STRIDE);
+    if (!odxBatchGetBodies(b, saved.data())) return 0;
+    // what the next step reads besides the bodies: the previous step's
+    // contacts and impulses (warm starts) and sweep counts (dispatch order)
+    DevWorlds &D = b->D;
+    const std::pair<void *, size_t> state[] = {{D.prev, (size_t)b->jcap * JOINT_STRIDE * sizeof(float)},
+                                               {D.nprev, (size_t)b->nw * sizeof(int)},
+                                               {D.lambda, (size_t)b->rcap * sizeof(float)},
+                                               {D.niters, (size_t)b->nw * sizeof(int)}};
+    std::vector<void *> copy;
+    auto release = [&]() {
+        for (void *p : copy) (void)hipFree(p);
+        copy.clear();
+    };
+    for (const auto &r : state) {
+        void *p = nullptr;
+        if (!odx_check(hipMalloc(&p, r.second), "hipMalloc")) { release(); return 0; }
+        copy.push_back(p);
+        if (!odx_check(hipMemcpy(p, r.first, r.second, hipMemcpyDeviceToDevice), "D2D")) { release(); return 0; }
+    }
+    auto restore = [&]() {
+        if (!odxBatchSetBodies(b, saved.data())) return false;
+        for (size_t i = 0; i < copy.size(); i++)
+            if (!odx_check(hipMemcpy(state[i].first, copy[i], state[i].second, hipMemcpyDeviceToDevice), "D2D"))
+                return false;
+        return true;
+    };
+    const int cur_sp = b->splits.empty() ? 1 : (int)b->splits.size(), cur_fz = b->fused, cur_gs = b->graph_steps;
+    const int split_c[] = {1, 2, 4, 8};
+    const int fused_c[] = {0, cur_gs};
+    float best = 0.f;
+    int bs = 1, bf = 0;
+    for (int fz : fused_c) {
+        for (int sp : split_c) {
+            if (sp > b->nw) continue;
+            if (!odxBatchSetSplits(b, sp, cur_gs) || !odxBatchSetFused(b, fz)) { release(); return 0; }
+            double tot = 0.0;
+            for (int r = 0; r < reps; r++) {
+                float ms = 0.f;
+                if (!restore() || !odxBatchTime(b, steps, &ms)) { release(); return 0; }
+                tot += ms;
+            }
+            float mean = (float)(tot / reps) / (float)steps;
+            if (sp == cur_sp && fz == cur_fz) mean *= 0.98f;
+            if (best == 0.f || mean < best) {
+                best = mean;
+                bs = sp;
+                bf = fz;
+            }
+        }
+    }
+    const bool ok = odxBatchSetSplits(b, bs, cur_gs) && odxBatchSetFused(b, bf) && restore();
+    release();
+    if (!ok) return 0;
+    if (schedule) {
+        schedule[0] = bs;
+        schedule[1] = cur_gs;
+        schedule[2] = bf;
+    }
+    if (ms_per_step) *ms_per_step = best;
+    return 1;
+}
+
 int odxBatchStep(odxBatch *b, int nsteps) {
     if (!b || nsteps < 0) return 0;
     if (nsteps == 0) return 1;

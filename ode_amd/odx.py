@@ -52,6 +52,7 @@ def lib():
         L.odxBatchGetLayout.argtypes = [vp, ip, ip, ip]
         L.odxBatchSetSplits.argtypes = [vp, i, i]
         L.odxBatchSetFused.argtypes = [vp, i]
+        L.odxBatchAutotune.argtypes = [vp, i, i, ip, fp]
         L.odxBatchStep.argtypes = [vp, i]
         L.odxBatchSync.argtypes = [vp]
         L.odxBatchTime.argtypes = [vp, i, fp]
@@ -198,6 +199,16 @@ class Batch:
         """> 0: that many complete steps per launch of the persistent fused
         kernel (0: staged launches)."""
         _check(lib().odxBatchSetFused(self.h, steps_per_launch), "odxBatchSetFused")
+
+    def autotune(self, steps=10, reps=2):
+        """Times the launch schedules (splits 1/2/4/8, staged or fused launches
+        of the current graph size) over `steps` steps from the current state
+        and keeps the fastest; returns
+        ({"splits", "graph_steps", "fused"}, ms per step)."""
+        sch = np.zeros(3, np.int32)
+        ms = np.zeros(1, np.float32)
+        _check(lib().odxBatchAutotune(self.h, steps, reps, _p(sch), _p(ms)), "odxBatchAutotune")
+        return {"splits": int(sch[0]), "graph_steps": int(sch[1]), "fused": int(sch[2])}, float(ms[0])
 
     def set_solve_options(self, restart_after=0, lds_rows=400):
         """restart_after: sweeps after which an unconverged warm start restarts

@@ -466,3 +466,29 @@ def test_snapshot_round_trip_continues_the_same_steps(tmp_path):
     assert np.max(np.abs(gc[:, vel] - gb[:, vel])) / max(np.max(np.abs(gb[:, vel])), 1.0) < TOL_STATE
     B.close()
     C.close()
+
+
+@pytest.mark.parametrize("warm", [0, 2])
+def test_autotuned_schedule_keeps_the_results(warm):
+    # the autotuner only picks a launch schedule and puts the state back;
+    # stepping afterwards gives the bodies that the default schedule gives
+    say("case: autotune, warm", warm)
+    from ode_amd import odx, scenes as S
+    prm = O.params(iters=3000)
+    worlds = S.tower_batch(64, seed=17, levels=8)
+    out = []
+    for tune in (False, True):
+        B = odx.Batch(worlds, prm)
+        B.set_solver(order=1, max_iters=3000, tol=1e-4, check_every=2, warm=warm)
+        B.step(3)
+        if tune:
+            B.set_splits(1, 4)
+            before = B.bodies().copy()
+            sch, ms = B.autotune(steps=6, reps=1)
+            say("  schedule", sch, "ms/step", ms)
+            assert sch["splits"] in (1, 2, 4, 8) and sch["fused"] in (0, 4) and ms > 0
+            assert np.array_equal(B.bodies(), before)
+        B.step(5)
+        out.append(B.bodies().copy())
+        B.close()
+    assert np.array_equal(out[0], out[1])
