@@ -21,6 +21,7 @@ enum {
 
 constexpr int LG_MAXCOL = 128;        // colours of the sweep schedule
 constexpr int LG_MAXSUB = 32;         // claim/commit rounds per colour
+constexpr int LG_PAIR_KEEP = 16;      // partners per geom the counting pass keeps for the writing pass
 constexpr long long LG_SPIN_LIMIT = 1ll << 21;  // polls of one barrier wait (~1 s)
 constexpr int LG_BLOCK = 256;         // threads per workgroup of the cooperative kernels
 // words of the barrier array: generation, 8 group counters (64 B apart), then
@@ -39,6 +40,7 @@ struct LargeDev {
     int *bucket_geom;                 // [ng] finite geoms grouped by bucket, ascending index in a bucket
     int *pair_cnt;                    // [ng + 1] pairs per geom -> first pair of each geom
     int2 *pairs;                      // [pair_cap] (lower geom, higher geom)
+    int *pair_keep;                   // [ng * LG_PAIR_KEEP] partners of each geom found by the counting pass
     int *contact_cnt;                 // [pair_cap + 1] contacts per pair -> first contact of each pair
     int *row_cnt;                     // [contact_cap + 1] rows per contact -> first row of each contact
     int *counters;                    // [LG_NCOUNTERS]

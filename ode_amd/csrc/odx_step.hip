@@ -2420,6 +2420,19 @@ __global__ void k_lg_pairs(DevWorlds D, LargeDev L, int write) {
         const int bi = (int)D.geom0[i].y;
         const bool ibig = L.isbig[i] != 0;
         const int off = write ? L.pair_cnt[i] : 0;
+        int *keep = L.pair_keep + (size_t)i * LG_PAIR_KEEP;
+        if (write) {
+            // the counting pass kept the partners, in the order found, when
+            // they fit: copy them instead of searching again
+            const int cnt = L.pair_cnt[i + 1] - off;
+            if (cnt <= LG_PAIR_KEEP) {
+                for (int q = 0; q < cnt && off + q < L.pair_cap; q++) {
+                    const int j = keep[q];
+                    L.pairs[off + q] = make_int2(min(i, j), max(i, j));
+                }
+                continue;
+            }
+        }
         int n = 0;
         if (!ibig) {
             const int4 c = L.gcell[i];
@@ -2437,6 +2450,7 @@ __global__ void k_lg_pairs(DevWorlds D, LargeDev L, int write) {
                             if (!lg_bodies_ok(bi, (int)D.geom0[j].y)) continue;
                             if (!lg_overlap(ai, L.aabb + (size_t)j * 6)) continue;
                             if (write && off + n < L.pair_cap) L.pairs[off + n] = make_int2(i, j);
+                            if (!write && n < LG_PAIR_KEEP) keep[n] = j;
                             n++;
                         }
                     }
@@ -2447,6 +2461,7 @@ __global__ void k_lg_pairs(DevWorlds D, LargeDev L, int write) {
             if (!lg_bodies_ok(bi, (int)D.geom0[j].y)) continue;
             if (!lg_overlap(ai, L.aabb + (size_t)j * 6)) continue;
             if (write && off + n < L.pair_cap) L.pairs[off + n] = make_int2(min(i, j), max(i, j));
+            if (!write && n < LG_PAIR_KEEP) keep[n] = j;
             n++;
         }
         if (!write) L.pair_cnt[i] = n;

@@ -132,6 +132,37 @@ def test_pairs_contacts_and_coloured_step_match_oracle(seed):
     L.close()
 
 
+def test_geom_with_more_partners_than_the_counting_pass_keeps():
+    # a box carrying 7 x 7 small spheres: its pair list (49 partners of a lower
+    # index) is longer than the 16 the counting pass keeps, so the writing pass
+    # searches again; the pair set still equals the hash space's
+    say("case: large, box with 49 partners")
+    bodies, geoms = [], []
+    m, I = S.box_mass(1000.0, 1.0, 1.0, 1.0)
+    bodies.append(S._body((0.0, 0.0, 0.5), (1, 0, 0, 0), m, I))
+    geoms.append(S.geom(S.BOX, 0, (1.0, 1.0, 1.0)))
+    ms, Is = S.sphere_mass(1000.0, 0.05)
+    for a in range(7):
+        for b in range(7):
+            k = len(bodies)
+            bodies.append(S._body((-0.42 + 0.14 * a, -0.42 + 0.14 * b, 1.045), (1, 0, 0, 0), ms, Is))
+            geoms.append(S.geom(S.SPHERE, k, (0.05,)))
+    geoms.append(S.plane(0, 0, 1, 0))
+    w = S.World(bodies, geoms)
+    prm = O.params(space=1, w=1.0)
+    L = make(w, prm)
+    L.step(1)
+    L.sync()
+    body = w.geoms[:, 1].astype(np.int32)
+    ab = O.aabbs(w.geoms, w.bodies, dtype=np.float32)
+    ref = {(int(a), int(b)) for a, b in O.pairs(ab, body, space="hash") if body[a] >= 0 or body[b] >= 0}
+    got = [tuple(map(int, p)) for p in L.pairs()]
+    assert len(got) == len(set(got))
+    assert sum(1 for a, b in got if a == 0) >= 49
+    assert set(got) == ref
+    L.close()
+
+
 def test_rows_match_oracle_contact_rows():
     say("case: large rows")
     w = S.mixed_pile(300, seed=70)
