@@ -2657,6 +2657,7 @@ __global__ __launch_bounds__(LG_BLOCK) void k_lg_colour(DevWorlds D, LargeDev L)
     bool ok = lg_grid_sync(L, gen);
     int ncol = 0, rounds = 0, left = nc;
     for (int c = 0; ok && left > 0 && c < LG_MAXCOL; c++) {
+        int joined = 0;   // contacts of colour c after its previous rounds
         for (int sub = 0; ok && sub < LG_MAXSUB; sub++) {
             const int stamp = c * LG_MAXSUB + sub;
             int e = 0;
@@ -2689,7 +2690,12 @@ __global__ __launch_bounds__(LG_BLOCK) void k_lg_colour(DevWorlds D, LargeDev L)
             }
             won = lg_block_sum(won, s_red);
             if (threadIdx.x == 0 && won) atomicAdd(&L.col_start[c], won);
-            ok = lg_grid_sync(L, gen);
+            int total = 0;
+            ok = lg_grid_sync(L, gen, &L.col_start[c], &total);
+            // every eligible contact joined: each contact still uncoloured
+            // touches a body of colour c, so another round would find none
+            if (total - joined == elig) break;
+            joined = total;
         }
         if (!ok) break;
         ncol = c + 1;
