@@ -137,7 +137,7 @@ int odxBatchSetFused(odxBatch *b, int steps_per_launch);
  * candidate -- nsplit 1, 2, 4, 8 (odxBatchSetSplits, graph_steps as set) by
  * staged or fused (odxBatchSetFused(b, graph_steps)) launches -- over `steps`
  * steps of this batch, `reps` times each, and keeps the fastest; the schedule
- * in effect at the call is kept unless a candidate beats it by more than 2 %.
+ * in effect at the call is kept unless a candidate beats it by more than 5 %.
  * Every run starts from the state at the call (bodies, previous-step contacts
  * and impulses, sweep counts), and that state is put back at the end.
  * schedule (may be NULL): [nsplit, graph_steps, steps per fused launch or 0];

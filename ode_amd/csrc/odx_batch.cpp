@@ -671,8 +671,8 @@ int odxBatchSetFused(odxBatch *b, int steps_per_launch) {
 // staged launches or fused launches of graph_steps steps, with the graph size
 // in effect) steps the batch `steps` steps `reps` times from
 // the same body state; the fastest mean wins, where the schedule in effect at
-// the call counts 2 % faster than measured (a change must pay for itself
-// beyond run-to-run noise).  Schedules do not change results, so only time is
+// the call counts 5 % faster than measured (a change must pay for itself
+// beyond the noise of trials shorter than the run they tune for).  Schedules do not change results, so only time is
 // compared.
 int odxBatchAutotune(odxBatch *b, int steps, int reps, int *schedule, float *ms_per_step) {
     if (!b || steps < 1 || reps < 1) {
@@ -722,7 +722,7 @@ int odxBatchAutotune(odxBatch *b, int steps, int reps, int *schedule, float *ms_
                 tot += ms;
             }
             float mean = (float)(tot / reps) / (float)steps;
-            if (sp == cur_sp && fz == cur_fz) mean *= 0.98f;
+            if (sp == cur_sp && fz == cur_fz) mean *= 0.95f;
             if (best == 0.f || mean < best) {
                 best = mean;
                 bs = sp;
