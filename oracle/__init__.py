@@ -258,3 +258,78 @@ def solve_to_residual(bodies, joints, prm, target, maxit, balls=None):
                                      ctypes.c_int(len(joints)), _p(joints), _p(np.ascontiguousarray(prm, np.float64)),
                                      ctypes.c_double(target), ctypes.c_int(maxit), _p(lam), ctypes.byref(res))
     return it, res.value, lam
+
+
+class SpaceModel:
+    """The callback order of one ODE space across calls (oracle/spaces.cpp):
+    kind "sap" (axisorder as dSweepAndPruneSpaceCreate), "hash" (levels) or
+    "simple".  Geoms are integer ids; add/remove/moved mirror dSpaceAdd,
+    dSpaceRemove and dGeomMoved."""
+
+    KINDS = {"sap": 0, "hash": 1, "simple": 2}
+
+    def __init__(self, kind, axisorder=0, levels=(-3, 10)):
+        self._lib = lib()
+        self._lib.orc_space_create.restype = ctypes.c_void_p
+        self.h = ctypes.c_void_p(self._lib.orc_space_create(ctypes.c_int(self.KINDS[kind]), ctypes.c_int(axisorder),
+                                                            ctypes.c_int(levels[0]), ctypes.c_int(levels[1])))
+
+    def add(self, gid):
+        self._lib.orc_space_add(self.h, ctypes.c_int(gid))
+
+    def remove(self, gid):
+        self._lib.orc_space_remove(self.h, ctypes.c_int(gid))
+
+    def moved(self, gid):
+        self._lib.orc_space_moved(self.h, ctypes.c_int(gid))
+
+    def order(self, cap=1 << 16):
+        out = np.zeros(cap, np.int32)
+        n = self._lib.orc_space_order(self.h, _p(out), ctypes.c_int(cap))
+        return [int(x) for x in out[:n]]
+
+    def collide(self, aabb, body, cat=None, col=None, enabled=None):
+        """The callback sequence [(o1, o2), ...] of one dSpaceCollide.  Per-id
+        arrays: aabb [n, 6] float32, body [n] (-1 none), cat/col [n] uint64
+        (default all bits), enabled [n] (default all)."""
+        aabb = np.ascontiguousarray(aabb, np.float32)
+        n = len(aabb)
+        body = np.ascontiguousarray(body, np.int32)
+        ones = np.full(n, np.iinfo(np.uint64).max, np.uint64)
+        cat = np.ascontiguousarray(ones if cat is None else cat, np.uint64)
+        col = np.ascontiguousarray(ones if col is None else col, np.uint64)
+        en = np.ascontiguousarray(np.ones(n, np.uint8) if enabled is None else enabled, np.uint8)
+        cap = max(64, n * n)
+        out = np.zeros(2 * cap, np.int32)
+        k = self._lib.orc_space_collide(self.h, _p(aabb), _p(body), _p(cat), _p(col), _p(en), _p(out), ctypes.c_int(cap))
+        return [(int(out[2 * i]), int(out[2 * i + 1])) for i in range(min(k, cap))]
+
+    def __del__(self):
+        try:
+            self._lib.orc_space_destroy(self.h)
+        except Exception:
+            pass
+
+
+class RadixModel:
+    """The SAP space's radix sort alone, ranks kept between calls."""
+
+    def __init__(self):
+        self._lib = lib()
+        self._lib.orc_radix_create.restype = ctypes.c_void_p
+        self.h = ctypes.c_void_p(self._lib.orc_radix_create())
+
+    def sort(self, keys):
+        keys = np.ascontiguousarray(keys, np.float32)
+        out = np.zeros(max(1, len(keys)), np.uint32)
+        self._lib.orc_radix_sort(self.h, _p(keys), ctypes.c_int(len(keys)), _p(out))
+        return [int(x) for x in out[: len(keys)]]
+
+    def valid(self):
+        return bool(self._lib.orc_radix_valid(self.h))
+
+    def __del__(self):
+        try:
+            self._lib.orc_radix_destroy(self.h)
+        except Exception:
+            pass

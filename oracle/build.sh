@@ -5,4 +5,4 @@
 set -e
 cd "$(dirname "$0")"
 g++ -O2 -std=c++17 -fPIC -shared -ffp-contract=off -fno-fast-math -Wall -Wno-unused-function \
-    -o liboracle.so oracle.cpp
+    -o liboracle.so oracle.cpp spaces.cpp
