@@ -296,6 +296,7 @@ void dSpaceDestroy(dSpaceID space);                          /* collision_space.
 void dHashSpaceSetLevels(dSpaceID space, int minlevel, int maxlevel); /* collision_space.h:72 */
 void dSpaceSetCleanup(dSpaceID space, int mode);             /* collision_space.h:75 */
 void dSpaceAdd(dSpaceID space, dGeomID g);                   /* collision_space.h:149 */
+void dSpaceRemove(dSpaceID space, dGeomID g);                /* collision_space.h:150 */
 int dSpaceGetNumGeoms(dSpaceID space);                       /* collision_space.h:153 */
 
 /* ----------------------------------------------------------------- misc */

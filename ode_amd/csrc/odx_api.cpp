@@ -25,8 +25,8 @@ namespace odx {
 void launch_single_step(const DevWorlds &D, const StepParams &P, int nb, int m, int nitems, int nphases,
                         hipStream_t s);
 void launch_aabbs(const float *gviews, int n, float *aabb, hipStream_t s);
-void launch_pairs_all(const float *gviews, const float *aabb, const unsigned *bits, int n, int space, int *pairs,
-                      int *count, int cap, hipStream_t s);
+void launch_space_collide(const SpaceCall &C, hipStream_t s);
+int space_max_geoms();
 void launch_collide_pairs(const float *gviews, const int *pairs, int np, int maxc, float *out, int *counts,
                           hipStream_t s);
 }  // namespace odx
@@ -50,9 +50,12 @@ struct DevBuf {
         cap = want;
         return true;
     }
-    ~DevBuf() {
+    void release() {
         if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
     }
+    ~DevBuf() { release(); }
     template <class T> T *as() const { return reinterpret_cast<T *>(p); }
 };
 
@@ -87,9 +90,12 @@ struct PinnedBuf {
         cap = want;
         return true;
     }
-    ~PinnedBuf() {
+    void release() {
         if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
     }
+    ~PinnedBuf() { release(); }
     template <class T> T *as() const { return reinterpret_cast<T *>(p); }
 };
 
@@ -245,11 +251,26 @@ struct dxJointGroup {
     std::vector<dxJoint *> joints;
 };
 
+// A space keeps its geoms as the reference's space classes do, because the
+// order of the near-callback calls depends on it:
+//  * sweep and prune (collision_sapspace.cpp:300-421): GeomList and
+//    DirtyList, a moved geom swap-removed from the first and appended to the
+//    second, the dirty ones appended to GeomList at the next collide;
+//  * simple and hash (collision_space.cpp:166-209): one list with new and
+//    moved geoms at its front.
+// The radix sort's ranks stay on the device between calls.
 struct dxSpace {
     int kind = 2;  // 0 SAP, 1 hash, 2 simple
+    int ax0 = 0, ax1 = 2, ax2 = 4;  // SAP axes as AABB indices (dSAP_AXES_*)
     int minlevel = -3, maxlevel = 10;
     int cleanup = 1;
-    std::vector<dxGeom *> geoms;
+    int count = 0;
+    dxGeom *first = nullptr;            // simple / hash list
+    std::vector<dxGeom *> glist, dlist;  // sweep and prune: GeomList, DirtyList
+    DevBuf ranks[2];                    // RaixSortContext ranks (ping-pong)
+    int rcur = 0;
+    bool ranks_valid = false;
+    size_t ranks_used = 0;
     dxSpace *parent = nullptr;
 };
 
@@ -262,6 +283,10 @@ struct dxGeom {
     dMatrix3 R = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
     unsigned long cat = ~0ul, col = ~0ul;
     bool enabled = true;
+    bool zero_sized = false;  // GEOM_ZERO_SIZED: never collides
+    bool dirty = true;        // GEOM_DIRTY: set at creation, cleared by the space's collide
+    int gidx = -1, didx = -1;                   // SAP GeomList / DirtyList slots
+    dxGeom *snext = nullptr, *sprev = nullptr;  // simple / hash list links
     void *data = nullptr;
 };
 
@@ -417,7 +442,26 @@ struct Session {
     DevBuf dpairs;                // candidate pairs (device)
     const float *d_gviews = nullptr;
 };
-Session g_sess;
+// The sessions live on the heap and are never destroyed at exit: a static
+// destructor would free device and pinned memory after the HIP runtime's own
+// exit handlers may have run.  dCloseODE releases their memory.
+Session &sess() {
+    static Session *s = new Session();
+    return *s;
+}
+Session &lone_session() {
+    static Session *s = new Session();
+    return *s;
+}
+void release_session(Session &S) {
+    S.space.dev.release();
+    S.space.host.release();
+    S.coll.dev.release();
+    S.coll.host.release();
+    S.hpairs.release();
+    S.dpairs.release();
+    S.d_gviews = nullptr;
+}
 
 // hipEvent timing of dWorldQuickStep's device step, on request
 struct KernelTiming {
@@ -552,6 +596,78 @@ bool gpu_collide_pairs(Session &S, int maxc) {
     return true;
 }
 
+
+// ---------------------------------------------------------- space lists
+void list_push_front(dxSpace *s, dxGeom *g) {
+    g->sprev = nullptr;
+    g->snext = s->first;
+    if (s->first) s->first->sprev = g;
+    s->first = g;
+}
+void list_unlink(dxSpace *s, dxGeom *g) {
+    if (g->sprev) g->sprev->snext = g->snext; else s->first = g->snext;
+    if (g->snext) g->snext->sprev = g->sprev;
+    g->snext = g->sprev = nullptr;
+}
+// swap the last element of a SAP list into slot idx (collision_sapspace.cpp:343-366)
+void sap_swap_remove(std::vector<dxGeom *> &list, int idx, bool geomlist) {
+    const int last = (int)list.size() - 1;
+    if (idx != last) {
+        dxGeom *moved = list[(size_t)last];
+        list[(size_t)idx] = moved;
+        if (geomlist) moved->gidx = idx; else moved->didx = idx;
+    }
+    list.pop_back();
+}
+// dxSpace::add / dxSAPSpace::add: the geom's flags are left as they are
+void space_add(dxSpace *s, dxGeom *g) {
+    g->space = s;
+    s->count++;
+    if (s->kind == 0) {
+        g->gidx = -1;
+        g->didx = (int)s->dlist.size();
+        s->dlist.push_back(g);
+    } else {
+        list_push_front(s, g);
+    }
+}
+void space_remove(dxGeom *g) {
+    dxSpace *s = g->space;
+    if (!s) return;
+    if (s->kind == 0) {
+        if (g->didx >= 0) sap_swap_remove(s->dlist, g->didx, false);
+        else if (g->gidx >= 0) sap_swap_remove(s->glist, g->gidx, true);
+    } else {
+        list_unlink(s, g);
+    }
+    g->gidx = g->didx = -1;
+    s->count--;
+    g->space = nullptr;
+}
+// dGeomMoved (collision_space.cpp:49-79): a clean geom of a space becomes
+// dirty and is moved by its space's dirty() (collision_sapspace.cpp:368-389;
+// collision_space.cpp:205-209); a geom in no space is only marked dirty.
+void geom_moved(dxGeom *g) {
+    dxSpace *s = g->space;
+    if (!s) { g->dirty = true; return; }
+    if (g->dirty) return;
+    g->dirty = true;
+    if (s->kind == 0) {
+        if (g->didx >= 0) return;
+        sap_swap_remove(s->glist, g->gidx, true);
+        g->gidx = -1;
+        g->didx = (int)s->dlist.size();
+        s->dlist.push_back(g);
+    } else {
+        list_unlink(s, g);
+        list_push_front(s, g);
+    }
+}
+// every geom of a body, most recently attached first (the body's geom list,
+// bodyAdd collision_kernel.h:202-206)
+void body_moved(dxBody *b) {
+    for (auto it = b->geoms.rbegin(); it != b->geoms.rend(); ++it) geom_moved(*it);
+}
 }  // namespace
 
 extern "C" {
@@ -560,7 +676,12 @@ extern "C" {
 void dInitODE(void) { (void)dInitODE2(0); }
 int dInitODE2(unsigned int) { return ensure_stream() ? 1 : 0; }
 void dCloseODE(void) {
-    if (g_stream) { (void)hipStreamDestroy(g_stream); g_stream = nullptr; }
+    if (!g_stream) return;
+    (void)hipStreamSynchronize(g_stream);
+    release_session(sess());
+    release_session(lone_session());
+    (void)hipStreamDestroy(g_stream);
+    g_stream = nullptr;
 }
 
 // ----------------------------------------------------------------- world
@@ -581,7 +702,11 @@ void dWorldDestroy(dWorldID w) {
     }
     for (dxBody *b = w->firstbody; b;) {
         dxBody *n = b->next;
-        for (dxGeom *g : b->geoms) g->body = nullptr;
+        // each geom keeps the body's last pose, as dGeomSetBody(g, 0) does
+    for (dxGeom *g : b->geoms) {
+        geom_pose(g, g->pos, g->R);
+        g->body = nullptr;
+    }
         delete b;
         b = n;
     }
@@ -877,7 +1002,11 @@ dBodyID dBodyCreate(dWorldID w) {
 void dBodyDestroy(dBodyID b) {
     if (!b) return;
     dxWorld *w = b->world;
-    for (dxGeom *g : b->geoms) g->body = nullptr;
+    // each geom keeps the body's last pose, as dGeomSetBody(g, 0) does
+    for (dxGeom *g : b->geoms) {
+        geom_pose(g, g->pos, g->R);
+        g->body = nullptr;
+    }
     // detach joints (ODE detaches group joints and destroys free ones' links)
     while (b->firstjoint) detach(b->firstjoint->joint);
     if (b->prev) b->prev->next = b->next;
@@ -889,17 +1018,23 @@ void dBodyDestroy(dBodyID b) {
 
 void dBodySetData(dBodyID b, void *data) { b->data = data; }
 void *dBodyGetData(dBodyID b) { return b->data; }
-void dBodySetPosition(dBodyID b, dReal x, dReal y, dReal z) { b->pos[0] = x; b->pos[1] = y; b->pos[2] = z; }
+// the body setters notify the attached geoms (ode.cpp:344-395)
+void dBodySetPosition(dBodyID b, dReal x, dReal y, dReal z) {
+    b->pos[0] = x; b->pos[1] = y; b->pos[2] = z;
+    body_moved(b);
+}
 void dBodySetRotation(dBodyID b, const dMatrix3 R) {
     std::memcpy(b->R, R, sizeof(dMatrix3));
     qfromr(b->q, b->R);
     normq(b->q);
     rfromq(b->R, b->q);
+    body_moved(b);
 }
 void dBodySetQuaternion(dBodyID b, const dQuaternion q) {
     for (int k = 0; k < 4; k++) b->q[k] = q[k];
     normq(b->q);
     rfromq(b->R, b->q);
+    body_moved(b);
 }
 void dBodySetLinearVel(dBodyID b, dReal x, dReal y, dReal z) { b->lvel[0] = x; b->lvel[1] = y; b->lvel[2] = z; }
 void dBodySetAngularVel(dBodyID b, dReal x, dReal y, dReal z) { b->avel[0] = x; b->avel[1] = y; b->avel[2] = z; }
@@ -1248,38 +1383,43 @@ void dMassSetCapsuleTotal(dMass *m, dReal total, int direction, dReal r, dReal l
 static dxGeom *new_geom(dSpaceID space, int cls) {
     auto *g = new dxGeom();
     g->cls = cls;
-    if (space) { g->space = space; space->geoms.push_back(g); }
+    if (space) space_add(space, g);
     return g;
 }
 dGeomID dCreateSphere(dSpaceID space, dReal radius) {
     dxGeom *g = new_geom(space, dSphereClass);
     g->p[0] = radius;
+    g->zero_sized = radius == 0;  // sphere.cpp:55
     return g;
 }
 dGeomID dCreateBox(dSpaceID space, dReal lx, dReal ly, dReal lz) {
     dxGeom *g = new_geom(space, dBoxClass);
     g->p[0] = lx; g->p[1] = ly; g->p[2] = lz;
+    g->zero_sized = lx == 0 || ly == 0 || lz == 0;  // box.cpp:56
     return g;
 }
+// make_sure_plane_normal_has_unit_length (plane.cpp:49-65)
 dGeomID dCreatePlane(dSpaceID space, dReal a, dReal b, dReal c, dReal d) {
     dxGeom *g = new_geom(space, dPlaneClass);
-    float l = std::sqrt(a * a + b * b + c * c);
-    if (l > 0) { a /= l; b /= l; c /= l; d /= l; }
-    else { a = 1; b = 0; c = 0; d = 0; }
+    float l = a * a + b * b + c * c;
+    if (l > 0) {
+        l = 1.0f / std::sqrt(l);
+        a *= l; b *= l; c *= l; d *= l;
+    } else {
+        a = 1; b = 0; c = 0; d = 0;
+    }
     g->p[0] = a; g->p[1] = b; g->p[2] = c; g->p[3] = d;
     return g;
 }
 dGeomID dCreateCapsule(dSpaceID space, dReal radius, dReal length) {
     dxGeom *g = new_geom(space, dCapsuleClass);
     g->p[0] = radius; g->p[1] = length;
+    g->zero_sized = radius == 0;  // capsule.cpp:56 (a zero length is not zero-sized)
     return g;
 }
 void dGeomDestroy(dGeomID g) {
     if (!g) return;
-    if (g->space) {
-        auto &v = g->space->geoms;
-        v.erase(std::remove(v.begin(), v.end(), g), v.end());
-    }
+    space_remove(g);
     if (g->body) {
         auto &v = g->body->geoms;
         v.erase(std::remove(v.begin(), v.end(), g), v.end());
@@ -1288,31 +1428,43 @@ void dGeomDestroy(dGeomID g) {
 }
 void dGeomSetData(dGeomID g, void *data) { g->data = data; }
 void *dGeomGetData(dGeomID g) { return g->data; }
+// dGeomSetBody collision_kernel.cpp:511-551: attaching moves the geom;
+// detaching keeps the body's last pose and does not
 void dGeomSetBody(dGeomID g, dBodyID b) {
-    if (g->body) {
+    if (b) {
+        if (g->body != b) {
+            if (g->body) {
+                auto &v = g->body->geoms;
+                v.erase(std::remove(v.begin(), v.end(), g), v.end());
+            }
+            g->body = b;
+            b->geoms.push_back(g);
+        }
+        geom_moved(g);
+    } else if (g->body) {
+        geom_pose(g, g->pos, g->R);
         auto &v = g->body->geoms;
         v.erase(std::remove(v.begin(), v.end(), g), v.end());
+        g->body = nullptr;
     }
-    if (!b && g->body) {  // keeps the last pose, as dGeomSetBody(g, 0) does
-        geom_pose(g, g->pos, g->R);
-    }
-    g->body = b;
-    if (b) b->geoms.push_back(g);
 }
 dBodyID dGeomGetBody(dGeomID g) { return g->body; }
 void dGeomSetPosition(dGeomID g, dReal x, dReal y, dReal z) {
-    if (g->body) dBodySetPosition(g->body, x, y, z);
-    else { g->pos[0] = x; g->pos[1] = y; g->pos[2] = z; }
+    if (g->body) { dBodySetPosition(g->body, x, y, z); return; }
+    g->pos[0] = x; g->pos[1] = y; g->pos[2] = z;
+    geom_moved(g);
 }
 void dGeomSetRotation(dGeomID g, const dMatrix3 R) {
-    if (g->body) dBodySetRotation(g->body, R);
-    else std::memcpy(g->R, R, sizeof(dMatrix3));
+    if (g->body) { dBodySetRotation(g->body, R); return; }
+    std::memcpy(g->R, R, sizeof(dMatrix3));
+    geom_moved(g);
 }
 void dGeomSetQuaternion(dGeomID g, const dQuaternion q) {
     if (g->body) { dBodySetQuaternion(g->body, q); return; }
     float qq[4] = {q[0], q[1], q[2], q[3]};
     normq(qq);
     rfromq(g->R, qq);
+    geom_moved(g);
 }
 const dReal *dGeomGetPosition(dGeomID g) { return g->body ? g->body->pos : g->pos; }
 dSpaceID dGeomGetSpace(dGeomID g) { return g->space; }
@@ -1358,133 +1510,219 @@ void dGeomGetAABB(dGeomID g, dReal aabb[6]) {
 // ---------------------------------------------------------------- spaces
 dSpaceID dSimpleSpaceCreate(dSpaceID parent) { auto *s = new dxSpace(); s->kind = 2; s->parent = parent; return s; }
 dSpaceID dHashSpaceCreate(dSpaceID parent) { auto *s = new dxSpace(); s->kind = 1; s->parent = parent; return s; }
-dSpaceID dSweepAndPruneSpaceCreate(dSpaceID parent, int) { auto *s = new dxSpace(); s->kind = 0; s->parent = parent; return s; }
+// axis order: ax0 = (axisorder & 3), ax1 = (axisorder >> 2) & 3, ax2 =
+// (axisorder >> 4) & 3 (collision_sapspace.cpp:291-293)
+dSpaceID dSweepAndPruneSpaceCreate(dSpaceID parent, int axisorder) {
+    auto *s = new dxSpace();
+    s->kind = 0;
+    s->parent = parent;
+    s->ax0 = (axisorder & 3) << 1;
+    s->ax1 = ((axisorder >> 2) & 3) << 1;
+    s->ax2 = ((axisorder >> 4) & 3) << 1;
+    return s;
+}
 void dSpaceDestroy(dSpaceID s) {
     if (!s) return;
-    std::vector<dxGeom *> gs = s->geoms;
+    std::vector<dxGeom *> gs;
+    for (dxGeom *g = s->first; g; g = g->snext) gs.push_back(g);
+    gs.insert(gs.end(), s->dlist.begin(), s->dlist.end());
+    gs.insert(gs.end(), s->glist.begin(), s->glist.end());
     for (dxGeom *g : gs) {
         if (s->cleanup) dGeomDestroy(g);
-        else g->space = nullptr;
+        else space_remove(g);
     }
     delete s;
 }
 void dHashSpaceSetLevels(dSpaceID s, int minlevel, int maxlevel) { s->minlevel = minlevel; s->maxlevel = maxlevel; }
 void dSpaceSetCleanup(dSpaceID s, int mode) { s->cleanup = mode; }
 void dSpaceAdd(dSpaceID s, dGeomID g) {
-    if (g->space) {
-        auto &v = g->space->geoms;
-        v.erase(std::remove(v.begin(), v.end(), g), v.end());
-    }
-    g->space = s;
-    s->geoms.push_back(g);
+    space_remove(g);
+    space_add(s, g);
 }
-int dSpaceGetNumGeoms(dSpaceID s) { return (int)s->geoms.size(); }
+void dSpaceRemove(dSpaceID s, dGeomID g) {
+    if (g && g->space == s) space_remove(g);
+}
+int dSpaceGetNumGeoms(dSpaceID s) { return s->count; }
 
-// dSpaceCollide: candidate pairs on the GPU, ordered on the host as the space
-// would report them, then the callback per pair.
+// dSpaceCollide (collision_space.cpp:779-784): the space's geoms in its own
+// order go to the device once; k_space_collide computes their AABBs and the
+// candidate pairs in the order the space reports them; the pairs with an
+// infinite (sweep and prune) or big (hash) AABB follow in the order the
+// space checks them; then the callback runs once per pair.
 void dSpaceCollide(dSpaceID space, void *data, dNearCallback *callback) {
     if (!space || !callback || !ensure_stream()) return;
-    Session &S = g_sess;
+    Session &S = sess();
     S.geoms.clear();
-    for (dxGeom *g : space->geoms)
-        if (g->enabled) S.geoms.push_back(g);
-    const int n = (int)S.geoms.size();
     S.pairs.clear();
     S.maxc = 0;
-    if (n >= 2) {
+    const int kind = space->kind;
+    // cleanGeoms: SAP appends its dirty list (collision_sapspace.cpp:391-421);
+    // the lists clear the dirty flags of their leading dirty geoms
+    // (collision_space.cpp:235-247, 394-409).  A hash space of fewer than two
+    // geoms returns before cleaning (:425).
+    if (kind == 0) {
+        for (dxGeom *g : space->dlist) {
+            g->dirty = false;
+            g->didx = -1;
+            g->gidx = (int)space->glist.size();
+            space->glist.push_back(g);
+        }
+        space->dlist.clear();
+    } else if (kind == 2 || space->count >= 2) {
+        for (dxGeom *g = space->first; g && g->dirty; g = g->snext) g->dirty = false;
+    }
+    std::vector<dxGeom *> order;
+    if (kind == 0) order = space->glist;
+    else if (kind == 2 || space->count >= 2)
+        for (dxGeom *g = space->first; g; g = g->snext) order.push_back(g);
+    int nsort = 0;
+    if (kind == 0) {
+        // TmpGeomList (finite on the first axis) then TmpInfGeomList (:437-449)
+        std::vector<dxGeom *> inf;
+        for (dxGeom *g : order) {
+            if (!g->enabled || g->zero_sized) continue;
+            float ab[6];
+            dGeomGetAABB(g, ab);
+            if (ab[space->ax0 + 1] == dInfinity) inf.push_back(g);
+            else S.geoms.push_back(g);
+        }
+        nsort = (int)S.geoms.size();
+        S.geoms.insert(S.geoms.end(), inf.begin(), inf.end());
+    } else {
+        for (dxGeom *g : order)
+            if (g->enabled && !g->zero_sized) S.geoms.push_back(g);
+        nsort = (int)S.geoms.size();
+    }
+    const int n = (int)S.geoms.size();
+    if (n > space_max_geoms()) {
+        odx_message(4, "dSpaceCollide: %d enabled geoms exceed this build's limit of %d per space", n,
+                    space_max_geoms());
+        return;
+    }
+    if (n >= 1) {
         std::vector<float> gv((size_t)n * GVIEW_STRIDE);
-        std::vector<unsigned> bits((size_t)n * 2);
+        std::vector<unsigned long long> bits((size_t)n * 2);
         for (int i = 0; i < n; i++) {
             geom_view(S.geoms[i], &gv[(size_t)i * GVIEW_STRIDE]);
-            bits[2 * i] = (unsigned)S.geoms[i]->cat;
-            bits[2 * i + 1] = (unsigned)S.geoms[i]->col;
+            bits[2 * i] = (unsigned long long)S.geoms[i]->cat;
+            bits[2 * i + 1] = (unsigned long long)S.geoms[i]->col;
         }
-        const long long T = (long long)n * (n - 1) / 2;
-        int cap = (int)std::min<long long>(T, 1 << 26);
-        // one block: geom views and bits up with a zeroed counter; the counter
-        // and the AABBs come back in one copy
+        // the sort keeps its ranks while the count of sorted geoms stays the
+        // same (RaixSortContext::ReallocateRanksIfNecessary, :111-125); no
+        // sort runs without finite geoms
+        const bool sorts = kind == 0 && nsort > 0;
+        if (sorts) {
+            if ((size_t)nsort != space->ranks_used) {
+                space->ranks_valid = false;
+                space->ranks_used = (size_t)nsort;
+            }
+            if (!space->ranks[0].ensure((size_t)nsort * sizeof(unsigned)) ||
+                !space->ranks[1].ensure((size_t)nsort * sizeof(unsigned)))
+                return;
+        }
         Staging &G = S.space;
         G.reset();
-        const size_t o_gv = G.slot(gv.size() * sizeof(float)), o_bits = G.slot(bits.size() * sizeof(unsigned));
-        const size_t o_cnt = G.slot(sizeof(int)), o_ab = G.slot((size_t)n * 6 * sizeof(float));
-        bool ok = G.alloc() && S.dpairs.ensure((size_t)cap * 2 * sizeof(int));
-        if (!ok) return;
+        const size_t o_info = G.slot(4 * sizeof(int)), o_ab = G.slot((size_t)n * 6 * sizeof(float));
+        const size_t out_end = G.used;
+        const size_t o_gv = G.slot(gv.size() * sizeof(float));
+        const size_t o_bits = G.slot(bits.size() * sizeof(unsigned long long));
+        const size_t o_cells = G.slot(kind == 1 ? (size_t)n * 8 * sizeof(int) : 0);
+        int cap = std::max(64, 16 * n);
+        if (!G.alloc() || !S.dpairs.ensure((size_t)cap * 2 * sizeof(int))) return;
+        cap = (int)std::min<size_t>(S.dpairs.cap / (2 * sizeof(int)), (size_t)INT32_MAX / 2);
         G.put(o_gv, gv.data(), gv.size());
         G.put(o_bits, bits.data(), bits.size());
-        *G.h<int>(o_cnt) = 0;
         S.d_gviews = G.d<float>(o_gv);
-        ok = G.send(0, o_cnt + sizeof(int));
-        if (ok) {
-            launch_aabbs(S.d_gviews, n, G.d<float>(o_ab), g_stream);
-            launch_pairs_all(S.d_gviews, G.d<float>(o_ab), G.d<unsigned>(o_bits), n, space->kind == 0 ? 0 : 1,
-                             S.dpairs.as<int>(), G.d<int>(o_cnt), cap, g_stream);
-            ok = odx_check(hipGetLastError(), "k_pairs_all") && G.fetch(o_cnt, G.used - o_cnt) &&
-                 odx_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize");
+        if (!G.send(o_gv, o_cells - o_gv)) return;
+        SpaceCall C{};
+        C.gviews = G.d<float>(o_gv);
+        C.bits = G.d<unsigned long long>(o_bits);
+        C.n = n;
+        C.nsort = nsort;
+        C.kind = kind;
+        C.ax0 = space->ax0; C.ax1 = space->ax1; C.ax2 = space->ax2;
+        C.minlevel = space->minlevel; C.maxlevel = space->maxlevel;
+        C.ranks_in = sorts ? space->ranks[space->rcur].as<unsigned>() : nullptr;
+        C.ranks_valid = sorts && space->ranks_valid ? 1 : 0;
+        C.ranks_out = sorts ? space->ranks[1 - space->rcur].as<unsigned>() : nullptr;
+        C.aabb = G.d<float>(o_ab);
+        C.cells = kind == 1 ? G.d<int>(o_cells) : nullptr;
+        C.info = G.d<int>(o_info);
+        int total = 0;
+        for (int attempt = 0; attempt < 2; attempt++) {
+            C.pairs = S.dpairs.as<int>();
+            C.cap = cap;
+            launch_space_collide(C, g_stream);
+            if (!odx_check(hipGetLastError(), "k_space_collide") || !G.fetch(0, out_end) ||
+                !odx_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize"))
+                return;
+            total = G.h<int>(o_info)[0];
+            if (total <= cap) break;
+            // more pairs than room: grow to the count and run again from the same ranks
+            if (attempt == 1 || !S.dpairs.ensure((size_t)total * 2 * sizeof(int))) {
+                odx_message(4, "dSpaceCollide: no room for %d candidate pairs", total);
+                return;
+            }
+            cap = (int)std::min<size_t>(S.dpairs.cap / (2 * sizeof(int)), (size_t)INT32_MAX / 2);
         }
-        if (!ok) return;
-        int count = *G.h<int>(o_cnt);
-        if (count > cap) {
-            odx_message(3, "dSpaceCollide: %d candidate pairs exceed the buffer (%d); extra pairs dropped", count, cap);
-            count = cap;
+        if (sorts) {
+            space->ranks_valid = G.h<int>(o_info)[1] != 0;
+            space->rcur = 1 - space->rcur;
         }
-        std::vector<int> raw((size_t)count * 2);
-        const float *abp = G.h<float>(o_ab);
-        std::vector<float> ab(abp, abp + (size_t)n * 6);
-        if (count > 0) {
-            if (!S.hpairs.ensure(raw.size() * sizeof(int)) ||
-                !odx_check(hipMemcpyAsync(S.hpairs.p, S.dpairs.p, raw.size() * sizeof(int), hipMemcpyDeviceToHost,
+        S.pairs.resize((size_t)total * 2);
+        if (total > 0) {
+            if (!S.hpairs.ensure(S.pairs.size() * sizeof(int)) ||
+                !odx_check(hipMemcpyAsync(S.hpairs.p, S.dpairs.p, S.pairs.size() * sizeof(int), hipMemcpyDeviceToHost,
                                           g_stream), "hipMemcpyAsync D2H") ||
                 !odx_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize"))
                 return;
-            std::memcpy(raw.data(), S.hpairs.p, raw.size() * sizeof(int));
+            std::memcpy(S.pairs.data(), S.hpairs.p, S.pairs.size() * sizeof(int));
         }
-        // order: sweep-and-prune emits finite pairs by the x-min sort order of
-        // the first geom then the second (BoxPruning collision_sapspace.cpp:560-621),
-        // then infinite x infinite, then infinite x finite (:517-532); the other
-        // spaces report pairs in geom order.
-        std::vector<std::pair<long long, int>> keyed((size_t)count);
-        if (space->kind == 0) {
-            std::vector<int> fin, inf;
-            for (int i = 0; i < n; i++) (ab[(size_t)i * 6 + 1] == dInfinity ? inf : fin).push_back(i);
-            auto fkey = [&](int i) {
-                float f = ab[(size_t)i * 6];
-                uint32_t u;
-                std::memcpy(&u, &f, 4);
-                return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-            };
-            std::vector<int> order = fin;
-            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return fkey(a) < fkey(b); });
-            std::vector<int> rank(n, -1), irank(n, -1), frank(n, -1);
-            for (size_t k = 0; k < order.size(); k++) rank[order[k]] = (int)k;
-            for (size_t k = 0; k < inf.size(); k++) irank[inf[k]] = (int)k;
-            for (size_t k = 0; k < fin.size(); k++) frank[fin[k]] = (int)k;
-            const long long F = (long long)fin.size() + 1, I = (long long)inf.size() + 1;
-            for (int k = 0; k < count; k++) {
-                int a = raw[2 * k], b = raw[2 * k + 1];
-                long long key;
-                if (rank[a] >= 0 && rank[b] >= 0) {
-                    if (rank[a] > rank[b]) std::swap(a, b);
-                    key = rank[a] * F + rank[b];
-                    raw[2 * k] = a; raw[2 * k + 1] = b;
-                } else if (irank[a] >= 0 && irank[b] >= 0) {
-                    if (irank[a] > irank[b]) std::swap(a, b);
-                    key = F * F + (long long)irank[a] * (I + F) + irank[b];
-                    raw[2 * k] = a; raw[2 * k + 1] = b;
-                } else {
-                    if (irank[a] < 0) std::swap(a, b);  // a infinite
-                    key = F * F + (long long)irank[a] * (I + F) + I + frank[b];
-                    raw[2 * k] = a; raw[2 * k + 1] = b;
-                }
-                keyed[k] = {key, k};
+        const float *ab = G.h<float>(o_ab);
+        auto filters = [&](int a, int b) {
+            const dxGeom *x = S.geoms[a], *y = S.geoms[b];
+            if (x->body && x->body == y->body) return false;
+            return ((x->cat & y->col) | (y->cat & x->col)) != 0ul;
+        };
+        auto overlap = [&](int a, int b) {
+            const float *x = ab + (size_t)a * 6, *y = ab + (size_t)b * 6;
+            return !(x[0] > y[1] || x[1] < y[0] || x[2] > y[3] || x[3] < y[2] || x[4] > y[5] || x[5] < y[4]);
+        };
+        auto add = [&](int a, int b) { S.pairs.push_back(a); S.pairs.push_back(b); };
+        if (kind == 0) {
+            // infinite x infinite, then infinite x finite, untested (:517-532)
+            for (int m = nsort; m < n; m++) {
+                for (int k = m + 1; k < n; k++)
+                    if (filters(m, k)) add(m, k);
+                for (int k = 0; k < nsort; k++)
+                    if (filters(m, k)) add(m, k);
             }
-        } else {
-            for (int k = 0; k < count; k++) keyed[k] = {(long long)raw[2 * k] * n + raw[2 * k + 1], k};
-        }
-        std::sort(keyed.begin(), keyed.end());
-        S.pairs.resize((size_t)count * 2);
-        for (int k = 0; k < count; k++) {
-            S.pairs[2 * k] = raw[2 * keyed[k].second];
-            S.pairs[2 * k + 1] = raw[2 * keyed[k].second + 1];
+        } else if (kind == 1) {
+            // the big boxes (infinite, or above the top level): every hashed
+            // geom against each of them, then the big boxes among themselves
+            // (collision_space.cpp:595-609)
+            std::vector<int> hashed, big;
+            for (int i = 0; i < n; i++) {
+                const float *x = ab + (size_t)i * 6;
+                bool inf = false;
+                for (int k = 0; k < 6; k += 2) inf = inf || x[k] <= -dInfinity || x[k + 1] >= dInfinity;
+                int level = 0;
+                if (!inf) {
+                    float q = x[1] - x[0], q2 = x[3] - x[2];
+                    if (q2 > q) q = q2;
+                    q2 = x[5] - x[4];
+                    if (q2 > q) q = q2;
+                    std::frexp(q, &level);
+                    if (level < space->minlevel) level = space->minlevel;
+                }
+                (inf || level > space->maxlevel ? big : hashed).push_back(i);
+            }
+            for (int a : hashed)
+                for (int b : big)
+                    if (filters(a, b) && overlap(a, b)) add(a, b);
+            for (size_t a = 0; a < big.size(); a++)
+                for (size_t b = a + 1; b < big.size(); b++)
+                    if (filters(big[a], big[b]) && overlap(big[a], big[b])) add(big[a], big[b]);
         }
     }
     S.active = true;
@@ -1508,7 +1746,7 @@ int dCollide(dGeomID o1, dGeomID o2, int flags, dContactGeom *contact, int skip)
     if (skip < (int)sizeof(dContactGeom)) skip = (int)sizeof(dContactGeom);
     if (maxc > ODX_MAXC) maxc = ODX_MAXC;
     if (!ensure_stream()) return 0;
-    Session &S = g_sess;
+    Session &S = sess();
     const float *res = nullptr;
     int n = 0;
     std::vector<float> single;
@@ -1519,7 +1757,7 @@ int dCollide(dGeomID o1, dGeomID o2, int flags, dContactGeom *contact, int skip)
         res = &S.contacts[(size_t)S.current * ODX_MAXC * 7];
     } else {
         // a lone pair: one launch on its own buffers
-        static Session one;
+        Session &one = lone_session();
         one.geoms = {o1, o2};
         std::vector<float> gv(2 * GVIEW_STRIDE);
         geom_view(o1, gv.data());
@@ -2105,6 +2343,8 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
         b->avel[0] = avel[i].x; b->avel[1] = avel[i].y; b->avel[2] = avel[i].z;
         for (int k = 0; k < 3; k++) { b->facc[k] = 0; b->tacc[k] = 0; }
     }
+    // dxStepBody notifies each body's geoms as it integrates it (util.cpp:661-668)
+    for (int i = 0; i < nb; i++) body_moved(bodies[i]);
     if (want_fb) {
         for (int k = 0; k < nj; k++) {
             dJointFeedback *fb = joints[k]->feedback;

@@ -29,6 +29,59 @@ struct DContact {
     float depth;
 };
 
+// computeAABB: sphere.cpp:59-67, box.cpp:60-76, capsule.cpp:60-74, plane.cpp:81-105
+ODX_DEV void geom_aabb(const DGeom &G, float *ab) {
+    const float *p = G.pos, *R = G.R;
+    switch (G.cls) {
+    case CLS_SPHERE: {
+        float r = G.p[0];
+        ab[0] = p[0] - r; ab[1] = p[0] + r; ab[2] = p[1] - r; ab[3] = p[1] + r; ab[4] = p[2] - r; ab[5] = p[2] + r;
+        break;
+    }
+    case CLS_BOX: {
+        const float *s = G.p;
+        float xr = 0.5f * (fabsf(R[0] * s[0]) + fabsf(R[1] * s[1]) + fabsf(R[2] * s[2]));
+        float yr = 0.5f * (fabsf(R[4] * s[0]) + fabsf(R[5] * s[1]) + fabsf(R[6] * s[2]));
+        float zr = 0.5f * (fabsf(R[8] * s[0]) + fabsf(R[9] * s[1]) + fabsf(R[10] * s[2]));
+        ab[0] = p[0] - xr; ab[1] = p[0] + xr; ab[2] = p[1] - yr; ab[3] = p[1] + yr; ab[4] = p[2] - zr; ab[5] = p[2] + zr;
+        break;
+    }
+    case CLS_CAPSULE: {
+        float r = G.p[0], lz = G.p[1];
+        float xr = fabsf(R[2] * lz) * 0.5f + r, yr = fabsf(R[6] * lz) * 0.5f + r, zr = fabsf(R[10] * lz) * 0.5f + r;
+        ab[0] = p[0] - xr; ab[1] = p[0] + xr; ab[2] = p[1] - yr; ab[3] = p[1] + yr; ab[4] = p[2] - zr; ab[5] = p[2] + zr;
+        break;
+    }
+    case CLS_PLANE: {
+        const float *n = G.p;
+        ab[0] = -kInf; ab[1] = kInf; ab[2] = -kInf; ab[3] = kInf; ab[4] = -kInf; ab[5] = kInf;
+        if (n[1] == 0.f && n[2] == 0.f) {
+            ab[0] = (n[0] > 0.f) ? -kInf : -n[3];
+            ab[1] = (n[0] > 0.f) ? n[3] : kInf;
+        } else if (n[0] == 0.f && n[2] == 0.f) {
+            ab[2] = (n[1] > 0.f) ? -kInf : -n[3];
+            ab[3] = (n[1] > 0.f) ? n[3] : kInf;
+        } else if (n[0] == 0.f && n[1] == 0.f) {
+            ab[4] = (n[2] > 0.f) ? -kInf : -n[3];
+            ab[5] = (n[2] > 0.f) ? n[3] : kInf;
+        }
+        break;
+    }
+    default:
+        ab[0] = -kInf; ab[1] = kInf; ab[2] = -kInf; ab[3] = kInf; ab[4] = -kInf; ab[5] = kInf;
+    }
+}
+
+
+// Geom view (cls, body, params[4], pos[3], R[12]) into a DGeom.
+ODX_DEV void load_gview(const float *gv, DGeom &G) {
+    G.cls = (int)gv[0];
+    G.body = (int)gv[1];
+    for (int k = 0; k < 4; k++) G.p[k] = gv[2 + k];
+    for (int k = 0; k < 3; k++) G.pos[k] = gv[6 + k];
+    for (int k = 0; k < 12; k++) G.R[k] = gv[9 + k];
+}
+
 // dCollideSpheres collision_util.cpp:38-67
 ODX_DEV int collide_spheres(const float *p1, float r1, const float *p2, float r2, DContact &c) {
     float dx = p1[0] - p2[0], dy = p1[1] - p2[1], dz = p1[2] - p2[2];

@@ -177,6 +177,25 @@ enum { DAMP_LINEAR = 1, DAMP_ANGULAR = 2, DAMP_MAX_ANGULAR_SPEED = 4 };
 // cls, body identity (-1 none), params[4], pos[3], R[12] (3x4).
 constexpr int GVIEW_STRIDE = 24;
 
+// One dSpaceCollide on the device (k_space_collide, odx_space.hip).
+struct SpaceCall {
+    const float *gviews;             // [n][GVIEW_STRIDE] in the space's order (SAP: the sorted set first)
+    const unsigned long long *bits;  // [n][2] category, collide
+    int n;
+    int nsort;                       // SAP: finite geoms (views 0..nsort-1); hash, simple: n
+    int kind;                        // 0 sweep and prune, 1 hash, 2 simple
+    int ax0, ax1, ax2;               // SAP axes as AABB indices (min = idx, max = idx + 1)
+    int minlevel, maxlevel;          // hash levels
+    const unsigned *ranks_in;        // SAP: ranks kept from the previous sort
+    int ranks_valid;
+    unsigned *ranks_out;
+    float *aabb;                     // [n][6]
+    int *cells;                      // hash: [n][8] level (INT_MAX: big box), discretised bounds [6]
+    int *pairs;                      // [cap][2] view indices (o1, o2)
+    int cap;
+    int *info;                       // [0] pairs; [1] SAP: ranks valid after the call
+};
+
 // Previous-contact record (floats) kept by the solve kernel for matching.
 enum { PV_G1 = 0, PV_G2 = 1, PV_LOCAL = 2, PV_N = 5, PV_LAM = 8 };
 constexpr float MATCH_DIST = 0.05f;   // max body-local distance (m) of a matched contact point

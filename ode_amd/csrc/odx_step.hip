@@ -142,49 +142,6 @@ __device__ void load_geom(const DevWorlds &D, int g, int b0, DGeom &G) {
     }
 }
 
-// computeAABB: sphere.cpp:59-67, box.cpp:60-76, capsule.cpp:60-74, plane.cpp:81-105
-__device__ void geom_aabb(const DGeom &G, float *ab) {
-    const float *p = G.pos, *R = G.R;
-    switch (G.cls) {
-    case CLS_SPHERE: {
-        float r = G.p[0];
-        ab[0] = p[0] - r; ab[1] = p[0] + r; ab[2] = p[1] - r; ab[3] = p[1] + r; ab[4] = p[2] - r; ab[5] = p[2] + r;
-        break;
-    }
-    case CLS_BOX: {
-        const float *s = G.p;
-        float xr = 0.5f * (fabsf(R[0] * s[0]) + fabsf(R[1] * s[1]) + fabsf(R[2] * s[2]));
-        float yr = 0.5f * (fabsf(R[4] * s[0]) + fabsf(R[5] * s[1]) + fabsf(R[6] * s[2]));
-        float zr = 0.5f * (fabsf(R[8] * s[0]) + fabsf(R[9] * s[1]) + fabsf(R[10] * s[2]));
-        ab[0] = p[0] - xr; ab[1] = p[0] + xr; ab[2] = p[1] - yr; ab[3] = p[1] + yr; ab[4] = p[2] - zr; ab[5] = p[2] + zr;
-        break;
-    }
-    case CLS_CAPSULE: {
-        float r = G.p[0], lz = G.p[1];
-        float xr = fabsf(R[2] * lz) * 0.5f + r, yr = fabsf(R[6] * lz) * 0.5f + r, zr = fabsf(R[10] * lz) * 0.5f + r;
-        ab[0] = p[0] - xr; ab[1] = p[0] + xr; ab[2] = p[1] - yr; ab[3] = p[1] + yr; ab[4] = p[2] - zr; ab[5] = p[2] + zr;
-        break;
-    }
-    case CLS_PLANE: {
-        const float *n = G.p;
-        ab[0] = -kInf; ab[1] = kInf; ab[2] = -kInf; ab[3] = kInf; ab[4] = -kInf; ab[5] = kInf;
-        if (n[1] == 0.f && n[2] == 0.f) {
-            ab[0] = (n[0] > 0.f) ? -kInf : -n[3];
-            ab[1] = (n[0] > 0.f) ? n[3] : kInf;
-        } else if (n[0] == 0.f && n[2] == 0.f) {
-            ab[2] = (n[1] > 0.f) ? -kInf : -n[3];
-            ab[3] = (n[1] > 0.f) ? n[3] : kInf;
-        } else if (n[0] == 0.f && n[1] == 0.f) {
-            ab[4] = (n[2] > 0.f) ? -kInf : -n[3];
-            ab[5] = (n[2] > 0.f) ? n[3] : kInf;
-        }
-        break;
-    }
-    default:
-        ab[0] = -kInf; ab[1] = kInf; ab[2] = -kInf; ab[3] = kInf; ab[4] = -kInf; ab[5] = kInf;
-    }
-}
-
 // ------------------------------------------------------------- rows -------
 __device__ __forceinline__ int contact_rows_m(const Surface &s) {
     // getInfo1 contact.cpp:48-123 (negative mu counts as zero)
@@ -1086,6 +1043,7 @@ __global__ __launch_bounds__(64) void k_prepare_small(DevWorlds D, StepParams P)
 
 // ============================================================== kernel C ===
 __device__ __forceinline__ void process_item(const DevWorlds &D, int r0, int item, float *s_fc) {
+#pragma clang fp contract(fast)
     const int rs = item >> 3, nr = item & 7;
     const float4 *row4 = reinterpret_cast<const float4 *>(D.rows + (size_t)(r0 + rs) * ROW_STRIDE);
     const float4 ids = row4[7];
@@ -1137,6 +1095,7 @@ __device__ __forceinline__ void process_item(const DevWorlds &D, int r0, int ite
 // in sweep order, mapped back to rows through inv).
 __device__ __forceinline__ void process_item_packed(const DevWorlds &D, int r0, const int *inv, int nb, int item,
                                                     float *s_fc) {
+#pragma clang fp contract(fast)
     const int rs = item_start(item), nr = item_rows(item);
     const int b1 = item_b1(item), b2 = item_b2(item) == nb ? -1 : item_b2(item);
     float f1[6], f2[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -1264,6 +1223,7 @@ __device__ __forceinline__ void lds_store_fc(float *s_body, int b, const float *
 __device__ __forceinline__ void row_update(const float4 a0, const float4 a1, const float4 a2, const float4 a3, int p,
                                            float *s_lam, float *f1, float *f2, const float *I1, const float *I2,
                                            float im1, float im2) {
+#pragma clang fp contract(fast)
     const int fidx = (as_int(a3.z) >> 16) - 1;
     const float old = s_lam[p];
     const float lf = s_lam[fidx >= 0 ? fidx : 0];
@@ -1289,6 +1249,7 @@ __device__ __forceinline__ void row_update(const float4 a0, const float4 a1, con
 }
 
 __device__ __forceinline__ void process_item_lds(const float *s_rows, float *s_lam, float *s_body, int nb, int item) {
+#pragma clang fp contract(fast)
     const int rs = item_start(item), nr = item_rows(item);
     const int b1 = item_b1(item), b2 = item_b2(item);
     float f1[6], f2[6], I1[9], I2[9], im1, im2;
@@ -1344,6 +1305,7 @@ __device__ float world_residual_lds(const float *s_rows, const float *s_lam, con
 
 // fc += lambda iMJ for the rows' current lambda (warm start), LDS atomics.
 __device__ void lds_accumulate_fc(const float *s_rows, const float *s_lam, float *s_body, int nb, int m) {
+#pragma clang fp contract(fast)
     for (int r = lane_id(); r < m; r += 64) {
         const float l = s_lam[r];
         if (l == 0.f) continue;
@@ -1781,6 +1743,7 @@ __device__ float block_max(float v, float *s_red) {
 __device__ __forceinline__ void process_item_l(const float4 *rows4, const float4 *pre, int item, float *lam,
                                                float *s_fc, float *fa = nullptr, const int *isl_b = nullptr,
                                                const int *ist = nullptr) {
+#pragma clang fp contract(fast)
     const int rs = item >> 3, nr = item & 7;
     const float4 *row4 = rows4 + (size_t)rs * (ROW_STRIDE / 4);
     float4 a[8];
@@ -2094,45 +2057,6 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
 }
 
 // -------------------------------------------- collision entry points -------
-__device__ void load_gview(const float *gv, DGeom &G) {
-    G.cls = (int)gv[0];
-    G.body = (int)gv[1];
-    for (int k = 0; k < 4; k++) G.p[k] = gv[2 + k];
-    for (int k = 0; k < 3; k++) G.pos[k] = gv[6 + k];
-    for (int k = 0; k < 12; k++) G.R[k] = gv[9 + k];
-}
-
-// All candidate pairs of n geoms (lower index first).  space 0: infinite
-// AABBs pair with everything untested (dxSAPSpace::collide,
-// collision_sapspace.cpp:479-532); space 1: every pair AABB-tested
-// (dxHashSpace, collision_space.cpp:421-614); space 2 (simple space): every
-// pair AABB-tested.  Geoms with body identity b share no pair with geoms of
-// the same non-negative b; category/collide bits filter as
-// collideAABBs (collision_space_internal.h:44-78).  Appends to pairs via an
-// atomic counter; the caller orders them.
-__global__ void k_pairs_all(const float *gviews, const float *aabb, const unsigned *bits, int n, int space,
-                            int *pairs, int *count, int cap) {
-    const long long T = (long long)n * (n - 1) / 2;
-    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < T; k += (long long)gridDim.x * blockDim.x) {
-        const double fn = 2.0 * n - 1.0;
-        int i = (int)floor((fn - sqrt(fn * fn - 8.0 * (double)k)) * 0.5);
-        if (i < 0) i = 0;
-        while (i > 0 && (long long)i * (2 * n - i - 1) / 2 > k) i--;
-        while ((long long)(i + 1) * (2 * n - i - 2) / 2 <= k) i++;
-        const int j = (int)(k - (long long)i * (2 * n - i - 1) / 2) + i + 1;
-        const int bi = (int)gviews[(size_t)i * GVIEW_STRIDE + 1], bj = (int)gviews[(size_t)j * GVIEW_STRIDE + 1];
-        if (bi == bj && bi >= 0) continue;
-        const unsigned ci = bits[2 * i], li = bits[2 * i + 1], cj = bits[2 * j], lj = bits[2 * j + 1];
-        if (!((ci & lj) || (cj & li))) continue;
-        const float *x = aabb + (size_t)i * 6, *y = aabb + (size_t)j * 6;
-        bool ok;
-        if (space == 0 && (x[1] == kInf || y[1] == kInf)) ok = true;
-        else ok = !(x[0] > y[1] || x[1] < y[0] || x[2] > y[3] || x[3] < y[2] || x[4] > y[5] || x[5] < y[4]);
-        if (!ok) continue;
-        const int slot = atomicAdd(count, 1);
-        if (slot < cap) { pairs[2 * slot] = i; pairs[2 * slot + 1] = j; }
-    }
-}
 
 // AABBs of geom views (computeAABB of each class).
 __global__ void k_aabbs(const float *gviews, int n, float *aabb) {
@@ -2240,15 +2164,6 @@ void launch_single_step(const DevWorlds &D, const StepParams &P0, int nb, int m,
 }
 void launch_aabbs(const float *gviews, int n, float *aabb, hipStream_t s) {
     hipLaunchKernelGGL(k_aabbs, dim3((n + 255) / 256), dim3(256), 0, s, gviews, n, aabb);
-}
-void launch_pairs_all(const float *gviews, const float *aabb, const unsigned *bits, int n, int space, int *pairs,
-                      int *count, int cap, hipStream_t s) {
-    const long long T = (long long)n * (n - 1) / 2;
-    long long blocks = (T + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_pairs_all, dim3((unsigned)blocks), dim3(256), 0, s, gviews, aabb, bits, n, space, pairs, count,
-                       cap);
 }
 void launch_collide_pairs(const float *gviews, const int *pairs, int np, int maxc, float *out, int *counts,
                           hipStream_t s) {
@@ -2775,6 +2690,7 @@ __device__ __forceinline__ void lg_fc_store(__amdgpu_buffer_rsrc_t rs, int b, co
 // One row update (IterationStep quickstep.cpp:2917-3033) on a row held in
 // registers; lf = lambda of the row's findex row (unused without findex).
 __device__ __forceinline__ float lg_row_update(const float4 *a, float old, float lf, float *f1, float *f2, bool two) {
+#pragma clang fp contract(fast)
     float delta = a[1].z - old * a[1].w;
     delta -= f1[0] * a[0].x + f1[1] * a[0].y + f1[2] * a[0].z + f1[3] * a[0].w + f1[4] * a[1].x + f1[5] * a[1].y;
     if (two) delta -= f2[0] * a[2].x + f2[1] * a[2].y + f2[2] * a[2].z + f2[3] * a[2].w + f2[4] * a[3].x + f2[5] * a[3].y;

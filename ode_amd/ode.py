@@ -43,6 +43,7 @@ dContactApprox1 = 0x7000
 dSphereClass, dBoxClass, dCapsuleClass, dPlaneClass = 0, 1, 2, 4
 dJointTypeBall, dJointTypeContact = 1, 4
 dSAP_AXES_XYZ = (0) | (1 << 2) | (2 << 4)
+dSAP_AXES_ZYX = (2) | (1 << 2) | (0 << 4)
 
 _api = None
 
@@ -85,6 +86,10 @@ def api():
         "dSimpleSpaceCreate": (vp, [vp]), "dHashSpaceCreate": (vp, [vp]),
         "dSweepAndPruneSpaceCreate": (vp, [vp, i]), "dSpaceDestroy": (None, [vp]),
         "dHashSpaceSetLevels": (None, [vp, i, i]), "dSpaceCollide": (None, [vp, vp, NearCallback]),
+        "dSpaceAdd": (None, [vp, vp]), "dSpaceRemove": (None, [vp, vp]), "dSpaceGetNumGeoms": (i, [vp]),
+        "dGeomGetAABB": (None, [vp, P(dReal)]), "dGeomEnable": (None, [vp]), "dGeomDisable": (None, [vp]),
+        "dGeomSetCategoryBits": (None, [vp, ctypes.c_ulong]), "dGeomSetCollideBits": (None, [vp, ctypes.c_ulong]),
+        "dGeomSetQuaternion": (None, [vp, P(dReal)]),
         "dCollide": (i, [vp, vp, i, P(dContactGeom), i]),
         "dJointGroupCreate": (vp, [i]), "dJointGroupDestroy": (None, [vp]), "dJointGroupEmpty": (None, [vp]),
         "dJointCreateContact": (vp, [vp, vp, P(dContact)]), "dJointCreateBall": (vp, [vp, vp]),

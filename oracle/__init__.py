@@ -262,13 +262,14 @@ def solve_to_residual(bodies, joints, prm, target, maxit, balls=None):
 
 class SpaceModel:
     """The callback order of one ODE space across calls (oracle/spaces.cpp):
-    kind "sap" (axisorder as dSweepAndPruneSpaceCreate), "hash" (levels) or
+    kind "sap" (axisorder as dSweepAndPruneSpaceCreate, dSAP_AXES_XYZ by
+    default), "hash" (levels) or
     "simple".  Geoms are integer ids; add/remove/moved mirror dSpaceAdd,
     dSpaceRemove and dGeomMoved."""
 
     KINDS = {"sap": 0, "hash": 1, "simple": 2}
 
-    def __init__(self, kind, axisorder=0, levels=(-3, 10)):
+    def __init__(self, kind, axisorder=(0) | (1 << 2) | (2 << 4), levels=(-3, 10)):
         self._lib = lib()
         self._lib.orc_space_create.restype = ctypes.c_void_p
         self.h = ctypes.c_void_p(self._lib.orc_space_create(ctypes.c_int(self.KINDS[kind]), ctypes.c_int(axisorder),

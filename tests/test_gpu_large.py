@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 TOL_STATE = 1e-4     # positions, quaternions, velocities after one step (relative)
 TOL_LAMBDA = 1e-4    # impulses after one step, float32 GPU vs float64 oracle (relative to max |lambda|)
 TOL_CONTACT_F64 = 1e-4  # contact positions / normals / depths, float32 GPU vs the float64 oracle
-AABB_EPS = 1e-5      # pairs whose AABB overlap margin is below this may differ (float32 rounding of R)
+TOL_CONTACT_F32 = 2e-5  # the same against the float32 oracle (same arithmetic; cullPoints' atan2 may differ)
 
 
 def say(*a):
@@ -42,12 +42,6 @@ def make(world, prm, **kw):
     return odx.Large(world, prm, **kw)
 
 
-def overlap_margin(ab, i, j):
-    x, y = ab[i], ab[j]
-    return min(min(x[1], y[1]) - max(x[0], y[0]), min(x[3], y[3]) - max(x[2], y[2]),
-               min(x[5], y[5]) - max(x[4], y[4]))
-
-
 def colour_order(colours):
     return np.argsort(colours, kind="stable")
 
@@ -64,42 +58,49 @@ def test_pairs_contacts_and_coloured_step_match_oracle(seed):
     assert cnt["pair_overflow"] == 0 and cnt["contact_overflow"] == 0 and cnt["uncoloured"] == 0
     assert cnt["barrier_error"] == 0
 
-    # broadphase: the hash-space pair set (pairs of two static geoms make no rows and are left out)
+    # broadphase: the hash-space pair set, exactly (pairs of two static geoms
+    # make no rows and are left out); AABBs and poses are computed without
+    # fused multiply-add on both sides, so they agree bit for bit
     body = w.geoms[:, 1].astype(np.int32)
-    ab = O.aabbs(w.geoms, w.bodies, dtype=np.float32).astype(np.float64)
-    ref = O.pairs(ab.astype(np.float32), body, space="hash")
+    ab = O.aabbs(w.geoms, w.bodies, dtype=np.float32)
+    ref = O.pairs(ab, body, space="hash")
     ref = {(int(a), int(b)) for a, b in ref if body[a] >= 0 or body[b] >= 0}
     got = [tuple(map(int, p)) for p in L.pairs()]
     assert len(got) == len(set(got)), "duplicate pairs"
     got = set(got)
     for a, b in got:
         assert a < b
-    diff = got ^ ref
-    assert all(abs(overlap_margin(ab, a, b)) < AABB_EPS for a, b in diff), sorted(diff)[:5]
-    assert len(diff) <= max(2, len(ref) // 1000)
+    assert got == ref, sorted(got ^ ref)[:5]
 
-    # narrowphase: the same contacts per pair (oracle contacts in float64)
+    # narrowphase: per pair the float32 oracle's contact count, exactly, and
+    # its geometry; the float64 oracle's geometry within the stated tolerance
+    _, ref32, _ = O.world_step(w.bodies, w.geoms, prm, dtype=np.float32)
     ref_b, ref_c, _ = O.world_step(w.bodies, w.geoms, prm, dtype=np.float64)
     gj = L.joints()
-    assert len(gj) == cnt["contacts"]
+    assert len(gj) == cnt["contacts"] == len(ref32)
     key = lambda g1, g2: (int(g1), int(g2))
-    by_pair = {}
-    for r in ref_c:
-        by_pair.setdefault(key(r[7], r[8]), []).append(r)
     gpu_pair = {}
     for r in gj:
         gpu_pair.setdefault(key(r[14], r[15]), []).append(r)
-    assert set(by_pair) == set(gpu_pair)
-    err = np.zeros(3)
-    for k, rs in by_pair.items():
-        gs = gpu_pair[k]
-        assert len(gs) == len(rs), k
-        for g, r in zip(gs, rs):
-            sg = -1.0 if body[k[0]] < 0 else 1.0   # dJointAttach reversal negates the normal
-            err = np.maximum(err, [np.max(np.abs(g[3:6] - r[0:3])) / max(1.0, np.max(np.abs(r[0:3]))),
-                                   np.max(np.abs(g[6:9] - sg * r[3:6])), abs(g[9] - r[6])])
-    say("contact errors (pos, normal, depth) vs float64:", err)
-    assert np.all(err < TOL_CONTACT_F64), err
+    for tol, refc in ((TOL_CONTACT_F32, ref32), (TOL_CONTACT_F64, ref_c)):
+        by_pair = {}
+        for r in refc:
+            by_pair.setdefault(key(r[7], r[8]), []).append(r)
+        if refc is ref32:
+            assert set(by_pair) == set(gpu_pair)
+        err = np.zeros(3)
+        for k, rs in by_pair.items():
+            gs = gpu_pair.get(k, [])
+            if refc is ref32:
+                assert len(gs) == len(rs), k
+            elif len(gs) != len(rs):
+                continue  # a float64 borderline count; the float32 counts are exact above
+            for g, r in zip(gs, rs):
+                sg = -1.0 if body[k[0]] < 0 else 1.0   # dJointAttach reversal negates the normal
+                err = np.maximum(err, [np.max(np.abs(g[3:6] - r[0:3])) / max(1.0, np.max(np.abs(r[0:3]))),
+                                       np.max(np.abs(g[6:9] - sg * r[3:6])), abs(g[9] - r[6])])
+        say("contact errors (pos, normal, depth) vs", "float32" if refc is ref32 else "float64", err)
+        assert np.all(err < tol), err
 
     # colouring: every contact coloured, no dynamic body twice in one colour
     col = L.colours()

@@ -22,7 +22,7 @@ def say(*a):
 class Frame:
     """Builds a world through the C API and records what the callback did."""
 
-    def __init__(self, world_scene, space="sap", mu=0.5, iters=20, dynamic=False):
+    def __init__(self, world_scene, space="sap", mu=0.5, iters=20, dynamic=False, levels=None):
         from ode_amd import ode as A
         self.A = A
         L = A.api()
@@ -37,6 +37,11 @@ class Frame:
         self.space = {"sap": lambda: L.dSweepAndPruneSpaceCreate(None, A.dSAP_AXES_XYZ),
                       "hash": lambda: L.dHashSpaceCreate(None),
                       "simple": lambda: L.dSimpleSpaceCreate(None)}[space]()
+        if levels is not None:
+            L.dHashSpaceSetLevels(self.space, *levels)
+        # the oracle's model of the space's lists, kept in step with the library
+        self.model = O.SpaceModel(space, levels=levels or (-3, 10))
+        self.space_kind = space
         self.group = L.dJointGroupCreate(0)
         self.mu = mu
         self.scene = world_scene
@@ -63,11 +68,17 @@ class Frame:
                 h = L.dCreateCapsule(self.space, float(g[2]), float(g[3]))
             else:
                 h = L.dCreatePlane(self.space, float(g[2]), float(g[3]), float(g[4]), float(g[5]))
+            self.model.add(gi)
             if int(g[1]) >= 0:
                 L.dGeomSetBody(h, self.bodies[int(g[1])])
             self.geoms.append(h)
             self.geom_index[h] = gi
         self.body_index = {b: i for i, b in enumerate(self.bodies)}
+        # each body's geoms, most recently attached first (the body's geom list)
+        self.body_geoms = {i: [] for i in range(len(self.bodies))}
+        for gi, g in enumerate(world_scene.geoms):
+            if int(g[1]) >= 0:
+                self.body_geoms[int(g[1])].insert(0, gi)
 
     def collide(self):
         A, L = self.A, self.L
@@ -92,6 +103,23 @@ class Frame:
 
         self._cb = A.NearCallback(near)
         L.dSpaceCollide(self.space, None, self._cb)
+
+    def expected_pairs(self, enabled=None, cat=None, col=None):
+        """The oracle's callback sequence for the library's current state."""
+        ab = O.aabbs(self.scene.geoms, self.state(), np.float32)
+        body = self.scene.geoms[:, 1].astype(np.int32)
+        return self.model.collide(ab, body, cat=cat, col=col, enabled=enabled)
+
+    def step(self, h=0.01):
+        """dWorldQuickStep; the oracle's model sees the geoms of the stepped
+        bodies move in stepping order (dxStepBody's dGeomMoved)."""
+        norm = [(c[0], c[1]) if c[0] >= 0 else (c[1], -1) for c in self.contacts]
+        bodies, _ = island_order(len(self.bodies), norm)
+        rc = self.L.dWorldQuickStep(self.w, h)
+        for b in bodies:
+            for gi in self.body_geoms[b]:
+                self.model.moved(gi)
+        return rc
 
     def state(self):
         A, L = self.A, self.L
@@ -118,57 +146,116 @@ def settle_scene(seed, n=14):
     return w
 
 
-@pytest.mark.parametrize("space", ["sap", "hash"])
-def test_space_collide_pairs_and_contacts_match_oracle(space):
-    say("case: dSpaceCollide", space)
-    scene = settle_scene(3, 20)
-    F = Frame(scene, space=space)
-    F.collide()
-    ab = O.aabbs(scene.geoms, scene.bodies, np.float32)
-    body = scene.geoms[:, 1].astype(np.int32)
-    want = {tuple(sorted(p)) for p in O.pairs(ab, body, space)}
-    got = {tuple(sorted(p)) for p in F.pairs}
-    assert got == want
-    assert len(F.pairs) == len(want)
-    # every callback's dCollide equals the oracle's dCollide of that ordered pair
+def check_contacts(F, exact_counts=True):
+    """Every callback's dCollide against the float32 oracle's dCollide of the
+    same ordered pair in the same state: the same count, the same geometry."""
+    st = F.state()
     by_pair = {}
     for c in F.contacts:
         by_pair.setdefault((c[5], c[6]), []).append(c)
+    worst = 0.0
     for g1, g2 in F.pairs:
-        bi, bj = int(scene.geoms[g1, 1]), int(scene.geoms[g2, 1])
-        ref = O.collide(scene.geoms[g1], scene.geoms[g2], scene.bodies[bi] if bi >= 0 else None,
-                        scene.bodies[bj] if bj >= 0 else None, maxc=4)
+        bi, bj = int(F.scene.geoms[g1, 1]), int(F.scene.geoms[g2, 1])
+        ref = O.collide(F.scene.geoms[g1], F.scene.geoms[g2], st[bi] if bi >= 0 else None,
+                        st[bj] if bj >= 0 else None, maxc=4, dtype=np.float32)
         got_c = by_pair.get((g1, g2), [])
-        assert len(got_c) == len(ref)
+        assert len(got_c) == len(ref), (g1, g2, len(got_c), len(ref))
         for gc, rc in zip(got_c, ref):
-            np.testing.assert_allclose(gc[2], rc[0:3], atol=2e-5)
-            np.testing.assert_allclose(gc[3], rc[3:6], atol=2e-5)
-            assert abs(gc[4] - rc[6]) < 2e-5
+            worst = max(worst, float(np.max(np.abs(np.array(gc[2] + gc[3] + [gc[4]]) - rc[0:7]))))
+    assert worst < 2e-5, worst
+    return worst
+
+
+@pytest.mark.parametrize("space", ["sap", "hash", "simple"])
+def test_space_collide_callback_order_matches_oracle_over_a_frame_loop(space):
+    say("case: dSpaceCollide callback order", space)
+    scene = settle_scene(3, 20)
+    F = Frame(scene, space=space)
+    for step in range(40):
+        F.collide()
+        want = F.expected_pairs()
+        assert F.pairs == want, (step, F.pairs[:8], want[:8])
+        if step % 10 == 0:
+            check_contacts(F)
+        assert F.step() == 1
+        F.L.dJointGroupEmpty(F.group)
     F.close()
 
 
-@pytest.mark.parametrize("dynamic", [False, True])
-@pytest.mark.parametrize("seed", [1, 2])
-def test_quickstep_matches_oracle_in_island_order(seed, dynamic):
-    say("case: dWorldQuickStep seed", seed, "dynamic iterations", dynamic)
-    scene = settle_scene(seed)
-    F = Frame(scene, dynamic=dynamic)
+def test_space_collide_hash_levels_and_big_boxes():
+    say("case: dHashSpace levels (-1, 0)")
+    # with maxlevel 0 the larger boxes go to the big-box list
+    scene = settle_scene(5, 24)
+    F = Frame(scene, space="hash", levels=(-1, 0))
+    for step in range(10):
+        F.collide()
+        assert F.pairs == F.expected_pairs(), step
+        assert F.step() == 1
+        F.L.dJointGroupEmpty(F.group)
+    F.close()
+
+
+def test_space_collide_enable_remove_bits_and_moves():
+    say("case: dSpaceCollide with disabled, removed and filtered geoms")
+    scene = settle_scene(6, 16)
+    F = Frame(scene, space="sap")
+    L, A = F.L, F.A
+    ng = len(scene.geoms)
+    enabled = np.ones(ng, np.uint8)
+    cat = np.full(ng, np.iinfo(np.uint64).max, np.uint64)
+    col = np.full(ng, np.iinfo(np.uint64).max, np.uint64)
     F.collide()
-    before = scene.bodies.copy()
-    assert F.L.dWorldQuickStep(F.w, 0.01) == 1
-    got = F.state()
-    sweeps, res, rows = ctypes.c_int(0), ctypes.c_float(0), ctypes.c_int(0)
-    F.L.odxWorldGetSolveStats(F.w, ctypes.byref(sweeps), ctypes.byref(res), ctypes.byref(rows))
-    joints = [(c[0], c[1]) for c in F.contacts]
-    # dJointAttach with a world geom first: body1 NULL -> reversed joint
+    assert F.pairs == F.expected_pairs()
+    # a disabled geom stays in the lists and is skipped
+    L.dGeomDisable(F.geoms[2])
+    enabled[2] = 0
+    # 64-bit category bits: geom 3 only meets geoms with collide bit 40
+    cat[3] = np.uint64(1) << np.uint64(40)
+    col[3] = np.uint64(1) << np.uint64(40)
+    L.dGeomSetCategoryBits(F.geoms[3], int(cat[3]))
+    L.dGeomSetCollideBits(F.geoms[3], int(col[3]))
+    col[5] = np.uint64(1) << np.uint64(40) | np.uint64(1)
+    L.dGeomSetCollideBits(F.geoms[5], int(col[5]))
+    # bodies moved by the API, in this order
+    for b in (4, 1, 4):
+        p = A.vec(L.dBodyGetPosition(F.bodies[b]))
+        L.dBodySetPosition(F.bodies[b], p[0] + 0.01, p[1], p[2])
+        for gi in F.body_geoms[b]:
+            F.model.moved(gi)
+    F.collide()
+    assert F.pairs == F.expected_pairs(enabled=enabled, cat=cat, col=col)
+    # removed and re-added: appended to the dirty list again
+    L.dSpaceRemove(F.space, F.geoms[0])
+    F.model.remove(0)
+    L.dSpaceAdd(F.space, F.geoms[0])
+    F.model.add(0)
+    F.collide()
+    assert F.pairs == F.expected_pairs(enabled=enabled, cat=cat, col=col)
+    assert F.step() == 1
+    F.L.dJointGroupEmpty(F.group)
+    F.collide()
+    assert F.pairs == F.expected_pairs(enabled=enabled, cat=cat, col=col)
+    F.close()
+
+
+def quickstep_parity(F, dynamic=False):
+    """One dWorldQuickStep against the float64 oracle's QuickStep of the same
+    contacts in the reference's island order; returns (err_x, err_v, sweeps,
+    oracle float32 sweeps)."""
+    before = F.state()
     recs = []
     for (i1, i2, pos, nrm, depth, g1, g2) in F.contacts:
         r = np.zeros(16)
         r[1], r[2] = i1, i2
         r[3:6], r[6:9], r[9] = pos, nrm, depth
         recs.append(r)
-    norm_pairs = [(a, b) if a >= 0 else (b, -1) for a, b in joints]
-    _, order = island_order(len(scene.bodies), norm_pairs)
+    # dJointAttach with a world geom first: body1 NULL -> reversed joint
+    norm_pairs = [(c[0], c[1]) if c[0] >= 0 else (c[1], -1) for c in F.contacts]
+    _, order = island_order(len(F.bodies), norm_pairs)
+    assert F.step() == 1
+    got = F.state()
+    sweeps, res, rows = ctypes.c_int(0), ctypes.c_float(0), ctypes.c_int(0)
+    F.L.odxWorldGetSolveStats(F.w, ctypes.byref(sweeps), ctypes.byref(res), ctypes.byref(rows))
     # dWorldCreate's dynamic-iteration defaults (include/ode/objects.h:473-475)
     prm = O.params(**(dict(dyn_premature=1e-8, dyn_extra_factor=1.0, dyn_extra_delta=1e-2) if dynamic else {}))
     jr = np.array([recs[k] for k in order]).reshape(-1, 16)
@@ -178,14 +265,56 @@ def test_quickstep_matches_oracle_in_island_order(seed, dynamic):
     # the sweep count (largest over islands) against the float32 oracle: float32
     # adjustments stall near rounding level, as in a dSINGLE build
     O.quickstep(before, jr, prm, dtype=np.float32)
-    want = O.last_sweeps()
-    say("  contacts", len(recs), "err_x", err_x, "err_v", err_v, "sweeps", sweeps.value, "oracle f32", want)
+    return err_x, err_v, sweeps.value, O.last_sweeps(), len(recs)
+
+
+@pytest.mark.parametrize("dynamic", [False, True])
+@pytest.mark.parametrize("seed", [1, 2])
+def test_quickstep_matches_oracle_in_island_order(seed, dynamic):
+    say("case: dWorldQuickStep seed", seed, "dynamic iterations", dynamic)
+    scene = settle_scene(seed)
+    F = Frame(scene, dynamic=dynamic)
+    F.collide()
+    assert F.pairs == F.expected_pairs()
+    err_x, err_v, sweeps, want, nc = quickstep_parity(F, dynamic)
+    say("  contacts", nc, "err_x", err_x, "err_v", err_v, "sweeps", sweeps, "oracle f32", want)
     assert err_x < 1e-4 and err_v < 1e-4
     if dynamic:
-        assert abs(sweeps.value - want) <= 1, (sweeps.value, want)
+        assert abs(sweeps - want) <= 1, (sweeps, want)
     else:
-        assert sweeps.value == 20
+        assert sweeps == 20
     F.L.dJointGroupEmpty(F.group)
+    F.close()
+
+
+def test_boxes_1000_frame_loop_matches_oracle():
+    """The 1,000-box configuration through the drop-in API: sweep-and-prune
+    space, near callback with dCollide (4 contacts, mu 0.5, Approx1) and
+    dJointCreateContact, dWorldQuickStep with 20 iterations (fixed count),
+    dJointGroupEmpty, 300 steps.  Every step's callback sequence equals the
+    oracle's; every 30th step each callback's contacts equal the float32
+    oracle's; the last step matches the float64 oracle's QuickStep."""
+    say("case: 1000 boxes, 300 steps")
+    scene = S.boxes_on_plane(1000, seed=1)
+    F = Frame(scene, space="sap", iters=20)
+    ncb = []
+    for step in range(300):
+        F.collide()
+        want = F.expected_pairs()
+        assert F.pairs == want, (step, len(F.pairs), len(want))
+        ncb.append(len(F.pairs))
+        if step % 30 == 29:
+            worst = check_contacts(F)
+            say("  step", step, "callbacks", len(F.pairs), "contacts", len(F.contacts), "worst", worst)
+        if step == 299:
+            err_x, err_v, sweeps, _, nc = quickstep_parity(F)
+            say("  last step: contacts", nc, "err_x", err_x, "err_v", err_v, "sweeps", sweeps)
+            assert nc > 1000
+            assert err_x < 1e-4 and err_v < 1e-4
+            assert sweeps == 20
+        else:
+            assert F.step() == 1
+        F.L.dJointGroupEmpty(F.group)
     F.close()
 
 

@@ -542,3 +542,268 @@ def test_random_reordering_with_dynamic_count_draws_for_the_sweeps_run():
     n = O.last_sweeps()
     assert 1 <= n <= 40
     assert O.rand_seed() == _lcg_advance(99, (n - 1) // 8 * (3 * len(pairs) - 1))
+
+
+# ---------------------------------------------- contact selection and order
+def qmul(a, b):
+    w1, x1, y1, z1 = a
+    w2, x2, y2, z2 = b
+    return np.array([w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2, w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                     w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2, w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2])
+
+
+def clip_polygon(poly, hx, hy):
+    """Sutherland-Hodgman clip of a 2-D polygon to |x| <= hx, |y| <= hy."""
+    for axis, sign, h in ((0, -1, hx), (0, 1, hx), (1, -1, hy), (1, 1, hy)):
+        out = []
+        for i in range(len(poly)):
+            a, b = poly[i], poly[(i + 1) % len(poly)]
+            ina, inb = sign * a[axis] <= h, sign * b[axis] <= h
+            if ina:
+                out.append(a)
+            if ina != inb:
+                t = (sign * h - a[axis]) / (b[axis] - a[axis])
+                out.append(a + t * (b - a))
+        poly = out
+    return poly
+
+
+def box_box_expected(side1, p1, R1, side2, p2, R2, maxc):
+    """Face-face dBoxBox contacts from geometry: the face axis of least
+    penetration (box 1's faces tested first), the incident face of the other
+    box clipped to the reference face, the points on the incident face with
+    their depth below the reference face, and, when there are more than maxc,
+    the deepest point first and then the points whose angle about the clipped
+    polygon's centroid is closest to its angle + j*2pi/maxc (cullPoints)."""
+    h1, h2 = np.asarray(side1) / 2, np.asarray(side2) / 2
+    d = np.asarray(p2) - np.asarray(p1)
+    best = None
+    for code, (R, k) in enumerate([(R1, k) for k in range(3)] + [(R2, k) for k in range(3)], start=1):
+        u = R[:, k]
+        s = abs(d @ u) - (h1 @ np.abs(R1.T @ u) + h2 @ np.abs(R2.T @ u))
+        if best is None or s > best[0]:
+            best = (s, code, u * np.sign(d @ u))
+    _, code, normal = best  # normal: from box 1 towards box 2
+    if code <= 3:
+        pa, Ra, Sa, pb, Rb, Sb, normal2 = p1, R1, h1, p2, R2, h2, normal
+    else:
+        pa, Ra, Sa, pb, Rb, Sb, normal2 = p2, R2, h2, p1, R1, h1, -normal
+    codeN = (code - 1) % 3
+    c1, c2 = [k for k in range(3) if k != codeN]
+    nr = Rb.T @ normal2
+    lanr = int(np.argmax(np.abs(nr)))
+    a1, a2 = [k for k in range(3) if k != lanr]
+    center = np.asarray(pb) - np.sign(nr[lanr]) * Sb[lanr] * Rb[:, lanr] - np.asarray(pa)
+    corners = [center + sx * Sb[a1] * Rb[:, a1] + sy * Sb[a2] * Rb[:, a2] for sx, sy in ((-1, -1), (-1, 1), (1, 1), (1, -1))]
+    u1, u2 = Ra[:, c1], Ra[:, c2]
+    poly = clip_polygon([np.array([c @ u1, c @ u2]) for c in corners], Sa[c1], Sa[c2])
+    M = np.array([[u1 @ Rb[:, a1], u1 @ Rb[:, a2]], [u2 @ Rb[:, a1], u2 @ Rb[:, a2]]])
+    pts, dep = [], []
+    for v in poly:
+        k = np.linalg.solve(M, v - np.array([center @ u1, center @ u2]))
+        pt = center + k[0] * Rb[:, a1] + k[1] * Rb[:, a2]
+        pts.append(pt + pa)
+        dep.append(Sa[codeN] - normal2 @ pt)
+    dep = np.array(dep)
+    assert np.all(dep > 0)
+    if len(poly) <= maxc:
+        return np.array(pts), dep, poly
+    x = np.array([v[0] for v in poly])
+    y = np.array([v[1] for v in poly])
+    q = x * np.roll(y, -1) - np.roll(x, -1) * y
+    cx = np.sum(q * (x + np.roll(x, -1))) / (3 * np.sum(q))
+    cy = np.sum(q * (y + np.roll(y, -1))) / (3 * np.sum(q))
+    ang = np.arctan2(y - cy, x - cx)
+    i0 = int(np.argmax(dep))
+    pick, avail = [i0], np.ones(len(poly), bool)
+    avail[i0] = False
+    for j in range(1, maxc):
+        a = j * 2 * math.pi / maxc + ang[i0]
+        if a > math.pi:
+            a -= 2 * math.pi
+        diff = np.abs(ang - a)
+        diff = np.where(diff > math.pi, 2 * math.pi - diff, diff)
+        diff[~avail] = np.inf
+        i = int(np.argmin(diff))
+        pick.append(i)
+        avail[i] = False
+    return np.array(pts)[pick], dep[pick], poly
+
+
+@pytest.mark.parametrize("tilt,offset", [(0.05, (0.04, -0.03)), (-0.07, (-0.05, 0.02)), (0.03, (0.0, 0.06))])
+def test_box_box_cull_points_on_an_octagonal_clip(tilt, offset):
+    # box 2 yawed 45 degrees and slightly tilted: its face clipped to box 1's
+    # top face is an octagon whose eight corners all penetrate, at distinct
+    # depths; four of them are kept
+    g1 = S.geom(S.BOX, 0, (1, 1, 1))
+    g2 = S.geom(S.BOX, 1, (1, 1, 1))
+    q2 = qmul(rotq((0, 0, 1), math.pi / 4), rotq((1, 0, 0), tilt))
+    p2 = np.array([offset[0], offset[1], 0.95])
+    c = O.collide(g1, g2, body((0, 0, 0)), body(p2, q2), maxc=4)
+    pts, dep, poly = box_box_expected((1, 1, 1), np.zeros(3), np.eye(3), (1, 1, 1), p2, rotm(q2), 4)
+    assert len(poly) == 8
+    assert len(c) == 4
+    np.testing.assert_allclose(c[:, 0:3], pts, atol=1e-9)
+    np.testing.assert_allclose(c[:, 6], dep, atol=1e-9)
+    # all contacts share the face normal, pointing into box 1
+    for row in c:
+        assert abs(abs(row[5]) - 1.0) < 0.01 and row[5] < 0
+    # with room for all eight, every clipped corner comes back
+    all_pts, all_dep, _ = box_box_expected((1, 1, 1), np.zeros(3), np.eye(3), (1, 1, 1), p2, rotm(q2), 8)
+    c8 = O.collide(g1, g2, body((0, 0, 0)), body(p2, q2), maxc=8)
+    assert len(c8) == 8
+    key = lambda a: sorted(map(tuple, np.round(a, 9)))
+    assert key(c8[:, 0:3]) == key(all_pts)
+
+
+def test_box_plane_point_order_follows_vertex_depths():
+    # a generic orientation: the deepest vertex first, then its neighbours
+    # along the edges of smallest and second smallest projected length (the
+    # next two deepest vertices), then the fourth corner of their face
+    side = np.array([1.0, 0.6, 0.4])
+    g = S.geom(S.BOX, 0, tuple(side))
+    q = rotq((0.3, 0.8, 0.2), 0.5)
+    R = rotm(q)
+    local = np.array([[sx, sy, sz] for sx in (-.5, .5) for sy in (-.5, .5) for sz in (-.5, .5)]) * side
+    hz = np.sort((local @ R.T)[:, 2])  # vertex heights over the centre, lowest first
+    # centre heights giving 4 (face corner below), 3, 2 and 1 contacts
+    cases = ((hz[0] - hz[1] - hz[2] - 0.01, 4), (0.5 * ((hz[0] - hz[1] - hz[2]) - hz[2]), 3),
+             (-0.5 * (hz[1] + hz[2]), 2), (-0.5 * (hz[0] + hz[1]), 1))
+    for zc, n_expect in cases:
+        verts = local @ R.T + (0, 0, zc)
+        depth = -verts[:, 2]
+        order = np.argsort(-depth)
+        c = O.collide(g, S.plane(0, 0, 1, 0), body((0, 0, zc), q), maxc=4)
+        v0, v1, v2 = verts[order[0]], verts[order[1]], verts[order[2]]
+        d0, d1, d2 = depth[order[0]], depth[order[1]], depth[order[2]]
+        want = [(v0, d0)]
+        if d1 >= 0:
+            want.append((v1, d1))
+            if d2 >= 0:
+                want.append((v2, d2))
+                if d1 + d2 - d0 > 0:
+                    want.append((v1 + v2 - v0, d1 + d2 - d0))
+        assert len(c) == len(want) == n_expect, (zc, len(c), len(want))
+        for row, (v, d) in zip(c, want):
+            np.testing.assert_allclose(row[0:3], v, atol=1e-9)
+            assert row[6] == pytest.approx(d, abs=1e-9)
+            np.testing.assert_allclose(row[3:6], (0, 0, 1))
+    # fewer contacts asked for: the deepest ones, in the same order
+    zc = cases[0][0]
+    verts = local @ R.T + (0, 0, zc)
+    order = np.argsort(verts[:, 2])
+    for maxc in (1, 2, 3):
+        c = O.collide(g, S.plane(0, 0, 1, 0), body((0, 0, zc), q), maxc=maxc)
+        assert len(c) == maxc
+        np.testing.assert_allclose(c[:, 0:3], verts[order[:maxc]], atol=1e-9)
+
+
+def test_capsule_box_axis_inside_the_box_uses_the_box_fallback():
+    # a capsule (r 0.1, length 1) lying along x with its axis inside a 2 m
+    # box, 5 cm below the top face: it is collided as a 0.2 x 0.2 x 1.2 box,
+    # giving the four corners of that box's bottom face at depth 0.15, the
+    # normal +z pointing into the capsule
+    cap = S.geom(S.CAPSULE, 0, (0.1, 1.0))
+    box = S.geom(S.BOX, 1, (2, 2, 2))
+    q = rotq((0, 1, 0), math.pi / 2)  # the capsule's axis (local z) along world x
+    c = O.collide(cap, box, body((0, 0, 0.95), q), body((0, 0, 0)), maxc=4)
+    assert len(c) == 4
+    xy = sorted(map(tuple, np.round(c[:, 0:2], 9)))
+    assert xy == sorted([(-0.6, -0.1), (-0.6, 0.1), (0.6, -0.1), (0.6, 0.1)])
+    np.testing.assert_allclose(c[:, 2], 0.85, atol=1e-9)
+    np.testing.assert_allclose(c[:, 6], 0.15, atol=1e-9)
+    np.testing.assert_allclose(c[:, 3:6], np.tile((0, 0, 1), (4, 1)), atol=1e-9)
+
+
+def test_capsule_box_axis_outside_the_box_is_a_sphere_contact():
+    # the axis 5 cm above the top face: one contact between the axis point
+    # and the face, depth r - 0.05, halfway into the overlap, normal +z
+    cap = S.geom(S.CAPSULE, 0, (0.1, 1.0))
+    box = S.geom(S.BOX, 1, (2, 2, 2))
+    q = rotq((0, 1, 0), math.pi / 2)
+    c = O.collide(cap, box, body((0.2, 0.1, 1.05), q), body((0, 0, 0)), maxc=4)
+    assert len(c) == 1
+    assert c[0, 6] == pytest.approx(0.05, abs=1e-12)
+    assert c[0, 2] == pytest.approx(1.05 - 0.075, abs=1e-12)
+    np.testing.assert_allclose(c[0, 3:6], (0, 0, 1), atol=1e-12)
+
+
+# ------------------------------------------------------ SOR iterates, exact
+def sor_reference(A, C, b, lo, hi, sweeps, w=1.3):
+    """Projected SOR on (A + diag(C)) x = b in row order, the bounds of a
+    row given as a function of x (friction rows: +-mu x_normal)."""
+    x = np.zeros(len(b))
+    out = []
+    for _ in range(sweeps):
+        for i in range(len(b)):
+            d = w * (b[i] - A[i] @ x - C[i] * x[i]) / (A[i, i] + C[i])
+            x[i] = min(max(x[i] + d, lo(i, x)), hi(i, x))
+        out.append(x.copy())
+    return out
+
+
+def test_sor_iterates_of_two_stacked_spheres_follow_the_recurrence():
+    # sphere A on the plane, sphere B on A, frictionless: two normal rows
+    # coupled through A.  J_1 = [0 0 1] on A; J_2 = [0 0 1] on B, [0 0 -1] on A.
+    # A = J M^-1 J^T, C = cfm / h, b_i = erp depth_i / h^2 - J_i (v / h + M^-1 f)
+    # (getInfo2 pushout, Stage 2 scaling and rhs, quickstep.cpp:1540-1560,
+    # 1670-1745); the rows are swept in joint order with over-relaxation w
+    # (IterationStep :2917-3033)
+    mA, mB, rA, rB, dA, dB = 3.0, 1.5, 0.2, 0.15, 0.01, 0.004
+    h, erp, cfm, w = 0.01, 0.2, 1e-5, 1.3
+    zA = rA - dA
+    zB = zA + rA + rB - dB
+    IA, IB = 0.4 * mA * rA * rA, 0.4 * mB * rB * rB
+    bodies = np.array([body((0, 0, zA), mass=mA, I=(IA,) * 3), body((0, 0, zB), mass=mB, I=(IB,) * 3)])
+    joints = np.zeros((2, 16))
+    joints[0, 1:3] = (0, -1)
+    joints[0, 3:6], joints[0, 6:9], joints[0, 9] = (0, 0, -dA), (0, 0, 1), dA
+    joints[1, 1:3] = (1, 0)
+    joints[1, 3:6], joints[1, 6:9], joints[1, 9] = (0, 0, zA + rA), (0, 0, 1), dB
+    g = 9.81
+    A = np.array([[1 / mA, -1 / mA], [-1 / mA, 1 / mA + 1 / mB]])
+    C = np.full(2, cfm / h)
+    b = np.array([erp * dA / h ** 2 + g, erp * dB / h ** 2])
+    want = sor_reference(A, C, b, lambda i, x: 0.0, lambda i, x: np.inf, 20, w)
+    for k in range(1, 21):
+        prm = O.params(iters=k, w=w, mu=0.0, erp=erp, cfm=cfm)
+        _, lam = O.quickstep(bodies, joints, prm, dtype=np.float64)
+        np.testing.assert_allclose(lam, want[k - 1], rtol=1e-11, atol=1e-11)
+    # a Jacobi sweep (both rows from the previous iterate) differs after one sweep
+    assert abs(want[0][1] - w * b[1] / (A[1, 1] + C[1])) > 1e-3
+
+
+def test_sor_iterates_of_a_sliding_sphere_clamp_friction_to_the_normal():
+    # one sphere sliding on the plane with mu = 0.5 (Approx1): a normal row
+    # and two friction rows whose bounds are +-mu times the normal row's
+    # current value.  For n = +z, dPlaneSpace gives t1 = (0, -1, 0), t2 =
+    # (1, 0, 0) (odemath.cpp:183-205); the contact is below the centre, so
+    # each friction row reaches the spin: J_t = [t, r x t] with r = (0, 0, -R)
+    m, R, d, mu = 2.0, 0.25, 0.003, 0.5
+    h, erp, cfm, w = 0.01, 0.2, 1e-5, 1.3
+    I = 0.4 * m * R * R
+    v = np.array([0.4, -0.3, 0.0])
+    bodies = np.array([body((0, 0, R - d), mass=m, I=(I,) * 3, lvel=v)])
+    joints = np.zeros((1, 16))
+    joints[0, 1:3] = (0, -1)
+    joints[0, 3:6], joints[0, 6:9], joints[0, 9] = (0, 0, -d), (0, 0, 1), d
+    r = np.array([0, 0, -R])
+    n, t1, t2 = np.array([0, 0, 1.0]), np.array([0, -1.0, 0]), np.array([1.0, 0, 0])
+    J = np.array([np.concatenate([n, np.cross(r, n)]), np.concatenate([t1, np.cross(r, t1)]),
+                  np.concatenate([t2, np.cross(r, t2)])])
+    Minv = np.diag([1 / m] * 3 + [1 / I] * 3)
+    A = J @ Minv @ J.T
+    vel = np.concatenate([v, np.zeros(3)])
+    acc = np.concatenate([[0, 0, -9.81], np.zeros(3)])
+    b = -(J @ (vel / h + acc))
+    b[0] += erp * d / h ** 2
+    C = np.full(3, cfm / h)
+    lo = lambda i, x: 0.0 if i == 0 else -mu * x[0]
+    hi = lambda i, x: np.inf if i == 0 else mu * x[0]
+    want = sor_reference(A, C, b, lo, hi, 20, w)
+    for k in range(1, 21):
+        prm = O.params(iters=k, w=w, mu=mu, erp=erp, cfm=cfm)
+        _, lam = O.quickstep(bodies, joints, prm, dtype=np.float64)
+        np.testing.assert_allclose(lam, want[k - 1], rtol=1e-11, atol=1e-11)
+    # the friction rows sit on their bound +-mu * normal after the first sweep
+    assert abs(abs(want[0][1]) - mu * want[0][0]) < 1e-12 or abs(abs(want[0][2]) - mu * want[0][0]) < 1e-12
