@@ -36,9 +36,35 @@ sys.path.insert(0, ROOT)
 
 METRIC = "world-steps/sec at LCP residual ≤1e-4: warm-start vs cold PGS, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
+# LDS bandwidth of the chip: 256 CUs x 128 B per clock x 2.4 GHz
+LDS_PEAK_GBS = 256 * 128 * 2.4
 BF16_PEAK_TFLOPS = 2500.0
 ROW_SWEEP_BYTES = 228  # per row per PGS sweep: row 124 B, lambda r/w 8 B, fc gather+scatter 96 B
 MLP_FLOP_PER_CONTACT = 2 * (16 * 128 + 128 * 128 + 128 * 3)
+# HBM bytes k_solve_small must move per launch by its design (rows are read
+# once and swept in LDS): per row the 128 B row record, its schedule entries
+# (item, perm, inv: 12 B) and lambda in and out (8 B); per body its seven
+# float4 streams and 128 B of body scratch read, pos / quat / lvel / avel
+# written; per contact the 48 B previous-contact record written
+SOLVE_ROW_BYTES = 128 + 12 + 8
+SOLVE_BODY_BYTES = 7 * 16 + 128 + 4 * 16
+SOLVE_CONTACT_BYTES = 48
+
+
+def cpu_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "available": avail, "cpu_model": model}
 
 
 def build_worlds(workload, nworlds, seed):
@@ -54,23 +80,76 @@ def build_worlds(workload, nworlds, seed):
 
 def cpu_baseline(worlds, bodies_now, body_start, prm, budget_s=12.0, steps_per_world=5):
     """The oracle's sequential float32 step (cold start, same stopping rule) on
-    a sample of the same worlds from the GPU's state after warmup, one core."""
+    a sample of the same worlds from the GPU's state after warmup: one core,
+    then one world per host thread on the host's cores (at most 16, the GPU's
+    share of the host)."""
+    import concurrent.futures as cf
     import oracle as O
-    done = 0
-    t0 = time.perf_counter()
-    nw = 0
-    for wi, w in enumerate(worlds):
+    info = cpu_info()
+
+    def run(wi):
+        w = worlds[wi]
         b = bodies_now[body_start[wi]:body_start[wi + 1]].astype(np.float32)
         for _ in range(steps_per_world):
             b, _, _ = O.world_step(b, w.geoms, prm, balls=w.balls, dtype=np.float32)
-            done += 1
+        return steps_per_world
+
+    # one core
+    done, nw = 0, 0
+    t0 = time.perf_counter()
+    for wi in range(len(worlds)):
+        done += run(wi)
         nw += 1
         if time.perf_counter() - t0 > budget_s:
             break
+    one = done / (time.perf_counter() - t0)
+    # all cores: the oracle's C++ step runs with the interpreter lock released
+    cores = max(1, min(16, info["available"]))
+    done, nw2 = 0, 0
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=cores) as ex:
+        pending = set()
+        wi = 0
+        while True:
+            while len(pending) < 2 * cores and wi < len(worlds) and time.perf_counter() - t0 < budget_s:
+                pending.add(ex.submit(run, wi))
+                wi += 1
+            if not pending:
+                break
+            fin, pending = cf.wait(pending, return_when=cf.FIRST_COMPLETED)
+            for f in fin:
+                done += f.result()
+                nw2 += 1
     dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "world-steps/s", "cores": 1, "kind": "port",
-            "sample": "%d worlds x %d steps from the GPU state after warmup (%.1f s; oracle float32, cold start, "
-                      "reference row order, same residual stopping rule)" % (nw, steps_per_world, dt)}
+    return {"value": done / dt, "unit": "world-steps/s", "cores": cores, "kind": "port",
+            "single_core": one, "nproc": info["nproc"], "cpu_model": info["cpu_model"],
+            "sample": "the oracle's float32 step (cold start, reference row order, same residual stopping rule), "
+                      "%d steps per world from the GPU state after warmup: %d worlds on one core, then %d worlds "
+                      "one per thread on %d threads (%.1f s)" % (steps_per_world, nw, nw2, cores, dt)}
+
+
+def fp64_residual(B, worlds, prm64, sample=64):
+    """The LCP residual of the GPU's impulses re-evaluated in float64 by the
+    oracle (rows rebuilt in double from the step-start state) for a sample of
+    worlds over one more step, beside the kernel's own float32 residual."""
+    import oracle as O
+    idx = np.unique(np.linspace(0, len(worlds) - 1, sample).astype(int))
+    before = B.bodies()
+    B.step(1)
+    st = B.stats()
+    res = []
+    for wi in idx:
+        bw = before[B.body_start[wi]:B.body_start[wi + 1]].astype(np.float64)
+        j = B.joints(int(wi)).astype(np.float64)
+        lam = B.lam(int(wi)).astype(np.float64)
+        if len(j) == 0:
+            continue
+        res.append(O.lcp_residual(bw, j, prm64, lam))
+    res = np.array(res)
+    return {"worlds": int(len(res)), "max": float(res.max()), "median": float(np.median(res)),
+            "gpu_f32_max": float(np.max(st["residual"][idx])),
+            "definition": "max_i |clamp(l_i - w_i/A_ii) - l_i| / max(max|l|, 1e-3), A = J M^-1 J^T + cfm/h, "
+                          "w = A l - b, rebuilt in float64 from the step-start state"}
 
 
 def measure(B, args, warm, nworlds):
@@ -110,18 +189,27 @@ def measure(B, args, warm, nworlds):
     bodies_after_warmup = B.bodies()
     total_ms = B.time(args.steps)
     st = B.stats()
-    ach, dur, sweeps, contacts = [], [[], [], []], [], []
+    # kernel durations per launch on the schedule the timed run uses (each
+    # world range on its own stream), with the per-launch algorithmic bytes
+    nb_w = np.diff(B.body_start)
+    launches = []
+    sweeps, contacts, inferred = [], [], []
     for _ in range(args.kernel_steps):
-        k = B.kernel_times(1)
+        ks = B.schedule_kernel_times(1)
         s = B.stats()
-        row_sweeps = float(np.sum(s["iters"].astype(np.float64) * s["rows"]))
-        ach.append(row_sweeps * ROW_SWEEP_BYTES / (k[2] * 1e-3) / 1e9)
-        for i in range(3):
-            dur[i].append(k[i])
+        rows = s["rows"].astype(np.float64)
+        hbm = SOLVE_ROW_BYTES * rows + SOLVE_BODY_BYTES * nb_w + SOLVE_CONTACT_BYTES * s["contacts"]
+        lds = ROW_SWEEP_BYTES * rows * s["iters"]
+        for w0, w1, ms in ks:
+            launches.append((ms[0], ms[1], ms[2], float(np.sum(hbm[w0:w1])), float(np.sum(lds[w0:w1])), w1 - w0))
         sweeps.append(float(np.mean(s["iters"])))
         contacts.append(float(np.sum(s["contacts"])))
+        if warm == 3:
+            inferred.append(len(B.samples()[0]))
+    L = np.array(launches)
+    solve_ms = float(np.mean(L[:, 2]))
     ms = total_ms / args.steps
-    return {
+    out = {
         "world_steps_per_s": nworlds / (ms * 1e-3),
         "ms_per_step": ms,
         "mean_sweeps": float(np.mean(st["iters"])),
@@ -129,16 +217,27 @@ def measure(B, args, warm, nworlds):
         "max_residual": float(np.max(st["residual"])),
         "worlds_at_target": float(np.mean(st["residual"] <= args.tol)),
         "contacts_per_step": float(np.sum(st["contacts"])),
-        "kernel_ms": {"prepare": float(np.mean(dur[0])), "infer": float(np.mean(dur[1])),
-                      "solve": float(np.mean(dur[2]))},
-        "solve_gbs": float(np.mean(ach)),
+        "kernel_ms_per_launch": {"prepare": float(np.mean(L[:, 0])), "infer": float(np.mean(L[:, 1])),
+                                 "solve": solve_ms},
+        "launches_per_step": len(launches) // args.kernel_steps,
+        "solve_hbm_bytes_per_launch": float(np.mean(L[:, 3])),
+        "solve_hbm_gbs": float(np.mean(L[:, 3])) / (solve_ms * 1e-3) / 1e9,
+        "solve_lds_gbs": float(np.mean(L[:, 4])) / (solve_ms * 1e-3) / 1e9,
         "kernel_window_contacts": float(np.mean(contacts)),
+        "kernel_window_sweeps": float(np.mean(sweeps)),
         "overflow": [int(x) for x in B.overflow()],
         "lds_rows": lds_rows,
         "schedule": schedule,
         "worlds_over_lds_budget": int(np.sum(st["rows"] > lds_rows)),
         "_bodies": bodies_after_warmup,
     }
+    if warm == 3 and inferred:
+        ki = float(np.mean(L[:, 1]))
+        per_launch = float(np.mean(inferred)) / max(1, out["launches_per_step"])
+        out["inferred_contacts_per_step"] = float(np.mean(inferred))
+        if ki > 0:
+            out["mlp_infer_tflops"] = per_launch * MLP_FLOP_PER_CONTACT / (ki * 1e-3) / 1e12
+    return out
 
 
 def bench_large(args):
@@ -415,6 +514,7 @@ def main():
             bodies0 = r["_bodies"]
             body_start = B.body_start
         r.pop("_bodies")
+        r["fp64_residual"] = fp64_residual(B, worlds, O.params(iters=args.max_iters, w=args.sor_w))
         results[mode] = r
         B.close()
 
@@ -449,35 +549,46 @@ def main():
         },
         "modes": results,
     }
-    # roofline of the dominant kernel of the plain GPU path (cold PGS: the solve)
+    # roofline of the dominant kernel of the plain GPU path (cold PGS: the
+    # solve), per launch on the schedule the timed run used
     RR = results["cold"] if "cold" in results else R
     out["roofline"] = {
         "bound": "hbm",
-        "achieved": RR["solve_gbs"],
+        "achieved": RR["solve_hbm_gbs"],
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": RR["solve_gbs"] / HBM_PEAK_GBS,
+        "frac": RR["solve_hbm_gbs"] / HBM_PEAK_GBS,
         "traffic": None,
-        "kernel": "k_solve_small (cold mode, whole-batch launch)",
-        "note": "achieved = 228 B per row per sweep (rows streamed from HBM every sweep); this kernel reads each "
-                "row from HBM once per step and sweeps it in LDS, so achieved can exceed the HBM peak",
+        "kernel": "k_solve_small, cold mode, %d launches per step on the autotuned schedule, %.3f ms and %.1f MB "
+                  "algorithmic per launch" % (RR["launches_per_step"], RR["kernel_ms_per_launch"]["solve"],
+                                              RR["solve_hbm_bytes_per_launch"] / 1e6),
+        "algorithmic": "per launch: 148 B per row (row record, schedule entries, lambda in and out), 304 B per body, "
+                       "48 B per contact; the rows are swept in LDS, not re-read from HBM",
+        "lds": {"achieved": RR["solve_lds_gbs"], "peak": LDS_PEAK_GBS, "unit": "GB/s",
+                "frac": RR["solve_lds_gbs"] / LDS_PEAK_GBS,
+                "model": "228 B per row per sweep (row 124 B, lambda 8 B, two bodies' fc 96 B) through LDS"},
+        "occupancy": "one wave per world; LDS budget %d rows per world" % RR["lds_rows"],
     }
-    # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size)
+    # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE over
+    # the k_solve_small dispatches of the same schedule, scaled by a copy of
+    # known size)
     tpath = os.path.join(ROOT, "profiles", "k_solve_small_%s4096_cold.json" % args.workload)
     if args.worlds == 4096 and os.path.exists(tpath):
         t = json.load(open(tpath))
-        out["roofline"]["traffic"] = t["traffic_bytes_per_launch"]
-        out["roofline"]["traffic_source"] = os.path.relpath(tpath, ROOT)
+        if t.get("launches_per_step") == RR["launches_per_step"]:
+            out["roofline"]["traffic"] = t["traffic_bytes_per_launch"]
+            out["roofline"]["traffic_source"] = os.path.relpath(tpath, ROOT)
+    out["fp64_residual"] = R.get("fp64_residual")
     if "cold" in results and head != "cold":
         out["speedup_vs_cold"] = R["world_steps_per_s"] / results["cold"]["world_steps_per_s"]
     if "plain" in results and head != "plain":
         out["speedup_vs_plain_warm"] = R["world_steps_per_s"] / results["plain"]["world_steps_per_s"]
     if train_info:
         out["training"] = train_info
-        ki = R["kernel_ms"]["infer"]
-        if ki > 0:
-            out["mlp_infer_tflops"] = R["kernel_window_contacts"] * MLP_FLOP_PER_CONTACT / (ki * 1e-3) / 1e12
-            out["mlp_infer_frac_bf16_peak"] = out["mlp_infer_tflops"] / BF16_PEAK_TFLOPS
+        if "mlp_infer_tflops" in R:
+            out["mlp_infer_tflops"] = R["mlp_infer_tflops"]
+            out["mlp_infer_frac_bf16_peak"] = R["mlp_infer_tflops"] / BF16_PEAK_TFLOPS
+            out["mlp_inferred_contacts_per_step"] = R["inferred_contacts_per_step"]
     if not args.no_cpu and bodies0 is not None:
         out["cpu_baseline"] = cpu_baseline(worlds, bodies0, body_start,
                                            O.params(iters=args.max_iters, tol=args.tol, check_every=args.check_every, w=args.sor_w))

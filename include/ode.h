@@ -122,6 +122,30 @@ void dWorldSetQuickStepDynamicIterationParameters(dWorldID w, const dReal *prema
 void dWorldGetQuickStepDynamicIterationParameters(dWorldID w, dReal *premature_exit_delta,
                                                   dReal *max_num_extra_factor,
                                                   dReal *extra_iteration_requirement_delta); /* objects.h:528 */
+/* Dynamic-iteration statistics (include/ode/objects.h:538-556): per island
+ * step, premature_exits counts solves that stopped before the iteration
+ * count, prolonged_execs solves that stopped inside the extra iterations,
+ * full_extra_execs solves that used every extra iteration; iteration_count
+ * is not written (as in the reference's QuickStep). */
+typedef struct {
+    unsigned struct_size;
+    unsigned iteration_count;
+    unsigned premature_exits;
+    unsigned prolonged_execs;
+    unsigned full_extra_execs;
+} dWorldQuickStepIterationCount_DynamicAdjustmentStatistics;
+static inline void dWorldInitializeQuickStepIterationCount_DynamicAdjustmentStatistics(
+    dWorldQuickStepIterationCount_DynamicAdjustmentStatistics *ptr_stat) {
+    ptr_stat->struct_size = sizeof(*ptr_stat);
+    ptr_stat->iteration_count = 0;
+    ptr_stat->premature_exits = 0;
+    ptr_stat->prolonged_execs = 0;
+    ptr_stat->full_extra_execs = 0;
+}
+/* Attaches var_stats (NULL: the world's own) to the world's QuickStep; 0 when
+ * var_stats->struct_size is too small or not a multiple of 4. */
+int dWorldAttachQuickStepDynamicIterationStatisticsSink(
+    dWorldID w, dWorldQuickStepIterationCount_DynamicAdjustmentStatistics *var_stats); /* objects.h:576 */
 void dWorldSetQuickStepW(dWorldID w, dReal over_relaxation);   /* objects.h:584 */
 dReal dWorldGetQuickStepW(dWorldID w);                         /* objects.h:591 */
 void dWorldSetContactMaxCorrectingVel(dWorldID w, dReal vel);  /* objects.h:603 */
@@ -258,6 +282,8 @@ void dMassSetSphere(dMass *m, dReal density, dReal radius);                /* ma
 void dMassSetSphereTotal(dMass *m, dReal total_mass, dReal radius);        /* mass.h:53 */
 void dMassSetCapsule(dMass *m, dReal density, int direction, dReal radius, dReal length);       /* mass.h:55 */
 void dMassSetCapsuleTotal(dMass *m, dReal total_mass, int direction, dReal radius, dReal length); /* mass.h:57 */
+void dMassSetCylinder(dMass *m, dReal density, int direction, dReal radius, dReal length);      /* mass.h:60 */
+void dMassSetCylinderTotal(dMass *m, dReal total_mass, int direction, dReal radius, dReal length); /* mass.h:62 */
 void dMassSetBox(dMass *m, dReal density, dReal lx, dReal ly, dReal lz);   /* mass.h:65 */
 void dMassSetBoxTotal(dMass *m, dReal total_mass, dReal lx, dReal ly, dReal lz); /* mass.h:67 */
 
@@ -288,6 +314,12 @@ dGeomID dCreatePlane(dSpaceID space, dReal a, dReal b, dReal c, dReal d); /* col
 void dGeomPlaneGetParams(dGeomID plane, dVector4 result);    /* collision.h:1047 */
 dGeomID dCreateCapsule(dSpaceID space, dReal radius, dReal length); /* collision.h:1050 */
 void dGeomCapsuleGetParams(dGeomID capsule, dReal *radius, dReal *length); /* collision.h:1052 */
+/* Cylinders collide with planes only in this build (dCollideCylinderPlane,
+ * ode/src/collision_cylinder_plane.cpp); dCollide of a cylinder with any
+ * other class returns 0. */
+dGeomID dCreateCylinder(dSpaceID space, dReal radius, dReal length); /* collision.h:1062 */
+void dGeomCylinderSetParams(dGeomID cylinder, dReal radius, dReal length); /* collision.h:1063 */
+void dGeomCylinderGetParams(dGeomID cylinder, dReal *radius, dReal *length); /* collision.h:1064 */
 
 dSpaceID dSimpleSpaceCreate(dSpaceID space);                 /* collision_space.h:52 */
 dSpaceID dHashSpaceCreate(dSpaceID space);                   /* collision_space.h:53 */

@@ -157,6 +157,14 @@ int odxBatchTime(odxBatch *b, int nsteps, float *total_ms);
  * returns the average durations [prepare, warm-start inference, solve] in
  * milliseconds (kernel_ms[3]; inference is 0 when not used). */
 int odxBatchKernelTimes(odxBatch *b, int nsteps, float *kernel_ms);
+/* Runs nsteps steps launched on the batch's current schedule (its world
+ * ranges on their own streams; no counterpart in the reference) with
+ * hipEvents on each range's stream around each kernel.  Writes, per range r,
+ * kernel_ms[3r..3r+2] = mean [prepare, inference, solve] milliseconds per
+ * launch (a fused schedule reports its one kernel as solve) and
+ * ranges[2r..2r+1] = its first and one-past-last world; returns the number
+ * of ranges (at most 8), 0 on failure. */
+int odxBatchScheduleKernelTimes(odxBatch *b, int nsteps, float *kernel_ms, int *ranges);
 
 /* Readback.  bodies: [NB][32].  stats per world: contacts, rows, sweeps,
  * residual (NULL pointers are skipped).  Returns the total count written. */

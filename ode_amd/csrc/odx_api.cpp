@@ -311,6 +311,11 @@ struct dxWorld {
     float max_angular_speed = dInfinity;
     int order = 0;
     int sweep_mode = 0;  // odxWorldSetSweepMode
+    // the dynamic-iteration statistics sink (the world's own unless one is
+    // attached; objects.cpp:88, objects.h:168-169)
+    dWorldQuickStepIterationCount_DynamicAdjustmentStatistics own_stats{
+        sizeof(dWorldQuickStepIterationCount_DynamicAdjustmentStatistics), 0, 0, 0, 0};
+    dWorldQuickStepIterationCount_DynamicAdjustmentStatistics *stats = &own_stats;
     int last_sweeps = 0, last_rows = 0;
     float last_res = -1.f;
     // device buffers of the step: rows, impulses and body scratch stay on the
@@ -738,6 +743,18 @@ void dWorldGetQuickStepDynamicIterationParameters(dWorldID w, dReal *premature_e
     if (extra_iteration_requirement_delta) *extra_iteration_requirement_delta = w->dyn_extra_delta;
     if (max_num_extra_factor) *max_num_extra_factor = w->dyn_extra_factor;
 }
+// dWorldAttachQuickStepDynamicIterationStatisticsSink ode.cpp:2111-2131
+int dWorldAttachQuickStepDynamicIterationStatisticsSink(
+    dWorldID w, dWorldQuickStepIterationCount_DynamicAdjustmentStatistics *var_stats) {
+    if (!w) return 0;
+    if (!var_stats) {
+        w->stats = &w->own_stats;
+        return 1;
+    }
+    if (var_stats->struct_size < sizeof(*var_stats) || var_stats->struct_size % sizeof(unsigned) != 0) return 0;
+    w->stats = var_stats;
+    return 1;
+}
 void dWorldSetQuickStepW(dWorldID w, dReal over_relaxation) { w->w = over_relaxation; }
 dReal dWorldGetQuickStepW(dWorldID w) { return w->w; }
 void dWorldSetContactMaxCorrectingVel(dWorldID w, dReal vel) { w->max_vel = vel; }
@@ -793,6 +810,7 @@ void dif_geom(DifWriter &c, const dxGeom *g) {
     case dSphereClass: c.str("type", "sphere"); c.num("radius", g->p[0]); break;
     case dBoxClass: c.str("type", "box"); c.vec("sides", g->p); break;
     case dCapsuleClass: c.str("type", "capsule"); c.num("radius", g->p[0]); c.num("length", g->p[1]); break;
+    case dCylinderClass: c.str("type", "cylinder"); c.num("radius", g->p[0]); c.num("length", g->p[1]); break;
     case dPlaneClass: c.str("type", "plane"); c.vec("normal", g->p); c.num("d", g->p[3]); break;
     default: break;
     }
@@ -1372,6 +1390,18 @@ void dMassSetCapsule(dMass *m, dReal density, int direction, dReal r, dReal len)
     m->I[0] = m->I[5] = m->I[10] = Ia;
     if (direction >= 1 && direction <= 3) m->I[(direction - 1) * 5] = Ib;
 }
+// dMassSetCylinder / dMassSetCylinderTotal mass.cpp:173-198
+void dMassSetCylinderTotal(dMass *m, dReal total, int direction, dReal r, dReal len) {
+    dMassSetZero(m);
+    const float r2 = r * r;
+    m->mass = total;
+    const float I = total * (0.25f * r2 + (1.0f / 12.0f) * len * len);
+    m->I[0] = m->I[5] = m->I[10] = I;
+    if (direction >= 1 && direction <= 3) m->I[(direction - 1) * 5] = total * 0.5f * r2;
+}
+void dMassSetCylinder(dMass *m, dReal density, int direction, dReal r, dReal len) {
+    dMassSetCylinderTotal(m, (float)(M_PI * r * r * len * density), direction, r, len);
+}
 void dMassSetCapsuleTotal(dMass *m, dReal total, int direction, dReal r, dReal len) {
     dMassSetCapsule(m, 1.f, direction, r, len);
     float s = total / m->mass;
@@ -1417,6 +1447,18 @@ dGeomID dCreateCapsule(dSpaceID space, dReal radius, dReal length) {
     g->zero_sized = radius == 0;  // capsule.cpp:56 (a zero length is not zero-sized)
     return g;
 }
+dGeomID dCreateCylinder(dSpaceID space, dReal radius, dReal length) {
+    dxGeom *g = new_geom(space, dCylinderClass);
+    g->p[0] = radius; g->p[1] = length;
+    g->zero_sized = radius == 0 || length == 0;  // cylinder.cpp:54
+    return g;
+}
+void dGeomCylinderSetParams(dGeomID g, dReal radius, dReal length) {
+    g->p[0] = radius; g->p[1] = length;
+    g->zero_sized = radius == 0 || length == 0;
+    geom_moved(g);  // cylinder.cpp:84-93
+}
+void dGeomCylinderGetParams(dGeomID g, dReal *radius, dReal *length) { *radius = g->p[0]; *length = g->p[1]; }
 void dGeomDestroy(dGeomID g) {
     if (!g) return;
     space_remove(g);
@@ -1496,6 +1538,13 @@ void dGeomGetAABB(dGeomID g, dReal aabb[6]) {
     case dCapsuleClass:
         for (int k = 0; k < 3; k++) {
             float r = std::fabs(R[4 * k + 2] * g->p[1]) * 0.5f + g->p[0];
+            aabb[2 * k] = pos[k] - r; aabb[2 * k + 1] = pos[k] + r;
+        }
+        break;
+    case dCylinderClass:
+        for (int k = 0; k < 3; k++) {
+            const float a = R[4 * k + 2], o = 1.0f - a * a;
+            float r = std::fabs(a * g->p[1] * 0.5f) + g->p[0] * std::sqrt(0.f > o ? 0.f : o);
             aabb[2 * k] = pos[k] - r; aabb[2 * k + 1] = pos[k] + r;
         }
         break;
@@ -2216,7 +2265,7 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     const size_t o_pos = G.slot(f4), o_quat = G.slot(f4), o_lvel = G.slot(f4), o_avel = G.slot(f4);
     const size_t o_it = G.slot(2 * sizeof(int));
     const size_t o_fb = G.slot(want_fb ? (size_t)nj * 12 * sizeof(float) : 0);
-    const size_t o_isl = G.slot(w->order == 2 ? (size_t)std::max(nisl, 1) * sizeof(int) : 0);
+    const size_t o_isl = G.slot(w->order == 2 || dyn_on ? (size_t)std::max(nisl, 1) * sizeof(int) : 0);
     const size_t out_end = G.used;
     const size_t o_facc = G.slot(f4), o_tacc = G.slot(f4), o_mass = G.slot(f4);
     const size_t o_Ib = G.slot(Ib.size() * sizeof(float)), o_invIb = G.slot(invIb.size() * sizeof(float));
@@ -2291,10 +2340,10 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
         P.nislands = nisl;
         D.body_island = P.dyn ? G.d<int>(o_island) : nullptr;
     }
+    D.island_sweeps = P.dyn ? G.d<int>(o_isl) : nullptr;
     if (w->order == 2) {
         D.seg_table = G.d<int>(o_seg);
         D.nseg = nseg;
-        D.island_sweeps = P.dyn ? G.d<int>(o_isl) : nullptr;
     }
     if (g_ktime.on && !g_ktime.ev[0]) {
         if (!odx_check(hipEventCreate(&g_ktime.ev[0]), "hipEventCreate") ||
@@ -2315,7 +2364,7 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     res = G.h<float>(o_it)[1];
     fbh.assign(want_fb ? (size_t)nj * 12 : 0, 0.f);
     if (want_fb) std::memcpy(fbh.data(), G.h<float>(o_fb), fbh.size() * sizeof(float));
-    if (w->order == 2 && P.dyn) std::memcpy(isl_sweeps.data(), G.h<int>(o_isl), (size_t)nisl * sizeof(int));
+    if (P.dyn) std::memcpy(isl_sweeps.data(), G.h<int>(o_isl), (size_t)nisl * sizeof(int));
     if (w->order != 2) break;
     // shuffles each island ran: before sweeps 8, 16, ... that it reached
     bool agree = true;
@@ -2353,6 +2402,23 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
             for (int i = 0; i < 3; i++) { fb->f1[i] = f[i]; fb->t1[i] = f[3 + i]; }
             if (joints[k]->node[1].body)
                 for (int i = 0; i < 3; i++) { fb->f2[i] = f[6 + i]; fb->t2[i] = f[9 + i]; }
+        }
+    }
+    // dynamic-iteration statistics per island with rows (quickstep.cpp:1829-1853):
+    // stopped before the iteration count, inside the extra sweeps, or after all
+    // of them (the last extra sweep is not followed by the exit test)
+    const float xcount = (float)w->iterations * w->dyn_extra_factor;
+    const int X = xcount < (float)ODX_MAX_EXTRA_SWEEPS ? (int)xcount : ODX_MAX_EXTRA_SWEEPS;
+    if (X != 0 || w->dyn_premature != 0.f) {
+        std::vector<int> isl_m((size_t)std::max(nisl, 1), 0);
+        for (int k = 0; k < nj; k++) isl_m[(size_t)island[(size_t)joints[k]->node[0].body->tag]] += jm[k];
+        const int N = w->iterations;
+        for (int i = 0; i < nisl; i++) {
+            if (isl_m[(size_t)i] == 0) continue;
+            const int sw = isl_sweeps[(size_t)i];
+            if (sw < N) w->stats->premature_exits++;
+            else if (X > 0 && sw == N + X) w->stats->full_extra_execs++;
+            else if (sw > N) w->stats->prolonged_execs++;
         }
     }
     w->last_sweeps = it;

@@ -818,6 +818,69 @@ int odxBatchKernelTimes(odxBatch *b, int nsteps, float *kernel_ms) {
     return 1;
 }
 
+int odxBatchScheduleKernelTimes(odxBatch *b, int nsteps, float *kernel_ms, int *ranges) {
+    if (!b || nsteps <= 0 || !kernel_ms || !ranges) return 0;
+    if (!odxBatchSync(b)) return 0;
+    struct Range { int w0, w1; hipStream_t s; };
+    std::vector<Range> rs;
+    if (b->splits.empty()) rs.push_back({0, b->nw, b->stream});
+    else
+        for (const odxSplit &sp : b->splits) rs.push_back({sp.w0, sp.w1, sp.stream});
+    const int nr = (int)rs.size();
+    // per step and range: before prepare, before inference, before solve, after
+    std::vector<hipEvent_t> ev((size_t)nsteps * nr * 4, nullptr);
+    bool ok = true;
+    for (auto &e : ev) ok = ok && odx_check(hipEventCreate(&e), "hipEventCreate");
+    auto E = [&](int s, int r, int k) { return ev[((size_t)s * nr + r) * 4 + k]; };
+    for (int s = 0; ok && s < nsteps; s++) {
+        if (!b->splits.empty()) ok = ok && odx_check(hipEventRecord(b->fork, b->stream), "hipEventRecord");
+        for (int r = 0; ok && r < nr; r++) {
+            const Range &R = rs[(size_t)r];
+            if (!b->splits.empty()) ok = ok && odx_check(hipStreamWaitEvent(R.s, b->fork, 0), "hipStreamWaitEvent");
+            StepParams q = b->launch_params(R.w0, R.w1, b->splits.empty() ? 0 : r);
+            ok = ok && odx_check(hipEventRecord(E(s, r, 0), R.s), "hipEventRecord");
+            if (b->fused > 0) {
+                q.feat_local = 1;
+                const MlpDev *M = b->infer_on() ? mlp_device_view(b->mlp) : nullptr;
+                ok = ok && odx_check(hipEventRecord(E(s, r, 1), R.s), "hipEventRecord") &&
+                     odx_check(hipEventRecord(E(s, r, 2), R.s), "hipEventRecord");
+                launch_step_fused(b->D, q, M, 1, R.s);
+                ok = ok && odx_check(hipGetLastError(), "k_step_fused launch");
+            } else {
+                launch_prepare_small(b->D, q, R.s);
+                ok = ok && odx_check(hipGetLastError(), "k_prepare_small launch");
+                ok = ok && odx_check(hipEventRecord(E(s, r, 1), R.s), "hipEventRecord");
+                ok = ok && b->enqueue_infer(q, R.s);
+                ok = ok && odx_check(hipEventRecord(E(s, r, 2), R.s), "hipEventRecord");
+                launch_solve_small(b->D, q, R.s);
+                ok = ok && odx_check(hipGetLastError(), "k_solve_small launch");
+            }
+            ok = ok && odx_check(hipEventRecord(E(s, r, 3), R.s), "hipEventRecord");
+            if (!b->splits.empty()) {
+                const odxSplit &sp = b->splits[(size_t)r];
+                ok = ok && odx_check(hipEventRecord(sp.done, R.s), "hipEventRecord") &&
+                     odx_check(hipStreamWaitEvent(b->stream, sp.done, 0), "hipStreamWaitEvent");
+            }
+        }
+    }
+    b->samples_split = b->fused > 0 ? -2 : (b->splits.empty() ? 0 : -1);
+    ok = ok && odxBatchSync(b);
+    for (int r = 0; ok && r < nr; r++) {
+        double t[3] = {0.0, 0.0, 0.0};
+        for (int s = 0; ok && s < nsteps; s++)
+            for (int k = 0; ok && k < 3; k++) {
+                float ms = 0.f;
+                ok = ok && odx_check(hipEventElapsedTime(&ms, E(s, r, k), E(s, r, k + 1)), "hipEventElapsedTime");
+                t[k] += ms;
+            }
+        for (int k = 0; k < 3; k++) kernel_ms[3 * r + k] = (float)(t[k] / nsteps);
+        ranges[2 * r] = rs[(size_t)r].w0;
+        ranges[2 * r + 1] = rs[(size_t)r].w1;
+    }
+    for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+    return ok ? nr : 0;
+}
+
 int odxBatchGetBodies(odxBatch *b, float *out) {
     if (!b || !out) return 0;
     if (!odxBatchSync(b)) return 0;

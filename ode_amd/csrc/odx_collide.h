@@ -13,7 +13,7 @@
 
 namespace odx {
 
-enum { CLS_SPHERE = 0, CLS_BOX = 1, CLS_CAPSULE = 2, CLS_PLANE = 4 };
+enum { CLS_SPHERE = 0, CLS_BOX = 1, CLS_CAPSULE = 2, CLS_CYLINDER = 3, CLS_PLANE = 4 };
 
 struct DGeom {
     int cls;
@@ -49,6 +49,15 @@ ODX_DEV void geom_aabb(const DGeom &G, float *ab) {
     case CLS_CAPSULE: {
         float r = G.p[0], lz = G.p[1];
         float xr = fabsf(R[2] * lz) * 0.5f + r, yr = fabsf(R[6] * lz) * 0.5f + r, zr = fabsf(R[10] * lz) * 0.5f + r;
+        ab[0] = p[0] - xr; ab[1] = p[0] + xr; ab[2] = p[1] - yr; ab[3] = p[1] + yr; ab[4] = p[2] - zr; ab[5] = p[2] + zr;
+        break;
+    }
+    case CLS_CYLINDER: {  // cylinder.cpp:61-78: axis half-length plus the disc's extent
+        float r = G.p[0], lz = G.p[1];
+        float o2 = 1.f - R[2] * R[2], o6 = 1.f - R[6] * R[6], o10 = 1.f - R[10] * R[10];
+        float xr = fabsf(R[2] * lz * 0.5f) + r * sqrtf(0.f > o2 ? 0.f : o2);
+        float yr = fabsf(R[6] * lz * 0.5f) + r * sqrtf(0.f > o6 ? 0.f : o6);
+        float zr = fabsf(R[10] * lz * 0.5f) + r * sqrtf(0.f > o10 ? 0.f : o10);
         ab[0] = p[0] - xr; ab[1] = p[0] + xr; ab[2] = p[1] - yr; ab[3] = p[1] + yr; ab[4] = p[2] - zr; ab[5] = p[2] + zr;
         break;
     }
@@ -702,6 +711,77 @@ ODX_DEV int collide_capsule_plane(const DGeom &cp, const DGeom &pl, int maxc, DC
 
 // Forward collider table entry (cls1, cls2) or -1 when the reference has no
 // forward entry (dInitColliders collision_kernel.cpp:166-268).
+// dCollideCylinderPlane collision_cylinder_plane.cpp:42-250.  The disc ends
+// e1 (-axis) and e2 (+axis).  When a disc is parallel to the plane (|a.n| = 1
+// within 1e-4) the nearer disc gives up to four rim points, two orthogonal
+// radius vectors V1, V2 from either side of its centre, each kept while
+// depth > 0; otherwise each disc gives its rim point deepest along -n, kept
+// while depth >= 0.  The expressions keep the reference's operation order
+// (the second tilted depth is summed term by term, as there).
+ODX_DEV int collide_cylinder_plane(const DGeom &cy, const DGeom &pl, int maxc, DContact *c) {
+    const float tol = 0.0001f;
+    const float radius = cy.p[0], s0 = cy.p[1] * 0.5f;
+    const float *cp = cy.pos, *pv = pl.p;
+    const float dir[3] = {cy.R[2], cy.R[6], cy.R[10]};
+    float e1[3], e2[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        e2[i] = dir[i] * s0 + cp[i];
+        e1[i] = dir[i] * -s0 + cp[i];
+    }
+    int n = 0;
+    float s = dir[0] * pv[0] + dir[1] * pv[1] + dir[2] * pv[2];
+    s = s < 0.f ? s + 1.f : s - 1.f;
+    if (s < tol && s > -tol) {
+        const float d1 = pv[3] - dot3(pv, e1), d2 = pv[3] - dot3(pv, e2);
+        const bool first = d1 >= d2;
+        if (!((first ? d1 : d2) >= 0.f)) return 0;
+        const float *P = first ? e1 : e2;
+        float V1[3] = {dir[0], dir[1], dir[2]}, V2[3];
+        if (dir[0] < tol && dir[0] > -tol) V1[0] += 1.f;
+        else V1[1] += 1.f;
+        cross3(V2, V1, dir);
+        const float t = radius / sqrtf(V2[0] * V2[0] + V2[1] * V2[1] + V2[2] * V2[2]);
+        V2[0] *= t; V2[1] *= t; V2[2] *= t;
+        cross3(V1, V2, dir);
+        for (int k = 0; k < 4 && n < maxc; k++) {
+            const float *off = k < 2 ? V1 : V2;
+            float q[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) q[i] = (k & 1) ? P[i] - off[i] : P[i] + off[i];
+            const float depth = pv[3] - dot3(pv, q);
+            if (depth > 0.f) {
+#pragma unroll
+                for (int i = 0; i < 3; i++) { c[n].pos[i] = q[i]; c[n].normal[i] = pv[i]; }
+                c[n].depth = depth;
+                n++;
+            }
+        }
+        return n;
+    }
+    const float t = dot3(pv, dir);
+    float C[3] = {dir[0] * t - pv[0], dir[1] * t - pv[1], dir[2] * t - pv[2]};
+    const float l = radius / sqrtf(C[0] * C[0] + C[1] * C[1] + C[2] * C[2]);
+    C[0] *= l; C[1] *= l; C[2] *= l;
+    float q[3] = {C[0] + e1[0], C[1] + e1[1], C[2] + e1[2]};
+    float depth = pv[3] - dot3(pv, q);
+    if (depth >= 0.f) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) { c[n].pos[i] = q[i]; c[n].normal[i] = pv[i]; }
+        c[n].depth = depth;
+        if (++n >= maxc) return n;
+    }
+    q[0] = C[0] + e2[0]; q[1] = C[1] + e2[1]; q[2] = C[2] + e2[2];
+    depth = pv[3] - pv[0] * q[0] - pv[1] * q[1] - pv[2] * q[2];
+    if (depth >= 0.f) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) { c[n].pos[i] = q[i]; c[n].normal[i] = pv[i]; }
+        c[n].depth = depth;
+        n++;
+    }
+    return n;
+}
+
 ODX_DEV int collide_fwd(const DGeom &a, const DGeom &b, int maxc, DContact *c) {
     switch (a.cls * 8 + b.cls) {
     case CLS_SPHERE * 8 + CLS_SPHERE: return collide_spheres(a.pos, a.p[0], b.pos, b.p[0], c[0]);
@@ -713,6 +793,7 @@ ODX_DEV int collide_fwd(const DGeom &a, const DGeom &b, int maxc, DContact *c) {
     case CLS_CAPSULE * 8 + CLS_BOX: return collide_capsule_box(a, b, maxc, c);
     case CLS_CAPSULE * 8 + CLS_CAPSULE: return collide_capsule_capsule(a, b, maxc, c);
     case CLS_CAPSULE * 8 + CLS_PLANE: return collide_capsule_plane(a, b, maxc, c);
+    case CLS_CYLINDER * 8 + CLS_PLANE: return collide_cylinder_plane(a, b, maxc, c);
     default: return -1;
     }
 }

@@ -154,7 +154,7 @@ odxLarge *odxLargeCreate(int nb, const float *bodies, int ng, const float *geoms
          W->alloc(&L.contact_cnt, (size_t)pair_cap + 1) && W->alloc(&L.row_cnt, (size_t)contact_cap + 1) &&
          W->alloc(&L.counters, LG_NCOUNTERS) && W->alloc(&L.cbody, contact_cap) && W->alloc(&L.colour, contact_cap) && W->alloc(&L.taken, nb) &&
          W->alloc(&L.owner, nb) && W->alloc(&L.elig, LG_MAXCOL * LG_MAXSUB) && W->alloc(&L.col_start, LG_MAXCOL + 1) &&
-         W->alloc(&L.corder, contact_cap) && W->alloc(&L.items, contact_cap) && W->alloc(&L.bar, LG_BAR_WORDS) && W->alloc(&L.fc, (size_t)nb * 8);
+         W->alloc(&L.corder, contact_cap) && W->alloc(&L.items, contact_cap) && W->alloc(&L.bar, LG_BAR_WORDS) && W->alloc(&L.fc, (size_t)nb * 8) && W->alloc(&L.ftot, (size_t)nb * 2);
     const size_t scan_cap = std::max({(size_t)hsize + 1, (size_t)ng + 1, (size_t)pair_cap + 1, (size_t)contact_cap + 1});
     ok = ok && W->alloc(&L.scan_tmp, (scan_cap + SCAN_BLOCK - 1) / SCAN_BLOCK);
     ok = ok && W->alloc(&W->sticky, LG_NCOUNTERS);

@@ -54,6 +54,7 @@ struct LargeDev {
     int4 *items;                      // [contact_cap] per slot: first row, first-pass rows | rows << 8, body 1, body 2
     unsigned *bar;                    // [LG_BAR_WORDS] grid barrier counters, colour fill counters
     float *fc;                        // [nb * 8] body accumulators of the solve (6 used)
+    float4 *ftot;                     // [2 * nb] total force, then total torque, of the body stage
     int *scan_tmp;                    // block sums of the scans
     int spin_limit;                   // polls of one barrier wait before it fails (LG_SPIN_LIMIT)
 };

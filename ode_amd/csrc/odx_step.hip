@@ -61,7 +61,11 @@ __device__ __forceinline__ float wave_max(float v) {
 }
 
 // ------------------------------------------------------------ body stage --
-__device__ void body_prep(const DevWorlds &D, const StepParams &P, int b) {
+// The total force and torque (accumulators, gravity, gyroscopic torque) go to
+// fout / tout: the batched and single-world paths pass the accumulators
+// themselves, the large world arrays of its own, so that a step that fails
+// after this stage leaves every body record as it was.
+__device__ void body_prep(const DevWorlds &D, const StepParams &P, int b, float4 *fout, float4 *tout) {
     float4 p4 = D.pos[b], q4 = D.quat[b], lv = D.lvel[b], av = D.avel[b], fa = D.facc[b], ta = D.tacc[b];
     float4 ms = D.mass[b];
     const float mass = ms.x, im = ms.y;
@@ -102,8 +106,8 @@ __device__ void body_prep(const DevWorlds &D, const StepParams &P, int b) {
             tq[0] += t0[0]; tq[1] += t0[1]; tq[2] += t0[2];
         }
     }
-    D.facc[b] = make_float4(f[0], f[1], f[2], 0.f);
-    D.tacc[b] = make_float4(tq[0], tq[1], tq[2], 0.f);
+    fout[b] = make_float4(f[0], f[1], f[2], 0.f);
+    tout[b] = make_float4(tq[0], tq[1], tq[2], 0.f);
     // Stage2b: rhs_tmp = -(v/h + M^-1 f)
     const float hr = 1.f / P.h;
     float ra[3];
@@ -681,7 +685,7 @@ __device__ __forceinline__ int prepare_world(const DevWorlds &D, const StepParam
     if (pt && t == 0) pt[0] = (long long)wall_clock64();
 
     // S1: bodies
-    for (int i = t; i < nb; i += 64) body_prep(D, P, b0 + i);
+    for (int i = t; i < nb; i += 64) body_prep(D, P, b0 + i, D.facc, D.tacc);
     __syncthreads();
     if (pt && t == 0) pt[1] = (long long)wall_clock64();
 
@@ -1402,11 +1406,12 @@ __global__ __launch_bounds__(1024) void k_order_worlds(const int *niters, int wb
 // (angular speed cap, x += h v, q += h dq(w, q), normalised, damping;
 // f = t = 0) of body b, quickstep.cpp:3082-3102, 3299-3337,
 // util.cpp:583-699 (infinitesimal rotation).
+// fin / tin: the total force and torque body_prep wrote.
 __device__ __forceinline__ void integrate_body(const DevWorlds &D, const StepParams &P, int b, const float *fc,
-                                               bool rows) {
+                                               bool rows, const float4 *fin, const float4 *tin) {
     const float h = P.h;
     const float *s = D.bscr + (size_t)b * BSCR_STRIDE;
-    float4 p4 = D.pos[b], q4 = D.quat[b], lv = D.lvel[b], av = D.avel[b], fa = D.facc[b], ta = D.tacc[b];
+    float4 p4 = D.pos[b], q4 = D.quat[b], lv = D.lvel[b], av = D.avel[b], fa = fin[b], ta = tin[b];
     if (rows) {
         lv.x += h * fc[0]; lv.y += h * fc[1]; lv.z += h * fc[2];
         av.x += h * fc[3]; av.y += h * fc[4]; av.z += h * fc[5];
@@ -1645,7 +1650,7 @@ __device__ __forceinline__ void solve_world(const DevWorlds &D, const StepParams
         float fc[6];
         if (lds) lds_fc(s_body, i, fc);
         else for (int k = 0; k < 6; k++) fc[k] = s_fc[i * 6 + k];
-        integrate_body(D, P, b0 + i, fc, m > 0);
+        integrate_body(D, P, b0 + i, fc, m > 0, D.facc, D.tacc);
     }
     if (t == 0 && D.wtime) D.wtime[2 * w + 1] = (long long)wall_clock64();
 }
@@ -1843,7 +1848,7 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
     int *isl_b = P.dyn ? reinterpret_cast<int *>(fa + nb * 2) : nullptr;
     int *ist = P.dyn ? isl_b + nb : nullptr;
     float *ilim = P.dyn ? reinterpret_cast<float *>(ist + nisl * 4) : nullptr;
-    for (int i = t; i < nb; i += NT) body_prep(D, P, i);
+    for (int i = t; i < nb; i += NT) body_prep(D, P, i, D.facc, D.tacc);
     for (int k = t; k < nb * 6; k += NT) s_fc[k] = 0.f;
     if (STAGED) {
         int *ss = reinterpret_cast<int *>(s_dyn + nb * 6 + m);
@@ -2053,7 +2058,7 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
         }
     }
     __syncthreads();
-    for (int i = t; i < nb; i += NT) integrate_body(D, P, i, s_fc + i * 6, m > 0);
+    for (int i = t; i < nb; i += NT) integrate_body(D, P, i, s_fc + i * 6, m > 0, D.facc, D.tacc);
 }
 
 // -------------------------------------------- collision entry points -------
@@ -2268,8 +2273,9 @@ void lg_scan(int *data, int cap, const int *dn, int *sums, hipStream_t s) {
     hipLaunchKernelGGL(k_lg_scan_add, dim3(nblk), dim3(1024), 0, s, data, cap, dn, sums);
 }
 
-__global__ void k_lg_bodies(DevWorlds D, StepParams P, int nb) {
-    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) body_prep(D, P, b);
+__global__ void k_lg_bodies(DevWorlds D, StepParams P, int nb, float4 *ftot) {
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x)
+        body_prep(D, P, b, ftot, ftot + nb);
 }
 
 // AABBs; each finite geom gets the cell of its AABB centre and counts into
@@ -2792,6 +2798,9 @@ constexpr int LG_TAIL = 256;
 // slots to the first workgroup's threads, in the sweeps and at the start.
 __global__ __launch_bounds__(LG_BLOCK) void k_lg_solve(DevWorlds D, StepParams P, LargeDev L, int iters) {
     __shared__ int s_cs[LG_MAXCOL + 1];
+    // a colouring that gave up on a barrier leaves no schedule: the step
+    // changes no body (the next sync reports it failed)
+    if (L.counters[LG_BARRIER_ERR]) return;
     unsigned gen = 0;
     const int nc = L.counters[LG_CONTACTS];
     const int ncol = L.counters[LG_NCOL];
@@ -2850,7 +2859,7 @@ __global__ __launch_bounds__(LG_BLOCK) void k_lg_solve(DevWorlds D, StepParams P
     for (int b = gt; b < L.nb; b += gs) {
         float f[6];
         lg_fc_load(rs, b, f);
-        integrate_body(D, P, b, f, nc > 0);
+        integrate_body(D, P, b, f, nc > 0, L.ftot, L.ftot + L.nb);
     }
 }
 
@@ -2882,7 +2891,7 @@ void lg_enqueue_prepare(const DevWorlds &D, const StepParams &P, const LargeDev 
     const int G = 1024, T = 256;
     hipLaunchKernelGGL(k_lg_fill, dim3(G), dim3(T), 0, s, L.bucket, L.hmask + 2, 0);
     hipLaunchKernelGGL(k_lg_fill, dim3(1), dim3(T), 0, s, L.counters, LG_NCOUNTERS, 0);
-    hipLaunchKernelGGL(k_lg_bodies, dim3(G), dim3(T), 0, s, D, P, L.nb);
+    hipLaunchKernelGGL(k_lg_bodies, dim3(G), dim3(T), 0, s, D, P, L.nb, L.ftot);
     hipLaunchKernelGGL(k_lg_aabb, dim3(G), dim3(T), 0, s, D, L);
     lg_scan(L.bucket, L.hmask + 2, nullptr, L.scan_tmp, s);
     hipLaunchKernelGGL(k_lg_scatter, dim3(G), dim3(T), 0, s, L);

@@ -57,6 +57,7 @@ def lib():
         L.odxBatchSync.argtypes = [vp]
         L.odxBatchTime.argtypes = [vp, i, fp]
         L.odxBatchKernelTimes.argtypes = [vp, i, fp]
+        L.odxBatchScheduleKernelTimes.argtypes = [vp, i, fp, ip]
         L.odxBatchGetBodies.argtypes = [vp, fp]
         L.odxBatchSetBodies.argtypes = [vp, fp]
         L.odxBatchGetStats.argtypes = [vp, ip, ip, ip, fp]
@@ -243,6 +244,16 @@ class Batch:
         out = np.zeros(3, np.float32)
         _check(lib().odxBatchKernelTimes(self.h, n, _p(out)), "odxBatchKernelTimes")
         return float(out[0]), float(out[1]), float(out[2])
+
+    def schedule_kernel_times(self, n):
+        """Per world range of the current schedule: (w0, w1, [prepare,
+        inference, solve] mean milliseconds per launch), timed on the range's
+        own stream."""
+        ms = np.zeros(3 * 8, np.float32)
+        rg = np.zeros(2 * 8, np.int32)
+        k = lib().odxBatchScheduleKernelTimes(self.h, n, _p(ms), _p(rg))
+        _check(k, "odxBatchScheduleKernelTimes")
+        return [(int(rg[2 * r]), int(rg[2 * r + 1]), [float(x) for x in ms[3 * r:3 * r + 3]]) for r in range(k)]
 
     def bodies(self):
         out = np.zeros((self.nbodies, BODY), np.float32)

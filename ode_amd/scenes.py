@@ -5,7 +5,7 @@ include/odx.h:
   BODY [32] float: 0-2 pos, 4-7 q (w,x,y,z), 8-10 lvel, 12-14 avel,
                    16-18 facc, 20-22 tacc, 24 mass, 25-30 I (xx,yy,zz,xy,xz,yz),
                    31 flags (bit0 no gravity, bit1 gyroscopic)
-  GEOM [16] float: 0 class (0 sphere, 1 box, 2 capsule, 4 plane),
+  GEOM [16] float: 0 class (0 sphere, 1 box, 2 capsule, 3 cylinder, 4 plane),
                    1 body (-1 static), 2-5 params, 6-8 static pos, 9-12 static q
 
 Mass properties follow dMassSetBox / dMassSetSphere / dMassSetCapsule
@@ -17,7 +17,7 @@ import numpy as np
 
 BODY = 32
 GEOM = 16
-SPHERE, BOX, CAPSULE, PLANE = 0, 1, 2, 4
+SPHERE, BOX, CAPSULE, CYLINDER, PLANE = 0, 1, 2, 3, 4
 FLAG_NO_GRAVITY = 1
 FLAG_GYRO = 2
 
@@ -54,6 +54,15 @@ def capsule_mass(density, r, length, direction=3):
     d = [ia, ia, ia]
     d[direction - 1] = ib
     return m1 + m2, tuple(d)
+
+
+def cylinder_mass(density, r, length, direction=3):
+    """dMassSetCylinder (ode/src/mass.cpp:173-198)."""
+    m = math.pi * r * r * length * density
+    ia = m * (0.25 * r * r + (1.0 / 12.0) * length * length)
+    d = [ia, ia, ia]
+    d[direction - 1] = m * 0.5 * r * r
+    return m, tuple(d)
 
 
 def geom(cls, body, params, pos=(0, 0, 0), q=(1, 0, 0, 0)):

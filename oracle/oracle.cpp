@@ -184,12 +184,12 @@ template <class R> bool invertPD3(R *out /*3x4*/, const R *I /*3x4*/) {
 }
 
 const double kInf = std::numeric_limits<double>::infinity();
-int g_last_sweeps = 0;
+thread_local int g_last_sweeps = 0;  /* per calling thread (CPU baselines step worlds on several threads) */
 unsigned int g_rand_seed = 0;  /* dRand's global state (misc.cpp:35) */
 extern "C" int orc_drand_int(unsigned int *seed, int n);
 
 /* ------------------------------------------------------------- records -- */
-enum { CLS_SPHERE = 0, CLS_BOX = 1, CLS_CAPSULE = 2, CLS_PLANE = 4 };
+enum { CLS_SPHERE = 0, CLS_BOX = 1, CLS_CAPSULE = 2, CLS_CYLINDER = 3, CLS_PLANE = 4 };
 
 template <class R> struct Pose { R pos[4]; R Rm[12]; };
 
@@ -245,6 +245,15 @@ template <class R> void geom_aabb(const GeomView<R> &g, R *aabb) {
         R xr = std::fabs(Rm[2] * lz) * R(0.5) + r;
         R yr = std::fabs(Rm[6] * lz) * R(0.5) + r;
         R zr = std::fabs(Rm[10] * lz) * R(0.5) + r;
+        aabb[0] = p[0] - xr; aabb[1] = p[0] + xr; aabb[2] = p[1] - yr; aabb[3] = p[1] + yr; aabb[4] = p[2] - zr; aabb[5] = p[2] + zr;
+        break;
+    }
+    case CLS_CYLINDER: {  /* cylinder.cpp:61-78 */
+        R r = g.p[0], lz = g.p[1];
+        R o2 = R(1) - Rm[2] * Rm[2], o6 = R(1) - Rm[6] * Rm[6], o10 = R(1) - Rm[10] * Rm[10];
+        R xr = std::fabs(Rm[2] * lz * R(0.5)) + r * std::sqrt(R(0) > o2 ? R(0) : o2);
+        R yr = std::fabs(Rm[6] * lz * R(0.5)) + r * std::sqrt(R(0) > o6 ? R(0) : o6);
+        R zr = std::fabs(Rm[10] * lz * R(0.5)) + r * std::sqrt(R(0) > o10 ? R(0) : o10);
         aabb[0] = p[0] - xr; aabb[1] = p[0] + xr; aabb[2] = p[1] - yr; aabb[3] = p[1] + yr; aabb[4] = p[2] - zr; aabb[5] = p[2] + zr;
         break;
     }
@@ -873,6 +882,85 @@ template <class R> int collide_capsule_plane(const GeomView<R> &cp, const GeomVi
     return nc;
 }
 
+/* dCollideCylinderPlane collision_cylinder_plane.cpp:42-250.  A disc parallel
+ * to the plane (axis within the tolerance of the normal): the nearer disc's
+ * centre +-V1, +-V2 (two radius vectors built by the reference's cross
+ * products), each kept while depth > 0.  Otherwise the rim point of each disc
+ * deepest along -n, kept while depth >= 0. */
+template <class R> int collide_cylinder_plane(const GeomView<R> &cy, const GeomView<R> &pl, int flags, Contact<R> *c) {
+    const R tol = sizeof(R) == 4 ? R(0.0001) : R(0.0000001);
+    const int maxc = flags & 0xffff;
+    const R radius = cy.p[0], length = cy.p[1];
+    const R *cp = cy.pose.pos, *Rm = cy.pose.Rm, *pv = pl.p;
+    const R dir[3] = {Rm[2], Rm[6], Rm[10]};
+    R s = length * R(0.5);
+    R e1[3], e2[3];
+    for (int i = 0; i < 3; i++) {
+        e2[i] = dir[i] * s + cp[i];
+        e1[i] = dir[i] * -s + cp[i];
+    }
+    auto dot = [](const R *a, const R *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    auto cross = [](R *r, const R *a, const R *b) {
+        const R r0 = a[1] * b[2] - a[2] * b[1], r1 = a[2] * b[0] - a[0] * b[2], r2 = a[0] * b[1] - a[1] * b[0];
+        r[0] = r0; r[1] = r1; r[2] = r2;
+    };
+    int n = 0;
+    s = dir[0] * pv[0] + dir[1] * pv[1] + dir[2] * pv[2];
+    if (s < 0) s += R(1); else s -= R(1);
+    if (s < tol && s > -tol) {
+        const R s1 = pv[3] - dot(pv, e1), t1 = pv[3] - dot(pv, e2);
+        const R *P;
+        if (s1 >= t1) {
+            if (!(s1 >= 0)) return 0;
+            P = e1;
+        } else {
+            if (!(t1 >= 0)) return 0;
+            P = e2;
+        }
+        R V1[3], V2[3];
+        if (dir[0] < tol && dir[0] > -tol) { V1[0] = dir[0] + R(1); V1[1] = dir[1]; V1[2] = dir[2]; }
+        else { V1[0] = dir[0]; V1[1] = dir[1] + R(1); V1[2] = dir[2]; }
+        cross(V2, V1, dir);
+        R t = std::sqrt(V2[0] * V2[0] + V2[1] * V2[1] + V2[2] * V2[2]);
+        t = radius / t;
+        for (int i = 0; i < 3; i++) V2[i] *= t;
+        cross(V1, V2, dir);
+        const R *off[4] = {V1, V1, V2, V2};
+        const R sgn[4] = {R(1), R(-1), R(1), R(-1)};
+        for (int k = 0; k < 4; k++) {
+            R q[3];
+            for (int i = 0; i < 3; i++) q[i] = sgn[k] > 0 ? P[i] + off[k][i] : P[i] - off[k][i];
+            const R depth = pv[3] - dot(pv, q);
+            if (depth > 0) {
+                for (int i = 0; i < 3; i++) { c[n].pos[i] = q[i]; c[n].normal[i] = pv[i]; }
+                c[n].depth = depth;
+                if (++n >= maxc) return n;
+            }
+        }
+    } else {
+        const R t = dot(pv, dir);
+        R C[3] = {dir[0] * t - pv[0], dir[1] * t - pv[1], dir[2] * t - pv[2]};
+        R l = std::sqrt(C[0] * C[0] + C[1] * C[1] + C[2] * C[2]);
+        l = radius / l;
+        for (int i = 0; i < 3; i++) C[i] *= l;
+        R q[3] = {C[0] + e1[0], C[1] + e1[1], C[2] + e1[2]};
+        R depth = pv[3] - dot(pv, q);
+        if (depth >= 0) {
+            for (int i = 0; i < 3; i++) { c[n].pos[i] = q[i]; c[n].normal[i] = pv[i]; }
+            c[n].depth = depth;
+            if (++n >= maxc) return n;
+        }
+        for (int i = 0; i < 3; i++) q[i] = C[i] + e2[i];
+        depth = pv[3] - pv[0] * q[0] - pv[1] * q[1] - pv[2] * q[2];
+        if (depth >= 0) {
+            for (int i = 0; i < 3; i++) { c[n].pos[i] = q[i]; c[n].normal[i] = pv[i]; }
+            c[n].depth = depth;
+            ++n;
+        }
+    }
+    return n;
+}
+
 /* Collider for (cls1, cls2) without reversal; returns -1 when no function exists. */
 template <class R> int collide_fwd(const GeomView<R> &a, const GeomView<R> &b, int flags, Contact<R> *c) {
     if (a.cls == CLS_SPHERE && b.cls == CLS_SPHERE) return collide_spheres(a.pose.pos, a.p[0], b.pose.pos, b.p[0], c);
@@ -884,6 +972,7 @@ template <class R> int collide_fwd(const GeomView<R> &a, const GeomView<R> &b, i
     if (a.cls == CLS_CAPSULE && b.cls == CLS_BOX) return collide_capsule_box(a, b, flags, c);
     if (a.cls == CLS_CAPSULE && b.cls == CLS_CAPSULE) return collide_capsule_capsule(a, b, flags, c);
     if (a.cls == CLS_CAPSULE && b.cls == CLS_PLANE) return collide_capsule_plane(a, b, flags, c);
+    if (a.cls == CLS_CYLINDER && b.cls == CLS_PLANE) return collide_cylinder_plane(a, b, flags, c);
     return -1;
 }
 

@@ -115,6 +115,62 @@ def test_single_step_matches_oracle_reference_order(seed, damped):
     B.close()
 
 
+def test_cylinders_on_a_plane_match_oracle():
+    # cylinders standing on a cap, lying on their side and tilted, sunk a
+    # little into the plane, among spheres: per world the float32 oracle's
+    # contacts exactly (count and pairs) and its geometry, the float64
+    # oracle's impulses and states after one step
+    say("case: cylinders on a plane")
+    rng = np.random.default_rng(21)
+    worlds = []
+    for w in range(8):
+        bodies, geoms = [], []
+        for k in range(6):
+            r, L = rng.uniform(0.1, 0.3), rng.uniform(0.3, 1.0)
+            m, I = S.cylinder_mass(1000.0, r, L)
+            kind = k % 3
+            if kind == 0:
+                q, z = (1, 0, 0, 0), L / 2 - 0.01
+            elif kind == 1:
+                q, z = (np.cos(np.pi / 4), 0, np.sin(np.pi / 4), 0), r - 0.01
+            else:
+                q = rng.normal(size=4)
+                z = 0.5 * L
+            x, y = 1.5 * (k - 2.5), rng.uniform(-0.5, 0.5)
+            bodies.append(S._body((x, y, z), q, m, I))
+            geoms.append(S.geom(S.CYLINDER, k, (r, L)))
+        m, I = S.sphere_mass(1000.0, 0.2)
+        bodies.append(S._body((0.0, 2.0, 0.19), (1, 0, 0, 0), m, I))
+        geoms.append(S.geom(S.SPHERE, 6, (0.2,)))
+        geoms.append(S.plane(0, 0, 1, 0))
+        worlds.append(S.World(bodies, geoms))
+    prm = O.params()
+    B = run_one_step(worlds, prm)
+    gb = B.bodies()
+    st = B.stats()
+    assert B.overflow().sum() == 0
+    total = 0
+    for wi, w in enumerate(worlds):
+        _, ref32, _ = O.world_step(w.bodies, w.geoms, prm, dtype=np.float32)
+        ref_b, ref_c, ref_l = O.world_step(w.bodies, w.geoms, prm, dtype=np.float64)
+        gj = B.joints(wi)
+        assert st["contacts"][wi] == len(ref32) == len(ref_c), wi
+        np.testing.assert_array_equal(gj[:, 14:16].astype(int), ref32[:, 7:9].astype(int))
+        rj = oracle_joints(ref32, w.geoms)
+        assert np.max(np.abs(gj[:, 3:10] - rj[:, 3:10])) < 2e-6
+        lam = B.lam(wi)
+        assert relerr(lam, ref_l) < TOL_LAMBDA, "lambda world %d: %g" % (wi, relerr(lam, ref_l))
+        wb = B.world_bodies(wi, gb)
+        assert relerr(wb[:, 0:3], ref_b[:, 0:3]) < TOL_STATE
+        vs = np.hstack([wb[:, 8:11], wb[:, 12:15]])
+        vr = np.hstack([ref_b[:, 8:11], ref_b[:, 12:15]])
+        assert np.max(np.abs(vs - vr)) / max(np.max(np.abs(vr)), 1.0) < TOL_STATE
+        total += len(ref_c)
+    say("  contacts", total)
+    assert total > 8 * 10
+    B.close()
+
+
 def test_towers_single_step_and_rows():
     say("case: towers single step")
     rng = np.random.default_rng(7)

@@ -265,10 +265,14 @@ def test_barrier_timeout_fails_the_step_and_the_world_recovers():
     L = make(w, prm)
     L.step(3)
     L.sync()
+    before = L.bodies()
     L.set_spin_limit(0)
     L.step(1)
     with pytest.raises(odx.OdxError, match="barrier 1"):
         L.sync()
+    # the failed step changed no body record (positions, velocities and the
+    # force accumulators)
+    assert np.array_equal(L.bodies(), before)
     L.set_spin_limit()
     L.step(2)
     L.sync()

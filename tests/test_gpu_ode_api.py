@@ -421,6 +421,67 @@ def test_dynamic_iterations_exit_per_island_at_the_closed_form_sweep():
     np.testing.assert_allclose(st_both[1, :15], st_fall[0, :15], rtol=0, atol=1e-6)
 
 
+def test_dynamic_iteration_statistics_sink_counts_exits_per_island():
+    # two resting spheres, two islands.  With w = 1.3 and the premature delta
+    # 1e-4 each stops after 11 of 20 sweeps: two premature exits per step.
+    # With both deltas 0 no sweep meets the requirement (adjustment < 0) and
+    # each uses all 20 extra sweeps.  With w = 1.9 the sweep-k adjustment of a
+    # resting sphere, g w |1 - w|^(k-1), decays slowly enough to stay exact in
+    # float32; a requirement between its values at sweeps 24 and 25 stops both
+    # inside the extra sweeps (objects.h:538-576, quickstep.cpp:1829-1853)
+    say("case: dynamic iteration statistics sink")
+    from ode_amd import ode as A
+    m, r = 10.0, 0.5
+    I = 0.4 * m * r * r
+
+    def sphere(x):
+        b = np.zeros(32)
+        b[0:3] = (x, 0, r)
+        b[4] = 1.0
+        b[24], b[25:28], b[31] = m, (I, I, I), 2
+        return b
+
+    class Stats(ctypes.Structure):
+        _fields_ = [("struct_size", ctypes.c_uint), ("iteration_count", ctypes.c_uint),
+                    ("premature_exits", ctypes.c_uint), ("prolonged_execs", ctypes.c_uint),
+                    ("full_extra_execs", ctypes.c_uint)]
+
+    adj = lambda k, w: 9.81 * w * abs(1 - w) ** (k - 1)
+    # (the closed form holds for the first step from rest: the third case
+    # runs that step only)
+    cases = [((1e-4, 1.0, 1e-2), 1.3, (2, 0, 0), 3),
+             ((0.0, 1.0, 0.0), 1.3, (0, 0, 2), 3),
+             ((0.0, 1.0, 0.5 * (adj(24, 1.9) + adj(25, 1.9))), 1.9, (0, 2, 0), 1)]
+    for (pre, fac, xd), w, want, steps in cases:
+        F = Frame(S.World([sphere(-3), sphere(3)], [S.geom(S.SPHERE, 0, (r,)), S.geom(S.SPHERE, 1, (r,)),
+                                                   S.plane(0, 0, 1, 0)]), dynamic=True)
+        L = F.L
+        L.dWorldAttachQuickStepDynamicIterationStatisticsSink.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        st = Stats()
+        st.struct_size = ctypes.sizeof(Stats) - 4   # too small: refused
+        assert L.dWorldAttachQuickStepDynamicIterationStatisticsSink(F.w, ctypes.byref(st)) == 0
+        st.struct_size = ctypes.sizeof(Stats)
+        assert L.dWorldAttachQuickStepDynamicIterationStatisticsSink(F.w, ctypes.byref(st)) == 1
+        L.dWorldSetQuickStepW(F.w, w)
+        p, f, x = A.dReal(pre), A.dReal(fac), A.dReal(xd)
+        L.dWorldSetQuickStepDynamicIterationParameters(F.w, ctypes.byref(p), ctypes.byref(f), ctypes.byref(x))
+        for step in range(steps):
+            F.collide()
+            assert L.dWorldQuickStep(F.w, 0.01) == 1
+            L.dJointGroupEmpty(F.group)
+        got = (st.premature_exits, st.prolonged_execs, st.full_extra_execs)
+        say("  parameters", (pre, fac, xd), "statistics", got)
+        assert got == tuple(steps * k for k in want)
+        assert st.iteration_count == 0
+        # detached: the world counts into its own statistics again
+        assert L.dWorldAttachQuickStepDynamicIterationStatisticsSink(F.w, None) == 1
+        F.collide()
+        assert L.dWorldQuickStep(F.w, 0.01) == 1
+        L.dJointGroupEmpty(F.group)
+        assert (st.premature_exits, st.prolonged_execs, st.full_extra_execs) == got
+        F.close()
+
+
 @pytest.mark.parametrize("dynamic", [False, True])
 def test_random_reordering_matches_oracle_and_draws_as_the_reference(dynamic):
     # row order 2: the reference's default build; the same dRand seed in the

@@ -807,3 +807,76 @@ def test_sor_iterates_of_a_sliding_sphere_clamp_friction_to_the_normal():
         np.testing.assert_allclose(lam, want[k - 1], rtol=1e-11, atol=1e-11)
     # the friction rows sit on their bound +-mu * normal after the first sweep
     assert abs(abs(want[0][1]) - mu * want[0][0]) < 1e-12 or abs(abs(want[0][2]) - mu * want[0][0]) < 1e-12
+
+
+# ------------------------------------------------------------ cylinder-plane
+def test_cylinder_aabb_closed_form():
+    g = S.geom(S.CYLINDER, 0, (0.3, 1.2))
+    ab = O.aabbs(np.array([g]), np.array([body((1, 2, 3))]), np.float64)[0]
+    np.testing.assert_allclose(ab, (0.7, 1.3, 1.7, 2.3, 2.4, 3.6), atol=1e-12)
+    th = 0.6  # axis tilted by th about x: extents L/2 |cos|, |sin| and r sin, cos
+    ab = O.aabbs(np.array([g]), np.array([body((0, 0, 0), rotq((1, 0, 0), th))]), np.float64)[0]
+    want = (0.3, 0.6 * abs(math.sin(th)) + 0.3 * math.cos(th), 0.6 * math.cos(th) + 0.3 * math.sin(th))
+    np.testing.assert_allclose(ab[1::2], want, atol=1e-12)
+    np.testing.assert_allclose(ab[0::2], -np.array(want), atol=1e-12)
+
+
+def test_cylinder_standing_on_its_cap_gives_four_rim_points():
+    # axis along +z, sunk by d: the bottom disc gives the rim points P + V1,
+    # P - V1, P + V2, P - V2 with V1 = (-r, 0, 0), V2 = (0, -r, 0) (the
+    # reference's construction for an axis with x-component 0), depth d
+    r, L, d = 0.3, 1.0, 0.01
+    g = S.geom(S.CYLINDER, 0, (r, L))
+    c = O.collide(g, S.plane(0, 0, 1, 0), body((0.5, -0.2, L / 2 - d)), maxc=4)
+    assert len(c) == 4
+    want = [(0.5 - r, -0.2), (0.5 + r, -0.2), (0.5, -0.2 - r), (0.5, -0.2 + r)]
+    np.testing.assert_allclose(c[:, 0:2], want, atol=1e-12)
+    np.testing.assert_allclose(c[:, 2], -d, atol=1e-12)
+    np.testing.assert_allclose(c[:, 6], d, atol=1e-12)
+    np.testing.assert_allclose(c[:, 3:6], np.tile((0, 0, 1), (4, 1)))
+    assert len(O.collide(g, S.plane(0, 0, 1, 0), body((0.5, -0.2, L / 2 - d)), maxc=2)) == 2
+    # lifted clear of the plane: no contact; upside down: the other disc
+    assert len(O.collide(g, S.plane(0, 0, 1, 0), body((0, 0, L / 2 + d)), maxc=4)) == 0
+    c = O.collide(g, S.plane(0, 0, 1, 0), body((0, 0, L / 2 - d), rotq((1, 0, 0), math.pi)), maxc=4)
+    assert len(c) == 4
+    np.testing.assert_allclose(c[:, 6], d, atol=1e-12)
+
+
+def test_cylinder_on_its_side_and_tilted_give_each_discs_lowest_rim_point():
+    r, L = 0.25, 0.8
+    g = S.geom(S.CYLINDER, 0, (r, L))
+    # lying along x, sunk by d: the lowest rim point of each disc, disc -x first
+    d = 0.02
+    c = O.collide(g, S.plane(0, 0, 1, 0), body((0, 0, r - d), rotq((0, 1, 0), math.pi / 2)), maxc=4)
+    assert len(c) == 2
+    np.testing.assert_allclose(sorted(c[:, 0]), (-L / 2, L / 2), atol=1e-12)
+    np.testing.assert_allclose(c[:, 6], d, atol=1e-12)
+    np.testing.assert_allclose(c[:, 2], -d, atol=1e-12)
+    # generic tilt: the rim point of a disc deepest along -n is
+    # centre - r (n - (n.a) a) / |n - (n.a) a|
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        a = rotm(q)[:, 2]
+        if abs(a[2]) > 0.99:
+            continue
+        pos = np.array([0.1, -0.3, 0.0])
+        ends = [pos - a * L / 2, pos + a * L / 2]
+        n = np.array([0, 0, 1.0])
+        u = n - (n @ a) * a
+        low = [e - r * u / np.linalg.norm(u) for e in ends]
+        pos[2] = -min(p[2] for p in low) - 0.5 * abs(low[0][2] - low[1][2])  # one disc in, one out
+        low = [p + (0, 0, pos[2]) for p in low]
+        want = [p for p in low if -p[2] >= 0]
+        c = O.collide(g, S.plane(0, 0, 1, 0), body(pos, q), maxc=4)
+        assert len(c) == len(want) == 1
+        np.testing.assert_allclose(c[0, 0:3], want[0], atol=1e-9)
+        assert c[0, 6] == pytest.approx(-want[0][2], abs=1e-9)
+        pos[2] -= 1.0  # both discs in: both points, disc -a first
+        c = O.collide(g, S.plane(0, 0, 1, 0), body(pos, q), maxc=4)
+        assert len(c) == 2
+        np.testing.assert_allclose(c[:, 0:3], [p - (0, 0, 1.0) for p in low], atol=1e-9)
+    # the reversed pair negates the normal (dCollide)
+    c = O.collide(S.plane(0, 0, 1, 0), g, None, body((0, 0, r - d), rotq((0, 1, 0), math.pi / 2)), maxc=4)
+    np.testing.assert_allclose(c[:, 3:6], np.tile((0, 0, -1), (2, 1)))
