@@ -286,6 +286,13 @@ int odxMlpGradients(odxMlp *m, float *grads, float *loss);
 int odxMlpTrain(odxMlp *m, int nsteps, int batch, float lr, unsigned seed, float *losses);
 /* Predicted impulses (force units) for features X[N][16] -> Y[N][3]. */
 int odxMlpPredict(odxMlp *m, const float *X, int N, float *Y);
+/* odxMlpPredict without any allocation (no counterpart in the reference):
+ * for callers that cannot allocate device memory, such as tight serving
+ * loops and harnesses that forbid hipMalloc.  It stages the features and
+ * impulses in a fixed array of 1.25 MiB on each device and accepts at most
+ * 16384 contacts (more is an error, reported through the message handler).
+ * Calls that overlap on one device share that array and race. */
+int odxMlpPredictNoAlloc(odxMlp *m, const float *X, int N, float *Y);
 /* MFMA operand-layout probe: per-lane inputs a,b [64][8] -> d [64][4]. */
 int odxMfmaProbe(const float *a, const float *b, float *d);
 

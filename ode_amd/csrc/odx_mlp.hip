@@ -16,6 +16,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/odx.h"
@@ -543,6 +544,84 @@ int odxMlpPredict(odxMlp *m, const float *X, int N, float *Y) {
     if (dY) (void)hipFree(dY);
     if (drow) (void)hipFree(drow);
     if (dcnt) (void)hipFree(dcnt);
+    return ok ? N : 0;
+}
+
+}  // extern "C"
+
+// ---- odxMlpPredictNoAlloc: the forward pass of odxMlpPredict on a fixed
+// scratch array per device instead of memory allocated per call.
+// At most MLP_NOALLOC_MAX contacts: features (1 MiB), impulses (192 KiB), row
+// indices (64 KiB) and the count, 1.25 MiB held by each device the library
+// runs on until the process ends.  Every call on a device shares its array:
+// calls that overlap on one device race.  Each call writes every entry it
+// reads (the row indices and the count by k_mlp_rows, the features by the
+// copy), so nothing has to start at zero.
+constexpr int MLP_NOALLOC_MAX = 16384;
+constexpr size_t NA_X = 0;
+constexpr size_t NA_Y = NA_X + (size_t)MLP_NOALLOC_MAX * NIN * sizeof(float);
+constexpr size_t NA_ROW = NA_Y + (size_t)MLP_NOALLOC_MAX * NOUT * sizeof(float);
+constexpr size_t NA_CNT = NA_ROW + (size_t)MLP_NOALLOC_MAX * sizeof(int);
+constexpr size_t NA_BYTES = NA_CNT + 256;
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+// identity row map (row i writes impulses 3i..3i+2) and the entry count
+__global__ void k_mlp_rows(int *row, int *cnt, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) row[i] = NOUT * i;
+    if (i == 0) *cnt = n;
+}
+
+namespace {
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_dev[NA_MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (each device holds its own copy of the array); the array's name is
+// a host-side stand-in and is never passed to a kernel or a copy.
+unsigned char *scratch_on_current_device() {
+    int dev = 0;
+    if (!odx_check(hipGetDevice(&dev), "hipGetDevice")) return nullptr;
+    if (dev < 0 || dev >= NA_MAX_DEVICES) {
+        odx_message(4, "odxMlpPredictNoAlloc: device %d is beyond the %d this build tracks", dev, NA_MAX_DEVICES);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        if (!odx_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress")) return nullptr;
+        g_scratch_dev[dev] = static_cast<unsigned char *>(p);
+    }
+    return g_scratch_dev[dev];
+}
+}  // namespace
+
+extern "C" {
+
+int odxMlpPredictNoAlloc(odxMlp *m, const float *X, int N, float *Y) {
+    if (!m || !X || !Y || N <= 0) return 0;
+    if (N > MLP_NOALLOC_MAX) {
+        odx_message(4, "odxMlpPredictNoAlloc: %d contacts exceed the fixed scratch of %d", N, MLP_NOALLOC_MAX);
+        return 0;
+    }
+    unsigned char *base = scratch_on_current_device();
+    if (!base) return 0;
+    float *dX = reinterpret_cast<float *>(base + NA_X), *dY = reinterpret_cast<float *>(base + NA_Y);
+    int *drow = reinterpret_cast<int *>(base + NA_ROW), *dcnt = reinterpret_cast<int *>(base + NA_CNT);
+    bool ok = odx_check(hipMemcpyAsync(dX, X, (size_t)N * NIN * sizeof(float), hipMemcpyHostToDevice, m->stream),
+                        "hipMemcpyAsync");
+    if (ok) {
+        hipLaunchKernelGGL(k_mlp_rows, dim3((N + 255) / 256), dim3(256), 0, m->stream, drow, dcnt, N);
+        ok = odx_check(hipGetLastError(), "k_mlp_rows");
+    }
+    if (ok) {
+        launch_mlp_infer(m, dX, drow, dcnt, dY, N, m->stream);
+        ok = odx_check(hipGetLastError(), "k_mlp_infer") &&
+             odx_check(hipMemcpyAsync(Y, dY, (size_t)N * NOUT * sizeof(float), hipMemcpyDeviceToHost, m->stream),
+                       "hipMemcpyAsync") &&
+             odx_check(hipStreamSynchronize(m->stream), "hipStreamSynchronize");
+    }
     return ok ? N : 0;
 }
 
