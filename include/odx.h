@@ -230,6 +230,12 @@ int odxLargeSetSolveBlocks(odxLarge *w, int per_cu);
  * once); every workgroup still leaves the kernel normally, and the next
  * odxLargeSync reports the failed step. */
 int odxLargeSetSpinLimit(odxLarge *w, int polls);
+/* The near-callback policy of odxBatchSetSurfaceTable for the large world:
+ * table [nmat][nmat][16] as there, geoms' materials in GEOM slot 13; with
+ * category_filter the hash-space pairs whose category / collide bits fail
+ * collideAABBs' test (collision_space_internal.h:50-57) are not formed.
+ * Waits for queued steps.  nmat = 0 returns to the parameters' surface. */
+int odxLargeSetSurfaceTable(odxLarge *w, int nmat, const float *table, int category_filter);
 /* Queues nsteps steps (dSpaceCollide + contacts + dWorldQuickStep +
  * dJointGroupEmpty each) on the world's stream.  Returns 1 when queued. */
 int odxLargeStep(odxLarge *w, int nsteps);

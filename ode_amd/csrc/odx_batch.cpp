@@ -462,7 +462,7 @@ odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies
         for (int g = B->geom_start[w]; g < B->geom_start[w + 1]; g++) {
             const int gb = (int)geoms[(size_t)g * GEOM_STRIDE + 1];
             const int cls = (int)geoms[(size_t)g * GEOM_STRIDE + 0];
-            if (gb < -1 || gb >= nbw || (cls != 0 && cls != 1 && cls != 2 && cls != 4)) {
+            if (gb < -1 || gb >= nbw || (cls < 0 || cls > 4)) {
                 odx_message(2, "odxBatchCreate: geom %d of world %d has class %d / body %d", g, w, cls, gb);
                 delete B;
                 return nullptr;
@@ -566,23 +566,29 @@ int odxBatchSetSolver(odxBatch *b, int order, int max_iters, float tol, int chec
     return 1;
 }
 
-int odxBatchSetSurfaceTable(odxBatch *b, int nmat, const float *table, int category_filter) {
-    if (!b || nmat < 0 || nmat > ODX_MAX_MATERIALS || (nmat > 0 && !table)) {
-        odx_message(1, "odxBatchSetSurfaceTable: bad arguments (0 <= nmat <= %d)", ODX_MAX_MATERIALS);
-        return 0;
+}  // extern "C"
+
+// A material-pair surface table (float[nmat][nmat][16]) checked and
+// converted for the device: symmetric, contact limits within 0..ODX_MAXC,
+// every geom's material inside the table.  `who` names the caller in messages.
+bool odx_surface_table(const char *who, int nmat, const float *table, const std::vector<int> &geom_mat,
+                       std::vector<Surface> &t) {
+    if (nmat < 0 || nmat > ODX_MAX_MATERIALS || (nmat > 0 && !table)) {
+        odx_message(1, "%s: bad arguments (0 <= nmat <= %d)", who, ODX_MAX_MATERIALS);
+        return false;
     }
-    std::vector<Surface> t((size_t)std::max(nmat * nmat, 1));
+    t.assign((size_t)std::max(nmat * nmat, 1), Surface{});
     for (int a = 0; a < nmat; a++)
         for (int c = 0; c < nmat; c++) {
             const float *e = table + ((size_t)a * nmat + c) * 16, *f = table + ((size_t)c * nmat + a) * 16;
             for (int k = 0; k < 13; k++)
                 if (!(e[k] == f[k]) && !(e[k] != e[k] && f[k] != f[k])) {
-                    odx_message(1, "odxBatchSetSurfaceTable: entry (%d, %d) differs from (%d, %d)", a, c, c, a);
-                    return 0;
+                    odx_message(1, "%s: entry (%d, %d) differs from (%d, %d)", who, a, c, c, a);
+                    return false;
                 }
             if (e[12] < 0 || e[12] > ODX_MAXC) {
-                odx_message(1, "odxBatchSetSurfaceTable: max contacts of (%d, %d) outside 0..%d", a, c, ODX_MAXC);
-                return 0;
+                odx_message(1, "%s: max contacts of (%d, %d) outside 0..%d", who, a, c, ODX_MAXC);
+                return false;
             }
             Surface &s = t[(size_t)a * nmat + c];
             s.mode = (int)e[0];
@@ -591,12 +597,20 @@ int odxBatchSetSurfaceTable(odxBatch *b, int nmat, const float *table, int categ
             s.motion1 = e[7]; s.motion2 = e[8]; s.motionN = e[9]; s.slip1 = e[10]; s.slip2 = e[11];
             s.max_contacts = (int)e[12];
         }
-    for (int g = 0; g < b->ng; g++)
-        if (nmat > 0 && (b->geom_mat[g] < 0 || b->geom_mat[g] >= nmat)) {
-            odx_message(1, "odxBatchSetSurfaceTable: geom %d has material %d, outside the table's %d", g,
-                        b->geom_mat[g], nmat);
-            return 0;
+    for (size_t g = 0; g < geom_mat.size(); g++)
+        if (nmat > 0 && (geom_mat[g] < 0 || geom_mat[g] >= nmat)) {
+            odx_message(1, "%s: geom %d has material %d, outside the table's %d", who, (int)g, geom_mat[g], nmat);
+            return false;
         }
+    return true;
+}
+
+extern "C" {
+
+int odxBatchSetSurfaceTable(odxBatch *b, int nmat, const float *table, int category_filter) {
+    if (!b) return 0;
+    std::vector<Surface> t;
+    if (!odx_surface_table("odxBatchSetSurfaceTable", nmat, table, b->geom_mat, t)) return 0;
     // no step in flight may still read the table being replaced
     if (!odx_check(hipDeviceSynchronize(), "hipDeviceSynchronize")) return 0;
     if (b->surf_table) { (void)hipFree(b->surf_table); b->surf_table = nullptr; }

@@ -16,3 +16,7 @@ bool odx_state_upload(odx::DevWorlds &D, int nb, const float *bodies, int ng, co
                       std::vector<void *> &allocs);
 bool odx_bodies_download(const odx::DevWorlds &D, int nb, float *out);
 bool odx_bodies_upload(const odx::DevWorlds &D, int nb, const float *bodies);
+// a material-pair surface table (float[nmat][nmat][16]) checked against the
+// geoms' materials and converted for the device; false (with a message) if invalid
+bool odx_surface_table(const char *who, int nmat, const float *table, const std::vector<int> &geom_mat,
+                       std::vector<odx::Surface> &t);

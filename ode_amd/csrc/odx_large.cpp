@@ -34,6 +34,9 @@ struct odxLarge {
     DevWorlds D{};
     StepParams P{};
     Surface surf{};
+    Surface *surf_default = nullptr;  // device copy of the parameters' surface
+    Surface *surf_table = nullptr;    // device material-pair table, owned (freed on replacement)
+    std::vector<int> geom_mat;        // material of each geom (GEOM slot 13)
     LargeDev L{};
     int *sticky = nullptr;  // [LG_NCOUNTERS] largest value each counter reached since the last sync
     int colour_blocks = 0, solve_blocks = 0;
@@ -53,6 +56,7 @@ struct odxLarge {
         for (auto &e : ev) if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
         for (void *p : allocs) (void)hipFree(p);
+        if (surf_table) (void)hipFree(surf_table);
     }
     bool enqueue_step() {
         lg_enqueue_prepare(D, P, L, stream);
@@ -91,7 +95,7 @@ odxLarge *odxLargeCreate(int nb, const float *bodies, int ng, const float *geoms
     for (int g = 0; g < ng; g++) {
         const float *r = geoms + (size_t)g * GEOM_STRIDE;
         const int cls = (int)r[0], gb = (int)r[1];
-        if (gb < -1 || gb >= nb || (cls != 0 && cls != 1 && cls != 2 && cls != 4)) {
+        if (gb < -1 || gb >= nb || (cls < 0 || cls > 4)) {
             odx_message(2, "odxLargeCreate: geom %d has class %d / body %d", g, cls, gb);
             return nullptr;
         }
@@ -125,6 +129,9 @@ odxLarge *odxLargeCreate(int nb, const float *bodies, int ng, const float *geoms
     Surface *sd = nullptr;
     ok = ok && W->alloc(&sd, 1) && odx_check(hipMemcpy(sd, &W->surf, sizeof(Surface), hipMemcpyHostToDevice), "H2D");
     D.surf = sd;
+    W->surf_default = sd;
+    W->geom_mat.resize(ng);
+    for (int g = 0; g < ng; g++) W->geom_mat[g] = (int)geoms[(size_t)g * GEOM_STRIDE + 13];
     ok = ok && W->alloc(&D.bscr, (size_t)nb * BSCR_STRIDE);
     ok = ok && W->alloc(&D.joints, (size_t)contact_cap * JOINT_STRIDE);
     ok = ok && W->alloc(&D.rows, (size_t)3 * contact_cap * ROW_STRIDE) && W->alloc(&D.lambda, (size_t)3 * contact_cap);
@@ -200,6 +207,29 @@ int odxLargeSetSolveBlocks(odxLarge *w, int per_cu) {
 int odxLargeSetSpinLimit(odxLarge *w, int polls) {
     if (!w || polls < 0) return 0;
     w->L.spin_limit = polls == 0 ? 0 : (polls < (int)LG_SPIN_LIMIT ? polls : (int)LG_SPIN_LIMIT);
+    return 1;
+}
+
+// The near-callback policy as data, as odxBatchSetSurfaceTable: the surface
+// and contact limit of each material pair, and the category / collide bit
+// test of collideAABBs (collision_space_internal.h:50-57) when
+// category_filter is set.
+int odxLargeSetSurfaceTable(odxLarge *w, int nmat, const float *table, int category_filter) {
+    if (!w) return 0;
+    std::vector<Surface> t;
+    if (!odx_surface_table("odxLargeSetSurfaceTable", nmat, table, w->geom_mat, t)) return 0;
+    if (!odx_check(hipStreamSynchronize(w->stream), "hipStreamSynchronize")) return 0;
+    if (w->surf_table) { (void)hipFree(w->surf_table); w->surf_table = nullptr; }
+    if (nmat > 0) {
+        if (!odx_check(hipMalloc((void **)&w->surf_table, t.size() * sizeof(Surface)), "hipMalloc") ||
+            !odx_check(hipMemcpy(w->surf_table, t.data(), t.size() * sizeof(Surface), hipMemcpyHostToDevice), "H2D"))
+            return 0;
+        w->D.surf = w->surf_table;
+    } else {
+        w->D.surf = w->surf_default;
+    }
+    w->P.nmat = nmat;
+    w->P.cat_filter = category_filter ? 1 : 0;
     return 1;
 }
 

@@ -2335,7 +2335,15 @@ __device__ __forceinline__ bool lg_bodies_ok(int bi, int bj) { return (bi >= 0 |
 // their cell matches exactly), then every big geom.  A big geom i pairs with
 // the big geoms after it.  write = 0 counts into pair_cnt[i]; write = 1 stores
 // at the scanned offsets.
-__global__ void k_lg_pairs(DevWorlds D, LargeDev L, int write) {
+// (cat_filter: the category / collide bit test of collideAABBs, GEOM slots
+// 14 and 15, collision_space_internal.h:50-57)
+__device__ __forceinline__ bool lg_bits_ok(const DevWorlds &D, int i, int j) {
+    const unsigned ci = (unsigned)D.geom1[i].w, li = (unsigned)D.gpos[i].w;
+    const unsigned cj = (unsigned)D.geom1[j].w, lj = (unsigned)D.gpos[j].w;
+    return ((ci & lj) | (cj & li)) != 0u;
+}
+
+__global__ void k_lg_pairs(DevWorlds D, LargeDev L, int write, int cat_filter) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < L.ng; i += gridDim.x * blockDim.x) {
         const float *ai = L.aabb + (size_t)i * 6;
         const int bi = (int)D.geom0[i].y;
@@ -2369,6 +2377,7 @@ __global__ void k_lg_pairs(DevWorlds D, LargeDev L, int write) {
                             const int4 cj = L.gcell[j];
                             if (cj.x != x || cj.y != y || cj.z != z) continue;
                             if (!lg_bodies_ok(bi, (int)D.geom0[j].y)) continue;
+                            if (cat_filter && !lg_bits_ok(D, i, j)) continue;
                             if (!lg_overlap(ai, L.aabb + (size_t)j * 6)) continue;
                             if (write && off + n < L.pair_cap) L.pairs[off + n] = make_int2(i, j);
                             if (!write && n < LG_PAIR_KEEP) keep[n] = j;
@@ -2380,6 +2389,7 @@ __global__ void k_lg_pairs(DevWorlds D, LargeDev L, int write) {
             const int j = L.big[q];
             if (ibig && j <= i) continue;
             if (!lg_bodies_ok(bi, (int)D.geom0[j].y)) continue;
+            if (cat_filter && !lg_bits_ok(D, i, j)) continue;
             if (!lg_overlap(ai, L.aabb + (size_t)j * 6)) continue;
             if (write && off + n < L.pair_cap) L.pairs[off + n] = make_int2(min(i, j), max(i, j));
             if (!write && n < LG_PAIR_KEEP) keep[n] = j;
@@ -2413,7 +2423,12 @@ __global__ void k_lg_contacts(DevWorlds D, StepParams P, LargeDev L, int write) 
         load_geom(D, pr.x, 0, A);
         load_geom(D, pr.y, 0, B);
         DContact c[ODX_MAXC];
-        const int n = (A.body < 0 && B.body < 0) ? 0 : dcollide(A, B, min(P.maxc, ODX_MAXC), c);
+        int maxc = P.maxc, si = 0;
+        if (P.nmat > 0) {  // the material pair's surface and contact limit
+            si = (int)D.geom1[pr.x].z * P.nmat + (int)D.geom1[pr.y].z;
+            if (D.surf[si].max_contacts > 0) maxc = D.surf[si].max_contacts;
+        }
+        const int n = (A.body < 0 && B.body < 0) ? 0 : dcollide(A, B, min(maxc, ODX_MAXC), c);
         if (!write) { L.contact_cnt[p] = n; continue; }
         const int off = L.contact_cnt[p];
         int b1 = A.body, b2 = B.body;
@@ -2424,7 +2439,7 @@ __global__ void k_lg_contacts(DevWorlds D, StepParams P, LargeDev L, int write) 
             o4[0] = make_float4(0.f, (float)b1, (float)b2, c[q].pos[0]);
             o4[1] = make_float4(c[q].pos[1], c[q].pos[2], sg * c[q].normal[0], sg * c[q].normal[1]);
             o4[2] = make_float4(sg * c[q].normal[2], c[q].depth, 0.f, 0.f);
-            o4[3] = make_float4(0.f, 0.f, (float)pr.x, (float)pr.y);
+            o4[3] = make_float4(0.f, (float)si, (float)pr.x, (float)pr.y);
             L.cbody[off + q] = make_int2(b1, b2);
         }
     }
@@ -2896,9 +2911,9 @@ void lg_enqueue_prepare(const DevWorlds &D, const StepParams &P, const LargeDev 
     lg_scan(L.bucket, L.hmask + 2, nullptr, L.scan_tmp, s);
     hipLaunchKernelGGL(k_lg_scatter, dim3(G), dim3(T), 0, s, L);
     hipLaunchKernelGGL(k_lg_sort_buckets, dim3(G), dim3(T), 0, s, L);
-    hipLaunchKernelGGL(k_lg_pairs, dim3(G), dim3(T), 0, s, D, L, 0);
+    hipLaunchKernelGGL(k_lg_pairs, dim3(G), dim3(T), 0, s, D, L, 0, P.cat_filter);
     lg_scan(L.pair_cnt, L.ng + 1, nullptr, L.scan_tmp, s);
-    hipLaunchKernelGGL(k_lg_pairs, dim3(G), dim3(T), 0, s, D, L, 1);
+    hipLaunchKernelGGL(k_lg_pairs, dim3(G), dim3(T), 0, s, D, L, 1, P.cat_filter);
     hipLaunchKernelGGL(k_lg_contacts, dim3(G), dim3(T), 0, s, D, P, L, 0);
     lg_scan(L.contact_cnt, L.pair_cap + 1, L.counters + LG_PAIRS, L.scan_tmp, s);
     hipLaunchKernelGGL(k_lg_contacts, dim3(G), dim3(T), 0, s, D, P, L, 1);

@@ -75,6 +75,7 @@ def lib():
         for name in ("odxLargeSetIterations", "odxLargeStep", "odxLargeSetSolveBlocks", "odxLargeSetSpinLimit"):
             getattr(L, name).argtypes = [vp, i]
         L.odxLargeSync.argtypes = [vp]
+        L.odxLargeSetSurfaceTable.argtypes = [vp, i, fp, i]
         for name in ("odxLargeTime", "odxLargeKernelTimes"):
             getattr(L, name).argtypes = [vp, i, fp]
         for name in ("odxLargeGetBodies", "odxLargeSetBodies", "odxLargeGetCounts"):
@@ -354,6 +355,17 @@ class Large:
     def set_spin_limit(self, polls=1 << 21):
         """Polls one grid-barrier wait may make before the step fails."""
         _check(lib().odxLargeSetSpinLimit(self.h, polls), "odxLargeSetSpinLimit")
+
+    def set_surface_table(self, table, category_filter=False):
+        """The material-pair surface table and category / collide bit test of
+        Batch.set_surface_table for the large world; None returns to the
+        parameters' surface."""
+        if table is None:
+            _check(lib().odxLargeSetSurfaceTable(self.h, 0, None, 0), "odxLargeSetSurfaceTable")
+            return
+        t = np.ascontiguousarray(table, np.float32)
+        _check(lib().odxLargeSetSurfaceTable(self.h, t.shape[0], _p(t), int(category_filter)),
+               "odxLargeSetSurfaceTable")
 
     def step(self, n=1):
         _check(lib().odxLargeStep(self.h, n), "odxLargeStep")

@@ -157,13 +157,22 @@ def contact_rows(contact, surface, body1, body2=None, reversed_=False, fps=100.0
     return J[:m], fi[:m]
 
 
-def quickstep(bodies, joints, prm, dtype=np.float64, jext=None):
+def quickstep(bodies, joints, prm, dtype=np.float64, jext=None, table=None):
     """One QuickStep of the given joints (JOINT records in stepping order).
     jext: [nj, 32] extended records of hinge / slider / fixed joints (types
-    2 / 3 / 4; layout in oracle.cpp's ORC_JEXT_STRIDE)."""
+    2 / 3 / 4; layout in oracle.cpp's ORC_JEXT_STRIDE).  table: a
+    surface_table; each contact record's slot 13 then names its entry."""
     bodies = np.ascontiguousarray(bodies, dtype).copy()
     joints = np.ascontiguousarray(joints, dtype).reshape(-1, JOINT)
     lam = np.zeros(max(1, 6 * len(joints)), dtype)
+    if table is not None:
+        t = np.ascontiguousarray(table, np.float64)
+        m = getattr(lib(), "orc_quickstep_table_" + _real(dtype))(
+            ctypes.c_int(len(bodies)), _p(bodies), ctypes.c_int(len(joints)), _p(joints),
+            _p(np.ascontiguousarray(prm, np.float64)), ctypes.c_int(t.shape[0]), _p(t), _p(lam), ctypes.c_int(len(lam)))
+        if m < 0:
+            raise ValueError("a contact record names a surface outside the table")
+        return bodies, lam[:m]
     if jext is not None:
         x = np.ascontiguousarray(jext, dtype).reshape(-1, 32)
         m = getattr(lib(), "orc_quickstep_ext_" + _real(dtype))(

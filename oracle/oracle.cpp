@@ -1951,13 +1951,21 @@ template <class R> int quickstep(int nb, R *bodies, std::vector<Joint<R>> &joint
     return S.m;
 }
 
+/* tab: contact joints take the table's surface at their record's slot 13
+ * (the material pair's entry, as the near-callback stored it). */
 template <class R> int quickstep_api(int nb, R *bodies, int nj, const R *joints, const double *params, R *lambda, int cap,
-                                     const R *jext = nullptr) {
+                                     const R *jext = nullptr, const SurfaceTable<R> *tab = nullptr) {
     Params<R> p = read_params<R>(params);
     std::vector<Joint<R>> js(nj);
     for (int i = 0; i < nj; i++) {
-        read_joint(joints + (size_t)i * ORC_JOINT_STRIDE, p, js[i]);
+        const R *rec = joints + (size_t)i * ORC_JOINT_STRIDE;
+        read_joint(rec, p, js[i]);
         if (jext && js[i].type >= JT_HINGE) read_joint_ext(jext + (size_t)i * ORC_JEXT_STRIDE, js[i]);
+        if (tab && tab->nmat && js[i].type == JT_CONTACT) {
+            const int e = (int)rec[13];
+            if (e < 0 || e >= tab->nmat * tab->nmat) return -1;
+            js[i].s = tab->s[e];
+        }
     }
     return quickstep(nb, bodies, js, p, lambda, cap);
 }
@@ -2195,6 +2203,15 @@ int orc_quickstep_ext_f64(int nb, double *bodies, int nj, const double *joints, 
                           const double *params, double *lambda, int cap) {
     return quickstep_api(nb, bodies, nj, joints, params, lambda, cap, jext);
 }
+
+#define ORC_QUICKSTEP_TABLE(SUF, R)                                                                          \
+    int orc_quickstep_table_##SUF(int nb, R *bodies, int nj, const R *joints, const double *params, int nmat,  \
+                                  const double *table, R *lambda, int cap) {                                  \
+        const SurfaceTable<R> T = read_table<R>(nmat, table, 0);                                            \
+        return quickstep_api(nb, bodies, nj, joints, params, lambda, cap, (const R *)nullptr, &T);           \
+    }
+ORC_QUICKSTEP_TABLE(f32, float)
+ORC_QUICKSTEP_TABLE(f64, double)
 
 int orc_world_step_f32(int nb, float *bodies, int ng, const float *geoms, int nball, const float *balls, const double *params,
                        float *contacts, int ccap, float *lambda, int lcap, int *m_out) {

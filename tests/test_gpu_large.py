@@ -164,6 +164,89 @@ def test_geom_with_more_partners_than_the_counting_pass_keeps():
     L.close()
 
 
+def contact_rows(surface):
+    """Rows of one contact joint for a surface-table entry (dxJointContact::getInfo1,
+    ode/src/joints/contact.cpp:49-122): the normal row, then one friction row
+    per positive mu (mu and mu2 with dContactMu2, else two rows for mu)."""
+    mode, mu, mu2 = int(surface[0]), surface[1], surface[2]
+    if mode & 0x001:
+        return 1 + int(mu > 0) + int(mu2 > 0)
+    return 1 + 2 * int(mu > 0)
+
+
+def test_surface_table_and_category_bits_match_oracle():
+    # the near-callback policy on the large world: the category / collide bit
+    # test removes hash-space pairs, each geom's material picks the pair's
+    # surface and contact limit; pairs, contacts, surface indices and the
+    # coloured step match the oracle's
+    say("case: large mixed pile with a material-pair surface table")
+    rng = np.random.default_rng(91)
+    w = S.mixed_pile(1500, seed=62)
+    body = w.geoms[:, 1].astype(np.int32)
+    dyn = body >= 0
+    w.geoms[dyn, 13] = rng.integers(1, 3, size=int(dyn.sum()))
+    w.geoms[~dyn, 13] = 0
+    grp = rng.integers(0, 2, size=len(w.geoms))
+    w.geoms[:, 14] = np.where(grp == 0, 1, 2)
+    w.geoms[:, 15] = np.where(grp == 0, 1 | 4, 2 | 4)
+    w.geoms[~dyn, 14] = 4
+    w.geoms[~dyn, 15] = 1 | 2
+    approx = 0x1000 | 0x2000
+    t = O.surface_table(3, {
+        (0, 1): {"mode": approx, "mu": 0.2, "max_contacts": 2},
+        (0, 2): {"mode": approx | 0x004 | 0x010, "mu": 0.8, "bounce": 0.5, "bounce_vel": 0.1, "soft_cfm": 1e-3},
+        (1, 1): {"mode": approx, "mu": 0.0, "max_contacts": 1},
+        (1, 2): {"mode": approx | 0x008, "mu": 0.5, "soft_erp": 0.5, "max_contacts": 3},
+        (2, 2): {"mode": 0x001 | approx, "mu": 0.4, "mu2": 0.1},
+    })
+    prm = O.params(space=1, w=1.0)
+    L = make(w, prm)
+    L.set_surface_table(t, category_filter=True)
+    L.step(1)
+    L.sync()
+    cnt = L.counts()
+    assert cnt["pair_overflow"] == 0 and cnt["contact_overflow"] == 0 and cnt["uncoloured"] == 0
+
+    cat = w.geoms[:, 14].astype(np.uint32)
+    col = w.geoms[:, 15].astype(np.uint32)
+    ab = O.aabbs(w.geoms, w.bodies, dtype=np.float32)
+    every = {(int(a), int(b)) for a, b in O.pairs(ab, body, space="hash") if body[a] >= 0 or body[b] >= 0}
+    ref = {(int(a), int(b)) for a, b in O.pairs(ab, body, space="hash", cat=cat, col=col)
+           if body[a] >= 0 or body[b] >= 0}
+    got = {tuple(map(int, p)) for p in L.pairs()}
+    assert got == ref, sorted(got ^ ref)[:5]
+    say("  pairs", len(got), "of", len(every), "pass the bit test")
+    assert len(ref) < len(every)
+
+    _, ref32, _ = O.world_step(w.bodies, w.geoms, prm, dtype=np.float32, table=t, category_filter=True)
+    gj = L.joints()
+    assert len(gj) == len(ref32)
+    count = lambda c: {k: n for k, n in zip(*np.unique(c[:, 0] * 4096 + c[:, 1], return_counts=True))}
+    assert count(gj[:, 14:16].astype(np.int64)) == count(ref32[:, 7:9].astype(np.int64))
+    mat = w.geoms[:, 13].astype(int)
+    np.testing.assert_array_equal(gj[:, 13].astype(int), mat[gj[:, 14].astype(int)] * 3 + mat[gj[:, 15].astype(int)])
+
+    col_ = L.colours()
+    order = colour_order(col_)
+    joints = gj[order].astype(np.float64)
+    joints[:, 0] = 0
+    ob, olam = O.quickstep(w.bodies, joints, prm, dtype=np.float64, table=t)
+    tab = t.reshape(-1, 16)
+    m = np.array([contact_rows(tab[int(s)]) for s in gj[:, 13]], int)
+    rc = np.concatenate([[0], np.cumsum(m)])
+    lam = L.lam()
+    assert len(lam) == rc[-1] == len(olam)
+    glam = np.concatenate([lam[rc[k]:rc[k + 1]] for k in order])
+    assert relerr(glam, olam) < TOL_LAMBDA, relerr(glam, olam)
+    gb = L.bodies()
+    assert relerr(gb[:, 0:3], ob[:, 0:3]) < TOL_STATE
+    vs = np.hstack([gb[:, 8:11], gb[:, 12:15]])
+    vr = np.hstack([ob[:, 8:11], ob[:, 12:15]])
+    assert np.max(np.abs(vs - vr)) / max(np.max(np.abs(vr)), 1.0) < TOL_STATE
+    say("  contacts", len(gj), "rows", rc[-1], "by surface", np.bincount(gj[:, 13].astype(int), minlength=9))
+    L.close()
+
+
 def test_rows_match_oracle_contact_rows():
     say("case: large rows")
     w = S.mixed_pile(300, seed=70)

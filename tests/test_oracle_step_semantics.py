@@ -287,6 +287,48 @@ def test_friction_follows_each_material_pair():
     assert out[0, 8] == pytest.approx(5.0 / 7.0, rel=1e-9) and out[1, 8] == 1.0
 
 
+def test_quickstep_takes_each_contacts_surface_from_the_table():
+    # QuickStep alone with contact records naming their table entry in slot 13:
+    # the sliding spheres of the friction case (one frictionless, one rolling
+    # at 5/7 m/s), and on a random pile the same step as world_step's
+    from ode_amd import scenes as S
+    m, I = S.sphere_mass(1000.0, 0.5)
+    bodies = np.array([S._body((0, 0, 0.49), (1, 0, 0, 0), m, I, lvel=(1.0, 0, 0)),
+                       S._body((5, 0, 0.49), (1, 0, 0, 0), m, I, lvel=(1.0, 0, 0))])
+    prm = O.params(iters=200, w=1.0, cfm=0.0)
+    t = O.surface_table(3, {(0, 1): {"mu": 0.0}, (0, 2): {"mu": 1e6}})
+    joints = np.zeros((2, O.JOINT))
+    for k, (x, surf) in enumerate(((0.0, 1), (5.0, 2))):
+        joints[k, 1:3] = (k, -1)
+        joints[k, 3:10] = (x, 0, -0.01, 0, 0, 1, 0.01)
+        joints[k, 13] = surf
+    out, lam = O.quickstep(bodies, joints, prm, table=t)
+    assert len(lam) == 1 + 3
+    assert out[0, 8] == 1.0
+    assert out[1, 8] == pytest.approx(5.0 / 7.0, rel=1e-9)
+    joints[1, 13] = 9
+    with pytest.raises(ValueError):
+        O.quickstep(bodies, joints, prm, table=t)
+
+    rng = np.random.default_rng(12)
+    w = S.uneven_world(rng, 10, 12)
+    w.bodies[:, 2] = np.minimum(w.bodies[:, 2], 0.3)
+    dyn = w.geoms[:, 1] >= 0
+    w.geoms[dyn, 13] = rng.integers(1, 3, size=int(dyn.sum()))
+    w.geoms[~dyn, 13] = 0
+    t = O.surface_table(3, {(0, 1): {"mu": 0.2}, (1, 2): {"mode": 0x001 | 0x7000, "mu": 0.5, "mu2": 0.0},
+                            (2, 2): {"mode": 0x7000 | 0x010, "mu": 0.9, "soft_cfm": 1e-3}})
+    prm = O.params()
+    ref, cs, ref_l = O.world_step(w.bodies, w.geoms, prm, dtype=np.float64, table=t)
+    assert len(cs) > 10
+    joints = O.contacts_to_joints(cs, w.geoms)
+    mat = w.geoms[:, 13].astype(int)
+    joints[:, 13] = mat[cs[:, 7].astype(int)] * 3 + mat[cs[:, 8].astype(int)]
+    got, lam = O.quickstep(w.bodies, joints, prm, table=t)
+    np.testing.assert_array_equal(lam, ref_l)
+    np.testing.assert_array_equal(got, ref)
+
+
 def test_bounce_of_a_material_pair_reverses_the_approach_speed():
     # a sphere touching the plane (depth 0) at -2 m/s with bounce 1 and
     # bounce_vel 0 leaves at +2 m/s (dContactBounce, contact.cpp getInfo2);
