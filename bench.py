@@ -559,7 +559,7 @@ def main():
         "unit": "GB/s",
         "frac": RR["solve_hbm_gbs"] / HBM_PEAK_GBS,
         "traffic": None,
-        "kernel": "k_solve_small, cold mode, %d launches per step on the autotuned schedule, %.3f ms and %.1f MB "
+        "kernel": "k_solve_small, cold mode, %d launches per step on the run's schedule, %.3f ms and %.1f MB "
                   "algorithmic per launch" % (RR["launches_per_step"], RR["kernel_ms_per_launch"]["solve"],
                                               RR["solve_hbm_bytes_per_launch"] / 1e6),
         "algorithmic": "per launch: 148 B per row (row record, schedule entries, lambda in and out), 304 B per body, "
