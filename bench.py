@@ -170,7 +170,7 @@ def measure(B, args, warm, nworlds):
         # LDS (rows grow as towers fall), or 1.25x the largest world seen
         nb_max = int(np.max(np.diff(B.body_start)))
         def lds_bytes(r):  # solve_small_lds_bytes
-            return (nb_max + 1) * 64 + r * 72 + (r + 1) * 2
+            return (nb_max + 1) * 64 + r * 72 + (r + 2) // 2 * 4 + 64 * 4
         five = max(r for r in range(16, 2049) if 5 * lds_bytes(r) <= 160 * 1024)
         want = max(five, -(-int(max_rows * 1.25) // 16) * 16)
         while want > max_rows:

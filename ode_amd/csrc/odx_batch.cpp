@@ -381,7 +381,9 @@ constexpr int ODX_DEFAULT_LDS_ROWS = 400;
 // (same accounting as solve_small_lds_bytes).
 static int odx_lds_rows_fit(int lds_bodies, int want) {
     const int per_row = (16 + 2) * (int)sizeof(float) + (int)sizeof(unsigned short);
-    const int room = (SOLVE_LDS_MAX - lds_bodies * 16 * (int)sizeof(float) - 2) / per_row;
+    // fixed part: bodies, the phase table's extra entry and padding, 64 trash words
+    const int fixed = lds_bodies * 16 * (int)sizeof(float) + 4 + 64 * (int)sizeof(float);
+    const int room = (SOLVE_LDS_MAX - fixed) / per_row;
     return std::max(0, std::min(want, std::min(room, SW_MAXROWS)));
 }
 

@@ -54,6 +54,15 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
     return v;
 }
 __device__ __forceinline__ int wave_bcast_last(int v) { return __shfl(v, 63, 64); }
+// Orders the LDS accesses of a one-wave workgroup: a wave's LDS operations
+// execute in issue order, so a wavefront-scope fence (no wait) and a compiler
+// barrier suffice where a workgroup barrier would also wait for every
+// outstanding LDS operation.  Only for kernels launched with 64 threads.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
     for (int o = 32; o; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
@@ -1186,6 +1195,13 @@ __device__ float world_residual(const DevWorlds &D, const StepParams &P, int r0,
 // Body (16 floats): 0-5 fc, 6 invMass, 7-15 invI_w (3x3 row-major).  A row
 // without a second body points at the zero body at index nb.
 constexpr int LDS_ROW = 16, LDS_BODY = 16;
+// Dynamic LDS of the solve, in floats: bodies [lds_bodies][16], rows
+// [lds_rows][16], lambda [lds_rows], items [lds_rows], phase table
+// [lds_rows + 1] (16-bit, padded to whole floats), then one trash word per
+// lane for the masked row updates.
+__host__ __device__ __forceinline__ int solve_lds_trash_offset(const StepParams &P) {
+    return P.lds_bodies * LDS_BODY + P.lds_rows * (LDS_ROW + 2) + (P.lds_rows + 2) / 2;
+}
 
 // 64-byte records start on one of only two 16-bank boundaries of LDS, so the
 // lanes of a 128-bit read that touch different records would pile onto the
@@ -1202,81 +1218,152 @@ __device__ __forceinline__ void st4(float *base, int r, int j, float4 v) {
 // element k (0..15) of record r
 __device__ __forceinline__ float *rec_elem(float *base, int r, int k) { return base + 4 * sw4(r, k >> 2) + (k & 3); }
 
+// Body record in LDS (16 floats): slot 0 = fc linear xyz, 1/m; slot 1 = fc
+// angular xyz, invI[0][0]; slots 2-3 = invI[0][1] .. invI[2][2] (world frame,
+// row-major).
 __device__ __forceinline__ void lds_fc(const float *s_body, int b, float *fc) {
-    const float4 a = ld4(s_body, b, 0);
-    const float2 c = *reinterpret_cast<const float2 *>(s_body + 4 * sw4(b, 1));
-    fc[0] = a.x; fc[1] = a.y; fc[2] = a.z; fc[3] = a.w; fc[4] = c.x; fc[5] = c.y;
+    const float4 a = ld4(s_body, b, 0), c = ld4(s_body, b, 1);
+    fc[0] = a.x; fc[1] = a.y; fc[2] = a.z; fc[3] = c.x; fc[4] = c.y; fc[5] = c.z;
 }
 
 __device__ __forceinline__ void lds_body(const float *s_body, int b, float *fc, float &im, float *I) {
     const float4 a = ld4(s_body, b, 0), c = ld4(s_body, b, 1), d = ld4(s_body, b, 2), e = ld4(s_body, b, 3);
-    fc[0] = a.x; fc[1] = a.y; fc[2] = a.z; fc[3] = a.w; fc[4] = c.x; fc[5] = c.y;
-    im = c.z;
+    fc[0] = a.x; fc[1] = a.y; fc[2] = a.z; fc[3] = c.x; fc[4] = c.y; fc[5] = c.z;
+    im = a.w;
     I[0] = c.w; I[1] = d.x; I[2] = d.y; I[3] = d.z; I[4] = d.w; I[5] = e.x; I[6] = e.y; I[7] = e.z; I[8] = e.w;
 }
 
+// fc component k (0..5) of body b
+__device__ __forceinline__ float *fc_elem(float *s_body, int b, int k) { return rec_elem(s_body, b, k < 3 ? k : k + 1); }
+
+// x, y, z into the first three floats of slot `slot` of body b
+__device__ __forceinline__ void lds_store3(float *s_body, int b, int slot, float x, float y, float z) {
+    float *o = s_body + 4 * sw4(b, slot);
+    o[0] = x; o[1] = y; o[2] = z;
+}
+
 __device__ __forceinline__ void lds_store_fc(float *s_body, int b, const float *fc) {
-    st4(s_body, b, 0, make_float4(fc[0], fc[1], fc[2], fc[3]));
-    *reinterpret_cast<float2 *>(s_body + 4 * sw4(b, 1)) = make_float2(fc[4], fc[5]);
+    lds_store3(s_body, b, 0, fc[0], fc[1], fc[2]);
+    lds_store3(s_body, b, 1, fc[3], fc[4], fc[5]);
 }
 
-// One SOR row update (IterationStep, quickstep.cpp:2917-3033) on an LDS row
-// a0..a3 at sweep position p, with the two bodies' fc and inverse mass terms
-// in registers.  Branch-free: the friction-cone bound's lambda is read for
-// every row (position 0 stands in when there is no findex) and selected.
-__device__ __forceinline__ void row_update(const float4 a0, const float4 a1, const float4 a2, const float4 a3, int p,
-                                           float *s_lam, float *f1, float *f2, const float *I1, const float *I2,
-                                           float im1, float im2) {
+// Row record in LDS (16 floats): slot 0 = J1 linear xyz, rhs; slot 1 = J1
+// angular xyz, cfm; slot 2 = J2 angular xyz, 0; slot 3 = lo, hi, ids (body 1
+// bits 0-7, body 2 bits 8-15, sweep position of the findex row + 1 from bit
+// 16), 1/Ad.  J2 linear is -J1 linear.  rhs, cfm and J are scaled by Ad as the
+// reference's SOR loop takes them, so a row's step is rhs - cfm lambda - J.fc.
+
+// Quad-cooperative sweeps.  The four lanes of a quad share one item (one body
+// pair) and split each row update of quickstep.cpp:2917-3033 (IterationStep)
+// by the blocks of J: lane q of the quad holds one 3-vector x of the pair's
+// fc, the matching 3x3 block of M^-1 and reads the J block that multiplies it:
+//   q = 0  body 1 linear   (slot 0, J1 linear; M^-1 block 1/m1)
+//   q = 1  body 1 angular  (slot 1, J1 angular; invI1)
+//   q = 2  body 2 angular  (slot 2, J2 angular; invI2)
+//   q = 3  body 2 linear, held negated (slot 0: J1 linear = -J2 linear; 1/m2)
+// The row's step rhs - cfm lambda - J.fc is the sum of the four lanes' parts
+// (lane 0 adds rhs, lane 1 subtracts cfm lambda), formed with two DPP adds
+// inside the quad; every lane of the quad then takes the same clamp and the
+// same delta, and adds delta/Ad (M^-1 J)_q to its own 3-vector.
+struct QuadRow {
+    float4 j;   // this lane's J block and its scalar (rhs, cfm, 0 or rhs)
+    float4 t;   // lo, hi, ids, 1/Ad
+    float old, lf;
+};
+
+__device__ __forceinline__ float quad_sum(float v) {
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));  // lanes 1,0,3,2
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));  // lanes 2,3,0,1
+    return v;
+}
+
+__device__ __forceinline__ void quad_load_row(const float *s_rows, int r, int js, QuadRow &x) {
+    x.j = ld4(s_rows, r, js);
+    x.t = ld4(s_rows, r, 3);
+}
+__device__ __forceinline__ void quad_load_lam(const float *s_lam, int r, QuadRow &x) {
+    const int fi = (as_int(x.t.z) >> 16) - 1;
+    x.old = s_lam[r];
+    x.lf = s_lam[fi >= 0 ? fi : 0];
+}
+
+// One row step of the quad (all four lanes active).  k1 / k2 select the
+// lane's share of rhs - cfm lambda.  Returns the new lambda (the same on the
+// quad's four lanes); live = false leaves x as it is.
+__device__ __forceinline__ float quad_row_step(const QuadRow &r, float k1, float k2, float *x, const float *M,
+                                               bool live) {
 #pragma clang fp contract(fast)
-    const int fidx = (as_int(a3.z) >> 16) - 1;
-    const float old = s_lam[p];
-    const float lf = s_lam[fidx >= 0 ? fidx : 0];
-    float delta = a2.y - old * a2.z;
-    delta -= a0.x * (f1[0] - f2[0]) + a0.y * (f1[1] - f2[1]) + a0.z * (f1[2] - f2[2]);
-    delta -= a0.w * f1[3] + a1.x * f1[4] + a1.y * f1[5];
-    delta -= a1.z * f2[3] + a1.w * f2[4] + a2.x * f2[5];
-    const float hf = fabsf(a3.y * lf);
-    const float hi = fidx >= 0 ? hf : a3.y, lo = fidx >= 0 ? -hf : a3.x;
-    const float raw = old + delta;
+    const float d = r.j.x * x[0] + r.j.y * x[1] + r.j.z * x[2];
+    float delta = quad_sum(r.j.w * (k2 * r.old + k1) - d);
+    const int fidx = (as_int(r.t.z) >> 16) - 1;
+    const float hf = fabsf(r.t.y * r.lf);
+    const float hi = fidx >= 0 ? hf : r.t.y, lo = fidx >= 0 ? -hf : r.t.x;
+    const float raw = r.old + delta;
     const float nl = fminf(fmaxf(raw, lo), hi);
-    if (nl != raw) delta = nl - old;  // clamped: delta to the bound, as the reference
-    s_lam[p] = nl;
-    const float e = delta * a2.w, e1 = e * im1, e2 = e * im2;
-    f1[0] += e1 * a0.x; f1[1] += e1 * a0.y; f1[2] += e1 * a0.z;
-    f2[0] -= e2 * a0.x; f2[1] -= e2 * a0.y; f2[2] -= e2 * a0.z;
-    f1[3] += e * (I1[0] * a0.w + I1[1] * a1.x + I1[2] * a1.y);
-    f1[4] += e * (I1[3] * a0.w + I1[4] * a1.x + I1[5] * a1.y);
-    f1[5] += e * (I1[6] * a0.w + I1[7] * a1.x + I1[8] * a1.y);
-    f2[3] += e * (I2[0] * a1.z + I2[1] * a1.w + I2[2] * a2.x);
-    f2[4] += e * (I2[3] * a1.z + I2[4] * a1.w + I2[5] * a2.x);
-    f2[5] += e * (I2[6] * a1.z + I2[7] * a1.w + I2[8] * a2.x);
+    if (nl != raw) delta = nl - r.old;  // clamped: delta to the bound, as the reference
+    const float e = live ? delta * r.t.w : 0.f;
+    x[0] += e * (M[0] * r.j.x + M[1] * r.j.y + M[2] * r.j.z);
+    x[1] += e * (M[3] * r.j.x + M[4] * r.j.y + M[5] * r.j.z);
+    x[2] += e * (M[6] * r.j.x + M[7] * r.j.y + M[8] * r.j.z);
+    return nl;
 }
 
-__device__ __forceinline__ void process_item_lds(const float *s_rows, float *s_lam, float *s_body, int nb, int item) {
+// One stage of the item pipeline: updates row i (in cur), reads the record of
+// row i + 2 into far and the lambdas of row i + 1 (whose record is in nxt).
+// Indices past the item's end re-read its last row.  A row's lambda is
+// written by this quad only and a friction row's normal lambda was settled in
+// an earlier phase (normals and friction rows sweep in separate passes), so
+// none of these reads depends on the item's own writes.
+__device__ __forceinline__ void quad_stage(const float *s_rows, float *s_lam, int rs, int nr, int last, int i, int js,
+                                           float k1, float k2, QuadRow &cur, QuadRow &nxt, QuadRow &far, float *x,
+                                           const float *M, float *trash) {
+    quad_load_row(s_rows, min(rs + i + 2, last), js, far);
+    quad_load_lam(s_lam, min(rs + i + 1, last), nxt);
+    const bool live = i < nr;
+    const float nl = quad_row_step(cur, k1, k2, x, M, live);
+    *(live ? s_lam + rs + i : trash) = nl;
+}
+
+// The rows of one item (one body pair, nr <= ITEM_MAXROWS rows in sweep
+// order; item < 0: none) on a quad, for a wave whose longest item has nmax
+// rows (wave-uniform).  Row i + 2's record and row i + 1's lambdas are read
+// while row i updates, in three register sets used in turn.  Every lane
+// issues the same LDS operations in the same order (lanes past their item's
+// end update nothing and write their lambda to the lane's trash word).
+__device__ __forceinline__ void process_item_quad(const float *s_rows, float *s_lam, float *s_body, int nb, int item,
+                                                  int nmax, float *trash) {
 #pragma clang fp contract(fast)
-    const int rs = item_start(item), nr = item_rows(item);
-    const int b1 = item_b1(item), b2 = item_b2(item);
-    float f1[6], f2[6], I1[9], I2[9], im1, im2;
-    // two row buffers in turn: the next row's LDS reads are in flight while
-    // the current row computes (issued with the body records for the first)
-    float4 a0 = ld4(s_rows, rs, 0), a1 = ld4(s_rows, rs, 1), a2 = ld4(s_rows, rs, 2), a3 = ld4(s_rows, rs, 3);
-    float4 c0, c1, c2, c3;
-    lds_body(s_body, b1, f1, im1, I1);
-    lds_body(s_body, b2, f2, im2, I2);
-    int q = 0;
-    for (; q + 1 < nr; q += 2) {
-        c0 = ld4(s_rows, rs + q + 1, 0); c1 = ld4(s_rows, rs + q + 1, 1);
-        c2 = ld4(s_rows, rs + q + 1, 2); c3 = ld4(s_rows, rs + q + 1, 3);
-        row_update(a0, a1, a2, a3, rs + q, s_lam, f1, f2, I1, I2, im1, im2);
-        if (q + 2 < nr) {
-            a0 = ld4(s_rows, rs + q + 2, 0); a1 = ld4(s_rows, rs + q + 2, 1);
-            a2 = ld4(s_rows, rs + q + 2, 2); a3 = ld4(s_rows, rs + q + 2, 3);
-        }
-        row_update(c0, c1, c2, c3, rs + q + 1, s_lam, f1, f2, I1, I2, im1, im2);
+    const int q = lane_id() & 3;
+    const int js = q == 3 ? 0 : q;
+    const float k1 = q == 0 ? 1.f : 0.f, k2 = q == 1 ? -1.f : 0.f;
+    const bool lin = q == 0 || q == 3;
+    const float sg = q == 3 ? -1.f : 1.f;
+    const bool has = item >= 0;
+    const int rs = has ? item_start(item) : 0, nr = has ? item_rows(item) : 0;
+    const int b = has ? (q < 2 ? item_b1(item) : item_b2(item)) : nb;
+    const int last = rs + max(nr - 1, 0);
+    QuadRow A, B, C;
+    quad_load_row(s_rows, rs, js, A);
+    quad_load_row(s_rows, min(rs + 1, last), js, B);
+    float x[3], M[9];
+    {
+        const float4 c0 = ld4(s_body, b, 0), c1 = ld4(s_body, b, 1), c2 = ld4(s_body, b, 2), c3 = ld4(s_body, b, 3);
+        x[0] = sg * (lin ? c0.x : c1.x);
+        x[1] = sg * (lin ? c0.y : c1.y);
+        x[2] = sg * (lin ? c0.z : c1.z);
+        M[0] = lin ? c0.w : c1.w; M[1] = lin ? 0.f : c2.x; M[2] = lin ? 0.f : c2.y;
+        M[3] = lin ? 0.f : c2.z; M[4] = lin ? c0.w : c2.w; M[5] = lin ? 0.f : c3.x;
+        M[6] = lin ? 0.f : c3.y; M[7] = lin ? 0.f : c3.z; M[8] = lin ? c0.w : c3.w;
     }
-    if (q < nr) row_update(a0, a1, a2, a3, rs + q, s_lam, f1, f2, I1, I2, im1, im2);
-    lds_store_fc(s_body, b1, f1);
-    if (b2 != nb) lds_store_fc(s_body, b2, f2);
+    quad_load_lam(s_lam, rs, A);
+    for (int i = 0; i < nmax; i += 3) {
+        quad_stage(s_rows, s_lam, rs, nr, last, i, js, k1, k2, A, B, C, x, M, trash);
+        if (i + 1 >= nmax) break;
+        quad_stage(s_rows, s_lam, rs, nr, last, i + 1, js, k1, k2, B, C, A, x, M, trash);
+        if (i + 2 >= nmax) break;
+        quad_stage(s_rows, s_lam, rs, nr, last, i + 2, js, k1, k2, C, A, B, x, M, trash);
+    }
+    if (has && b != nb) lds_store3(s_body, b, lin ? 0 : 1, sg * x[0], sg * x[1], sg * x[2]);
 }
 
 __device__ float world_residual_lds(const float *s_rows, const float *s_lam, const float *s_body, int m,
@@ -1291,10 +1378,10 @@ __device__ float world_residual_lds(const float *s_rows, const float *s_lam, con
         lds_fc(s_body, (ids >> 8) & 255, f2);
         const int fidx = (ids >> 16) - 1;
         const float l = s_lam[r];
-        float d = a2.y - l * a2.z;
+        float d = a0.w - l * a1.w;
         d -= a0.x * (f1[0] - f2[0]) + a0.y * (f1[1] - f2[1]) + a0.z * (f1[2] - f2[2]);
-        d -= a0.w * f1[3] + a1.x * f1[4] + a1.y * f1[5];
-        d -= a1.z * f2[3] + a1.w * f2[4] + a2.x * f2[5];
+        d -= a1.x * f1[3] + a1.y * f1[4] + a1.z * f1[5];
+        d -= a2.x * f2[3] + a2.y * f2[4] + a2.z * f2[5];
         d *= iw;
         float lo = a3.x, hi = a3.y;
         if (fidx >= 0) { hi = fabsf(hi * s_lam[fidx]); lo = -hi; }
@@ -1317,22 +1404,22 @@ __device__ void lds_accumulate_fc(const float *s_rows, const float *s_lam, float
         const int ids = as_int(a3.z);
         const int b1 = ids & 255, b2 = (ids >> 8) & 255;
         float f[6], I[9], im;
-        const float e = l * a2.w;
+        const float e = l * a3.w;
         lds_body(s_body, b1, f, im, I);
-        atomicAdd(rec_elem(s_body, b1, 0), e * im * a0.x);
-        atomicAdd(rec_elem(s_body, b1, 1), e * im * a0.y);
-        atomicAdd(rec_elem(s_body, b1, 2), e * im * a0.z);
-        atomicAdd(rec_elem(s_body, b1, 3), e * (I[0] * a0.w + I[1] * a1.x + I[2] * a1.y));
-        atomicAdd(rec_elem(s_body, b1, 4), e * (I[3] * a0.w + I[4] * a1.x + I[5] * a1.y));
-        atomicAdd(rec_elem(s_body, b1, 5), e * (I[6] * a0.w + I[7] * a1.x + I[8] * a1.y));
+        atomicAdd(fc_elem(s_body, b1, 0), e * im * a0.x);
+        atomicAdd(fc_elem(s_body, b1, 1), e * im * a0.y);
+        atomicAdd(fc_elem(s_body, b1, 2), e * im * a0.z);
+        atomicAdd(fc_elem(s_body, b1, 3), e * (I[0] * a1.x + I[1] * a1.y + I[2] * a1.z));
+        atomicAdd(fc_elem(s_body, b1, 4), e * (I[3] * a1.x + I[4] * a1.y + I[5] * a1.z));
+        atomicAdd(fc_elem(s_body, b1, 5), e * (I[6] * a1.x + I[7] * a1.y + I[8] * a1.z));
         if (b2 != nb) {
             lds_body(s_body, b2, f, im, I);
-            atomicAdd(rec_elem(s_body, b2, 0), -e * im * a0.x);
-            atomicAdd(rec_elem(s_body, b2, 1), -e * im * a0.y);
-            atomicAdd(rec_elem(s_body, b2, 2), -e * im * a0.z);
-            atomicAdd(rec_elem(s_body, b2, 3), e * (I[0] * a1.z + I[1] * a1.w + I[2] * a2.x));
-            atomicAdd(rec_elem(s_body, b2, 4), e * (I[3] * a1.z + I[4] * a1.w + I[5] * a2.x));
-            atomicAdd(rec_elem(s_body, b2, 5), e * (I[6] * a1.z + I[7] * a1.w + I[8] * a2.x));
+            atomicAdd(fc_elem(s_body, b2, 0), -e * im * a0.x);
+            atomicAdd(fc_elem(s_body, b2, 1), -e * im * a0.y);
+            atomicAdd(fc_elem(s_body, b2, 2), -e * im * a0.z);
+            atomicAdd(fc_elem(s_body, b2, 3), e * (I[0] * a2.x + I[1] * a2.y + I[2] * a2.z));
+            atomicAdd(fc_elem(s_body, b2, 4), e * (I[3] * a2.x + I[4] * a2.y + I[5] * a2.z));
+            atomicAdd(fc_elem(s_body, b2, 5), e * (I[6] * a2.x + I[7] * a2.y + I[8] * a2.z));
         }
     }
 }
@@ -1345,15 +1432,21 @@ __device__ bool lds_stage(const DevWorlds &D, int b0, int nb, int r0, int m, int
                           const int *sched, const int *perm, float *s_body, float *s_rows, int *s_sched,
                           unsigned short *s_ps) {
     const int t = lane_id();
-    for (int p = t; p <= L; p += 64) s_ps[p] = (unsigned short)ps[p];
+    // phase starts (bits 0-10) with the phase's longest item in rows (bits 11-14)
+    for (int p = t; p <= L; p += 64) {
+        int nmax = 0;
+        if (p < L)
+            for (int k = ps[p]; k < ps[p + 1]; k++) nmax = max(nmax, item_rows(sched[k]));
+        s_ps[p] = (unsigned short)(ps[p] | (nmax << 11));
+    }
     const int nitems = ps[L];
     for (int k = t; k < nitems; k += 64) s_sched[k] = sched[k];
     for (int i = t; i <= nb; i += 64) {
         const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
         if (i < nb) {
             const float *s = D.bscr + (size_t)(b0 + i) * BSCR_STRIDE;
-            st4(s_body, i, 0, z);
-            st4(s_body, i, 1, make_float4(0.f, 0.f, s[30], s[12]));
+            st4(s_body, i, 0, make_float4(0.f, 0.f, 0.f, s[30]));
+            st4(s_body, i, 1, make_float4(0.f, 0.f, 0.f, s[12]));
             st4(s_body, i, 2, make_float4(s[13], s[14], s[16], s[17]));
             st4(s_body, i, 3, make_float4(s[18], s[20], s[21], s[22]));
         } else {
@@ -1368,10 +1461,10 @@ __device__ bool lds_stage(const DevWorlds &D, int b0, int nb, int r0, int m, int
         if (b2 >= 0 && (a2.x != -a0.x || a2.y != -a0.y || a2.z != -a0.z)) bad = 1;
         const int ids = b1 | ((b2 >= 0 ? b2 : nb) << 8) | (((fidx >= 0 ? perm[fidx] : -1) + 1) << 16);
         const int p = perm[r];
-        st4(s_rows, p, 0, make_float4(a0.x, a0.y, a0.z, a0.w));
-        st4(s_rows, p, 1, make_float4(a1.x, a1.y, a2.w, a3.x));
-        st4(s_rows, p, 2, make_float4(a3.y, a1.z, a1.w, 1.f / a7.w));
-        st4(s_rows, p, 3, make_float4(a3.z, a3.w, as_float(ids), 0.f));
+        st4(s_rows, p, 0, make_float4(a0.x, a0.y, a0.z, a1.z));
+        st4(s_rows, p, 1, make_float4(a0.w, a1.x, a1.y, a1.w));
+        st4(s_rows, p, 2, make_float4(a2.w, a3.x, a3.y, 0.f));
+        st4(s_rows, p, 3, make_float4(a3.z, a3.w, as_float(ids), 1.f / a7.w));
     }
     return !__any(bad);
 }
@@ -1485,6 +1578,7 @@ __device__ __forceinline__ void solve_world(const DevWorlds &D, const StepParams
     float *s_lam = s_rows + P.lds_rows * LDS_ROW;
     int *s_sched = reinterpret_cast<int *>(s_lam + P.lds_rows);
     unsigned short *s_ps = reinterpret_cast<unsigned short *>(s_sched + P.lds_rows);
+    float *s_trash = s_dyn + solve_lds_trash_offset(P);
     bool lds = m > 0 && m <= P.lds_rows && nb < P.lds_bodies;
     if (lds) {
         lds = lds_stage(D, b0, nb, r0, m, L, ps, sched, perm, s_body, s_rows, s_sched, s_ps);
@@ -1512,11 +1606,30 @@ __device__ __forceinline__ void solve_world(const DevWorlds &D, const StepParams
         }
         __syncthreads();
         bool restarted = false;
+        const int nitem = s_ps[L] & 2047;
+        float *trash = s_trash + t;
+        const int quad = t >> 2;
         while (it < P.max_iters) {
+            // 16 quads take a phase's items; each quad's first item of the
+            // next phase is read while the current phase runs
+            int s = 0, pe = s_ps[1], e = pe & 2047;
+            int nmax = __builtin_amdgcn_readfirstlane(s_ps[0] >> 11);
+            int cur = quad < e ? s_sched[quad] : -1;
             for (int p = 0; p < L; p++) {
-                const int s = s_ps[p], e = s_ps[p + 1];
-                for (int k = s + t; k < e; k += 64) process_item_lds(s_rows, s_lam, s_body, nb, s_sched[k]);
-                __syncthreads();
+                const int pe2 = p + 1 < L ? s_ps[p + 2] : pe;
+                const int e2 = pe2 & 2047;
+                const int cand = e + quad < nitem ? s_sched[e + quad] : -1;
+                process_item_quad(s_rows, s_lam, s_body, nb, cur, nmax, trash);
+                for (int k0 = s + 16; k0 < e; k0 += 16) {
+                    const int k = k0 + quad;
+                    process_item_quad(s_rows, s_lam, s_body, nb, k < e ? s_sched[k] : -1, nmax, trash);
+                }
+                wave_sync();
+                cur = e + quad < e2 ? cand : -1;
+                nmax = __builtin_amdgcn_readfirstlane(pe >> 11);
+                s = e;
+                e = e2;
+                pe = pe2;
             }
             it++;
             if (P.tol > 0.f && (it % P.check_every == 0 || it == P.max_iters)) {
@@ -2100,8 +2213,7 @@ void launch_prepare_small(const DevWorlds &D, const StepParams &P, hipStream_t s
     hipLaunchKernelGGL(k_prepare_small, dim3(P.wcount), dim3(64), 0, s, D, P);
 }
 size_t solve_small_lds_bytes(const StepParams &P) {
-    const size_t lds = ((size_t)P.lds_bodies * LDS_BODY + (size_t)P.lds_rows * (LDS_ROW + 2)) * sizeof(float) +
-                       ((size_t)P.lds_rows + 1) * sizeof(unsigned short);
+    const size_t lds = (size_t)solve_lds_trash_offset(P) * sizeof(float) + 64 * sizeof(float);
     const size_t fc = (size_t)SW_MAXB * 6 * sizeof(float);
     return lds > fc ? lds : fc;
 }
