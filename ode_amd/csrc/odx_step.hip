@@ -955,48 +955,91 @@ __device__ __forceinline__ int prepare_world(const DevWorlds &D, const StepParam
         // a limit cycle short of the residual target.
         unsigned short *s_bstart = reinterpret_cast<unsigned short *>(s_pairs);  // free after S4
         unsigned char *s_blen = reinterpret_cast<unsigned char *>(&s_ab[0][0]);
-        if (t == 0) {
-            int nblk = 0, pa = -2, pb = -2, acc_a = 0, acc_f = 0;
-            for (int k = 0; k < nj_used; k++) {
+        // per joint: its rows of the first pass (na) and of the findex pass
+        // (nf), packed na | nf << 4 in place of its row count
+        for (int k = t; k < nj_used; k += 64) {
+            int na, nf;
+            split_joint_rows(s_rfl + s_joff[k], (int)s_jm[k], na, nf);
+            s_jm[k] = (unsigned char)(na | (nf << 4));
+        }
+        __syncthreads();
+        // Blocks: one walk over the joints in order, the same on every lane
+        // (scalar registers); each chunk of 64 joints is read into a register
+        // first and walked with readlane, so no LDS read sits on the walk.
+        int nblk = 0, pa = -2, pb = -2, acc_a = 0, acc_f = 0, cur_len = 0;
+        for (int base = 0; base < nj_used; base += 64) {
+            const int k = base + t;
+            int pk = 0;
+            if (k < nj_used) {
                 const int r = s_joff[k];
-                const int a = s_rb1[r], bb = s_rb2[r];
-                int na, nf;
-                split_joint_rows(s_rfl + r, (int)s_jm[k], na, nf);
+                pk = (int)s_rb1[r] | ((int)s_rb2[r] << 8) | ((int)s_jm[k] << 16);
+            }
+            const int n = min(64, nj_used - base);
+            for (int j = 0; j < n; j++) {
+                const int v = __builtin_amdgcn_readlane(pk, j);
+                const int a = v & 255, bb = (v >> 8) & 255, na = (v >> 16) & 15, nf = (v >> 20) & 15;
                 if (nblk > 0 && a == pa && bb == pb && acc_a + na <= ITEM_MAXROWS && acc_f + nf <= ITEM_MAXROWS) {
-                    s_blen[nblk - 1]++;
+                    cur_len++;
                     acc_a += na;
                     acc_f += nf;
                 } else {
-                    s_bstart[nblk] = (unsigned short)k;
-                    s_blen[nblk] = 1;
+                    if (t == 0) {
+                        if (nblk > 0) s_blen[nblk - 1] = (unsigned char)cur_len;
+                        s_bstart[nblk] = (unsigned short)(base + j);
+                    }
                     nblk++;
+                    cur_len = 1;
                     pa = a; pb = bb; acc_a = na; acc_f = nf;
                 }
             }
-            // greedy block colouring with per-body colour masks
-            for (int b = 0; b < nb; b++) s_mask[b] = 0ull;
-            int L = 0, extra = 0;
-            for (int q = 0; q < nblk; q++) {
+        }
+        if (t == 0 && nblk > 0) s_blen[nblk - 1] = (unsigned char)cur_len;
+        __syncthreads();
+        // Greedy block colouring: block q takes the lowest colour neither of
+        // its bodies holds.  Each body's 64-bit colour mask lives in lane b
+        // (nb <= 64); the walk reads it with readlane and updates it by lane select.
+        unsigned mlo = 0u, mhi = 0u;
+        int L = 0, extra = 0;
+        for (int base = 0; base < nblk; base += 64) {
+            const int q = base + t;
+            int ab = 0;
+            if (q < nblk) {
                 const int r = s_joff[s_bstart[q]];
-                const int a = s_rb1[r], bb = s_rb2[r];
-                const unsigned long long used = s_mask[a] | (bb != 255 ? s_mask[bb] : 0ull);
+                ab = (int)s_rb1[r] | ((int)s_rb2[r] << 8);
+            }
+            int lvl = 0;
+            const int n = min(64, nblk - base);
+            for (int j = 0; j < n; j++) {
+                const int v = __builtin_amdgcn_readlane(ab, j);
+                const int a = v & 255, bb = (v >> 8) & 255;
+                const unsigned alo = __builtin_amdgcn_readlane(mlo, a), ahi = __builtin_amdgcn_readlane(mhi, a);
+                unsigned ulo = alo, uhi = ahi, blo = 0u, bhi = 0u;
+                if (bb != 255) {
+                    blo = __builtin_amdgcn_readlane(mlo, bb);
+                    bhi = __builtin_amdgcn_readlane(mhi, bb);
+                    ulo |= blo;
+                    uhi |= bhi;
+                }
                 int c;
-                if (~used) {
-                    c = __ffsll((long long)~used) - 1;
-                    const unsigned long long bit = 1ull << c;
-                    s_mask[a] |= bit;
-                    if (bb != 255) s_mask[bb] |= bit;
+                if (~ulo) {
+                    c = __ffs((int)~ulo) - 1;
+                    const unsigned bit = 1u << c;
+                    mlo = t == a ? (alo | bit) : mlo;
+                    mlo = t == bb ? (blo | bit) : mlo;
+                } else if (~uhi) {
+                    c = 32 + __ffs((int)~uhi) - 1;
+                    const unsigned bit = 1u << (c - 32);
+                    mhi = t == a ? (ahi | bit) : mhi;
+                    mhi = t == bb ? (bhi | bit) : mhi;
                 } else {
                     c = 64 + extra++;  // no free colour: a phase of its own
                 }
-                s_lvl[q] = (unsigned short)c;
+                lvl = t == j ? c : lvl;
                 if (c + 1 > L) L = c + 1;
             }
-            s_L = L;
-            s_nblk = nblk;
+            if (q < nblk) s_lvl[q] = (unsigned short)lvl;
         }
         __syncthreads();
-        const int L = s_L, nblk = s_nblk;
         // Emit the items colour by colour (pass A then pass B); lanes take
         // blocks in order, a wave scan gives each item its slot and rows.
         int nitem = 0, npos = 0, nph = 0;
@@ -1008,11 +1051,7 @@ __device__ __forceinline__ int prepare_world(const DevWorlds &D, const StepParam
                     int nrow = 0;
                     if (q < nblk && s_lvl[q] == c) {
                         const int k0 = s_bstart[q], k1 = k0 + s_blen[q];
-                        for (int k = k0; k < k1; k++) {
-                            int na, nf;
-                            split_joint_rows(s_rfl + s_joff[k], (int)s_jm[k], na, nf);
-                            nrow += pass == 0 ? na : nf;
-                        }
+                        for (int k = k0; k < k1; k++) nrow += pass == 0 ? (s_jm[k] & 15) : (s_jm[k] >> 4);
                     }
                     const unsigned long long bal = __ballot(nrow > 0);
                     const int incl = wave_incl_scan(nrow);
@@ -1024,8 +1063,7 @@ __device__ __forceinline__ int prepare_world(const DevWorlds &D, const StepParam
                         int p = start;
                         const int k0 = s_bstart[q], k1 = k0 + s_blen[q];
                         for (int k = k0; k < k1; k++) {
-                            int na, nf;
-                            split_joint_rows(s_rfl + s_joff[k], (int)s_jm[k], na, nf);
+                            const int na = s_jm[k] & 15, nf = s_jm[k] >> 4;
                             const int rr = s_joff[k] + (pass == 0 ? 0 : na), n = pass == 0 ? na : nf;
                             for (int i = 0; i < n; i++, p++) { perm[rr + i] = p; inv[p] = rr + i; }
                         }
