@@ -76,6 +76,26 @@ typedef struct odxBatch odxBatch;
 odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies, const int *geom_start,
                          const float *geoms, const int *ball_start, const float *balls, const double *params,
                          int contacts_per_body);
+/* odxBatchCreate with hinge, slider and fixed joints among the joints
+ * (dJointCreateHinge / Slider / Fixed, ode/src/joints/hinge.cpp, slider.cpp,
+ * fixed.cpp; rows of getInfo2 and the limit / motor of joint.cpp
+ * dxJointLimitMotor).  joints are JOINT records of type 1 (ball), 2 (hinge),
+ * 3 (slider) or 4 (fixed): 3-5 anchor1 (body1 frame), 6-8 anchor2 (body2
+ * frame, or world when body2 is -1), 9 erp and 10 cfm of the hinge's ball rows
+ * (the world's).  jext holds one extended record of 32 floats per joint (those
+ * of ball joints are ignored): 0-2 axis1 (body1 frame), 3-5 axis2 (body2 frame,
+ * or world), 6-9 qrel (dJointSetFixed / setFixedOrientation), 10-12 offset
+ * (slider: body1's position from body2 in body2's frame; fixed: body1's
+ * position from body2 in body1's frame; either: body1's world position when
+ * body2 is -1), 14 limit state
+ * (0 to start), 15 limit error, 16 lostop, 17 histop, 18 vel, 19 fmax, 20 fudge,
+ * 21 normal cfm, 22 stop erp, 23 stop cfm, 24 bounce, 25 slider reverse, 26 erp
+ * and 27 cfm of the fixed joint.  The limit state, the limit / motor row and
+ * the force of a motor at its limit (getInfo1, addLimot) are evaluated on the
+ * device every step.  jext may be NULL when every joint is a ball joint. */
+odxBatch *odxBatchCreateJoints(int nworlds, const int *body_start, const float *bodies, const int *geom_start,
+                               const float *geoms, const int *joint_start, const float *joints, const float *jext,
+                               const double *params, int contacts_per_body);
 void odxBatchDestroy(odxBatch *b);
 
 /* Solver controls.  order: 0 = reference row order (deterministic; results

@@ -270,6 +270,7 @@ struct odxBatch {
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     float *lambda0 = nullptr;
     int samples_split = 0;  // counter pair of the launch mode that ran last (for odxBatchGetSamples)
+    bool has_jext = false;  // hinge / slider / fixed joints (extended records on the device)
 
     template <class T> bool alloc(T **p, size_t n) {
         if (!dalloc(p, n)) return false;
@@ -426,6 +427,15 @@ int odxDeviceCount(void) {
 odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies, const int *geom_start,
                          const float *geoms, const int *ball_start, const float *balls, const double *params,
                          int contacts_per_body) {
+    return odxBatchCreateJoints(nworlds, body_start, bodies, geom_start, geoms, ball_start, balls, nullptr, params,
+                                contacts_per_body);
+}
+
+odxBatch *odxBatchCreateJoints(int nworlds, const int *body_start, const float *bodies, const int *geom_start,
+                               const float *geoms, const int *joint_start, const float *joints, const float *jext,
+                               const double *params, int contacts_per_body) {
+    const int *ball_start = joint_start;
+    const float *balls = joints;
     if (nworlds <= 0 || !body_start || !bodies || !geom_start || !geoms || !params) {
         odx_message(1, "odxBatchCreate: bad arguments");
         return nullptr;
@@ -472,16 +482,23 @@ odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies
         }
         for (int l = B->ball_start[w]; l < B->ball_start[w + 1]; l++) {
             const float *r = balls + (size_t)l * JOINT_STRIDE;
-            const int x1 = (int)r[1], x2 = (int)r[2];
-            if ((int)r[0] != 1 || x1 < 0 || x1 >= nbw || x2 < -1 || x2 >= nbw || x1 == x2) {
-                odx_message(2, "odxBatchCreate: ball joint %d of world %d has bodies %d,%d", l, w, x1, x2);
+            const int ty = (int)r[0], x1 = (int)r[1], x2 = (int)r[2];
+            if (ty < 1 || ty > JT_FIXED || x1 < 0 || x1 >= nbw || x2 < -1 || x2 >= nbw || x1 == x2) {
+                odx_message(2, "odxBatchCreate: joint %d of world %d has type %d, bodies %d,%d", l, w, ty, x1, x2);
                 delete B;
                 return nullptr;
             }
+            if (ty >= JT_HINGE && !jext) {
+                odx_message(2, "odxBatchCreate: joint %d of world %d is a hinge, slider or fixed joint and no "
+                               "extended records were given", l, w);
+                delete B;
+                return nullptr;
+            }
+            if (ty >= JT_HINGE) B->has_jext = true;
         }
         int cc = contacts_per_body * nbw + 16;
         int jc = nlw + cc;
-        int rc = 3 * jc;
+        int rc = 3 * jc + 3 * nlw;  // hinge / slider / fixed joints take up to six rows
         if (rc > SW_MAXROWS) { rc = SW_MAXROWS; jc = rc / 3; }
         B->joint_start[w + 1] = B->joint_start[w] + jc;
         B->row_start[w + 1] = B->row_start[w] + rc;
@@ -513,6 +530,10 @@ odxBatch *odxBatchCreate(int nworlds, const int *body_start, const float *bodies
     ok = ok && B->alloc(&f, (size_t)B->nball * JOINT_STRIDE) &&
          upload(f, balls ? balls : (const float *)nullptr, balls ? (size_t)B->nball * JOINT_STRIDE : 0);
     D.balls = f;
+    if (B->has_jext) {
+        ok = ok && B->alloc(&f, (size_t)B->nball * JEXT_STRIDE) && upload(f, jext, (size_t)B->nball * JEXT_STRIDE);
+        D.jext = f;
+    }
     Surface *sd = nullptr;
     ok = ok && B->alloc(&sd, 1) && upload(sd, &B->surf, 1);
     D.surf = sd;
@@ -922,6 +943,10 @@ constexpr int SNAP_NINT = 13;
 int odxBatchSaveSnapshot(odxBatch *b, const char *path) {
     if (!b || !path) {
         odx_message(1, "odxBatchSaveSnapshot: bad arguments");
+        return 0;
+    }
+    if (b->has_jext) {
+        odx_message(1, "odxBatchSaveSnapshot: batches with hinge, slider or fixed joints are not saved");
         return 0;
     }
     std::vector<float> bodies((size_t)b->nb * BODY_STRIDE);

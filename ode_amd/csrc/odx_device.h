@@ -150,7 +150,8 @@ struct DevWorlds {
     // [3s+2] = phases; the last segment serves any later sweep)
     // drop-in world: one extended record per joint for hinge, slider and fixed
     // joints (JEXT_* layout), may be null
-    const float *jext;      // [joint cap * JEXT_STRIDE]
+    float *jext;            // extended records of hinge / slider / fixed joints: drop-in world [joint
+                            // slot * JEXT_STRIDE]; batched worlds [ball-joint index * JEXT_STRIDE]
     const int *seg_table;   // [3 * nseg], may be null
     int nseg;
     int *island_sweeps;     // [nislands] sweeps each island ran, may be null

@@ -467,7 +467,108 @@ __device__ int hinge_slider_fixed_info2(const DevWorlds &D, const StepParams &P,
 
 // Builds the rows of one joint (Stage2a scaling, Stage2c rhs, iMJ, Ad) and
 // writes them at world-local row index ri.
-__device__ int build_joint_rows(const DevWorlds &D, const StepParams &P, const float *rec, int b0, int r0, int ri) {
+// getInfo1 of a hinge (hinge.cpp:58-77) or slider (slider.cpp:112-143) of a
+// batched world, from the bodies' poses at the start of the step: the limit
+// state (testRotationalLimit joint.cpp:574-595 for the hinge angle of
+// joint.cpp:421-490; the slider position of slider.cpp:45-82), whether the
+// limit / motor row is present (x[JX_LIMOT_ROW]) and the force of a motor
+// powered at its limit, added to the bodies' accumulators as addLimot does
+// (joint.cpp:640-705).  A hinge whose stops are not finite keeps the limit
+// state it has.  Accumulator adds are atomic: several joints may share a body.
+__device__ void joint_info1(const DevWorlds &D, int b0, const float *rec, float *x) {
+    const int type = (int)rec[0];
+    if (type != JT_HINGE && type != JT_SLIDER) return;
+    const float pi = 3.14159265358979f;
+    const int b1 = (int)rec[1], b2 = (int)rec[2];
+    const float4 q1v = D.quat[b0 + b1];
+    const float q1[4] = {q1v.x, q1v.y, q1v.z, q1v.w};
+    float R1[12];
+    RfromQ(R1, q1);
+    const float lo = x[JX_LOSTOP], hi = x[JX_HISTOP];
+    int limit = (int)x[JX_LIMIT];
+    float err = x[JX_LIMIT_ERR];
+    if (type == JT_HINGE) {
+        if ((lo >= -pi || hi <= pi) && lo <= hi) {
+            float qr[4];
+            if (b2 >= 0) {
+                const float4 q2v = D.quat[b0 + b2];
+                const float q2[4] = {q2v.x, q2v.y, q2v.z, q2v.w};
+                float qq[4];
+                qmul(qq, q1, q2, true, false);
+                qmul(qr, qq, x + JX_QREL, false, true);
+            } else {
+                qmul(qr, q1, x + JX_QREL, true, true);
+            }
+            const float cost2 = qr[0], sint2 = sqrtf(qr[1] * qr[1] + qr[2] * qr[2] + qr[3] * qr[3]);
+            const float d = qr[1] * x[JX_AXIS1] + qr[2] * x[JX_AXIS1 + 1] + qr[3] * x[JX_AXIS1 + 2];
+            float th = d >= 0.f ? 2.f * atan2f(sint2, cost2) : 2.f * atan2f(sint2, -cost2);
+            if (th > pi) th -= 2.f * pi;
+            const float a = -th;
+            if (a <= lo) { limit = 1; err = a - lo; }
+            else if (a >= hi) { limit = 2; err = a - hi; }
+            else limit = 0;
+        }
+    } else {
+        limit = 0;
+        if ((lo > -kInf || hi < kInf) && lo <= hi) {
+            float ax1[3], q[3];
+            mul0_331(ax1, R1, x + JX_AXIS1);
+            const float4 p1 = D.pos[b0 + b1];
+            if (b2 >= 0) {
+                const float4 q2v = D.quat[b0 + b2], p2 = D.pos[b0 + b2];
+                const float q2[4] = {q2v.x, q2v.y, q2v.z, q2v.w};
+                float R2[12];
+                RfromQ(R2, q2);
+                mul0_331(q, R2, x + JX_OFFSET);
+                q[0] = p1.x - q[0] - p2.x; q[1] = p1.y - q[1] - p2.y; q[2] = p1.z - q[2] - p2.z;
+            } else {
+                q[0] = p1.x - x[JX_OFFSET]; q[1] = p1.y - x[JX_OFFSET + 1]; q[2] = p1.z - x[JX_OFFSET + 2];
+                if (x[JX_REVERSE] != 0.f) { ax1[0] = -ax1[0]; ax1[1] = -ax1[1]; ax1[2] = -ax1[2]; }
+            }
+            const float pos = dot3(ax1, q);
+            if (pos <= lo) { limit = 1; err = pos - lo; }
+            else if (pos >= hi) { limit = 2; err = pos - hi; }
+        }
+    }
+    const float fmax = x[JX_FMAX];
+    x[JX_LIMIT] = (float)limit;
+    x[JX_LIMIT_ERR] = err;
+    x[JX_LIMOT_ROW] = (limit || fmax > 0.f) ? 1.f : 0.f;
+    if (fmax > 0.f && limit && lo != hi) {
+        float fm = fmax;
+        const float vel = x[JX_VEL];
+        if (vel > 0.f || (vel == 0.f && limit == 2)) fm = -fm;
+        if ((limit == 1 && vel > 0.f) || (limit == 2 && vel < 0.f)) fm *= x[JX_FUDGE];
+        float ax1[3];
+        mul0_331(ax1, R1, x + JX_AXIS1);
+        if (type == JT_SLIDER && b2 < 0 && x[JX_REVERSE] != 0.f) { ax1[0] = -ax1[0]; ax1[1] = -ax1[1]; ax1[2] = -ax1[2]; }
+        const float f[3] = {fm * ax1[0], fm * ax1[1], fm * ax1[2]};
+        float4 *T1 = D.tacc + b0 + b1, *F1 = D.facc + b0 + b1;
+        if (type == JT_HINGE) {
+            if (b2 >= 0) {
+                float4 *T2 = D.tacc + b0 + b2;
+                atomicAdd(&T2->x, f[0]); atomicAdd(&T2->y, f[1]); atomicAdd(&T2->z, f[2]);
+            }
+            atomicAdd(&T1->x, -f[0]); atomicAdd(&T1->y, -f[1]); atomicAdd(&T1->z, -f[2]);
+        } else {
+            if (b2 >= 0) {
+                const float4 p1 = D.pos[b0 + b1], p2 = D.pos[b0 + b2];
+                const float c[3] = {0.5f * (p2.x - p1.x), 0.5f * (p2.y - p1.y), 0.5f * (p2.z - p1.z)};
+                float ltd[3];
+                cross3(ltd, c, ax1);
+                float4 *T2 = D.tacc + b0 + b2, *F2 = D.facc + b0 + b2;
+                atomicAdd(&T1->x, -fm * ltd[0]); atomicAdd(&T1->y, -fm * ltd[1]); atomicAdd(&T1->z, -fm * ltd[2]);
+                atomicAdd(&T2->x, -fm * ltd[0]); atomicAdd(&T2->y, -fm * ltd[1]); atomicAdd(&T2->z, -fm * ltd[2]);
+                atomicAdd(&F2->x, f[0]); atomicAdd(&F2->y, f[1]); atomicAdd(&F2->z, f[2]);
+            }
+            atomicAdd(&F1->x, -f[0]); atomicAdd(&F1->y, -f[1]); atomicAdd(&F1->z, -f[2]);
+        }
+    }
+}
+
+// x: the joint's extended record (hinge / slider / fixed), else unused.
+__device__ int build_joint_rows(const DevWorlds &D, const StepParams &P, const float *rec, const float *x, int b0,
+                                int r0, int ri) {
     RowBuf rows[6];
 #pragma unroll
     for (int r = 0; r < 6; r++) {
@@ -483,8 +584,7 @@ __device__ int build_joint_rows(const DevWorlds &D, const StepParams &P, const f
     int m;
     if (type == 0) m = contact_info2(D, P, rec, b0, rows);
     else if (type == 1) m = ball_info2(D, P, rec, b0, rows);
-    else m = hinge_slider_fixed_info2(D, P, rec, D.jext + (size_t)((rec - D.joints) / JOINT_STRIDE) * JEXT_STRIDE,
-                                      b0, rows);
+    else m = hinge_slider_fixed_info2(D, P, rec, x, b0, rows);
     const int b1 = (int)rec[1], b2 = (int)rec[2];
     const float fps = 1.f / P.h;
     const float *s1 = D.bscr + (size_t)(b0 + b1) * BSCR_STRIDE;
@@ -693,6 +793,15 @@ __device__ __forceinline__ int prepare_world(const DevWorlds &D, const StepParam
     long long *pt = D.ptime ? D.ptime + (size_t)w * PTIME_SLOTS : nullptr;
     if (pt && t == 0) pt[0] = (long long)wall_clock64();
 
+    // getInfo1 of the world's hinges and sliders before the body stage reads
+    // the accumulators a motor at its limit pushes through
+    if (D.jext && nball > 0) {
+        for (int k = t; k < nball; k += 64) {
+            const float *rec = D.balls + (size_t)(l0 + k) * JOINT_STRIDE;
+            if ((int)rec[0] >= JT_HINGE) joint_info1(D, b0, rec, D.jext + (size_t)(l0 + k) * JEXT_STRIDE);
+        }
+        __syncthreads();
+    }
     // S1: bodies
     for (int i = t; i < nb; i += 64) body_prep(D, P, b0 + i, D.facc, D.tacc);
     __syncthreads();
@@ -812,12 +921,18 @@ __device__ __forceinline__ int prepare_world(const DevWorlds &D, const StepParam
         const int k = base + t;
         int mj = 0;
         const float *rec = D.joints + (size_t)(j0 + k) * JOINT_STRIDE;
-        if (k < njoint) mj = ((int)rec[0] == 0) ? contact_rows_m(D.surf[(int)rec[13]]) : 3;
+        // ball joints and the extended types come first (k < nball); their
+        // extended records are indexed by ball-joint index
+        const int ty = k < njoint ? (int)rec[0] : 0;
+        const float *xr = (ty >= JT_HINGE && D.jext) ? D.jext + (size_t)(l0 + k) * JEXT_STRIDE : nullptr;
+        if (k < njoint)
+            mj = ty == 0 ? contact_rows_m(D.surf[(int)rec[13]])
+                         : ty == 1 ? 3 : ty == JT_FIXED ? 6 : 5 + (xr && xr[JX_LIMOT_ROW] != 0.f ? 1 : 0);
         const int incl = wave_incl_scan(mj);
         const int ri = mrow + incl - mj;
         if (k < njoint) {
             if (ri + mj <= rcap && k < SW_MAXROWS) {
-                build_joint_rows(D, P, rec, b0, r0, ri);
+                build_joint_rows(D, P, rec, xr, b0, r0, ri);
                 const int bb1 = (int)rec[1], bb2 = (int)rec[2];
                 s_joff[k] = (unsigned short)ri;
                 s_jm[k] = (unsigned char)mj;
@@ -836,7 +951,7 @@ __device__ __forceinline__ int prepare_world(const DevWorlds &D, const StepParam
                         if (rr < mj) s_rfl[ri + rr] = (mode & 0x2000) ? 1 : 0;
                     }
                 } else {
-                    s_rfl[ri] = 0; s_rfl[ri + 1] = 0; s_rfl[ri + 2] = 0;
+                    for (int r = 0; r < mj; r++) s_rfl[ri + r] = 0;
                 }
                 D.joff[j0 + k] = ri;
             } else {
@@ -2007,7 +2122,9 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
         for (int k = t; k <= L; k += NT) ss[nit + k] = D.phase_start[k];
     }
     __syncthreads();
-    for (int j = t; j < nj; j += NT) build_joint_rows(D, P, D.joints + (size_t)j * JOINT_STRIDE, 0, 0, D.joff[j]);
+    for (int j = t; j < nj; j += NT)
+        build_joint_rows(D, P, D.joints + (size_t)j * JOINT_STRIDE, D.jext ? D.jext + (size_t)j * JEXT_STRIDE : nullptr,
+                         0, 0, D.joff[j]);
     for (int r = t; r < m; r += NT) lam[r] = 0.f;
     __syncthreads();
     if (ROWS_LDS) {
@@ -2612,7 +2729,7 @@ __global__ void k_lg_rowcount(DevWorlds D, LargeDev L) {
 __global__ void k_lg_rows(DevWorlds D, StepParams P, LargeDev L) {
     const int nc = L.counters[LG_CONTACTS];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x)
-        build_joint_rows(D, P, D.joints + (size_t)k * JOINT_STRIDE, 0, 0, L.row_cnt[k]);
+        build_joint_rows(D, P, D.joints + (size_t)k * JOINT_STRIDE, nullptr, 0, 0, L.row_cnt[k]);
     if (blockIdx.x == 0 && threadIdx.x == 0) L.counters[LG_ROWS] = L.row_cnt[nc];
 }
 

@@ -42,6 +42,8 @@ def lib():
         L.odxVersion.restype = ctypes.c_char_p
         L.odxBatchCreate.restype = ctypes.c_void_p
         L.odxBatchCreate.argtypes = [i, ip, fp, ip, fp, ip, fp, dp, i]
+        L.odxBatchCreateJoints.restype = ctypes.c_void_p
+        L.odxBatchCreateJoints.argtypes = [i, ip, fp, ip, fp, ip, fp, fp, dp, i]
         L.odxBatchDestroy.argtypes = [vp]
         L.odxBatchSetSolver.argtypes = [vp, i, i, ctypes.c_float, i, i]
         L.odxBatchSetSolveOptions.argtypes = [vp, i, i]
@@ -146,10 +148,22 @@ class Batch:
         self.body_start, self.geom_start, self.ball_start = bs, gs, ls
         self.nworlds = len(worlds)
         self.nbodies = int(bs[-1])
-        self._keep = (bodies, geoms, balls)
         self.params = np.ascontiguousarray(params, np.float64)
-        h = L.odxBatchCreate(self.nworlds, _p(bs), _p(bodies), _p(gs), _p(geoms), _p(ls), _p(balls),
-                             _p(self.params), contacts_per_body)
+        # extended records of hinge / slider / fixed joints (World.jext, one per
+        # joint record; zeros for ball joints)
+        jext = None
+        if any(getattr(w, "jext", None) is not None for w in worlds):
+            jext = np.zeros((len(balls), 32), np.float32)
+            for wi, w in enumerate(worlds):
+                if getattr(w, "jext", None) is not None and len(w.balls):
+                    jext[ls[wi]:ls[wi + 1]] = np.asarray(w.jext, np.float32).reshape(-1, 32)
+        self._keep = (bodies, geoms, balls, jext)
+        if jext is None:
+            h = L.odxBatchCreate(self.nworlds, _p(bs), _p(bodies), _p(gs), _p(geoms), _p(ls), _p(balls),
+                                 _p(self.params), contacts_per_body)
+        else:
+            h = L.odxBatchCreateJoints(self.nworlds, _p(bs), _p(bodies), _p(gs), _p(geoms), _p(ls), _p(balls),
+                                       _p(jext), _p(self.params), contacts_per_body)
         _check(h, "odxBatchCreate")
         self.h = h
         rs = np.zeros(self.nworlds + 1, np.int32)

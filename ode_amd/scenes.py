@@ -81,12 +81,14 @@ def plane(a, b, c, d):
 
 
 class World:
-    """One world: body records, geom records, ball-joint records."""
+    """One world: body records, geom records, joint records (ball, or hinge /
+    slider / fixed with their extended records in jext, 32 floats each)."""
 
-    def __init__(self, bodies, geoms, balls=None):
+    def __init__(self, bodies, geoms, balls=None, jext=None):
         self.bodies = np.asarray(bodies, np.float64).reshape(-1, BODY)
         self.geoms = np.asarray(geoms, np.float64).reshape(-1, GEOM)
         self.balls = np.zeros((0, 16)) if balls is None else np.asarray(balls, np.float64).reshape(-1, 16)
+        self.jext = None if jext is None else np.asarray(jext, np.float64).reshape(-1, 32)
 
 
 def _randq(rng):

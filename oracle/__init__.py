@@ -205,11 +205,14 @@ def surface_table(nmat, entries, default=None):
     return t
 
 
-def world_step(bodies, geoms, prm, balls=None, dtype=np.float32, ccap=None, table=None, category_filter=False):
+def world_step(bodies, geoms, prm, balls=None, dtype=np.float32, ccap=None, table=None, category_filter=False,
+               jext=None):
     """One canonical world step; returns (new_bodies, contacts[nc,12], lambda[m]).
     ccap bounds the contacts (default: 4 per geom pair).  table: a
     surface_table; geoms then carry their material in slot 13 (and category /
-    collide bits in slots 14 / 15, tested when category_filter)."""
+    collide bits in slots 14 / 15, tested when category_filter).  jext:
+    [len(balls), 32] extended records of the hinge, slider and fixed joints
+    among `balls` (types 2 / 3 / 4; rows of ball joints are ignored)."""
     bodies = np.ascontiguousarray(bodies, dtype).copy()
     geoms = np.ascontiguousarray(geoms, dtype)
     balls = np.zeros((0, JOINT), dtype) if balls is None else np.ascontiguousarray(balls, dtype)
@@ -217,8 +220,17 @@ def world_step(bodies, geoms, prm, balls=None, dtype=np.float32, ccap=None, tabl
     if ccap is None:
         ccap = 4 * ng * ng + 64
     contacts = np.zeros((ccap, CONTACT), dtype)
-    lam = np.zeros(3 * ccap + 3 * len(balls) + 8, dtype)
+    lam = np.zeros(3 * ccap + 6 * len(balls) + 8, dtype)
     m = ctypes.c_int(0)
+    if jext is not None:
+        if table is not None:
+            raise ValueError("world_step: jext and table together are not supported")
+        x = np.ascontiguousarray(jext, dtype).reshape(-1, 32)
+        nc = getattr(lib(), "orc_world_step_ext_" + _real(dtype))(
+            ctypes.c_int(len(bodies)), _p(bodies), ctypes.c_int(ng), _p(geoms), ctypes.c_int(len(balls)), _p(balls),
+            _p(x), _p(np.ascontiguousarray(prm, np.float64)), _p(contacts), ctypes.c_int(ccap), _p(lam),
+            ctypes.c_int(len(lam)), ctypes.byref(m))
+        return bodies, contacts[:nc], lam[: m.value]
     if table is not None:
         t = np.ascontiguousarray(table, np.float64)
         nc = getattr(lib(), "orc_world_step_table_" + _real(dtype))(

@@ -2027,13 +2027,16 @@ template <class R> int gen_contacts(int nb, const R *bodies, int ng, const R *ge
     return (int)out.size();
 }
 
+/* bext: extended records (ORC_JEXT_STRIDE each) of the hinge, slider and
+ * fixed joints among `balls` (null when every one is a ball joint). */
 template <class R> void build_joints(const R *geoms, int nball, const R *balls, const std::vector<Contact<R>> &cs,
                                      const Params<R> &p, std::vector<Joint<R>> &js,
-                                     const SurfaceTable<R> *tab = nullptr) {
+                                     const SurfaceTable<R> *tab = nullptr, const R *bext = nullptr) {
     js.clear();
     for (int k = 0; k < nball; k++) {
         Joint<R> j;
         read_joint(balls + (size_t)k * ORC_JOINT_STRIDE, p, j);
+        if (bext && j.type >= JT_HINGE) read_joint_ext(bext + (size_t)k * ORC_JEXT_STRIDE, j);
         js.push_back(j);
     }
     for (const Contact<R> &c : cs) {
@@ -2052,7 +2055,7 @@ template <class R> void build_joints(const R *geoms, int nball, const R *balls, 
 
 template <class R> int world_step(int nb, R *bodies, int ng, const R *geoms, int nball, const R *balls, const double *P,
                                   R *contacts, int ccap, R *lambda, int lcap, int *m_out,
-                                  const SurfaceTable<R> *tab = nullptr) {
+                                  const SurfaceTable<R> *tab = nullptr, const R *bext = nullptr) {
     Params<R> p = read_params<R>(P);
     std::vector<Contact<R>> cs;
     int nc = gen_contacts(nb, bodies, ng, geoms, nball, balls, p, cs, tab);
@@ -2062,7 +2065,7 @@ template <class R> int world_step(int nb, R *bodies, int ng, const R *geoms, int
         o[6] = cs[i].depth; o[7] = (R)cs[i].g1; o[8] = (R)cs[i].g2; o[9] = -1; o[10] = -1; o[11] = 0;
     }
     std::vector<Joint<R>> js;
-    build_joints(geoms, nball, balls, cs, p, js, tab);
+    build_joints(geoms, nball, balls, cs, p, js, tab, bext);
     int m = quickstep(nb, bodies, js, p, lambda, lcap);
     if (m_out) *m_out = m;
     return nc;
@@ -2221,6 +2224,16 @@ int orc_world_step_f64(int nb, double *bodies, int ng, const double *geoms, int 
                        double *contacts, int ccap, double *lambda, int lcap, int *m_out) {
     return world_step(nb, bodies, ng, geoms, nball, balls, params, contacts, ccap, lambda, lcap, m_out);
 }
+
+#define ORC_WORLD_STEP_EXT(SUF, R)                                                                             \
+    int orc_world_step_ext_##SUF(int nb, R *bodies, int ng, const R *geoms, int nball, const R *balls,          \
+                                 const R *bext, const double *params, R *contacts, int ccap, R *lambda,        \
+                                 int lcap, int *m_out) {                                                       \
+        return world_step(nb, bodies, ng, geoms, nball, balls, params, contacts, ccap, lambda, lcap, m_out,   \
+                          (const SurfaceTable<R> *)nullptr, bext);                                             \
+    }
+ORC_WORLD_STEP_EXT(f32, float)
+ORC_WORLD_STEP_EXT(f64, double)
 
 #define ORC_WORLD_STEP_TABLE(SUF, R)                                                                           \
     int orc_world_step_table_##SUF(int nb, R *bodies, int ng, const R *geoms, int nball, const R *balls,        \

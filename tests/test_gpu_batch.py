@@ -239,6 +239,109 @@ def test_capsule_chain_with_ball_joints():
     B.close()
 
 
+def hinge_slider_fixed_world(rng):
+    """Five spheres (r 0.3 m) hung in a chain of hinges along x from the world
+    (a motor on one hinge, a limit active on another, a motor pushing against
+    its limit on a third), a sphere on a powered slider to the world, a fixed
+    pair, and the z = 0 plane under the chain's lowest link so that it has
+    contacts.  Joint frames are set with every body at identity orientation,
+    as tests/test_gpu_joints.py does through the drop-in API; velocities are
+    random."""
+    INF = float("inf")
+    m, I = S.sphere_mass(1000.0 / 9.0, 0.3)
+    bodies, geoms, recs, exts = [], [], [], []
+
+    def body(pos):
+        bodies.append(S._body(pos, (1, 0, 0, 0), m, I, lvel=rng.normal(0, 0.5, 3), avel=rng.normal(0, 1.0, 3)))
+        geoms.append(S.geom(S.SPHERE, len(bodies) - 1, (0.3,)))
+        return len(bodies) - 1
+
+    def ext(**kw):
+        x = np.zeros(32)
+        x[6] = 1.0
+        x[16], x[17], x[18], x[19], x[20] = kw.get("lostop", -INF), kw.get("histop", INF), kw.get("vel", 0.0), \
+            kw.get("fmax", 0.0), 1.0
+        x[21], x[22], x[23], x[26], x[27] = 1e-5, 0.2, 1e-5, 0.2, 1e-5
+        return x
+
+    chain = [body((float(i), 0.0, 0.29 + 0.05 * i)) for i in range(5)]
+    limits = {2: dict(fmax=3.0, vel=-1.0), 3: dict(lostop=0.1), 4: dict(lostop=0.2, fmax=2.0, vel=1.0)}
+    for i in range(5):
+        b1, b2 = (chain[0], -1) if i == 0 else (chain[i - 1], chain[i])
+        anchor = np.array([i - 0.5, 0.0, 0.29 + 0.05 * i - 0.025])
+        rec = np.zeros(16)
+        rec[0], rec[1], rec[2] = 2, b1, b2
+        rec[3:6] = anchor - bodies[b1][0:3]
+        rec[6:9] = anchor - bodies[b2][0:3] if b2 >= 0 else anchor
+        rec[9], rec[10] = 0.2, 1e-5
+        x = ext(**limits.get(i, {}))
+        x[0:3] = x[3:6] = (0.0, 1.0, 0.0)
+        recs.append(rec)
+        exts.append(x)
+    s = body((0.0, 4.0, 1.0))
+    rec = np.zeros(16)
+    rec[0], rec[1], rec[2] = 3, s, -1
+    x = ext(fmax=50.0, vel=0.5, lostop=-0.5, histop=0.5)
+    x[0:3] = np.array([1.0, 0.0, 1.0]) / np.sqrt(2.0)
+    x[10:13] = bodies[s][0:3]
+    recs.append(rec)
+    exts.append(x)
+    fa, fb = body((3.0, 4.0, 1.0)), body((3.5, 4.5, 1.5))
+    rec = np.zeros(16)
+    rec[0], rec[1], rec[2] = 4, fa, fb
+    x = ext()
+    x[10:13] = bodies[fa][0:3] - bodies[fb][0:3]
+    recs.append(rec)
+    exts.append(x)
+    geoms.append(S.plane(0, 0, 1, 0))
+    return S.World(bodies, geoms, recs, exts)
+
+
+def test_hinge_slider_fixed_joints_match_oracle():
+    # batched worlds with hinge, slider and fixed joints (limits, motors and a
+    # motor pushing against its limit evaluated on the device every step) and
+    # contacts, in the reference row order: one step against the float64
+    # oracle, then five more steps tracking it
+    say("case: batched hinge / slider / fixed joints")
+    rng = np.random.default_rng(14)
+    worlds = [hinge_slider_fixed_world(rng) for _ in range(6)]
+    prm = O.params(iters=30, skip_connected=1)
+    odx = _odx()
+    B = odx.Batch(worlds, prm)
+    B.set_solver(order=0, max_iters=30, tol=0.0)
+    B.step(1)
+    gb = B.bodies()
+    assert B.overflow().sum() == 0
+    st = B.stats()
+    for wi, w in enumerate(worlds):
+        ref_b, ref_c, ref_l = O.world_step(w.bodies, w.geoms, prm, balls=w.balls, jext=w.jext, dtype=np.float64)
+        assert len(ref_c) > 0 and st["contacts"][wi] == len(ref_c)
+        lam = B.lam(wi)
+        assert len(lam) == len(ref_l), (len(lam), len(ref_l))
+        assert relerr(lam, ref_l) < TOL_LAMBDA, relerr(lam, ref_l)
+        wb = B.world_bodies(wi, gb)
+        assert relerr(wb[:, 0:3], ref_b[:, 0:3]) < TOL_STATE
+        assert np.max(np.abs(wb[:, 4:8] - ref_b[:, 4:8])) < TOL_STATE
+        vs = np.hstack([wb[:, 8:11], wb[:, 12:15]])
+        vr = np.hstack([ref_b[:, 8:11], ref_b[:, 12:15]])
+        assert np.max(np.abs(vs - vr)) / max(np.max(np.abs(vr)), 1.0) < TOL_STATE
+    # rows per joint: 6 for the hinges with a motor or an active limit, the
+    # slider and the fixed joint, 5 for the others
+    m = len(B.lam(0)) - 3 * st["contacts"][0]
+    say("  joint rows", m, "contacts", st["contacts"][:6])
+    assert m == 5 + 5 + 6 + 6 + 6 + 6 + 6
+    B.step(5)
+    gb = B.bodies()
+    for wi, w in enumerate(worlds):
+        b = w.bodies.copy()
+        for _ in range(6):
+            b, _, _ = O.world_step(b, w.geoms, prm, balls=w.balls, jext=w.jext, dtype=np.float64)
+        wb = B.world_bodies(wi, gb)
+        err = max(relerr(wb[:, 0:3], b[:, 0:3]), np.max(np.abs(wb[:, 4:8] - b[:, 4:8])))
+        assert err < 5 * TOL_STATE, (wi, err)
+    B.close()
+
+
 def test_hash_space_semantics_same_contacts():
     say("case: hash space")
     rng = np.random.default_rng(5)
