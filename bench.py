@@ -167,10 +167,12 @@ def measure(B, args, warm, nworlds):
     lds_rows = args.lds_rows if args.lds_rows >= 0 else 400
     if args.lds_rows < 0:
         # the largest budget that still fits five worlds in a CU's 160 KB of
-        # LDS (rows grow as towers fall), or 1.25x the largest world seen
+        # LDS, or 1.25x the largest world seen if that is more: rows keep
+        # growing after the warmup (towers settle into more contacts), so the
+        # budget is not sized to the warmup's largest world alone
         nb_max = int(np.max(np.diff(B.body_start)))
-        def lds_bytes(r):  # solve_small_lds_bytes
-            return (nb_max + 1) * 64 + r * 72 + (r + 2) // 2 * 4 + 64 * 4
+        def lds_bytes(r):  # solve_lds_floats * 4
+            return 4 * ((nb_max + 1) * 16 + r * 17 + r * 3 // 4 + 16 + 65 + 64)
         five = max(r for r in range(16, 2049) if 5 * lds_bytes(r) <= 160 * 1024)
         want = max(five, -(-int(max_rows * 1.25) // 16) * 16)
         while want > max_rows:

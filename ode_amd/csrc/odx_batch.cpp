@@ -381,11 +381,9 @@ constexpr int ODX_DEFAULT_LDS_ROWS = 400;
 // Largest row count <= want whose LDS footprint fits the kernel's limit
 // (same accounting as solve_small_lds_bytes).
 static int odx_lds_rows_fit(int lds_bodies, int want) {
-    const int per_row = (16 + 2) * (int)sizeof(float) + (int)sizeof(unsigned short);
-    // fixed part: bodies, the phase table's extra entry and padding, 64 trash words
-    const int fixed = lds_bodies * 16 * (int)sizeof(float) + 4 + 64 * (int)sizeof(float);
-    const int room = (SOLVE_LDS_MAX - fixed) / per_row;
-    return std::max(0, std::min(want, std::min(room, SW_MAXROWS)));
+    int r = std::max(0, std::min(want, SW_MAXROWS));
+    while (r > 0 && solve_lds_floats(lds_bodies, r) * (int)sizeof(float) > SOLVE_LDS_MAX) r--;
+    return r;
 }
 
 void odx_params_from(const double *p, StepParams &P, Surface &S) {

@@ -90,6 +90,17 @@ constexpr int ODX_MAX_SPLITS = 8;
 
 // Largest dynamic LDS of the batched solve kernel (bytes per workgroup).
 constexpr int SOLVE_LDS_MAX = 65536;
+// LDS budgets of the batched solve besides the rows: work items per world
+// (three quarters of the row budget, plus 16) and sweep phases per world.  A
+// world over any budget sweeps from global memory instead.
+__host__ __device__ inline int lds_items_of(int lds_rows) { return lds_rows * 3 / 4 + 16; }
+constexpr int LDS_PHASES = 128;
+// dynamic LDS of the batched solve, in floats: bodies [lds_bodies][16], rows
+// [lds_rows][16], lambda [lds_rows], items [lds_items_of(lds_rows)], phase
+// table [LDS_PHASES + 1] (16-bit, padded to whole floats), 64 trash words
+__host__ __device__ inline int solve_lds_floats(int lds_bodies, int lds_rows) {
+    return lds_bodies * 16 + lds_rows * 17 + lds_items_of(lds_rows) + (LDS_PHASES + 2) / 2 + 64;
+}
 
 // Split of a joint's rows for the coloured schedule, which keeps the
 // reference's two passes (ReorderPrep, quickstep.cpp:2329-2355: rows without

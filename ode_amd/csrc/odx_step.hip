@@ -1348,12 +1348,10 @@ __device__ float world_residual(const DevWorlds &D, const StepParams &P, int r0,
 // Body (16 floats): 0-5 fc, 6 invMass, 7-15 invI_w (3x3 row-major).  A row
 // without a second body points at the zero body at index nb.
 constexpr int LDS_ROW = 16, LDS_BODY = 16;
-// Dynamic LDS of the solve, in floats: bodies [lds_bodies][16], rows
-// [lds_rows][16], lambda [lds_rows], items [lds_rows], phase table
-// [lds_rows + 1] (16-bit, padded to whole floats), then one trash word per
-// lane for the masked row updates.
+// Offset (floats) of the trash words, one per lane for the masked row
+// updates, at the end of the solve's dynamic LDS (solve_lds_floats).
 __host__ __device__ __forceinline__ int solve_lds_trash_offset(const StepParams &P) {
-    return P.lds_bodies * LDS_BODY + P.lds_rows * (LDS_ROW + 2) + (P.lds_rows + 2) / 2;
+    return solve_lds_floats(P.lds_bodies, P.lds_rows) - 64;
 }
 
 // 64-byte records start on one of only two 16-bank boundaries of LDS, so the
@@ -1725,14 +1723,14 @@ __device__ __forceinline__ void solve_world(const DevWorlds &D, const StepParams
     const bool warm = P.warm >= 1 && D.lambda0;
     if (t == 0 && D.wtime) D.wtime[2 * w] = (long long)wall_clock64();
     // LDS path: bodies [lds_bodies][16], rows [lds_rows][16], lambda [lds_rows],
-    // items [lds_rows], phase starts [lds_rows + 1] (16-bit)
+    // items [lds_items_of(lds_rows)], phase starts [LDS_PHASES + 1] (16-bit)
     float *s_body = s_dyn;
     float *s_rows = s_dyn + P.lds_bodies * LDS_BODY;
     float *s_lam = s_rows + P.lds_rows * LDS_ROW;
     int *s_sched = reinterpret_cast<int *>(s_lam + P.lds_rows);
-    unsigned short *s_ps = reinterpret_cast<unsigned short *>(s_sched + P.lds_rows);
+    unsigned short *s_ps = reinterpret_cast<unsigned short *>(s_sched + lds_items_of(P.lds_rows));
     float *s_trash = s_dyn + solve_lds_trash_offset(P);
-    bool lds = m > 0 && m <= P.lds_rows && nb < P.lds_bodies;
+    bool lds = m > 0 && m <= P.lds_rows && nb < P.lds_bodies && L <= LDS_PHASES && ps[L] <= lds_items_of(P.lds_rows);
     if (lds) {
         lds = lds_stage(D, b0, nb, r0, m, L, ps, sched, perm, s_body, s_rows, s_sched, s_ps);
         __syncthreads();
@@ -2368,7 +2366,7 @@ void launch_prepare_small(const DevWorlds &D, const StepParams &P, hipStream_t s
     hipLaunchKernelGGL(k_prepare_small, dim3(P.wcount), dim3(64), 0, s, D, P);
 }
 size_t solve_small_lds_bytes(const StepParams &P) {
-    const size_t lds = (size_t)solve_lds_trash_offset(P) * sizeof(float) + 64 * sizeof(float);
+    const size_t lds = (size_t)solve_lds_floats(P.lds_bodies, P.lds_rows) * sizeof(float);
     const size_t fc = (size_t)SW_MAXB * 6 * sizeof(float);
     return lds > fc ? lds : fc;
 }
