@@ -760,7 +760,7 @@ dReal dWorldGetQuickStepW(dWorldID w) { return w->w; }
 void dWorldSetContactMaxCorrectingVel(dWorldID w, dReal vel) { w->max_vel = vel; }
 void dWorldSetContactSurfaceLayer(dWorldID w, dReal depth) { w->min_depth = depth; }
 void odxWorldSetRowOrder(dWorldID w, int order) { w->order = (order == 1 || order == 2) ? order : 0; }
-void odxWorldSetSweepMode(dWorldID w, int mode) { w->sweep_mode = (mode == 1 || mode == 2) ? mode : 0; }
+void odxWorldSetSweepMode(dWorldID w, int mode) { w->sweep_mode = (mode >= 1 && mode <= 3) ? mode : 0; }
 
 }  // extern "C"
 
@@ -2071,6 +2071,7 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     // ---- schedule (see odx_step.hip): dependency levels over the ReorderPrep
     // order, or greedy colours of whole joints
     std::vector<int> sched(m), phase;
+    int wave_groups = 0;
     {
         std::vector<int> level;
         int L = 0;
@@ -2088,11 +2089,85 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
                     level[r] = lv - 1;
                     L = std::max(L, lv);
                 }
-            std::vector<int> cnt(L + 1, 0);
-            for (int r = 0; r < m; r++) cnt[level[r] + 1]++;
-            for (int p = 1; p <= L; p++) cnt[p] += cnt[p - 1];
-            phase = cnt;
-            for (int r = 0; r < m; r++) sched[cnt[level[r]]++] = (r << 3) | 1;
+            // sweep mode 3 (automatic for the reference order with a fixed
+            // sweep count and enough islands): islands never share a body, so
+            // each wave of the step kernel can sweep the levels of its own
+            // islands without meeting the others; the Gauss-Seidel order
+            // inside every island is unchanged
+            int isl_rows = 0;
+            {
+                std::vector<char> has(std::max(nisl, 1), 0);
+                for (int r = 0; r < m; r++) has[island[rb1[r]]] = 1;
+                for (int i = 0; i < nisl; i++) isl_rows += has[i];
+            }
+            const float xc = (float)w->iterations * w->dyn_extra_factor;
+            const bool dyn_sched = (xc < (float)ODX_MAX_EXTRA_SWEEPS ? (int)xc : ODX_MAX_EXTRA_SWEEPS) != 0 ||
+                                   w->dyn_premature != 0.f;
+            wave_groups = 0;
+            if (w->order == 0 && !dyn_sched &&
+                (w->sweep_mode == 3 || (w->sweep_mode == 0 && isl_rows >= 4 * SL_STAGED_WAVES)))
+                wave_groups = SL_STAGED_WAVES;
+            if (wave_groups > 0) {
+                const int G = wave_groups;
+                // items per level of each island, then islands (deepest first)
+                // to the group whose passes (sum over levels of ceil(items / 64))
+                // grow least
+                std::vector<std::vector<int>> ilc(std::max(nisl, 1));
+                for (int r = 0; r < m; r++) {
+                    std::vector<int> &c = ilc[island[rb1[r]]];
+                    if ((int)c.size() <= level[r]) c.resize(level[r] + 1, 0);
+                    c[level[r]]++;
+                }
+                std::vector<int> order_isl;
+                for (int i = 0; i < nisl; i++)
+                    if (!ilc[i].empty()) order_isl.push_back(i);
+                std::stable_sort(order_isl.begin(), order_isl.end(),
+                                 [&](int a, int b) { return ilc[a].size() > ilc[b].size(); });
+                std::vector<std::vector<int>> gc(G);
+                std::vector<long> gcost(G, 0);
+                std::vector<int> group(std::max(nisl, 1), 0);
+                auto passes = [](int n) { return (n + 63) / 64; };
+                for (int i : order_isl) {
+                    int best = 0;
+                    long bestc = -1;
+                    for (int g = 0; g < G; g++) {
+                        long d = 0;
+                        for (size_t l = 0; l < ilc[i].size(); l++) {
+                            const int have = l < gc[g].size() ? gc[g][l] : 0;
+                            d += passes(have + ilc[i][l]) - passes(have);
+                        }
+                        if (bestc < 0 || gcost[g] + d < bestc) { bestc = gcost[g] + d; best = g; }
+                    }
+                    std::vector<int> &c = gc[best];
+                    if (c.size() < ilc[i].size()) c.resize(ilc[i].size(), 0);
+                    for (size_t l = 0; l < ilc[i].size(); l++) c[l] += ilc[i][l];
+                    gcost[best] = bestc;
+                    group[i] = best;
+                }
+                // phase table: G + 1 offsets of the groups' level lists (from
+                // entry G + 1), then per group its level starts and end; items
+                // of a level in row order
+                std::vector<int> first(G + 1, 0);
+                for (int g = 0; g < G; g++) first[g + 1] = first[g] + (int)gc[g].size();
+                std::vector<int> start(first[G] + 1, 0);
+                for (int g = 0; g < G; g++)
+                    for (size_t l = 0; l < gc[g].size(); l++) start[first[g] + l + 1] = gc[g][l];
+                for (int k = 1; k <= first[G]; k++) start[k] += start[k - 1];
+                phase.assign(G + 1, 0);
+                for (int g = 0; g < G; g++) {
+                    phase[g] = (int)(phase.size() - (G + 1));
+                    for (size_t l = 0; l <= gc[g].size(); l++) phase.push_back(start[first[g] + l]);
+                }
+                phase[G] = (int)(phase.size() - (G + 1));
+                std::vector<int> pos(start.begin(), start.end() - 1);
+                for (int r = 0; r < m; r++) sched[pos[first[group[island[rb1[r]]]] + level[r]]++] = (r << 3) | 1;
+            } else {
+                std::vector<int> cnt(L + 1, 0);
+                for (int r = 0; r < m; r++) cnt[level[r] + 1]++;
+                for (int p = 1; p <= L; p++) cnt[p] += cnt[p - 1];
+                phase = cnt;
+                for (int r = 0; r < m; r++) sched[cnt[level[r]]++] = (r << 3) | 1;
+            }
         } else {
             level.assign(nj, 0);
             std::vector<std::vector<char>> used(nb);
@@ -2236,6 +2311,21 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
             return 0;
         };
         std::vector<int> segs = w->order == 2 ? seg_table : std::vector<int>{0, 0, L};
+        if (wave_groups > 0) {
+            // island groups: a header of wave_groups + 1 offsets (from entry
+            // wave_groups + 1) of each group's level list, read as segments
+            const int G = wave_groups;
+            if ((int)phase.size() < G + 2 || phase[0] != 0) return bad("group table");
+            segs.clear();
+            for (int g = 0; g < G; g++) {
+                const int a = phase[g], b = phase[g + 1];
+                if (a < 0 || b <= a || (size_t)(G + 1 + b) > phase.size()) return bad("group table");
+                segs.push_back(0);
+                segs.push_back(G + 1 + a);
+                segs.push_back(b - a - 1);
+            }
+            if ((size_t)(G + 1 + phase[G]) != phase.size()) return bad("group table");
+        }
         for (size_t q = 0; q + 2 < segs.size(); q += 3) {
             const int so = segs[q], po = segs[q + 1], Ls = segs[q + 2];
             if (so < 0 || po < 0 || Ls < 0 || (size_t)po + Ls >= phase.size()) return bad("segment table");
@@ -2324,7 +2414,8 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     P.max_vel = w->max_vel; P.min_depth = w->min_depth;
     P.maxc = 4;
     P.order = w->order;
-    P.wave_sweep = w->sweep_mode;
+    P.wave_sweep = w->sweep_mode == 3 ? 0 : w->sweep_mode;
+    P.wave_groups = wave_groups;
     P.lambda_floor = 1e-3f;
     P.collide = 0;
     P.feedback = want_fb ? 1 : 0;

@@ -83,6 +83,7 @@ struct StepParams {
     // cat_filter a pair needs (cat1 & col2) | (cat2 & col1) != 0
     int nmat, cat_filter;
     int wave_sweep;      // drop-in world: 0 auto, 1 block-wide levels, 2 one wave (the kernel sees 0 or 1)
+    int wave_groups;     // drop-in world: > 0 = islands split into this many wave groups (phase table header)
 };
 
 // Independent world ranges a batch may be split into (one stream each).
@@ -90,6 +91,9 @@ constexpr int ODX_MAX_SPLITS = 8;
 
 // Largest dynamic LDS of the batched solve kernel (bytes per workgroup).
 constexpr int SOLVE_LDS_MAX = 65536;
+// waves of the staged single-world step kernel (512 threads): the island
+// groups of sweep mode 3
+constexpr int SL_STAGED_WAVES = 8;
 // LDS budgets of the batched solve besides the rows: work items per world
 // (three quarters of the row budget, plus 16) and sweep phases per world.  A
 // world over any budget sweeps from global memory instead.

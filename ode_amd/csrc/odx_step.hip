@@ -2151,10 +2151,40 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
                 return;
             }
             nsw++;
+            // island groups (sweep mode 3): the phase table opens with
+            // wave_groups + 1 offsets of each group's level list; wave g sweeps
+            // group g's levels alone, ordered by wavefront fences, and the
+            // block meets once per sweep
+            auto group_levels = [&](auto &&levels_of) {
+                const int G = P.wave_groups, wv = t >> 6;
+                if (wv < G) {
+                    const int *gph = ph + (G + 1) + ph[wv];
+                    levels_of(t & 63, 64, gph, ph[wv + 1] - ph[wv] - 1, [] {
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    });
+                }
+                __syncthreads();
+            };
+            if (!STAGED && P.wave_groups > 0) {
+                group_levels([&](const int ln, const int stride, const int *gp, const int Lg, auto sync) {
+                    for (int p = 0; p < Lg; p++) {
+                        const int s = gp[p], e = gp[p + 1];
+                        for (int k = s + ln; k < e; k += stride)
+                            process_item_l(rows4, nullptr, sched[k], lam, s_fc, fa, isl_b, ist);
+                        sync();
+                    }
+                });
+                return;
+            }
             if (STAGED) {
                 // levels from LDS with the next level's rows prefetched; lane
-                // `ln` of `stride` takes items ln, ln + stride, ...; `sync` ends a level
-                auto levels = [&](const int ln, const int stride, auto sync) {
+                // `ln` of `stride` takes items ln, ln + stride, ...; `sync` ends a
+                // level; levels `ph_[0 .. L_]`
+                auto levels = [&](const int ln, const int stride, const int *ph_, const int L_, auto sync) {
+                    const int *ph = ph_;
+                    const int L = L_;
                     int s = ph[0], e = ph[1];
                     int cur = s + ln < e ? sched[s + ln] : -1;
                     float4 pre[8], npre[8];
@@ -2188,20 +2218,24 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
                         }
                     }
                 };
-                if (P.wave_sweep) {
+                if (P.wave_groups > 0) {
+                    group_levels([&](const int ln, const int stride, const int *gp, const int Lg, auto sync) {
+                        levels(ln, stride, gp, Lg, sync);
+                    });
+                } else if (P.wave_sweep) {
                     // narrow levels: the first wave sweeps alone.  A wave's LDS
                     // accesses complete in order, so a level needs no block
                     // barrier, only a fence that keeps the compiler from moving
                     // accesses across it; the block meets once per sweep.
                     if (t < 64)
-                        levels(t, 64, [] {
+                        levels(t, 64, ph, L, [] {
                             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                             __builtin_amdgcn_wave_barrier();
                             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                         });
                     __syncthreads();
                 } else {
-                    levels(t, NT, [] { __syncthreads(); });
+                    levels(t, NT, ph, L, [] { __syncthreads(); });
                 }
             } else {
                 for (int p = 0; p < L; p++) {

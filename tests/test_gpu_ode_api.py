@@ -693,12 +693,14 @@ def test_auto_disable_matches_oracle_through_sleep_and_wake():
 
 @pytest.mark.parametrize("dynamic", [False, True])
 def test_sweep_modes_give_identical_steps(dynamic):
-    # the workgroup-wide and the one-wave level sweeps run the same
-    # Gauss-Seidel order (a level's items touch disjoint bodies): bit-equal
-    # states over a 30-step frame loop
+    # the workgroup-wide, the one-wave and the island-group level sweeps run
+    # the same Gauss-Seidel order (a level's items touch disjoint bodies, and
+    # islands share none): bit-equal states over a 30-step frame loop (the
+    # island groups need a fixed sweep count; with the dynamic count mode 3
+    # falls back to the automatic choice)
     say("case: sweep modes, dynamic", dynamic)
     out = []
-    for mode in (1, 2):
+    for mode in (1, 2, 3):
         scene = settle_scene(6, 30)
         F = Frame(scene, dynamic=dynamic)
         F.L.odxWorldSetSweepMode(F.w, mode)
@@ -709,6 +711,7 @@ def test_sweep_modes_give_identical_steps(dynamic):
         out.append(F.state())
         F.close()
     np.testing.assert_array_equal(out[0], out[1])
+    np.testing.assert_array_equal(out[0], out[2])
 
 
 def test_quickstep_matches_oracle_with_rows_beyond_lds():
