@@ -29,6 +29,29 @@ struct DContact {
     float depth;
 };
 
+// Working arrays of the box-box clipper, indexed by a count that differs per
+// lane: two clip buffers of 8 points (CW_A, CW_B), the clipped points in 3D
+// (CW_PT) with their depths (CW_DEP), and the centroid angles of cullPoints
+// (CW_ANG).  ClipPriv keeps them in a private array; ClipLds in one lane's
+// column of an LDS block (element i at i * 64, so that a wave's 64 lanes read
+// 64 consecutive words).
+enum { CW_A = 0, CW_B = 16, CW_PT = 32, CW_DEP = 56, CW_ANG = 64, CLIP_WS = 72 };
+struct ClipPriv {
+    float v[CLIP_WS];
+    ODX_DEV float &operator[](int i) { return v[i]; }
+};
+struct ClipLds {
+    float *p;  // this lane's first element
+    ODX_DEV float &operator[](int i) const { return p[i << 6]; }
+};
+
+// v[k] for k in 0..2 by selects, so that v can stay in registers.
+ODX_DEV float sel3(const float *v, int k) { return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]); }
+// Column k (0..2) of a 3x4 row-major matrix.
+ODX_DEV void col3(float *o, const float *R, int k) {
+    o[0] = sel3(R, k); o[1] = sel3(R + 4, k); o[2] = sel3(R + 8, k);
+}
+
 // computeAABB: sphere.cpp:59-67, box.cpp:60-76, capsule.cpp:60-74, plane.cpp:81-105
 ODX_DEV void geom_aabb(const DGeom &G, float *ab) {
     const float *p = G.pos, *R = G.R;
@@ -136,9 +159,9 @@ ODX_DEV int collide_sphere_box(const DGeom &s, const DGeom &b, DContact &c) {
         float tsel = mini == 0 ? t[0] : (mini == 1 ? t[1] : t[2]);
         float sg = tsel > 0.f ? 1.f : -1.f;
         // R * (sg * e_mini)
-        c.normal[0] = sg * b.R[0 + mini];
-        c.normal[1] = sg * b.R[4 + mini];
-        c.normal[2] = sg * b.R[8 + mini];
+        c.normal[0] = sg * sel3(b.R + 0, mini);
+        c.normal[1] = sg * sel3(b.R + 4, mini);
+        c.normal[2] = sg * sel3(b.R + 8, mini);
         c.depth = mind + s.p[0];
         return 1;
     }
@@ -165,25 +188,29 @@ ODX_DEV int collide_sphere_plane(const DGeom &s, const DGeom &pl, DContact &c) {
     return 1;
 }
 
-// Clip a polygon (n <= 8 points) against sign * x[dir] < h.  One pass of
-// intersectRectQuad box.cpp:212-263; returns the new count, or a count with
-// bit 8 set when the output filled up (the reference stops clipping then).
-ODX_DEV int clip_pass(const float *q, int nq, float *r, int dir, float sign, float h) {
+// Clip a polygon (n <= 8 points, at ws[q..]) against sign * x[dir] < h into
+// ws[r..].  One pass of intersectRectQuad box.cpp:212-263; returns the new
+// count, or a count with bit 8 set when the output filled up (the reference
+// stops clipping then).
+template <class W>
+ODX_DEV int clip_pass(W &ws, int q, int nq, int r, int dir, float sign, float h) {
     int nr = 0;
     for (int i = 0; i < nq; i++) {
-        const float *pq = q + 2 * i;
-        const float *nx = (i + 1 < nq) ? pq + 2 : q;
-        bool in0 = sign * pq[dir] < h;
+        const int pq = q + 2 * i;
+        const int nx = (i + 1 < nq) ? pq + 2 : q;
+        const float p0 = ws[pq], p1 = ws[pq + 1], n0 = ws[nx], n1 = ws[nx + 1];
+        const float pd = dir ? p1 : p0, po = dir ? p0 : p1, nd = dir ? n1 : n0, no = dir ? n0 : n1;
+        bool in0 = sign * pd < h;
         if (in0) {
-            r[2 * nr] = pq[0]; r[2 * nr + 1] = pq[1];
+            ws[r + 2 * nr] = p0; ws[r + 2 * nr + 1] = p1;
             nr++;
             if (nr & 8) return nr | 0x100;
         }
-        bool in1 = sign * nx[dir] < h;
+        bool in1 = sign * nd < h;
         if (in0 ^ in1) {
-            float t = (nx[1 - dir] - pq[1 - dir]) / (nx[dir] - pq[dir]) * (sign * h - pq[dir]);
-            r[2 * nr + (1 - dir)] = pq[1 - dir] + t;
-            r[2 * nr + dir] = sign * h;
+            float t = (no - po) / (nd - pd) * (sign * h - pd);
+            ws[r + 2 * nr + (1 - dir)] = po + t;
+            ws[r + 2 * nr + dir] = sign * h;
             nr++;
             if (nr & 8) return nr | 0x100;
         }
@@ -191,59 +218,62 @@ ODX_DEV int clip_pass(const float *q, int nq, float *r, int dir, float sign, flo
     return nr;
 }
 
-// intersectRectQuad box.cpp:212-263: clip quad p against the rectangle
-// |x| < h[0], |y| < h[1]; result in ret (<= 8 points).
-ODX_DEV int intersect_rect_quad(const float h[2], const float p[8], float ret[16]) {
-    float a[16], b[16];
-#pragma unroll
-    for (int i = 0; i < 8; i++) a[i] = p[i];
+// intersectRectQuad box.cpp:212-263: clip the quad at ws[CW_A..CW_A+7]
+// against the rectangle |x| < h0, |y| < h1.  Returns the point count (<= 8);
+// *at = the buffer (CW_A or CW_B) that holds the points.
+template <class W>
+ODX_DEV int intersect_rect_quad(float h0, float h1, W &ws, int *at) {
     int n = 4;
-    // pass order: (dir 0, sign -1), (0, +1), (1, -1), (1, +1); ping-pong a/b
-    n = clip_pass(a, n, b, 0, -1.f, h[0]);
-    if (n & 0x100) { n &= 0xff; for (int i = 0; i < 2 * n; i++) ret[i] = b[i]; return n; }
-    n = clip_pass(b, n, a, 0, 1.f, h[0]);
-    if (n & 0x100) { n &= 0xff; for (int i = 0; i < 2 * n; i++) ret[i] = a[i]; return n; }
-    n = clip_pass(a, n, b, 1, -1.f, h[1]);
-    if (n & 0x100) { n &= 0xff; for (int i = 0; i < 2 * n; i++) ret[i] = b[i]; return n; }
-    n = clip_pass(b, n, a, 1, 1.f, h[1]);
-    n &= 0xff;
-    for (int i = 0; i < 2 * n; i++) ret[i] = a[i];
-    return n;
+    // pass order: (dir 0, sign -1), (0, +1), (1, -1), (1, +1); ping-pong A/B
+    n = clip_pass(ws, CW_A, n, CW_B, 0, -1.f, h0);
+    if (n & 0x100) { *at = CW_B; return n & 0xff; }
+    n = clip_pass(ws, CW_B, n, CW_A, 0, 1.f, h0);
+    if (n & 0x100) { *at = CW_A; return n & 0xff; }
+    n = clip_pass(ws, CW_A, n, CW_B, 1, -1.f, h1);
+    if (n & 0x100) { *at = CW_B; return n & 0xff; }
+    n = clip_pass(ws, CW_B, n, CW_A, 1, 1.f, h1);
+    *at = CW_A;
+    return n & 0xff;
 }
 
-// cullPoints box.cpp:274-336: choose m of n points spread by angle around the
-// centroid, starting with i0.
-ODX_DEV void cull_points(int n, const float *p, int m, int i0, int *iret) {
+// cullPoints box.cpp:274-336: choose m (<= ODX_MAXC) of the n points at
+// ws[p..] spread by angle around the centroid, starting with i0.
+template <class W>
+ODX_DEV void cull_points(int n, W &ws, int p, int m, int i0, int *iret) {
     float a, cx, cy, q;
-    if (n == 1) { cx = p[0]; cy = p[1]; }
-    else if (n == 2) { cx = 0.5f * (p[0] + p[2]); cy = 0.5f * (p[1] + p[3]); }
+    if (n == 1) { cx = ws[p]; cy = ws[p + 1]; }
+    else if (n == 2) { cx = 0.5f * (ws[p] + ws[p + 2]); cy = 0.5f * (ws[p + 1] + ws[p + 3]); }
     else {
         a = 0.f; cx = 0.f; cy = 0.f;
         for (int i = 0; i < n - 1; i++) {
-            q = p[i * 2] * p[i * 2 + 3] - p[i * 2 + 2] * p[i * 2 + 1];
+            const float x0 = ws[p + i * 2], y0 = ws[p + i * 2 + 1], x1 = ws[p + i * 2 + 2], y1 = ws[p + i * 2 + 3];
+            q = x0 * y1 - x1 * y0;
             a += q;
-            cx += q * (p[i * 2] + p[i * 2 + 2]);
-            cy += q * (p[i * 2 + 1] + p[i * 2 + 3]);
+            cx += q * (x0 + x1);
+            cy += q * (y0 + y1);
         }
-        q = p[n * 2 - 2] * p[1] - p[0] * p[n * 2 - 1];
+        const float xl = ws[p + n * 2 - 2], yl = ws[p + n * 2 - 1], x0 = ws[p], y0 = ws[p + 1];
+        q = xl * y0 - x0 * yl;
         a = 1.f / (3.f * (a + q));
-        cx = a * (cx + q * (p[n * 2 - 2] + p[0]));
-        cy = a * (cy + q * (p[n * 2 - 1] + p[1]));
+        cx = a * (cx + q * (xl + x0));
+        cy = a * (cy + q * (yl + y0));
     }
-    float A[8];
-    for (int i = 0; i < n; i++) A[i] = atan2f(p[i * 2 + 1] - cy, p[i * 2] - cx);
+    for (int i = 0; i < n; i++) ws[CW_ANG + i] = atan2f(ws[p + i * 2 + 1] - cy, ws[p + i * 2] - cx);
     unsigned avail = (1u << n) - 1u;
     avail &= ~(1u << i0);
     iret[0] = i0;
     const double PI = 3.14159265358979323846;
-    for (int j = 1; j < m; j++) {
-        a = (float)(double(j) * (2.0 * PI / m) + (double)A[i0]);
+    const float ai0 = ws[CW_ANG + i0];
+#pragma unroll
+    for (int j = 1; j < ODX_MAXC; j++) {
+        if (j >= m) break;
+        a = (float)(double(j) * (2.0 * PI / m) + (double)ai0);
         if ((double)a > PI) a -= (float)(2.0 * PI);
         float maxdiff = 1e9f;
         int best = i0;
         for (int i = 0; i < n; i++) {
             if (avail & (1u << i)) {
-                float diff = fabsf(A[i] - a);
+                float diff = fabsf(ws[CW_ANG + i] - a);
                 if ((double)diff > PI) diff = (float)(2.0 * PI - (double)diff);
                 if (diff < maxdiff) { maxdiff = diff; best = i; }
             }
@@ -269,8 +299,12 @@ ODX_DEV void line_closest_approach(const float *pa, const float *ua, const float
 
 // dBoxBox box.cpp:356-737.  Writes pos/depth of the contacts, the box-1-to-box-2
 // separating normal and returns the count (0 = separated).
+// The geoms' matrices, positions and half sides are read with static indices
+// and selects only (no pointer chosen between two geoms), so that they stay in
+// registers; the arithmetic is the reference's, term for term.
+template <class W>
 ODX_DEV int box_box(const float *p1, const float *R1, const float *side1, const float *p2, const float *R2,
-                    const float *side2, float *normal, int maxc, DContact *con) {
+                    const float *side2, float *normal, int maxc, DContact *con, W &ws) {
     const float fudge = 1.05f;
     float p[3], pp[3], nC[3] = {0.f, 0.f, 0.f};
     int nRcol = -1, nRbox = 0;  // normal = column nRcol of R1 (nRbox 0) or R2 (1); -1 => nC
@@ -323,8 +357,11 @@ ODX_DEV int box_box(const float *p1, const float *R1, const float *side1, const 
 #undef ODX_TSTE
     if (!code) return 0;
     if (nRcol >= 0) {
-        const float *Rn = nRbox ? R2 : R1;
-        normal[0] = Rn[nRcol]; normal[1] = Rn[4 + nRcol]; normal[2] = Rn[8 + nRcol];
+        float n1[3], n2[3];
+        col3(n1, R1, nRcol);
+        col3(n2, R2, nRcol);
+#pragma unroll
+        for (int i = 0; i < 3; i++) normal[i] = nRbox ? n2[i] : n1[i];
     } else {
         mul0_331(normal, R1, nC);
     }
@@ -348,8 +385,8 @@ ODX_DEV int box_box(const float *p1, const float *R1, const float *side1, const 
             for (int i = 0; i < 3; i++) pb[i] += sg * B[j] * R2[i * 4 + j];
         }
         int ca = (code - 7) / 3, cb = (code - 7) % 3;
-#pragma unroll
-        for (int i = 0; i < 3; i++) { ua[i] = R1[ca + i * 4]; ub[i] = R2[cb + i * 4]; }
+        col3(ua, R1, ca);
+        col3(ub, R2, cb);
         line_closest_approach(pa, ua, pb, ub, alpha, beta);
 #pragma unroll
         for (int i = 0; i < 3; i++) {
@@ -362,8 +399,14 @@ ODX_DEV int box_box(const float *p1, const float *R1, const float *side1, const 
     }
 
     const bool one = code <= 3;
-    const float *Ra = one ? R1 : R2, *Rb = one ? R2 : R1, *pa = one ? p1 : p2, *pb = one ? p2 : p1;
-    const float *Sa = one ? A : B, *Sb = one ? B : A;
+    float Ra[12], Rb[12], pa[3], pb[3], Sa[3], Sb[3];
+#pragma unroll
+    for (int i = 0; i < 12; i++) { Ra[i] = one ? R1[i] : R2[i]; Rb[i] = one ? R2[i] : R1[i]; }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        pa[i] = one ? p1[i] : p2[i]; pb[i] = one ? p2[i] : p1[i];
+        Sa[i] = one ? A[i] : B[i]; Sb[i] = one ? B[i] : A[i];
+    }
     float normal2[3], nr[3], anr[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) normal2[i] = one ? normal[i] : -normal[i];
@@ -378,84 +421,99 @@ ODX_DEV int box_box(const float *p1, const float *R1, const float *side1, const 
         if (anr[0] > anr[2]) { lanr = 0; a1 = 1; a2 = 2; }
         else { a1 = 0; a2 = 1; lanr = 2; }
     }
-    float center[3];
-    const float nlanr = lanr == 0 ? nr[0] : (lanr == 1 ? nr[1] : nr[2]);
-    const float sbl = lanr == 0 ? Sb[0] : (lanr == 1 ? Sb[1] : Sb[2]);
+    float center[3], rbl[3];
+    const float nlanr = sel3(nr, lanr);
+    const float sbl = sel3(Sb, lanr);
+    col3(rbl, Rb, lanr);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        float rb = Rb[i * 4 + lanr];
+        float rb = rbl[i];
         center[i] = (nlanr < 0.f) ? (pb[i] - pa[i] + sbl * rb) : (pb[i] - pa[i] - sbl * rb);
     }
     int codeN = one ? code - 1 : code - 4;
     int code1 = codeN == 0 ? 1 : 0;
     int code2 = codeN == 2 ? 1 : 2;
-    float c1 = dot14(center, Ra + code1), c2 = dot14(center, Ra + code2);
-    float m11 = dot44(Ra + code1, Rb + a1), m12 = dot44(Ra + code1, Rb + a2);
-    float m21 = dot44(Ra + code2, Rb + a1), m22 = dot44(Ra + code2, Rb + a2);
-    float quad[8];
+    // columns code1, code2 of Ra and a1, a2 of Rb
+    float ra1[3], ra2[3], rb1[3], rb2[3];
+    col3(ra1, Ra, code1); col3(ra2, Ra, code2);
+    col3(rb1, Rb, a1); col3(rb2, Rb, a2);
+    float c1 = dot3(center, ra1), c2 = dot3(center, ra2);
+    float m11 = dot3(ra1, rb1), m12 = dot3(ra1, rb2);
+    float m21 = dot3(ra2, rb1), m22 = dot3(ra2, rb2);
     {
-        float sba1 = a1 == 0 ? Sb[0] : (a1 == 1 ? Sb[1] : Sb[2]);
-        float sba2 = a2 == 0 ? Sb[0] : (a2 == 1 ? Sb[1] : Sb[2]);
+        float sba1 = sel3(Sb, a1);
+        float sba2 = sel3(Sb, a2);
         float k1 = m11 * sba1, k2 = m21 * sba1, k3 = m12 * sba2, k4 = m22 * sba2;
-        quad[0] = c1 - k1 - k3; quad[1] = c2 - k2 - k4;
-        quad[2] = c1 - k1 + k3; quad[3] = c2 - k2 + k4;
-        quad[4] = c1 + k1 + k3; quad[5] = c2 + k2 + k4;
-        quad[6] = c1 + k1 - k3; quad[7] = c2 + k2 - k4;
+        ws[CW_A + 0] = c1 - k1 - k3; ws[CW_A + 1] = c2 - k2 - k4;
+        ws[CW_A + 2] = c1 - k1 + k3; ws[CW_A + 3] = c2 - k2 + k4;
+        ws[CW_A + 4] = c1 + k1 + k3; ws[CW_A + 5] = c2 + k2 + k4;
+        ws[CW_A + 6] = c1 + k1 - k3; ws[CW_A + 7] = c2 + k2 - k4;
     }
-    float rect[2] = {code1 == 0 ? Sa[0] : Sa[1], code2 == 1 ? Sa[1] : Sa[2]};
-    float ret[16];
-    int n = intersect_rect_quad(rect, quad, ret);
+    int rt;  // buffer holding the clipped 2D points
+    const int n = intersect_rect_quad(code1 == 0 ? Sa[0] : Sa[1], code2 == 1 ? Sa[1] : Sa[2], ws, &rt);
     if (n < 1) return 0;
-    float point[24], dep[8];
     float det1 = 1.f / (m11 * m22 - m12 * m21);
     m11 *= det1; m12 *= det1; m21 *= det1; m22 *= det1;
-    const float saN = codeN == 0 ? Sa[0] : (codeN == 1 ? Sa[1] : Sa[2]);
+    const float saN = sel3(Sa, codeN);
     int cnum = 0;
     for (int j = 0; j < n; j++) {
-        float k1 = m22 * (ret[j * 2] - c1) - m12 * (ret[j * 2 + 1] - c2);
-        float k2 = -m21 * (ret[j * 2] - c1) + m11 * (ret[j * 2 + 1] - c2);
+        const float rx = ws[rt + j * 2], ry = ws[rt + j * 2 + 1];
+        float k1 = m22 * (rx - c1) - m12 * (ry - c2);
+        float k2 = -m21 * (rx - c1) + m11 * (ry - c2);
         float pt[3];
 #pragma unroll
-        for (int i = 0; i < 3; i++) pt[i] = center[i] + k1 * Rb[i * 4 + a1] + k2 * Rb[i * 4 + a2];
+        for (int i = 0; i < 3; i++) pt[i] = center[i] + k1 * rb1[i] + k2 * rb2[i];
         float dd = saN - dot3(normal2, pt);
         if (dd >= 0.f) {
-            point[cnum * 3 + 0] = pt[0]; point[cnum * 3 + 1] = pt[1]; point[cnum * 3 + 2] = pt[2];
-            dep[cnum] = dd;
-            ret[cnum * 2] = ret[j * 2];
-            ret[cnum * 2 + 1] = ret[j * 2 + 1];
+            ws[CW_PT + cnum * 3 + 0] = pt[0]; ws[CW_PT + cnum * 3 + 1] = pt[1]; ws[CW_PT + cnum * 3 + 2] = pt[2];
+            ws[CW_DEP + cnum] = dd;
+            ws[rt + cnum * 2] = rx;
+            ws[rt + cnum * 2 + 1] = ry;
             cnum++;
         }
     }
     if (cnum < 1) return 0;
     if (maxc > cnum) maxc = cnum;
     if (maxc < 1) maxc = 1;
-    if (cnum <= maxc) {
-        for (int j = 0; j < cnum; j++) {
-#pragma unroll
-            for (int i = 0; i < 3; i++) con[j].pos[i] = point[j * 3 + i] + pa[i];
-            con[j].depth = dep[j];
+    // all points when they fit, else maxc of them culled by angle from the
+    // deepest one
+    int iret[ODX_MAXC] = {0, 1, 2, 3};
+    const bool cull = cnum > maxc;
+    if (cull) {
+        int i1 = 0;
+        float maxdepth = ws[CW_DEP];
+        for (int i = 1; i < cnum; i++) {
+            const float d = ws[CW_DEP + i];
+            if (d > maxdepth) { maxdepth = d; i1 = i; }
         }
-        return cnum;
+        cull_points(cnum, ws, rt, maxc, i1, iret);
     }
-    int i1 = 0;
-    float maxdepth = dep[0];
-    for (int i = 1; i < cnum; i++) if (dep[i] > maxdepth) { maxdepth = dep[i]; i1 = i; }
-    int iret[8];
-    cull_points(cnum, ret, maxc, i1, iret);
-    for (int j = 0; j < maxc; j++) {
-        int k = iret[j];
+    const int nout = cull ? maxc : cnum;
 #pragma unroll
-        for (int i = 0; i < 3; i++) con[j].pos[i] = point[k * 3 + i] + pa[i];
-        con[j].depth = dep[k];
+    for (int j = 0; j < ODX_MAXC; j++) {
+        if (j >= nout) break;
+        const int k = iret[j];
+#pragma unroll
+        for (int i = 0; i < 3; i++) con[j].pos[i] = ws[CW_PT + k * 3 + i] + pa[i];
+        con[j].depth = ws[CW_DEP + k];
     }
-    return maxc;
+    return nout;
+}
+
+// Sets the normal of contacts 0..n-1 (unrolled over the ODX_MAXC slots, so
+// that the contacts stay in registers).
+ODX_DEV void set_normals(DContact *c, int n, float n0, float n1, float n2) {
+#pragma unroll
+    for (int i = 0; i < ODX_MAXC; i++)
+        if (i < n) { c[i].normal[0] = n0; c[i].normal[1] = n1; c[i].normal[2] = n2; }
 }
 
 // dCollideBoxBox box.cpp:741-767
-ODX_DEV int collide_box_box(const DGeom &b1, const DGeom &b2, int maxc, DContact *c) {
+template <class W>
+ODX_DEV int collide_box_box(const DGeom &b1, const DGeom &b2, int maxc, DContact *c, W &ws) {
     float normal[3];
-    int n = box_box(b1.pos, b1.R, b1.p, b2.pos, b2.R, b2.p, normal, maxc, c);
-    for (int i = 0; i < n; i++) { c[i].normal[0] = -normal[0]; c[i].normal[1] = -normal[1]; c[i].normal[2] = -normal[2]; }
+    int n = box_box(b1.pos, b1.R, b1.p, b2.pos, b2.R, b2.p, normal, maxc, c, ws);
+    set_normals(c, n, -normal[0], -normal[1], -normal[2]);
     return n;
 }
 
@@ -492,18 +550,20 @@ ODX_DEV int collide_box_plane(const DGeom &bx, const DGeom &pl, int maxc, DConta
         else { o0 = 1; o1 = (B1 < B3) ? 0 : 2; }
     }
     if (maxc > 1) {
+#pragma unroll
         for (int t = 0; t < 2; t++) {
             int sd = t == 0 ? o0 : o1;
             float bsd = sd == 0 ? B1 : (sd == 1 ? B2 : B3);
             float asd = sd == 0 ? Aq[0] : (sd == 1 ? Aq[1] : Aq[2]);
             float ssd = sd == 0 ? side[0] : (sd == 1 ? side[1] : side[2]);
             if (depth - bsd < 0.f) break;
+            // ret == 1 + t here: the loop stops at its first miss
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                float v = ssd * R[k * 4 + sd];
-                c[ret].pos[k] = asd > 0.f ? p[k] + v : p[k] - v;
+                float v = ssd * sel3(R + k * 4, sd);
+                c[1 + t].pos[k] = asd > 0.f ? p[k] + v : p[k] - v;
             }
-            c[ret].depth = depth - bsd;
+            c[1 + t].depth = depth - bsd;
             ret++;
             if (maxc == 2) break;
         }
@@ -517,7 +577,7 @@ ODX_DEV int collide_box_plane(const DGeom &bx, const DGeom &pl, int maxc, DConta
             ret++;
         }
     }
-    for (int i = 0; i < ret; i++) { c[i].normal[0] = n[0]; c[i].normal[1] = n[1]; c[i].normal[2] = n[2]; }
+    set_normals(c, ret, n[0], n[1], n[2]);
     return ret;
 }
 
@@ -632,7 +692,8 @@ ODX_DEV void closest_line_box_points(const float *p1, const float *p2, const flo
 }
 
 // dCollideCapsuleBox capsule.cpp:164-235
-ODX_DEV int collide_capsule_box(const DGeom &cp, const DGeom &bx, int maxc, DContact *c) {
+template <class W>
+ODX_DEV int collide_capsule_box(const DGeom &cp, const DGeom &bx, int maxc, DContact *c, W &ws) {
     const float *R = cp.R;
     float clen = cp.p[1] * 0.5f, radius = cp.p[0];
     float p1[3] = {cp.pos[0] + clen * R[2], cp.pos[1] + clen * R[6], cp.pos[2] + clen * R[10]};
@@ -644,8 +705,8 @@ ODX_DEV int collide_capsule_box(const DGeom &cp, const DGeom &bx, int maxc, DCon
         float normal[3];
         float r2 = radius * 2.f;
         float capside[3] = {r2, r2, cp.p[1] + r2};
-        int n = box_box(bx.pos, bx.R, bx.p, cp.pos, cp.R, capside, normal, maxc, c);
-        for (int i = 0; i < n; i++) { c[i].normal[0] = normal[0]; c[i].normal[1] = normal[1]; c[i].normal[2] = normal[2]; }
+        int n = box_box(bx.pos, bx.R, bx.p, cp.pos, cp.R, capside, normal, maxc, c, ws);
+        set_normals(c, n, normal[0], normal[1], normal[2]);
         return n;
     }
     return collide_spheres(pl, radius, pb, 0.f, c[0]);
@@ -718,6 +779,21 @@ ODX_DEV int collide_capsule_plane(const DGeom &cp, const DGeom &pl, int maxc, DC
 // depth > 0; otherwise each disc gives its rim point deepest along -n, kept
 // while depth >= 0.  The expressions keep the reference's operation order
 // (the second tilted depth is summed term by term, as there).
+// Writes contact slot n (0 <= n < ODX_MAXC): every slot is rewritten with a
+// select, so that the slots keep static indices and stay in registers.
+ODX_DEV void put_contact(DContact *c, int n, const float *q, const float *nrm, float depth) {
+#pragma unroll
+    for (int s = 0; s < ODX_MAXC; s++) {
+        const bool h = s == n;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            c[s].pos[i] = h ? q[i] : c[s].pos[i];
+            c[s].normal[i] = h ? nrm[i] : c[s].normal[i];
+        }
+        c[s].depth = h ? depth : c[s].depth;
+    }
+}
+
 ODX_DEV int collide_cylinder_plane(const DGeom &cy, const DGeom &pl, int maxc, DContact *c) {
     const float tol = 0.0001f;
     const float radius = cy.p[0], s0 = cy.p[1] * 0.5f;
@@ -744,16 +820,16 @@ ODX_DEV int collide_cylinder_plane(const DGeom &cy, const DGeom &pl, int maxc, D
         const float t = radius / sqrtf(V2[0] * V2[0] + V2[1] * V2[1] + V2[2] * V2[2]);
         V2[0] *= t; V2[1] *= t; V2[2] *= t;
         cross3(V1, V2, dir);
-        for (int k = 0; k < 4 && n < maxc; k++) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (n >= maxc) break;
             const float *off = k < 2 ? V1 : V2;
             float q[3];
 #pragma unroll
             for (int i = 0; i < 3; i++) q[i] = (k & 1) ? P[i] - off[i] : P[i] + off[i];
             const float depth = pv[3] - dot3(pv, q);
             if (depth > 0.f) {
-#pragma unroll
-                for (int i = 0; i < 3; i++) { c[n].pos[i] = q[i]; c[n].normal[i] = pv[i]; }
-                c[n].depth = depth;
+                put_contact(c, n, q, pv, depth);
                 n++;
             }
         }
@@ -767,30 +843,29 @@ ODX_DEV int collide_cylinder_plane(const DGeom &cy, const DGeom &pl, int maxc, D
     float depth = pv[3] - dot3(pv, q);
     if (depth >= 0.f) {
 #pragma unroll
-        for (int i = 0; i < 3; i++) { c[n].pos[i] = q[i]; c[n].normal[i] = pv[i]; }
-        c[n].depth = depth;
+        for (int i = 0; i < 3; i++) { c[0].pos[i] = q[i]; c[0].normal[i] = pv[i]; }
+        c[0].depth = depth;
         if (++n >= maxc) return n;
     }
     q[0] = C[0] + e2[0]; q[1] = C[1] + e2[1]; q[2] = C[2] + e2[2];
     depth = pv[3] - pv[0] * q[0] - pv[1] * q[1] - pv[2] * q[2];
     if (depth >= 0.f) {
-#pragma unroll
-        for (int i = 0; i < 3; i++) { c[n].pos[i] = q[i]; c[n].normal[i] = pv[i]; }
-        c[n].depth = depth;
+        put_contact(c, n, q, pv, depth);
         n++;
     }
     return n;
 }
 
-ODX_DEV int collide_fwd(const DGeom &a, const DGeom &b, int maxc, DContact *c) {
+template <class W>
+ODX_DEV int collide_fwd(const DGeom &a, const DGeom &b, int maxc, DContact *c, W &ws) {
     switch (a.cls * 8 + b.cls) {
     case CLS_SPHERE * 8 + CLS_SPHERE: return collide_spheres(a.pos, a.p[0], b.pos, b.p[0], c[0]);
     case CLS_SPHERE * 8 + CLS_BOX: return collide_sphere_box(a, b, c[0]);
     case CLS_SPHERE * 8 + CLS_PLANE: return collide_sphere_plane(a, b, c[0]);
-    case CLS_BOX * 8 + CLS_BOX: return collide_box_box(a, b, maxc, c);
+    case CLS_BOX * 8 + CLS_BOX: return collide_box_box(a, b, maxc, c, ws);
     case CLS_BOX * 8 + CLS_PLANE: return collide_box_plane(a, b, maxc, c);
     case CLS_CAPSULE * 8 + CLS_SPHERE: return collide_capsule_sphere(a, b, c[0]);
-    case CLS_CAPSULE * 8 + CLS_BOX: return collide_capsule_box(a, b, maxc, c);
+    case CLS_CAPSULE * 8 + CLS_BOX: return collide_capsule_box(a, b, maxc, c, ws);
     case CLS_CAPSULE * 8 + CLS_CAPSULE: return collide_capsule_capsule(a, b, maxc, c);
     case CLS_CAPSULE * 8 + CLS_PLANE: return collide_capsule_plane(a, b, maxc, c);
     case CLS_CYLINDER * 8 + CLS_PLANE: return collide_cylinder_plane(a, b, maxc, c);
@@ -799,14 +874,18 @@ ODX_DEV int collide_fwd(const DGeom &a, const DGeom &b, int maxc, DContact *c) {
 }
 
 // dCollide: forward entry, else the reversed entry with negated normals.
-// maxc must be >= 1 (flags & 0xffff of the reference).
-ODX_DEV int dcollide(const DGeom &g1, const DGeom &g2, int maxc, DContact *c) {
+// maxc must be in 1..ODX_MAXC (flags & 0xffff of the reference, capped).
+// ws: the box-box clipper's working arrays (ClipPriv or ClipLds).
+template <class W>
+ODX_DEV int dcollide(const DGeom &g1, const DGeom &g2, int maxc, DContact *c, W &ws) {
     if (g1.body == g2.body && g1.body >= 0) return 0;
-    int n = collide_fwd(g1, g2, maxc, c);
+    int n = collide_fwd(g1, g2, maxc, c, ws);
     if (n >= 0) return n;
-    n = collide_fwd(g2, g1, maxc, c);
+    n = collide_fwd(g2, g1, maxc, c, ws);
     if (n < 0) return 0;
-    for (int i = 0; i < n; i++) { c[i].normal[0] = -c[i].normal[0]; c[i].normal[1] = -c[i].normal[1]; c[i].normal[2] = -c[i].normal[2]; }
+#pragma unroll
+    for (int i = 0; i < ODX_MAXC; i++)
+        if (i < n) { c[i].normal[0] = -c[i].normal[0]; c[i].normal[1] = -c[i].normal[1]; c[i].normal[2] = -c[i].normal[2]; }
     return n;
 }
 

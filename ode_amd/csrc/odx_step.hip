@@ -74,7 +74,7 @@ __device__ __forceinline__ float wave_max(float v) {
 // fout / tout: the batched and single-world paths pass the accumulators
 // themselves, the large world arrays of its own, so that a step that fails
 // after this stage leaves every body record as it was.
-__device__ void body_prep(const DevWorlds &D, const StepParams &P, int b, float4 *fout, float4 *tout) {
+ODX_DEV void body_prep(const DevWorlds &D, const StepParams &P, int b, float4 *fout, float4 *tout) {
     float4 p4 = D.pos[b], q4 = D.quat[b], lv = D.lvel[b], av = D.avel[b], fa = D.facc[b], ta = D.tacc[b];
     float4 ms = D.mass[b];
     const float mass = ms.x, im = ms.y;
@@ -136,7 +136,7 @@ __device__ void body_prep(const DevWorlds &D, const StepParams &P, int b, float4
 }
 
 // ------------------------------------------------------------- geoms ------
-__device__ void load_geom(const DevWorlds &D, int g, int b0, DGeom &G) {
+ODX_DEV void load_geom(const DevWorlds &D, int g, int b0, DGeom &G) {
     float4 a = D.geom0[g], c = D.geom1[g];
     G.cls = (int)a.x;
     G.body = (int)a.y;
@@ -169,7 +169,7 @@ struct RowBuf {
 
 // getInfo2 of a contact joint, contact.cpp:125-345 (rolling friction not on
 // this path).  b1: body row index >= 0, b2 may be -1.
-__device__ int contact_info2(const DevWorlds &D, const StepParams &P, const float *rec, int b0, RowBuf *rows) {
+ODX_DEV int contact_info2(const DevWorlds &D, const StepParams &P, const float *rec, int b0, RowBuf *rows) {
     const Surface s = D.surf[(int)rec[13]];
     const int m = contact_rows_m(s);
     const int b1 = (int)rec[1], b2 = (int)rec[2];
@@ -224,40 +224,33 @@ __device__ int contact_info2(const DevWorlds &D, const StepParams &P, const floa
         } else {
             planeSpace(n, t1, t2);
         }
-        int row = 1;
-        if (s.mu > 0.f) {
-            rows[row].J1[0] = t1[0]; rows[row].J1[1] = t1[1]; rows[row].J1[2] = t1[2];
-            cross3(rows[row].J1 + 3, c1, t1);
+        // friction rows: t1 in row 1 when mu > 0, then t2 in the next row
+        // (static row indices keep the rows in registers)
+        auto friction = [&](RowBuf &w, const float *t, float mu_, int mrhs, float rhs_, int mcfm, float cfm_, int mf) {
+            w.J1[0] = t[0]; w.J1[1] = t[1]; w.J1[2] = t[2];
+            cross3(w.J1 + 3, c1, t);
             if (b2 >= 0) {
-                rows[row].J2[0] = -t1[0]; rows[row].J2[1] = -t1[1]; rows[row].J2[2] = -t1[2];
-                cross3(rows[row].J2 + 3, t1, c2);
+                w.J2[0] = -t[0]; w.J2[1] = -t[1]; w.J2[2] = -t[2];
+                cross3(w.J2 + 3, t, c2);
             }
-            if (mode & 0x020) rows[row].rhs = s.motion1;
-            if (mode & 0x100) rows[row].cfm = s.slip1;
-            rows[row].lo = -s.mu; rows[row].hi = s.mu;
-            if (mode & 0x1000) rows[row].findex = 0;
-            row++;
-        }
+            if (mode & mrhs) w.rhs = rhs_;
+            if (mode & mcfm) w.cfm = cfm_;
+            w.lo = -mu_; w.hi = mu_;
+            if (mode & mf) w.findex = 0;
+        };
+        const bool f1 = s.mu > 0.f;
+        if (f1) friction(rows[1], t1, s.mu, 0x020, s.motion1, 0x100, s.slip1, 0x1000);
         const float mu2 = (mode & 0x001) ? s.mu2 : s.mu;
-        if (mu2 > 0.f && row < m) {
-            rows[row].J1[0] = t2[0]; rows[row].J1[1] = t2[1]; rows[row].J1[2] = t2[2];
-            cross3(rows[row].J1 + 3, c1, t2);
-            if (b2 >= 0) {
-                rows[row].J2[0] = -t2[0]; rows[row].J2[1] = -t2[1]; rows[row].J2[2] = -t2[2];
-                cross3(rows[row].J2 + 3, t2, c2);
-            }
-            if (mode & 0x040) rows[row].rhs = s.motion2;
-            if (mode & 0x200) rows[row].cfm = s.slip2;
-            rows[row].lo = -mu2; rows[row].hi = mu2;
-            if (mode & 0x2000) rows[row].findex = 0;
-            row++;
+        if (mu2 > 0.f) {
+            if (f1) { if (m > 2) friction(rows[2], t2, mu2, 0x040, s.motion2, 0x200, s.slip2, 0x2000); }
+            else friction(rows[1], t2, mu2, 0x040, s.motion2, 0x200, s.slip2, 0x2000);
         }
     }
     return m;
 }
 
 // getInfo2 of a ball joint, ball.cpp:58-67 -> setBall joint.cpp:111-162
-__device__ int ball_info2(const DevWorlds &D, const StepParams &P, const float *rec, int b0, RowBuf *rows) {
+ODX_DEV int ball_info2(const DevWorlds &D, const StepParams &P, const float *rec, int b0, RowBuf *rows) {
     const int b1 = (int)rec[1], b2 = (int)rec[2];
     const float cfm = rec[10], erp = rec[9];
     for (int r = 0; r < 3; r++) rows[r].cfm = cfm;
@@ -299,7 +292,7 @@ __device__ __forceinline__ void qmul(float *qa, const float *qb, const float *qc
 
 // setFixedOrientation joints/joint.cpp:228-285: angular rows making the
 // relative rotation equal qrel; rows[0..2]
-__device__ void fixed_orientation_rows(const DevWorlds &D, int b0, int b1, int b2, const float *x, float fps, float erp,
+ODX_DEV void fixed_orientation_rows(const DevWorlds &D, int b0, int b1, int b2, const float *x, float fps, float erp,
                                        RowBuf *rows) {
     rows[0].J1[3] = 1.f; rows[1].J1[4] = 1.f; rows[2].J1[5] = 1.f;
     if (b2 >= 0) { rows[0].J2[3] = -1.f; rows[1].J2[4] = -1.f; rows[2].J2[5] = -1.f; }
@@ -325,7 +318,7 @@ __device__ void fixed_orientation_rows(const DevWorlds &D, int b0, int b1, int b
 // dxJointLimitMotor::addLimot joints/joint.cpp:598-786 for the row along ax1;
 // the force of a motor powered at its limit is added to the body accumulators
 // on the host before the step
-__device__ void limot_row(const DevWorlds &D, int b0, int b1, int b2, const float *x, float fps, const float *ax1,
+ODX_DEV void limot_row(const DevWorlds &D, int b0, int b1, int b2, const float *x, float fps, const float *ax1,
                           bool rotational, RowBuf &row) {
     const int limit = (int)x[JX_LIMIT];
     bool powered = x[JX_FMAX] > 0.f;
@@ -382,7 +375,7 @@ __device__ void limot_row(const DevWorlds &D, int b0, int b1, int b2, const floa
 
 // getInfo2 of hinge (hinge.cpp:80-153), slider (slider.cpp:146-250) and fixed
 // (fixed.cpp:68-105) joints
-__device__ int hinge_slider_fixed_info2(const DevWorlds &D, const StepParams &P, const float *rec, const float *x,
+ODX_DEV int hinge_slider_fixed_info2(const DevWorlds &D, const StepParams &P, const float *rec, const float *x,
                                         int b0, RowBuf *rows) {
     const int type = (int)rec[0];
     const int b1 = (int)rec[1], b2 = (int)rec[2];
@@ -475,7 +468,7 @@ __device__ int hinge_slider_fixed_info2(const DevWorlds &D, const StepParams &P,
 // powered at its limit, added to the bodies' accumulators as addLimot does
 // (joint.cpp:640-705).  A hinge whose stops are not finite keeps the limit
 // state it has.  Accumulator adds are atomic: several joints may share a body.
-__device__ void joint_info1(const DevWorlds &D, int b0, const float *rec, float *x) {
+ODX_DEV void joint_info1(const DevWorlds &D, int b0, const float *rec, float *x) {
     const int type = (int)rec[0];
     if (type != JT_HINGE && type != JT_SLIDER) return;
     const float pi = 3.14159265358979f;
@@ -567,7 +560,7 @@ __device__ void joint_info1(const DevWorlds &D, int b0, const float *rec, float 
 }
 
 // x: the joint's extended record (hinge / slider / fixed), else unused.
-__device__ int build_joint_rows(const DevWorlds &D, const StepParams &P, const float *rec, const float *x, int b0,
+ODX_DEV int build_joint_rows(const DevWorlds &D, const StepParams &P, const float *rec, const float *x, int b0,
                                 int r0, int ri) {
     RowBuf rows[6];
 #pragma unroll
@@ -589,7 +582,9 @@ __device__ int build_joint_rows(const DevWorlds &D, const StepParams &P, const f
     const float fps = 1.f / P.h;
     const float *s1 = D.bscr + (size_t)(b0 + b1) * BSCR_STRIDE;
     const float *s2 = b2 >= 0 ? D.bscr + (size_t)(b0 + b2) * BSCR_STRIDE : nullptr;
-    for (int r = 0; r < m; r++) {
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+        if (r >= m) break;
         RowBuf &w = rows[r];
         w.rhs *= fps;
         w.cfm *= fps;
@@ -642,7 +637,7 @@ __device__ int build_joint_rows(const DevWorlds &D, const StepParams &P, const f
 // identity, quickstep.cpp:46-51).  The previous friction impulse is carried
 // over as a tangential force vector and re-expressed in the new tangent basis.
 // Returns 1 and lam[3] = (normal, t1, t2) when a match is found.
-__device__ int match_previous(const DevWorlds &D, int j0, int nprev, const float *rec, int b0, float *lam) {
+ODX_DEV int match_previous(const DevWorlds &D, int j0, int nprev, const float *rec, int b0, float *lam) {
     lam[0] = lam[1] = lam[2] = 0.f;
     if (nprev <= 0) return 0;
     const int g1 = (int)rec[14], g2 = (int)rec[15];
@@ -683,7 +678,7 @@ __device__ int match_previous(const DevWorlds &D, int j0, int nprev, const float
     return 1;
 }
 
-__device__ void contact_features(const DevWorlds &D, const float *rec, int b0, const float *lam_prev, int matched,
+ODX_DEV void contact_features(const DevWorlds &D, const float *rec, int b0, const float *lam_prev, int matched,
                                  float *out) {
     const int b1 = (int)rec[1], b2 = (int)rec[2];
     float n[3] = {rec[6], rec[7], rec[8]}, t1[3], t2[3];
@@ -718,7 +713,7 @@ __device__ void contact_features(const DevWorlds &D, const float *rec, int b0, c
 
 // Warm-start value of each row of a contact from (normal, t1, t2) impulses,
 // following the row order of contact_info2.
-__device__ void write_contact_lambda0(const DevWorlds &D, const float *rec, int r0, int ri, const float *lam) {
+ODX_DEV void write_contact_lambda0(const DevWorlds &D, const float *rec, int r0, int ri, const float *lam) {
     const Surface s = D.surf[(int)rec[13]];
     const int m = contact_rows_m(s);
     float *o = D.lambda0 + r0 + ri;
@@ -744,6 +739,8 @@ struct PrepLds {
     unsigned char *rb1, *rb2, *rfl, *jm;  // [SW_MAXROWS]
     signed char *gb;           // [SW_MAXG]
     int *L, *nblk;             // scalars
+    float *clip;               // [CLIP_WS * 64]: the narrowphase's box-box clipper, one column per lane;
+                               // aliases cnt..jm, which the row and schedule stages fill afterwards
 };
 constexpr size_t PREP_LDS_BYTES = SW_MAXB * 8 + SW_MAXG * 6 * 4 + SW_MAXPAIRS * 4 + (SW_MAXPH + 1) * 4 +
                                   2 * SW_MAXROWS * 2 + 4 * SW_MAXROWS + SW_MAXG + 2 * 4 + 16;
@@ -763,6 +760,9 @@ __device__ inline PrepLds prep_lds_carve(void *base) {
     S.rfl = reinterpret_cast<unsigned char *>(p); p += SW_MAXROWS;
     S.jm = reinterpret_cast<unsigned char *>(p); p += SW_MAXROWS;
     S.gb = reinterpret_cast<signed char *>(p);
+    S.clip = reinterpret_cast<float *>(S.cnt);
+    static_assert((SW_MAXPH + 1) * 4 + 2 * 4 + 2 * SW_MAXROWS * 2 + 4 * SW_MAXROWS >= CLIP_WS * 64 * 4,
+                  "the clipper columns fit in cnt..jm");
     return S;
 }
 
@@ -871,6 +871,7 @@ __device__ __forceinline__ int prepare_world(const DevWorlds &D, const StepParam
             const int k = base + t;
             int n = 0, gi = 0, gj = 0, si = 0;
             DContact c[ODX_MAXC];
+            ClipLds ws{S.clip + t};
             if (k < npairs) {
                 gi = (int)(s_pairs[k] >> 16);
                 gj = (int)(s_pairs[k] & 0xffff);
@@ -882,11 +883,13 @@ __device__ __forceinline__ int prepare_world(const DevWorlds &D, const StepParam
                     si = (int)D.geom1[g0 + gi].z * P.nmat + (int)D.geom1[g0 + gj].z;
                     if (D.surf[si].max_contacts > 0) maxc = D.surf[si].max_contacts;
                 }
-                n = dcollide(A, B, min(maxc, ODX_MAXC), c);
+                n = dcollide(A, B, min(maxc, ODX_MAXC), c, ws);
             }
             const int incl = wave_incl_scan(n);
             const int off = incl - n;
-            for (int q = 0; q < n; q++) {
+#pragma unroll
+            for (int q = 0; q < ODX_MAXC; q++) {
+                if (q >= n) break;
                 const int slot = ncont + off + q;
                 if (slot < ccap) {
                     int bb1 = s_gb[gi], bb2 = s_gb[gj];
@@ -2374,16 +2377,22 @@ __global__ void k_aabbs(const float *gviews, int n, float *aabb) {
 
 // dCollide for a list of geom-view pairs: up to maxc contacts per pair
 // (pos[3] normal[3] depth) into out[pair][maxc][7], counts[pair].
-__global__ void k_collide_pairs(const float *gviews, const int *pairs, int np, int maxc, float *out, int *counts) {
+__global__ __launch_bounds__(128) void k_collide_pairs(const float *gviews, const int *pairs, int np, int maxc,
+                                                        float *out, int *counts) {
+    // the box-box clipper's arrays, one 64-column block per wave
+    __shared__ float s_clip[2 * 64 * CLIP_WS];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     DGeom A, B;
     load_gview(gviews + (size_t)pairs[2 * p] * GVIEW_STRIDE, A);
     load_gview(gviews + (size_t)pairs[2 * p + 1] * GVIEW_STRIDE, B);
     DContact c[ODX_MAXC];
-    const int n = dcollide(A, B, min(maxc, ODX_MAXC), c);
+    ClipLds ws{s_clip + (threadIdx.x >> 6) * 64 * CLIP_WS + (threadIdx.x & 63)};
+    const int n = dcollide(A, B, min(maxc, ODX_MAXC), c, ws);
     counts[p] = n;
-    for (int q = 0; q < n; q++) {
+#pragma unroll
+    for (int q = 0; q < ODX_MAXC; q++) {
+        if (q >= n) break;
         float *o = out + ((size_t)p * ODX_MAXC + q) * 7;
         o[0] = c[q].pos[0]; o[1] = c[q].pos[1]; o[2] = c[q].pos[2];
         o[3] = c[q].normal[0]; o[4] = c[q].normal[1]; o[5] = c[q].normal[2];
@@ -2727,13 +2736,16 @@ __global__ void k_lg_contacts(DevWorlds D, StepParams P, LargeDev L, int write) 
             si = (int)D.geom1[pr.x].z * P.nmat + (int)D.geom1[pr.y].z;
             if (D.surf[si].max_contacts > 0) maxc = D.surf[si].max_contacts;
         }
-        const int n = (A.body < 0 && B.body < 0) ? 0 : dcollide(A, B, min(maxc, ODX_MAXC), c);
+        ClipPriv ws;
+        const int n = (A.body < 0 && B.body < 0) ? 0 : dcollide(A, B, min(maxc, ODX_MAXC), c, ws);
         if (!write) { L.contact_cnt[p] = n; continue; }
         const int off = L.contact_cnt[p];
         int b1 = A.body, b2 = B.body;
         float sg = 1.f;
         if (b1 < 0) { b1 = b2; b2 = -1; sg = -1.f; }
-        for (int q = 0; q < n && off + q < L.contact_cap; q++) {
+#pragma unroll
+        for (int q = 0; q < ODX_MAXC; q++) {
+            if (q >= n || off + q >= L.contact_cap) break;
             float4 *o4 = reinterpret_cast<float4 *>(D.joints + (size_t)(off + q) * JOINT_STRIDE);
             o4[0] = make_float4(0.f, (float)b1, (float)b2, c[q].pos[0]);
             o4[1] = make_float4(c[q].pos[1], c[q].pos[2], sg * c[q].normal[0], sg * c[q].normal[1]);
