@@ -23,7 +23,7 @@ void launch_prepare_small(const DevWorlds &D, const StepParams &P, hipStream_t s
 void launch_solve_small(const DevWorlds &D, const StepParams &P, hipStream_t s);
 void launch_step_fused(const DevWorlds &D, const StepParams &P, const MlpDev *M, int nsteps, hipStream_t s);
 void launch_mlp_infer(const odxMlp *m, const float *feat, const int *feat_row, const int *count, float *lambda0,
-                      int max_entries, hipStream_t s);
+                      int max_entries, hipStream_t s, int max_blocks);
 const MlpDev *mlp_device_view(const odxMlp *m);
 int mlp_gate(const odxMlp *m);
 }  // namespace odx
@@ -311,7 +311,7 @@ struct odxBatch {
     bool enqueue_infer(const StepParams &q, hipStream_t s) {
         if (!infer_on()) return true;
         launch_mlp_infer(mlp, D.feat + (size_t)q.feat_base * FEAT_DIM, D.feat_row + q.feat_base, D.feat_count + 2 * q.split,
-                         D.lambda0, joint_start[q.wbase + q.wcount] - q.feat_base, s);
+                         D.lambda0, joint_start[q.wbase + q.wcount] - q.feat_base, s, P.gate ? 64 : 4096);
         return odx_check(hipGetLastError(), "k_mlp_infer launch");
     }
     bool enqueue_step(const StepParams &q, hipStream_t s) {

@@ -27,32 +27,23 @@
 namespace odx {
 
 // ------------------------------------------------------------ inference --
-__global__ __launch_bounds__(256) void k_mlp_infer(MlpDev M, const float *feat, const int *feat_row,
-                                                   const int *count, float *lambda0) {
-    __shared__ __align__(16) __bf16 sW1[NH * KIN];
-    __shared__ __align__(16) __bf16 sW2[NH * NH];
-    __shared__ __align__(16) __bf16 sW3[NOUTP * NH];
-    __shared__ float sb[NH + NH + 4 + 38];
-    __shared__ InferBuf tb[4];
-    const int tid = threadIdx.x;
+// One wave per workgroup, one 16-contact tile at a time; the weights are read
+// from global memory (L2) rather than staged in LDS, so that a workgroup needs
+// only its 8 KB activation buffer and finds room on CUs whose LDS is held by
+// the concurrent solve kernels.
+__global__ __launch_bounds__(64) void k_mlp_infer(MlpDev M, const float *feat, const int *feat_row,
+                                                  const int *count, float *lambda0) {
+    __shared__ InferBuf tb;
     const int n = *count;
     const int ntiles = (n + 15) / 16;
-    // the grid is sized for every contact; workgroups with no tile (the count is
-    // known only on the device) leave before staging the weights
-    if (n <= 0 || (int)blockIdx.x * 4 >= ntiles) return;
-    for (int i = tid; i < NH * KIN / 8; i += 256) reinterpret_cast<bf16x8 *>(sW1)[i] = reinterpret_cast<const bf16x8 *>(M.W1h)[i];
-    for (int i = tid; i < NH * NH / 8; i += 256) reinterpret_cast<bf16x8 *>(sW2)[i] = reinterpret_cast<const bf16x8 *>(M.W2h)[i];
-    for (int i = tid; i < NOUTP * NH / 8; i += 256) reinterpret_cast<bf16x8 *>(sW3)[i] = reinterpret_cast<const bf16x8 *>(M.W3h)[i];
-    for (int i = tid; i < NH; i += 256) { sb[i] = M.b1[i]; sb[NH + i] = M.b2[i]; }
-    if (tid < NOUT) sb[2 * NH + tid] = M.b3[tid];
-    if (tid < 38) sb[2 * NH + 4 + tid] = M.norm[tid];
-    __syncthreads();
-    const int wv = tid >> 6, l = tid & 63;
-    const float *nrm = sb + 2 * NH + 4;
-    for (int tile = blockIdx.x * 4 + wv; tile < ntiles; tile += gridDim.x * 4) {
+    // the count is known only on the device: workgroups beyond it leave
+    if (n <= 0 || (int)blockIdx.x >= ntiles) return;
+    const int l = threadIdx.x & 63;
+    const float *nrm = M.norm;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int s0 = tile * 16;
         const int nv = min(16, n - s0);
-        f32x4 y = forward_tile<false>(sW1, sW2, sW3, sb, sb + NH, sb + 2 * NH, nrm, tb[wv], feat + (size_t)s0 * NIN, nv);
+        f32x4 y = forward_tile<false>(M.W1h, M.W2h, M.W3h, M.b1, M.b2, M.b3, nrm, tb, feat + (size_t)s0 * NIN, nv);
         const int o = l & 15, g = l >> 4;
         if (o < NOUT) {
 #pragma unroll
@@ -69,6 +60,9 @@ __global__ __launch_bounds__(256) void k_mlp_infer(MlpDev M, const float *feat, 
                 }
             }
         }
+        // the next tile reuses tb
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
 }
 
@@ -336,13 +330,14 @@ namespace odx {
 // device view of a network, for the fused step kernel
 const MlpDev *mlp_device_view(const odxMlp *m) { return m ? &m->M : nullptr; }
 // used by the batch runtime to run inference inside the step graph
+// max_entries bounds the device-side count; max_blocks caps the grid (a
+// gated network infers only the few unmatched contacts of a step)
 void launch_mlp_infer(const odxMlp *m, const float *feat, const int *feat_row, const int *count, float *lambda0,
-                      int max_entries, hipStream_t s) {
-    int tiles = (max_entries + 15) / 16;
-    int blocks = (tiles + 3) / 4;
-    if (blocks > 1024) blocks = 1024;
+                      int max_entries, hipStream_t s, int max_blocks) {
+    int blocks = (max_entries + 15) / 16;
+    if (blocks > max_blocks) blocks = max_blocks;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, s, m->M, feat, feat_row, count, lambda0);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(64), 0, s, m->M, feat, feat_row, count, lambda0);
 }
 int mlp_gate(const odxMlp *m) { return m ? m->M.gate : 0; }
 }  // namespace odx
@@ -536,7 +531,7 @@ int odxMlpPredict(odxMlp *m, const float *X, int N, float *Y) {
          odx_check(hipMemcpy(drow, rows.data(), (size_t)N * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
          odx_check(hipMemcpy(dcnt, &N, 4, hipMemcpyHostToDevice), "hipMemcpy");
     if (ok) {
-        launch_mlp_infer(m, dX, drow, dcnt, dY, N, m->stream);
+        launch_mlp_infer(m, dX, drow, dcnt, dY, N, m->stream, 4096);
         ok = odx_check(hipGetLastError(), "k_mlp_infer") && odx_check(hipStreamSynchronize(m->stream), "sync") &&
              odx_check(hipMemcpy(Y, dY, (size_t)N * NOUT * 4, hipMemcpyDeviceToHost), "hipMemcpy");
     }
@@ -616,7 +611,7 @@ int odxMlpPredictNoAlloc(odxMlp *m, const float *X, int N, float *Y) {
         ok = odx_check(hipGetLastError(), "k_mlp_rows");
     }
     if (ok) {
-        launch_mlp_infer(m, dX, drow, dcnt, dY, N, m->stream);
+        launch_mlp_infer(m, dX, drow, dcnt, dY, N, m->stream, 4096);
         ok = odx_check(hipGetLastError(), "k_mlp_infer") &&
              odx_check(hipMemcpyAsync(Y, dY, (size_t)N * NOUT * sizeof(float), hipMemcpyDeviceToHost, m->stream),
                        "hipMemcpyAsync") &&
