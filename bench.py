@@ -12,7 +12,8 @@ Workloads (seeded synthetic scenes, ode_amd/scenes.py):
           previous-step contact keep its impulses, the network predicts the
           new ones) vs cold PGS at the same residual.
           value = learned warm-start world-steps/s.
-  uneven  4,096 worlds of 8-64 spheres/boxes dropped on a plane, cold PGS.
+  uneven  4,096 worlds of 8-64 spheres/boxes dropped on a plane, 20 sweeps
+          per step (--tol 0 --max-iters 20 unless given), 500 steps, cold PGS.
   chains  1,024 worlds of a 16-capsule chain on ball joints (anchored at one
           end) with 8 free boxes, 40 sweeps per step (--tol 0 --max-iters 40
           unless given), cold PGS.
@@ -262,8 +263,9 @@ def measure(B, args, warm, nworlds):
 def bench_large(args):
     """One world of 200,000 spheres (radius 0.05-0.15 m) poured from a lattice
     into a 10x10 m open box of 5 static planes; hash-space broadphase, mu=0.3,
-    30 sweeps per step.  The pour runs `warmup` steps before the timed
-    `steps`."""
+    30 sweeps per step.  The pour runs `pour` + `warmup` steps (the lattice's
+    top layer lands after about 340 steps; the pile has settled by about 800)
+    before the timed `steps`."""
     from ode_amd import odx
     from ode_amd import scenes as S
     import oracle as O
@@ -274,12 +276,13 @@ def bench_large(args):
     L.set_solve_blocks(args.solve_blocks)
     t0 = time.perf_counter()
     done = 0
-    while done < args.warmup:
-        n = min(50, args.warmup - done)
+    total = args.pour + args.warmup
+    while done < total:
+        n = min(50, total - done)
         L.step(n)
         L.sync()
         done += n
-        print("bench: pour %d/%d steps, %d contacts" % (done, args.warmup, L.counts()["contacts"]),
+        print("bench: pour %d/%d steps, %d contacts" % (done, total, L.counts()["contacts"]),
               file=sys.stderr, flush=True)
     pour_s = time.perf_counter() - t0
     total_ms = L.time(args.steps)
@@ -305,7 +308,7 @@ def bench_large(args):
         "data": "synthetic",
         "config": {"workload": "large", "bodies": bodies, "geoms": bodies + 5, "space": "hash",
                    "sweeps": args.large_iters, "mu": 0.3, "erp": 0.2, "cfm": 1e-5, "h": 0.01, "sor_w": args.sor_w,
-                   "seed": args.seed, "pour_steps": args.warmup},
+                   "seed": args.seed, "pour_steps": args.pour + args.warmup},
         "contacts_per_step": cnt["contacts"],
         "contacts_per_s": cnt["contacts"] / (ms * 1e-3),
         "counts": cnt,
@@ -441,9 +444,9 @@ def main():
     ap.add_argument("--restart-after", type=int, default=600,
                     help="sweeps after which an unconverged warm start restarts from lambda = 0")
     ap.add_argument("--lds-rows", type=int, default=-1, help="LDS row budget per world (-1: sized during warmup)")
-    ap.add_argument("--sor-w", type=float, default=1.0,
-                    help="SOR relaxation of every mode (ODE's default 1.3 sends some warm-started solves into "
-                         "limit cycles; cold convergence is the same at 1.0)")
+    ap.add_argument("--sor-w", type=float, default=None,
+                    help="SOR relaxation (default: ODE's 1.3 for the fixed-sweep workloads; 1.0 for towers, where "
+                         "1.3 sends some warm-started solves into limit cycles and cold convergence is the same)")
     ap.add_argument("--splits", type=int, default=4,
                     help="world ranges stepped on parallel streams (with --no-autotune)")
     ap.add_argument("--tune-steps", type=int, default=40, help="steps per autotuner trial")
@@ -460,10 +463,14 @@ def main():
     ap.add_argument("--bodies", type=int, default=None, help="bodies of the large (200,000) or boxes (1,000) workload")
     ap.add_argument("--solve-blocks", type=int, default=1,
                     help="large workload: workgroups per CU of the cooperative solve (odxLargeSetSolveBlocks)")
+    ap.add_argument("--pour", type=int, default=800,
+                    help="large workload: steps poured (untimed) before the warmup and the timed steps")
     ap.add_argument("--large-iters", type=int, default=None,
                     help="sweeps per step of the large (30) or boxes (20) workload")
     args = ap.parse_args()
 
+    if args.sor_w is None:
+        args.sor_w = 1.0 if args.workload == "towers" else 1.3
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (worlds shard by index "
               "across GPUs with no exchange)" % args.gpus, file=sys.stderr)
@@ -478,13 +485,18 @@ def main():
         return bench_large(args)
     if args.workload == "boxes":
         return bench_boxes(args)
-    if args.workload == "chains":
-        if "--worlds" not in sys.argv:
+    if args.workload in ("chains", "uneven"):
+        # fixed sweeps per step: 40 (chains), 20 (uneven, the solver of the
+        # 1,000-box world); uneven runs its 500 steps
+        chains = args.workload == "chains"
+        if chains and "--worlds" not in sys.argv:
             args.worlds = 1024
         if "--tol" not in sys.argv:
             args.tol = 0.0
         if "--max-iters" not in sys.argv:
-            args.max_iters = 40
+            args.max_iters = 40 if chains else 20
+        if not chains and "--steps" not in sys.argv:
+            args.steps = 500
 
     from ode_amd import odx
     import oracle as O
