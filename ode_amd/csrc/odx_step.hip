@@ -2012,9 +2012,34 @@ __device__ float block_max(float v, float *s_red) {
 // stopped are skipped, and every non-zero delta adds delta * max|iMJ side| to
 // the body's positive or negative force adjustment (quickstep.cpp:3001-3031;
 // iMJ is stored unscaled, as compute_invM_JT's maxabs reads it).
-__device__ __forceinline__ void process_item_l(const float4 *rows4, const float4 *pre, int item, float *lam,
-                                               float *s_fc, float *fa = nullptr, const int *isl_b = nullptr,
-                                               const int *ist = nullptr) {
+// The single-world row update's arithmetic with every fused multiply-add
+// written out, so that each sweep variant rounds alike: J . f over one body's
+// six entries (j0.xyzw, j4, j5), the product of entry 0 first.
+__device__ __forceinline__ float row_dot6(const float *f, float4 j0, float j4, float j5) {
+    float s = f[0] * j0.x;
+    s = fmaf(f[1], j0.y, s);
+    s = fmaf(f[2], j0.z, s);
+    s = fmaf(f[3], j0.w, s);
+    s = fmaf(f[4], j4, s);
+    return fmaf(f[5], j5, s);
+}
+// f += delta * iMJ for one body's six entries
+__device__ __forceinline__ void row_axpy6(float *f, float delta, float4 m0, float m4, float m5) {
+    f[0] = fmaf(delta, m0.x, f[0]);
+    f[1] = fmaf(delta, m0.y, f[1]);
+    f[2] = fmaf(delta, m0.z, f[2]);
+    f[3] = fmaf(delta, m0.w, f[3]);
+    f[4] = fmaf(delta, m4, f[4]);
+    f[5] = fmaf(delta, m5, f[5]);
+}
+
+// after_loads() runs once the item's bodies' fc and first lambda are
+// requested and before they are used: a caller requests the next levels'
+// operands there, behind this item's (a wave's LDS reads return in order).
+template <class F>
+__device__ __forceinline__ void process_item_lf(const float4 *rows4, const float4 *pre, int item, float *lam,
+                                                float *s_fc, float *fa, const int *isl_b, const int *ist,
+                                                F &&after_loads) {
 #pragma clang fp contract(fast)
     const int rs = item >> 3, nr = item & 7;
     const float4 *row4 = rows4 + (size_t)rs * (ROW_STRIDE / 4);
@@ -2027,7 +2052,10 @@ __device__ __forceinline__ void process_item_l(const float4 *rows4, const float4
         for (int k = 0; k < 8; k++) a[k] = row4[k];
     }
     const int b1 = as_int(a[7].x), b2 = as_int(a[7].y);
-    if (fa && !ist[isl_b[b1] * 4 + 2]) return;
+    if (fa && !ist[isl_b[b1] * 4 + 2]) {
+        after_loads();
+        return;
+    }
     float f1[6], f2[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int k = 0; k < 6; k++) f1[k] = s_fc[b1 * 6 + k];
@@ -2035,17 +2063,18 @@ __device__ __forceinline__ void process_item_l(const float4 *rows4, const float4
 #pragma unroll
         for (int k = 0; k < 6; k++) f2[k] = s_fc[b2 * 6 + k];
     }
+    const float old0 = lam[rs];
+    after_loads();
     float lam_first = 0.f;
     for (int q = 0; q < nr; q++) {
         if (q > 0) {
 #pragma unroll
             for (int k = 0; k < 8; k++) a[k] = row4[q * 8 + k];
         }
-        const float old = lam[rs + q];
-        float delta = a[1].z - old * a[1].w;
-        delta -= f1[0] * a[0].x + f1[1] * a[0].y + f1[2] * a[0].z + f1[3] * a[0].w + f1[4] * a[1].x + f1[5] * a[1].y;
-        if (b2 >= 0)
-            delta -= f2[0] * a[2].x + f2[1] * a[2].y + f2[2] * a[2].z + f2[3] * a[2].w + f2[4] * a[3].x + f2[5] * a[3].y;
+        const float old = q == 0 ? old0 : lam[rs + q];
+        float delta = fmaf(-old, a[1].w, a[1].z);
+        delta -= row_dot6(f1, a[0], a[1].x, a[1].y);
+        if (b2 >= 0) delta -= row_dot6(f2, a[2], a[3].x, a[3].y);
         float lo = a[3].z, hi = a[3].w;
         const int fidx = as_int(a[7].z);
         if (fidx >= 0) {
@@ -2058,12 +2087,8 @@ __device__ __forceinline__ void process_item_l(const float4 *rows4, const float4
         else if (nl > hi) { delta = hi - old; nl = hi; }
         lam[rs + q] = nl;
         if (q == 0) lam_first = nl;
-        f1[0] += delta * a[4].x; f1[1] += delta * a[4].y; f1[2] += delta * a[4].z;
-        f1[3] += delta * a[4].w; f1[4] += delta * a[5].x; f1[5] += delta * a[5].y;
-        if (b2 >= 0) {
-            f2[0] += delta * a[5].z; f2[1] += delta * a[5].w; f2[2] += delta * a[6].x;
-            f2[3] += delta * a[6].y; f2[4] += delta * a[6].z; f2[5] += delta * a[6].w;
-        }
+        row_axpy6(f1, delta, a[4], a[5].x, a[5].y);
+        if (b2 >= 0) row_axpy6(f2, delta, make_float4(a[5].z, a[5].w, a[6].x, a[6].y), a[6].z, a[6].w);
         if (fa && delta != 0.f) {
             const int sg = delta > 0.f ? 1 : 0;
             const float m1 = fmaxf(fmaxf(fmaxf(fabsf(a[4].x), fabsf(a[4].y)), fmaxf(fabsf(a[4].z), fabsf(a[4].w))),
@@ -2082,6 +2107,12 @@ __device__ __forceinline__ void process_item_l(const float4 *rows4, const float4
 #pragma unroll
         for (int k = 0; k < 6; k++) s_fc[b2 * 6 + k] = f2[k];
     }
+}
+
+__device__ __forceinline__ void process_item_l(const float4 *rows4, const float4 *pre, int item, float *lam,
+                                               float *s_fc, float *fa = nullptr, const int *isl_b = nullptr,
+                                               const int *ist = nullptr) {
+    process_item_lf(rows4, pre, item, lam, s_fc, fa, isl_b, ist, [] {});
 }
 
 // The single-world step.  STAGED = false: fc in LDS, lambda, the schedule and
@@ -2185,43 +2216,134 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
                 // levels from LDS with the next level's rows prefetched; lane
                 // `ln` of `stride` takes items ln, ln + stride, ...; `sync` ends a
                 // level; levels `ph_[0 .. L_]`
-                auto levels = [&](const int ln, const int stride, const int *ph_, const int L_, auto sync) {
-                    const int *ph = ph_;
-                    const int L = L_;
+                // Software-pipelined: at the top of level p a lane holds its
+                // first items of levels p (cur, first row in pre) and p + 1
+                // (nxt) and the bounds of levels p + 1 and p + 2.  Level p
+                // requests nxt's row and level p + 2's end and item after its
+                // own item's operands, so that nothing waits on them before the
+                // next level.  Those reads are clamped to the arrays and their
+                // values discarded where a level has no item for the lane.
+                auto levels = [&](const int ln, const int stride, const int *ph, const int L, auto sync) {
+                    const int last_item = nit - 1;
                     int s = ph[0], e = ph[1];
+                    int s1 = L > 1 ? ph[1] : 0, e1 = L > 1 ? ph[2] : 0;
                     int cur = s + ln < e ? sched[s + ln] : -1;
+                    int nxt = s1 + ln < e1 ? sched[s1 + ln] : -1;
                     float4 pre[8], npre[8];
-                    if (cur >= 0) {
-                        const float4 *rr = rows4 + (size_t)(cur >> 3) * (ROW_STRIDE / 4);
+                    {
+                        const float4 *rr = rows4 + (size_t)(cur >= 0 ? cur >> 3 : 0) * (ROW_STRIDE / 4);
 #pragma unroll
                         for (int k = 0; k < 8; k++) pre[k] = rr[k];
                     }
                     for (int p = 0; p < L; p++) {
-                        int nxt = -1, s2 = 0, e2 = 0;
-                        if (p + 1 < L) {
-                            s2 = ph[p + 1];
-                            e2 = ph[p + 2];
-                            if (s2 + ln < e2) {
-                                nxt = sched[s2 + ln];
-                                const float4 *rr = rows4 + (size_t)(nxt >> 3) * (ROW_STRIDE / 4);
+                        int e2 = 0, n2 = -1;  // level p + 2: end, this lane's item
+                        auto prefetch = [&] {
+                            const float4 *rr = rows4 + (size_t)(nxt >= 0 ? nxt >> 3 : 0) * (ROW_STRIDE / 4);
 #pragma unroll
-                                for (int k = 0; k < 8; k++) npre[k] = rr[k];
-                            }
-                        }
-                        if (cur >= 0) process_item_l(rows4, pre, cur, lam, s_fc, fa, isl_b, ist);
+                            for (int k = 0; k < 8; k++) npre[k] = rr[k];
+                            e2 = ph[min(p + 3, L)];
+                            n2 = sched[min(e1 + ln, last_item)];
+                        };
+                        if (cur >= 0) process_item_lf(rows4, pre, cur, lam, s_fc, fa, isl_b, ist, prefetch);
+                        else prefetch();
                         for (int k = s + ln + stride; k < e; k += stride)
                             process_item_l(rows4, nullptr, sched[k], lam, s_fc, fa, isl_b, ist);
                         sync();
-                        s = s2;
-                        e = e2;
+                        const int s2 = e1;
+                        s = s1;
+                        e = e1;
+                        s1 = s2;
+                        e1 = p + 2 < L ? e2 : s2;
                         cur = nxt;
-                        if (nxt >= 0) {
 #pragma unroll
-                            for (int k = 0; k < 8; k++) pre[k] = npre[k];
-                        }
+                        for (int k = 0; k < 8; k++) pre[k] = npre[k];
+                        nxt = s1 + ln < e1 ? n2 : -1;
                     }
                 };
-                if (P.wave_groups > 0) {
+                // Island groups without the dynamic iteration count: every item
+                // is one row (the reference order's levels), so a level is one
+                // row update per lane with no branch on its operands.  The two
+                // row buffers alternate between levels instead of being copied;
+                // a row without body 2 reads body 1's fc in its place, uses
+                // zeros, and stores body 2's slot first so that body 1's values
+                // win.  The arithmetic is process_item_l's, term for term.
+                auto lean_levels = [&](const int ln, const int stride, const int *ph, const int L, auto sync) {
+#pragma clang fp contract(fast)
+                    const int last_item = nit - 1;
+                    int s = ph[0], e = ph[1];
+                    int s1 = L > 1 ? ph[1] : 0, e1 = L > 1 ? ph[2] : 0;
+                    int cur = s + ln < e ? sched[s + ln] : -1;
+                    int nxt = s1 + ln < e1 ? sched[s1 + ln] : -1;
+                    float4 A[8], B[8];
+                    {
+                        const float4 *rr = rows4 + (size_t)(cur >= 0 ? cur >> 3 : 0) * (ROW_STRIDE / 4);
+#pragma unroll
+                        for (int k = 0; k < 8; k++) A[k] = rr[k];
+                    }
+                    auto level = [&](const int p, const float4 *a, float4 *Y) {
+#pragma clang fp contract(fast)
+                        const int rs = cur >= 0 ? cur >> 3 : 0;
+                        const int b1 = as_int(a[7].x), b2 = as_int(a[7].y), fidx = as_int(a[7].z);
+                        const int b2c = b2 >= 0 ? b2 : b1;
+                        float f1[6], f2[6];
+#pragma unroll
+                        for (int k = 0; k < 6; k++) f1[k] = s_fc[b1 * 6 + k];
+#pragma unroll
+                        for (int k = 0; k < 6; k++) f2[k] = s_fc[b2c * 6 + k];
+                        const float old = lam[rs];
+                        const float lf = lam[fidx >= 0 ? fidx : rs];
+                        // the next level's row and the level after's end and item
+                        {
+                            const float4 *rr = rows4 + (size_t)(nxt >= 0 ? nxt >> 3 : 0) * (ROW_STRIDE / 4);
+#pragma unroll
+                            for (int k = 0; k < 8; k++) Y[k] = rr[k];
+                        }
+                        const int e2 = ph[min(p + 3, L)];
+                        const int n2 = sched[min(e1 + ln, last_item)];
+                        if (cur >= 0) {
+#pragma unroll
+                            for (int k = 0; k < 6; k++) f2[k] = b2 >= 0 ? f2[k] : 0.f;
+                            float delta = fmaf(-old, a[1].w, a[1].z);
+                            delta -= row_dot6(f1, a[0], a[1].x, a[1].y);
+                            if (b2 >= 0) delta -= row_dot6(f2, a[2], a[3].x, a[3].y);
+                            float lo = a[3].z, hi = a[3].w;
+                            if (fidx >= 0) {
+                                hi = fabsf(hi * lf);
+                                lo = -hi;
+                            }
+                            const float nl0 = old + delta;
+                            const bool bl = nl0 < lo, bh = !bl && nl0 > hi;
+                            const float nl = bl ? lo : (bh ? hi : nl0);
+                            delta = bl ? lo - old : (bh ? hi - old : delta);
+                            lam[rs] = nl;
+                            row_axpy6(f1, delta, a[4], a[5].x, a[5].y);
+                            row_axpy6(f2, delta, make_float4(a[5].z, a[5].w, a[6].x, a[6].y), a[6].z, a[6].w);
+#pragma unroll
+                            for (int k = 0; k < 6; k++) s_fc[b2c * 6 + k] = f2[k];
+#pragma unroll
+                            for (int k = 0; k < 6; k++) s_fc[b1 * 6 + k] = f1[k];
+                        }
+                        for (int k = s + ln + stride; k < e; k += stride)
+                            process_item_l(rows4, nullptr, sched[k], lam, s_fc);
+                        sync();
+                        const int s2 = e1;
+                        s = s1;
+                        e = e1;
+                        s1 = s2;
+                        e1 = p + 2 < L ? e2 : s2;
+                        cur = nxt;
+                        nxt = s1 + ln < e1 ? n2 : -1;
+                    };
+                    for (int p = 0; p < L; p += 2) {
+                        level(p, A, B);
+                        if (p + 1 < L) level(p + 1, B, A);
+                    }
+                };
+                if (P.wave_groups > 0 && !fa) {
+                    group_levels([&](const int ln, const int stride, const int *gp, const int Lg, auto sync) {
+                        lean_levels(ln, stride, gp, Lg, sync);
+                    });
+                } else if (P.wave_groups > 0) {
                     group_levels([&](const int ln, const int stride, const int *gp, const int Lg, auto sync) {
                         levels(ln, stride, gp, Lg, sync);
                     });
