@@ -24,6 +24,8 @@
 namespace odx {
 void launch_single_step(const DevWorlds &D, const StepParams &P, int nb, int m, int nitems, int nphases,
                         hipStream_t s);
+bool launch_single_groups(const DevWorlds &D, const StepParams &P, size_t lds, hipStream_t s);
+int single_lds_max();
 void launch_aabbs(const float *gviews, int n, float *aabb, hipStream_t s);
 void launch_space_collide(const SpaceCall &C, hipStream_t s);
 int space_max_geoms();
@@ -2072,6 +2074,12 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     // order, or greedy colours of whole joints
     std::vector<int> sched(m), phase;
     int wave_groups = 0;
+    // island groups on separate workgroups (k_single_groups): the group of each
+    // island and the LDS one such workgroup takes
+    bool group_blocks = false;
+    std::vector<int> isl_group;
+    size_t group_lds = 0;
+    const size_t group_lds_max = std::min((size_t)single_lds_max(), (size_t)SL_GROUP_LDS);
     {
         std::vector<int> level;
         int L = 0;
@@ -2105,19 +2113,61 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
                                    w->dyn_premature != 0.f;
             wave_groups = 0;
             if (w->order == 0 && !dyn_sched &&
-                (w->sweep_mode == 3 || (w->sweep_mode == 0 && isl_rows >= 4 * SL_STAGED_WAVES)))
+                (w->sweep_mode == 3 || (w->sweep_mode == 0 && isl_rows >= 4 * SL_STAGED_WAVES))) {
                 wave_groups = SL_STAGED_WAVES;
-            if (wave_groups > 0) {
-                const int G = wave_groups;
-                // items per level of each island, then islands (deepest first)
-                // to the group whose passes (sum over levels of ceil(items / 64))
-                // grow least
-                std::vector<std::vector<int>> ilc(std::max(nisl, 1));
+                group_blocks = true;
+            }
+            // items per level of each island
+            std::vector<std::vector<int>> ilc(std::max(nisl, 1));
+            if (wave_groups > 0)
                 for (int r = 0; r < m; r++) {
                     std::vector<int> &c = ilc[island[rb1[r]]];
                     if ((int)c.size() <= level[r]) c.resize(level[r] + 1, 0);
                     c[level[r]]++;
                 }
+            // one workgroup per group: runs of consecutive islands (their
+            // bodies, joints and rows are consecutive), cut before an island
+            // that would give a level of the run more than 64 items (a second
+            // pass of the wave) or take the run's LDS past the budget; more
+            // than SL_GROUP_BLOCKS runs fall back to groups sharing one
+            // workgroup
+            std::vector<int> run_start;  // first island of each run, then nisl
+            if (group_blocks) {
+                std::vector<long> ibod(std::max(nisl, 1), 0), irow(std::max(nisl, 1), 0);
+                for (int i = 0; i < nb; i++) ibod[island[i]]++;
+                for (int r = 0; r < m; r++) irow[island[rb1[r]]]++;
+                std::vector<int> width;  // items per level of the current run
+                size_t words = 1;        // its LDS words: 6 per body, 2 per row, 1 per level, 1
+                run_start.assign(1, 0);
+                for (int i = 0; i < nisl; i++) {
+                    const size_t wi = (size_t)ibod[i] * 6 + 2 * (size_t)irow[i];
+                    bool fits = (words + wi + std::max(ilc[i].size(), width.size()) - width.size()) *
+                                    sizeof(float) <= group_lds_max;
+                    for (size_t l = 0; fits && l < ilc[i].size(); l++)
+                        fits = (l < width.size() ? width[l] : 0) + ilc[i][l] <= 64;
+                    if (!fits && i > run_start.back()) {
+                        run_start.push_back(i);
+                        width.clear();
+                        words = 1;
+                    }
+                    if (width.size() < ilc[i].size()) {
+                        words += ilc[i].size() - width.size();
+                        width.resize(ilc[i].size(), 0);
+                    }
+                    for (size_t l = 0; l < ilc[i].size(); l++) width[l] += ilc[i][l];
+                    words += wi;
+                }
+                run_start.push_back(std::max(nisl, 1));
+                wave_groups = (int)run_start.size() - 1;
+                if (wave_groups > SL_GROUP_BLOCKS) {
+                    group_blocks = false;
+                    wave_groups = SL_STAGED_WAVES;
+                }
+            }
+            for (int attempt_g = 0; wave_groups > 0 && attempt_g < 2; attempt_g++) {
+                const int G = wave_groups;
+                // islands (deepest first) to the group whose passes (sum over
+                // levels of ceil(items / 64)) grow least, or runs of islands
                 std::vector<int> order_isl;
                 for (int i = 0; i < nisl; i++)
                     if (!ilc[i].empty()) order_isl.push_back(i);
@@ -2127,22 +2177,54 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
                 std::vector<long> gcost(G, 0);
                 std::vector<int> group(std::max(nisl, 1), 0);
                 auto passes = [](int n) { return (n + 63) / 64; };
-                for (int i : order_isl) {
-                    int best = 0;
-                    long bestc = -1;
-                    for (int g = 0; g < G; g++) {
-                        long d = 0;
-                        for (size_t l = 0; l < ilc[i].size(); l++) {
-                            const int have = l < gc[g].size() ? gc[g][l] : 0;
-                            d += passes(have + ilc[i][l]) - passes(have);
+                if (group_blocks) {
+                    for (int g = 0; g < G; g++)
+                        for (int i = run_start[g]; i < run_start[g + 1] && i < nisl; i++) {
+                            group[i] = g;
+                            std::vector<int> &c = gc[g];
+                            if (c.size() < ilc[i].size()) c.resize(ilc[i].size(), 0);
+                            for (size_t l = 0; l < ilc[i].size(); l++) c[l] += ilc[i][l];
                         }
-                        if (bestc < 0 || gcost[g] + d < bestc) { bestc = gcost[g] + d; best = g; }
+                } else {
+                    for (int i : order_isl) {
+                        int best = 0;
+                        long bestc = -1;
+                        for (int g = 0; g < G; g++) {
+                            long d = 0;
+                            for (size_t l = 0; l < ilc[i].size(); l++) {
+                                const int have = l < gc[g].size() ? gc[g][l] : 0;
+                                d += passes(have + ilc[i][l]) - passes(have);
+                            }
+                            if (bestc < 0 || gcost[g] + d < bestc) { bestc = gcost[g] + d; best = g; }
+                        }
+                        std::vector<int> &c = gc[best];
+                        if (c.size() < ilc[i].size()) c.resize(ilc[i].size(), 0);
+                        for (size_t l = 0; l < ilc[i].size(); l++) c[l] += ilc[i][l];
+                        gcost[best] = bestc;
+                        group[i] = best;
                     }
-                    std::vector<int> &c = gc[best];
-                    if (c.size() < ilc[i].size()) c.resize(ilc[i].size(), 0);
-                    for (size_t l = 0; l < ilc[i].size(); l++) c[l] += ilc[i][l];
-                    gcost[best] = bestc;
-                    group[i] = best;
+                }
+                if (group_blocks) {
+                    // LDS of a run: fc of its bodies, lambda of its rows (one
+                    // item each), its level starts; over the budget, the
+                    // groups share one workgroup instead
+                    std::vector<long> ibod(std::max(nisl, 1), 0), irow(std::max(nisl, 1), 0);
+                    for (int i = 0; i < nb; i++) ibod[island[i]]++;
+                    for (int r = 0; r < m; r++) irow[island[rb1[r]]]++;
+                    size_t most = 0;
+                    for (int g = 0; g < G; g++) {
+                        size_t fl = gc[g].size() + 1;
+                        for (int i = run_start[g]; i < run_start[g + 1] && i < nisl; i++)
+                            fl += (size_t)ibod[i] * 6 + 2 * (size_t)irow[i];
+                        most = std::max(most, fl);
+                    }
+                    group_lds = most * sizeof(float);
+                    if (group_lds > group_lds_max) {
+                        group_blocks = false;
+                        wave_groups = SL_STAGED_WAVES;
+                        continue;
+                    }
+                    isl_group = group;
                 }
                 // phase table: G + 1 offsets of the groups' level lists (from
                 // entry G + 1), then per group its level starts and end; items
@@ -2161,7 +2243,9 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
                 phase[G] = (int)(phase.size() - (G + 1));
                 std::vector<int> pos(start.begin(), start.end() - 1);
                 for (int r = 0; r < m; r++) sched[pos[first[group[island[rb1[r]]]] + level[r]]++] = (r << 3) | 1;
-            } else {
+                break;
+            }
+            if (wave_groups == 0) {
                 std::vector<int> cnt(L + 1, 0);
                 for (int r = 0; r < m; r++) cnt[level[r] + 1]++;
                 for (int p = 1; p <= L; p++) cnt[p] += cnt[p - 1];
@@ -2349,6 +2433,29 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
         const int xm = xc < (float)ODX_MAX_EXTRA_SWEEPS ? (int)xc : ODX_MAX_EXTRA_SWEEPS;
         return xm != 0 || w->dyn_premature != 0.f;
     }();
+    // island groups on separate workgroups: the first body and joint of each
+    // group's run of islands (bodies and joints are listed island by island)
+    std::vector<int> gbs, gjs;
+    if (group_blocks) {
+        const int NG = wave_groups;
+        gbs.assign(NG + 1, 0);
+        gjs.assign(NG + 1, 0);
+        for (int i = 0; i < nb; i++) gbs[isl_group[island[i]] + 1]++;
+        for (int k = 0; k < nj; k++) gjs[isl_group[island[(int)jrec[(size_t)k * JOINT_STRIDE + 1]]] + 1]++;
+        for (int g = 0; g < NG; g++) { gbs[g + 1] += gbs[g]; gjs[g + 1] += gjs[g]; }
+        // the runs must cover the lists in order (a body or joint out of its
+        // run's range would be stepped by no workgroup)
+        for (int i = 0; i < nb; i++)
+            if (i < gbs[isl_group[island[i]]] || i >= gbs[isl_group[island[i]] + 1]) group_blocks = false;
+        for (int k = 0; k < nj && group_blocks; k++) {
+            const int g = isl_group[island[(int)jrec[(size_t)k * JOINT_STRIDE + 1]]];
+            if (k < gjs[g] || k >= gjs[g + 1]) group_blocks = false;
+        }
+        if (!group_blocks) {
+            odx_message(4, "dWorldQuickStep: island runs out of list order");
+            return 0;
+        }
+    }
     Staging &G = w->stage;
     G.reset();
     const size_t f4 = (size_t)nb * sizeof(float4);
@@ -2366,6 +2473,7 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     const size_t o_island = G.slot(dyn_on ? island.size() * sizeof(int) : 0);
     const size_t o_seg = G.slot(seg_table.size() * sizeof(int));
     const size_t o_jext = G.slot(jext.size() * sizeof(float));
+    const size_t o_gbs = G.slot(gbs.size() * sizeof(int)), o_gjs = G.slot(gjs.size() * sizeof(int));
     bool ok = G.alloc() && w->rows.ensure((size_t)std::max(m, 1) * ROW_STRIDE * sizeof(float)) &&
               w->lambda.ensure((size_t)std::max(m, 1) * sizeof(float)) &&
               w->bscr.ensure((size_t)std::max(nb, 1) * BSCR_STRIDE * sizeof(float));
@@ -2380,6 +2488,7 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     if (dyn_on) G.put(o_island, island.data(), island.size());
     G.put(o_seg, seg_table.data(), seg_table.size());
     G.put(o_jext, jext.data(), jext.size());
+    G.put(o_gbs, gbs.data(), gbs.size()); G.put(o_gjs, gjs.data(), gjs.size());
     if (!G.send(0, G.used)) return 0;
     DevWorlds D{};
     D.nworlds = 1;
@@ -2404,6 +2513,10 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     D.fb = want_fb ? G.d<float>(o_fb) : nullptr;
     D.damp = damp.empty() ? nullptr : G.d<float4>(o_damp);
     D.jext = jext.empty() ? nullptr : G.d<float>(o_jext);
+    if (group_blocks) {
+        D.grp_body_start = G.d<int>(o_gbs);
+        D.grp_joint_start = G.d<int>(o_gjs);
+    }
     StepParams P{};
     P.h = stepsize;
     P.gx = w->gravity[0]; P.gy = w->gravity[1]; P.gz = w->gravity[2];
@@ -2442,7 +2555,13 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
             return 0;
     }
     if (g_ktime.on && !odx_check(hipEventRecord(g_ktime.ev[0], g_stream), "hipEventRecord")) return 0;
-    launch_single_step(D, P, nb, m, (int)sched.size(), L, g_stream);
+    if (!(group_blocks && launch_single_groups(D, P, group_lds, g_stream))) {
+        if (group_blocks) {
+            odx_message(4, "dWorldQuickStep: the island groups' workgroups could not be launched");
+            return 0;
+        }
+        launch_single_step(D, P, nb, m, (int)sched.size(), L, g_stream);
+    }
     if (!odx_check(hipGetLastError(), "k_single_step")) return 0;
     if (g_ktime.on && !odx_check(hipEventRecord(g_ktime.ev[1], g_stream), "hipEventRecord")) return 0;
     ok = G.fetch(0, out_end) && odx_check(hipStreamSynchronize(g_stream), "hipStreamSynchronize");

@@ -92,8 +92,13 @@ constexpr int ODX_MAX_SPLITS = 8;
 // Largest dynamic LDS of the batched solve kernel (bytes per workgroup).
 constexpr int SOLVE_LDS_MAX = 65536;
 // waves of the staged single-world step kernel (512 threads): the island
-// groups of sweep mode 3
+// groups of sweep mode 3 when they share one workgroup
 constexpr int SL_STAGED_WAVES = 8;
+// island groups of sweep mode 3 on one workgroup each: at most this many
+// groups, and the LDS one may take (fc of its bodies, lambda of its rows, its
+// schedule)
+constexpr int SL_GROUP_BLOCKS = 128;
+constexpr int SL_GROUP_LDS = 65536;
 // LDS budgets of the batched solve besides the rows: work items per world
 // (three quarters of the row budget, plus 16) and sweep phases per world.  A
 // world over any budget sweeps from global memory instead.
@@ -170,6 +175,9 @@ struct DevWorlds {
     const int *seg_table;   // [3 * nseg], may be null
     int nseg;
     int *island_sweeps;     // [nislands] sweeps each island ran, may be null
+    // drop-in world, island groups on separate workgroups: the first body and
+    // joint of each group's run of islands ([G + 1] each), may be null
+    const int *grp_body_start, *grp_joint_start;
 };
 constexpr int PTIME_SLOTS = 6;
 

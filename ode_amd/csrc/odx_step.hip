@@ -2033,13 +2033,12 @@ __device__ __forceinline__ void row_axpy6(float *f, float delta, float4 m0, floa
     f[5] = fmaf(delta, m5, f[5]);
 }
 
-// after_loads() runs once the item's bodies' fc and first lambda are
-// requested and before they are used: a caller requests the next levels'
-// operands there, behind this item's (a wave's LDS reads return in order).
-template <class F>
-__device__ __forceinline__ void process_item_lf(const float4 *rows4, const float4 *pre, int item, float *lam,
-                                                float *s_fc, float *fa, const int *isl_b, const int *ist,
-                                                F &&after_loads) {
+// bbase / rbase: the first body and row whose fc and lambda the arrays hold
+// (a workgroup holding one island group's range); rows and items keep global
+// indices.
+__device__ __forceinline__ void process_item_l(const float4 *rows4, const float4 *pre, int item, float *lam,
+                                               float *s_fc, float *fa = nullptr, const int *isl_b = nullptr,
+                                               const int *ist = nullptr, int bbase = 0, int rbase = 0) {
 #pragma clang fp contract(fast)
     const int rs = item >> 3, nr = item & 7;
     const float4 *row4 = rows4 + (size_t)rs * (ROW_STRIDE / 4);
@@ -2052,40 +2051,36 @@ __device__ __forceinline__ void process_item_lf(const float4 *rows4, const float
         for (int k = 0; k < 8; k++) a[k] = row4[k];
     }
     const int b1 = as_int(a[7].x), b2 = as_int(a[7].y);
-    if (fa && !ist[isl_b[b1] * 4 + 2]) {
-        after_loads();
-        return;
-    }
+    if (fa && !ist[isl_b[b1] * 4 + 2]) return;
+    const int l1 = b1 - bbase, l2 = b2 - bbase, rl = rs - rbase;
     float f1[6], f2[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int k = 0; k < 6; k++) f1[k] = s_fc[b1 * 6 + k];
+    for (int k = 0; k < 6; k++) f1[k] = s_fc[l1 * 6 + k];
     if (b2 >= 0) {
 #pragma unroll
-        for (int k = 0; k < 6; k++) f2[k] = s_fc[b2 * 6 + k];
+        for (int k = 0; k < 6; k++) f2[k] = s_fc[l2 * 6 + k];
     }
-    const float old0 = lam[rs];
-    after_loads();
     float lam_first = 0.f;
     for (int q = 0; q < nr; q++) {
         if (q > 0) {
 #pragma unroll
             for (int k = 0; k < 8; k++) a[k] = row4[q * 8 + k];
         }
-        const float old = q == 0 ? old0 : lam[rs + q];
+        const float old = lam[rl + q];
         float delta = fmaf(-old, a[1].w, a[1].z);
         delta -= row_dot6(f1, a[0], a[1].x, a[1].y);
         if (b2 >= 0) delta -= row_dot6(f2, a[2], a[3].x, a[3].y);
         float lo = a[3].z, hi = a[3].w;
         const int fidx = as_int(a[7].z);
         if (fidx >= 0) {
-            const float lf = (fidx == rs) ? lam_first : lam[fidx];
+            const float lf = (fidx == rs) ? lam_first : lam[fidx - rbase];
             hi = fabsf(hi * lf);
             lo = -hi;
         }
         float nl = old + delta;
         if (nl < lo) { delta = lo - old; nl = lo; }
         else if (nl > hi) { delta = hi - old; nl = hi; }
-        lam[rs + q] = nl;
+        lam[rl + q] = nl;
         if (q == 0) lam_first = nl;
         row_axpy6(f1, delta, a[4], a[5].x, a[5].y);
         if (b2 >= 0) row_axpy6(f2, delta, make_float4(a[5].z, a[5].w, a[6].x, a[6].y), a[6].z, a[6].w);
@@ -2102,17 +2097,114 @@ __device__ __forceinline__ void process_item_lf(const float4 *rows4, const float
         }
     }
 #pragma unroll
-    for (int k = 0; k < 6; k++) s_fc[b1 * 6 + k] = f1[k];
+    for (int k = 0; k < 6; k++) s_fc[l1 * 6 + k] = f1[k];
     if (b2 >= 0) {
 #pragma unroll
-        for (int k = 0; k < 6; k++) s_fc[b2 * 6 + k] = f2[k];
+        for (int k = 0; k < 6; k++) s_fc[l2 * 6 + k] = f2[k];
     }
 }
 
-__device__ __forceinline__ void process_item_l(const float4 *rows4, const float4 *pre, int item, float *lam,
-                                               float *s_fc, float *fa = nullptr, const int *isl_b = nullptr,
-                                               const int *ist = nullptr) {
-    process_item_lf(rows4, pre, item, lam, s_fc, fa, isl_b, ist, [] {});
+// Levels ph[0 .. L] of island groups swept without the dynamic iteration
+// count: every item is one row (the reference order's levels), so a level is
+// one row update per lane with no branch on its operands.  The rows come from
+// global memory (L2) three levels ahead: at the top of level p a lane holds
+// its items of levels p .. p + 3, the rows of p .. p + 2 in three of four
+// register buffers that rotate (the fourth receives p + 3's row during level
+// p), and the level ends up to p + 3; level p also requests level p + 4's end
+// and item.  Reads past a level's end are clamped to the arrays and
+// discarded.  A row without body 2 reads body 1's fc in its place, uses zeros,
+// and stores body 2's slot first so that body 1's values win.  The arithmetic
+// is process_item_l's, term for term.  bbase / rbase as for process_item_l.
+template <class Sync>
+__device__ __forceinline__ void lean_levels_fn(const float4 *rows4, const int *sched, const int nit, float *lam,
+                                               float *s_fc, const int ln, const int stride, const int *ph,
+                                               const int L, Sync sync, const int bbase = 0, const int rbase = 0) {
+#pragma clang fp contract(fast)
+    const int last_item = nit - 1;
+    // level q spans [ph[q], ph[q + 1]); levels past L - 1 are empty
+    int s = ph[0], e = ph[1];
+    int e1 = L > 1 ? ph[2] : e, e2 = L > 2 ? ph[3] : e1, e3 = L > 3 ? ph[4] : e2;
+    int cur = s + ln < e ? sched[s + ln] : -1;
+    int i1 = e + ln < e1 ? sched[e + ln] : -1;
+    int i2 = e1 + ln < e2 ? sched[e1 + ln] : -1;
+    int i3 = e2 + ln < e3 ? sched[e2 + ln] : -1;
+    auto fetch = [&](float4 *X, const int item) {
+        const float4 *rr = rows4 + (size_t)(item >= 0 ? item >> 3 : 0) * (ROW_STRIDE / 4);
+#pragma unroll
+        for (int k = 0; k < 8; k++) X[k] = rr[k];
+    };
+    float4 A[8], B[8], C[8], Dd[8];
+    fetch(A, cur);
+    fetch(B, i1);
+    fetch(C, i2);
+    // a: the row of level p; Y: the buffer for level p + 3's row
+    auto level = [&](const int p, const float4 *a, float4 *Y) {
+#pragma clang fp contract(fast)
+        // the row's last word (Ad) is not needed here; keeping it live until
+        // its level stops the compiler from reusing its register while the
+        // load that fills it is in flight (which would wait for that load)
+        asm volatile("" ::"v"(a[7].w));
+        // a lane without an item reads slot 0 (its row buffer may hold a row
+        // of another range)
+        const bool on = cur >= 0;
+        const int rs = on ? (cur >> 3) - rbase : 0;
+        const int b2 = as_int(a[7].y), fidx = as_int(a[7].z);
+        const int b1 = on ? as_int(a[7].x) - bbase : 0;
+        const int b2c = on && b2 >= 0 ? b2 - bbase : b1;
+        float f1[6], f2[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) f1[k] = s_fc[b1 * 6 + k];
+#pragma unroll
+        for (int k = 0; k < 6; k++) f2[k] = s_fc[b2c * 6 + k];
+        const float old = lam[rs];
+        const float lf = lam[on && fidx >= 0 ? fidx - rbase : rs];
+        // level p + 3's row, level p + 4's end and item
+        fetch(Y, i3);
+        const int e4 = ph[min(p + 5, L)];
+        const int n4 = sched[min(e3 + ln, last_item)];
+        if (on) {
+#pragma unroll
+            for (int k = 0; k < 6; k++) f2[k] = b2 >= 0 ? f2[k] : 0.f;
+            float delta = fmaf(-old, a[1].w, a[1].z);
+            delta -= row_dot6(f1, a[0], a[1].x, a[1].y);
+            if (b2 >= 0) delta -= row_dot6(f2, a[2], a[3].x, a[3].y);
+            float lo = a[3].z, hi = a[3].w;
+            if (fidx >= 0) {
+                hi = fabsf(hi * lf);
+                lo = -hi;
+            }
+            const float nl0 = old + delta;
+            const bool bl = nl0 < lo, bh = !bl && nl0 > hi;
+            const float nl = bl ? lo : (bh ? hi : nl0);
+            delta = bl ? lo - old : (bh ? hi - old : delta);
+            lam[rs] = nl;
+            row_axpy6(f1, delta, a[4], a[5].x, a[5].y);
+            row_axpy6(f2, delta, make_float4(a[5].z, a[5].w, a[6].x, a[6].y), a[6].z, a[6].w);
+#pragma unroll
+            for (int k = 0; k < 6; k++) s_fc[b2c * 6 + k] = f2[k];
+#pragma unroll
+            for (int k = 0; k < 6; k++) s_fc[b1 * 6 + k] = f1[k];
+        }
+        for (int k = s + ln + stride; k < e; k += stride)
+            process_item_l(rows4, nullptr, sched[k], lam, s_fc, nullptr, nullptr, nullptr, bbase, rbase);
+        sync();
+        const int e4v = p + 4 < L ? e4 : e3;
+        s = e;
+        e = e1;
+        e1 = e2;
+        e2 = e3;
+        cur = i1;
+        i1 = i2;
+        i2 = i3;
+        i3 = e3 + ln < e4v ? n4 : -1;
+        e3 = e4v;
+    };
+    for (int p = 0; p < L; p += 4) {
+        level(p, A, Dd);
+        if (p + 1 < L) level(p + 1, B, A);
+        if (p + 2 < L) level(p + 2, C, B);
+        if (p + 3 < L) level(p + 3, Dd, C);
+    }
 }
 
 // The single-world step.  STAGED = false: fc in LDS, lambda, the schedule and
@@ -2216,128 +2308,44 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
                 // levels from LDS with the next level's rows prefetched; lane
                 // `ln` of `stride` takes items ln, ln + stride, ...; `sync` ends a
                 // level; levels `ph_[0 .. L_]`
-                // Software-pipelined: at the top of level p a lane holds its
-                // first items of levels p (cur, first row in pre) and p + 1
-                // (nxt) and the bounds of levels p + 1 and p + 2.  Level p
-                // requests nxt's row and level p + 2's end and item after its
-                // own item's operands, so that nothing waits on them before the
-                // next level.  Those reads are clamped to the arrays and their
-                // values discarded where a level has no item for the lane.
-                auto levels = [&](const int ln, const int stride, const int *ph, const int L, auto sync) {
-                    const int last_item = nit - 1;
+                auto levels = [&](const int ln, const int stride, const int *ph_, const int L_, auto sync) {
+                    const int *ph = ph_;
+                    const int L = L_;
                     int s = ph[0], e = ph[1];
-                    int s1 = L > 1 ? ph[1] : 0, e1 = L > 1 ? ph[2] : 0;
                     int cur = s + ln < e ? sched[s + ln] : -1;
-                    int nxt = s1 + ln < e1 ? sched[s1 + ln] : -1;
                     float4 pre[8], npre[8];
-                    {
-                        const float4 *rr = rows4 + (size_t)(cur >= 0 ? cur >> 3 : 0) * (ROW_STRIDE / 4);
+                    if (cur >= 0) {
+                        const float4 *rr = rows4 + (size_t)(cur >> 3) * (ROW_STRIDE / 4);
 #pragma unroll
                         for (int k = 0; k < 8; k++) pre[k] = rr[k];
                     }
                     for (int p = 0; p < L; p++) {
-                        int e2 = 0, n2 = -1;  // level p + 2: end, this lane's item
-                        auto prefetch = [&] {
-                            const float4 *rr = rows4 + (size_t)(nxt >= 0 ? nxt >> 3 : 0) * (ROW_STRIDE / 4);
+                        int nxt = -1, s2 = 0, e2 = 0;
+                        if (p + 1 < L) {
+                            s2 = ph[p + 1];
+                            e2 = ph[p + 2];
+                            if (s2 + ln < e2) {
+                                nxt = sched[s2 + ln];
+                                const float4 *rr = rows4 + (size_t)(nxt >> 3) * (ROW_STRIDE / 4);
 #pragma unroll
-                            for (int k = 0; k < 8; k++) npre[k] = rr[k];
-                            e2 = ph[min(p + 3, L)];
-                            n2 = sched[min(e1 + ln, last_item)];
-                        };
-                        if (cur >= 0) process_item_lf(rows4, pre, cur, lam, s_fc, fa, isl_b, ist, prefetch);
-                        else prefetch();
+                                for (int k = 0; k < 8; k++) npre[k] = rr[k];
+                            }
+                        }
+                        if (cur >= 0) process_item_l(rows4, pre, cur, lam, s_fc, fa, isl_b, ist);
                         for (int k = s + ln + stride; k < e; k += stride)
                             process_item_l(rows4, nullptr, sched[k], lam, s_fc, fa, isl_b, ist);
                         sync();
-                        const int s2 = e1;
-                        s = s1;
-                        e = e1;
-                        s1 = s2;
-                        e1 = p + 2 < L ? e2 : s2;
+                        s = s2;
+                        e = e2;
                         cur = nxt;
+                        if (nxt >= 0) {
 #pragma unroll
-                        for (int k = 0; k < 8; k++) pre[k] = npre[k];
-                        nxt = s1 + ln < e1 ? n2 : -1;
+                            for (int k = 0; k < 8; k++) pre[k] = npre[k];
+                        }
                     }
                 };
-                // Island groups without the dynamic iteration count: every item
-                // is one row (the reference order's levels), so a level is one
-                // row update per lane with no branch on its operands.  The two
-                // row buffers alternate between levels instead of being copied;
-                // a row without body 2 reads body 1's fc in its place, uses
-                // zeros, and stores body 2's slot first so that body 1's values
-                // win.  The arithmetic is process_item_l's, term for term.
                 auto lean_levels = [&](const int ln, const int stride, const int *ph, const int L, auto sync) {
-#pragma clang fp contract(fast)
-                    const int last_item = nit - 1;
-                    int s = ph[0], e = ph[1];
-                    int s1 = L > 1 ? ph[1] : 0, e1 = L > 1 ? ph[2] : 0;
-                    int cur = s + ln < e ? sched[s + ln] : -1;
-                    int nxt = s1 + ln < e1 ? sched[s1 + ln] : -1;
-                    float4 A[8], B[8];
-                    {
-                        const float4 *rr = rows4 + (size_t)(cur >= 0 ? cur >> 3 : 0) * (ROW_STRIDE / 4);
-#pragma unroll
-                        for (int k = 0; k < 8; k++) A[k] = rr[k];
-                    }
-                    auto level = [&](const int p, const float4 *a, float4 *Y) {
-#pragma clang fp contract(fast)
-                        const int rs = cur >= 0 ? cur >> 3 : 0;
-                        const int b1 = as_int(a[7].x), b2 = as_int(a[7].y), fidx = as_int(a[7].z);
-                        const int b2c = b2 >= 0 ? b2 : b1;
-                        float f1[6], f2[6];
-#pragma unroll
-                        for (int k = 0; k < 6; k++) f1[k] = s_fc[b1 * 6 + k];
-#pragma unroll
-                        for (int k = 0; k < 6; k++) f2[k] = s_fc[b2c * 6 + k];
-                        const float old = lam[rs];
-                        const float lf = lam[fidx >= 0 ? fidx : rs];
-                        // the next level's row and the level after's end and item
-                        {
-                            const float4 *rr = rows4 + (size_t)(nxt >= 0 ? nxt >> 3 : 0) * (ROW_STRIDE / 4);
-#pragma unroll
-                            for (int k = 0; k < 8; k++) Y[k] = rr[k];
-                        }
-                        const int e2 = ph[min(p + 3, L)];
-                        const int n2 = sched[min(e1 + ln, last_item)];
-                        if (cur >= 0) {
-#pragma unroll
-                            for (int k = 0; k < 6; k++) f2[k] = b2 >= 0 ? f2[k] : 0.f;
-                            float delta = fmaf(-old, a[1].w, a[1].z);
-                            delta -= row_dot6(f1, a[0], a[1].x, a[1].y);
-                            if (b2 >= 0) delta -= row_dot6(f2, a[2], a[3].x, a[3].y);
-                            float lo = a[3].z, hi = a[3].w;
-                            if (fidx >= 0) {
-                                hi = fabsf(hi * lf);
-                                lo = -hi;
-                            }
-                            const float nl0 = old + delta;
-                            const bool bl = nl0 < lo, bh = !bl && nl0 > hi;
-                            const float nl = bl ? lo : (bh ? hi : nl0);
-                            delta = bl ? lo - old : (bh ? hi - old : delta);
-                            lam[rs] = nl;
-                            row_axpy6(f1, delta, a[4], a[5].x, a[5].y);
-                            row_axpy6(f2, delta, make_float4(a[5].z, a[5].w, a[6].x, a[6].y), a[6].z, a[6].w);
-#pragma unroll
-                            for (int k = 0; k < 6; k++) s_fc[b2c * 6 + k] = f2[k];
-#pragma unroll
-                            for (int k = 0; k < 6; k++) s_fc[b1 * 6 + k] = f1[k];
-                        }
-                        for (int k = s + ln + stride; k < e; k += stride)
-                            process_item_l(rows4, nullptr, sched[k], lam, s_fc);
-                        sync();
-                        const int s2 = e1;
-                        s = s1;
-                        e = e1;
-                        s1 = s2;
-                        e1 = p + 2 < L ? e2 : s2;
-                        cur = nxt;
-                        nxt = s1 + ln < e1 ? n2 : -1;
-                    };
-                    for (int p = 0; p < L; p += 2) {
-                        level(p, A, B);
-                        if (p + 1 < L) level(p + 1, B, A);
-                    }
+                    lean_levels_fn(rows4, sched, nit, lam, s_fc, ln, stride, ph, L, sync);
                 };
                 if (P.wave_groups > 0 && !fa) {
                     group_levels([&](const int ln, const int stride, const int *gp, const int Lg, auto sync) {
@@ -2486,6 +2494,74 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
     for (int i = t; i < nb; i += NT) integrate_body(D, P, i, s_fc + i * 6, m > 0, D.facc, D.tacc);
 }
 
+// The single-world step of island groups on one workgroup (one wave) each
+// (sweep mode 3 with a fixed sweep count): islands share no body, so each
+// group's bodies, rows, sweeps and integration need nothing from another
+// group, and the groups run on separate CUs with no grid barrier.  A group is
+// a run of consecutive islands, so it owns the consecutive bodies
+// [grp_body_start[g], grp_body_start[g + 1]), joints [grp_joint_start[g], ..)
+// and their rows; the phase table opens with G + 1 offsets of the groups'
+// level lists, whose items are consecutive in the schedule.  LDS: fc of the
+// group's bodies and lambda of its rows, then its items and level starts.
+__global__ __launch_bounds__(64) void k_single_groups(DevWorlds D, StepParams P) {
+    extern __shared__ float4 s_dyn4[];
+    const int g = blockIdx.x, t = threadIdx.x, G = P.wave_groups;
+    const int nj = D.njoints[0], m = D.nrows[0];
+    const int *gph = D.phase_start + (G + 1) + D.phase_start[g];
+    const int Lg = D.phase_start[g + 1] - D.phase_start[g] - 1;
+    const int gs = gph[0], ni = gph[Lg] - gs;
+    const int b0 = D.grp_body_start[g], b1 = D.grp_body_start[g + 1];
+    const int j0 = D.grp_joint_start[g], j1 = D.grp_joint_start[g + 1];
+    const int r0 = j0 < nj ? D.joff[j0] : m, r1 = j1 < nj ? D.joff[j1] : m;
+    float *s_fc = reinterpret_cast<float *>(s_dyn4);
+    float *lam = s_fc + (size_t)(b1 - b0) * 6;
+    int *s_sched = reinterpret_cast<int *>(lam + (r1 - r0));
+    int *s_ph = s_sched + ni;
+    for (int i = b0 + t; i < b1; i += 64) {
+        body_prep(D, P, i, D.facc, D.tacc);
+#pragma unroll
+        for (int c = 0; c < 6; c++) s_fc[(i - b0) * 6 + c] = 0.f;
+    }
+    for (int k = t; k < ni; k += 64) s_sched[k] = D.sched[gs + k];
+    for (int k = t; k <= Lg; k += 64) s_ph[k] = gph[k] - gs;
+    for (int r = r0 + t; r < r1; r += 64) lam[r - r0] = 0.f;
+    __syncthreads();
+    for (int j = j0 + t; j < j1; j += 64)
+        build_joint_rows(D, P, D.joints + (size_t)j * JOINT_STRIDE, D.jext ? D.jext + (size_t)j * JEXT_STRIDE : nullptr,
+                         0, 0, D.joff[j]);
+    // the rows this workgroup wrote are read back by it below
+    __syncthreads();
+    if (Lg > 0 && ni > 0) {
+        const float4 *rows4 = reinterpret_cast<const float4 *>(D.rows);
+        for (int it = 0; it < P.max_iters; it++)
+            lean_levels_fn(
+                rows4, s_sched, ni, lam, s_fc, t, 64, s_ph, Lg,
+                [] {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                },
+                b0, r0);
+    }
+    __syncthreads();
+    for (int r = r0 + t; r < r1; r += 64) D.lambda[r] = lam[r - r0];
+    // joint feedback: f = J^T lambda with the unscaled J (J_row / Ad)
+    if (P.feedback) {
+        for (int j = j0 + t; j < j1; j += 64) {
+            const int ri = D.joff[j], re = j + 1 < nj ? D.joff[j + 1] : m;
+            float f[12] = {0.f};
+            for (int r = ri; r < re; r++) {
+                const float *rw = D.rows + (size_t)r * ROW_STRIDE;
+                const float sc = lam[r - r0] / rw[R_AD];
+                for (int c = 0; c < 6; c++) { f[c] += rw[R_J1 + c] * sc; f[6 + c] += rw[R_J2 + c] * sc; }
+            }
+            for (int c = 0; c < 12; c++) D.fb[(size_t)j * 12 + c] = f[c];
+        }
+    }
+    for (int i = b0 + t; i < b1; i += 64) integrate_body(D, P, i, s_fc + (i - b0) * 6, m > 0, D.facc, D.tacc);
+    if (g == 0 && t == 0) { D.niters[0] = m > 0 ? P.max_iters : 0; D.residual[0] = -1.f; }
+}
+
 // -------------------------------------------- collision entry points -------
 
 // AABBs of geom views (computeAABB of each class).
@@ -2559,12 +2635,8 @@ void launch_step_fused(const DevWorlds &D, const StepParams &P, const MlpDev *M,
 namespace odx {
 // The staged kernel when fc, lambda, the items and the phase starts fit the
 // device's LDS per workgroup; otherwise the global-memory kernel.
-void launch_single_step(const DevWorlds &D, const StepParams &P0, int nb, int m, int nitems, int nphases,
-                        hipStream_t s) {
-    // sweep mode: 1 the whole block per level, 2 the first wave alone, 0 the
-    // wave when levels hold at most 64 items on average
-    StepParams P = P0;
-    P.wave_sweep = P0.wave_sweep == 2 || (P0.wave_sweep == 0 && nitems <= 64 * (nphases > 0 ? nphases : 1)) ? 1 : 0;
+// Largest dynamic LDS of one workgroup on the current device (bytes).
+int single_lds_max() {
     static int lds_max = -1;
     if (lds_max < 0) {
         int dev = 0, v = 0;
@@ -2573,6 +2645,29 @@ void launch_single_step(const DevWorlds &D, const StepParams &P0, int nb, int m,
                       ? v
                       : 65536;
     }
+    return lds_max;
+}
+
+// k_single_groups: P.wave_groups workgroups; lds = bytes of fc, lambda and
+// the largest group's items and level starts.  False if the LDS does not fit.
+bool launch_single_groups(const DevWorlds &D, const StepParams &P, size_t lds, hipStream_t s) {
+    if (lds > (size_t)single_lds_max()) return false;
+    if (lds > 65536 && hipFuncSetAttribute((const void *)k_single_groups, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    hipLaunchKernelGGL(k_single_groups, dim3(P.wave_groups), dim3(64), lds, s, D, P);
+    return true;
+}
+
+void launch_single_step(const DevWorlds &D, const StepParams &P0, int nb, int m, int nitems, int nphases,
+                        hipStream_t s) {
+    // sweep mode: 1 the whole block per level, 2 the first wave alone, 0 the
+    // wave when levels hold at most 64 items on average
+    StepParams P = P0;
+    P.wave_sweep = P0.wave_sweep == 2 || (P0.wave_sweep == 0 && nitems <= 64 * (nphases > 0 ? nphases : 1)) ? 1 : 0;
+    const int lds_max = single_lds_max();
     const size_t dynw = P.dyn ? (size_t)nb * 3 + (size_t)P.nislands * 5 : 0;  // fa, island of body, island state
     const size_t staged = ((size_t)nb * 6 + m + nitems + nphases + 1 + dynw) * sizeof(float);
     // rows in LDS as well when they fit next to the staged arrays
