@@ -20,7 +20,8 @@ Workloads (seeded synthetic scenes, ode_amd/scenes.py):
   large   one world of 200,000 spheres poured into an open box (hash space,
           30 sweeps per step); value = world-steps/s.
   boxes   one world of 1,000 boxes dropped on a plane, stepped through the
-          drop-in ODE API by a C++ frame loop; value = world-steps/s.
+          drop-in ODE API by a C++ frame loop, 300 steps after the warmup;
+          value = world-steps/s.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload towers]
 """
@@ -479,6 +480,8 @@ def main():
         large = args.workload == "large"
         if args.bodies is None:
             args.bodies = 200_000 if large else 1000
+        if not large and "--steps" not in sys.argv:
+            args.steps = 300  # the 1,000-box world's 300 steps (after the warmup)
         if args.large_iters is None:
             args.large_iters = 30 if large else 20
     if args.workload == "large":
