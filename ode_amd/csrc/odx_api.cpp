@@ -24,7 +24,7 @@
 namespace odx {
 void launch_single_step(const DevWorlds &D, const StepParams &P, int nb, int m, int nitems, int nphases,
                         hipStream_t s);
-bool launch_single_groups(const DevWorlds &D, const StepParams &P, size_t lds, hipStream_t s);
+bool launch_single_groups(const DevWorlds &D, const StepParams &P, size_t lds, bool rows_lds, hipStream_t s);
 int single_lds_max();
 void launch_aabbs(const float *gviews, int n, float *aabb, hipStream_t s);
 void launch_space_collide(const SpaceCall &C, hipStream_t s);
@@ -2076,7 +2076,7 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
     int wave_groups = 0;
     // island groups on separate workgroups (k_single_groups): the group of each
     // island and the LDS one such workgroup takes
-    bool group_blocks = false;
+    bool group_blocks = false, group_rows_lds = false;
     std::vector<int> isl_group;
     size_t group_lds = 0;
     const size_t group_lds_max = std::min((size_t)single_lds_max(), (size_t)SL_GROUP_LDS);
@@ -2128,36 +2128,47 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
             // one workgroup per group: runs of consecutive islands (their
             // bodies, joints and rows are consecutive), cut before an island
             // that would give a level of the run more than 64 items (a second
-            // pass of the wave) or take the run's LDS past the budget; more
-            // than SL_GROUP_BLOCKS runs fall back to groups sharing one
-            // workgroup
+            // pass of the wave) or take the run's LDS past the budget.  The
+            // rows go into LDS too when every run fits with them (the chip's
+            // per-workgroup LDS), else the runs are cut for the 64 KB budget
+            // without rows.  More than SL_GROUP_BLOCKS runs fall back to
+            // groups sharing one workgroup.
             std::vector<int> run_start;  // first island of each run, then nisl
             if (group_blocks) {
                 std::vector<long> ibod(std::max(nisl, 1), 0), irow(std::max(nisl, 1), 0);
                 for (int i = 0; i < nb; i++) ibod[island[i]]++;
                 for (int r = 0; r < m; r++) irow[island[rb1[r]]]++;
-                std::vector<int> width;  // items per level of the current run
-                size_t words = 1;        // its LDS words: 6 per body, 2 per row, 1 per level, 1
-                run_start.assign(1, 0);
-                for (int i = 0; i < nisl; i++) {
-                    const size_t wi = (size_t)ibod[i] * 6 + 2 * (size_t)irow[i];
-                    bool fits = (words + wi + std::max(ilc[i].size(), width.size()) - width.size()) *
-                                    sizeof(float) <= group_lds_max;
-                    for (size_t l = 0; fits && l < ilc[i].size(); l++)
-                        fits = (l < width.size() ? width[l] : 0) + ilc[i][l] <= 64;
-                    if (!fits && i > run_start.back()) {
-                        run_start.push_back(i);
-                        width.clear();
-                        words = 1;
+                // LDS words: 6 per body, per row 2 (+ 32 with its record), 1 per level, 1
+                auto cut_runs = [&](size_t row_words, size_t budget) {
+                    std::vector<int> width;  // items per level of the current run
+                    size_t words = 1;
+                    bool ok = true;
+                    run_start.assign(1, 0);
+                    for (int i = 0; i < nisl; i++) {
+                        const size_t wi = (size_t)ibod[i] * 6 + row_words * (size_t)irow[i];
+                        bool fits = (words + wi + std::max(ilc[i].size(), width.size()) - width.size()) *
+                                        sizeof(float) <= budget;
+                        for (size_t l = 0; fits && l < ilc[i].size(); l++)
+                            fits = (l < width.size() ? width[l] : 0) + ilc[i][l] <= 64;
+                        if (!fits && i > run_start.back()) {
+                            run_start.push_back(i);
+                            width.clear();
+                            words = 1;
+                        }
+                        if (width.size() < ilc[i].size()) {
+                            words += ilc[i].size() - width.size();
+                            width.resize(ilc[i].size(), 0);
+                        }
+                        for (size_t l = 0; l < ilc[i].size(); l++) width[l] += ilc[i][l];
+                        words += wi;
+                        if (words * sizeof(float) > budget) ok = false;  // one island alone is over
                     }
-                    if (width.size() < ilc[i].size()) {
-                        words += ilc[i].size() - width.size();
-                        width.resize(ilc[i].size(), 0);
-                    }
-                    for (size_t l = 0; l < ilc[i].size(); l++) width[l] += ilc[i][l];
-                    words += wi;
-                }
-                run_start.push_back(std::max(nisl, 1));
+                    run_start.push_back(std::max(nisl, 1));
+                    return ok && (int)run_start.size() - 1 <= SL_GROUP_BLOCKS;
+                };
+                const size_t rows_budget = std::min((size_t)single_lds_max(), (size_t)SL_GROUP_LDS_ROWS);
+                group_rows_lds = cut_runs(2 + ROW_STRIDE, rows_budget);
+                if (!group_rows_lds) cut_runs(2, group_lds_max);
                 wave_groups = (int)run_start.size() - 1;
                 if (wave_groups > SL_GROUP_BLOCKS) {
                     group_blocks = false;
@@ -2211,15 +2222,19 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
                     std::vector<long> ibod(std::max(nisl, 1), 0), irow(std::max(nisl, 1), 0);
                     for (int i = 0; i < nb; i++) ibod[island[i]]++;
                     for (int r = 0; r < m; r++) irow[island[rb1[r]]]++;
+                    const size_t row_words = group_rows_lds ? 2 + ROW_STRIDE : 2;
                     size_t most = 0;
                     for (int g = 0; g < G; g++) {
                         size_t fl = gc[g].size() + 1;
                         for (int i = run_start[g]; i < run_start[g + 1] && i < nisl; i++)
-                            fl += (size_t)ibod[i] * 6 + 2 * (size_t)irow[i];
+                            fl += (size_t)ibod[i] * 6 + row_words * (size_t)irow[i];
                         most = std::max(most, fl);
                     }
                     group_lds = most * sizeof(float);
-                    if (group_lds > group_lds_max) {
+                    const size_t cap = group_rows_lds
+                                           ? std::min((size_t)single_lds_max(), (size_t)SL_GROUP_LDS_ROWS)
+                                           : group_lds_max;
+                    if (group_lds > cap) {
                         group_blocks = false;
                         wave_groups = SL_STAGED_WAVES;
                         continue;
@@ -2555,7 +2570,7 @@ int dWorldQuickStep(dWorldID w, dReal stepsize) {
             return 0;
     }
     if (g_ktime.on && !odx_check(hipEventRecord(g_ktime.ev[0], g_stream), "hipEventRecord")) return 0;
-    if (!(group_blocks && launch_single_groups(D, P, group_lds, g_stream))) {
+    if (!(group_blocks && launch_single_groups(D, P, group_lds, group_rows_lds, g_stream))) {
         if (group_blocks) {
             odx_message(4, "dWorldQuickStep: the island groups' workgroups could not be launched");
             return 0;

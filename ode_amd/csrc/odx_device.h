@@ -99,6 +99,8 @@ constexpr int SL_STAGED_WAVES = 8;
 // schedule)
 constexpr int SL_GROUP_BLOCKS = 128;
 constexpr int SL_GROUP_LDS = 65536;
+// ... and with the run's rows in LDS as well (capped by the device's limit)
+constexpr int SL_GROUP_LDS_ROWS = 160 * 1024;
 // LDS budgets of the batched solve besides the rows: work items per world
 // (three quarters of the row budget, plus 16) and sweep phases per world.  A
 // world over any budget sweeps from global memory instead.

@@ -2033,15 +2033,16 @@ __device__ __forceinline__ void row_axpy6(float *f, float delta, float4 m0, floa
     f[5] = fmaf(delta, m5, f[5]);
 }
 
-// bbase / rbase: the first body and row whose fc and lambda the arrays hold
-// (a workgroup holding one island group's range); rows and items keep global
-// indices.
+// bbase / rbase: the first body and row whose fc and lambda the arrays hold,
+// and row_base the first row of rows4 (a workgroup holding one island group's
+// range); row records and items keep global indices.
 __device__ __forceinline__ void process_item_l(const float4 *rows4, const float4 *pre, int item, float *lam,
                                                float *s_fc, float *fa = nullptr, const int *isl_b = nullptr,
-                                               const int *ist = nullptr, int bbase = 0, int rbase = 0) {
+                                               const int *ist = nullptr, int bbase = 0, int rbase = 0,
+                                               int row_base = 0) {
 #pragma clang fp contract(fast)
     const int rs = item >> 3, nr = item & 7;
-    const float4 *row4 = rows4 + (size_t)rs * (ROW_STRIDE / 4);
+    const float4 *row4 = rows4 + (size_t)(rs - row_base) * (ROW_STRIDE / 4);
     float4 a[8];
     if (pre) {
 #pragma unroll
@@ -2114,11 +2115,13 @@ __device__ __forceinline__ void process_item_l(const float4 *rows4, const float4
 // and item.  Reads past a level's end are clamped to the arrays and
 // discarded.  A row without body 2 reads body 1's fc in its place, uses zeros,
 // and stores body 2's slot first so that body 1's values win.  The arithmetic
-// is process_item_l's, term for term.  bbase / rbase as for process_item_l.
+// is process_item_l's, term for term.  bbase / rbase / row_base as for
+// process_item_l.
 template <class Sync>
 __device__ __forceinline__ void lean_levels_fn(const float4 *rows4, const int *sched, const int nit, float *lam,
                                                float *s_fc, const int ln, const int stride, const int *ph,
-                                               const int L, Sync sync, const int bbase = 0, const int rbase = 0) {
+                                               const int L, Sync sync, const int bbase = 0, const int rbase = 0,
+                                               const int row_base = 0) {
 #pragma clang fp contract(fast)
     const int last_item = nit - 1;
     // level q spans [ph[q], ph[q + 1]); levels past L - 1 are empty
@@ -2129,7 +2132,7 @@ __device__ __forceinline__ void lean_levels_fn(const float4 *rows4, const int *s
     int i2 = e1 + ln < e2 ? sched[e1 + ln] : -1;
     int i3 = e2 + ln < e3 ? sched[e2 + ln] : -1;
     auto fetch = [&](float4 *X, const int item) {
-        const float4 *rr = rows4 + (size_t)(item >= 0 ? item >> 3 : 0) * (ROW_STRIDE / 4);
+        const float4 *rr = rows4 + (size_t)(item >= 0 ? (item >> 3) - row_base : 0) * (ROW_STRIDE / 4);
 #pragma unroll
         for (int k = 0; k < 8; k++) X[k] = rr[k];
     };
@@ -2186,7 +2189,7 @@ __device__ __forceinline__ void lean_levels_fn(const float4 *rows4, const int *s
             for (int k = 0; k < 6; k++) s_fc[b1 * 6 + k] = f1[k];
         }
         for (int k = s + ln + stride; k < e; k += stride)
-            process_item_l(rows4, nullptr, sched[k], lam, s_fc, nullptr, nullptr, nullptr, bbase, rbase);
+            process_item_l(rows4, nullptr, sched[k], lam, s_fc, nullptr, nullptr, nullptr, bbase, rbase, row_base);
         sync();
         const int e4v = p + 4 < L ? e4 : e3;
         s = e;
@@ -2501,8 +2504,10 @@ __global__ __launch_bounds__(NT) void k_single_step_t(DevWorlds D, StepParams P)
 // a run of consecutive islands, so it owns the consecutive bodies
 // [grp_body_start[g], grp_body_start[g + 1]), joints [grp_joint_start[g], ..)
 // and their rows; the phase table opens with G + 1 offsets of the groups'
-// level lists, whose items are consecutive in the schedule.  LDS: fc of the
-// group's bodies and lambda of its rows, then its items and level starts.
+// level lists, whose items are consecutive in the schedule.  LDS: (ROWS: the
+// run's row records,) fc of the group's bodies and lambda of its rows, then
+// its items and level starts.
+template <bool ROWS>
 __global__ __launch_bounds__(64) void k_single_groups(DevWorlds D, StepParams P) {
     extern __shared__ float4 s_dyn4[];
     const int g = blockIdx.x, t = threadIdx.x, G = P.wave_groups;
@@ -2513,7 +2518,8 @@ __global__ __launch_bounds__(64) void k_single_groups(DevWorlds D, StepParams P)
     const int b0 = D.grp_body_start[g], b1 = D.grp_body_start[g + 1];
     const int j0 = D.grp_joint_start[g], j1 = D.grp_joint_start[g + 1];
     const int r0 = j0 < nj ? D.joff[j0] : m, r1 = j1 < nj ? D.joff[j1] : m;
-    float *s_fc = reinterpret_cast<float *>(s_dyn4);
+    float4 *s_rows = s_dyn4;
+    float *s_fc = reinterpret_cast<float *>(s_dyn4 + (ROWS ? (size_t)(r1 - r0) * (ROW_STRIDE / 4) : 0));
     float *lam = s_fc + (size_t)(b1 - b0) * 6;
     int *s_sched = reinterpret_cast<int *>(lam + (r1 - r0));
     int *s_ph = s_sched + ni;
@@ -2531,8 +2537,13 @@ __global__ __launch_bounds__(64) void k_single_groups(DevWorlds D, StepParams P)
                          0, 0, D.joff[j]);
     // the rows this workgroup wrote are read back by it below
     __syncthreads();
+    if (ROWS) {
+        const float4 *g4 = reinterpret_cast<const float4 *>(D.rows) + (size_t)r0 * (ROW_STRIDE / 4);
+        for (int k = t; k < (r1 - r0) * (ROW_STRIDE / 4); k += 64) s_rows[k] = g4[k];
+        __syncthreads();
+    }
     if (Lg > 0 && ni > 0) {
-        const float4 *rows4 = reinterpret_cast<const float4 *>(D.rows);
+        const float4 *rows4 = ROWS ? s_rows : reinterpret_cast<const float4 *>(D.rows);
         for (int it = 0; it < P.max_iters; it++)
             lean_levels_fn(
                 rows4, s_sched, ni, lam, s_fc, t, 64, s_ph, Lg,
@@ -2541,7 +2552,7 @@ __global__ __launch_bounds__(64) void k_single_groups(DevWorlds D, StepParams P)
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 },
-                b0, r0);
+                b0, r0, ROWS ? r0 : 0);
     }
     __syncthreads();
     for (int r = r0 + t; r < r1; r += 64) D.lambda[r] = lam[r - r0];
@@ -2648,16 +2659,20 @@ int single_lds_max() {
     return lds_max;
 }
 
-// k_single_groups: P.wave_groups workgroups; lds = bytes of fc, lambda and
-// the largest group's items and level starts.  False if the LDS does not fit.
-bool launch_single_groups(const DevWorlds &D, const StepParams &P, size_t lds, hipStream_t s) {
+// k_single_groups: P.wave_groups workgroups; lds = bytes of the largest
+// group's (rows,) fc, lambda, items and level starts.  False if the LDS does
+// not fit.
+bool launch_single_groups(const DevWorlds &D, const StepParams &P, size_t lds, bool rows_lds, hipStream_t s) {
     if (lds > (size_t)single_lds_max()) return false;
-    if (lds > 65536 && hipFuncSetAttribute((const void *)k_single_groups, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)lds) != hipSuccess) {
+    const void *fn = rows_lds ? (const void *)k_single_groups<true> : (const void *)k_single_groups<false>;
+    if (lds > 65536 && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
         (void)hipGetLastError();
         return false;
     }
-    hipLaunchKernelGGL(k_single_groups, dim3(P.wave_groups), dim3(64), lds, s, D, P);
+    if (rows_lds)
+        hipLaunchKernelGGL(k_single_groups<true>, dim3(P.wave_groups), dim3(64), lds, s, D, P);
+    else
+        hipLaunchKernelGGL(k_single_groups<false>, dim3(P.wave_groups), dim3(64), lds, s, D, P);
     return true;
 }
 
