@@ -66,6 +66,26 @@ __device__ void sp_bitonic(unsigned long long *k, int P) {
         }
 }
 
+// Up to this many keys a rank sort beats the bitonic network (n^2 / 1024
+// comparisons per thread, no barrier per stage).
+constexpr int SP_RANK_MAX = 4096;
+
+// Ranks of the n distinct keys k[0, n): r[p] = #{q : k[q] < k[p]}, the
+// position of key p in ascending order.  Every lane of a wave reads the same
+// k[q] at once (an LDS broadcast).
+__device__ void sp_rank(const unsigned long long *k, int n, int *r) {
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+        const unsigned long long kp = k[p];
+        int c = 0;
+        int q = 0;
+        for (; q + 4 <= n; q += 4)
+            c += (k[q] < kp ? 1 : 0) + (k[q + 1] < kp ? 1 : 0) + (k[q + 2] < kp ? 1 : 0) + (k[q + 3] < kp ? 1 : 0);
+        for (; q < n; q++) c += k[q] < kp ? 1 : 0;
+        r[p] = c;
+    }
+    __syncthreads();
+}
+
 // Exclusive scan of a[0, rows) in place; a[rows] = total, which is returned.
 __device__ int sp_scan(int *a, int rows, int *tmp) {
     const int t = threadIdx.x, T = blockDim.x;
@@ -104,6 +124,27 @@ __device__ int sap_row(const SpaceCall &C, const unsigned short *sorted, int k, 
             sp_filters(C, a, b)) {
             if (out) { out[2 * c] = a; out[2 * c + 1] = b; }
             c++;
+        }
+    }
+    return c;
+}
+
+// sap_row over AABBs staged in LDS in sorted order (sab[6 q .. 6 q + 5] is
+// the AABB of the geom at sorted position q).
+__device__ int sap_row_lds(const SpaceCall &C, const unsigned short *sorted, const float *sab, int k, int *out) {
+    const int a = sorted[k];
+    const float *A = sab + (size_t)k * 6;
+    const float x0max = A[C.ax0 + 1], x1min = A[C.ax1], x1max = A[C.ax1 + 1], x2min = A[C.ax2], x2max = A[C.ax2 + 1];
+    int c = 0;
+    for (int q = k + 1; q < C.nsort; q++) {
+        const float *B = sab + (size_t)q * 6;
+        if (!(B[C.ax0] <= x0max)) break;
+        if (x1max >= B[C.ax1] && B[C.ax1 + 1] >= x1min && x2max >= B[C.ax2] && B[C.ax2 + 1] >= x2min) {
+            const int b = sorted[q];
+            if (sp_filters(C, a, b)) {
+                if (out) { out[2 * c] = a; out[2 * c + 1] = b; }
+                c++;
+            }
         }
     }
     return c;
@@ -208,13 +249,24 @@ __global__ __launch_bounds__(SP_THREADS) void k_space_collide(SpaceCall C) {
                 s_key[p] = k;
             }
             __syncthreads();
-            sp_bitonic(s_key, P);
-            for (int q = t; q < ns; q += T) {
-                const unsigned long long k = s_key[q];
-                const unsigned tie = (unsigned)k;
-                const int p = ((unsigned)(k >> 32) & 0x80000000u) ? (int)tie : ns - 1 - (int)tie;
-                s_sorted[q] = s_init[p];
-                C.ranks_out[q] = s_init[p];
+            if (ns <= SP_RANK_MAX) {
+                // the keys are distinct (the tie field): key p goes to its rank
+                sp_rank(s_key, ns, s_cnt);
+                for (int p = t; p < ns; p += T) {
+                    const int q = s_cnt[p];
+                    s_sorted[q] = s_init[p];
+                    C.ranks_out[q] = s_init[p];
+                }
+                __syncthreads();
+            } else {
+                sp_bitonic(s_key, P);
+                for (int q = t; q < ns; q += T) {
+                    const unsigned long long k = s_key[q];
+                    const unsigned tie = (unsigned)k;
+                    const int p = ((unsigned)(k >> 32) & 0x80000000u) ? (int)tie : ns - 1 - (int)tie;
+                    s_sorted[q] = s_init[p];
+                    C.ranks_out[q] = s_init[p];
+                }
             }
             if (t == 0) C.info[1] = 1;
         }
@@ -230,8 +282,25 @@ __global__ __launch_bounds__(SP_THREADS) void k_space_collide(SpaceCall C) {
             s_key[p] = k;
         }
         __syncthreads();
-        sp_bitonic(s_key, P);
-        for (int q = t; q < s_nh; q += T) s_sorted[q] = (unsigned short)(s_key[q] & 0xffffu);
+        if (n <= SP_RANK_MAX) {
+            // hashed geoms' keys are distinct (the geom index); the others are
+            // all ones and rank after them
+            sp_rank(s_key, n, s_cnt);
+            for (int p = t; p < n; p += T)
+                if (s_key[p] != ~0ull) s_sorted[s_cnt[p]] = (unsigned short)p;
+            __syncthreads();
+        } else {
+            sp_bitonic(s_key, P);
+            for (int q = t; q < s_nh; q += T) s_sorted[q] = (unsigned short)(s_key[q] & 0xffffu);
+        }
+    }
+    // sweep and prune: the AABBs in sorted order into LDS (over the sort keys,
+    // no longer needed) when they fit
+    float *s_ab = reinterpret_cast<float *>(s_key);
+    const bool lds_ab = C.kind == 0 && rows * 6 <= SP_MAX * 2;
+    if (lds_ab) {
+        __syncthreads();
+        for (int q = t; q < rows * 6; q += T) s_ab[q] = C.aabb[(size_t)s_sorted[q / 6] * 6 + q % 6];
     }
     for (int i = t; i <= rows; i += T) s_cnt[i] = 0;
     __syncthreads();
@@ -239,7 +308,8 @@ __global__ __launch_bounds__(SP_THREADS) void k_space_collide(SpaceCall C) {
     // count the pairs of every row
     const int nh = s_nh;
     if (C.kind == 0) {
-        for (int k = t; k < rows; k += T) s_cnt[k] = sap_row(C, s_sorted, k, nullptr);
+        for (int k = t; k < rows; k += T)
+            s_cnt[k] = lds_ab ? sap_row_lds(C, s_sorted, s_ab, k, nullptr) : sap_row(C, s_sorted, k, nullptr);
     } else if (C.kind == 2) {
         for (int i = t; i < rows; i += T) s_cnt[i] = simple_row(C, i, nullptr);
     } else {
@@ -264,7 +334,11 @@ __global__ __launch_bounds__(SP_THREADS) void k_space_collide(SpaceCall C) {
 
     // write them at their offsets
     if (C.kind == 0) {
-        for (int k = t; k < rows; k += T) sap_row(C, s_sorted, k, C.pairs + 2 * (size_t)s_cnt[k]);
+        for (int k = t; k < rows; k += T) {
+            int *o = C.pairs + 2 * (size_t)s_cnt[k];
+            if (lds_ab) sap_row_lds(C, s_sorted, s_ab, k, o);
+            else sap_row(C, s_sorted, k, o);
+        }
     } else if (C.kind == 2) {
         for (int i = t; i < rows; i += T) simple_row(C, i, C.pairs + 2 * (size_t)s_cnt[i]);
     } else {
