@@ -351,10 +351,11 @@ void odxWorldSetRowOrder(dWorldID w, int order);
  * groups when the reference row order runs a fixed sweep count over at least
  * 32 islands with rows; else one wave when levels hold at most 64 items on
  * average; default), 1 = the whole workgroup per level, 2 = one wave, 3 =
- * island groups: the islands split among the workgroup's waves, each wave
- * sweeping its own islands' levels (reference row order with a fixed sweep
- * count; otherwise as 0).  The Gauss-Seidel order and the results are the same
- * in every mode. */
+ * island groups: runs of consecutive islands on workgroups of their own (one
+ * wave each, on separate CUs), or among the waves of one workgroup when the
+ * runs do not fit, each sweeping its own islands' levels (reference row order
+ * with a fixed sweep count; otherwise as 0).  The Gauss-Seidel order and the
+ * results are the same in every mode. */
 void odxWorldSetSweepMode(dWorldID w, int mode);
 
 /* Writes the world in the Dynamics Interchange Format (export-dif.h:33,
