@@ -18,7 +18,8 @@ Workloads (seeded synthetic scenes, ode_amd/scenes.py):
           end) with 8 free boxes, 40 sweeps per step (--tol 0 --max-iters 40
           unless given), cold PGS.
   large   one world of 200,000 spheres poured into an open box (hash space,
-          30 sweeps per step); value = world-steps/s.
+          30 sweeps per step, each contact's rows together in one pass over
+          the colours: --row-order 2); value = world-steps/s.
   boxes   one world of 1,000 boxes dropped on a plane, stepped through the
           drop-in ODE API by a C++ frame loop, 300 steps after the warmup;
           value = world-steps/s.
@@ -272,7 +273,7 @@ def bench_large(args):
     import oracle as O
     bodies = args.bodies
     world = S.sphere_pour(bodies, seed=args.seed)
-    prm = odx.params_vector(iters=args.large_iters, w=args.sor_w, mu=0.3, space=1)
+    prm = odx.params_vector(iters=args.large_iters, w=args.sor_w, mu=0.3, space=1, row_order=args.row_order)
     L = odx.Large(world, prm)
     L.set_solve_blocks(args.solve_blocks)
     t0 = time.perf_counter()
@@ -292,6 +293,7 @@ def bench_large(args):
     c2 = L.counts()
     km = np.mean(np.array(ks), axis=0)
     rows = c2["rows"]
+    colour_sizes = np.bincount(L.colours(), minlength=c2["colours"]).tolist()
     solve_gbs = rows * args.large_iters * ROW_SWEEP_BYTES / (km[2] * 1e-3) / 1e9
     ms = total_ms / args.steps
     out = {
@@ -309,11 +311,12 @@ def bench_large(args):
         "data": "synthetic",
         "config": {"workload": "large", "bodies": bodies, "geoms": bodies + 5, "space": "hash",
                    "sweeps": args.large_iters, "mu": 0.3, "erp": 0.2, "cfm": 1e-5, "h": 0.01, "sor_w": args.sor_w,
-                   "seed": args.seed, "pour_steps": args.pour + args.warmup},
+                   "seed": args.seed, "pour_steps": args.pour + args.warmup, "row_order": args.row_order},
         "contacts_per_step": cnt["contacts"],
         "contacts_per_s": cnt["contacts"] / (ms * 1e-3),
         "counts": cnt,
         "kernel_ms": {"prepare": float(km[0]), "colour": float(km[1]), "solve": float(km[2])},
+        "colour_sizes": colour_sizes,
         "pour_s": pour_s,
         "roofline": {
             "bound": "hbm", "achieved": solve_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -324,15 +327,19 @@ def bench_large(args):
     tpath = os.path.join(ROOT, "profiles", "k_lg_solve_large%d.json" % bodies)
     if os.path.exists(tpath):
         t = json.load(open(tpath))
-        out["roofline"]["traffic"] = t["traffic_bytes_per_launch"]
-        out["roofline"]["traffic_source"] = os.path.relpath(tpath, ROOT)
+        if t.get("row_order", 0) == args.row_order:
+            out["roofline"]["traffic"] = t["traffic_bytes_per_launch"]
+            out["roofline"]["traffic_source"] = os.path.relpath(tpath, ROOT)
+        else:
+            out["roofline"]["traffic_note"] = ("%s holds row order %d's counters" %
+                                               (os.path.relpath(tpath, ROOT), t.get("row_order", 0)))
     if not args.no_cpu:
         # the oracle's QuickStep (rows, 30 sweeps, integration) over the GPU's
         # contacts of the current state; broadphase and narrowphase are left out
         # (the oracle's hash pairs are all-pairs), so the CPU rate is optimistic
         b = L.bodies().astype(np.float64)
         j = L.joints().astype(np.float64)
-        p64 = O.params(space=1, mu=0.3, iters=args.large_iters, w=args.sor_w)
+        p64 = O.params(space=1, mu=0.3, iters=args.large_iters, w=args.sor_w, row_order=args.row_order)
         t0 = time.perf_counter()
         O.quickstep(b, j, p64, dtype=np.float32)
         dt = time.perf_counter() - t0
@@ -457,8 +464,10 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="finally copy MIB MiB device to device (a reference kernel of known traffic for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
-    ap.add_argument("--row-order", type=int, default=0,
-                    help="boxes workload: 0 the reference's row order, 1 colour phases")
+    ap.add_argument("--row-order", type=int, default=None,
+                    help="boxes workload: 0 the reference's row order (default), 1 colour phases; large "
+                         "workload: 0 ReorderPrep's order over the colours (two passes per sweep), 2 each "
+                         "contact's rows together (one pass per sweep, the default)")
     ap.add_argument("--sweep-mode", type=int, default=0,
                     help="boxes workload: 0 automatic, 1 workgroup-wide levels, 2 one wave (odxWorldSetSweepMode)")
     ap.add_argument("--bodies", type=int, default=None, help="bodies of the large (200,000) or boxes (1,000) workload")
@@ -470,6 +479,8 @@ def main():
                     help="sweeps per step of the large (30) or boxes (20) workload")
     args = ap.parse_args()
 
+    if args.row_order is None:
+        args.row_order = 2 if args.workload == "large" else 0
     if args.sor_w is None:
         args.sor_w = 1.0 if args.workload == "towers" else 1.3
     if args.gpus != 1:

@@ -43,7 +43,8 @@
  *               15 soft_erp, 16 soft_cfm, 17 max contacts per pair, 18 space
  *               (0 sweep-and-prune, 1 hash), 19 motion1, 20 motion2, 21 motionN,
  *               22 slip1, 23 slip2, 24 skip pairs joined by a ball joint,
- *               25 residual floor
+ *               25 residual floor, 31 row order of odxLarge's sweeps (0
+ *               ReorderPrep's, 2 each contact's rows together)
  */
 #ifndef ODX_H
 #define ODX_H
@@ -228,9 +229,13 @@ int odxApiGetKernelTiming(double *stats);
  * (dJointCreateContact ode.cpp:1192-1200 with dJointAttach's reversal), and
  * dWorldQuickStep (quickstep.cpp) with the rows swept colour by colour: a
  * colour is a set of contacts with disjoint bodies, so a phase runs in
- * parallel and a sweep is a Gauss-Seidel pass in colour order (normal rows
- * first, then friction rows, as ReorderPrep quickstep.cpp:2329-2355).
- * Contacts only (no ball joints).  bodies [nb][32], geoms [ng][16], params as
+ * parallel and a sweep is a Gauss-Seidel pass in colour order.  params[31]
+ * picks the rows' order within a sweep: 0, as ReorderPrep
+ * (quickstep.cpp:2329-2355), every contact's normal row colour by colour,
+ * then its friction rows colour by colour (two passes over the colours); 2,
+ * each contact's rows together in joint order (one pass: half the colour
+ * phases, a different Gauss-Seidel order of the same LCP).  Random reordering
+ * (1) is not supported.  Contacts only (no ball joints).  bodies [nb][32], geoms [ng][16], params as
  * odxBatchCreate (params[7] = sweeps per step).  pair_cap / contact_cap bound
  * the candidate pairs and contacts of one step (<= 0: 12 and 8 per geom).
  * Returns NULL on failure. */

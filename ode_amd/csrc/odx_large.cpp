@@ -24,7 +24,7 @@ bool lg_enqueue_solve(const DevWorlds &D, const StepParams &P, const LargeDev &L
                       hipStream_t s);
 void lg_enqueue_fold(const LargeDev &L, int *sticky, hipStream_t s);
 int lg_colour_blocks();
-int lg_solve_blocks(int per_cu);
+int lg_solve_blocks(int per_cu, int one_pass);
 }  // namespace odx
 
 using namespace odx;
@@ -117,6 +117,11 @@ odxLarge *odxLargeCreate(int nb, const float *bodies, int ng, const float *geoms
         odx_message(2, "odxLargeCreate: capacities %d pairs / %d contacts too large", pair_cap, contact_cap);
         return nullptr;
     }
+    const int row_order = (int)params[31];
+    if (row_order != 0 && row_order != 2) {
+        odx_message(2, "odxLargeCreate: row order %d (0 ReorderPrep's, 2 each contact's rows together)", row_order);
+        return nullptr;
+    }
     auto *W = new odxLarge();
     W->nb = nb;
     W->ng = ng;
@@ -144,6 +149,7 @@ odxLarge *odxLargeCreate(int nb, const float *bodies, int ng, const float *geoms
     while (hsize < 2 * ng) hsize <<= 1;
     L.hmask = hsize - 1;
     L.spin_limit = (int)LG_SPIN_LIMIT;
+    L.one_pass = row_order == 2;
     cell = cell > 0.f ? cell * 1.001f : 1.f;  // slack keeps floor() of overlapping centres within one cell
     L.inv_cell = 1.f / cell;
     L.pair_cap = pair_cap;
@@ -175,7 +181,7 @@ odxLarge *odxLargeCreate(int nb, const float *bodies, int ng, const float *geoms
     for (auto &e : W->ev) ok = ok && odx_check(hipEventCreate(&e), "hipEventCreate");
     if (ok) {
         W->colour_blocks = lg_colour_blocks();
-        W->solve_blocks = lg_solve_blocks(1);
+        W->solve_blocks = lg_solve_blocks(1, L.one_pass);
         if (W->colour_blocks <= 0 || W->solve_blocks <= 0) {
             odx_message(3, "odxLargeCreate: no cooperative launch configuration");
             ok = false;
@@ -198,7 +204,7 @@ int odxLargeSetIterations(odxLarge *w, int iters) {
 
 int odxLargeSetSolveBlocks(odxLarge *w, int per_cu) {
     if (!w || per_cu < 1 || per_cu > 8) return 0;
-    const int b = lg_solve_blocks(per_cu);
+    const int b = lg_solve_blocks(per_cu, w->L.one_pass);
     if (b <= 0) return 0;
     w->solve_blocks = b;
     return b;

@@ -3214,7 +3214,7 @@ __global__ __launch_bounds__(LG_BLOCK) void k_lg_colour(DevWorlds D, LargeDev L)
         const int r = L.row_cnt[k], n = L.row_cnt[k + 1] - r;
         const int2 bb = L.cbody[k];
         L.corder[pos] = k;
-        L.items[pos] = make_int4(r, lg_pass_a(D, r, n) | (n << 8), bb.x, bb.y);
+        L.items[pos] = make_int4(r, (L.one_pass ? n : lg_pass_a(D, r, n)) | (n << 8), bb.x, bb.y);
     }
     ok = lg_grid_sync(L, gen);   // every block has read col_start[] counts
     if (!ok) return;
@@ -3277,13 +3277,13 @@ __device__ __forceinline__ float lg_row_update(const float4 *a, float old, float
 // findex points at a row of the same contact: inside [r, r + n) the value
 // updated here is used, else the contact's first-pass row (fixed in this
 // pass).  lambda of a contact is touched only by the thread owning its slot.
-constexpr int LG_PRE = 2;
-struct LgSlot {
+template <int PRE> struct LgSlot {
     int r, n, b1, b2;
-    float4 pre[LG_PRE][8];
+    float4 pre[PRE][8];
 };
 
-__device__ __forceinline__ void lg_slot_load(const DevWorlds &D, const int4 it, int pass, LgSlot &S) {
+template <int LG_PRE>
+__device__ __forceinline__ void lg_slot_load(const DevWorlds &D, const int4 it, int pass, LgSlot<LG_PRE> &S) {
     const int na = it.y & 255, n = it.y >> 8;
     S.r = pass == 0 ? it.x : it.x + na;
     S.n = pass == 0 ? na : n - na;
@@ -3300,7 +3300,8 @@ __device__ __forceinline__ void lg_slot_load(const DevWorlds &D, const int4 it, 
     }
 }
 
-__device__ __forceinline__ void lg_slot_run(const DevWorlds &D, __amdgpu_buffer_rsrc_t rs, const LgSlot &S) {
+template <int LG_PRE>
+__device__ __forceinline__ void lg_slot_run(const DevWorlds &D, __amdgpu_buffer_rsrc_t rs, const LgSlot<LG_PRE> &S) {
     if (S.n <= 0) return;
     const bool two = S.b2 >= 0;
     float f1[6], f2[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -3347,13 +3348,17 @@ __device__ __forceinline__ void lg_slot_run(const DevWorlds &D, __amdgpu_buffer_
 // whose grid barriers would cost as much as a full colour's.
 constexpr int LG_TAIL = 256;
 
-// The solve (cooperative): lambda = 0, `iters` sweeps, each the contacts'
-// first-pass rows colour by colour, then their findex rows colour by colour,
-// then Stage4b/6 and dxStepBody of every body.  The thread at a slot of the
-// colour order handles that contact in every phase (its lambda is touched by
-// no other thread); each thread loads its slot of the next phase (item and
-// rows: read-only here) before the barrier.  Tail colours (LG_TAIL) map their
-// slots to the first workgroup's threads, in the sweeps and at the start.
+// The solve (cooperative): lambda = 0, `iters` sweeps, then Stage4b/6 and
+// dxStepBody of every body.  A sweep is the contacts' first-pass rows colour
+// by colour, then their findex rows colour by colour (two passes); with
+// L.one_pass every contact's rows are one pass, colour by colour, each
+// contact's rows in joint order.  The thread at a slot of the colour order
+// handles that contact in every phase (its lambda is touched by no other
+// thread); each thread loads its slot of the next phase (item and rows:
+// read-only here) before the barrier.  Tail colours (LG_TAIL) map their slots
+// to the first workgroup's threads, in the sweeps and at the start.  PRE rows
+// per slot are preloaded (the rows of one pass of a contact with friction).
+template <int PRE>
 __global__ __launch_bounds__(LG_BLOCK) void k_lg_solve(DevWorlds D, StepParams P, LargeDev L, int iters) {
     __shared__ int s_cs[LG_MAXCOL + 1];
     // a colouring that gave up on a barrier leaves no schedule: the step
@@ -3381,28 +3386,29 @@ __global__ __launch_bounds__(LG_BLOCK) void k_lg_solve(DevWorlds D, StepParams P
     }
     bool ok = lg_grid_sync(L, gen);
     const int nph = ct + (ct < ncol ? 1 : 0);  // grid phases per pass
-    const int T = (nc > 0 && ncol > 0) ? iters * 2 * nph : 0;
+    const int npass = L.one_pass ? 1 : 2;
+    const int T = (nc > 0 && ncol > 0) ? iters * npass * nph : 0;
     int c = 0, pass = 0;
     int idx = s_cs[0] + gt, e = ct > 0 ? s_cs[1] : 0;
-    LgSlot cur;
+    LgSlot<PRE> cur;
     lg_slot_load(D, idx < e && T > 0 ? L.items[idx] : make_int4(-1, 0, 0, 0), 0, cur);
     for (int p = 0; ok && p < T; p++) {
-        const int c2 = (p + 1) % nph, pass2 = ((p + 1) / nph) & 1;
+        const int c2 = (p + 1) % nph, pass2 = ((p + 1) / nph) % npass;
         const bool grid2 = p + 1 < T && c2 < ct;
         const int idx2 = grid2 ? s_cs[c2] + gt : 0, e2 = grid2 ? s_cs[c2 + 1] : 0;
-        LgSlot nxt;
+        LgSlot<PRE> nxt;
         lg_slot_load(D, idx2 < e2 ? L.items[idx2] : make_int4(-1, 0, 0, 0), pass2, nxt);
         if (c < ct) {
             lg_slot_run(D, rs, cur);
             for (int j = idx + gs; j < e; j += gs) {   // slots beyond one per thread
-                LgSlot x;
+                LgSlot<PRE> x;
                 lg_slot_load(D, L.items[j], pass, x);
                 lg_slot_run(D, rs, x);
             }
         } else if (blockIdx.x == 0) {
             for (int tc = ct; tc < ncol; tc++) {
                 for (int j = s_cs[tc] + (int)threadIdx.x; j < s_cs[tc + 1]; j += blockDim.x) {
-                    LgSlot x;
+                    LgSlot<PRE> x;
                     lg_slot_load(D, L.items[j], pass, x);
                     lg_slot_run(D, rs, x);
                 }
@@ -3441,7 +3447,10 @@ int lg_coop_blocks(const void *kernel, int per_cu) {
     return cus * (per < per_cu ? per : per_cu);
 }
 int lg_colour_blocks() { return lg_coop_blocks((const void *)k_lg_colour, 2); }
-int lg_solve_blocks(int per_cu) { return lg_coop_blocks((const void *)k_lg_solve, per_cu); }
+static const void *lg_solve_kernel(int one_pass) {
+    return one_pass ? (const void *)k_lg_solve<3> : (const void *)k_lg_solve<2>;
+}
+int lg_solve_blocks(int per_cu, int one_pass) { return lg_coop_blocks(lg_solve_kernel(one_pass), per_cu); }
 
 // The prepare stage of a step: bodies, broadphase, narrowphase, rows.
 // Capacities: ng <= pair scan cap etc. are checked by the host.
@@ -3486,7 +3495,7 @@ bool lg_enqueue_solve(const DevWorlds &D, const StepParams &P, const LargeDev &L
     LargeDev l = L;
     int n = iters;
     void *args[] = {(void *)&d, (void *)&p, (void *)&l, (void *)&n};
-    return hipLaunchCooperativeKernel((const void *)k_lg_solve, dim3(blocks), dim3(LG_BLOCK), args, 0, s) ==
+    return hipLaunchCooperativeKernel(lg_solve_kernel(L.one_pass), dim3(blocks), dim3(LG_BLOCK), args, 0, s) ==
            hipSuccess;
 }
 }  // namespace odx

@@ -108,8 +108,10 @@ def _check(ok, what):
 
 def params_vector(h=0.01, gravity=(0, 0, -9.81), erp=0.2, cfm=1e-5, w=1.3, iters=20, max_vel=np.inf, min_depth=0.0,
                   mode=0x7000, mu=0.5, mu2=0.0, bounce=0.0, bounce_vel=0.0, soft_erp=0.0, soft_cfm=0.0, maxc=4,
-                  space=0, skip_connected=0, lambda_floor=1e-3):
-    """PARAMS vector of include/odx.h (dWorldCreate defaults for dSINGLE)."""
+                  space=0, skip_connected=0, lambda_floor=1e-3, row_order=0):
+    """PARAMS vector of include/odx.h (dWorldCreate defaults for dSINGLE).
+    row_order (slot 31) is the large world's sweep order: 0 ReorderPrep's,
+    2 each contact's rows together."""
     p = np.zeros(32, np.float64)
     p[0] = h
     p[1:4] = gravity
@@ -118,6 +120,7 @@ def params_vector(h=0.01, gravity=(0, 0, -9.81), erp=0.2, cfm=1e-5, w=1.3, iters
     p[10], p[11], p[12], p[13], p[14], p[15], p[16] = mode, mu, mu2, bounce, bounce_vel, soft_erp, soft_cfm
     p[17], p[18] = maxc, space
     p[24], p[25] = skip_connected, lambda_floor
+    p[31] = row_order
     return p
 
 

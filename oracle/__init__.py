@@ -76,6 +76,7 @@ def params(**kw):
         "tol": P_TOL, "check_every": P_CHECK_EVERY,
         "dyn_premature": P_DYN_PREMATURE, "dyn_extra_factor": P_DYN_EXTRA_FACTOR, "dyn_extra_delta": P_DYN_EXTRA_DELTA,
         "random_reorder": P_RANDOM_REORDER,
+        "row_order": P_RANDOM_REORDER,   # 0 ReorderPrep, 1 random reordering, 2 joint order
     }
     for k, v in kw.items():
         if k == "gravity":

@@ -1474,6 +1474,7 @@ template <class R> struct Params {
     unsigned max_extra;
     R premature_delta, extra_delta;
     bool random_reorder;  /* REORDERING_METHOD__RANDOMLY, frequency 8 (quickstep.cpp:54-85) */
+    bool joint_order;     /* rows swept in joint order, each joint's rows together (no ReorderPrep split) */
 };
 
 /* The near-callback policy as a table (what a program's callback decides
@@ -1537,7 +1538,8 @@ template <class R> Params<R> read_params(const double *P) {
     p.premature_delta = (R)(P[ORC_P_DYN_PREMATURE] > 0 ? P[ORC_P_DYN_PREMATURE] : 0.0);
     p.extra_delta = (R)(P[ORC_P_DYN_EXTRA_DELTA] > 0 ? P[ORC_P_DYN_EXTRA_DELTA] : 0.0);
     p.dyn = p.max_extra != 0 || p.premature_delta != 0;
-    p.random_reorder = P[ORC_P_RANDOM_REORDER] != 0;
+    p.random_reorder = P[ORC_P_RANDOM_REORDER] == 1;
+    p.joint_order = P[ORC_P_RANDOM_REORDER] == 2;
     return p;
 }
 
@@ -1706,8 +1708,12 @@ template <class R> void assemble(int nb, R *bodies, std::vector<Joint<R>> &joint
     }
     /* Stage4LCP_ReorderPrep :2329-2355 */
     S.order.clear();
-    for (int i = 0; i < m; i++) if (S.findex[i] == -1) S.order.push_back(i);
-    for (int i = 0; i < m; i++) if (S.findex[i] != -1) S.order.push_back(i);
+    if (p.joint_order) {
+        for (int i = 0; i < m; i++) S.order.push_back(i);
+    } else {
+        for (int i = 0; i < m; i++) if (S.findex[i] == -1) S.order.push_back(i);
+        for (int i = 0; i < m; i++) if (S.findex[i] != -1) S.order.push_back(i);
+    }
 }
 
 /* dxQuickStepIsland_Stage4LCP_IterationStep :2917-3033.  fa, when given,

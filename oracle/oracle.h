@@ -49,7 +49,10 @@ enum {
      * premature delta is non-zero (all zero: a fixed iteration count). */
     ORC_P_DYN_PREMATURE, ORC_P_DYN_EXTRA_FACTOR, ORC_P_DYN_EXTRA_DELTA,
     /* 1: random constraint reordering every 8 iterations (quickstep.cpp:72,
-     * the reference's default build), drawing from the oracle's dRand state */
+     * the reference's default build), drawing from the oracle's dRand state;
+     * 2: no ReorderPrep split, the rows swept in joint order (each joint's
+     * rows together: the order of the large-world solve's one-pass colour
+     * phases); 0: ReorderPrep (rows without findex first) */
     ORC_P_RANDOM_REORDER,
     ORC_P__MAX = 32
 };

@@ -2,11 +2,13 @@
 
 The hash-space pair set and the contacts are compared with the oracle's
 (dxHashSpace semantics, dCollide).  The solve sweeps contacts colour by
-colour, normal rows first and friction rows after; giving the oracle's
-QuickStep the contacts in the GPU's colour order makes its ReorderPrep order
-(non-findex rows, then findex rows, each in joint order) the same
-Gauss-Seidel order, so impulses and body states after the step must match the
-float64 oracle within the stated tolerance.
+colour.  In row order 0 the normal rows of all colours come first and the
+friction rows after; giving the oracle's QuickStep the contacts in the GPU's
+colour order makes its ReorderPrep order (non-findex rows, then findex rows,
+each in joint order) the same Gauss-Seidel order.  In row order 2 each
+contact's rows are swept together, which is the oracle's joint order over the
+same colour-ordered contacts.  Either way impulses and body states after the
+step must match the float64 oracle within the stated tolerance.
 """
 import sys
 
@@ -46,11 +48,11 @@ def colour_order(colours):
     return np.argsort(colours, kind="stable")
 
 
-@pytest.mark.parametrize("seed", [0, 1])
-def test_pairs_contacts_and_coloured_step_match_oracle(seed):
-    say("case: large mixed pile seed", seed)
+@pytest.mark.parametrize("seed,row_order", [(0, 0), (1, 0), (0, 2), (1, 2)])
+def test_pairs_contacts_and_coloured_step_match_oracle(seed, row_order):
+    say("case: large mixed pile seed", seed, "row order", row_order)
     w = S.mixed_pile(1500, seed=60 + seed)
-    prm = O.params(space=1, w=1.0)
+    prm = O.params(space=1, w=1.0, row_order=row_order)
     L = make(w, prm)
     L.step(1)
     L.sync()
@@ -174,12 +176,13 @@ def contact_rows(surface):
     return 1 + 2 * int(mu > 0)
 
 
-def test_surface_table_and_category_bits_match_oracle():
+@pytest.mark.parametrize("row_order", [0, 2])
+def test_surface_table_and_category_bits_match_oracle(row_order):
     # the near-callback policy on the large world: the category / collide bit
     # test removes hash-space pairs, each geom's material picks the pair's
     # surface and contact limit; pairs, contacts, surface indices and the
-    # coloured step match the oracle's
-    say("case: large mixed pile with a material-pair surface table")
+    # coloured step match the oracle's (contacts of 1, 2 and 3 rows)
+    say("case: large mixed pile with a material-pair surface table, row order", row_order)
     rng = np.random.default_rng(91)
     w = S.mixed_pile(1500, seed=62)
     body = w.geoms[:, 1].astype(np.int32)
@@ -199,7 +202,7 @@ def test_surface_table_and_category_bits_match_oracle():
         (1, 2): {"mode": approx | 0x008, "mu": 0.5, "soft_erp": 0.5, "max_contacts": 3},
         (2, 2): {"mode": 0x001 | approx, "mu": 0.4, "mu2": 0.1},
     })
-    prm = O.params(space=1, w=1.0)
+    prm = O.params(space=1, w=1.0, row_order=row_order)
     L = make(w, prm)
     L.set_surface_table(t, category_filter=True)
     L.step(1)
@@ -285,10 +288,11 @@ def test_rows_match_oracle_contact_rows():
     L.close()
 
 
-def test_step_is_deterministic():
-    say("case: large determinism")
+@pytest.mark.parametrize("row_order", [0, 2])
+def test_step_is_deterministic(row_order):
+    say("case: large determinism, row order", row_order)
     w = S.mixed_pile(800, seed=71)
-    prm = O.params(space=1, iters=10)
+    prm = O.params(space=1, iters=10, row_order=row_order)
     out = []
     for _ in range(2):
         L = make(w, prm)
@@ -300,12 +304,13 @@ def test_step_is_deterministic():
         np.testing.assert_array_equal(a, b)
 
 
-def test_more_sweeps_lower_the_lcp_residual():
-    say("case: large residual")
+@pytest.mark.parametrize("row_order", [0, 2])
+def test_more_sweeps_lower_the_lcp_residual(row_order):
+    say("case: large residual, row order", row_order)
     w = S.mixed_pile(600, seed=72)
     res = []
     for iters in (5, 60):
-        prm = O.params(space=1, iters=iters, w=1.0)
+        prm = O.params(space=1, iters=iters, w=1.0, row_order=row_order)
         L = make(w, prm)
         L.step(1)
         L.sync()

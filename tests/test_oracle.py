@@ -809,6 +809,72 @@ def test_sor_iterates_of_a_sliding_sphere_clamp_friction_to_the_normal():
     assert abs(abs(want[0][1]) - mu * want[0][0]) < 1e-12 or abs(abs(want[0][2]) - mu * want[0][0]) < 1e-12
 
 
+def test_sor_iterates_of_two_stacked_sliding_spheres_in_both_row_orders():
+    # sphere A on the plane, sphere B on A, both sliding, mu = 0.5: two
+    # contacts of three rows each (normal, t1, t2), coupled through A.  Row
+    # order 0 sweeps ReorderPrep's order (quickstep.cpp:2329-2355: the normal
+    # rows of both contacts, then the four friction rows); row order 2 sweeps
+    # the rows in joint order (n1 t1 t2 of contact 1, then of contact 2).  In
+    # both, a friction row's bound is +-mu times its own contact's normal
+    # impulse at the time the row is updated.
+    mA, mB, rA, rB, dA, dB, mu = 3.0, 1.5, 0.2, 0.15, 0.01, 0.004, 0.5
+    h, erp, cfm, w = 0.01, 0.2, 1e-5, 1.3
+    zA = rA - dA
+    zB = zA + rA + rB - dB
+    IA, IB = 0.4 * mA * rA * rA, 0.4 * mB * rB * rB
+    vA, vB = np.array([-0.2, 0.1, 0.0]), np.array([0.4, -0.3, 0.0])
+    bodies = np.array([body((0, 0, zA), mass=mA, I=(IA,) * 3, lvel=vA),
+                       body((0, 0, zB), mass=mB, I=(IB,) * 3, lvel=vB)])
+    joints = np.zeros((2, 16))
+    joints[0, 1:3] = (0, -1)
+    joints[0, 3:6], joints[0, 6:9], joints[0, 9] = (0, 0, -dA), (0, 0, 1), dA
+    joints[1, 1:3] = (1, 0)
+    # contact 2 sits off A's axis (x = s): its normal row turns A, which
+    # couples it to contact 1's t2 row through A's inverse inertia
+    s = 0.05
+    joints[1, 3:6], joints[1, 6:9], joints[1, 9] = (s, 0, zA + rA), (0, 0, 1), dB
+    n, t1, t2 = np.array([0, 0, 1.0]), np.array([0, -1.0, 0]), np.array([1.0, 0, 0])   # dPlaneSpace(+z)
+    rA1 = np.array([0, 0, -rA])                 # contact 1 from A
+    rB2, rA2 = np.array([s, 0, zA + rA - zB]), np.array([s, 0, rA])   # contact 2 from B (body 1) and A (body 2)
+    J = np.zeros((6, 12))                        # columns: A (v, w), B (v, w)
+    for k, t in enumerate((n, t1, t2)):
+        J[k, 0:6] = np.concatenate([t, np.cross(rA1, t)])
+        J[3 + k, 6:12] = np.concatenate([t, np.cross(rB2, t)])
+        J[3 + k, 0:6] = -np.concatenate([t, np.cross(rA2, t)])
+    Minv = np.diag([1 / mA] * 3 + [1 / IA] * 3 + [1 / mB] * 3 + [1 / IB] * 3)
+    A = J @ Minv @ J.T
+    vel = np.concatenate([vA, np.zeros(3), vB, np.zeros(3)])
+    acc = np.array([0, 0, -9.81, 0, 0, 0, 0, 0, -9.81, 0, 0, 0])
+    b = -(J @ (vel / h + acc))
+    b[0] += erp * dA / h ** 2
+    b[3] += erp * dB / h ** 2
+    C = np.full(6, cfm / h)
+    normal = [0, 0, 0, 3, 3, 3]
+    lo = lambda i, x: 0.0 if i in (0, 3) else -mu * x[normal[i]]
+    hi = lambda i, x: np.inf if i in (0, 3) else mu * x[normal[i]]
+
+    def sweeps(order, k):
+        x = np.zeros(6)
+        for _ in range(k):
+            for i in order:
+                d = w * (b[i] - A[i] @ x - C[i] * x[i]) / (A[i, i] + C[i])
+                x[i] = min(max(x[i] + d, lo(i, x)), hi(i, x))
+        return x
+
+    got = {}
+    for row_order, seq in ((0, [0, 3, 1, 2, 4, 5]), (2, [0, 1, 2, 3, 4, 5])):
+        for k in (1, 2, 5, 20):
+            prm = O.params(iters=k, w=w, mu=mu, erp=erp, cfm=cfm, row_order=row_order)
+            _, lam = O.quickstep(bodies, joints, prm, dtype=np.float64)
+            np.testing.assert_allclose(lam, sweeps(seq, k), rtol=1e-11, atol=1e-11)
+            got[row_order, k] = lam
+    # the two orders are different Gauss-Seidel sweeps: contact 1's friction
+    # sees contact 2's normal impulse of the same sweep only in ReorderPrep's
+    # order
+    assert abs(A[2, 3]) > 1e-3
+    assert np.max(np.abs(got[0, 1] - got[2, 1])) > 1e-3
+
+
 # ------------------------------------------------------------ cylinder-plane
 def test_cylinder_aabb_closed_form():
     g = S.geom(S.CYLINDER, 0, (0.3, 1.2))
