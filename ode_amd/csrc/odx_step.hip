@@ -3118,6 +3118,8 @@ __device__ __forceinline__ unsigned long long lg_key(int k, int stamp) {
            (unsigned long long)(k & 0xffffff);
 }
 
+constexpr int LG_CCACHE = 8;   // contacts per thread the colouring keeps in registers
+
 // Colouring (cooperative): colour c is a maximal set of contacts with
 // disjoint dynamic bodies, built by claim/commit rounds (Luby): every
 // uncoloured contact whose bodies no contact of colour c holds claims both
@@ -3134,13 +3136,38 @@ __global__ __launch_bounds__(LG_BLOCK) void k_lg_colour(DevWorlds D, LargeDev L)
     for (int k = gt; k < nc; k += gs) L.colour[k] = -1;
     for (int b = gt; b < L.nb; b += gs) { st_dev(&L.taken[b], -1); st_dev(&L.owner[b], 0ull); }
     bool ok = lg_grid_sync(L, gen);
+    // the thread's first LG_CCACHE contacts (k = gt + i * gs) stay in
+    // registers: their bodies, and a bit per contact still uncoloured
+    int2 cb[LG_CCACHE];
+    unsigned live = 0;
+#pragma unroll
+    for (int i = 0; i < LG_CCACHE; i++) {
+        const int k = gt + i * gs;
+        cb[i] = make_int2(0, -1);
+        if (k < nc) {
+            cb[i] = L.cbody[k];
+            live |= 1u << i;
+        }
+    }
+    const int kfar = gt + LG_CCACHE * gs;   // contacts past the cache take the loads
     int ncol = 0, rounds = 0, left = nc;
     for (int c = 0; ok && left > 0 && c < LG_MAXCOL; c++) {
         int joined = 0;   // contacts of colour c after its previous rounds
         for (int sub = 0; ok && sub < LG_MAXSUB; sub++) {
             const int stamp = c * LG_MAXSUB + sub;
             int e = 0;
-            for (int k = gt; k < nc; k += gs) {
+#pragma unroll
+            for (int i = 0; i < LG_CCACHE; i++) {
+                if (!((live >> i) & 1u)) continue;
+                const int k = gt + i * gs;
+                const int2 bb = cb[i];
+                if (ld_dev(&L.taken[bb.x]) == c || (bb.y >= 0 && ld_dev(&L.taken[bb.y]) == c)) continue;
+                const unsigned long long key = lg_key(k, stamp);
+                atomicMax(&L.owner[bb.x], key);
+                if (bb.y >= 0) atomicMax(&L.owner[bb.y], key);
+                e++;
+            }
+            for (int k = kfar; k < nc; k += gs) {
                 if (L.colour[k] >= 0) continue;
                 const int2 bb = L.cbody[k];
                 if (ld_dev(&L.taken[bb.x]) == c || (bb.y >= 0 && ld_dev(&L.taken[bb.y]) == c)) continue;
@@ -3157,7 +3184,20 @@ __global__ __launch_bounds__(LG_BLOCK) void k_lg_colour(DevWorlds D, LargeDev L)
             rounds++;
             if (elig == 0) break;
             int won = 0;
-            for (int k = gt; k < nc; k += gs) {
+#pragma unroll
+            for (int i = 0; i < LG_CCACHE; i++) {
+                if (!((live >> i) & 1u)) continue;
+                const int k = gt + i * gs;
+                const int2 bb = cb[i];
+                const unsigned long long key = lg_key(k, stamp);
+                if (ld_dev(&L.owner[bb.x]) != key || (bb.y >= 0 && ld_dev(&L.owner[bb.y]) != key)) continue;
+                L.colour[k] = c;
+                live &= ~(1u << i);
+                st_dev(&L.taken[bb.x], c);
+                if (bb.y >= 0) st_dev(&L.taken[bb.y], c);
+                won++;
+            }
+            for (int k = kfar; k < nc; k += gs) {
                 if (L.colour[k] >= 0) continue;
                 const int2 bb = L.cbody[k];
                 const unsigned long long key = lg_key(k, stamp);
