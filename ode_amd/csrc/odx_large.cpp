@@ -162,7 +162,8 @@ odxLarge *odxLargeCreate(int nb, const float *bodies, int ng, const float *geoms
          (big.empty() || odx_check(hipMemcpy(bg, big.data(), big.size() * sizeof(int), hipMemcpyHostToDevice), "H2D"));
     L.big = bg;
     ok = ok && W->alloc(&L.aabb, (size_t)ng * 6) && W->alloc(&L.gcell, ng) && W->alloc(&L.bucket, hsize + 1) &&
-         W->alloc(&L.bucket_geom, ng) && W->alloc(&L.pair_cnt, (size_t)ng + 1) && W->alloc(&L.pairs, pair_cap) &&
+         W->alloc(&L.bucket_geom, ng) && W->alloc(&L.slot_cell, ng) && W->alloc(&L.slot_box, (size_t)ng * 2) &&
+         W->alloc(&L.pair_cnt, (size_t)ng + 1) && W->alloc(&L.pairs, pair_cap) &&
          W->alloc(&L.pair_keep, (size_t)ng * LG_PAIR_KEEP) &&
          W->alloc(&L.contact_cnt, (size_t)pair_cap + 1) && W->alloc(&L.row_cnt, (size_t)contact_cap + 1) &&
          W->alloc(&L.counters, LG_NCOUNTERS) && W->alloc(&L.cbody, contact_cap) && W->alloc(&L.colour, contact_cap) && W->alloc(&L.taken, nb) &&

@@ -38,6 +38,8 @@ struct LargeDev {
     int4 *gcell;                      // [ng] cell x, y, z and hash bucket of each finite geom
     int *bucket;                      // [hmask + 2] counts -> starts -> ends
     int *bucket_geom;                 // [ng] finite geoms grouped by bucket, ascending index in a bucket
+    int4 *slot_cell;                  // [ng] per bucket slot: the geom's cell x, y, z and the geom
+    float4 *slot_box;                 // [2 * ng] per bucket slot: the geom's AABB, then its body (as float)
     int *pair_cnt;                    // [ng + 1] pairs per geom -> first pair of each geom
     int2 *pairs;                      // [pair_cap] (lower geom, higher geom)
     int *pair_keep;                   // [ng * LG_PAIR_KEEP] partners of each geom found by the counting pass

@@ -2846,8 +2846,10 @@ __global__ void k_lg_scatter(LargeDev L) {
     }
 }
 
-// geoms of a bucket in index order (deterministic pair order)
-__global__ void k_lg_sort_buckets(LargeDev L) {
+// geoms of a bucket in index order (deterministic pair order), then each
+// slot's copy of its geom's cell, AABB and body, so that the pair search
+// reads a candidate with two loads that do not wait for its geom index
+__global__ void k_lg_sort_buckets(DevWorlds D, LargeDev L) {
     for (int h = blockIdx.x * blockDim.x + threadIdx.x; h <= L.hmask; h += gridDim.x * blockDim.x) {
         const int s = h ? L.bucket[h - 1] : 0, e = L.bucket[h];
         for (int i = s + 1; i < e; i++) {
@@ -2855,6 +2857,14 @@ __global__ void k_lg_sort_buckets(LargeDev L) {
             int j = i - 1;
             while (j >= s && L.bucket_geom[j] > v) { L.bucket_geom[j + 1] = L.bucket_geom[j]; j--; }
             L.bucket_geom[j + 1] = v;
+        }
+        for (int k = s; k < e; k++) {
+            const int g = L.bucket_geom[k];
+            const int4 c = L.gcell[g];
+            const float *a = L.aabb + (size_t)g * 6;
+            L.slot_cell[k] = make_int4(c.x, c.y, c.z, g);
+            L.slot_box[2 * (size_t)k] = make_float4(a[0], a[1], a[2], a[3]);
+            L.slot_box[2 * (size_t)k + 1] = make_float4(a[4], a[5], D.geom0[g].y, 0.f);
         }
     }
 }
@@ -2905,25 +2915,34 @@ __global__ void k_lg_pairs(DevWorlds D, LargeDev L, int write, int cat_filter) {
         int n = 0;
         if (!ibig) {
             const int4 c = L.gcell[i];
-            for (int dz = -1; dz <= 1; dz++)
-                for (int dy = -1; dy <= 1; dy++)
-                    for (int dx = -1; dx <= 1; dx++) {
-                        const int x = c.x + dx, y = c.y + dy, z = c.z + dz;
-                        const int h = lg_hash(x, y, z, L.hmask);
-                        const int s = h ? L.bucket[h - 1] : 0, e = L.bucket[h];
-                        for (int k = s; k < e; k++) {
-                            const int j = L.bucket_geom[k];
-                            if (j <= i) continue;
-                            const int4 cj = L.gcell[j];
-                            if (cj.x != x || cj.y != y || cj.z != z) continue;
-                            if (!lg_bodies_ok(bi, (int)D.geom0[j].y)) continue;
-                            if (cat_filter && !lg_bits_ok(D, i, j)) continue;
-                            if (!lg_overlap(ai, L.aabb + (size_t)j * 6)) continue;
-                            if (write && off + n < L.pair_cap) L.pairs[off + n] = make_int2(i, j);
-                            if (!write && n < LG_PAIR_KEEP) keep[n] = j;
-                            n++;
-                        }
-                    }
+            // the 27 cells' bucket ranges first (z outer, x inner), loaded together
+            int bs[27], be[27];
+#pragma unroll
+            for (int t = 0; t < 27; t++) {
+                const int h = lg_hash(c.x + t % 3 - 1, c.y + (t / 3) % 3 - 1, c.z + t / 9 - 1, L.hmask);
+                bs[t] = h ? L.bucket[h - 1] : 0;
+                be[t] = L.bucket[h];
+            }
+            float a[6];
+            for (int q = 0; q < 6; q++) a[q] = ai[q];
+#pragma unroll
+            for (int t = 0; t < 27; t++) {
+                const int x = c.x + t % 3 - 1, y = c.y + (t / 3) % 3 - 1, z = c.z + t / 9 - 1;
+                for (int k = bs[t]; k < be[t]; k++) {
+                    const int4 sc = L.slot_cell[k];
+                    const float4 b0 = L.slot_box[2 * (size_t)k], b1 = L.slot_box[2 * (size_t)k + 1];
+                    const int j = sc.w;
+                    if (j <= i) continue;
+                    if (sc.x != x || sc.y != y || sc.z != z) continue;
+                    if (!lg_bodies_ok(bi, (int)b1.z)) continue;
+                    if (cat_filter && !lg_bits_ok(D, i, j)) continue;
+                    const float bj[6] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y};
+                    if (!lg_overlap(a, bj)) continue;
+                    if (write && off + n < L.pair_cap) L.pairs[off + n] = make_int2(i, j);
+                    if (!write && n < LG_PAIR_KEEP) keep[n] = j;
+                    n++;
+                }
+            }
         }
         for (int q = 0; q < L.nbig; q++) {
             const int j = L.big[q];
@@ -3543,7 +3562,7 @@ void lg_enqueue_prepare(const DevWorlds &D, const StepParams &P, const LargeDev 
     hipLaunchKernelGGL(k_lg_aabb, dim3(G), dim3(T), 0, s, D, L);
     lg_scan(L.bucket, L.hmask + 2, nullptr, L.scan_tmp, s);
     hipLaunchKernelGGL(k_lg_scatter, dim3(G), dim3(T), 0, s, L);
-    hipLaunchKernelGGL(k_lg_sort_buckets, dim3(G), dim3(T), 0, s, L);
+    hipLaunchKernelGGL(k_lg_sort_buckets, dim3(G), dim3(T), 0, s, D, L);
     hipLaunchKernelGGL(k_lg_pairs, dim3(G), dim3(T), 0, s, D, L, 0, P.cat_filter);
     lg_scan(L.pair_cnt, L.ng + 1, nullptr, L.scan_tmp, s);
     hipLaunchKernelGGL(k_lg_pairs, dim3(G), dim3(T), 0, s, D, L, 1, P.cat_filter);
