@@ -294,6 +294,8 @@ def bench_large(args):
     km = np.mean(np.array(ks), axis=0)
     rows = c2["rows"]
     colour_sizes = np.bincount(L.colours(), minlength=c2["colours"]).tolist()
+    import zlib
+    state_crc = zlib.crc32(np.ascontiguousarray(L.bodies()).tobytes())   # the state after every step run
     solve_gbs = rows * args.large_iters * ROW_SWEEP_BYTES / (km[2] * 1e-3) / 1e9
     ms = total_ms / args.steps
     out = {
@@ -317,6 +319,7 @@ def bench_large(args):
         "counts": cnt,
         "kernel_ms": {"prepare": float(km[0]), "colour": float(km[1]), "solve": float(km[2])},
         "colour_sizes": colour_sizes,
+        "state_crc32": state_crc,
         "pour_s": pour_s,
         "roofline": {
             "bound": "hbm", "achieved": solve_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -620,9 +623,16 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "k_solve_small_%s4096_cold.json" % args.workload)
     if args.worlds == 4096 and os.path.exists(tpath):
         t = json.load(open(tpath))
-        if t.get("launches_per_step") == RR["launches_per_step"]:
+        tl = t.get("launches_per_step")
+        if tl == RR["launches_per_step"]:
             out["roofline"]["traffic"] = t["traffic_bytes_per_launch"]
             out["roofline"]["traffic_source"] = os.path.relpath(tpath, ROOT)
+        elif tl:
+            # the profile's launches each cover 4096 / tl worlds: the bytes
+            # scale with the worlds a launch covers
+            out["roofline"]["traffic"] = t["traffic_bytes_per_launch"] * tl / RR["launches_per_step"]
+            out["roofline"]["traffic_source"] = ("%s (measured at %d launches per step, scaled by worlds per launch)"
+                                                 % (os.path.relpath(tpath, ROOT), tl))
     out["fp64_residual"] = R.get("fp64_residual")
     if "cold" in results and head != "cold":
         out["speedup_vs_cold"] = R["world_steps_per_s"] / results["cold"]["world_steps_per_s"]

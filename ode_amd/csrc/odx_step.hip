@@ -3341,9 +3341,6 @@ template <int PRE> struct LgSlot {
     float4 pre[PRE][8];
 };
 
-// Preloads the slot's rows.  The loads are unconditional (a slot with fewer
-// rows, or none, reads row 0, which always exists) so that a phase issues the
-// same loads on every path and the wait for a later load need not cover them.
 template <int LG_PRE>
 __device__ __forceinline__ void lg_slot_load(const DevWorlds &D, const int4 it, int pass, LgSlot<LG_PRE> &S) {
     const int na = it.y & 255, n = it.y >> 8;
@@ -3354,56 +3351,42 @@ __device__ __forceinline__ void lg_slot_load(const DevWorlds &D, const int4 it, 
     if (it.x < 0) S.n = 0;
 #pragma unroll
     for (int q = 0; q < LG_PRE; q++) {
-        const int row = q < S.n ? S.r + q : 0;
-        const float4 *rr = reinterpret_cast<const float4 *>(D.rows + (size_t)row * ROW_STRIDE);
+        if (q < S.n) {
+            const float4 *rr = reinterpret_cast<const float4 *>(D.rows + (size_t)(S.r + q) * ROW_STRIDE);
 #pragma unroll
-        for (int k = 0; k < 8; k++) S.pre[q][k] = rr[k];
-    }
-}
-
-// A slot's accumulators and impulses between lg_slot_begin (their loads) and
-// lg_slot_finish (row updates, stores).  The loads are unconditional (body 0
-// and lambda 0 stand in where the slot has none), as in lg_slot_load.  With
-// OUTER a row's findex may point outside the slot (the findex rows of a
-// second pass); without it every findex row is one of the slot's own.
-template <int PRE> struct LgAcc {
-    float f1[6], f2[6], lam[PRE], lf[PRE];
-};
-
-template <int LG_PRE, bool OUTER>
-__device__ __forceinline__ void lg_slot_begin(const DevWorlds &D, __amdgpu_buffer_rsrc_t rs, const LgSlot<LG_PRE> &S,
-                                              LgAcc<LG_PRE> &A) {
-    const int b1 = S.b1 >= 0 ? S.b1 : 0, b2 = S.b2 >= 0 ? S.b2 : b1;
-    lg_fc_load(rs, b1, A.f1);
-    lg_fc_load(rs, b2, A.f2);
-#pragma unroll
-    for (int q = 0; q < LG_PRE; q++) {
-        A.lam[q] = D.lambda[q < S.n ? S.r + q : 0];
-        A.lf[q] = 0.f;
-        if (OUTER && q < S.n) {
-            const int fidx = as_int(S.pre[q][7].z);
-            if (fidx >= 0 && (fidx < S.r || fidx >= S.r + S.n)) A.lf[q] = D.lambda[fidx];
+            for (int k = 0; k < 8; k++) S.pre[q][k] = rr[k];
         }
     }
 }
 
 template <int LG_PRE>
-__device__ __forceinline__ void lg_slot_finish(const DevWorlds &D, __amdgpu_buffer_rsrc_t rs, const LgSlot<LG_PRE> &S,
-                                               LgAcc<LG_PRE> &A) {
+__device__ __forceinline__ void lg_slot_run(const DevWorlds &D, __amdgpu_buffer_rsrc_t rs, const LgSlot<LG_PRE> &S) {
     if (S.n <= 0) return;
     const bool two = S.b2 >= 0;
-    if (!two)
-        for (int k = 0; k < 6; k++) A.f2[k] = 0.f;
+    float f1[6], f2[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    lg_fc_load(rs, S.b1, f1);
+    if (two) lg_fc_load(rs, S.b2, f2);
+    float lam[LG_PRE], lf[LG_PRE];
+#pragma unroll
+    for (int q = 0; q < LG_PRE; q++) {
+        lam[q] = 0.f;
+        lf[q] = 0.f;
+        if (q < S.n) {
+            lam[q] = D.lambda[S.r + q];
+            const int fidx = as_int(S.pre[q][7].z);
+            if (fidx >= 0 && (fidx < S.r || fidx >= S.r + S.n)) lf[q] = D.lambda[fidx];
+        }
+    }
 #pragma unroll
     for (int q = 0; q < LG_PRE; q++) {
         if (q < S.n) {
             const int fidx = as_int(S.pre[q][7].z);
-            float l = A.lf[q];
+            float l = lf[q];
 #pragma unroll
             for (int u = 0; u < q; u++)
-                if (fidx == S.r + u) l = A.lam[u];
-            A.lam[q] = lg_row_update(S.pre[q], A.lam[q], l, A.f1, A.f2, two);
-            D.lambda[S.r + q] = A.lam[q];
+                if (fidx == S.r + u) l = lam[u];
+            lam[q] = lg_row_update(S.pre[q], lam[q], l, f1, f2, two);
+            D.lambda[S.r + q] = lam[q];
         }
     }
     for (int q = LG_PRE; q < S.n; q++) {   // rows beyond the preloaded ones
@@ -3412,17 +3395,10 @@ __device__ __forceinline__ void lg_slot_finish(const DevWorlds &D, __amdgpu_buff
         for (int k = 0; k < 8; k++) a[k] = rr[k];
         const float old = D.lambda[S.r + q];
         const float l = D.lambda[max(as_int(a[7].z), 0)];
-        D.lambda[S.r + q] = lg_row_update(a, old, l, A.f1, A.f2, two);
+        D.lambda[S.r + q] = lg_row_update(a, old, l, f1, f2, two);
     }
-    lg_fc_store(rs, S.b1, A.f1);
-    if (two) lg_fc_store(rs, S.b2, A.f2);
-}
-
-template <int LG_PRE, bool OUTER>
-__device__ __forceinline__ void lg_slot_run(const DevWorlds &D, __amdgpu_buffer_rsrc_t rs, const LgSlot<LG_PRE> &S) {
-    LgAcc<LG_PRE> A;
-    lg_slot_begin<LG_PRE, OUTER>(D, rs, S, A);
-    lg_slot_finish(D, rs, S, A);
+    lg_fc_store(rs, S.b1, f1);
+    if (two) lg_fc_store(rs, S.b2, f2);
 }
 
 // Colours of at most this many contacts at the end of the colour order run on
@@ -3471,53 +3447,36 @@ __global__ __launch_bounds__(LG_BLOCK) void k_lg_solve(DevWorlds D, StepParams P
     const int nph = ct + (ct < ncol ? 1 : 0);  // grid phases per pass
     const int npass = L.one_pass ? 1 : 2;
     const int T = (nc > 0 && ncol > 0) ? iters * npass * nph : 0;
-    constexpr bool OUTER = PRE < 3;   // the two-pass form reads findex rows of the other pass
-    const int4 none = make_int4(-1, 0, 0, 0);
-    const int4 *items = L.items;
     int c = 0, pass = 0;
     int idx = s_cs[0] + gt, e = ct > 0 ? s_cs[1] : 0;
     LgSlot<PRE> cur;
-    lg_slot_load(D, idx < e && T > 0 ? items[idx] : none, 0, cur);
-    int4 itn = none;   // the next phase's item, loaded a phase ahead of its rows
-    {
-        const int c1 = 1 % nph, i1 = s_cs[c1] + gt;
-        if (1 < T && c1 < ct && i1 < s_cs[c1 + 1]) itn = items[i1];
-    }
+    lg_slot_load(D, idx < e && T > 0 ? L.items[idx] : make_int4(-1, 0, 0, 0), 0, cur);
     for (int p = 0; ok && p < T; p++) {
         const int c2 = (p + 1) % nph, pass2 = ((p + 1) / nph) % npass;
         const bool grid2 = p + 1 < T && c2 < ct;
         const int idx2 = grid2 ? s_cs[c2] + gt : 0, e2 = grid2 ? s_cs[c2 + 1] : 0;
-        // this phase's accumulator loads first, then the next phase's rows and
-        // the item after it: the row updates wait for the accumulators only
-        LgAcc<PRE> A;
-        lg_slot_begin<PRE, OUTER>(D, rs, cur, A);
         LgSlot<PRE> nxt;
-        lg_slot_load(D, itn, pass2, nxt);
-        // the item of phase p + 2, loaded unconditionally (slot 0 stands in)
-        const int c3 = (p + 2) % nph, i3 = s_cs[c3] + gt;
-        const bool has3 = p + 2 < T && c3 < ct && i3 < s_cs[c3 + 1];
-        const int4 it3 = items[has3 ? i3 : 0];
-        const int4 itnn = make_int4(has3 ? it3.x : -1, has3 ? it3.y : 0, has3 ? it3.z : 0, has3 ? it3.w : 0);
+        lg_slot_load(D, idx2 < e2 ? L.items[idx2] : make_int4(-1, 0, 0, 0), pass2, nxt);
         if (c < ct) {
-            lg_slot_finish(D, rs, cur, A);
+            lg_slot_run(D, rs, cur);
             for (int j = idx + gs; j < e; j += gs) {   // slots beyond one per thread
                 LgSlot<PRE> x;
                 lg_slot_load(D, L.items[j], pass, x);
-                lg_slot_run<PRE, OUTER>(D, rs, x);
+                lg_slot_run(D, rs, x);
             }
         } else if (blockIdx.x == 0) {
             for (int tc = ct; tc < ncol; tc++) {
                 for (int j = s_cs[tc] + (int)threadIdx.x; j < s_cs[tc + 1]; j += blockDim.x) {
                     LgSlot<PRE> x;
                     lg_slot_load(D, L.items[j], pass, x);
-                    lg_slot_run<PRE, OUTER>(D, rs, x);
+                    lg_slot_run(D, rs, x);
                 }
                 mem_done();
                 __syncthreads();
             }
         }
         ok = lg_grid_sync(L, gen);
-        c = c2; pass = pass2; idx = idx2; e = e2; cur = nxt; itn = itnn;
+        c = c2; pass = pass2; idx = idx2; e = e2; cur = nxt;
     }
     if (!ok) return;
     for (int b = gt; b < L.nb; b += gs) {

@@ -386,3 +386,24 @@ def test_sphere_pour_steps_and_settles():
     assert np.all(np.abs(b[:, 0:2]) < 5.0 + 0.05)
     say("counts", cnt)
     L.close()
+
+
+def test_long_pour_is_deterministic():
+    # 200k spheres poured for 900 steps, twice: the settled pile has colours of
+    # more than one contact per thread of the solve's grid, and a race in the
+    # solve would show as a different state after this many steps
+    say("case: 200k sphere pour, 900 steps, twice")
+    w = S.sphere_pour(200000, seed=5)
+    prm = O.params(space=1, mu=0.3, iters=30, w=1.3, row_order=2)
+    out = []
+    for _ in range(2):
+        L = make(w, prm)
+        for _ in range(9):
+            L.step(100)
+            L.sync()
+        cnt = L.counts()
+        out.append((L.bodies(), cnt))
+        say("  counts", cnt, "largest colour", int(np.bincount(L.colours()).max()))
+        L.close()
+    assert out[0][1] == out[1][1]
+    np.testing.assert_array_equal(out[0][0], out[1][0])
