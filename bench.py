@@ -31,6 +31,7 @@ import json
 import os
 import sys
 import time
+import zlib
 
 import numpy as np
 
@@ -294,7 +295,6 @@ def bench_large(args):
     km = np.mean(np.array(ks), axis=0)
     rows = c2["rows"]
     colour_sizes = np.bincount(L.colours(), minlength=c2["colours"]).tolist()
-    import zlib
     state_crc = zlib.crc32(np.ascontiguousarray(L.bodies()).tobytes())   # the state after every step run
     solve_gbs = rows * args.large_iters * ROW_SWEEP_BYTES / (km[2] * 1e-3) / 1e9
     ms = total_ms / args.steps
