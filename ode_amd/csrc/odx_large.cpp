@@ -183,6 +183,7 @@ odxLarge *odxLargeCreate(int nb, const float *bodies, int ng, const float *geoms
     if (ok) {
         W->colour_blocks = lg_colour_blocks();
         W->solve_blocks = lg_solve_blocks(1, L.one_pass);
+        L.colour_cap = W->solve_blocks * LG_BLOCK;
         if (W->colour_blocks <= 0 || W->solve_blocks <= 0) {
             odx_message(3, "odxLargeCreate: no cooperative launch configuration");
             ok = false;
@@ -208,6 +209,7 @@ int odxLargeSetSolveBlocks(odxLarge *w, int per_cu) {
     const int b = lg_solve_blocks(per_cu, w->L.one_pass);
     if (b <= 0) return 0;
     w->solve_blocks = b;
+    w->L.colour_cap = b * LG_BLOCK;
     return b;
 }
 

@@ -59,6 +59,7 @@ struct LargeDev {
     float4 *ftot;                     // [2 * nb] total force, then total torque, of the body stage
     int *scan_tmp;                    // block sums of the scans
     int spin_limit;                   // polls of one barrier wait before it fails (LG_SPIN_LIMIT)
+    int colour_cap;                   // contacts per colour at most (the solve's threads; 0: no cap)
     int one_pass;                     // 1: a contact's rows together in one colour phase per sweep (joint
                                       // order); 0: ReorderPrep's two passes (rows without findex, then findex rows)
 };

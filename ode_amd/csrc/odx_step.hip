@@ -3116,6 +3116,27 @@ __device__ int lg_block_sum(int v, int *s_red) {
     return s;
 }
 
+// exclusive prefix of v over the block in thread order, and the block total
+__device__ void lg_block_scan(int v, int *s_red, int &pre, int &tot) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(x, o, 64);
+        if (lane >= o) x += t;
+    }
+    __syncthreads();
+    if (lane == 63) s_red[wv] = x;
+    __syncthreads();
+    int base = 0, all = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
+        if (i < wv) base += s_red[i];
+        all += s_red[i];
+    }
+    pre = base + x - v;
+    tot = all;
+}
+
 // Rows of a contact in the first pass of ReorderPrep (quickstep.cpp:2329-2355:
 // rows without findex first, findex rows after), by the batched path's rule:
 // n_a leading rows without findex followed only by findex rows split there;
@@ -3169,15 +3190,23 @@ __global__ __launch_bounds__(LG_BLOCK) void k_lg_colour(DevWorlds D, LargeDev L)
         }
     }
     const int kfar = gt + LG_CCACHE * gs;   // contacts past the cache take the loads
+    // a colour takes at most L.colour_cap contacts (one per thread of the
+    // solve's grid): each workgroup admits at most `quota` of its winners per
+    // colour, in thread order, so the cap is deterministic
+    const int quota = L.colour_cap > 0 ? max(1, L.colour_cap / (int)gridDim.x) : 0x7fffffff;
     int ncol = 0, rounds = 0, left = nc;
     for (int c = 0; ok && left > 0 && c < LG_MAXCOL; c++) {
         int joined = 0;   // contacts of colour c after its previous rounds
+        int used = 0;     // this workgroup's contacts in colour c
         for (int sub = 0; ok && sub < LG_MAXSUB; sub++) {
             const int stamp = c * LG_MAXSUB + sub;
             int e = 0;
+            // a workgroup whose quota is full makes no claims for colour c
+            // (its winners could not join, and their claims would block others)
+            const unsigned claim = used < quota ? live : 0u;
 #pragma unroll
             for (int i = 0; i < LG_CCACHE; i++) {
-                if (!((live >> i) & 1u)) continue;
+                if (!((claim >> i) & 1u)) continue;
                 const int k = gt + i * gs;
                 const int2 bb = cb[i];
                 if (ld_dev(&L.taken[bb.x]) == c || (bb.y >= 0 && ld_dev(&L.taken[bb.y]) == c)) continue;
@@ -3203,6 +3232,7 @@ __global__ __launch_bounds__(LG_BLOCK) void k_lg_colour(DevWorlds D, LargeDev L)
             rounds++;
             if (elig == 0) break;
             int won = 0;
+            unsigned winm = 0;   // cached contacts holding both claims
 #pragma unroll
             for (int i = 0; i < LG_CCACHE; i++) {
                 if (!((live >> i) & 1u)) continue;
@@ -3210,12 +3240,24 @@ __global__ __launch_bounds__(LG_BLOCK) void k_lg_colour(DevWorlds D, LargeDev L)
                 const int2 bb = cb[i];
                 const unsigned long long key = lg_key(k, stamp);
                 if (ld_dev(&L.owner[bb.x]) != key || (bb.y >= 0 && ld_dev(&L.owner[bb.y]) != key)) continue;
+                winm |= 1u << i;
+            }
+            int pre, btot;
+            lg_block_scan(__popc(winm), s_red, pre, btot);
+            const int room = quota - used;
+#pragma unroll
+            for (int i = 0; i < LG_CCACHE; i++) {
+                if (!((winm >> i) & 1u)) continue;
+                if (pre + __popc(winm & ((1u << i) - 1u)) >= room) continue;   // the workgroup's quota is full
+                const int k = gt + i * gs;
+                const int2 bb = cb[i];
                 L.colour[k] = c;
                 live &= ~(1u << i);
                 st_dev(&L.taken[bb.x], c);
                 if (bb.y >= 0) st_dev(&L.taken[bb.y], c);
                 won++;
             }
+            used += min(btot, max(room, 0));
             for (int k = kfar; k < nc; k += gs) {
                 if (L.colour[k] >= 0) continue;
                 const int2 bb = L.cbody[k];
@@ -3231,8 +3273,9 @@ __global__ __launch_bounds__(LG_BLOCK) void k_lg_colour(DevWorlds D, LargeDev L)
             int total = 0;
             ok = lg_grid_sync(L, gen, &L.col_start[c], &total);
             // every eligible contact joined: each contact still uncoloured
-            // touches a body of colour c, so another round would find none
-            if (total - joined == elig) break;
+            // touches a body of colour c, so another round would find none;
+            // a round that added none means the quotas are full
+            if (total - joined == elig || total == joined) break;
             joined = total;
         }
         if (!ok) break;
